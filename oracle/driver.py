@@ -1,0 +1,208 @@
+"""Oracle time loop: a restatement of PyClaw's Python-side stepping semantics
+driving the oracle's FP64 kernels.
+
+TEST INFRASTRUCTURE ONLY (never imported by ``pyclaw_amd``).  It follows:
+  Controller.run output times          controller.py:275-395 (output_style 1)
+  Solver.evolve_to_time                solver.py:554-666
+  Solver.get_dt / get_dt_new           solver.py:531-552
+  ClawSolver.step                      classic/solver.py:96-146
+  ClawSolver2D/3D.step_hyperbolic      classic/solver.py:498-549, 648-706
+  SharpClawSolver.step (SSP33, SSP104) sharpclaw/solver.py:283-323, 400-430
+  SharpClawSolver.get_dt_new           sharpclaw/solver.py:653-685
+  SharpClawSolver2D.dq_hyperbolic      sharpclaw/solver.py:848-898
+"""
+import numpy as np
+
+from . import oracle as O
+
+
+class OracleRun:
+    """One problem: interior q (meqn, mx, my[, mz]) plus method settings."""
+
+    def __init__(self, rp, q, delta, bc_lower, bc_upper, solver="classic",
+                 dimensional_split=True, transverse=1, limiters=1, order=2,
+                 cfl_desired=0.9, cfl_max=1.0, dt_initial=0.1, dt_max=1e99,
+                 max_steps=10000, dt_variable=True, time_integrator="SSP104",
+                 char_decomp=0, num_ghost=None):
+        self.rp = rp
+        self.ndim = q.ndim - 1
+        self.delta = [float(d) for d in delta]
+        self.bc_lower, self.bc_upper = list(bc_lower), list(bc_upper)
+        self.solver = solver
+        self.split = bool(dimensional_split)
+        self.transverse = transverse
+        self.order = order
+        self.cfl_desired, self.cfl_max = cfl_desired, cfl_max
+        self.dt_initial, self.dt_max = dt_initial, dt_max
+        self.max_steps, self.dt_variable = max_steps, dt_variable
+        self.time_integrator = time_integrator
+        self.char_decomp = char_decomp
+        mw = O.MWAVES[rp.kind]
+        lim = limiters if isinstance(limiters, list) else [limiters]
+        self.mthlim = lim * mw if len(lim) == 1 else lim
+        if num_ghost is None:
+            num_ghost = 2 if solver == "classic" else 3
+        self.ng = num_ghost
+        self.q = np.asfortranarray(q, dtype=np.float64).copy(order="F")
+        self.qbc = np.zeros((q.shape[0],) + tuple(n + 2 * self.ng for n in q.shape[1:]), order="F")
+        self.t = 0.0
+        self.dt = dt_initial
+        self.dt_old = None
+        self.cfl = 0.0              # the CFL object's cached max
+        self.accept_step = True
+        self.dq_dt = None
+        self.numsteps = 0
+        self.dt_history = []
+
+    # ---- boundary conditions: Solver._apply_bcs (solver.py:351-437)
+    def _apply_bcs(self, q):
+        ng = self.ng
+        sl = (slice(None),) + tuple(slice(ng, -ng) for _ in range(self.ndim))
+        self.qbc[sl] = q
+        if self.ndim == 2:
+            O.bc2(self.qbc, ng, self.bc_lower, self.bc_upper)
+        else:
+            O.bc3(self.qbc, ng, self.bc_lower, self.bc_upper)
+        return self.qbc
+
+    def _interior(self, a):
+        ng = self.ng
+        sl = (slice(None),) + tuple(slice(ng, -ng) for _ in range(self.ndim))
+        return np.asfortranarray(a[sl]).copy(order="F")
+
+    # ---- Solver.get_dt (solver.py:535-552)
+    def _get_dt(self, t, tstart, tend, take_one_step):
+        cfl = self.cfl
+        if self.dt_variable and self.dt_old is not None:
+            if cfl > 0.0:
+                if self.solver == "classic":
+                    self.dt = min(self.dt_max, self.dt * self.cfl_desired / cfl)
+                else:
+                    self.dt_old = self.dt
+                    self.dt = self.dt * self.cfl_desired / cfl
+            else:
+                self.dt = self.dt_max
+        else:
+            self.dt_old = self.dt
+        if not take_one_step:
+            if t + self.dt > tend and tstart < tend:
+                self.dt = tend - t
+            if tend - t - self.dt < 1.e-14 * t:
+                self.dt = tend - t
+
+    # ---- classic step (classic/solver.py:96-146, 498-549, 648-706)
+    def _step_classic(self, t, tstart, tend, take_one_step):
+        self._get_dt(t, tstart, tend, take_one_step)
+        self.cfl = 0.0
+        qbc = self._apply_bcs(self.q)
+        qold = qbc.copy(order="F")
+        ng, dt = self.ng, self.dt
+        if self.ndim == 2:
+            dx, dy = self.delta
+            if self.split:
+                meth = O.method_array(order=self.order, dimensional_split=True,
+                                      dt_variable=self.dt_variable)
+                c1 = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=1)
+                c2 = O.step2(self.rp, qbc, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=2)
+                cfl = max(c1, c2)
+            else:
+                meth = O.method_array(order=self.order, transverse=self.transverse,
+                                      dt_variable=self.dt_variable)
+                cfl = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=0)
+        else:
+            dx, dy, dz = self.delta
+            meth = O.method_array(order=self.order, dimensional_split=True,
+                                  dt_variable=self.dt_variable)
+            c1 = O.step3ds(self.rp, qold, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 1)
+            c2 = O.step3ds(self.rp, qbc, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 2)
+            c3 = O.step3ds(self.rp, qbc, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 3)
+            cfl = max(c1, c2, c3)
+        self.cfl = max(self.cfl, cfl)
+        self.q = self._interior(qbc)
+
+    # ---- SharpClaw dq (sharpclaw/solver.py:848-898), returns dt*L(q) interior
+    def _dq(self, q):
+        qbc = self._apply_bcs(q)
+        dx, dy = self.delta
+        dq, cfl = O.sharp_flux2(self.rp, qbc, self.ng, dx, dy, self.dt, self.char_decomp)
+        self.cfl = max(self.cfl, cfl)
+        return self._interior(dq)
+
+    # ---- SharpClawSolver.step (sharpclaw/solver.py:283-430)
+    def _step_sharp(self, t, tstart, tend, take_one_step):
+        if self.accept_step:
+            self.cfl = 0.0
+            self.dq_dt = self._dq(self.q) / self.dt
+        self._get_dt(t, tstart, tend, take_one_step)
+        cfl = self.cfl
+        self.cfl = self.dt / self.dt_old * cfl
+        self.dt_old = self.dt
+        q, dt = self.q, self.dt
+        if self.time_integrator == "SSP33":
+            r = q + dt * self.dq_dt
+            r = 0.75 * q + 0.25 * (r + self._dq(r))
+            self.q = 1. / 3. * q + 2. / 3. * (r + self._dq(r))
+        elif self.time_integrator == "SSP104":
+            s1 = q + dt * self.dq_dt / 6.
+            for _ in range(4):
+                s1 = s1 + self._dq(s1) / 6.
+            q = q / 25. + 0.36 * s1
+            s1 = 15. * q - 5. * s1
+            for _ in range(4):
+                s1 = s1 + self._dq(s1) / 6.
+            self.q = q + (0.6 * s1 + 0.1 * self._dq(s1))
+        else:
+            raise ValueError(self.time_integrator)
+        self.q = np.asfortranarray(self.q)
+
+    # ---- Solver.evolve_to_time (solver.py:554-666)
+    def evolve_to_time(self, tend):
+        tstart = self.t
+        take_one_step = tend is None
+        max_steps = self.max_steps
+        if not self.dt_variable:
+            max_steps = 1 if take_one_step else int((tend - tstart + 1e-10) / self.dt)
+        if self.dt_variable and self.cfl_desired > self.cfl_max:
+            raise Exception("Variable time-stepping and desired CFL > maximum CFL")
+        num_steps = 0
+        for n in range(max_steps):
+            q_backup = self.q.copy(order="F")
+            told = self.t
+            if self.solver == "classic":
+                self._step_classic(self.t, tstart, tend, take_one_step)
+            else:
+                self._step_sharp(self.t, tstart, tend, take_one_step)
+            self.accept_step = not (self.cfl > self.cfl_max)
+            if self.accept_step:
+                if self.dt_variable:
+                    self.t += self.dt
+                else:
+                    self.t = tstart + (n + 1) * self.dt
+                num_steps += 1
+                self.numsteps += 1
+                self.dt_history.append(self.dt)
+            else:
+                if self.dt_variable:
+                    self.q = q_backup
+                    self.t = told
+                else:
+                    raise Exception("CFL too large, giving up!")
+            if take_one_step or self.t >= tend:
+                break
+        if not take_one_step and self.dt_variable and self.t < tend and num_steps == max_steps:
+            raise Exception("Maximum number of timesteps have been taken")
+
+    # ---- Controller.run (controller.py:275-395), output_style 1
+    def run(self, tfinal, num_output_times=10):
+        self.dt = self.dt_initial
+        frames = [self.q.copy(order="F")]
+        for t in np.linspace(self.t, tfinal, num_output_times + 1)[1:]:
+            self.evolve_to_time(t)
+            frames.append(self.q.copy(order="F"))
+        return frames
+
+
+def cell_centers(lower, upper, n):
+    """Dimension.centers (geometry.py:546-553): lower + (i+0.5)*delta."""
+    delta = (upper - lower) / float(n)
+    return np.array([lower + (i + 0.5) * delta for i in range(n)]), delta

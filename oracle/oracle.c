@@ -1,0 +1,1136 @@
+/* oracle.c -- CPU restatement (FP64, serial) of the PyClaw time-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Built with
+ * -ffp-contract=off so every expression rounds in the order the reference's
+ * Fortran writes it.  Each function cites the reference lines it follows.
+ *
+ * The Riemann solvers live in the external clawpack.riemann package, which the
+ * reference does not vendor; they are restated here from the published Roe /
+ * Harten-Hyman formulas in the form of the reference's own in-tree Euler solver
+ * (development/rp_approaches/rpn2_euler_5wave.f, rpt2_euler_5wave.f) and are
+ * pinned end to end by the reference's example fixtures.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* The REAL(4) literal 1.e-36 / 1.0e-36 the reference adds in double precision
+ * (reconstruct.f90:11, weno.f90:52-57): the float nearest 1e-36, widened. */
+#define EPWENO ((double)1.0e-36f)
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
+int orc_rp_meqn(int kind) {
+  switch (kind) {
+    case ORC_RP_ACOUSTICS: return 3;
+    case ORC_RP_EULER: return 4;
+    case ORC_RP_SHALLOW: return 3;
+    case ORC_RP_ADVECTION: return 1;
+    case ORC_RP_EULER3: return 5;
+  }
+  return 0;
+}
+
+int orc_rp_mwaves(int kind) {
+  switch (kind) {
+    case ORC_RP_ACOUSTICS: return 2;
+    case ORC_RP_EULER: return 4;
+    case ORC_RP_SHALLOW: return 3;
+    case ORC_RP_ADVECTION: return 1;
+    case ORC_RP_EULER3: return 3;
+  }
+  return 0;
+}
+
+/* ======================================================================
+ * Riemann solvers (pointwise)
+ * ====================================================================== */
+
+/* acoustics_2D rpn2: characteristic decomposition with impedance zz, speeds -cc, +cc. */
+static void rpn_acoustics(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                          double *wave, double *s, double *amdq, double *apdq) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double zz = p->zz, cc = p->cc;
+  double delta1 = qR[0] - qL[0];
+  double delta2 = qR[mu] - qL[mu];
+  double a1 = (-delta1 + zz * delta2) / (2.0 * zz);
+  double a2 = (delta1 + zz * delta2) / (2.0 * zz);
+  wave[0] = -a1 * zz; wave[mu] = a1; wave[mv] = 0.0; s[0] = -cc;
+  wave[3 + 0] = a2 * zz; wave[3 + mu] = a2; wave[3 + mv] = 0.0; s[1] = cc;
+  for (int m = 0; m < 3; m++) {
+    amdq[m] = s[0] * wave[m];
+    apdq[m] = s[1] * wave[3 + m];
+  }
+}
+
+static void rpt_acoustics(const orc_rp *p, int ixy, const double *asdq,
+                          double *bm, double *bp) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double zz = p->zz, cc = p->cc;
+  double a1 = (-asdq[0] + zz * asdq[mv]) / (2.0 * zz);
+  double a2 = (asdq[0] + zz * asdq[mv]) / (2.0 * zz);
+  bm[0] = cc * a1 * zz; bm[mu] = 0.0; bm[mv] = -cc * a1;
+  bp[0] = cc * a2 * zz; bp[mu] = 0.0; bp[mv] = cc * a2;
+}
+
+/* Roe averages for 2D Euler (rpn2_euler_5wave.f:60-80). */
+typedef struct { double u, v, enth, u2v2, a, g1a2, euv; } roe2;
+
+static roe2 roe_euler2(double gamma1, int mu, int mv, const double *qL, const double *qR) {
+  roe2 r;
+  double rhsqrtl = sqrt(qL[0]);
+  double rhsqrtr = sqrt(qR[0]);
+  double pl = gamma1 * (qL[3] - 0.5 * (qL[1] * qL[1] + qL[2] * qL[2]) / qL[0]);
+  double pr = gamma1 * (qR[3] - 0.5 * (qR[1] * qR[1] + qR[2] * qR[2]) / qR[0]);
+  double rhsq2 = rhsqrtl + rhsqrtr;
+  r.u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
+  r.v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
+  r.enth = (((qL[3] + pl) / rhsqrtl + (qR[3] + pr) / rhsqrtr)) / rhsq2;
+  r.u2v2 = r.u * r.u + r.v * r.v;
+  double a2 = gamma1 * (r.enth - .5 * r.u2v2);
+  r.a = sqrt(a2);
+  r.g1a2 = gamma1 / a2;
+  r.euv = r.enth - r.u2v2;
+  return r;
+}
+
+/* euler_4wave_2D rpn2: Roe solver, waves (u-a, shear u, entropy u, u+a), with the
+ * Harten-Hyman entropy fix of rpn2_euler_5wave.f:137-234 (tracer wave dropped). */
+static void rpn_euler(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                      double *wave, double *s, double *amdq, double *apdq) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double gamma = p->gamma, gamma1 = p->gamma - 1.0;
+  roe2 r = roe_euler2(gamma1, mu, mv, qL, qR);
+  double delta[4];
+  delta[0] = qR[0] - qL[0];
+  delta[1] = qR[mu] - qL[mu];
+  delta[2] = qR[mv] - qL[mv];
+  delta[3] = qR[3] - qL[3];
+  double a3 = r.g1a2 * (r.euv * delta[0] + r.u * delta[1] + r.v * delta[2] - delta[3]);
+  double a2 = delta[2] - r.v * delta[0];
+  double a4 = (delta[1] + (r.a - r.u) * delta[0] - r.a * a3) / (2.0 * r.a);
+  double a1 = delta[0] - a3 - a4;
+  double *w1 = wave, *w2 = wave + 4, *w3 = wave + 8, *w4 = wave + 12;
+  w1[0] = a1; w1[mu] = a1 * (r.u - r.a); w1[mv] = a1 * r.v; w1[3] = a1 * (r.enth - r.u * r.a);
+  s[0] = r.u - r.a;
+  w2[0] = 0.0; w2[mu] = 0.0; w2[mv] = a2; w2[3] = a2 * r.v;
+  s[1] = r.u;
+  w3[0] = a3; w3[mu] = a3 * r.u; w3[mv] = a3 * r.v; w3[3] = a3 * 0.5 * r.u2v2;
+  s[2] = r.u;
+  w4[0] = a4; w4[mu] = a4 * (r.u + r.a); w4[mv] = a4 * r.v; w4[3] = a4 * (r.enth + r.u * r.a);
+  s[3] = r.u + r.a;
+
+  if (!p->efix) {
+    for (int m = 0; m < 4; m++) {
+      amdq[m] = 0.0; apdq[m] = 0.0;
+      for (int mw = 0; mw < 4; mw++) {
+        if (s[mw] < 0.0) amdq[m] = amdq[m] + s[mw] * wave[mw * 4 + m];
+        else apdq[m] = apdq[m] + s[mw] * wave[mw * 4 + m];
+      }
+    }
+    return;
+  }
+  /* entropy fix: amdq first, apdq = sum(s*wave) - amdq */
+  do {
+    double rhoim1 = qL[0];
+    double pim1 = gamma1 * (qL[3] - 0.5 * (qL[mu] * qL[mu] + qL[mv] * qL[mv]) / rhoim1);
+    double cim1 = sqrt(gamma * pim1 / rhoim1);
+    double s0 = qL[mu] / rhoim1 - cim1;
+    if (s0 >= 0.0 && s[0] > 0.0) {
+      for (int m = 0; m < 4; m++) amdq[m] = 0.0;
+      break;
+    }
+    double rho1 = qL[0] + w1[0];
+    double rhou1 = qL[mu] + w1[mu];
+    double rhov1 = qL[mv] + w1[mv];
+    double en1 = qL[3] + w1[3];
+    double p1 = gamma1 * (en1 - 0.5 * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+    double c1 = sqrt(gamma * p1 / rho1);
+    double s1 = rhou1 / rho1 - c1;
+    double sfract;
+    if (s0 < 0.0 && s1 > 0.0) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+    else if (s[0] < 0.0) sfract = s[0];
+    else sfract = 0.0;
+    for (int m = 0; m < 4; m++) amdq[m] = sfract * w1[m];
+    if (s[1] >= 0.0) break;
+    for (int m = 0; m < 4; m++) {
+      amdq[m] = amdq[m] + s[1] * w2[m];
+      amdq[m] = amdq[m] + s[2] * w3[m];
+    }
+    double rhoi = qR[0];
+    double pi = gamma1 * (qR[3] - 0.5 * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
+    double ci = sqrt(gamma * pi / rhoi);
+    double s3 = qR[mu] / rhoi + ci;
+    double rho2 = qR[0] - w4[0];
+    double rhou2 = qR[mu] - w4[mu];
+    double rhov2 = qR[mv] - w4[mv];
+    double en2 = qR[3] - w4[3];
+    double p2 = gamma1 * (en2 - 0.5 * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+    double c2 = sqrt(gamma * p2 / rho2);
+    double s2 = rhou2 / rho2 + c2;
+    if (s2 < 0.0 && s3 > 0.0) sfract = s2 * (s3 - s[3]) / (s3 - s2);
+    else if (s[3] < 0.0) sfract = s[3];
+    else break;
+    for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + sfract * w4[m];
+  } while (0);
+  for (int m = 0; m < 4; m++) {
+    double df = 0.0;
+    for (int mw = 0; mw < 4; mw++) df = df + s[mw] * wave[mw * 4 + m];
+    apdq[m] = df - amdq[m];
+  }
+}
+
+/* euler_4wave_2D rpt2: split asdq into waves of the transverse Roe matrix at the
+ * Roe average of the slice's interface states (rpt2_euler_5wave.f:47-95). */
+static void rpt_euler(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                      const double *asdq, double *bm, double *bp) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double gamma1 = p->gamma - 1.0;
+  roe2 r = roe_euler2(gamma1, mu, mv, qL, qR);
+  double a3 = r.g1a2 * (r.euv * asdq[0] + r.u * asdq[mu] + r.v * asdq[mv] - asdq[3]);
+  double a2 = asdq[mu] - r.u * asdq[0];
+  double a4 = (asdq[mv] + (r.a - r.v) * asdq[0] - r.a * a3) / (2.0 * r.a);
+  double a1 = asdq[0] - a3 - a4;
+  double wb[3][4], sb[3];
+  wb[0][0] = a1; wb[0][mu] = a1 * r.u; wb[0][mv] = a1 * (r.v - r.a);
+  wb[0][3] = a1 * (r.enth - r.v * r.a); sb[0] = r.v - r.a;
+  wb[1][0] = a3; wb[1][mu] = a3 * r.u + a2; wb[1][mv] = a3 * r.v;
+  wb[1][3] = a3 * 0.5 * r.u2v2 + a2 * r.u; sb[1] = r.v;
+  wb[2][0] = a4; wb[2][mu] = a4 * r.u; wb[2][mv] = a4 * (r.v + r.a);
+  wb[2][3] = a4 * (r.enth + r.v * r.a); sb[2] = r.v + r.a;
+  for (int m = 0; m < 4; m++) {
+    bm[m] = 0.0; bp[m] = 0.0;
+    for (int mw = 0; mw < 3; mw++) {
+      bm[m] = bm[m] + dmin(sb[mw], 0.0) * wb[mw][m];
+      bp[m] = bp[m] + dmax(sb[mw], 0.0) * wb[mw][m];
+    }
+  }
+}
+
+/* shallow_roe_with_efix_2D rpn2: Roe solver (h-bar = arithmetic mean, u-bar
+ * sqrt(h)-weighted), waves (u-c, u, u+c), Harten-Hyman fix on waves 1 and 3 in
+ * the form of the Euler solver above. */
+static void rpn_shallow(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                        double *wave, double *s, double *amdq, double *apdq) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double grav = p->grav;
+  double h = (qL[0] + qR[0]) * 0.50;
+  double hsqrtl = sqrt(qL[0]);
+  double hsqrtr = sqrt(qR[0]);
+  double hsq2 = hsqrtl + hsqrtr;
+  double u = (qL[mu] / hsqrtl + qR[mu] / hsqrtr) / hsq2;
+  double v = (qL[mv] / hsqrtl + qR[mv] / hsqrtr) / hsq2;
+  double a = sqrt(grav * h);
+  double delta0 = qR[0] - qL[0];
+  double delta1 = qR[mu] - qL[mu];
+  double delta2 = qR[mv] - qL[mv];
+  double a1 = ((u + a) * delta0 - delta1) * (0.50 / a);
+  double a2 = -v * delta0 + delta2;
+  double a3 = (-(u - a) * delta0 + delta1) * (0.50 / a);
+  double *w1 = wave, *w2 = wave + 3, *w3 = wave + 6;
+  w1[0] = a1; w1[mu] = a1 * (u - a); w1[mv] = a1 * v; s[0] = u - a;
+  w2[0] = 0.0; w2[mu] = 0.0; w2[mv] = a2; s[1] = u;
+  w3[0] = a3; w3[mu] = a3 * (u + a); w3[mv] = a3 * v; s[2] = u + a;
+
+  if (!p->efix) {
+    for (int m = 0; m < 3; m++) {
+      amdq[m] = 0.0; apdq[m] = 0.0;
+      for (int mw = 0; mw < 3; mw++) {
+        if (s[mw] < 0.0) amdq[m] = amdq[m] + s[mw] * wave[mw * 3 + m];
+        else apdq[m] = apdq[m] + s[mw] * wave[mw * 3 + m];
+      }
+    }
+    return;
+  }
+  do {
+    double him1 = qL[0];
+    double s0 = qL[mu] / him1 - sqrt(grav * him1);
+    if (s0 >= 0.0 && s[0] > 0.0) {
+      for (int m = 0; m < 3; m++) amdq[m] = 0.0;
+      break;
+    }
+    double h1 = qL[0] + w1[0];
+    double hu1 = qL[mu] + w1[mu];
+    double s1 = hu1 / h1 - sqrt(grav * h1);
+    double sfract;
+    if (s0 < 0.0 && s1 > 0.0) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+    else if (s[0] < 0.0) sfract = s[0];
+    else sfract = 0.0;
+    for (int m = 0; m < 3; m++) amdq[m] = sfract * w1[m];
+    if (s[1] >= 0.0) break;
+    for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + s[1] * w2[m];
+    double hi = qR[0];
+    double s03 = qR[mu] / hi + sqrt(grav * hi);
+    double h3 = qR[0] - w3[0];
+    double hu3 = qR[mu] - w3[mu];
+    double s3 = hu3 / h3 + sqrt(grav * h3);
+    if (s3 < 0.0 && s03 > 0.0) sfract = s3 * (s03 - s[2]) / (s03 - s3);
+    else if (s[2] < 0.0) sfract = s[2];
+    else break;
+    for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + sfract * w3[m];
+  } while (0);
+  for (int m = 0; m < 3; m++) {
+    double df = 0.0;
+    for (int mw = 0; mw < 3; mw++) df = df + s[mw] * wave[mw * 3 + m];
+    apdq[m] = df - amdq[m];
+  }
+}
+
+static void rpt_shallow(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                        const double *asdq, double *bm, double *bp) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double grav = p->grav;
+  double hsqrtl = sqrt(qL[0]);
+  double hsqrtr = sqrt(qR[0]);
+  double hsq2 = hsqrtl + hsqrtr;
+  double u = (qL[mu] / hsqrtl + qR[mu] / hsqrtr) / hsq2;
+  double v = (qL[mv] / hsqrtl + qR[mv] / hsqrtr) / hsq2;
+  double a = sqrt(grav * ((qL[0] + qR[0]) * 0.50));
+  double a1 = (0.50 / a) * ((v + a) * asdq[0] - asdq[mv]);
+  double a2 = asdq[mu] - u * asdq[0];
+  double a3 = (0.50 / a) * (-(v - a) * asdq[0] + asdq[mv]);
+  double wb[3][3], sb[3];
+  wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
+  wb[1][0] = 0.0; wb[1][mu] = a2; wb[1][mv] = 0.0; sb[1] = v;
+  wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
+  for (int m = 0; m < 3; m++) {
+    bm[m] = 0.0; bp[m] = 0.0;
+    for (int mw = 0; mw < 3; mw++) {
+      bm[m] = bm[m] + dmin(sb[mw], 0.0) * wb[mw][m];
+      bp[m] = bp[m] + dmax(sb[mw], 0.0) * wb[mw][m];
+    }
+  }
+}
+
+/* advection_2D: one wave, speed u (x) or v (y); transverse split by the other speed. */
+static void rpn_advection(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                          double *wave, double *s, double *amdq, double *apdq) {
+  wave[0] = qR[0] - qL[0];
+  s[0] = ixy == 1 ? p->u : p->v;
+  amdq[0] = dmin(s[0], 0.0) * wave[0];
+  apdq[0] = dmax(s[0], 0.0) * wave[0];
+}
+
+static void rpt_advection(const orc_rp *p, int ixy, const double *asdq, double *bm,
+                          double *bp) {
+  double stran = ixy == 1 ? p->v : p->u;
+  bm[0] = dmin(stran, 0.0) * asdq[0];
+  bp[0] = dmax(stran, 0.0) * asdq[0];
+}
+
+/* euler_3D rpn3: Roe solver with the entropy and both shear waves lumped into
+ * one wave of speed u (3 waves: u-a, u, u+a), entropy fix on waves 1 and 3.
+ * ixyz selects the normal momentum component. */
+static void rpn_euler3(const orc_rp *p, int ixyz, const double *qL, const double *qR,
+                       double *wave, double *s, double *amdq, double *apdq) {
+  int mu, mv, mw;
+  if (ixyz == 1) { mu = 1; mv = 2; mw = 3; }
+  else if (ixyz == 2) { mu = 2; mv = 3; mw = 1; }
+  else { mu = 3; mv = 1; mw = 2; }
+  const double gamma = p->gamma, gamma1 = p->gamma - 1.0;
+  double rhsqrtl = sqrt(qL[0]);
+  double rhsqrtr = sqrt(qR[0]);
+  double pl = gamma1 * (qL[4] - 0.5 * (qL[1] * qL[1] + qL[2] * qL[2] + qL[3] * qL[3]) / qL[0]);
+  double pr = gamma1 * (qR[4] - 0.5 * (qR[1] * qR[1] + qR[2] * qR[2] + qR[3] * qR[3]) / qR[0]);
+  double rhsq2 = rhsqrtl + rhsqrtr;
+  double u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
+  double v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
+  double w = (qL[mw] / rhsqrtl + qR[mw] / rhsqrtr) / rhsq2;
+  double enth = (((qL[4] + pl) / rhsqrtl + (qR[4] + pr) / rhsqrtr)) / rhsq2;
+  double u2v2w2 = u * u + v * v + w * w;
+  double a2 = gamma1 * (enth - .5 * u2v2w2);
+  double a = sqrt(a2);
+  double g1a2 = gamma1 / a2;
+  double euv = enth - u2v2w2;
+  double delta[5];
+  delta[0] = qR[0] - qL[0];
+  delta[1] = qR[mu] - qL[mu];
+  delta[2] = qR[mv] - qL[mv];
+  delta[3] = qR[mw] - qL[mw];
+  delta[4] = qR[4] - qL[4];
+  double a4 = g1a2 * (euv * delta[0] + u * delta[1] + v * delta[2] + w * delta[3] - delta[4]);
+  double b2 = delta[2] - v * delta[0];
+  double b3 = delta[3] - w * delta[0];
+  double a5 = (delta[1] + (a - u) * delta[0] - a * a4) / (2.0 * a);
+  double a1 = delta[0] - a4 - a5;
+  double *w1 = wave, *w2 = wave + 5, *w3 = wave + 10;
+  w1[0] = a1; w1[mu] = a1 * (u - a); w1[mv] = a1 * v; w1[mw] = a1 * w;
+  w1[4] = a1 * (enth - u * a); s[0] = u - a;
+  w2[0] = a4; w2[mu] = a4 * u; w2[mv] = a4 * v + b2; w2[mw] = a4 * w + b3;
+  w2[4] = a4 * 0.5 * u2v2w2 + b2 * v + b3 * w; s[1] = u;
+  w3[0] = a5; w3[mu] = a5 * (u + a); w3[mv] = a5 * v; w3[mw] = a5 * w;
+  w3[4] = a5 * (enth + u * a); s[2] = u + a;
+
+  if (!p->efix) {
+    for (int m = 0; m < 5; m++) {
+      amdq[m] = 0.0; apdq[m] = 0.0;
+      for (int k = 0; k < 3; k++) {
+        if (s[k] < 0.0) amdq[m] = amdq[m] + s[k] * wave[k * 5 + m];
+        else apdq[m] = apdq[m] + s[k] * wave[k * 5 + m];
+      }
+    }
+    return;
+  }
+  do {
+    double rhoim1 = qL[0];
+    double pim1 = gamma1 * (qL[4] - 0.5 * (qL[mu] * qL[mu] + qL[mv] * qL[mv] + qL[mw] * qL[mw]) / rhoim1);
+    double cim1 = sqrt(gamma * pim1 / rhoim1);
+    double s0 = qL[mu] / rhoim1 - cim1;
+    if (s0 >= 0.0 && s[0] > 0.0) {
+      for (int m = 0; m < 5; m++) amdq[m] = 0.0;
+      break;
+    }
+    double rho1 = qL[0] + w1[0];
+    double rhou1 = qL[mu] + w1[mu];
+    double rhov1 = qL[mv] + w1[mv];
+    double rhow1 = qL[mw] + w1[mw];
+    double en1 = qL[4] + w1[4];
+    double p1 = gamma1 * (en1 - 0.5 * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) / rho1);
+    double c1 = sqrt(gamma * p1 / rho1);
+    double s1 = rhou1 / rho1 - c1;
+    double sfract;
+    if (s0 < 0.0 && s1 > 0.0) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+    else if (s[0] < 0.0) sfract = s[0];
+    else sfract = 0.0;
+    for (int m = 0; m < 5; m++) amdq[m] = sfract * w1[m];
+    if (s[1] >= 0.0) break;
+    for (int m = 0; m < 5; m++) amdq[m] = amdq[m] + s[1] * w2[m];
+    double rhoi = qR[0];
+    double pi = gamma1 * (qR[4] - 0.5 * (qR[mu] * qR[mu] + qR[mv] * qR[mv] + qR[mw] * qR[mw]) / rhoi);
+    double ci = sqrt(gamma * pi / rhoi);
+    double s3 = qR[mu] / rhoi + ci;
+    double rho2 = qR[0] - w3[0];
+    double rhou2 = qR[mu] - w3[mu];
+    double rhov2 = qR[mv] - w3[mv];
+    double rhow2 = qR[mw] - w3[mw];
+    double en2 = qR[4] - w3[4];
+    double p2 = gamma1 * (en2 - 0.5 * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) / rho2);
+    double c2 = sqrt(gamma * p2 / rho2);
+    double s2 = rhou2 / rho2 + c2;
+    if (s2 < 0.0 && s3 > 0.0) sfract = s2 * (s3 - s[2]) / (s3 - s2);
+    else if (s[2] < 0.0) sfract = s[2];
+    else break;
+    for (int m = 0; m < 5; m++) amdq[m] = amdq[m] + sfract * w3[m];
+  } while (0);
+  for (int m = 0; m < 5; m++) {
+    double df = 0.0;
+    for (int k = 0; k < 3; k++) df = df + s[k] * wave[k * 5 + m];
+    apdq[m] = df - amdq[m];
+  }
+}
+
+void orc_rpn(const orc_rp *p, int ixy, const double *qL, const double *qR,
+             double *wave, double *s, double *amdq, double *apdq) {
+  switch (p->kind) {
+    case ORC_RP_ACOUSTICS: rpn_acoustics(p, ixy, qL, qR, wave, s, amdq, apdq); break;
+    case ORC_RP_EULER: rpn_euler(p, ixy, qL, qR, wave, s, amdq, apdq); break;
+    case ORC_RP_SHALLOW: rpn_shallow(p, ixy, qL, qR, wave, s, amdq, apdq); break;
+    case ORC_RP_ADVECTION: rpn_advection(p, ixy, qL, qR, wave, s, amdq, apdq); break;
+    case ORC_RP_EULER3: rpn_euler3(p, ixy, qL, qR, wave, s, amdq, apdq); break;
+  }
+}
+
+void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
+             const double *asdq, double *bmasdq, double *bpasdq) {
+  (void)imp;
+  switch (p->kind) {
+    case ORC_RP_ACOUSTICS: rpt_acoustics(p, ixy, asdq, bmasdq, bpasdq); break;
+    case ORC_RP_EULER: rpt_euler(p, ixy, qL, qR, asdq, bmasdq, bpasdq); break;
+    case ORC_RP_SHALLOW: rpt_shallow(p, ixy, qL, qR, asdq, bmasdq, bpasdq); break;
+    case ORC_RP_ADVECTION: rpt_advection(p, ixy, asdq, bmasdq, bpasdq); break;
+    default: break;
+  }
+}
+
+/* ======================================================================
+ * Boundary conditions (solver.py:439-519)
+ * ====================================================================== */
+
+/* Apply one side of one dimension on a generic strided array.
+ * n = interior cells along the dimension, stride_d = element stride along it,
+ * the "lines" are every combination of the other indices (given as a list of
+ * base offsets). */
+static void bc_side(double *a, int meqn, int n, int ng, long stride_d, const long *bases,
+                    long nbases, int upper, int bc, int refl) {
+  const int nt = n + 2 * ng;
+  /* the reference indexes q[reflect_index] and raises for a system that has no
+   * such component; here a wall on such a system only mirrors */
+  const int negate = refl >= 0 && refl < meqn;
+  for (long b = 0; b < nbases; b++) {
+    double *L = a + bases[b];
+#define A(m, ii) L[(m) + (long)(ii) * stride_d]
+    if (!upper) {
+      if (bc == ORC_BC_EXTRAP) {
+        for (int i = 0; i < ng; i++)
+          for (int m = 0; m < meqn; m++) A(m, i) = A(m, ng);
+      } else if (bc == ORC_BC_PERIODIC) {
+        for (int i = 0; i < ng; i++)
+          for (int m = 0; m < meqn; m++) A(m, i) = A(m, nt - 2 * ng + i);
+      } else if (bc == ORC_BC_WALL) {
+        for (int i = 0; i < ng; i++) {
+          for (int m = 0; m < meqn; m++) A(m, i) = A(m, 2 * ng - 1 - i);
+          if (negate) A(refl, i) = -A(refl, 2 * ng - 1 - i);
+        }
+      }
+    } else {
+      if (bc == ORC_BC_EXTRAP) {
+        for (int i = 0; i < ng; i++)
+          for (int m = 0; m < meqn; m++) A(m, nt - 1 - i) = A(m, nt - ng - 1);
+      } else if (bc == ORC_BC_PERIODIC) {
+        for (int i = 0; i < ng; i++)
+          for (int m = 0; m < meqn; m++) A(m, nt - ng + i) = A(m, ng + i);
+      } else if (bc == ORC_BC_WALL) {
+        for (int i = 0; i < ng; i++) {
+          for (int m = 0; m < meqn; m++) A(m, nt - 1 - i) = A(m, nt - 2 * ng + i);
+          if (negate) A(refl, nt - 1 - i) = -A(refl, nt - 2 * ng + i);
+        }
+      }
+    }
+#undef A
+  }
+}
+
+void orc_bc2(double *qbc, int meqn, int mx, int my, int ng, const int *bc_lower,
+             const int *bc_upper, const int *reflect) {
+  const long nx = mx + 2 * ng, ny = my + 2 * ng;
+  long *bases = (long *)calloc((size_t)(nx > ny ? nx : ny), sizeof(long));
+  /* x: one line per row j (all rows, ghosts included) */
+  for (long j = 0; j < ny; j++) bases[j] = (long)meqn * nx * j;
+  bc_side(qbc, meqn, mx, ng, meqn, bases, ny, 0, bc_lower[0], reflect[0]);
+  bc_side(qbc, meqn, mx, ng, meqn, bases, ny, 1, bc_upper[0], reflect[0]);
+  /* y: one line per column i */
+  for (long i = 0; i < nx; i++) bases[i] = (long)meqn * i;
+  bc_side(qbc, meqn, my, ng, (long)meqn * nx, bases, nx, 0, bc_lower[1], reflect[1]);
+  bc_side(qbc, meqn, my, ng, (long)meqn * nx, bases, nx, 1, bc_upper[1], reflect[1]);
+  free(bases);
+}
+
+void orc_bc3(double *qbc, int meqn, int mx, int my, int mz, int ng, const int *bc_lower,
+             const int *bc_upper, const int *reflect) {
+  const long nx = mx + 2 * ng, ny = my + 2 * ng, nz = mz + 2 * ng;
+  long *bases = (long *)calloc((size_t)(nx * ny + ny * nz + nx * nz), sizeof(long));
+  long nb = 0;
+  for (long k = 0; k < nz; k++)
+    for (long j = 0; j < ny; j++) bases[nb++] = (long)meqn * (nx * (j + ny * k));
+  bc_side(qbc, meqn, mx, ng, meqn, bases, nb, 0, bc_lower[0], reflect[0]);
+  bc_side(qbc, meqn, mx, ng, meqn, bases, nb, 1, bc_upper[0], reflect[0]);
+  nb = 0;
+  for (long k = 0; k < nz; k++)
+    for (long i = 0; i < nx; i++) bases[nb++] = (long)meqn * (i + nx * ny * k);
+  bc_side(qbc, meqn, my, ng, (long)meqn * nx, bases, nb, 0, bc_lower[1], reflect[1]);
+  bc_side(qbc, meqn, my, ng, (long)meqn * nx, bases, nb, 1, bc_upper[1], reflect[1]);
+  nb = 0;
+  for (long j = 0; j < ny; j++)
+    for (long i = 0; i < nx; i++) bases[nb++] = (long)meqn * (i + nx * j);
+  bc_side(qbc, meqn, mz, ng, (long)meqn * nx * ny, bases, nb, 0, bc_lower[2], reflect[2]);
+  bc_side(qbc, meqn, mz, ng, (long)meqn * nx * ny, bases, nb, 1, bc_upper[2], reflect[2]);
+  free(bases);
+}
+
+/* ======================================================================
+ * Classic: limiter, philim, flux2, flux3 (split part)
+ * ====================================================================== */
+
+/* philim.f90:2-36 */
+static double philim(double a, double b, int meth) {
+  double r = b / a;
+  switch (meth) {
+    case 1: return dmax(0.0, dmin(1.0, r));
+    case 2: return dmax(dmax(0.0, dmin(1.0, 2.0 * r)), dmin(2.0, r));
+    case 3: return (r + fabs(r)) / (1.0 + fabs(r));
+    case 4: {
+      double c = (1.0 + r) / 2.0;
+      return dmax(0.0, dmin(dmin(c, 2.0), 2.0 * r));
+    }
+    case 5: return r;
+  }
+  return 0.0;
+}
+
+/* Slice work arrays, index i runs over 1-ng .. n+ng (Fortran), stored at i+ng-1. */
+typedef struct {
+  int meqn, mwaves, ng, len;
+  double *wave, *s, *amdq, *apdq, *cqxx, *bmasdq, *bpasdq, *qadd, *fadd, *gadd;
+} slice_ws;
+
+static void ws_alloc(slice_ws *w, int meqn, int mwaves, int ng, int maxm) {
+  w->meqn = meqn; w->mwaves = mwaves; w->ng = ng; w->len = maxm + 2 * ng;
+  long L = w->len;
+  w->wave = (double *)calloc(L * meqn * mwaves, sizeof(double));
+  w->s = (double *)calloc(L * mwaves, sizeof(double));
+  w->amdq = (double *)calloc(L * meqn, sizeof(double));
+  w->apdq = (double *)calloc(L * meqn, sizeof(double));
+  w->cqxx = (double *)calloc(L * meqn, sizeof(double));
+  w->bmasdq = (double *)calloc(L * meqn, sizeof(double));
+  w->bpasdq = (double *)calloc(L * meqn, sizeof(double));
+  w->qadd = (double *)calloc(L * meqn, sizeof(double));
+  w->fadd = (double *)calloc(L * meqn, sizeof(double));
+  w->gadd = (double *)calloc(L * meqn * 2, sizeof(double));
+}
+
+static void ws_free(slice_ws *w) {
+  free(w->wave); free(w->s); free(w->amdq); free(w->apdq); free(w->cqxx);
+  free(w->bmasdq); free(w->bpasdq); free(w->qadd); free(w->fadd); free(w->gadd);
+}
+
+#define IX(i) ((i) + ng - 1)
+#define V1(a, m, i) (a)[(m) + (long)meqn * IX(i)]
+#define WV(m, mw, i) w->wave[(m) + (long)meqn * ((mw) + (long)mwaves * IX(i))]
+#define SV(mw, i) w->s[(mw) + (long)mwaves * IX(i)]
+#define GA(m, k, i) w->gadd[(m) + (long)meqn * ((k) + 2L * IX(i))]
+
+/* limiter.f90:2-57 */
+static void limiter(slice_ws *w, int mx, const int *mthlim) {
+  const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng;
+  for (int mw = 0; mw < mwaves; mw++) {
+    if (mthlim[mw] == 0) continue;
+    double dotr = 0.0;
+    for (int i = 0; i <= mx + 1; i++) {
+      double wnorm2 = 0.0;
+      double dotl = dotr;
+      dotr = 0.0;
+      for (int m = 0; m < meqn; m++) {
+        wnorm2 = wnorm2 + WV(m, mw, i) * WV(m, mw, i);
+        dotr = dotr + WV(m, mw, i) * WV(m, mw, i + 1);
+      }
+      if (i == 0) continue;
+      if (wnorm2 == 0.0) continue;
+      double wlimitr;
+      if (SV(mw, i) > 0.0) wlimitr = philim(wnorm2, dotl, mthlim[mw]);
+      else wlimitr = philim(wnorm2, dotr, mthlim[mw]);
+      for (int m = 0; m < meqn; m++) WV(m, mw, i) = wlimitr * WV(m, mw, i);
+    }
+  }
+}
+
+/* Riemann solves along a slice, i = 2-ng .. mx+ng (reference rp ABI loop). */
+static void slice_rpn(slice_ws *w, int ixy, int mx, const double *q1d, const orc_rp *p) {
+  const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng;
+  for (int i = 2 - ng; i <= mx + ng; i++)
+    orc_rpn(p, ixy, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &WV(0, 0, i), &SV(0, i),
+            &V1(w->amdq, 0, i), &V1(w->apdq, 0, i));
+}
+
+/* classic/flux2.f90:2-207.  dtdx1d is constant along the slice (no capacity). */
+static double flux2(slice_ws *w, int ixy, int mx, const double *q1d, double dtdx,
+                    const int *method, const int *mthlim, int use_fwave, const orc_rp *p) {
+  const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng;
+  int limit = 0;
+  for (int mw = 0; mw < mwaves; mw++)
+    if (mthlim[mw] > 0) limit = 1;
+  for (int i = 1 - ng; i <= mx + ng; i++)
+    for (int m = 0; m < meqn; m++) {
+      V1(w->qadd, m, i) = 0.0;
+      V1(w->fadd, m, i) = 0.0;
+      GA(m, 0, i) = 0.0;
+      GA(m, 1, i) = 0.0;
+    }
+  slice_rpn(w, ixy, mx, q1d, p);
+  /* forall: the first statement completes for all i before the second runs */
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++) V1(w->qadd, m, i) = V1(w->qadd, m, i) - dtdx * V1(w->apdq, m, i);
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++)
+      V1(w->qadd, m, i - 1) = V1(w->qadd, m, i - 1) - dtdx * V1(w->amdq, m, i);
+  double cfl1d = 0.0;
+  for (int mw = 0; mw < mwaves; mw++)
+    for (int i = 1; i <= mx + 1; i++)
+      cfl1d = dmax(dmax(cfl1d, dtdx * SV(mw, i)), -dtdx * SV(mw, i));
+  if (method[1] != 1) {
+    if (limit) limiter(w, mx, mthlim);
+    for (int i = 2 - ng; i <= mx + ng; i++) {
+      double dtdxave = 0.5 * (dtdx + dtdx);
+      for (int m = 0; m < meqn; m++) {
+        V1(w->cqxx, m, i) = 0.0;
+        for (int mw = 0; mw < mwaves; mw++) {
+          double sa = fabs(SV(mw, i));
+          double f = use_fwave ? (SV(mw, i) >= 0.0 ? 1.0 : -1.0) : sa;
+          V1(w->cqxx, m, i) = V1(w->cqxx, m, i) + f * (1.0 - sa * dtdxave) * WV(m, mw, i);
+        }
+        V1(w->fadd, m, i) = V1(w->fadd, m, i) + 0.5 * V1(w->cqxx, m, i);
+      }
+    }
+  }
+  if (method[2] <= 0) return cfl1d;
+  if (method[1] > 1 && method[2] == 2) {
+    for (int i = 1; i <= mx + 1; i++)
+      for (int m = 0; m < meqn; m++) {
+        V1(w->amdq, m, i) = V1(w->amdq, m, i) + V1(w->cqxx, m, i);
+        V1(w->apdq, m, i) = V1(w->apdq, m, i) - V1(w->cqxx, m, i);
+      }
+  }
+  for (int i = 2 - ng; i <= mx + ng; i++)
+    orc_rpt(p, ixy, 1, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &V1(w->amdq, 0, i),
+            &V1(w->bmasdq, 0, i), &V1(w->bpasdq, 0, i));
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++) {
+      GA(m, 0, i - 1) = GA(m, 0, i - 1) - 0.5 * dtdx * V1(w->bmasdq, m, i);
+      GA(m, 1, i - 1) = GA(m, 1, i - 1) - 0.5 * dtdx * V1(w->bpasdq, m, i);
+    }
+  for (int i = 2 - ng; i <= mx + ng; i++)
+    orc_rpt(p, ixy, 2, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &V1(w->apdq, 0, i),
+            &V1(w->bmasdq, 0, i), &V1(w->bpasdq, 0, i));
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++) {
+      GA(m, 0, i) = GA(m, 0, i) - 0.5 * dtdx * V1(w->bmasdq, m, i);
+      GA(m, 1, i) = GA(m, 1, i) - 0.5 * dtdx * V1(w->bpasdq, m, i);
+    }
+  return cfl1d;
+}
+
+/* flux3.f90:168-263 with method(3) < 0 (dimensional splitting): the 0.5 is
+ * folded into cqxx and fadd += cqxx (flux3.f90:244,257). */
+static double flux3_split(slice_ws *w, int ixyz, int mx, const double *q1d, double dtdx,
+                          const int *method, const int *mthlim, int use_fwave,
+                          const orc_rp *p) {
+  const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng;
+  int limit = 0;
+  for (int mw = 0; mw < mwaves; mw++)
+    if (mthlim[mw] > 0) limit = 1;
+  for (int i = 1 - ng; i <= mx + ng; i++)
+    for (int m = 0; m < meqn; m++) {
+      V1(w->qadd, m, i) = 0.0;
+      V1(w->fadd, m, i) = 0.0;
+    }
+  slice_rpn(w, ixyz, mx, q1d, p);
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++) V1(w->qadd, m, i) = V1(w->qadd, m, i) - dtdx * V1(w->apdq, m, i);
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++)
+      V1(w->qadd, m, i - 1) = V1(w->qadd, m, i - 1) - dtdx * V1(w->amdq, m, i);
+  double cfl1d = 0.0;
+  for (int i = 1; i <= mx + 1; i++)
+    for (int mw = 0; mw < mwaves; mw++)
+      cfl1d = dmax(dmax(cfl1d, dtdx * SV(mw, i)), -dtdx * SV(mw, i));
+  if (method[1] == 1) return cfl1d;
+  if (limit) limiter(w, mx, mthlim);
+  for (int i = 2 - ng; i <= mx + ng; i++) {
+    double dtdxave = 0.5 * (dtdx + dtdx);
+    for (int m = 0; m < meqn; m++) V1(w->cqxx, m, i) = 0.0;
+    for (int mw = 0; mw < mwaves; mw++)
+      for (int m = 0; m < meqn; m++) {
+        double sa = fabs(SV(mw, i));
+        double f = use_fwave ? (SV(mw, i) >= 0.0 ? 1.0 : -1.0) : sa;
+        V1(w->cqxx, m, i) = V1(w->cqxx, m, i) + 0.5 * f * (1.0 - sa * dtdxave) * WV(m, mw, i);
+      }
+    for (int m = 0; m < meqn; m++) V1(w->fadd, m, i) = V1(w->fadd, m, i) + V1(w->cqxx, m, i);
+  }
+  return cfl1d;
+}
+
+/* 2D array access q(m, i, j) with ghosts, Fortran order. */
+#define Q2(a, m, i, j) (a)[(m) + (long)meqn * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
+
+/* classic/step2.f90:2-250 (index_capa = 0) */
+double orc_step2(int meqn, int mwaves, int ng, int mx, int my, const double *qold,
+                 double *qnew, double dx, double dy, double dt, const int *method,
+                 const int *mthlim, int use_fwave, const orc_rp *p) {
+  int maxm = mx > my ? mx : my;
+  slice_ws W, *w = &W;
+  ws_alloc(w, meqn, mwaves, ng, maxm);
+  double *q1d = (double *)calloc((long)(maxm + 2 * ng) * meqn, sizeof(double));
+  double cfl = 0.0;
+  double dtdx = dt / dx, dtdy = dt / dy;
+  for (int j = 0; j <= my + 1; j++) {
+    for (int i = 1 - ng; i <= mx + ng; i++)
+      for (int m = 0; m < meqn; m++) V1(q1d, m, i) = Q2(qold, m, i, j);
+    double cfl1d = flux2(w, 1, mx, q1d, dtdx, method, mthlim, use_fwave, p);
+    cfl = dmax(cfl, cfl1d);
+    for (int m = 0; m < meqn; m++)
+      for (int i = 1; i <= mx; i++) {
+        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, i)
+                            - dtdx * (V1(w->fadd, m, i + 1) - V1(w->fadd, m, i))
+                            - dtdy * (GA(m, 1, i) - GA(m, 0, i));
+        Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) - dtdy * GA(m, 0, i);
+        Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) + dtdy * GA(m, 1, i);
+      }
+  }
+  for (int i = 0; i <= mx + 1; i++) {
+    for (int j = 1 - ng; j <= my + ng; j++)
+      for (int m = 0; m < meqn; m++) V1(q1d, m, j) = Q2(qold, m, i, j);
+    double cfl1d = flux2(w, 2, my, q1d, dtdy, method, mthlim, use_fwave, p);
+    cfl = dmax(cfl, cfl1d);
+    for (int m = 0; m < meqn; m++)
+      for (int j = 1; j <= my; j++) {
+        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + (V1(w->qadd, m, j)
+                            - dtdy * (V1(w->fadd, m, j + 1) - V1(w->fadd, m, j))
+                            - dtdx * (GA(m, 1, j) - GA(m, 0, j)));
+        Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) - dtdx * GA(m, 0, j);
+        Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) + dtdx * GA(m, 1, j);
+      }
+  }
+  free(q1d);
+  ws_free(w);
+  return cfl;
+}
+
+/* classic/step2ds.f90:2-264 (index_capa = 0).  x-sweeps cover every row
+ * including ghost rows; y-sweeps cover every column including ghost columns. */
+double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *qold,
+                   double *qnew, double dx, double dy, double dt, const int *method,
+                   const int *mthlim, int ids, int use_fwave, const orc_rp *p) {
+  int maxm = mx > my ? mx : my;
+  slice_ws W, *w = &W;
+  ws_alloc(w, meqn, mwaves, ng, maxm);
+  double *q1d = (double *)calloc((long)(maxm + 2 * ng) * meqn, sizeof(double));
+  double cfl = 0.0;
+  double dtdx = dt / dx, dtdy = dt / dy;
+  if (ids == 1) {
+    for (int j = 1 - ng; j <= my + ng; j++) {
+      for (int i = 1 - ng; i <= mx + ng; i++)
+        for (int m = 0; m < meqn; m++) V1(q1d, m, i) = Q2(qold, m, i, j);
+      double cfl1d = flux2(w, 1, mx, q1d, dtdx, method, mthlim, use_fwave, p);
+      cfl = dmax(cfl, cfl1d);
+      for (int i = 1; i <= mx; i++)
+        for (int m = 0; m < meqn; m++)
+          Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, i)
+                              - dtdx * (V1(w->fadd, m, i + 1) - V1(w->fadd, m, i));
+    }
+  } else if (ids == 2) {
+    for (int i = 1 - ng; i <= mx + ng; i++) {
+      for (int j = 1 - ng; j <= my + ng; j++)
+        for (int m = 0; m < meqn; m++) V1(q1d, m, j) = Q2(qold, m, i, j);
+      double cfl1d = flux2(w, 2, my, q1d, dtdy, method, mthlim, use_fwave, p);
+      cfl = dmax(cfl, cfl1d);
+      for (int j = 1; j <= my; j++)
+        for (int m = 0; m < meqn; m++)
+          Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, j)
+                              - dtdy * (V1(w->fadd, m, j + 1) - V1(w->fadd, m, j));
+    }
+  }
+  free(q1d);
+  ws_free(w);
+  return cfl;
+}
+#undef Q2
+
+#define Q3(a, m, i, j, k)                                                              \
+  (a)[(m) + (long)meqn * (((i) + ng - 1) +                                          \
+                          (long)(mx + 2 * ng) * (((j) + ng - 1) + (long)(my + 2 * ng) * ((k) + ng - 1)))]
+
+/* classic/step3ds.f90:112-378 (index_capa = 0). */
+double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz, const double *qold,
+                   double *qnew, double dx, double dy, double dz, double dt,
+                   const int *method, const int *mthlim, int idir, int use_fwave,
+                   const orc_rp *p) {
+  int maxm = mx > my ? mx : my;
+  if (mz > maxm) maxm = mz;
+  slice_ws W, *w = &W;
+  ws_alloc(w, meqn, mwaves, ng, maxm);
+  double *q1d = (double *)calloc((long)(maxm + 2 * ng) * meqn, sizeof(double));
+  double cfl = 0.0;
+  double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
+  if (idir == 1) {
+    for (int k = 0; k <= mz + 1; k++)
+      for (int j = 0; j <= my + 1; j++) {
+        for (int i = 1 - ng; i <= mx + ng; i++)
+          for (int m = 0; m < meqn; m++) V1(q1d, m, i) = Q3(qold, m, i, j, k);
+        double cfl1d = flux3_split(w, 1, mx, q1d, dtdx, method, mthlim, use_fwave, p);
+        cfl = dmax(cfl, cfl1d);
+        for (int i = 1; i <= mx; i++)
+          for (int m = 0; m < meqn; m++)
+            Q3(qnew, m, i, j, k) = Q3(qnew, m, i, j, k) + V1(w->qadd, m, i)
+                                   - dtdx * (V1(w->fadd, m, i + 1) - V1(w->fadd, m, i));
+      }
+  } else if (idir == 2) {
+    for (int k = 0; k <= mz + 1; k++)
+      for (int i = 0; i <= mx + 1; i++) {
+        for (int j = 1 - ng; j <= my + ng; j++)
+          for (int m = 0; m < meqn; m++) V1(q1d, m, j) = Q3(qold, m, i, j, k);
+        double cfl1d = flux3_split(w, 2, my, q1d, dtdy, method, mthlim, use_fwave, p);
+        cfl = dmax(cfl, cfl1d);
+        for (int j = 1; j <= my; j++)
+          for (int m = 0; m < meqn; m++)
+            Q3(qnew, m, i, j, k) = Q3(qnew, m, i, j, k) + V1(w->qadd, m, j)
+                                   - dtdy * (V1(w->fadd, m, j + 1) - V1(w->fadd, m, j));
+      }
+  } else {
+    for (int j = 0; j <= my + 1; j++)
+      for (int i = 0; i <= mx + 1; i++) {
+        for (int k = 1 - ng; k <= mz + ng; k++)
+          for (int m = 0; m < meqn; m++) V1(q1d, m, k) = Q3(qold, m, i, j, k);
+        double cfl1d = flux3_split(w, 3, mz, q1d, dtdz, method, mthlim, use_fwave, p);
+        cfl = dmax(cfl, cfl1d);
+        for (int k = 1; k <= mz; k++)
+          for (int m = 0; m < meqn; m++)
+            Q3(qnew, m, i, j, k) = Q3(qnew, m, i, j, k) + V1(w->qadd, m, k)
+                                   - dtdz * (V1(w->fadd, m, k + 1) - V1(w->fadd, m, k));
+      }
+  }
+  free(q1d);
+  ws_free(w);
+  return cfl;
+}
+#undef Q3
+#undef IX
+#undef V1
+#undef WV
+#undef SV
+#undef GA
+
+/* ======================================================================
+ * SharpClaw: WENO5, characteristic WENO5, eigenvectors, flux1, flux2
+ * All slice arrays here are 1-based of length n2 (as the reference's
+ * explicit-shape dummies q(num_eqn, maxnx+2*num_ghost) see them).
+ * ====================================================================== */
+#define B1(a, m, i) (a)[(m) + (long)meqn * ((i) - 1)]
+
+/* weno.f90:5-83 (PyWENO-generated), cells i = ng .. n2-ng+1 */
+void orc_weno5(int meqn, int ng, int n2, const double *q, double *ql, double *qr) {
+  for (int i = ng; i <= n2 - ng + 1; i++) {
+    for (int m = 0; m < meqn; m++) {
+      double qm2 = B1(q, m, i - 2), qm1 = B1(q, m, i - 1), q0 = B1(q, m, i + 0);
+      double qp1 = B1(q, m, i + 1), qp2 = B1(q, m, i + 2);
+      double sigma0 = 3.3333333333333333333333333333333333 * (q0 * q0) -
+                      10.333333333333333333333333333333333 * q0 * qp1 +
+                      3.6666666666666666666666666666666667 * q0 * qp2 +
+                      8.3333333333333333333333333333333333 * (qp1 * qp1) -
+                      6.3333333333333333333333333333333333 * qp1 * qp2 +
+                      1.3333333333333333333333333333333333 * (qp2 * qp2);
+      double sigma1 = 4.3333333333333333333333333333333333 * (q0 * q0) -
+                      4.3333333333333333333333333333333333 * q0 * qp1 -
+                      4.3333333333333333333333333333333333 * q0 * qm1 +
+                      1.3333333333333333333333333333333333 * (qp1 * qp1) +
+                      1.6666666666666666666666666666666667 * qp1 * qm1 +
+                      1.3333333333333333333333333333333333 * (qm1 * qm1);
+      double sigma2 = 3.3333333333333333333333333333333333 * (q0 * q0) -
+                      10.333333333333333333333333333333333 * q0 * qm1 +
+                      3.6666666666666666666666666666666667 * q0 * qm2 +
+                      8.3333333333333333333333333333333333 * (qm1 * qm1) -
+                      6.3333333333333333333333333333333333 * qm1 * qm2 +
+                      1.3333333333333333333333333333333333 * (qm2 * qm2);
+      double d0 = EPWENO + sigma0, d1 = EPWENO + sigma1, d2 = EPWENO + sigma2;
+      double omega0 = 0.1 / (d0 * d0);
+      double omega1 = 0.6 / (d1 * d1);
+      double omega2 = 0.3 / (d2 * d2);
+      double omega3 = 0.3 / (d0 * d0);
+      double omega4 = 0.6 / (d1 * d1);
+      double omega5 = 0.1 / (d2 * d2);
+      double fr0 = 1.8333333333333333333333333333333333 * q0 -
+                   1.1666666666666666666666666666666667 * qp1 +
+                   0.33333333333333333333333333333333333 * qp2;
+      double fr1 = 0.83333333333333333333333333333333333 * q0 -
+                   0.16666666666666666666666666666666667 * qp1 +
+                   0.33333333333333333333333333333333333 * qm1;
+      double fr2 = 0.33333333333333333333333333333333333 * q0 +
+                   0.83333333333333333333333333333333333 * qm1 -
+                   0.16666666666666666666666666666666667 * qm2;
+      double fr3 = 0.33333333333333333333333333333333333 * q0 +
+                   0.83333333333333333333333333333333333 * qp1 -
+                   0.16666666666666666666666666666666667 * qp2;
+      double fr4 = 0.83333333333333333333333333333333333 * q0 +
+                   0.33333333333333333333333333333333333 * qp1 -
+                   0.16666666666666666666666666666666667 * qm1;
+      double fr5 = 1.8333333333333333333333333333333333 * q0 -
+                   1.1666666666666666666666666666666667 * qm1 +
+                   0.33333333333333333333333333333333333 * qm2;
+      B1(ql, m, i) = (fr0 * omega0 + fr1 * omega1 + fr2 * omega2) / (omega0 + omega1 + omega2);
+      B1(qr, m, i) = (fr3 * omega3 + fr4 * omega4 + fr5 * omega5) / (omega3 + omega4 + omega5);
+    }
+  }
+}
+
+/* reconstruct.f90:201-299 */
+void orc_weno5_char(int meqn, int ng, int n2, const double *q, double *ql, double *qr,
+                    const double *evl, const double *evr) {
+  double *dq = (double *)calloc((long)meqn * (n2 + 1), sizeof(double));
+  double *hh = (double *)calloc(5L * (n2 + 1), sizeof(double));
+  double *uu = (double *)calloc(2L * (n2 + 1), sizeof(double));
+#define EVL(ip, m, i) evl[(ip) + (long)meqn * ((m) + (long)meqn * ((i) - 1))]
+#define EVR(m, ip, i) evr[(m) + (long)meqn * ((ip) + (long)meqn * ((i) - 1))]
+#define HH(m2, i) hh[((m2) + 2) + 5L * (i)]
+#define DQ(m, i) dq[(m) + (long)meqn * (i)]
+#define UU(k, i) uu[((k) - 1) + 2L * (i)]
+  for (int i = 2; i <= n2; i++)
+    for (int m = 0; m < meqn; m++) DQ(m, i) = B1(q, m, i) - B1(q, m, i - 1);
+  for (int i = ng; i <= n2 - ng + 1; i++)
+    for (int m = 0; m < meqn; m++) {
+      B1(qr, m, i - 1) = (-B1(q, m, i - 2) + 7.0 * (B1(q, m, i - 1) + B1(q, m, i)) - B1(q, m, i + 1)) / 12.0;
+      B1(ql, m, i) = B1(qr, m, i - 1);
+    }
+  for (int ip = 0; ip < meqn; ip++) {
+    for (int m2 = -2; m2 <= 2; m2++)
+      for (int i = ng + 1; i <= n2 - 2; i++) {
+        HH(m2, i) = 0.0;
+        for (int m = 0; m < meqn; m++) HH(m2, i) = HH(m2, i) + EVL(ip, m, i) * DQ(m, i + m2);
+      }
+    for (int m1 = 1; m1 <= 2; m1++) {
+      int im = m1 == 1 ? 1 : -1;
+      int ione = im, inone = -im, intwo = -2 * im;
+      for (int i = ng + 1; i <= n2 - ng + 1; i++) {
+        double t1 = im * (HH(intwo, i) - HH(inone, i));
+        double t2 = im * (HH(inone, i) - HH(0, i));
+        double t3 = im * (HH(0, i) - HH(ione, i));
+        double e1 = HH(intwo, i) - 3.0 * HH(inone, i);
+        double e2 = HH(inone, i) + HH(0, i);
+        double e3 = 3.0 * HH(0, i) - HH(ione, i);
+        double tt1 = 13.0 * (t1 * t1) + 3.0 * (e1 * e1);
+        double tt2 = 13.0 * (t2 * t2) + 3.0 * (e2 * e2);
+        double tt3 = 13.0 * (t3 * t3) + 3.0 * (e3 * e3);
+        tt1 = (EPWENO + tt1) * (EPWENO + tt1);
+        tt2 = (EPWENO + tt2) * (EPWENO + tt2);
+        tt3 = (EPWENO + tt3) * (EPWENO + tt3);
+        double s1 = tt2 * tt3;
+        double s2 = 6.0 * tt1 * tt3;
+        double s3 = 3.0 * tt1 * tt2;
+        double t0 = 1.0 / (s1 + s2 + s3);
+        s1 = s1 * t0;
+        s3 = s3 * t0;
+        UU(m1, i) = (s1 * (t2 - t1) + (0.5 * s3 - 0.25) * (t3 - t2)) / 3.0;
+      }
+    }
+    for (int m = 0; m < meqn; m++)
+      for (int i = ng + 1; i <= n2 - ng + 1; i++) {
+        B1(qr, m, i - 1) = B1(qr, m, i - 1) + EVR(m, ip, i) * UU(1, i);
+        B1(ql, m, i) = B1(ql, m, i) + EVR(m, ip, i) * UU(2, i);
+      }
+  }
+#undef EVL
+#undef EVR
+#undef HH
+#undef DQ
+#undef UU
+  free(dq); free(hh); free(uu);
+}
+
+/* Eigenvectors of the flux Jacobian in direction ixy at the Roe average of cells
+ * i-1, i (evec ABI: examples/euler_1d/evec.f90:2-21).  The reference ships only
+ * a stub for 2D (sharpclaw/evec.f90:5-15); these are the textbook shallow-water
+ * and Euler eigenvectors, with the Roe averages of the Riemann solvers above. */
+void orc_evec(const orc_rp *p, int ixy, int meqn, int n2, const double *q, double *evl,
+              double *evr) {
+#define EVL(ip, m, i) evl[(ip) + (long)meqn * ((m) + (long)meqn * ((i) - 1))]
+#define EVR(m, ip, i) evr[(m) + (long)meqn * ((ip) + (long)meqn * ((i) - 1))]
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  for (int i = 2; i <= n2; i++) {
+    const double *qL = &B1(q, 0, i - 1), *qR = &B1(q, 0, i);
+    for (int a = 0; a < meqn; a++)
+      for (int b = 0; b < meqn; b++) { EVL(a, b, i) = 0.0; EVR(a, b, i) = 0.0; }
+    if (p->kind == ORC_RP_SHALLOW) {
+      double h = (qL[0] + qR[0]) * 0.50;
+      double hsqrtl = sqrt(qL[0]), hsqrtr = sqrt(qR[0]);
+      double hsq2 = hsqrtl + hsqrtr;
+      double u = (qL[mu] / hsqrtl + qR[mu] / hsqrtr) / hsq2;
+      double v = (qL[mv] / hsqrtl + qR[mv] / hsqrtr) / hsq2;
+      double c = sqrt(p->grav * h);
+      /* R columns: (1, u-c, v), (0, 0, 1), (1, u+c, v) in (h, normal, transverse) */
+      EVR(0, 0, i) = 1.0; EVR(mu, 0, i) = u - c; EVR(mv, 0, i) = v;
+      EVR(0, 1, i) = 0.0; EVR(mu, 1, i) = 0.0; EVR(mv, 1, i) = 1.0;
+      EVR(0, 2, i) = 1.0; EVR(mu, 2, i) = u + c; EVR(mv, 2, i) = v;
+      double r2c = 0.5 / c;
+      EVL(0, 0, i) = (u + c) * r2c; EVL(0, mu, i) = -r2c; EVL(0, mv, i) = 0.0;
+      EVL(1, 0, i) = -v; EVL(1, mu, i) = 0.0; EVL(1, mv, i) = 1.0;
+      EVL(2, 0, i) = -(u - c) * r2c; EVL(2, mu, i) = r2c; EVL(2, mv, i) = 0.0;
+    } else if (p->kind == ORC_RP_EULER) {
+      const double gamma1 = p->gamma - 1.0;
+      roe2 r = roe_euler2(gamma1, mu, mv, qL, qR);
+      double u = r.u, v = r.v, c = r.a, H = r.enth;
+      double q2 = 0.5 * r.u2v2;
+      EVR(0, 0, i) = 1.0; EVR(mu, 0, i) = u - c; EVR(mv, 0, i) = v; EVR(3, 0, i) = H - u * c;
+      EVR(0, 1, i) = 0.0; EVR(mu, 1, i) = 0.0; EVR(mv, 1, i) = 1.0; EVR(3, 1, i) = v;
+      EVR(0, 2, i) = 1.0; EVR(mu, 2, i) = u; EVR(mv, 2, i) = v; EVR(3, 2, i) = q2;
+      EVR(0, 3, i) = 1.0; EVR(mu, 3, i) = u + c; EVR(mv, 3, i) = v; EVR(3, 3, i) = H + u * c;
+      double b1 = gamma1 / (c * c), b2 = q2 * b1;
+      double r2c = 0.5 / c;
+      EVL(0, 0, i) = 0.5 * b2 + u * r2c; EVL(0, mu, i) = -(0.5 * b1 * u + r2c);
+      EVL(0, mv, i) = -0.5 * b1 * v; EVL(0, 3, i) = 0.5 * b1;
+      EVL(1, 0, i) = -v; EVL(1, mu, i) = 0.0; EVL(1, mv, i) = 1.0; EVL(1, 3, i) = 0.0;
+      EVL(2, 0, i) = 1.0 - b2; EVL(2, mu, i) = b1 * u; EVL(2, mv, i) = b1 * v; EVL(2, 3, i) = -b1;
+      EVL(3, 0, i) = 0.5 * b2 - u * r2c; EVL(3, mu, i) = -(0.5 * b1 * u - r2c);
+      EVL(3, mv, i) = -0.5 * b1 * v; EVL(3, 3, i) = 0.5 * b1;
+    } else {
+      for (int a = 0; a < meqn; a++) { EVL(a, a, i) = 1.0; EVR(a, a, i) = 1.0; }
+    }
+  }
+#undef EVL
+#undef EVR
+}
+
+/* sharpclaw/flux1.f90:76-268 with lim_type = 2, tfluct_solver = False, no capacity. */
+double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy, const double *q1d,
+                       double *dq1d, double dtdx, int char_decomp, const orc_rp *p) {
+  const int n2 = n + 2 * ng;
+  double *ql = (double *)calloc((long)meqn * n2, sizeof(double));
+  double *qr = (double *)calloc((long)meqn * n2, sizeof(double));
+  for (long k = 0; k < (long)meqn * n2; k++) dq1d[k] = 0.0;
+  if (char_decomp == 2) {
+    double *evl = (double *)calloc((long)meqn * meqn * n2, sizeof(double));
+    double *evr = (double *)calloc((long)meqn * meqn * n2, sizeof(double));
+    orc_evec(p, ixy, meqn, n2, q1d, evl, evr);
+    orc_weno5_char(meqn, ng, n2, q1d, ql, qr, evl, evr);
+    free(evl); free(evr);
+  } else {
+    orc_weno5(meqn, ng, n2, q1d, ql, qr);
+  }
+  /* slice cell c (1-ng..n+ng) is 1-based index c+ng */
+  double wave[5 * 5], s[5], amdq[5], apdq[5], amdq_n[5];
+  double cfl = 0.0;
+  /* interface c (1..n+1) between cells c-1 and c: left qr(c-1), right ql(c) */
+  double *amdq_i = (double *)calloc((long)meqn * (n + 2), sizeof(double));
+  double *apdq_i = (double *)calloc((long)meqn * (n + 2), sizeof(double));
+  for (int c = 1; c <= n + 1; c++) {
+    orc_rpn(p, ixy, &B1(qr, 0, c - 1 + ng), &B1(ql, 0, c + ng), wave, s, amdq, apdq);
+    for (int m = 0; m < meqn; m++) {
+      amdq_i[m + (long)meqn * c] = amdq[m];
+      apdq_i[m + (long)meqn * c] = apdq[m];
+    }
+    for (int mw = 0; mw < mwaves; mw++)
+      cfl = dmax(dmax(cfl, dtdx * s[mw]), -dtdx * s[mw]);
+  }
+  /* the CFL loop in the reference is mw-outer; max is exact so the order is free */
+  for (int c = 1; c <= n; c++) {
+    double amdq2[5], apdq2[5];
+    orc_rpn(p, ixy, &B1(ql, 0, c + ng), &B1(qr, 0, c + ng), wave, s, amdq2, apdq2);
+    for (int m = 0; m < meqn; m++) {
+      amdq_n[m] = amdq_i[m + (long)meqn * (c + 1)];
+      B1(dq1d, m, c + ng) = B1(dq1d, m, c + ng) -
+                            dtdx * (amdq_n[m] + apdq_i[m + (long)meqn * c] + amdq2[m] + apdq2[m]);
+    }
+  }
+  free(amdq_i); free(apdq_i); free(ql); free(qr);
+  return cfl;
+}
+
+#define Q2(a, m, i, j) (a)[(m) + (long)meqn * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
+/* sharpclaw/flux2.f90:2-74; dq is zeroed first (f2py optional intent(in,out)). */
+double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my, const double *q,
+                       double *dq, double dx, double dy, double dt, int char_decomp,
+                       const orc_rp *p) {
+  int maxm = mx > my ? mx : my;
+  long nq = (long)meqn * (mx + 2 * ng) * (my + 2 * ng);
+  for (long k = 0; k < nq; k++) dq[k] = 0.0;
+  double *q1d = (double *)calloc((long)meqn * (maxm + 2 * ng), sizeof(double));
+  double *dq1d = (double *)calloc((long)meqn * (maxm + 2 * ng), sizeof(double));
+  double cfl = 0.0;
+  double dtdx = dt / dx, dtdy = dt / dy;
+  for (int j = 0; j <= my + 1; j++) {
+    for (int i = 1 - ng; i <= mx + ng; i++)
+      for (int m = 0; m < meqn; m++) B1(q1d, m, i + ng) = Q2(q, m, i, j);
+    double c1 = orc_sharp_flux1(meqn, mwaves, ng, mx, 1, q1d, dq1d, dtdx, char_decomp, p);
+    cfl = dmax(cfl, c1);
+    for (int i = 1; i <= mx; i++)
+      for (int m = 0; m < meqn; m++) Q2(dq, m, i, j) = Q2(dq, m, i, j) + B1(dq1d, m, i + ng);
+  }
+  for (int i = 0; i <= mx + 1; i++) {
+    for (int j = 1 - ng; j <= my + ng; j++)
+      for (int m = 0; m < meqn; m++) B1(q1d, m, j + ng) = Q2(q, m, i, j);
+    double c1 = orc_sharp_flux1(meqn, mwaves, ng, my, 2, q1d, dq1d, dtdy, char_decomp, p);
+    cfl = dmax(cfl, c1);
+    for (int j = 1; j <= my; j++)
+      for (int m = 0; m < meqn; m++) Q2(dq, m, i, j) = Q2(dq, m, i, j) + B1(dq1d, m, j + ng);
+  }
+  free(q1d); free(dq1d);
+  return cfl;
+}
+
+void orc_interior_sums2(const double *q, int meqn, int mx, int my, int ng, double *sums) {
+  for (int m = 0; m < meqn; m++) sums[m] = 0.0;
+  for (int j = 1; j <= my; j++)
+    for (int i = 1; i <= mx; i++)
+      for (int m = 0; m < meqn; m++) sums[m] += Q2(q, m, i, j);
+}
+#undef Q2
+#undef B1

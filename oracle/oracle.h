@@ -1,0 +1,117 @@
+/* oracle.h -- CPU restatement (FP64, serial, strict IEEE evaluation order) of the
+ * PyClaw time-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (pyclaw_amd/, libclawhip.so)
+ * links, loads or calls this code.  It is the checker that tests/, the smoke test
+ * and bench.py's cpu_baseline leg compare the HIP path against.
+ *
+ * Array layout is the reference's: Fortran column-major with the equation index
+ * fastest, ghost cells included, e.g. q(meqn, 1-ng:mx+ng, 1-ng:my+ng)
+ * (classic/step2.f90:21-24).  Indices in the comments below are the reference's
+ * 1-based Fortran indices.
+ */
+#ifndef CLAW_ORACLE_H
+#define CLAW_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Riemann solver identifiers (the clawpack.riemann modules the examples use). */
+enum {
+  ORC_RP_ACOUSTICS = 1,  /* acoustics_2D:            meqn 3, mwaves 2 */
+  ORC_RP_EULER = 2,      /* euler_4wave_2D:          meqn 4, mwaves 4 */
+  ORC_RP_SHALLOW = 3,    /* shallow_roe_with_efix_2D: meqn 3, mwaves 3 */
+  ORC_RP_ADVECTION = 4,  /* advection_2D:            meqn 1, mwaves 1 */
+  ORC_RP_EULER3 = 5      /* euler_3D:                meqn 5, mwaves 3 */
+};
+
+/* Problem parameters: the reference passes these through the Fortran /cparam/
+ * common block filled from state.problem_data (state.py:206-231). */
+typedef struct {
+  int kind;
+  int efix;                     /* Harten-Hyman entropy fix on/off (Euler, SWE) */
+  double rho, bulk, cc, zz;     /* acoustics */
+  double gamma;                 /* Euler */
+  double grav;                  /* shallow water */
+  double u, v;                  /* advection velocities */
+} orc_rp;
+
+/* Boundary condition codes: pyclaw.BC (solver.py:13-19). */
+enum { ORC_BC_CUSTOM = 0, ORC_BC_EXTRAP = 1, ORC_BC_PERIODIC = 2, ORC_BC_WALL = 3 };
+
+/* --- pointwise Riemann solvers --------------------------------------------
+ * qL = state left of the interface (qr(:,i-1) in the reference ABI),
+ * qR = state right of it (ql(:,i)).  wave[mw*meqn + m] = wave(m,mw,i). */
+void orc_rpn(const orc_rp *p, int ixy, const double *qL, const double *qR,
+             double *wave, double *s, double *amdq, double *apdq);
+void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
+             const double *asdq, double *bmasdq, double *bpasdq);
+int orc_rp_meqn(int kind);
+int orc_rp_mwaves(int kind);
+
+/* --- boundary conditions (solver.py:351-519) ------------------------------
+ * Fills the ghost cells of qbc in place: dimension by dimension (x, then y,
+ * then z), lower side before upper side.  reflect[d] is the component negated
+ * by a wall in dimension d (solver.reflect_index). */
+void orc_bc2(double *qbc, int meqn, int mx, int my, int ng,
+             const int *bc_lower, const int *bc_upper, const int *reflect);
+void orc_bc3(double *qbc, int meqn, int mx, int my, int mz, int ng,
+             const int *bc_lower, const int *bc_upper, const int *reflect);
+
+/* --- classic Clawpack steps -----------------------------------------------
+ * method[7] and mthlim[mwaves] as built by ClawSolver._set_method /
+ * _set_mthlim (classic/solver.py:162-193).  qold and qnew may alias for
+ * step2ds(ids=2) and step3ds(idir=2,3), exactly as the reference calls them. */
+double orc_step2(int meqn, int mwaves, int ng, int mx, int my,
+                 const double *qold, double *qnew, double dx, double dy, double dt,
+                 const int *method, const int *mthlim, int use_fwave, const orc_rp *p);
+double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my,
+                   const double *qold, double *qnew, double dx, double dy, double dt,
+                   const int *method, const int *mthlim, int ids, int use_fwave,
+                   const orc_rp *p);
+double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz,
+                   const double *qold, double *qnew, double dx, double dy, double dz,
+                   double dt, const int *method, const int *mthlim, int idir,
+                   int use_fwave, const orc_rp *p);
+
+/* --- SharpClaw semi-discrete operator (sharpclaw/flux2.f90) ----------------
+ * dq (same shape as q, ghosts included) is overwritten: dq = dt * L(q) on the
+ * interior, 0 elsewhere.  char_decomp 0: component WENO5 (weno.f90:5-83);
+ * 2: characteristic WENO5 (reconstruct.f90:201-299) with the eigenvectors of
+ * orc_evec.  Returns the CFL number. */
+double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my,
+                       const double *q, double *dq, double dx, double dy, double dt,
+                       int char_decomp, const orc_rp *p);
+
+/* One slice of the SharpClaw operator (sharpclaw/flux1.f90) for a slice of n
+ * cells plus ng ghosts each side: q1d(meqn, 1-ng:n+ng) -> dq1d(meqn, 1-ng:n+ng).
+ * Returns the slice CFL. */
+double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy,
+                       const double *q1d, double *dq1d, double dtdx,
+                       int char_decomp, const orc_rp *p);
+
+/* Eigenvectors at each interface of a slice (evec ABI of
+ * examples/euler_1d/evec.f90:2-21, with a direction argument added):
+ * q(meqn, n2) 1-based; evl/evr(meqn, meqn, n2) filled for i = 2..n2 with the
+ * Roe average of cells i-1 and i.  evl(ip,m,i) = row ip of R^-1. */
+void orc_evec(const orc_rp *p, int ixy, int meqn, int n2, const double *q,
+              double *evl, double *evr);
+
+/* Characteristic WENO5 of one slice (reconstruct.f90:201-299).  All arrays are
+ * 1-based of length n2 = size(q,2): q(meqn,n2), ql/qr(meqn,n2),
+ * evl/evr(meqn,meqn,n2).  Interfaces i = ng+1 .. n2-ng+1 are reconstructed
+ * (interface ng, whose projection the reference reads before it is computed,
+ * is left alone; flux1 never uses it). */
+void orc_weno5_char(int meqn, int ng, int n2, const double *q, double *ql, double *qr,
+                    const double *evl, const double *evr);
+/* Component WENO5 (weno.f90:5-83) over cells i = ng .. n2-ng+1 (1-based). */
+void orc_weno5(int meqn, int ng, int n2, const double *q, double *ql, double *qr);
+
+/* Sums over the interior cells (conservation checks), FP64, row order. */
+void orc_interior_sums2(const double *q, int meqn, int mx, int my, int ng, double *sums);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""ctypes bindings for the oracle (oracle/liboracle.so) and, where it has been
+built, the reference's own compiled Fortran step (oracle/_ref/*.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()`` and
+by ``bench.py``'s cpu_baseline leg, as the checker -- never by the product
+package ``pyclaw_amd``.
+
+All arrays use the reference layout: numpy float64, Fortran order, equation
+index first, ghost cells included, e.g. ``qbc.shape == (meqn, mx+2ng, my+2ng)``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_CLASSIC_PATH = os.path.join(HERE, "_ref", "libref_classic.so")
+REF_SHARP_PATH = os.path.join(HERE, "_ref", "libref_sharp.so")
+
+ACOUSTICS, EULER, SHALLOW, ADVECTION, EULER3 = 1, 2, 3, 4, 5
+CUSTOM, EXTRAP, PERIODIC, WALL = 0, 1, 2, 3
+
+MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5}
+MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3}
+
+
+class RP(ctypes.Structure):
+    """Mirror of ``orc_rp`` (oracle.h)."""
+    _fields_ = [("kind", ctypes.c_int), ("efix", ctypes.c_int),
+                ("rho", ctypes.c_double), ("bulk", ctypes.c_double),
+                ("cc", ctypes.c_double), ("zz", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("grav", ctypes.c_double),
+                ("u", ctypes.c_double), ("v", ctypes.c_double)]
+
+
+def make_rp(kind, efix=1, rho=1.0, bulk=4.0, gamma=1.4, grav=1.0, u=0.5, v=1.0):
+    cc = float(np.sqrt(bulk / rho))
+    return RP(kind, efix, rho, bulk, cc, rho * cc, gamma, grav, u, v)
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1)
+    return a.ctypes.data_as(_D)
+
+
+def _ip(seq):
+    arr = (ctypes.c_int * len(seq))(*[int(x) for x in seq])
+    return arr
+
+
+_lib = None
+_ref_classic = None
+_ref_sharp = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library not built: {LIB_PATH} (run make -f oracle/Makefile)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_step2.restype = ctypes.c_double
+        L.orc_step2ds.restype = ctypes.c_double
+        L.orc_step3ds.restype = ctypes.c_double
+        L.orc_sharp_flux2.restype = ctypes.c_double
+        L.orc_sharp_flux1.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_CLASSIC_PATH) and os.path.exists(REF_SHARP_PATH)
+
+
+def ref_classic():
+    global _ref_classic
+    if _ref_classic is None:
+        L = ctypes.CDLL(REF_CLASSIC_PATH)
+        L.ref_step2.restype = ctypes.c_double
+        L.ref_step3ds.restype = ctypes.c_double
+        _ref_classic = L
+    return _ref_classic
+
+
+def ref_sharp():
+    global _ref_sharp
+    if _ref_sharp is None:
+        L = ctypes.CDLL(REF_SHARP_PATH)
+        L.ref_sharp_flux2.restype = ctypes.c_double
+        L.ref_sharp_flux1.restype = ctypes.c_double
+        _ref_sharp = L
+    return _ref_sharp
+
+
+def method_array(order=2, transverse=2, dimensional_split=False, dt_variable=True):
+    """ClawSolver._set_method (classic/solver.py:173-193) for 2D/3D, no aux."""
+    m = [int(dt_variable), order, -1 if dimensional_split else transverse, 0, 0, 0, 0]
+    return m
+
+
+# ---------------------------------------------------------------- BCs
+def bc2(qbc, ng, bc_lower, bc_upper, reflect=(1, 2)):
+    meqn, nx, ny = qbc.shape
+    lib().orc_bc2(_dp(qbc), meqn, nx - 2 * ng, ny - 2 * ng, ng, _ip(bc_lower), _ip(bc_upper),
+                  _ip(reflect))
+    return qbc
+
+
+def bc3(qbc, ng, bc_lower, bc_upper, reflect=(1, 2, 3)):
+    meqn, nx, ny, nz = qbc.shape
+    lib().orc_bc3(_dp(qbc), meqn, nx - 2 * ng, ny - 2 * ng, nz - 2 * ng, ng, _ip(bc_lower),
+                  _ip(bc_upper), _ip(reflect))
+    return qbc
+
+
+def with_ghosts(q, ng):
+    """Interior q (meqn, mx, my[, mz]) -> zero-padded qbc in Fortran order."""
+    shape = (q.shape[0],) + tuple(n + 2 * ng for n in q.shape[1:])
+    qbc = np.zeros(shape, order="F")
+    sl = (slice(None),) + tuple(slice(ng, -ng) for _ in q.shape[1:])
+    qbc[sl] = q
+    return qbc
+
+
+def interior(qbc, ng):
+    sl = (slice(None),) + tuple(slice(ng, -ng) for _ in qbc.shape[1:])
+    return np.asfortranarray(qbc[sl])
+
+
+# ---------------------------------------------------------------- classic
+def step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0, use_fwave=False):
+    """ids=0: unsplit step2; ids=1/2: step2ds sweep.  qnew updated in place."""
+    meqn, nx, ny = qold.shape
+    mx, my = nx - 2 * ng, ny - 2 * ng
+    mw = MWAVES[rp.kind]
+    if ids == 0:
+        return lib().orc_step2(meqn, mw, ng, mx, my, _dp(qold), _dp(qnew), ctypes.c_double(dx),
+                               ctypes.c_double(dy), ctypes.c_double(dt), _ip(method),
+                               _ip(mthlim), int(use_fwave), ctypes.byref(rp))
+    return lib().orc_step2ds(meqn, mw, ng, mx, my, _dp(qold), _dp(qnew), ctypes.c_double(dx),
+                             ctypes.c_double(dy), ctypes.c_double(dt), _ip(method), _ip(mthlim),
+                             int(ids), int(use_fwave), ctypes.byref(rp))
+
+
+def step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir, use_fwave=False):
+    meqn, nx, ny, nz = qold.shape
+    return lib().orc_step3ds(meqn, MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, nz - 2 * ng,
+                             _dp(qold), _dp(qnew), ctypes.c_double(dx), ctypes.c_double(dy),
+                             ctypes.c_double(dz), ctypes.c_double(dt), _ip(method), _ip(mthlim),
+                             int(idir), int(use_fwave), ctypes.byref(rp))
+
+
+# ---------------------------------------------------------------- SharpClaw
+def sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp):
+    meqn, nx, ny = q.shape
+    dq = np.zeros_like(q, order="F")
+    cfl = lib().orc_sharp_flux2(meqn, MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, _dp(q),
+                                _dp(dq), ctypes.c_double(dx), ctypes.c_double(dy),
+                                ctypes.c_double(dt), int(char_decomp), ctypes.byref(rp))
+    return dq, cfl
+
+
+def sharp_flux1(rp, q1d, ng, ixy, dtdx, char_decomp):
+    meqn, n2 = q1d.shape
+    dq1d = np.zeros_like(q1d, order="F")
+    cfl = lib().orc_sharp_flux1(meqn, MWAVES[rp.kind], ng, n2 - 2 * ng, ixy, _dp(q1d),
+                                _dp(dq1d), ctypes.c_double(dtdx), int(char_decomp),
+                                ctypes.byref(rp))
+    return dq1d, cfl
+
+
+def evec(rp, ixy, q1d):
+    meqn, n2 = q1d.shape
+    evl = np.zeros((meqn, meqn, n2), order="F")
+    evr = np.zeros((meqn, meqn, n2), order="F")
+    lib().orc_evec(ctypes.byref(rp), ixy, meqn, n2, _dp(q1d), _dp(evl), _dp(evr))
+    return evl, evr
+
+
+def rpn(rp, ixy, qL, qR):
+    meqn, mw = MEQN[rp.kind], MWAVES[rp.kind]
+    wave = np.zeros(meqn * mw)
+    s = np.zeros(mw)
+    amdq = np.zeros(meqn)
+    apdq = np.zeros(meqn)
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    lib().orc_rpn(ctypes.byref(rp), ixy, _dp(qL), _dp(qR), _dp(wave), _dp(s), _dp(amdq),
+                  _dp(apdq))
+    return wave.reshape(mw, meqn), s, amdq, apdq
+
+
+# ---------------------------------------------------------------- reference build
+def ref_step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0):
+    meqn, nx, ny = qold.shape
+    return ref_classic().ref_step2(ctypes.byref(rp), int(ids), meqn, MWAVES[rp.kind], ng,
+                                   nx - 2 * ng, ny - 2 * ng, _dp(qold), _dp(qnew),
+                                   ctypes.c_double(dx), ctypes.c_double(dy),
+                                   ctypes.c_double(dt), _ip(method), _ip(mthlim))
+
+
+def ref_step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir):
+    meqn, nx, ny, nz = qold.shape
+    return ref_classic().ref_step3ds(ctypes.byref(rp), int(idir), meqn, MWAVES[rp.kind], ng,
+                                     nx - 2 * ng, ny - 2 * ng, nz - 2 * ng, _dp(qold),
+                                     _dp(qnew), ctypes.c_double(dx), ctypes.c_double(dy),
+                                     ctypes.c_double(dz), ctypes.c_double(dt), _ip(method),
+                                     _ip(mthlim))
+
+
+def ref_sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp):
+    meqn, nx, ny = q.shape
+    dq = np.zeros_like(q, order="F")
+    cfl = ref_sharp().ref_sharp_flux2(ctypes.byref(rp), int(char_decomp), meqn,
+                                      MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, _dp(q),
+                                      _dp(dq), ctypes.c_double(dx), ctypes.c_double(dy),
+                                      ctypes.c_double(dt))
+    return dq, cfl
+
+
+def ref_sharp_flux1(rp, q1d, ng, ixy, dxd, dt, char_decomp):
+    meqn, n2 = q1d.shape
+    dq1d = np.zeros_like(q1d, order="F")
+    cfl = ref_sharp().ref_sharp_flux1(ctypes.byref(rp), int(char_decomp), meqn,
+                                      MWAVES[rp.kind], ng, n2 - 2 * ng, int(ixy), _dp(q1d),
+                                      _dp(dq1d), ctypes.c_double(dxd), ctypes.c_double(dt))
+    return dq1d, cfl

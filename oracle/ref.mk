@@ -1,0 +1,38 @@
+# Builds the reference's own Fortran hot-path sources, where they lie under
+# /root/reference, into oracle/_ref/ (test infrastructure; never shipped in the
+# product).  The classic and SharpClaw routines go into separate libraries
+# because both define `flux2`.  The C shim supplies the Riemann callbacks and the
+# user `evec` the reference routines take (see ref_shim.c).
+REF ?= /root/reference/src/pyclaw
+FC = /opt/rocm/bin/amdflang
+CC ?= gcc
+HERE := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
+OUT := $(HERE)_ref
+FFLAGS = -O2 -fPIC -ffp-contract=off
+CFLAGS = -O2 -fPIC -ffp-contract=off -std=c11
+
+CLASSIC = $(REF)/classic/step2.f90 $(REF)/classic/flux2.f90 $(REF)/classic/limiter.f90 \
+          $(REF)/classic/philim.f90 $(REF)/classic/step2ds.f90 $(REF)/classic/step3ds.f90 \
+          $(REF)/classic/flux3.f90
+SHARP = $(REF)/sharpclaw/ClawParams.f90 $(REF)/sharpclaw/workspace.f90 \
+        $(REF)/sharpclaw/weno.f90 $(REF)/sharpclaw/reconstruct.f90 \
+        $(REF)/sharpclaw/flux1.f90 $(REF)/sharpclaw/flux2.f90
+
+all: $(OUT)/libref_classic.so $(OUT)/libref_sharp.so
+
+$(OUT)/libref_classic.so: $(CLASSIC) $(HERE)ref_shim.c $(HERE)oracle.c $(HERE)oracle.h
+	mkdir -p $(OUT)/classic
+	cd $(OUT)/classic && $(FC) $(FFLAGS) -c $(CLASSIC)
+	$(CC) $(CFLAGS) -DREF_CLASSIC -c $(HERE)ref_shim.c -o $(OUT)/classic/ref_shim.o
+	$(CC) $(CFLAGS) -c $(HERE)oracle.c -o $(OUT)/classic/oracle.o
+	$(FC) -shared -fPIC -Wl,-Bsymbolic -o $@ $(OUT)/classic/*.o -lm
+
+$(OUT)/libref_sharp.so: $(SHARP) $(HERE)ref_sharp_driver.f90 $(HERE)ref_shim.c $(HERE)oracle.c $(HERE)oracle.h
+	mkdir -p $(OUT)/sharp
+	cd $(OUT)/sharp && $(FC) $(FFLAGS) -c $(SHARP) $(HERE)ref_sharp_driver.f90
+	$(CC) $(CFLAGS) -DREF_SHARP -c $(HERE)ref_shim.c -o $(OUT)/sharp/ref_shim.o
+	$(CC) $(CFLAGS) -c $(HERE)oracle.c -o $(OUT)/sharp/oracle.o
+	$(FC) -shared -fPIC -Wl,-Bsymbolic -o $@ $(OUT)/sharp/*.o -lm
+
+clean:
+	rm -rf $(OUT)

@@ -1,0 +1,83 @@
+"""Shared set-ups of the reference's example problems (used by oracle and GPU tests).
+
+Each returns (interior q, delta, bc_lower, bc_upper, problem kwargs) following the
+reference example script named in the docstring.
+"""
+import numpy as np
+
+from oracle import oracle as O
+from oracle.driver import cell_centers
+
+
+def grid(lower, upper, n):
+    xs = [cell_centers(l, h, m) for l, h, m in zip(lower, upper, n)]
+    return np.meshgrid(*[c for c, _ in xs], indexing="ij"), [d for _, d in xs]
+
+
+def advection_2d():
+    """examples/advection_2d/advection_2d.py: qinit 19-26, setup 28-75."""
+    (X, Y), d = grid([0, 0], [1, 1], [50, 50])
+    q = (0.9 * (0.1 < X) * (X < 0.6) * (0.1 < Y) * (Y < 0.6) + 0.1)[None]
+    return q, d
+
+
+def euler_quadrants():
+    """examples/euler_2d/quadrants.py:80-110."""
+    (X, Y), d = grid([0, 0], [1, 1], [100, 100])
+    l, r, b, t = X < 0.8, X >= 0.8, Y < 0.8, Y >= 0.8
+    rho = 1.5 * r * t + 0.532258064516129 * l * t + 0.137992831541219 * l * b + 0.532258064516129 * r * b
+    u = 0.0 * r * t + 1.206045378311055 * l * t + 1.206045378311055 * l * b + 0.0 * r * b
+    v = 0.0 * r * t + 0.0 * l * t + 1.206045378311055 * l * b + 1.206045378311055 * r * b
+    p = 1.5 * r * t + 0.3 * l * t + 0.029032258064516 * l * b + 0.3 * r * b
+    q = np.array([rho, rho * u, rho * v, 0.5 * rho * (u ** 2 + v ** 2) + p / (1.4 - 1.0)])
+    return q, d
+
+
+def radial_dam_break():
+    """examples/shallow_2d/radial_dam_break.py: qinit 22-30, setup 33-80."""
+    (X, Y), d = grid([-2.5, -2.5], [2.5, 2.5], [150, 150])
+    r = np.sqrt(X ** 2 + Y ** 2)
+    q = np.array([2. * (r <= 0.5) + 1. * (r > 0.5), 0 * X, 0 * X])
+    return q, d
+
+
+def acoustics_2d():
+    """examples/acoustics_2d_homogeneous/acoustics_2d.py: setup 58-90, qinit 92-98."""
+    (X, Y), d = grid([-1, -1], [1, 1], [100, 100])
+    r = np.sqrt(X ** 2 + Y ** 2)
+    w = 0.2
+    q = np.array([(np.abs(r - 0.5) <= w) * (1. + np.cos(np.pi * (r - 0.5) / w)), 0 * X, 0 * X])
+    return q, d
+
+
+def shocktube_3d():
+    """examples/euler_3d/shocktube.py:30-52 (mx=10, my=10, mz=128)."""
+    (X, Y, Z), d = grid([-1, -1, -1], [1, 1, 1], [10, 10, 128])
+    p = 3. * (Z <= 0) + 1. * (Z > 0)
+    q = np.array([3. * (Z <= 0) + 1. * (Z > 0), 0 * X, 0 * X, 0 * X, p / (1.4 - 1.)])
+    return q, d
+
+
+def seeded_state(rng, kind, shape):
+    """Seeded random admissible states (positive density/depth and pressure)."""
+    meqn = O.MEQN[kind]
+    q = np.zeros((meqn,) + shape)
+    if kind == O.ACOUSTICS:
+        q[:] = rng.uniform(-1, 1, q.shape)
+    elif kind in (O.EULER, O.EULER3):
+        nd = 2 if kind == O.EULER else 3
+        rho = rng.uniform(0.5, 1.5, shape)
+        vel = [rng.uniform(-0.5, 0.5, shape) for _ in range(nd)]
+        p = rng.uniform(0.1, 1.0, shape)
+        q[0] = rho
+        for d in range(nd):
+            q[1 + d] = rho * vel[d]
+        q[nd + 1] = p / 0.4 + 0.5 * rho * sum(v * v for v in vel)
+    elif kind == O.SHALLOW:
+        h = rng.uniform(0.5, 1.5, shape)
+        q[0] = h
+        q[1] = h * rng.uniform(-0.5, 0.5, shape)
+        q[2] = h * rng.uniform(-0.5, 0.5, shape)
+    elif kind == O.ADVECTION:
+        q[:] = rng.uniform(0, 1, q.shape)
+    return np.asfortranarray(q)

@@ -1,0 +1,98 @@
+"""Convert the reference's end-to-end regression fixtures into plain ``.npz`` files.
+
+The reference stores its expected solutions as ``expected_sols.npy``: a 0-d object
+array whose payload is a pickled ``dict`` of flat float64 arrays.  No unpickler is
+run on those files here.  ``pickletools.genops`` only *disassembles* the opcode
+stream (it constructs nothing and imports nothing), and this script reads the
+float64 payload bytes straight out of the ``BINBYTES`` opcode arguments, pairing
+each payload with the dict key (``BINUNICODE``) that precedes it and checking the
+element count against the shape integers that precede the payload.
+
+Plain-text fixtures (``verify_classic.txt``) are read with ``numpy.loadtxt``.
+
+Run in the development container only (it reads ``/root/reference``); the
+outputs are committed under ``tests/golden/``.
+
+Sources (reference test that consumes each fixture):
+  examples/advection_2d/test_2d_advection.py:20-38
+  examples/euler_2d/test_euler2d.py:28-37
+  examples/shallow_2d/test_shallow2d.py:25-36
+  examples/euler_3d/test_euler3d.py:26-57
+  examples/acoustics_2d_homogeneous/test_2d_acoustics.py:19-30
+"""
+import os
+import pickletools
+import sys
+
+import numpy as np
+from numpy.lib import format as npformat
+
+REF = "/root/reference/examples"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+WANT = {
+    "advection_2d": ["classic_dim_split", "classic_unsplit_no_trans",
+                     "classic_unsplit_trans_inc", "classic_unsplit_trans_cor",
+                     "sharpclaw"],
+    "euler_2d": ["quadrants_roe"],
+    "shallow_2d": ["radialdambreak_classic_roe", "radialdambreak_sharpclaw_roe"],
+    "euler_3d": ["shocktube", "shock_bubble"],
+}
+
+_DTYPE_CODES = {"O8", "f8", "|", "<", ">", "=", "i8", "f4", "i4"}
+
+
+def extract_float_arrays(path):
+    """Return {key: float64 array} from an object-array .npy without unpickling."""
+    with open(path, "rb") as f:
+        version = npformat.read_magic(f)
+        if version == (1, 0):
+            shape, fortran, dtype = npformat.read_array_header_1_0(f)
+        else:
+            shape, fortran, dtype = npformat.read_array_header_2_0(f)
+        assert dtype == np.dtype("O") and shape == ()
+        stream = f.read()
+    out = {}
+    key = None
+    ints = []
+    data_codes = set()
+    for op, arg, _pos in pickletools.genops(stream):
+        name = op.name
+        if name in ("BINUNICODE", "SHORT_BINUNICODE", "UNICODE"):
+            if arg in _DTYPE_CODES:
+                if arg not in ("O8", "|", "<", ">", "="):
+                    data_codes.add(arg)
+                continue
+            key, ints = arg, []
+        elif name in ("BININT", "BININT1", "BININT2") and key is not None:
+            ints.append(arg)
+        elif name in ("BINBYTES", "SHORT_BINBYTES", "BINBYTES8") and key is not None:
+            if len(arg) < 8:
+                continue
+            # dtype objects after the first are memo references (BINGET), so the
+            # check is on the set of element codes the whole stream declares.
+            assert data_codes == {"f8"}, f"{path}: element dtypes {data_codes}"
+            n = len(arg) // 8
+            assert n in ints, f"{path}:{key}: shape {ints} does not match {n} values"
+            out[key] = np.frombuffer(arg, dtype="<f8").copy()
+            key = None
+    return out
+
+
+def main():
+    for example, keys in WANT.items():
+        arrays = extract_float_arrays(os.path.join(REF, example, "expected_sols.npy"))
+        missing = [k for k in keys if k not in arrays]
+        if missing:
+            sys.exit(f"{example}: missing keys {missing}; found {sorted(arrays)}")
+        dst = os.path.join(HERE, f"{example}_expected.npz")
+        np.savez_compressed(dst, **{k: arrays[k] for k in keys})
+        print(dst, {k: arrays[k].shape for k in keys})
+    p = np.loadtxt(os.path.join(REF, "acoustics_2d_homogeneous", "verify_classic.txt"))
+    dst = os.path.join(HERE, "acoustics_2d_verify_classic.npz")
+    np.savez_compressed(dst, pressure=p)
+    print(dst, p.shape)
+
+
+if __name__ == "__main__":
+    main()

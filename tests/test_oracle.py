@@ -1,0 +1,337 @@
+"""Pins the oracle (oracle/oracle.c) before anything is compared against it.
+
+1. Per-step golden vectors written by the reference's own compiled Fortran step
+   (tests/golden/ref_step_vectors.npz, from tests/golden/make_ref_vectors.py):
+   the oracle must reproduce them bit for bit.
+2. The live reference build (oracle/_ref), when present: more seeded cases, bit
+   for bit.
+3. The reference's end-to-end regression fixtures (examples/*/test_*.py), which
+   pin the Riemann solvers that the reference takes from clawpack.riemann.
+4. Properties fixed by the mathematics: the Roe property of every normal solver
+   (amdq+apdq = f(qR)-f(qL)), the transverse split (bmasdq+bpasdq = B(q_roe)
+   asdq), WENO5 exactness on linear data, evl*evr = I, conservation under
+   periodic boundaries, and the limiter closed forms.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle.driver import OracleRun
+from tests import _cases as C
+from tests._cases import seeded_state
+
+
+def _run_case(vec, key):
+    kind, ids, trans, lim, ng, mx, my, mz = [int(x) for x in vec[key + "_meta"]]
+    dx, dy, dt, cfl_ref, dz = vec[key + "_real"]
+    bc = [int(b) for b in vec[key + "_bc"]]
+    qin = np.asfortranarray(vec[key + "_qin"])
+    rp = O.make_rp(kind)
+    mw = O.MWAVES[kind]
+    if ids in (0, 1, 2):
+        meth = O.method_array(transverse=trans, dimensional_split=ids > 0)
+        out = qin.copy(order="F")
+        cfl = O.step2(rp, qin, out, ng, dx, dy, dt, meth, [lim] * mw, ids=ids)
+    elif ids in (11, 12, 13):
+        meth = O.method_array(dimensional_split=True)
+        out = qin.copy(order="F")
+        cfl = O.step3ds(rp, qin, out, ng, dx, dy, dz, dt, meth, [lim] * mw, ids - 10)
+    else:
+        out, cfl = O.sharp_flux2(rp, qin, ng, dx, dy, dt, ids - 20)
+    return out, cfl, cfl_ref
+
+
+def test_golden_vectors_bit_exact(golden_dir):
+    vec = np.load(f"{golden_dir}/ref_step_vectors.npz")
+    keys = sorted({k.split("_")[0] for k in vec.files})
+    assert len(keys) == 32
+    for key in keys:
+        out, cfl, cfl_ref = _run_case(vec, key)
+        ref = vec[key + "_qout"]
+        ids = int(vec[key + "_meta"][1])
+        ng = int(vec[key + "_meta"][4])
+        if ids >= 20:   # SharpClaw: the reference returns dq on the interior only
+            sl = (slice(None),) + (slice(ng, -ng),) * 2
+            assert np.array_equal(out[sl], ref[sl]), key
+        elif ids in (11, 12, 13):
+            sl = (slice(None),) + (slice(ng, -ng),) * 3
+            assert np.array_equal(out[sl], ref[sl]), key
+        else:
+            sl = (slice(None),) + (slice(ng, -ng),) * 2
+            assert np.array_equal(out[sl], ref[sl]), key
+        assert cfl == cfl_ref, key
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_live_reference_bit_exact():
+    rng = np.random.default_rng(7)
+    for kind in (O.ACOUSTICS, O.EULER, O.SHALLOW):
+        rp = O.make_rp(kind)
+        mw = O.MWAVES[kind]
+        for trans in (1, 2):
+            ng = 2
+            qbc = O.with_ghosts(seeded_state(rng, kind, (17, 8)), ng)
+            O.bc2(qbc, ng, [O.WALL, O.EXTRAP], [O.EXTRAP, O.WALL])
+            meth = O.method_array(transverse=trans)
+            a, b = qbc.copy(order="F"), qbc.copy(order="F")
+            c1 = O.step2(rp, qbc, a, ng, 0.07, 0.05, 0.011, meth, [4] * mw)
+            c2 = O.ref_step2(rp, qbc, b, ng, 0.07, 0.05, 0.011, meth, [4] * mw)
+            assert np.array_equal(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) and c1 == c2
+    for kind, cd in [(O.SHALLOW, 2), (O.EULER, 2), (O.SHALLOW, 0)]:
+        rp = O.make_rp(kind)
+        ng = 3
+        qbc = O.with_ghosts(seeded_state(rng, kind, (9, 13)), ng)
+        O.bc2(qbc, ng, [O.EXTRAP, O.WALL], [O.PERIODIC, O.PERIODIC])
+        d1, c1 = O.sharp_flux2(rp, qbc, ng, 0.1, 0.08, 0.01, cd)
+        d2, c2 = O.ref_sharp_flux2(rp, qbc, ng, 0.1, 0.08, 0.01, cd)
+        assert np.array_equal(d1[:, ng:-ng, ng:-ng], d2[:, ng:-ng, ng:-ng]) and c1 == c2
+
+
+# ---------------------------------------------------------------- fixtures
+def _l1(frame, expected, d):
+    return np.prod(d) * np.abs(frame.reshape(-1) - expected).sum()
+
+
+@pytest.mark.parametrize("key,split,trans", [
+    ("classic_dim_split", True, 0), ("classic_unsplit_no_trans", False, 0),
+    ("classic_unsplit_trans_inc", False, 1), ("classic_unsplit_trans_cor", False, 2)])
+def test_fixture_advection_classic(golden_dir, key, split, trans):
+    """examples/advection_2d/test_2d_advection.py:20-38 (L1 < 1e-6)."""
+    q, d = C.advection_2d()
+    cd, cm = (0.45, 0.5) if (not split and trans == 0) else (0.9, 1.0)
+    run = OracleRun(O.make_rp(O.ADVECTION, u=0.5, v=1.0), q, d, [2, 2], [2, 2],
+                    dimensional_split=split, transverse=trans, limiters=3,
+                    cfl_desired=cd, cfl_max=cm)
+    fr = run.run(2.0, 10)
+    assert _l1(fr[-1], np.load(f"{golden_dir}/advection_2d_expected.npz")[key], d) < 1e-6
+
+
+def test_fixture_advection_sharpclaw(golden_dir):
+    q, d = C.advection_2d()
+    run = OracleRun(O.make_rp(O.ADVECTION, u=0.5, v=1.0), q, d, [2, 2], [2, 2],
+                    solver="sharp", cfl_desired=0.9, cfl_max=1.0, time_integrator="SSP104")
+    fr = run.run(2.0, 10)
+    assert _l1(fr[-1], np.load(f"{golden_dir}/advection_2d_expected.npz")["sharpclaw"], d) < 1e-6
+
+
+def test_fixture_euler_quadrants_roe(golden_dir):
+    """examples/euler_2d/test_euler2d.py:33-34 (split, minmod, Roe + entropy fix)."""
+    q, d = C.euler_quadrants()
+    run = OracleRun(O.make_rp(O.EULER, efix=1, gamma=1.4), q, d, [1, 1], [1, 1], limiters=1)
+    fr = run.run(0.8, 10)
+    assert _l1(fr[-1][0], np.load(f"{golden_dir}/euler_2d_expected.npz")["quadrants_roe"], d) < 1e-6
+    assert run.dt_history[0] < 0.1   # dt_initial = 0.1 is rejected first (cfl >> 1)
+
+
+def test_fixture_radial_dam_break_classic(golden_dir):
+    q, d = C.radial_dam_break()
+    run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, d, [1, 1], [3, 3], limiters=4)
+    fr = run.run(2.5, 10)
+    exp = np.load(f"{golden_dir}/shallow_2d_expected.npz")["radialdambreak_classic_roe"]
+    assert _l1(fr[-1][0], exp, d) < 1e-5
+
+
+@pytest.mark.slow
+def test_fixture_radial_dam_break_sharpclaw(golden_dir):
+    q, d = C.radial_dam_break()
+    run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, d, [1, 1], [3, 3],
+                    solver="sharp", cfl_desired=2.45, cfl_max=2.5, time_integrator="SSP104")
+    fr = run.run(2.5, 10)
+    exp = np.load(f"{golden_dir}/shallow_2d_expected.npz")["radialdambreak_sharpclaw_roe"]
+    assert _l1(fr[-1][0], exp, d) < 1e-5
+
+
+def test_fixture_acoustics_verify_classic(golden_dir):
+    """examples/acoustics_2d_homogeneous/test_2d_acoustics.py:8-20 (rel L2 < 1e-3)."""
+    q, d = C.acoustics_2d()
+    rp = O.make_rp(O.ACOUSTICS, rho=1.0, bulk=4.0)
+    run = OracleRun(rp, q, d, [1, 1], [1, 1], limiters=4, cfl_desired=0.45, cfl_max=0.5,
+                    dt_initial=min(d) / rp.cc * 0.45)
+    fr = run.run(0.12, 10)
+    exp = np.load(f"{golden_dir}/acoustics_2d_verify_classic.npz")["pressure"]
+    dd = np.prod(d)
+    assert dd * np.linalg.norm(exp - fr[-1][0]) / (dd * np.linalg.norm(exp)) < 1e-3
+
+
+def test_fixture_shocktube_3d(golden_dir):
+    """examples/euler_3d/test_euler3d.py:27-28 (split MC, 10x10x128, t=0.2)."""
+    q, d = C.shocktube_3d()
+    run = OracleRun(O.make_rp(O.EULER3, efix=1, gamma=1.4), q, d, [1, 1, 1], [1, 1, 1],
+                    limiters=4, cfl_desired=0.8, cfl_max=1.0)
+    fr = run.run(0.2, 10)
+    assert _l1(fr[-1][0], np.load(f"{golden_dir}/euler_3d_expected.npz")["shocktube"], d) < 1e-6
+
+
+# ---------------------------------------------------------------- mathematics
+def _euler_flux(q, d, gamma=1.4):
+    nd = len(q) - 2
+    rho, E = q[0], q[-1]
+    vel = [q[1 + k] / rho for k in range(nd)]
+    p = (gamma - 1) * (E - 0.5 * rho * sum(v * v for v in vel))
+    f = [rho * vel[d]] + [q[1 + k] * vel[d] + (p if k == d else 0.0) for k in range(nd)]
+    return np.array(f + [vel[d] * (E + p)])
+
+
+def _swe_flux(q, d, g=1.0):
+    h, hu, hv = q
+    vel = [hu / h, hv / h]
+    f = [q[1 + d], hu * vel[d] + (0.5 * g * h * h if d == 0 else 0.0),
+         hv * vel[d] + (0.5 * g * h * h if d == 1 else 0.0)]
+    return np.array(f)
+
+
+def test_roe_property_all_solvers():
+    """amdq + apdq equals the physical flux difference (conservative Roe split)."""
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        for ixy in (1, 2):
+            qL = seeded_state(rng, O.EULER, (1,))[:, 0]
+            qR = seeded_state(rng, O.EULER, (1,))[:, 0]
+            for efix in (0, 1):
+                _, s, am, ap = O.rpn(O.make_rp(O.EULER, efix=efix), ixy, qL, qR)
+                np.testing.assert_allclose(am + ap, _euler_flux(qR, ixy - 1) - _euler_flux(qL, ixy - 1),
+                                           rtol=1e-12, atol=1e-12)
+            qL = seeded_state(rng, O.SHALLOW, (1,))[:, 0]
+            qR = seeded_state(rng, O.SHALLOW, (1,))[:, 0]
+            _, s, am, ap = O.rpn(O.make_rp(O.SHALLOW), ixy, qL, qR)
+            np.testing.assert_allclose(am + ap, _swe_flux(qR, ixy - 1) - _swe_flux(qL, ixy - 1),
+                                       rtol=1e-12, atol=1e-12)
+            qL = seeded_state(rng, O.ACOUSTICS, (1,))[:, 0]
+            qR = seeded_state(rng, O.ACOUSTICS, (1,))[:, 0]
+            _, s, am, ap = O.rpn(O.make_rp(O.ACOUSTICS, rho=1.0, bulk=4.0), ixy, qL, qR)
+            A = np.zeros((3, 3))
+            A[0, ixy] = 4.0
+            A[ixy, 0] = 1.0
+            np.testing.assert_allclose(am + ap, A @ (qR - qL), rtol=1e-13, atol=1e-14)
+        for ixyz in (1, 2, 3):
+            qL = seeded_state(rng, O.EULER3, (1,))[:, 0]
+            qR = seeded_state(rng, O.EULER3, (1,))[:, 0]
+            _, s, am, ap = O.rpn(O.make_rp(O.EULER3), ixyz, qL, qR)
+            np.testing.assert_allclose(am + ap, _euler_flux(qR, ixyz - 1) - _euler_flux(qL, ixyz - 1),
+                                       rtol=1e-12, atol=1e-12)
+
+
+def _euler_jacobian(u, v, H, d, gamma=1.4):
+    """Flux Jacobian of 2D Euler in direction d as a function of (u, v, H)."""
+    g1 = gamma - 1
+    q2 = u * u + v * v
+    Jx = np.array([[0, 1, 0, 0],
+                   [g1 * q2 / 2 - u * u, (3 - gamma) * u, -g1 * v, g1],
+                   [-u * v, v, u, 0],
+                   [u * (g1 * q2 / 2 - H), H - g1 * u * u, -g1 * u * v, gamma * u]])
+    if d == 0:
+        return Jx
+    P = np.array([0, 2, 1, 3])   # swap x/y roles
+    Jy = _euler_jacobian(v, u, H, 0, gamma)
+    return Jy[np.ix_(P, P)]
+
+
+def test_transverse_split_is_roe_matrix():
+    """bmasdq + bpasdq = B(q_roe) asdq for the transverse direction."""
+    import ctypes
+    rng = np.random.default_rng(11)
+    L = O.lib()
+    for _ in range(30):
+        for ixy in (1, 2):
+            qL = seeded_state(rng, O.EULER, (1,))[:, 0].copy()
+            qR = seeded_state(rng, O.EULER, (1,))[:, 0].copy()
+            asdq = rng.normal(size=4)
+            bm, bp = np.zeros(4), np.zeros(4)
+            L.orc_rpt(ctypes.byref(O.make_rp(O.EULER)), ixy, 1, O._dp(qL), O._dp(qR),
+                      O._dp(asdq), O._dp(bm), O._dp(bp))
+            sl, sr = np.sqrt(qL[0]), np.sqrt(qR[0])
+            u = (qL[1] / sl + qR[1] / sr) / (sl + sr)
+            v = (qL[2] / sl + qR[2] / sr) / (sl + sr)
+            pl = 0.4 * (qL[3] - 0.5 * (qL[1] ** 2 + qL[2] ** 2) / qL[0])
+            pr = 0.4 * (qR[3] - 0.5 * (qR[1] ** 2 + qR[2] ** 2) / qR[0])
+            H = ((qL[3] + pl) / sl + (qR[3] + pr) / sr) / (sl + sr)
+            B = _euler_jacobian(u, v, H, 2 - ixy)   # transverse direction
+            np.testing.assert_allclose(bm + bp, B @ asdq, rtol=1e-11, atol=1e-12)
+            assert np.all(np.isfinite(bm))
+            # acoustics: B = [[0,0,K],[0,0,0],[1/rho,0,0]] in the transverse direction
+            a = rng.normal(size=3)
+            bm, bp = np.zeros(3), np.zeros(3)
+            L.orc_rpt(ctypes.byref(O.make_rp(O.ACOUSTICS, rho=1.0, bulk=4.0)), ixy, 1,
+                      O._dp(qL[:3].copy()), O._dp(qR[:3].copy()), O._dp(a), O._dp(bm), O._dp(bp))
+            t = 3 - ixy
+            B = np.zeros((3, 3))
+            B[0, t] = 4.0
+            B[t, 0] = 1.0
+            np.testing.assert_allclose(bm + bp, B @ a, rtol=1e-13, atol=1e-14)
+
+
+def test_weno5_exact_on_linear_data_and_constants():
+    n2, ng = 20, 3
+    x = np.arange(1, n2 + 1, dtype=float)
+    for meqn in (1, 3):
+        q = np.asfortranarray(np.vstack([2.0 + 0.5 * x * (m + 1) for m in range(meqn)]))
+        ql, qr = np.zeros_like(q, order="F"), np.zeros_like(q, order="F")
+        O.lib().orc_weno5(meqn, ng, n2, O._dp(q), O._dp(ql), O._dp(qr))
+        for i in range(ng, n2 - ng + 2):      # 1-based cells ng..n2-ng+1
+            np.testing.assert_allclose(ql[:, i - 1], q[:, i - 1] - 0.5 * (np.arange(meqn) + 1) * 0.5, rtol=1e-14)
+            np.testing.assert_allclose(qr[:, i - 1], q[:, i - 1] + 0.5 * (np.arange(meqn) + 1) * 0.5, rtol=1e-14)
+
+
+def test_evec_inverse_and_char_weno_constant_state():
+    rng = np.random.default_rng(5)
+    for kind in (O.SHALLOW, O.EULER):
+        rp = O.make_rp(kind)
+        q = seeded_state(rng, kind, (16,))
+        for ixy in (1, 2):
+            evl, evr = O.evec(rp, ixy, q)
+            for i in range(1, q.shape[1]):
+                np.testing.assert_allclose(evl[:, :, i] @ evr[:, :, i], np.eye(q.shape[0]), atol=1e-12)
+        const = np.asfortranarray(np.repeat(q[:, :1], 16, axis=1))
+        dq, cfl = O.sharp_flux1(rp, const, 3, 1, 0.1, 2)
+        assert np.abs(dq).max() < 1e-14 and cfl > 0
+
+
+def test_conservation_periodic():
+    """Flux-difference form (step2.f90:139-145): totals constant under periodic BCs."""
+    rng = np.random.default_rng(9)
+    for kind in (O.EULER, O.SHALLOW, O.ACOUSTICS):
+        rp = O.make_rp(kind)
+        mw = O.MWAVES[kind]
+        q = seeded_state(rng, kind, (24, 20))
+        for split, trans in ((False, 2), (False, 1), (True, 0)):
+            run = OracleRun(rp, q, [0.05, 0.05], [2, 2], [2, 2], dimensional_split=split,
+                            transverse=trans, limiters=4, dt_initial=0.004)
+            for _ in range(3):
+                run.evolve_to_time(None)
+            np.testing.assert_allclose(run.q.sum(axis=(1, 2)), q.sum(axis=(1, 2)), rtol=1e-13, atol=1e-12)
+        run = OracleRun(rp, q, [0.05, 0.05], [2, 2], [2, 2], solver="sharp", cfl_desired=0.5,
+                        cfl_max=0.55, time_integrator="SSP33", char_decomp=2 if kind != O.ACOUSTICS else 0,
+                        dt_initial=0.004)
+        for _ in range(2):
+            run.evolve_to_time(None)
+        np.testing.assert_allclose(run.q.sum(axis=(1, 2)), q.sum(axis=(1, 2)), rtol=1e-13, atol=1e-12)
+
+
+def test_limiter_closed_forms_through_the_step():
+    """philim.f90:16-33 on a 1-wave scalar slice: with u > 0 the corrected flux is
+    0.5*|u|*(1-|u|dt/dx)*phi(theta)*W with theta the upwind jump ratio."""
+    dx, dt, u = 0.1, 0.05, 1.0
+    nu = u * dt / dx
+    for lim, phi in [(1, lambda r: max(0, min(1, r))),
+                     (2, lambda r: max(0, min(1, 2 * r), min(2, r))),
+                     (3, lambda r: (r + abs(r)) / (1 + abs(r))),
+                     (4, lambda r: max(0, min((1 + r) / 2, 2, 2 * r))),
+                     (5, lambda r: r)]:
+        vals = np.array([0.0, 0.0, 1.0, 1.5, 3.5, 3.0, 2.0, 2.0, 2.0, 2.0, 2.0])
+        mx, ng = 7, 2
+        qbc = np.asfortranarray(np.tile(vals, (1, 1)).reshape(1, 11, 1).repeat(5, axis=2))
+        qold = qbc.copy(order="F")
+        qnew = qbc.copy(order="F")
+        meth = O.method_array(dimensional_split=True)
+        O.step2(O.make_rp(O.ADVECTION, u=u, v=0.0), qold, qnew, ng, dx, dx, dt, meth, [lim], ids=1)
+        # reference value for cell i (1-based) in the middle row
+        W = np.diff(vals)          # W[k] = jump at interface between cells k and k+1 (0-based)
+        def F(k):                  # correction flux at interface k (0-based cells k-1|k)
+            w = W[k - 1]
+            if w == 0:
+                return 0.0
+            theta = W[k - 2] / w
+            return 0.5 * u * (1 - nu) * phi(theta) * w
+        for c in range(3, 8):       # 0-based cells with full stencils
+            expect = vals[c] - nu * W[c - 1] - nu * (F(c + 1) - F(c))
+            assert abs(qnew[0, c, 2] - expect) < 1e-14, (lim, c)
