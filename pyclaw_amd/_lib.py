@@ -1,0 +1,105 @@
+"""ctypes binding of libclawhip.so (declared in include/clawhip.h).
+
+The library is the product's compute path: there is no CPU fallback.  Loading
+fails loudly when the shared object has not been built.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclawhip.so")
+
+F64, F32K = 0, 1
+RP_ACOUSTICS, RP_EULER, RP_SHALLOW, RP_ADVECTION, RP_EULER3 = 1, 2, 3, 4, 5
+BC_EXTRAP, BC_PERIODIC, BC_WALL, BC_GIVEN = 1, 2, 3, 4
+
+_ERRORS = {-1: "invalid argument (size, stride or pointer)",
+           -2: "unknown Riemann solver or unsupported combination",
+           -3: "unsupported method flags"}
+
+
+class ClawhipError(RuntimeError):
+    pass
+
+
+class RP(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("efix", ctypes.c_int),
+                ("rho", ctypes.c_double), ("bulk", ctypes.c_double),
+                ("cc", ctypes.c_double), ("zz", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("grav", ctypes.c_double),
+                ("u", ctypes.c_double), ("v", ctypes.c_double)]
+
+
+class Geom(ctypes.Structure):
+    _fields_ = [("mx", ctypes.c_int), ("my", ctypes.c_int), ("mz", ctypes.c_int),
+                ("ghosts", ctypes.c_int), ("field_stride", ctypes.c_longlong),
+                ("row_stride", ctypes.c_longlong), ("slab_stride", ctypes.c_longlong)]
+
+
+def geom2(mx, my, ghosts=0):
+    """Planar 2D array (field, j, i) with `ghosts` layers stored on every side."""
+    nx, ny = mx + 2 * ghosts, my + 2 * ghosts
+    return Geom(mx, my, 1, ghosts, nx * ny, nx, nx * ny)
+
+
+def geom3(mx, my, mz, ghosts=0):
+    """Planar 3D array (field, k, j, i) with `ghosts` layers stored on every side."""
+    nx, ny, nz = mx + 2 * ghosts, my + 2 * ghosts, mz + 2 * ghosts
+    return Geom(mx, my, mz, ghosts, nx * ny * nz, nx, nx * ny)
+
+
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_IP = ctypes.POINTER(ctypes.c_int)
+
+_SIGS = {
+    "clawhip_device_count": [_IP],
+    "clawhip_cfl_reset": [_P, _P],
+    "clawhip_step2_unsplit": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
+                              ctypes.POINTER(Geom), _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
+                              _D, _D, _D, _P, _P],
+    "clawhip_step2_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
+                            ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
+                            _D, _D, _D, _P, _P],
+    "clawhip_step3_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
+                            ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
+                            _D, _D, _D, _D, _P, _P],
+}
+
+_lib = None
+
+
+def lib():
+    """Load libclawhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ClawhipError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                f"g.build()'` (hipcc, gfx950).  pyclaw_amd has no CPU fallback.")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argtypes in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = argtypes
+            f.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = _ERRORS.get(rc, f"HIP error {rc}")
+        raise ClawhipError(f"{what}: {msg}")
+
+
+def ints(seq):
+    return (ctypes.c_int * len(seq))(*[int(x) for x in seq])
+
+
+def declared_symbols():
+    """Names declared in include/clawhip.h (parsed), for the export check."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "clawhip.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"\bint\s+(clawhip_\w+)\s*\(", text)))

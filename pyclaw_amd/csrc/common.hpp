@@ -1,0 +1,185 @@
+// common.hpp -- grid descriptors, boundary-mapped loads, wavefront helpers and the
+// wave limiter shared by the classic and SharpClaw kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "riemann.hpp"
+
+namespace claw {
+
+// Boundary-condition codes: pyclaw.BC (solver.py:13-19) plus GIVEN, which means
+// the ghost cells are stored in the array (a user-filled qbc) and read as they are.
+enum BCCode { BC_CUSTOM = 0, BC_EXTRAP = 1, BC_PERIODIC = 2, BC_WALL = 3, BC_GIVEN = 4 };
+
+// Storage of one state array on the device.  Cell (i, j, k) (1-based interior
+// indices, ghosts at <= 0 and > m) of field f lives at
+//   base + f*fs + (i-1+g)*1 + (j-1+g)*rs + (k-1+g)*ss
+// where g is the number of ghost layers present in storage (0: compact).
+struct Geom {
+  int mx, my, mz;
+  int g;
+  long long fs, rs, ss;
+};
+
+struct BCSpec {
+  int lo[3], hi[3];      // per dimension, BCCode
+  int refl[3];           // component negated by a wall in that dimension
+};
+
+// Map one index into the stored range following Solver._bc_lower/_bc_upper
+// (solver.py:439-519).  Indices are first clamped to the reference's ghost range
+// [1-ng, m+ng]; returns the storage index and whether a wall flipped it.
+__device__ __forceinline__ int bc_map(int i, int m, int ng, int lo, int hi, bool& flip) {
+  i = i < 1 - ng ? 1 - ng : (i > m + ng ? m + ng : i);
+  if (i < 1) {
+    if (lo == BC_EXTRAP) i = 1;
+    else if (lo == BC_PERIODIC) i += m;
+    else if (lo == BC_WALL) { i = 1 - i; flip = !flip; }
+  } else if (i > m) {
+    if (hi == BC_EXTRAP) i = m;
+    else if (hi == BC_PERIODIC) i -= m;
+    else if (hi == BC_WALL) { i = 2 * m + 1 - i; flip = !flip; }
+  }
+  return i;
+}
+
+// Load cell (c, r[, k]) with the ghost-cell value the reference's BC fill would
+// have placed there (x, then y, then z, so corners compose the per-dimension maps).
+template <int ME, typename T>
+__device__ __forceinline__ void load_cell2(const T* __restrict__ base, const Geom& g,
+                                           const BCSpec& bc, int ng, int c, int r, T (&out)[ME]) {
+  bool fx = false, fy = false;
+  c = bc_map(c, g.mx, ng, bc.lo[0], bc.hi[0], fx);
+  r = bc_map(r, g.my, ng, bc.lo[1], bc.hi[1], fy);
+  const T* p = base + (long long)(c - 1 + g.g) + (long long)(r - 1 + g.g) * g.rs;
+#pragma unroll
+  for (int m = 0; m < ME; m++) out[m] = p[(long long)m * g.fs];
+#pragma unroll
+  for (int m = 0; m < ME; m++) {
+    if (fx && m == bc.refl[0]) out[m] = -out[m];
+    if (fy && m == bc.refl[1]) out[m] = -out[m];
+  }
+}
+
+template <int ME, typename T>
+__device__ __forceinline__ void load_cell3(const T* __restrict__ base, const Geom& g,
+                                           const BCSpec& bc, int ng, int c, int r, int k,
+                                           T (&out)[ME]) {
+  bool fx = false, fy = false, fz = false;
+  c = bc_map(c, g.mx, ng, bc.lo[0], bc.hi[0], fx);
+  r = bc_map(r, g.my, ng, bc.lo[1], bc.hi[1], fy);
+  k = bc_map(k, g.mz, ng, bc.lo[2], bc.hi[2], fz);
+  const T* p = base + (long long)(c - 1 + g.g) + (long long)(r - 1 + g.g) * g.rs +
+               (long long)(k - 1 + g.g) * g.ss;
+#pragma unroll
+  for (int m = 0; m < ME; m++) out[m] = p[(long long)m * g.fs];
+#pragma unroll
+  for (int m = 0; m < ME; m++) {
+    if (fx && m == bc.refl[0]) out[m] = -out[m];
+    if (fy && m == bc.refl[1]) out[m] = -out[m];
+    if (fz && m == bc.refl[2]) out[m] = -out[m];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T* cell_ptr2(T* base, const Geom& g, int c, int r) {
+  return base + (long long)(c - 1 + g.g) + (long long)(r - 1 + g.g) * g.rs;
+}
+template <typename T>
+__device__ __forceinline__ T* cell_ptr3(T* base, const Geom& g, int c, int r, int k) {
+  return base + (long long)(c - 1 + g.g) + (long long)(r - 1 + g.g) * g.rs +
+         (long long)(k - 1 + g.g) * g.ss;
+}
+
+// ---- cross-lane moves (ds_bpermute): value of lane-1 / lane+1 --------------
+template <typename T>
+__device__ __forceinline__ T from_left(T v) { return __shfl_up(v, 1, 64); }
+template <typename T>
+__device__ __forceinline__ T from_right(T v) { return __shfl_down(v, 1, 64); }
+
+template <int N, typename T>
+__device__ __forceinline__ void from_left(const T (&v)[N], T (&o)[N]) {
+#pragma unroll
+  for (int m = 0; m < N; m++) o[m] = from_left(v[m]);
+}
+template <int N, typename T>
+__device__ __forceinline__ void from_right(const T (&v)[N], T (&o)[N]) {
+#pragma unroll
+  for (int m = 0; m < N; m++) o[m] = from_right(v[m]);
+}
+
+// Wave-wide max of a non-negative double, then one atomic per wave.  Ordering of
+// non-negative IEEE doubles equals the ordering of their bit patterns.
+__device__ __forceinline__ void wave_atomic_max(double v, unsigned long long* dst) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(v));
+}
+
+// philim.f90:2-36
+template <typename T>
+__device__ __forceinline__ T philim(T a, T b, int meth) {
+  T r = b / a;
+  switch (meth) {
+    case 1: return dmax(T(0), dmin(T(1), r));
+    case 2: return dmax(dmax(T(0), dmin(T(1), T(2) * r)), dmin(T(2), r));
+    case 3: return (r + fabs(r)) / (T(1) + fabs(r));
+    case 4: {
+      T c = (T(1) + r) / T(2);
+      return dmax(T(0), dmin(dmin(c, T(2)), T(2) * r));
+    }
+    case 5: return r;
+  }
+  return T(0);
+}
+
+// One interface of limiter.f90:30-54: wnorm2 and the two dot products are of the
+// unlimited waves; the upwind one (by the sign of s) enters phi.
+template <int ME, int MW, typename T>
+__device__ __forceinline__ void limit_waves(T (&W)[MW][ME], const T (&s)[MW], const T (&dotl)[MW],
+                                            const T (&dotr)[MW], const int (&mthlim)[4]) {
+#pragma unroll
+  for (int w = 0; w < MW; w++) {
+    if (mthlim[w] == 0) continue;
+    T wnorm2 = T(0);
+#pragma unroll
+    for (int m = 0; m < ME; m++) wnorm2 = wnorm2 + W[w][m] * W[w][m];
+    if (wnorm2 == T(0)) continue;
+    T phi = s[w] > T(0) ? philim(wnorm2, dotl[w], mthlim[w]) : philim(wnorm2, dotr[w], mthlim[w]);
+#pragma unroll
+    for (int m = 0; m < ME; m++) W[w][m] = phi * W[w][m];
+  }
+}
+
+template <int ME, int MW, typename T>
+__device__ __forceinline__ void wave_dots(const T (&A)[MW][ME], const T (&B)[MW][ME], T (&d)[MW]) {
+#pragma unroll
+  for (int w = 0; w < MW; w++) {
+    d[w] = T(0);
+#pragma unroll
+    for (int m = 0; m < ME; m++) d[w] = d[w] + A[w][m] * B[w][m];
+  }
+}
+
+// flux2.f90:134-146: cqxx from limited waves, fadd = 0.5*cqxx.
+template <int ME, int MW, typename T>
+__device__ __forceinline__ void second_order(const T (&W)[MW][ME], const T (&s)[MW], T dtdx,
+                                             T (&cq)[ME], T (&F)[ME]) {
+  T dtdxave = T(0.5) * (dtdx + dtdx);
+#pragma unroll
+  for (int m = 0; m < ME; m++) {
+    cq[m] = T(0);
+#pragma unroll
+    for (int w = 0; w < MW; w++) {
+      T sa = fabs(s[w]);
+      cq[m] = cq[m] + sa * (T(1) - sa * dtdxave) * W[w][m];
+    }
+    F[m] = T(0) + T(0.5) * cq[m];
+  }
+}
+
+}  // namespace claw

@@ -1,0 +1,48 @@
+// launch.hpp -- host-side launchers behind the C ABI (capi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/clawhip.h"
+#include "common.hpp"
+
+namespace claw {
+
+template <typename T>
+inline RPParams<T> to_params(const clawhip_rp& r) {
+  RPParams<T> p;
+  p.rho = T(r.rho); p.bulk = T(r.bulk); p.cc = T(r.cc); p.zz = T(r.zz);
+  p.gamma = T(r.gamma); p.gamma1 = T(r.gamma - 1.0);
+  p.grav = T(r.grav); p.u = T(r.u); p.v = T(r.v); p.efix = r.efix;
+  return p;
+}
+
+inline BCSpec to_bc(const int* lo, const int* hi, const int* refl, int ndim) {
+  BCSpec b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = d < ndim ? lo[d] : BC_EXTRAP;
+    b.hi[d] = d < ndim ? hi[d] : BC_EXTRAP;
+    b.refl[d] = d < ndim ? refl[d] : -1;
+  }
+  return b;
+}
+
+int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
+                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
+                   const int* bc_lower,
+                   const int* bc_upper, const int* reflect, int ng, const int* method,
+                   const int* mthlim, double dx, double dy, double dt,
+                   unsigned long long* cfl_bits, hipStream_t stream);
+
+int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
+                  void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
+                  void* work, void* cwork, const int* lo, const int* hi, const int* refl,
+                  int ng, const int* method, const int* mthlim, double dx, double dy,
+                  double dt, unsigned long long* cfl, hipStream_t s);
+
+int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
+                  void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
+                  void* work, void* cwork, const int* lo, const int* hi, const int* refl,
+                  int ng, const int* method, const int* mthlim, double dx, double dy,
+                  double dz, double dt, unsigned long long* cfl, hipStream_t s);
+
+}  // namespace claw

@@ -1,0 +1,460 @@
+// riemann.hpp -- device Riemann solvers for the classic and SharpClaw kernels.
+//
+// Each solver is a struct with compile-time meqn/mwaves and two pointwise
+// device functions:
+//   rpn<IXY>(p, qL, qR, wave, s, amdq, apdq, roe)   normal solve at one interface
+//   rpt<IXY>(p, roe, asdq, bmasdq, bpasdq)           transverse split of asdq
+// qL is the state left of the interface (the reference ABI's qr(:,i-1)), qR the
+// state right of it (ql(:,i)).  IXY selects the normal momentum component.  The
+// Roe averages rpn computes are handed to rpt in `roe`; the reference's rpt2
+// recomputes them from the same two states with the same operations, so the
+// values are identical.
+//
+// Algorithms (clawpack.riemann, restated; the in-tree form is
+// development/rp_approaches/rpn2_euler_5wave.f and rpt2_euler_5wave.f):
+//   acoustics_2D              2 waves (-c, +c)
+//   euler_4wave_2D            Roe, 4 waves (u-a, shear, entropy, u+a), Harten-Hyman fix
+//   shallow_roe_with_efix_2D  Roe, 3 waves (u-c, u, u+c), Harten-Hyman fix
+//   advection_2D              1 wave
+//   euler_3D                  Roe, 3 waves (u-a, lumped entropy+shear, u+a), fix
+// Every expression keeps the reference's evaluation order; the library is
+// compiled with -ffp-contract=off so FP64 results match the CPU oracle bitwise.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace claw {
+
+template <typename T>
+struct RPParams {
+  T rho, bulk, cc, zz;   // acoustics
+  T gamma, gamma1;       // Euler
+  T grav;                // shallow water
+  T u, v;                // advection
+  int efix;
+};
+
+template <typename T>
+__device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
+
+// ---------------------------------------------------------------- acoustics
+template <typename T>
+struct Acoustics {
+  static constexpr int meqn = 3, mwaves = 2;
+  struct Roe {};
+  template <int IXY>
+  __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T zz = p.zz, cc = p.cc;
+    T delta1 = qR[0] - qL[0];
+    T delta2 = qR[mu] - qL[mu];
+    T a1 = (-delta1 + zz * delta2) / (T(2) * zz);
+    T a2 = (delta1 + zz * delta2) / (T(2) * zz);
+    wave[0][0] = -a1 * zz; wave[0][mu] = a1; wave[0][mv] = T(0); s[0] = -cc;
+    wave[1][0] = a2 * zz; wave[1][mu] = a2; wave[1][mv] = T(0); s[1] = cc;
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      amdq[m] = s[0] * wave[0][m];
+      apdq[m] = s[1] * wave[1][m];
+    }
+  }
+  template <int IXY>
+  __device__ static __forceinline__ void rpt(const RPParams<T>& p, const Roe&, const T (&asdq)[meqn],
+                                             T (&bm)[meqn], T (&bp)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T zz = p.zz, cc = p.cc;
+    T a1 = (-asdq[0] + zz * asdq[mv]) / (T(2) * zz);
+    T a2 = (asdq[0] + zz * asdq[mv]) / (T(2) * zz);
+    bm[0] = cc * a1 * zz; bm[mu] = T(0); bm[mv] = -cc * a1;
+    bp[0] = cc * a2 * zz; bp[mu] = T(0); bp[mv] = cc * a2;
+  }
+};
+
+// ---------------------------------------------------------------- 2D Euler
+template <typename T>
+struct Euler {
+  static constexpr int meqn = 4, mwaves = 4;
+  struct Roe { T u, v, enth, u2v2, a, g1a2, euv; };
+
+  template <int IXY>
+  __device__ static __forceinline__ Roe roe_avg(T gamma1, const T* qL, const T* qR) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    Roe r;
+    T rhsqrtl = sqrt(qL[0]);
+    T rhsqrtr = sqrt(qR[0]);
+    T pl = gamma1 * (qL[3] - T(0.5) * (qL[1] * qL[1] + qL[2] * qL[2]) / qL[0]);
+    T pr = gamma1 * (qR[3] - T(0.5) * (qR[1] * qR[1] + qR[2] * qR[2]) / qR[0]);
+    T rhsq2 = rhsqrtl + rhsqrtr;
+    r.u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
+    r.v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
+    r.enth = (((qL[3] + pl) / rhsqrtl + (qR[3] + pr) / rhsqrtr)) / rhsq2;
+    r.u2v2 = r.u * r.u + r.v * r.v;
+    T a2 = gamma1 * (r.enth - T(.5) * r.u2v2);
+    r.a = sqrt(a2);
+    r.g1a2 = gamma1 / a2;
+    r.euv = r.enth - r.u2v2;
+    return r;
+  }
+
+  template <int IXY>
+  __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe& r) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T gamma = p.gamma, gamma1 = p.gamma1;
+    r = roe_avg<IXY>(gamma1, qL, qR);
+    T d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv], d3 = qR[3] - qL[3];
+    T a3 = r.g1a2 * (r.euv * d0 + r.u * d1 + r.v * d2 - d3);
+    T a2 = d2 - r.v * d0;
+    T a4 = (d1 + (r.a - r.u) * d0 - r.a * a3) / (T(2) * r.a);
+    T a1 = d0 - a3 - a4;
+    wave[0][0] = a1; wave[0][mu] = a1 * (r.u - r.a); wave[0][mv] = a1 * r.v;
+    wave[0][3] = a1 * (r.enth - r.u * r.a); s[0] = r.u - r.a;
+    wave[1][0] = T(0); wave[1][mu] = T(0); wave[1][mv] = a2; wave[1][3] = a2 * r.v; s[1] = r.u;
+    wave[2][0] = a3; wave[2][mu] = a3 * r.u; wave[2][mv] = a3 * r.v;
+    wave[2][3] = a3 * T(0.5) * r.u2v2; s[2] = r.u;
+    wave[3][0] = a4; wave[3][mu] = a4 * (r.u + r.a); wave[3][mv] = a4 * r.v;
+    wave[3][3] = a4 * (r.enth + r.u * r.a); s[3] = r.u + r.a;
+
+    if (!p.efix) {
+#pragma unroll
+      for (int m = 0; m < meqn; m++) {
+        amdq[m] = T(0); apdq[m] = T(0);
+#pragma unroll
+        for (int w = 0; w < mwaves; w++) {
+          if (s[w] < T(0)) amdq[m] = amdq[m] + s[w] * wave[w][m];
+          else apdq[m] = apdq[m] + s[w] * wave[w][m];
+        }
+      }
+      return;
+    }
+    do {
+      T rhoim1 = qL[0];
+      T pim1 = gamma1 * (qL[3] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]) / rhoim1);
+      T cim1 = sqrt(gamma * pim1 / rhoim1);
+      T s0 = qL[mu] / rhoim1 - cim1;
+      if (s0 >= T(0) && s[0] > T(0)) {
+#pragma unroll
+        for (int m = 0; m < meqn; m++) amdq[m] = T(0);
+        break;
+      }
+      T rho1 = qL[0] + wave[0][0];
+      T rhou1 = qL[mu] + wave[0][mu];
+      T rhov1 = qL[mv] + wave[0][mv];
+      T en1 = qL[3] + wave[0][3];
+      T p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+      T c1 = sqrt(gamma * p1 / rho1);
+      T s1 = rhou1 / rho1 - c1;
+      T sfract;
+      if (s0 < T(0) && s1 > T(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+      else if (s[0] < T(0)) sfract = s[0];
+      else sfract = T(0);
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = sfract * wave[0][m];
+      if (s[1] >= T(0)) break;
+#pragma unroll
+      for (int m = 0; m < meqn; m++) {
+        amdq[m] = amdq[m] + s[1] * wave[1][m];
+        amdq[m] = amdq[m] + s[2] * wave[2][m];
+      }
+      T rhoi = qR[0];
+      T pi = gamma1 * (qR[3] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
+      T ci = sqrt(gamma * pi / rhoi);
+      T s3 = qR[mu] / rhoi + ci;
+      T rho2 = qR[0] - wave[3][0];
+      T rhou2 = qR[mu] - wave[3][mu];
+      T rhov2 = qR[mv] - wave[3][mv];
+      T en2 = qR[3] - wave[3][3];
+      T p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+      T c2 = sqrt(gamma * p2 / rho2);
+      T s2 = rhou2 / rho2 + c2;
+      if (s2 < T(0) && s3 > T(0)) sfract = s2 * (s3 - s[3]) / (s3 - s2);
+      else if (s[3] < T(0)) sfract = s[3];
+      else break;
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = amdq[m] + sfract * wave[3][m];
+    } while (0);
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      T df = T(0);
+#pragma unroll
+      for (int w = 0; w < mwaves; w++) df = df + s[w] * wave[w][m];
+      apdq[m] = df - amdq[m];
+    }
+  }
+
+  template <int IXY>
+  __device__ static __forceinline__ void rpt(const RPParams<T>&, const Roe& r, const T (&asdq)[meqn],
+                                             T (&bm)[meqn], T (&bp)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    T a3 = r.g1a2 * (r.euv * asdq[0] + r.u * asdq[mu] + r.v * asdq[mv] - asdq[3]);
+    T a2 = asdq[mu] - r.u * asdq[0];
+    T a4 = (asdq[mv] + (r.a - r.v) * asdq[0] - r.a * a3) / (T(2) * r.a);
+    T a1 = asdq[0] - a3 - a4;
+    T wb[3][meqn], sb[3];
+    wb[0][0] = a1; wb[0][mu] = a1 * r.u; wb[0][mv] = a1 * (r.v - r.a);
+    wb[0][3] = a1 * (r.enth - r.v * r.a); sb[0] = r.v - r.a;
+    wb[1][0] = a3; wb[1][mu] = a3 * r.u + a2; wb[1][mv] = a3 * r.v;
+    wb[1][3] = a3 * T(0.5) * r.u2v2 + a2 * r.u; sb[1] = r.v;
+    wb[2][0] = a4; wb[2][mu] = a4 * r.u; wb[2][mv] = a4 * (r.v + r.a);
+    wb[2][3] = a4 * (r.enth + r.v * r.a); sb[2] = r.v + r.a;
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      bm[m] = T(0); bp[m] = T(0);
+#pragma unroll
+      for (int w = 0; w < 3; w++) {
+        bm[m] = bm[m] + dmin(sb[w], T(0)) * wb[w][m];
+        bp[m] = bp[m] + dmax(sb[w], T(0)) * wb[w][m];
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------- shallow water
+template <typename T>
+struct Shallow {
+  static constexpr int meqn = 3, mwaves = 3;
+  struct Roe { T u, v, a; };
+
+  template <int IXY>
+  __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe& r) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T grav = p.grav;
+    T h = (qL[0] + qR[0]) * T(0.50);
+    T hsqrtl = sqrt(qL[0]);
+    T hsqrtr = sqrt(qR[0]);
+    T hsq2 = hsqrtl + hsqrtr;
+    T u = (qL[mu] / hsqrtl + qR[mu] / hsqrtr) / hsq2;
+    T v = (qL[mv] / hsqrtl + qR[mv] / hsqrtr) / hsq2;
+    T a = sqrt(grav * h);
+    r.u = u; r.v = v; r.a = a;
+    T delta0 = qR[0] - qL[0];
+    T delta1 = qR[mu] - qL[mu];
+    T delta2 = qR[mv] - qL[mv];
+    T a1 = ((u + a) * delta0 - delta1) * (T(0.50) / a);
+    T a2 = -v * delta0 + delta2;
+    T a3 = (-(u - a) * delta0 + delta1) * (T(0.50) / a);
+    wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; s[0] = u - a;
+    wave[1][0] = T(0); wave[1][mu] = T(0); wave[1][mv] = a2; s[1] = u;
+    wave[2][0] = a3; wave[2][mu] = a3 * (u + a); wave[2][mv] = a3 * v; s[2] = u + a;
+    if (!p.efix) {
+#pragma unroll
+      for (int m = 0; m < meqn; m++) {
+        amdq[m] = T(0); apdq[m] = T(0);
+#pragma unroll
+        for (int w = 0; w < mwaves; w++) {
+          if (s[w] < T(0)) amdq[m] = amdq[m] + s[w] * wave[w][m];
+          else apdq[m] = apdq[m] + s[w] * wave[w][m];
+        }
+      }
+      return;
+    }
+    do {
+      T him1 = qL[0];
+      T s0 = qL[mu] / him1 - sqrt(grav * him1);
+      if (s0 >= T(0) && s[0] > T(0)) {
+#pragma unroll
+        for (int m = 0; m < meqn; m++) amdq[m] = T(0);
+        break;
+      }
+      T h1 = qL[0] + wave[0][0];
+      T hu1 = qL[mu] + wave[0][mu];
+      T s1 = hu1 / h1 - sqrt(grav * h1);
+      T sfract;
+      if (s0 < T(0) && s1 > T(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+      else if (s[0] < T(0)) sfract = s[0];
+      else sfract = T(0);
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = sfract * wave[0][m];
+      if (s[1] >= T(0)) break;
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = amdq[m] + s[1] * wave[1][m];
+      T hi = qR[0];
+      T s03 = qR[mu] / hi + sqrt(grav * hi);
+      T h3 = qR[0] - wave[2][0];
+      T hu3 = qR[mu] - wave[2][mu];
+      T s3 = hu3 / h3 + sqrt(grav * h3);
+      if (s3 < T(0) && s03 > T(0)) sfract = s3 * (s03 - s[2]) / (s03 - s3);
+      else if (s[2] < T(0)) sfract = s[2];
+      else break;
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = amdq[m] + sfract * wave[2][m];
+    } while (0);
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      T df = T(0);
+#pragma unroll
+      for (int w = 0; w < mwaves; w++) df = df + s[w] * wave[w][m];
+      apdq[m] = df - amdq[m];
+    }
+  }
+
+  // The transverse split recomputes the Roe average with the same operations as
+  // rpn except that the wave speed uses the mean depth; it is carried in `r`.
+  template <int IXY>
+  __device__ static __forceinline__ void rpt(const RPParams<T>&, const Roe& r, const T (&asdq)[meqn],
+                                             T (&bm)[meqn], T (&bp)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T u = r.u, v = r.v, a = r.a;
+    T a1 = (T(0.50) / a) * ((v + a) * asdq[0] - asdq[mv]);
+    T a2 = asdq[mu] - u * asdq[0];
+    T a3 = (T(0.50) / a) * (-(v - a) * asdq[0] + asdq[mv]);
+    T wb[3][meqn], sb[3];
+    wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
+    wb[1][0] = T(0); wb[1][mu] = a2; wb[1][mv] = T(0); sb[1] = v;
+    wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      bm[m] = T(0); bp[m] = T(0);
+#pragma unroll
+      for (int w = 0; w < 3; w++) {
+        bm[m] = bm[m] + dmin(sb[w], T(0)) * wb[w][m];
+        bp[m] = bp[m] + dmax(sb[w], T(0)) * wb[w][m];
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------- advection
+template <typename T>
+struct Advection {
+  static constexpr int meqn = 1, mwaves = 1;
+  struct Roe {};
+  template <int IXY>
+  __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+    wave[0][0] = qR[0] - qL[0];
+    s[0] = IXY == 1 ? p.u : p.v;
+    amdq[0] = dmin(s[0], T(0)) * wave[0][0];
+    apdq[0] = dmax(s[0], T(0)) * wave[0][0];
+  }
+  template <int IXY>
+  __device__ static __forceinline__ void rpt(const RPParams<T>& p, const Roe&, const T (&asdq)[meqn],
+                                             T (&bm)[meqn], T (&bp)[meqn]) {
+    T stran = IXY == 1 ? p.v : p.u;
+    bm[0] = dmin(stran, T(0)) * asdq[0];
+    bp[0] = dmax(stran, T(0)) * asdq[0];
+  }
+};
+
+// ---------------------------------------------------------------- 3D Euler
+template <typename T>
+struct Euler3 {
+  static constexpr int meqn = 5, mwaves = 3;
+  struct Roe {};
+  // IXY = 1, 2, 3 selects x, y, z as the normal direction.
+  template <int IXY>
+  __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+    constexpr int mu = IXY == 1 ? 1 : (IXY == 2 ? 2 : 3);
+    constexpr int mv = IXY == 1 ? 2 : (IXY == 2 ? 3 : 1);
+    constexpr int mw = IXY == 1 ? 3 : (IXY == 2 ? 1 : 2);
+    const T gamma = p.gamma, gamma1 = p.gamma1;
+    T rhsqrtl = sqrt(qL[0]);
+    T rhsqrtr = sqrt(qR[0]);
+    T pl = gamma1 * (qL[4] - T(0.5) * (qL[1] * qL[1] + qL[2] * qL[2] + qL[3] * qL[3]) / qL[0]);
+    T pr = gamma1 * (qR[4] - T(0.5) * (qR[1] * qR[1] + qR[2] * qR[2] + qR[3] * qR[3]) / qR[0]);
+    T rhsq2 = rhsqrtl + rhsqrtr;
+    T u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
+    T v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
+    T w = (qL[mw] / rhsqrtl + qR[mw] / rhsqrtr) / rhsq2;
+    T enth = (((qL[4] + pl) / rhsqrtl + (qR[4] + pr) / rhsqrtr)) / rhsq2;
+    T u2v2w2 = u * u + v * v + w * w;
+    T a2 = gamma1 * (enth - T(.5) * u2v2w2);
+    T a = sqrt(a2);
+    T g1a2 = gamma1 / a2;
+    T euv = enth - u2v2w2;
+    T d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv], d3 = qR[mw] - qL[mw],
+      d4 = qR[4] - qL[4];
+    T a4 = g1a2 * (euv * d0 + u * d1 + v * d2 + w * d3 - d4);
+    T b2 = d2 - v * d0;
+    T b3 = d3 - w * d0;
+    T a5 = (d1 + (a - u) * d0 - a * a4) / (T(2) * a);
+    T a1 = d0 - a4 - a5;
+    wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; wave[0][mw] = a1 * w;
+    wave[0][4] = a1 * (enth - u * a); s[0] = u - a;
+    wave[1][0] = a4; wave[1][mu] = a4 * u; wave[1][mv] = a4 * v + b2; wave[1][mw] = a4 * w + b3;
+    wave[1][4] = a4 * T(0.5) * u2v2w2 + b2 * v + b3 * w; s[1] = u;
+    wave[2][0] = a5; wave[2][mu] = a5 * (u + a); wave[2][mv] = a5 * v; wave[2][mw] = a5 * w;
+    wave[2][4] = a5 * (enth + u * a); s[2] = u + a;
+    if (!p.efix) {
+#pragma unroll
+      for (int m = 0; m < meqn; m++) {
+        amdq[m] = T(0); apdq[m] = T(0);
+#pragma unroll
+        for (int k = 0; k < mwaves; k++) {
+          if (s[k] < T(0)) amdq[m] = amdq[m] + s[k] * wave[k][m];
+          else apdq[m] = apdq[m] + s[k] * wave[k][m];
+        }
+      }
+      return;
+    }
+    do {
+      T rhoim1 = qL[0];
+      T pim1 = gamma1 * (qL[4] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv] + qL[mw] * qL[mw]) / rhoim1);
+      T cim1 = sqrt(gamma * pim1 / rhoim1);
+      T s0 = qL[mu] / rhoim1 - cim1;
+      if (s0 >= T(0) && s[0] > T(0)) {
+#pragma unroll
+        for (int m = 0; m < meqn; m++) amdq[m] = T(0);
+        break;
+      }
+      T rho1 = qL[0] + wave[0][0];
+      T rhou1 = qL[mu] + wave[0][mu];
+      T rhov1 = qL[mv] + wave[0][mv];
+      T rhow1 = qL[mw] + wave[0][mw];
+      T en1 = qL[4] + wave[0][4];
+      T p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) / rho1);
+      T c1 = sqrt(gamma * p1 / rho1);
+      T s1 = rhou1 / rho1 - c1;
+      T sfract;
+      if (s0 < T(0) && s1 > T(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
+      else if (s[0] < T(0)) sfract = s[0];
+      else sfract = T(0);
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = sfract * wave[0][m];
+      if (s[1] >= T(0)) break;
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = amdq[m] + s[1] * wave[1][m];
+      T rhoi = qR[0];
+      T pi = gamma1 * (qR[4] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv] + qR[mw] * qR[mw]) / rhoi);
+      T ci = sqrt(gamma * pi / rhoi);
+      T s3 = qR[mu] / rhoi + ci;
+      T rho2 = qR[0] - wave[2][0];
+      T rhou2 = qR[mu] - wave[2][mu];
+      T rhov2 = qR[mv] - wave[2][mv];
+      T rhow2 = qR[mw] - wave[2][mw];
+      T en2 = qR[4] - wave[2][4];
+      T p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) / rho2);
+      T c2 = sqrt(gamma * p2 / rho2);
+      T s2 = rhou2 / rho2 + c2;
+      if (s2 < T(0) && s3 > T(0)) sfract = s2 * (s3 - s[2]) / (s3 - s2);
+      else if (s[2] < T(0)) sfract = s[2];
+      else break;
+#pragma unroll
+      for (int m = 0; m < meqn; m++) amdq[m] = amdq[m] + sfract * wave[2][m];
+    } while (0);
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      T df = T(0);
+#pragma unroll
+      for (int k = 0; k < mwaves; k++) df = df + s[k] * wave[k][m];
+      apdq[m] = df - amdq[m];
+    }
+  }
+  template <int IXY>
+  __device__ static __forceinline__ void rpt(const RPParams<T>&, const Roe&, const T (&)[meqn],
+                                             T (&bm)[meqn], T (&bp)[meqn]) {
+#pragma unroll
+    for (int m = 0; m < meqn; m++) { bm[m] = T(0); bp[m] = T(0); }
+  }
+};
+
+}  // namespace claw

@@ -1,0 +1,7 @@
+"""Limiter identifiers (pyclaw/limiters/tvd.py:74-77) and philim's Beam-Warming
+case (classic/philim.f90:296-297)."""
+minmod = 1
+superbee = 2
+vanleer = 3
+MC = 4
+beam_warming = 5

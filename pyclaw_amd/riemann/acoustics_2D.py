@@ -1,0 +1,10 @@
+"""Stand-in for clawpack.riemann.acoustics_2D: metadata and the device solver tag."""
+from ._rp import RPHandle
+
+kind = 1
+num_eqn = 3
+num_waves = 2
+fwave = False
+rpn2 = RPHandle(kind, "rpn2")
+rpt2 = RPHandle(kind, "rpt2")
+rpn3 = RPHandle(kind, "rpn3")

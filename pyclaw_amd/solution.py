@@ -1,0 +1,78 @@
+"""Solution (pyclaw/solution.py:15-420): a list of States on a Domain.  Frame
+I/O is not part of this package."""
+import copy
+
+from .geometry import Dimension, Domain, Patch
+from .state import State
+
+
+class Solution(object):
+    def __getattr__(self, key):
+        if key in ("t", "num_eqn", "num_aux", "mp", "mF", "q", "p", "F", "aux", "capa",
+                   "problem_data", "num_dim", "p_centers", "p_nodes", "c_centers", "c_nodes",
+                   "meqn", "mbc", "grid"):
+            states = self.__dict__.get("states")
+            if not states:
+                raise AttributeError(key)
+            return getattr(states[0], key)
+        raise AttributeError(key)
+
+    def __setattr__(self, key, value):
+        if key in ("t", "mp", "mF", "problem_data", "q", "aux"):
+            for s in self.states:
+                setattr(s, key, value)
+        else:
+            self.__dict__[key] = value
+
+    state = property(lambda self: self.states[0])
+    patch = property(lambda self: self.domain.patch)
+
+    @property
+    def start_frame(self):
+        return self.__dict__.get("_start_frame", 0)
+
+    def __init__(self, *arg, **kargs):
+        self.__dict__["states"] = []
+        self.__dict__["domain"] = None
+        if len(arg) == 2:
+            geom = arg[1]
+            if isinstance(geom, Domain):
+                self.domain = geom
+            else:
+                geom = list(geom) if isinstance(geom, (list, tuple)) else [geom]
+                if isinstance(geom[0], Dimension):
+                    self.domain = Domain(Patch(geom))
+                elif isinstance(geom[0], Patch):
+                    self.domain = Domain(geom)
+                else:
+                    raise Exception("Invalid arguments for Solution initialization.")
+            if isinstance(arg[0], State):
+                self.states.append(arg[0])
+            elif isinstance(arg[0], (list, tuple)) and isinstance(arg[0][0], State):
+                self.__dict__["states"] = list(arg[0])
+            elif isinstance(arg[0], (list, tuple)) and isinstance(arg[0][0], int):
+                self.states.append(State(self.domain, arg[0][0], arg[0][1]))
+            elif isinstance(arg[0], int):
+                self.states.append(State(self.domain, arg[0]))
+            if not self.states or self.domain is None:
+                raise Exception("Invalid arguments for Solution initialization.")
+        elif len(arg) == 0 and not kargs:
+            pass
+        else:
+            raise Exception("Reading frames is not supported by pyclaw_amd; "
+                            "construct Solution(state_or_num_eqn, domain).")
+
+    def is_valid(self):
+        return all(s.is_valid() for s in self.states)
+
+    def __copy__(self):
+        return self.__deepcopy__()
+
+    def __deepcopy__(self, memo={}):
+        result = Solution.__new__(Solution)
+        result.__dict__["states"] = [copy.deepcopy(s) for s in self.states]
+        result.__dict__["domain"] = copy.deepcopy(self.domain)
+        return result
+
+    def write(self, *args, **kwargs):
+        raise NotImplementedError("frame output is not part of pyclaw_amd")
