@@ -1,0 +1,207 @@
+"""Benchmark: cell-updates/s of the PyClaw time-step hot path on one MI355X.
+
+Workload (the headline configuration): 2D Euler, gamma = 1.4, Roe solver with
+Harten entropy fix, classic unsplit wave propagation with transverse
+corrections (trans_cor) and the MC limiter, 1024 x 1024 cells on [0,1]^2,
+extrapolation boundaries, CFL 0.8 (cfl_max 1.0), 400 steps.  Initial data: a
+four-quadrant Riemann problem whose quadrant states are drawn from
+numpy.random.default_rng(2): rho ~ U[0.5,1.5], p ~ U[0.1,1.0], u, v ~ U[-0.5,0.5].
+
+A step is one full ClawSolver2D time step through the PyClaw API (dt from the
+previous step's CFL, accept/reject), state resident in HBM.  `value` is the FP64
+path (bitwise equal to the reference step); the FP32 Kahan-compensated path runs
+the same steps beside it and its error against the FP64 run is reported.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MX = MY = 1024
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_F64 = 64     # read q + write q, 4 fields x 8 B
+BYTES_PER_CELL_F32K = 64    # q + compensation, read + write, 4 fields x 4 B x 2
+
+
+def quadrant_state(mx=MX, my=MY, seed=2, gamma=1.4):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.5, 1.5, 4)
+    p = rng.uniform(0.1, 1.0, 4)
+    u = rng.uniform(-0.5, 0.5, 4)
+    v = rng.uniform(-0.5, 0.5, 4)
+    xc = (np.arange(mx) + 0.5) / mx
+    yc = (np.arange(my) + 0.5) / my
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    quad = (X >= 0.5).astype(int) + 2 * (Y >= 0.5).astype(int)
+    q = np.empty((4, mx, my), order="F")
+    q[0] = rho[quad]
+    q[1] = rho[quad] * u[quad]
+    q[2] = rho[quad] * v[quad]
+    q[3] = p[quad] / (gamma - 1.0) + 0.5 * rho[quad] * (u[quad] ** 2 + v[quad] ** 2)
+    return q
+
+
+def make_problem(precision, mx=MX, my=MY):
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    solver = pyclaw.ClawSolver2D(riemann.euler_4wave_2D)
+    solver.dimensional_split = False
+    solver.transverse_waves = solver.trans_cor
+    solver.limiters = pyclaw.limiters.tvd.MC
+    solver.cfl_desired = 0.8
+    solver.cfl_max = 1.0
+    solver.all_bcs = pyclaw.BC.extrap
+    solver.precision = precision
+    domain = pyclaw.Domain([0., 0.], [1., 1.], [mx, my])
+    solution = pyclaw.Solution(4, domain)
+    solution.problem_data["gamma"] = 1.4
+    solution.q = quadrant_state(mx, my)
+    solver.dt = 0.8 * (1.0 / mx) / 3.0
+    solver.dt_initial = solver.dt
+    return solver, solution
+
+
+class KernelTimer(object):
+    """HIP events around every step launch, on the stream the kernel runs on."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.pairs = []
+
+    def wrap(self, solver):
+        from pyclaw_amd import kernels as K
+        orig = K.step2_unsplit
+        timer = self
+
+        def timed(*a, **kw):
+            s = timer.torch.cuda.current_stream()
+            e0 = timer.torch.cuda.Event(enable_timing=True)
+            e1 = timer.torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            orig(*a, **kw)
+            e1.record(s)
+            timer.pairs.append((e0, e1))
+        K.step2_unsplit = timed
+        return orig
+
+    def mean_ms(self):
+        self.torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in self.pairs]))
+
+
+def run_gpu(precision, steps, warmup):
+    import torch
+    solver, solution = make_problem(precision)
+    for _ in range(warmup):
+        solver.evolve_to_time(solution)
+    torch.cuda.synchronize()
+    timer = KernelTimer()
+    from pyclaw_amd import kernels as K
+    orig = timer.wrap(solver)
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solver.evolve_to_time(solution)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        K.step2_unsplit = orig
+    return wall, timer.mean_ms(), solver, solution
+
+
+def cpu_baseline(sample_steps=3):
+    """The reference's own Fortran step2 (classic/step2.f90 + flux2.f90, built from
+    the reference sources by oracle/ref.mk), single thread, on the same 1024^2
+    state: `sample_steps` unsplit steps with the reference BC fill before each."""
+    from oracle import oracle as O
+    kind = "reference"
+    if not O.ref_available():
+        kind = "port"
+    q = quadrant_state()
+    ng = 2
+    qbc = O.with_ghosts(q, ng)
+    rp = O.make_rp(O.EULER, efix=1, gamma=1.4)
+    meth = O.method_array(transverse=2)
+    dx = 1.0 / MX
+    dt = 0.8 * dx / 3.0
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        O.bc2(qbc, ng, [1, 1], [1, 1])
+        qnew = qbc.copy(order="F")
+        if kind == "reference":
+            O.ref_step2(rp, qbc, qnew, ng, dx, dx, dt, meth, [4] * 4)
+        else:
+            O.step2(rp, qbc, qnew, ng, dx, dx, dt, meth, [4] * 4)
+        qbc = qnew
+    wall = time.perf_counter() - t0
+    return {"value": MX * MY * sample_steps / wall, "unit": "cell-updates/s", "cores": 1,
+            "kind": kind,
+            "sample": f"{sample_steps} unsplit Euler steps on the same 1024x1024 state "
+                      f"(reference BC fill + classic/step2.f90 compiled with amdflang -O2), "
+                      f"{wall:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the hot path runs on a single MI355X in this build "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    cells = MX * MY
+    wall64, k64, solver64, sol64 = run_gpu("fp64", args.steps, args.warmup)
+    wall32, k32, solver32, sol32 = run_gpu("fp32k", args.steps, args.warmup)
+    q64 = sol64.state.q
+    q32 = sol32.state.q
+    err_l1 = float(np.abs(q32 - q64).sum() / np.abs(q64).sum())
+    err_max = float(np.abs(q32 - q64).max() / np.abs(q64).max())
+    value = cells * args.steps / wall64
+    achieved = BYTES_PER_CELL_F64 * cells / (k64 * 1e-3) / 1e9
+    rec = {
+        "metric": "cell-updates/s, 2D Euler/SWE 2048²: FP64 plain vs FP32+learned-WENO; % HBM peak",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall64 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "euler2d_quadrants_roe_efix_unsplit_transcor_mc_1024x1024",
+                   "grid": [MX, MY], "steps": args.steps, "cfl": 0.8, "bc": "extrap",
+                   "seed": 2, "t_end": sol64.t},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_unsplit<Euler<double>>", "kernel_ms": k64,
+                     "bytes_per_cell": BYTES_PER_CELL_F64},
+        "fp32_kahan": {"value": cells * args.steps / wall32, "ms_per_step": 1e3 * wall32 / args.steps,
+                       "kernel_ms": k32,
+                       "hbm_frac": BYTES_PER_CELL_F32K * cells / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "rel_l1_vs_fp64": err_l1, "rel_max_vs_fp64": err_max,
+                       "t_end": sol32.t},
+    }
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

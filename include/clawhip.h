@@ -65,6 +65,21 @@ typedef struct {
   long long field_stride, row_stride, slab_stride;
 } clawhip_geom;
 
+/* Device-resident time-step control: the state Solver.evolve_to_time keeps
+ * between steps (pyclaw/solver.py:554-666).  The caller fills it (t, dt of the
+ * first attempt after its own get_dt, tstart, tend, CFL targets, max_steps,
+ * dt_variable, cur = buffer holding the current state; zeros elsewhere) and
+ * reads it back. */
+typedef struct {
+  double t, dt, tstart, tend;
+  double cfl, cfl_desired, cfl_max, dt_max;
+  double cflmax, dtmin, dtmax;
+  int dt_variable, done, cur, attempts;
+  int max_steps, accepted, nrejected, pad0;
+  unsigned int waves_done, pad1;
+  unsigned long long cfl_bits;
+} clawhip_stepctl;
+
 /* Library / device information: writes the HIP device count; 0 on success. */
 int clawhip_device_count(int *count);
 
@@ -88,6 +103,20 @@ int clawhip_step2_unsplit(const clawhip_rp *rp, int dtype, const void *q_in,
                           const int *reflect, int num_ghost, const int *method,
                           const int *mthlim, double dx, double dy, double dt,
                           double *cfl_dev, void *stream);
+
+/* Up to nsteps attempted unsplit steps with the control on the device: each
+ * launch reads dt and the current buffer (q0/q1, comp0/comp1 by ctl->cur) from
+ * ctl_dev, steps, and its last wavefront applies accept_reject_step and the next
+ * get_dt (pyclaw/solver.py:524-552, 621-657) -- the arithmetic of
+ * evolve_to_time(solution, tend).  Launches after ctl->done != 0 exit at once:
+ * 1 reached tend, 2 CFL too large with fixed dt, 3 max_steps attempts made.
+ * The launches contain no host synchronisation and may be captured in a graph.
+ * Boundary codes 1..3. */
+int clawhip_step2_unsplit_run(const clawhip_rp *rp, int dtype, const clawhip_geom *g,
+                              const int *bc_lower, const int *bc_upper, const int *reflect,
+                              int num_ghost, const int *method, const int *mthlim, double dx,
+                              double dy, void *q0, void *q1, void *comp0, void *comp1,
+                              clawhip_stepctl *ctl_dev, int nsteps, void *stream);
 
 /* One dimension-split 2D step: both sweeps of ClawSolver2D.step_hyperbolic with
  * dimensional_split=True (pyclaw/classic/solver.py:523-535), i.e. fmod.step2ds
@@ -114,6 +143,73 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
                         const int *bc_lower, const int *bc_upper, const int *reflect,
                         int num_ghost, const int *method, const int *mthlim, double dx,
                         double dy, double dz, double dt, double *cfl_dev, void *stream);
+
+/* SharpClaw 2D operator with a fused Runge-Kutta stage.  Computes
+ * D = dt*L(Y) -- SharpClawSolver2D.dq_hyperbolic (sharpclaw/solver.py:848-898),
+ * i.e. the ghost fill plus fmod.flux2 (sharpclaw/flux2.f90, flux1.f90; WENO5 via
+ * weno.f90:5-83 for char_decomp 0, reconstruct.f90:201-299 for char_decomp 2) --
+ * and writes one stage of SharpClawSolver.step (sharpclaw/solver.py:283-430):
+ *   y_mode   0: Y = P;  1: Y = P + s*Q;  2: Y = P + (s*Q)/6.
+ *   out_mode 0: out = D/dt                      (dq_dt, solver.py:293)
+ *            1: out = 0.75*A + 0.25*(Y + D)     (SSP33 stage 2, solver.py:318)
+ *            2: out = 1/3*A + 2/3*(Y + D)       (SSP33 stage 3, solver.py:323)
+ *            3: out = Y + D/6.                  (SSP104 stages, solver.py:415,425)
+ *            4: out = A + (0.6*Y + 0.1*D)       (SSP104 last stage, solver.py:430)
+ *            5: FP32 SSP33 stage 2: inc = s*B + D; out = A + 0.25*inc; aux_out = inc/6
+ *            6: FP32 SSP33 stage 3: Kahan update of A by aux_in + 2/3*D (comp_in->comp_out)
+ *            7: out = D                         (fmod.flux2's dq on the interior)
+ * All arrays compact (ghosts = 0) with geometry g; num_ghost is 3 (WENO5).
+ * Boundary codes 1..3 only.  *cfl_dev = max(*cfl_dev, CFL of this evaluation).
+ * flags (may be NULL): per-interface troubled flags from
+ * clawhip_mlp_flags; interfaces flagged 0 use WENO5's linear 5th-order
+ * reconstruction instead of the characteristic WENO5.  NULL = full WENO5. */
+int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
+                         const clawhip_geom *g, const int *bc_lower, const int *bc_upper,
+                         const int *reflect, int y_mode, int out_mode, double dt, double s,
+                         double dx, double dy, const void *P, const void *Q, const void *A,
+                         const void *B, const void *aux_in, const void *comp_in, void *out,
+                         void *aux_out, void *comp_out, const unsigned char *flags,
+                         double *cfl_dev, void *stream);
+
+/* The elementwise middle of SSP104 (sharpclaw/solver.py:418-419):
+ *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */
+int clawhip_ssp104_mid(int dtype, void *q, void *s1, long long n, void *stream);
+
+/* ---- learned troubled-interface indicator (no counterpart in the reference;
+ * the reference always runs the full WENO5 of reconstruct.f90:201-299).
+ * A 6->32->32->1 MLP in BF16 MFMA with FP32 accumulation.  Parameters: one FP32
+ * array of clawhip_mlp_param_count() floats: W1[16][32] (row 6 = bias, rows 7..15
+ * zero), W2[32][32], b2[32], w3[32], b3.
+ * Samples are (interface, component): x-interfaces (i = 1..mx+1, j = 1..my)
+ * then y-interfaces (i = 1..mx, j = 1..my+1), row-major, times num_eqn. */
+int clawhip_mlp_param_count(void);
+
+/* Features (8 floats per sample: the min-max normalised 6-cell stencil, 1, 0)
+ * and labels (1 = the component's WENO5 weights depart from the linear weights by
+ * more than wtol while the stencil range exceeds flat) of every sample of q. */
+int clawhip_mlp_features(int dtype, const void *q, const clawhip_geom *g, const int *bc_lower,
+                         const int *bc_upper, const int *reflect, int meqn, float flat,
+                         float wtol, float *feat, unsigned char *label, void *stream);
+
+/* One training step on the samples batch_idx[0..nbatch) of feat/label (nsamples
+ * rows; indices outside [0, nsamples) are skipped): grad += d(loss)/d(params)
+ * (weighted binary cross-entropy, positive weight pos_weight, mean over nbatch),
+ * *loss += loss.  step > 0 then applies Adam step `step` (lr, 0.9, 0.999, 1e-8) to
+ * params and clears grad; step == 0 only accumulates (gradient checks). */
+int clawhip_mlp_train_step(const float *feat, long long nsamples, const unsigned char *label,
+                           const int *batch_idx, int nbatch, float *params, float *grad,
+                           float *adam_m, float *adam_v, int step, float lr, float pos_weight,
+                           float *loss, void *stream);
+
+/* Logits of n samples (feat as from clawhip_mlp_features). */
+int clawhip_mlp_logits(const float *feat, long long n, const float *params, float *logits,
+                       void *stream);
+
+/* Per-interface flags of a state: 1 when any component's logit > logit_threshold. */
+int clawhip_mlp_flags(int dtype, const void *q, const clawhip_geom *g, const int *bc_lower,
+                      const int *bc_upper, const int *reflect, int meqn, float flat,
+                      const float *params, float logit_threshold, unsigned char *flags,
+                      void *stream);
 
 #ifdef __cplusplus
 }

@@ -174,6 +174,17 @@ def sharp_flux1(rp, q1d, ng, ixy, dtdx, char_decomp):
     return dq1d, cfl
 
 
+def weno5(q1d, ng):
+    """Component-wise WENO5 edge values (orc_weno5, weno.f90:5-83) of a
+    1D slice (meqn, n2) with ng ghosts: returns (ql, qr), left and right edges."""
+    q1d = np.asfortranarray(q1d, dtype=np.float64)
+    meqn, n2 = q1d.shape
+    ql = np.zeros_like(q1d, order="F")
+    qr = np.zeros_like(q1d, order="F")
+    lib().orc_weno5(meqn, ng, n2, _dp(q1d), _dp(ql), _dp(qr))
+    return ql, qr
+
+
 def evec(rp, ixy, q1d):
     meqn, n2 = q1d.shape
     evl = np.zeros((meqn, meqn, n2), order="F")

@@ -30,6 +30,20 @@ class RP(ctypes.Structure):
                 ("u", ctypes.c_double), ("v", ctypes.c_double)]
 
 
+class StepCtl(ctypes.Structure):
+    """clawhip_stepctl (include/clawhip.h)."""
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("tstart", ctypes.c_double),
+                ("tend", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("cfl_desired", ctypes.c_double), ("cfl_max", ctypes.c_double),
+                ("dt_max", ctypes.c_double), ("cflmax", ctypes.c_double),
+                ("dtmin", ctypes.c_double), ("dtmax", ctypes.c_double),
+                ("dt_variable", ctypes.c_int), ("done", ctypes.c_int), ("cur", ctypes.c_int),
+                ("attempts", ctypes.c_int), ("max_steps", ctypes.c_int),
+                ("accepted", ctypes.c_int), ("nrejected", ctypes.c_int), ("pad0", ctypes.c_int),
+                ("waves_done", ctypes.c_uint), ("pad1", ctypes.c_uint),
+                ("cfl_bits", ctypes.c_ulonglong)]
+
+
 class Geom(ctypes.Structure):
     _fields_ = [("mx", ctypes.c_int), ("my", ctypes.c_int), ("mz", ctypes.c_int),
                 ("ghosts", ctypes.c_int), ("field_stride", ctypes.c_longlong),
@@ -51,6 +65,7 @@ def geom3(mx, my, mz, ghosts=0):
 _I = ctypes.c_int
 _P = ctypes.c_void_p
 _D = ctypes.c_double
+_F = ctypes.c_float
 _IP = ctypes.POINTER(ctypes.c_int)
 
 _SIGS = {
@@ -65,6 +80,19 @@ _SIGS = {
     "clawhip_step3_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                             _D, _D, _D, _D, _P, _P],
+    "clawhip_sharp2_stage": [ctypes.POINTER(RP), _I, _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
+                             _I, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                             _P],
+    "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
+    "clawhip_step2_unsplit_run": [ctypes.POINTER(RP), _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
+                                  _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _I, _P],
+    "clawhip_mlp_param_count": [],
+    "clawhip_mlp_features": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _F, _P, _P,
+                             _P],
+    "clawhip_mlp_train_step": [_P, ctypes.c_longlong, _P, _P, _I, _P, _P, _P, _P, _I, _F, _F,
+                               _P, _P],
+    "clawhip_mlp_logits": [_P, ctypes.c_longlong, _P, _P, _P],
+    "clawhip_mlp_flags": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _P, _F, _P, _P],
 }
 
 _lib = None

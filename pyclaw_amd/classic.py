@@ -105,6 +105,13 @@ class ClawSolver2D(ClawSolver):
         self.aux3 = None
         self.work = None
         self._work = None
+        # Run evolve_to_time(solution, tend) as device-controlled launches (no
+        # host synchronisation per step) when no host callback is involved.
+        self.device_loop = True
+        self.use_graph = True
+        self.graph_chunk = 16
+        self._ctl = None
+        self._graphs = {}
         super(ClawSolver2D, self).__init__(riemann_solver, claw_package)
 
     def _check_cfl_settings(self):
@@ -144,6 +151,114 @@ class ClawSolver2D(ClawSolver):
         self.cfl.update_global_max(cfl)
         state.set_device_q(*out)
         self._scratch = qin
+
+    # ---- device-resident loop
+    def _device_loop_ok(self, solution):
+        lo, hi = self._bc_codes()
+        return (self.device_loop and not self.dimensional_split and self.before_step is None
+                and self.step_source is None and BC.custom not in lo and BC.custom not in hi)
+
+    def evolve_to_time(self, solution, tend=None):
+        if not self._is_set_up:
+            self.setup(solution)
+        if tend is None or not self._device_loop_ok(solution):
+            return super(ClawSolver2D, self).evolve_to_time(solution, tend)
+        tstart = solution.t
+        max_steps = self.max_steps
+        if not self.dt_variable:
+            max_steps = int((tend - tstart + 1e-10) / self.dt)
+            self.max_steps = max_steps
+            if abs(max_steps * self.dt - (tend - tstart)) > 1e-5 * (tend - tstart):
+                raise Exception("dt does not divide (tend-tstart) and dt is fixed!")
+        if self.dt_variable == 1 and self.cfl_desired > self.cfl_max:
+            raise Exception("Variable time-stepping and desired CFL > maximum CFL")
+        if tend <= tstart:
+            self.logger.info("Already at or beyond end time: no evolution required.")
+            return self.status
+        self.run_device(solution, tend, max_steps)
+        return self.status
+
+    def run_device(self, solution, tend, max_steps):
+        """Up to max_steps attempted steps towards tend, controlled on the device.
+        Same arithmetic and decisions as Solver.evolve_to_time (solver.py:554-666)."""
+        import torch
+        from . import _lib as L
+        state = solution.states[0]
+        self.get_dt(solution.t, solution.t, tend, False)   # first attempt's dt (host)
+        qin = state.device_q(self._dtype_code)
+        other = self._take_scratch(qin)
+        key = (qin[0].data_ptr(), other[0].data_ptr())
+        if key in self._graphs or (other[0].data_ptr(), qin[0].data_ptr()) in self._graphs:
+            # keep buffer identities fixed for captured graphs: slot 0 / slot 1
+            k2 = key if key in self._graphs else (other[0].data_ptr(), qin[0].data_ptr())
+            bufs = self._graphs[k2]["bufs"]
+            cur = 0 if bufs[0][0].data_ptr() == qin[0].data_ptr() else 1
+        else:
+            bufs = (qin, other)
+            cur = 0
+        ctl = L.StepCtl()
+        ctl.t, ctl.dt, ctl.tstart, ctl.tend = solution.t, self.dt, solution.t, tend
+        ctl.cfl = self.cfl.get_cached_max()
+        ctl.cfl_desired, ctl.cfl_max, ctl.dt_max = self.cfl_desired, self.cfl_max, self.dt_max
+        ctl.cflmax, ctl.dtmin, ctl.dtmax = (self.status["cflmax"], self.status["dtmin"],
+                                           self.status["dtmax"])
+        ctl.dt_variable = int(bool(self.dt_variable))
+        ctl.cur = cur
+        ctl.max_steps = int(max_steps)
+        raw = bytes(ctl)
+        if self._ctl is None:
+            self._ctl = torch.empty(len(raw), dtype=torch.uint8, device="cuda")
+        self._ctl.copy_(torch.frombuffer(bytearray(raw), dtype=torch.uint8))
+        grid = state.grid
+        dx, dy = grid.delta
+        lo, hi = self._bc_codes()
+        args = (self._rp_struct, (bufs[0][0], bufs[1][0]), (bufs[0][1], bufs[1][1]), self._ctl,
+                lo, hi, self.reflect_index, self._method, self._mthlim, dx, dy)
+        launched = 0
+        host = L.StepCtl()
+        while True:
+            n = min(self.graph_chunk, max_steps - launched)
+            if n <= 0:
+                break
+            if self.use_graph and n == self.graph_chunk:
+                self._replay_graph(bufs, args, n)
+            else:
+                K.step2_unsplit_run(*args, n)
+            launched += n
+            raw = self._ctl.cpu().numpy().tobytes()
+            ctypes_memmove(host, raw)
+            if host.done:
+                break
+        solution.t = host.t
+        self.dt = host.dt
+        self.cfl.set_global_max(host.cfl)
+        self.status["cflmax"], self.status["dtmin"], self.status["dtmax"] = (host.cflmax,
+                                                                              host.dtmin, host.dtmax)
+        self.status["numsteps"] += host.accepted
+        state.set_device_q(*bufs[host.cur])
+        self._scratch = bufs[1 - host.cur]
+        self.accept_step = host.done != 2
+        if host.done == 2:
+            raise Exception("CFL too large, giving up!")
+        if self.dt_variable and solution.t < tend and host.accepted == max_steps:
+            raise Exception("Maximum number of timesteps have been taken")
+        return host
+
+    def _replay_graph(self, bufs, args, n):
+        import torch
+        key = (bufs[0][0].data_ptr(), bufs[1][0].data_ptr())
+        ent = self._graphs.get(key)
+        if ent is None:
+            g = torch.cuda.CUDAGraph()
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    K.step2_unsplit_run(*args, n)
+            torch.cuda.current_stream().wait_stream(s)
+            ent = {"graph": g, "bufs": bufs, "n": n}
+            self._graphs[key] = ent
+        ent["graph"].replay()
 
     def _custom_bc_input(self, state, lo, hi):
         """User BC functions act on a host qbc (solver.py:393-398); the kernel then
@@ -205,6 +320,11 @@ class ClawSolver3D(ClawSolver):
         self.cfl.update_global_max(float(cfl_dev.item()))
         state.set_device_q(*out)
         self._scratch = qin
+
+
+def ctypes_memmove(struct, raw):
+    import ctypes
+    ctypes.memmove(ctypes.addressof(struct), raw, ctypes.sizeof(struct))
 
 
 def _fill_bcs_host(solver, state, qbc, lo, hi):

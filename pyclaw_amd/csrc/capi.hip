@@ -138,4 +138,114 @@ int clawhip_step3_split(const clawhip_rp* rp, int dtype, const void* q_in,
                        dx, dy, dz, dt, (unsigned long long*)cfl_dev, (hipStream_t)stream);
 }
 
+int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
+                         const clawhip_geom* g, const int* bc_lower, const int* bc_upper,
+                         const int* reflect, int y_mode, int out_mode, double dt, double s,
+                         double dx, double dy, const void* P, const void* Q, const void* A,
+                         const void* B, const void* aux_in, const void* comp_in, void* out,
+                         void* aux_out, void* comp_out, const unsigned char* flags,
+                         double* cfl_dev, void* stream) {
+  if (!rp || !g || !bc_lower || !bc_upper || !reflect || !P || !out || !cfl_dev)
+    return CLAWHIP_E_ARG;
+  if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
+  if (y_mode < 0 || y_mode > 2 || out_mode < 0 || out_mode > 7) return CLAWHIP_E_ARG;
+  if (y_mode != 0 && !Q) return CLAWHIP_E_ARG;
+  if ((out_mode == 1 || out_mode == 2 || (out_mode >= 4 && out_mode <= 6)) && !A)
+    return CLAWHIP_E_ARG;
+  if (out_mode == 5 && (!B || !aux_out || dtype != CLAWHIP_F32K)) return CLAWHIP_E_ARG;
+  if (out_mode == 6 && (!aux_in || !comp_in || !comp_out || dtype != CLAWHIP_F32K))
+    return CLAWHIP_E_ARG;
+  if (out == P || (Q && out == Q) || (A && out == A)) return CLAWHIP_E_ARG;
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dt > 0.0)) return CLAWHIP_E_ARG;
+  for (int d = 0; d < 2; d++)
+    if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
+      return CLAWHIP_E_ARG;
+  return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), bc_lower, bc_upper, reflect,
+                            y_mode, out_mode, dt, s, dx, dy, P, Q, A, B, aux_in, comp_in, out,
+                            aux_out, comp_out, (unsigned long long*)cfl_dev, flags,
+                            (hipStream_t)stream);
+}
+
+int clawhip_ssp104_mid(int dtype, void* q, void* s1, long long n, void* stream) {
+  if (!q || !s1 || n < 0 || q == s1) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  return launch_ssp104_mid(dtype, q, s1, n, (hipStream_t)stream);
+}
+
+int clawhip_step2_unsplit_run(const clawhip_rp* rp, int dtype, const clawhip_geom* g,
+                              const int* bc_lower, const int* bc_upper, const int* reflect,
+                              int num_ghost, const int* method, const int* mthlim, double dx,
+                              double dy, void* q0, void* q1, void* comp0, void* comp1,
+                              clawhip_stepctl* ctl_dev, int nsteps, void* stream) {
+  if (!rp || !g || !bc_lower || !bc_upper || !reflect || !method || !mthlim || !q0 || !q1 ||
+      !ctl_dev || nsteps < 0)
+    return CLAWHIP_E_ARG;
+  if (!geom_ok(g, 2) || g->ghosts != 0 || num_ghost != 2 || g->mx < 2 || g->my < 2)
+    return CLAWHIP_E_ARG;
+  if (q0 == q1) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (dtype == CLAWHIP_F32K && (!comp0 || !comp1 || comp0 == comp1)) return CLAWHIP_E_ARG;
+  if (!(dx > 0.0) || !(dy > 0.0)) return CLAWHIP_E_ARG;
+  if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
+  if (method[2] < 0 || method[2] > 2) return CLAWHIP_E_METHOD;
+  if (method[5] != 0 || method[6] != 0) return CLAWHIP_E_METHOD;
+  for (int d = 0; d < 2; d++)
+    if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
+      return CLAWHIP_E_ARG;
+  static_assert(sizeof(clawhip_stepctl) == sizeof(StepCtl), "clawhip_stepctl layout");
+  CtlBuffers b;
+  b.q[0] = q0; b.q[1] = q1; b.comp[0] = comp0; b.comp[1] = comp1; b.dx = dx; b.dy = dy;
+  return launch_unsplit_ctl(*rp, dtype, to_geom(g), bc_lower, bc_upper, reflect, num_ghost,
+                            method, mthlim, (StepCtl*)ctl_dev, b, nsteps, (hipStream_t)stream);
+}
+
+int clawhip_mlp_param_count(void) { return 16 * 32 + 32 * 32 + 32 + 32 + 1; }
+
+int clawhip_mlp_features(int dtype, const void* q, const clawhip_geom* g, const int* bc_lower,
+                         const int* bc_upper, const int* reflect, int meqn, float flat,
+                         float wtol, float* feat, unsigned char* label, void* stream) {
+  if (!q || !g || !bc_lower || !bc_upper || !reflect || !feat || !label) return CLAWHIP_E_ARG;
+  if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3 || meqn < 1) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  for (int d = 0; d < 2; d++)
+    if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
+      return CLAWHIP_E_ARG;
+  return launch_features_labels(dtype, q, to_geom(g), bc_lower, bc_upper, reflect, meqn, flat,
+                                wtol, feat, label, (hipStream_t)stream);
+}
+
+int clawhip_mlp_train_step(const float* feat, long long nsamples, const unsigned char* label,
+                           const int* batch_idx, int nbatch, float* params, float* grad,
+                           float* adam_m, float* adam_v, int step, float lr, float pos_weight,
+                           float* loss, void* stream) {
+  if (!feat || !label || !batch_idx || nbatch < 1 || nsamples < 1 || !params || !grad || !loss)
+    return CLAWHIP_E_ARG;
+  if (step < 0 || (step > 0 && (!adam_m || !adam_v))) return CLAWHIP_E_ARG;
+  return launch_mlp_train(feat, nsamples, label, batch_idx, nbatch, params, grad, adam_m, adam_v,
+                          step, lr, pos_weight, loss, (hipStream_t)stream);
+}
+
+int clawhip_mlp_logits(const float* feat, long long n, const float* params, float* logits,
+                       void* stream) {
+  if (!feat || !params || !logits || n < 0) return CLAWHIP_E_ARG;
+  if (n == 0) return 0;
+  return launch_mlp_logits(feat, n, params, logits, (hipStream_t)stream);
+}
+
+int clawhip_mlp_flags(int dtype, const void* q, const clawhip_geom* g, const int* bc_lower,
+                      const int* bc_upper, const int* reflect, int meqn, float flat,
+                      const float* params, float logit_threshold, unsigned char* flags,
+                      void* stream) {
+  if (!q || !g || !bc_lower || !bc_upper || !reflect || !params || !flags) return CLAWHIP_E_ARG;
+  if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3 || meqn < 1) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  for (int d = 0; d < 2; d++)
+    if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
+      return CLAWHIP_E_ARG;
+  return launch_mlp_flags(dtype, q, to_geom(g), bc_lower, bc_upper, reflect, meqn, flat, params,
+                          logit_threshold, flags, (hipStream_t)stream);
+}
+
 }  // extern "C"

@@ -104,7 +104,7 @@ __device__ __forceinline__ void xpass(const UnsplitArgs<T>& a, const T (&Qc)[RS:
 }
 
 template <class RS, typename T, bool KAHAN>
-__global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
+__device__ __forceinline__ void unsplit_body(const UnsplitArgs<T>& a) {
   constexpr int ME = RS::meqn, MW = RS::mwaves;
   using Roe = typename RS::Roe;
   const int lane = threadIdx.x & 63;
@@ -266,6 +266,34 @@ __global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
   wave_atomic_max(cfl, a.cfl_bits);
 }
 
+template <class RS, typename T, bool KAHAN>
+__global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
+  unsplit_body<RS, T, KAHAN>(a);
+}
+
+// Device-resident stepping: one launch = one attempted step of
+// Solver.evolve_to_time.  The state ping-pongs between two buffers; `ctl->cur`
+// names the current one.  The last workgroup to finish applies the CFL check,
+// accepts or rejects the step and computes the next dt (step_ctl_finish).
+template <class RS, typename T, bool KAHAN>
+__global__ void __launch_bounds__(256) k_unsplit_ctl(UnsplitArgs<T> a, StepCtl* ctl,
+                                                     CtlBuffers bufs) {
+  if (ctl->done) return;   // uniform across the grid
+  const int cur = ctl->cur;
+  const double dt = ctl->dt;
+  a.qin = (const T*)bufs.q[cur];
+  a.qout = (T*)bufs.q[1 - cur];
+  a.comp_in = (const T*)bufs.comp[cur];
+  a.comp_out = (T*)bufs.comp[1 - cur];
+  a.dtdx = T(dt / bufs.dx);
+  a.dtdy = T(dt / bufs.dy);
+  a.cfl_bits = &ctl->cfl_bits;
+  unsplit_body<RS, T, KAHAN>(a);
+  const unsigned total = (unsigned)(a.nstrips * a.nrb);
+  const unsigned wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wid < total) step_ctl_finish(ctl, total);
+}
+
 }  // namespace claw
 
 // ---------------------------------------------------------------- launchers
@@ -299,6 +327,43 @@ int run_unsplit(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q
   return e == hipSuccess ? 0 : (int)e;
 }
 
+template <class RS, typename T, bool KAHAN>
+int run_unsplit_ctl(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
+                    const int* refl, int ng, const int* method, const int* mthlim,
+                    StepCtl* ctl, const CtlBuffers& bufs, int nsteps, hipStream_t stream) {
+  UnsplitArgs<T> a;
+  a.rp = to_params<T>(rp);
+  a.gin = g; a.gout = g;
+  a.bc = to_bc(lo, hi, refl, 2);
+  a.ng = ng; a.order = method[1]; a.trans = method[2];
+  for (int w = 0; w < 4; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
+  a.dtdx = T(0); a.dtdy = T(0);
+  a.H = kUnsplitRows;
+  a.nstrips = (g.mx + 59) / 60;
+  a.nrb = (g.my + a.H - 1) / a.H;
+  a.qin = nullptr; a.qout = nullptr; a.comp_in = nullptr; a.comp_out = nullptr;
+  a.cfl_bits = nullptr;
+  long long waves = (long long)a.nstrips * a.nrb;
+  int blocks = (int)((waves + 3) / 4);
+  for (int n = 0; n < nsteps; n++)
+    hipLaunchKernelGGL((k_unsplit_ctl<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, stream, a,
+                       ctl, bufs);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+template <template <typename> class RSF>
+int dispatch_unsplit_ctl(int dtype, const clawhip_rp& rp, const Geom& g, const int* lo,
+                         const int* hi, const int* refl, int ng, const int* method,
+                         const int* mthlim, StepCtl* ctl, const CtlBuffers& bufs, int nsteps,
+                         hipStream_t stream) {
+  if (dtype == CLAWHIP_F64)
+    return run_unsplit_ctl<RSF<double>, double, false>(rp, g, lo, hi, refl, ng, method, mthlim,
+                                                       ctl, bufs, nsteps, stream);
+  return run_unsplit_ctl<RSF<float>, float, true>(rp, g, lo, hi, refl, ng, method, mthlim, ctl,
+                                                  bufs, nsteps, stream);
+}
+
 template <template <typename> class RSF>
 int dispatch_unsplit(int dtype, const clawhip_rp& rp, const void* q_in, const Geom& gin,
                      void* q_out, const Geom& gout, const void* comp_in, void* comp_out, const int* lo, const int* hi,
@@ -330,6 +395,27 @@ int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom
     case CLAWHIP_RP_ADVECTION:
       return dispatch_unsplit<Advection>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
                                          ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
+  }
+  return CLAWHIP_E_KIND;
+}
+
+int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int* lo,
+                       const int* hi, const int* refl, int ng, const int* method,
+                       const int* mthlim, StepCtl* ctl, const CtlBuffers& bufs, int nsteps,
+                       hipStream_t stream) {
+  switch (rp.kind) {
+    case CLAWHIP_RP_ACOUSTICS:
+      return dispatch_unsplit_ctl<Acoustics>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
+                                             bufs, nsteps, stream);
+    case CLAWHIP_RP_EULER:
+      return dispatch_unsplit_ctl<Euler>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
+                                         bufs, nsteps, stream);
+    case CLAWHIP_RP_SHALLOW:
+      return dispatch_unsplit_ctl<Shallow>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
+                                           bufs, nsteps, stream);
+    case CLAWHIP_RP_ADVECTION:
+      return dispatch_unsplit_ctl<Advection>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
+                                             bufs, nsteps, stream);
   }
   return CLAWHIP_E_KIND;
 }

@@ -1,0 +1,525 @@
+// indicator.hip -- learned troubled-interface indicator for the SharpClaw WENO5
+// path: a 6 -> 32 -> 32 -> 1 MLP on BF16 MFMA (v_mfma_f32_16x16x16bf16_1k, FP32
+// accumulation) with hand-written training (forward, backward, Adam) and
+// inference kernels.
+//
+// Sample = (interface, conserved component): the six cell values of the WENO5
+// stencil of that interface (cells c-3..c+2), min-max normalised to [-1, 1]
+// (all zeros when the stencil range is below `flat`).  Label = the component's
+// Jiang-Shu WENO5 weights (the weight formulas of reconstruct.f90:264-284
+// applied to the component's differences, both sides) depart from the linear
+// weights (0.1, 0.6, 0.3) by more than `wtol` while the stencil range exceeds
+// `flat`.  An interface is smooth when no component is flagged; the stage kernel
+// then uses the linear 5th-order reconstruction (WENO5 with its linear weights)
+// and skips eigenvectors, projections and nonlinear weights.
+//
+// MFMA 16x16x16 bf16 operand layout (wave64): lane l holds A[l%16][4*(l/16)+0..3],
+// B[4*(l/16)+0..3][l%16]; the 4 FP32 results are D[4*(l/16)+0..3][l%16].
+#include <hip/hip_bf16.h>
+
+#include "launch.hpp"
+
+namespace claw {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NF = 16;      // padded input width (6 features, 1.0 bias at index 6)
+constexpr int NH = 32;      // hidden width
+
+// Parameter block layout (FP32 master and BF16 working copies use the same order)
+//   W1[NF][NH] (row 6 is b1), W2[NH][NH], b2[NH], w3[NH], b3
+constexpr int P_W1 = 0, P_W2 = P_W1 + NF * NH, P_B2 = P_W2 + NH * NH, P_W3 = P_B2 + NH,
+              P_B3 = P_W3 + NH, P_N = P_B3 + 1;
+
+__device__ __forceinline__ short f2bf(float f) {
+  // round to nearest even (finite inputs)
+  unsigned u = __float_as_uint(f);
+  unsigned r = u + 0x7fffu + ((u >> 16) & 1u);
+  return (short)(r >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
+
+// ---------------------------------------------------------------- features
+// stencil v[0..5] -> normalised features f[0..5]; returns the range.
+__device__ __forceinline__ float norm_stencil(const double (&v)[6], float flat, float (&f)[6]) {
+  double lo = v[0], hi = v[0];
+#pragma unroll
+  for (int k = 1; k < 6; k++) { lo = v[k] < lo ? v[k] : lo; hi = v[k] > hi ? v[k] : hi; }
+  double rng = hi - lo;
+  if (rng <= (double)flat) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) f[k] = 0.f;
+  } else {
+    double inv = 2.0 / rng;
+#pragma unroll
+    for (int k = 0; k < 6; k++) f[k] = (float)((v[k] - lo) * inv - 1.0);
+  }
+  return (float)rng;
+}
+
+// Departure of the scalar WENO5 weights from the linear weights, both sides.
+__device__ __forceinline__ double weno_departure(const double (&v)[6]) {
+  double hh[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) hh[k] = v[k + 1] - v[k];
+  double worst = 0.0;
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const int im = side == 0 ? 1 : -1;
+    double a2 = im == 1 ? hh[0] : hh[4], a1 = im == 1 ? hh[1] : hh[3];
+    double h0 = hh[2], b1 = im == 1 ? hh[3] : hh[1];
+    double t1 = im * (a2 - a1), t2 = im * (a1 - h0), t3 = im * (h0 - b1);
+    double e1 = a2 - 3.0 * a1, e2 = a1 + h0, e3 = 3.0 * h0 - b1;
+    const double eps = (double)1.0e-36f;
+    double tt1 = 13.0 * (t1 * t1) + 3.0 * (e1 * e1);
+    double tt2 = 13.0 * (t2 * t2) + 3.0 * (e2 * e2);
+    double tt3 = 13.0 * (t3 * t3) + 3.0 * (e3 * e3);
+    tt1 = (eps + tt1) * (eps + tt1);
+    tt2 = (eps + tt2) * (eps + tt2);
+    tt3 = (eps + tt3) * (eps + tt3);
+    double s1 = tt2 * tt3, s2 = 6.0 * tt1 * tt3, s3 = 3.0 * tt1 * tt2;
+    double sum = s1 + s2 + s3;
+    double w1 = s1 / sum, w2 = s2 / sum, w3 = s3 / sum;
+    double d = fabs(w1 - 0.1);
+    d = fmax(d, fabs(w2 - 0.6));
+    d = fmax(d, fabs(w3 - 0.3));
+    worst = fmax(worst, d);
+  }
+  return worst;
+}
+
+// Gather the stencil of interface `idx` (x-interfaces first, then y) for field m.
+// x-interface (c, r): c in 1..mx+1, r in 1..my; y-interface (c, r): r in 1..my+1.
+template <typename T>
+__device__ __forceinline__ void gather_stencil(const T* q, const Geom& g, const BCSpec& bc,
+                                               long long idx, int m, double (&v)[6]) {
+  const long long nx = (long long)(g.mx + 1) * g.my;
+  int c, r, dir;
+  if (idx < nx) {
+    c = 1 + (int)(idx % (g.mx + 1)); r = 1 + (int)(idx / (g.mx + 1)); dir = 0;
+  } else {
+    long long k = idx - nx;
+    c = 1 + (int)(k % g.mx); r = 1 + (int)(k / g.mx); dir = 1;
+  }
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int cc = dir == 0 ? c - 3 + s : c;
+    const int rr = dir == 0 ? r : r - 3 + s;
+    bool fx = false, fy = false;
+    int ci = bc_map(cc, g.mx, 3, bc.lo[0], bc.hi[0], fx);
+    int ri = bc_map(rr, g.my, 3, bc.lo[1], bc.hi[1], fy);
+    double x = (double)q[(long long)m * g.fs + (ci - 1) + (long long)(ri - 1) * g.rs];
+    if ((fx && m == bc.refl[0]) != (fy && m == bc.refl[1])) x = -x;
+    v[s] = x;
+  }
+}
+
+// Features and labels of every (interface, component) sample of a state.
+template <typename T>
+__global__ void k_features_labels(const T* q, Geom g, BCSpec bc, int meqn, float flat,
+                                  float wtol, float* feat, unsigned char* label) {
+  const long long nint = (long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1);
+  const long long n = nint * meqn;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int m = (int)(i % meqn);
+  const long long idx = i / meqn;
+  double v[6];
+  gather_stencil(q, g, bc, idx, m, v);
+  float f[6];
+  float rng = norm_stencil(v, flat, f);
+#pragma unroll
+  for (int k = 0; k < 6; k++) feat[i * 8 + k] = f[k];
+  feat[i * 8 + 6] = 1.0f;
+  feat[i * 8 + 7] = 0.0f;
+  label[i] = (rng > flat && weno_departure(v) > (double)wtol) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- MLP forward
+// One wavefront evaluates 16 samples.  X: 16 rows x NF (bf16) in LDS; returns
+// the logit for rows 4*(lane/16)+a in z3[a] (identical across the 16 lanes of a
+// row group), and leaves Z1, Z2 (pre-activation, FP32, D layout) for backward.
+struct Fwd {
+  f32x4 z1[2], z2[2];
+  float z3[4];
+};
+
+__device__ __forceinline__ bf16x4 load_a(const short* base, int row, int k0, int ld) {
+  // A operand: row `row`, k0..k0+3 from a row-major [row][k] bf16 matrix
+  bf16x4 a;
+  a.x = base[row * ld + k0 + 0]; a.y = base[row * ld + k0 + 1];
+  a.z = base[row * ld + k0 + 2]; a.w = base[row * ld + k0 + 3];
+  return a;
+}
+__device__ __forceinline__ bf16x4 load_b(const short* base, int k0, int col, int ld) {
+  // B operand: k0..k0+3 of column `col` from a row-major [k][col] bf16 matrix
+  bf16x4 b;
+  b.x = base[(k0 + 0) * ld + col]; b.y = base[(k0 + 1) * ld + col];
+  b.z = base[(k0 + 2) * ld + col]; b.w = base[(k0 + 3) * ld + col];
+  return b;
+}
+
+// P: bf16 parameters in LDS (layout P_*); sX: [16][NF] bf16; sH: [16][NH] bf16 scratch
+__device__ __forceinline__ void mlp_forward(const short* P, const short* sX, short* sH, Fwd& o) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, kq = 4 * (lane >> 4);
+  // layer 1: Z1 = X W1 (bias row folded in)
+#pragma unroll
+  for (int nb = 0; nb < 2; nb++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    bf16x4 a = load_a(sX, col, kq, NF);
+    bf16x4 b = load_b(P + P_W1, kq, nb * 16 + col, NH);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    o.z1[nb] = acc;
+#pragma unroll
+    for (int e = 0; e < 4; e++) sH[(kq + e) * NH + nb * 16 + col] = f2bf(fmaxf(acc[e], 0.f));
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  // layer 2: Z2 = H1 W2 + b2
+#pragma unroll
+  for (int nb = 0; nb < 2; nb++) {
+    const float bias = bf2f(P[P_B2 + nb * 16 + col]);
+    f32x4 acc = {bias, bias, bias, bias};
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) {
+      bf16x4 a = load_a(sH, col, kb * 16 + kq, NH);
+      bf16x4 b = load_b(P + P_W2, kb * 16 + kq, nb * 16 + col, NH);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    o.z2[nb] = acc;
+  }
+  // layer 3: z3 = relu(Z2) . w3 + b3  (rows 4*(lane/16)+e, reduce over 16 lanes x 2 blocks)
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    float s = fmaxf(o.z2[0][e], 0.f) * bf2f(P[P_W3 + col]) +
+              fmaxf(o.z2[1][e], 0.f) * bf2f(P[P_W3 + 16 + col]);
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+    o.z3[e] = s + bf2f(P[P_B3]);
+  }
+}
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.f / (1.f + __expf(-z)); }
+
+// ---------------------------------------------------------------- training
+// Each wavefront takes tiles of 16 samples; gradients (FP32) are summed per
+// wavefront and added to `grad` (P_N floats) with atomics.  Loss: weighted
+// binary cross-entropy, mean over the batch.
+__global__ void __launch_bounds__(256) k_mlp_train(const float* feat, long long nsamples,
+                                                   const unsigned char* label,
+                                                   const int* batch_idx, int nbatch,
+                                                   const float* master, float pos_weight,
+                                                   float* grad, float* loss_out) {
+  __shared__ short sP[P_N + 1];
+  __shared__ short sX[4][16 * NF];
+  __shared__ short sH[4][16 * NH];
+  __shared__ short sD[4][16 * NH];
+  __shared__ float sG[P_N];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < P_N; i += 256) { sP[i] = f2bf(master[i]); sG[i] = 0.f; }
+  __syncthreads();
+  const int col = lane & 15, kq = 4 * (lane >> 4);
+  const int ntiles = (nbatch + 15) / 16;
+  const float invb = 1.f / (float)nbatch;
+  float loss_acc = 0.f;
+  // per-wave gradient accumulators in D layout
+  f32x4 gW2[2][2], gW1[2];
+  for (int a = 0; a < 2; a++) { gW1[a] = f32x4{0, 0, 0, 0}; for (int b = 0; b < 2; b++) gW2[a][b] = f32x4{0, 0, 0, 0}; }
+  float gb2[2] = {0.f, 0.f}, gw3[2] = {0.f, 0.f}, gb3 = 0.f;
+  for (int tile = blockIdx.x * 4 + wv; tile < ntiles; tile += gridDim.x * 4) {
+    // stage X (bf16); batch entries outside [0, nsamples) count as padding
+    for (int e = lane; e < 16 * NF; e += 64) {
+      int r = e / NF, k = e % NF;
+      int s = tile * 16 + r;
+      float x = 0.f;
+      if (s < nbatch && k < 8) {
+        const int bi = batch_idx[s];
+        if (bi >= 0 && bi < nsamples) x = feat[(long long)bi * 8 + k];
+      }
+      sX[wv][r * NF + k] = f2bf(x);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    Fwd f;
+    mlp_forward(sP, sX[wv], sH[wv], f);
+    // dz3 for rows kq+e
+    float dz3[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int s = tile * 16 + kq + e;
+      const int bi = s < nbatch ? batch_idx[s] : -1;
+      const bool valid = bi >= 0 && bi < nsamples;
+      float y = valid ? (float)label[bi] : 0.f;
+      float p = sigmoidf(f.z3[e]);
+      float w = pos_weight * y + (1.f - y);
+      dz3[e] = valid ? (p * w - pos_weight * y) * invb : 0.f;
+      if (valid && col == 0) {
+        float pc = fminf(fmaxf(p, 1e-7f), 1.f - 1e-7f);
+        loss_acc += -(pos_weight * y * __logf(pc) + (1.f - y) * __logf(1.f - pc)) * invb;
+      }
+      if (col == 0) gb3 += dz3[e];
+    }
+    // layer 3 grads and dZ2 = dz3 * w3 * [Z2 > 0]  (D layout), staged to sD [row][j]
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+      const float w3 = bf2f(sP[P_W3 + nb * 16 + col]);
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        float a2 = fmaxf(f.z2[nb][e], 0.f);
+        gw3[nb] += a2 * dz3[e];
+        float dz2 = f.z2[nb][e] > 0.f ? dz3[e] * w3 : 0.f;
+        gb2[nb] += dz2;
+        sD[wv][(kq + e) * NH + nb * 16 + col] = f2bf(dz2);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // dW2[k][j] += sum_s H1[s][k] dZ2[s][j]  (M = k, K = s, N = j)
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+      for (int nb = 0; nb < 2; nb++) {
+        bf16x4 a, b;
+        // A = H1^T: row k = mb*16+col, samples kq..kq+3 -> H1[s][k]
+        a.x = sH[wv][(kq + 0) * NH + mb * 16 + col]; a.y = sH[wv][(kq + 1) * NH + mb * 16 + col];
+        a.z = sH[wv][(kq + 2) * NH + mb * 16 + col]; a.w = sH[wv][(kq + 3) * NH + mb * 16 + col];
+        b = load_b(sD[wv], kq, nb * 16 + col, NH);
+        gW2[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, gW2[mb][nb], 0, 0, 0);
+      }
+    // dA1[s][k] = sum_j dZ2[s][j] W2[k][j] ; dZ1 = dA1 * [Z1 > 0]
+    f32x4 dA1[2];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {   // nb: block of k
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++) {  // kb: block of j
+        bf16x4 a = load_a(sD[wv], col, kb * 16 + kq, NH);
+        bf16x4 b;   // B[j][k] = W2[k][j]
+        const int k = nb * 16 + col;
+        b.x = sP[P_W2 + k * NH + kb * 16 + kq + 0]; b.y = sP[P_W2 + k * NH + kb * 16 + kq + 1];
+        b.z = sP[P_W2 + k * NH + kb * 16 + kq + 2]; b.w = sP[P_W2 + k * NH + kb * 16 + kq + 3];
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      }
+      dA1[nb] = acc;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        float dz1 = f.z1[nb][e] > 0.f ? dA1[nb][e] : 0.f;
+        sD[wv][(kq + e) * NH + nb * 16 + col] = f2bf(dz1);
+      }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    // dW1[f][k] += sum_s X[s][f] dZ1[s][k]  (M = f (16), K = s, N = k)
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++) {
+      bf16x4 a;
+      a.x = sX[wv][(kq + 0) * NF + col]; a.y = sX[wv][(kq + 1) * NF + col];
+      a.z = sX[wv][(kq + 2) * NF + col]; a.w = sX[wv][(kq + 3) * NF + col];
+      bf16x4 b = load_b(sD[wv], kq, nb * 16 + col, NH);
+      gW1[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, gW1[nb], 0, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // reduce the wave's accumulators into LDS, then global
+#pragma unroll
+  for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        atomicAdd(&sG[P_W2 + (mb * 16 + kq + e) * NH + nb * 16 + col], gW2[mb][nb][e]);
+#pragma unroll
+  for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) atomicAdd(&sG[P_W1 + (kq + e) * NH + nb * 16 + col], gW1[nb][e]);
+#pragma unroll
+  for (int nb = 0; nb < 2; nb++) {
+    atomicAdd(&sG[P_B2 + nb * 16 + col], gb2[nb]);
+    atomicAdd(&sG[P_W3 + nb * 16 + col], gw3[nb]);
+  }
+  atomicAdd(&sG[P_B3], gb3);
+  __syncthreads();
+  for (int i = tid; i < P_N; i += 256) atomicAdd(&grad[i], sG[i]);
+  // loss (lanes with col == 0 hold partial sums)
+  for (int off = 32; off > 0; off >>= 1) loss_acc += __shfl_xor(loss_acc, off, 64);
+  if (lane == 0) atomicAdd(loss_out, loss_acc);
+}
+
+// Adam on the FP32 master weights; clears the gradient buffer.  Rows 7..15 of W1
+// (padding) are kept at zero.
+__global__ void k_adam(float* master, float* grad, float* m1, float* m2, int step, float lr,
+                       float b1, float b2, float eps) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P_N) return;
+  float g = grad[i];
+  grad[i] = 0.f;
+  if (i < P_W2 && (i / NH) >= 7) { master[i] = 0.f; return; }
+  float mm = b1 * m1[i] + (1.f - b1) * g;
+  float vv = b2 * m2[i] + (1.f - b2) * g * g;
+  m1[i] = mm;
+  m2[i] = vv;
+  float mh = mm / (1.f - powf(b1, (float)step));
+  float vh = vv / (1.f - powf(b2, (float)step));
+  master[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// ---------------------------------------------------------------- inference
+// Logits of a list of samples (for tests and evaluation).
+__global__ void __launch_bounds__(256) k_mlp_logits(const float* feat, long long n,
+                                                    const float* master, float* logits) {
+  __shared__ short sP[P_N + 1];
+  __shared__ short sX[4][16 * NF];
+  __shared__ short sH[4][16 * NH];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < P_N; i += 256) sP[i] = f2bf(master[i]);
+  __syncthreads();
+  const long long ntiles = (n + 15) / 16;
+  const int col = lane & 15, kq = 4 * (lane >> 4);
+  for (long long tile = (long long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long long)gridDim.x * 4) {
+    for (int e = lane; e < 16 * NF; e += 64) {
+      int r = e / NF, k = e % NF;
+      long long s = tile * 16 + r;
+      sX[wv][r * NF + k] = f2bf((s < n && k < 8) ? feat[s * 8 + k] : 0.f);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    Fwd f;
+    mlp_forward(sP, sX[wv], sH[wv], f);
+    if (col == 0) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        long long s = tile * 16 + kq + e;
+        if (s < n) logits[s] = f.z3[e];
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Interface flags of a state: flag = 1 when any component whose stencil range
+// exceeds `flat` has a logit above `zthr` (troubled -> full WENO).  Flags: x-interfaces (mx+1)*my, then
+// y-interfaces mx*(my+1), row-major as in gather_stencil.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc, int meqn,
+                                                   float flat, const float* master, float zthr,
+                                                   unsigned char* flags) {
+  __shared__ short sP[P_N + 1];
+  __shared__ short sX[4][16 * NF];
+  __shared__ short sH[4][16 * NH];
+  __shared__ unsigned char sRough[4][16];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < P_N; i += 256) sP[i] = f2bf(master[i]);
+  __syncthreads();
+  const long long nint = (long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1);
+  const long long ntiles = (nint + 15) / 16;
+  const int col = lane & 15, kq = 4 * (lane >> 4);
+  for (long long tile = (long long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long long)gridDim.x * 4) {
+    unsigned char any[4] = {0, 0, 0, 0};
+    for (int m = 0; m < meqn; m++) {
+      // lanes 0..15 build one sample row each
+      if (lane < 16) {
+        long long idx = tile * 16 + lane;
+        float f[6] = {0, 0, 0, 0, 0, 0};
+        unsigned char rough = 0;
+        if (idx < nint) {
+          double v[6];
+          gather_stencil(q, g, bc, idx, m, v);
+          rough = norm_stencil(v, flat, f) > flat ? 1 : 0;
+        }
+        sRough[wv][lane] = rough;
+#pragma unroll
+        for (int k = 0; k < 6; k++) sX[wv][lane * NF + k] = f2bf(f[k]);
+        sX[wv][lane * NF + 6] = f2bf(1.f);
+#pragma unroll
+        for (int k = 7; k < NF; k++) sX[wv][lane * NF + k] = 0;
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+      Fwd f;
+      mlp_forward(sP, sX[wv], sH[wv], f);
+#pragma unroll
+      for (int e = 0; e < 4; e++) any[e] |= (sRough[wv][kq + e] && f.z3[e] > zthr) ? 1 : 0;
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (col == 0) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        long long idx = tile * 16 + kq + e;
+        if (idx < nint) flags[idx] = any[e];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+int launch_features_labels(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
+                           const int* refl, int meqn, float flat, float wtol, float* feat,
+                           unsigned char* label, hipStream_t s) {
+  BCSpec bc = to_bc(lo, hi, refl, 2);
+  long long n = ((long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1)) * meqn;
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (dtype == CLAWHIP_F64)
+    hipLaunchKernelGGL(k_features_labels<double>, dim3(blocks), dim3(256), 0, s, (const double*)q,
+                       g, bc, meqn, flat, wtol, feat, label);
+  else
+    hipLaunchKernelGGL(k_features_labels<float>, dim3(blocks), dim3(256), 0, s, (const float*)q,
+                       g, bc, meqn, flat, wtol, feat, label);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int launch_mlp_train(const float* feat, long long nsamples, const unsigned char* label,
+                     const int* batch_idx, int nbatch, float* master, float* grad, float* m1,
+                     float* m2, int step, float lr, float pos_weight, float* loss, hipStream_t s) {
+  int tiles = (nbatch + 15) / 16;
+  int blocks = (tiles + 3) / 4;
+  if (blocks > 256) blocks = 256;
+  hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, s, feat, nsamples, label,
+                     batch_idx, nbatch, master, pos_weight, grad, loss);
+  if (step > 0)
+    hipLaunchKernelGGL(k_adam, dim3((P_N + 255) / 256), dim3(256), 0, s, master, grad, m1, m2,
+                       step, lr, 0.9f, 0.999f, 1e-8f);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int launch_mlp_logits(const float* feat, long long n, const float* master, float* logits,
+                      hipStream_t s) {
+  long long tiles = (n + 15) / 16;
+  long long blocks = (tiles + 3) / 4;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_mlp_logits, dim3((unsigned)blocks), dim3(256), 0, s, feat, n, master,
+                     logits);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
+                     const int* refl, int meqn, float flat, const float* master, float zthr,
+                     unsigned char* flags, hipStream_t s) {
+  BCSpec bc = to_bc(lo, hi, refl, 2);
+  long long nint = (long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1);
+  long long tiles = (nint + 15) / 16;
+  long long blocks = (tiles + 3) / 4;
+  if (blocks > 8192) blocks = 8192;
+  if (dtype == CLAWHIP_F64)
+    hipLaunchKernelGGL(k_mlp_flags<double>, dim3((unsigned)blocks), dim3(256), 0, s,
+                       (const double*)q, g, bc, meqn, flat, master, zthr, flags);
+  else
+    hipLaunchKernelGGL(k_mlp_flags<float>, dim3((unsigned)blocks), dim3(256), 0, s,
+                       (const float*)q, g, bc, meqn, flat, master, zthr, flags);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+}  // namespace claw

@@ -4,6 +4,7 @@
 
 #include "../../include/clawhip.h"
 #include "common.hpp"
+#include "stepctl.hpp"
 
 namespace claw {
 
@@ -44,5 +45,30 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
                   void* work, void* cwork, const int* lo, const int* hi, const int* refl,
                   int ng, const int* method, const int* mthlim, double dx, double dy,
                   double dz, double dt, unsigned long long* cfl, hipStream_t s);
+
+int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
+                       const int* lo, const int* hi, const int* refl, int ymode, int omode,
+                       double dt, double s, double dx, double dy, const void* P, const void* Q,
+                       const void* A, const void* B, const void* aux_in, const void* comp_in,
+                       void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
+                       const unsigned char* flags, hipStream_t stream);
+int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream);
+
+int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int* lo,
+                       const int* hi, const int* refl, int ng, const int* method,
+                       const int* mthlim, StepCtl* ctl, const CtlBuffers& bufs, int nsteps,
+                       hipStream_t stream);
+
+int launch_features_labels(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
+                           const int* refl, int meqn, float flat, float wtol, float* feat,
+                           unsigned char* label, hipStream_t s);
+int launch_mlp_train(const float* feat, long long nsamples, const unsigned char* label,
+                     const int* batch_idx, int nbatch, float* master, float* grad, float* m1,
+                     float* m2, int step, float lr, float pos_weight, float* loss, hipStream_t s);
+int launch_mlp_logits(const float* feat, long long n, const float* master, float* logits,
+                      hipStream_t s);
+int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
+                     const int* refl, int meqn, float flat, const float* master, float zthr,
+                     unsigned char* flags, hipStream_t s);
 
 }  // namespace claw

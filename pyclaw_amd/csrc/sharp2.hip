@@ -1,0 +1,574 @@
+// sharp2.hip -- SharpClaw 2D semi-discrete operator with fused SSP-RK stage
+// combination, gfx950.
+//
+// Follows sharpclaw/flux2.f90:2-74 and flux1.f90:76-268 (lim_type = 2,
+// tfluct_solver = False): per slice, WENO5 reconstruction -- component-wise
+// (weno.f90:5-83) or characteristic-wise (reconstruct.f90:201-299) --, the
+// Riemann solve at every interface, the CFL, the Riemann solve inside every
+// cell between its two reconstructed edge values, and
+//   dq1d(i) = -dtdx*(amdq(i+1) + apdq(i) + amdq2(i) + apdq2(i)).
+// The x-slices' and y-slices' contributions are summed as flux2.f90 does
+// (dq = 0 + dqx + dqy).  The x-slices cover rows 0..my+1 and the y-slices
+// columns 0..mx+1; their ghost-row / ghost-column parts only enter the CFL.
+//
+// One workgroup owns a TX x TY tile; the input state plus a 3-cell halo (ghost
+// values from the boundary map) is staged in LDS; interface results for one
+// direction live in LDS, per-cell x results in registers.  The stage combination
+// of SharpClawSolver.step (sharpclaw/solver.py:312-323, 400-430) is applied when
+// the tile's dq is complete, so each RK stage is a single pass over HBM.
+#include "launch.hpp"
+
+namespace claw {
+
+// The WENO epsilon.  FP64: the reference's REAL(4) literal 1.e-36 widened to
+// double (reconstruct.f90:11, weno.f90:52-57).  FP32: (1e-36)^2 underflows to 0,
+// and flat data would give 0/0, so the FP32 path uses 1e-18 (square 1e-36 is a
+// normal float); the weights of any stencil whose smoothness indicators exceed
+// ~1e-18 are unchanged by this.
+template <typename T>
+__device__ __forceinline__ T weno_eps() { return sizeof(T) == 8 ? T((double)1.0e-36f) : T(1.0e-18f); }
+
+// ---------------------------------------------------------------- stage modes
+enum StageOut {
+  OUT_DQDT = 0,      // out = D / dt               (dq_dt = dq(state)/dt, solver.py:293)
+  OUT_SSP33_2 = 1,   // out = 0.75*A + 0.25*(Y + D)          (solver.py:318)
+  OUT_SSP33_3 = 2,   // out = 1/3*A + 2/3*(Y + D)            (solver.py:323)
+  OUT_ADD6 = 3,      // out = Y + D/6.                       (solver.py:415, 425)
+  OUT_SSP104_FIN = 4,// out = A + (0.6*Y + 0.1*D)            (solver.py:430)
+  OUT_F32_S2 = 5,    // FP32: out = A + 0.25*(e1 + D), aux = (e1 + D)/6, e1 = s*B
+  OUT_F32_S3 = 6,    // FP32: Kahan(A, aux + 2/3*D, comp)
+  OUT_D = 7          // out = D (the operator itself, fmod.flux2's dq)
+};
+enum StageY {
+  Y_ARRAY = 0,       // Y = P
+  Y_AXPY = 1,        // Y = P + s*Q          (r = q + dt*dq_dt, solver.py:313)
+  Y_AXPY6 = 2        // Y = P + (s*Q)/6.     (s1 = q + dt*dq_dt/6., solver.py:409)
+};
+
+template <typename T>
+struct SharpArgs {
+  RPParams<T> rp;
+  Geom g;               // all arrays: compact, this geometry
+  BCSpec bc;
+  int ng;
+  int char_decomp;
+  int ymode, omode;
+  T dt;                 // dt passed to the operator (dq = dt*L)
+  T s;                  // scalar of the Y / e1 axpy
+  T dtdx, dtdy;
+  int ntx, nty;
+  const T* P;           // Y source
+  const T* Q;           // axpy partner of P
+  const T* A;           // stage anchor (q at the start of the step)
+  const T* B;           // OUT_F32_S2: dq_dt
+  const T* aux_in;      // OUT_F32_S3: accumulated increment
+  const T* comp_in;     // OUT_F32_S3: Kahan compensation
+  T* out;
+  T* aux_out;
+  T* comp_out;
+  unsigned long long* cfl_bits;
+  const unsigned char* flags;   // troubled-interface flags (learned indicator) or null
+};
+
+constexpr int TX = 32, TY = 8, HALO = 3;
+constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
+
+// ---------------------------------------------------------------- WENO5
+// weno.f90:32-79 for one cell: ql = left edge (fs0), qr = right edge (fs1).
+// `v` holds cells i-2..i+2.
+template <typename T>
+__device__ __forceinline__ void weno5_cell(const T (&v)[5], T& fs0, T& fs1) {
+  const T qm2 = v[0], qm1 = v[1], q0 = v[2], qp1 = v[3], qp2 = v[4];
+  const T eps = weno_eps<T>();
+  T sigma0 = T(3.3333333333333333333333333333333333) * (q0 * q0) -
+             T(10.333333333333333333333333333333333) * q0 * qp1 +
+             T(3.6666666666666666666666666666666667) * q0 * qp2 +
+             T(8.3333333333333333333333333333333333) * (qp1 * qp1) -
+             T(6.3333333333333333333333333333333333) * qp1 * qp2 +
+             T(1.3333333333333333333333333333333333) * (qp2 * qp2);
+  T sigma1 = T(4.3333333333333333333333333333333333) * (q0 * q0) -
+             T(4.3333333333333333333333333333333333) * q0 * qp1 -
+             T(4.3333333333333333333333333333333333) * q0 * qm1 +
+             T(1.3333333333333333333333333333333333) * (qp1 * qp1) +
+             T(1.6666666666666666666666666666666667) * qp1 * qm1 +
+             T(1.3333333333333333333333333333333333) * (qm1 * qm1);
+  T sigma2 = T(3.3333333333333333333333333333333333) * (q0 * q0) -
+             T(10.333333333333333333333333333333333) * q0 * qm1 +
+             T(3.6666666666666666666666666666666667) * q0 * qm2 +
+             T(8.3333333333333333333333333333333333) * (qm1 * qm1) -
+             T(6.3333333333333333333333333333333333) * qm1 * qm2 +
+             T(1.3333333333333333333333333333333333) * (qm2 * qm2);
+  T d0 = eps + sigma0, d1 = eps + sigma1, d2 = eps + sigma2;
+  T omega0 = T(0.1) / (d0 * d0);
+  T omega1 = T(0.6) / (d1 * d1);
+  T omega2 = T(0.3) / (d2 * d2);
+  T omega3 = T(0.3) / (d0 * d0);
+  T omega4 = T(0.6) / (d1 * d1);
+  T omega5 = T(0.1) / (d2 * d2);
+  T fr0 = T(1.8333333333333333333333333333333333) * q0 -
+          T(1.1666666666666666666666666666666667) * qp1 +
+          T(0.33333333333333333333333333333333333) * qp2;
+  T fr1 = T(0.83333333333333333333333333333333333) * q0 -
+          T(0.16666666666666666666666666666666667) * qp1 +
+          T(0.33333333333333333333333333333333333) * qm1;
+  T fr2 = T(0.33333333333333333333333333333333333) * q0 +
+          T(0.83333333333333333333333333333333333) * qm1 -
+          T(0.16666666666666666666666666666666667) * qm2;
+  T fr3 = T(0.33333333333333333333333333333333333) * q0 +
+          T(0.83333333333333333333333333333333333) * qp1 -
+          T(0.16666666666666666666666666666666667) * qp2;
+  T fr4 = T(0.83333333333333333333333333333333333) * q0 +
+          T(0.33333333333333333333333333333333333) * qp1 -
+          T(0.16666666666666666666666666666666667) * qm1;
+  T fr5 = T(1.8333333333333333333333333333333333) * q0 -
+          T(1.1666666666666666666666666666666667) * qm1 +
+          T(0.33333333333333333333333333333333333) * qm2;
+  fs0 = (fr0 * omega0 + fr1 * omega1 + fr2 * omega2) / (omega0 + omega1 + omega2);
+  fs1 = (fr3 * omega3 + fr4 * omega4 + fr5 * omega5) / (omega3 + omega4 + omega5);
+}
+
+// reconstruct.f90:264-284 for one side: hh(-2..2) -> uu.
+template <typename T>
+__device__ __forceinline__ T weno_char_side(T hm2, T hm1, T h0, T hp1, T hp2, int im) {
+  // im = +1: (intwo, inone, ione) = (-2, -1, +1); im = -1: (2, 1, -1)
+  T a2 = im == 1 ? hm2 : hp2;   // hh(intwo)
+  T a1 = im == 1 ? hm1 : hp1;   // hh(inone)
+  T b1 = im == 1 ? hp1 : hm1;   // hh(ione)
+  const T eps = weno_eps<T>();
+  T fim = T(im);
+  T t1 = fim * (a2 - a1);
+  T t2 = fim * (a1 - h0);
+  T t3 = fim * (h0 - b1);
+  T e1 = a2 - T(3) * a1;
+  T e2 = a1 + h0;
+  T e3 = T(3) * h0 - b1;
+  T tt1 = T(13) * (t1 * t1) + T(3) * (e1 * e1);
+  T tt2 = T(13) * (t2 * t2) + T(3) * (e2 * e2);
+  T tt3 = T(13) * (t3 * t3) + T(3) * (e3 * e3);
+  tt1 = (eps + tt1) * (eps + tt1);
+  tt2 = (eps + tt2) * (eps + tt2);
+  tt3 = (eps + tt3) * (eps + tt3);
+  T s1 = tt2 * tt3;
+  T s2 = T(6) * tt1 * tt3;
+  T s3 = T(3) * tt1 * tt2;
+  T t0 = T(1) / (s1 + s2 + s3);
+  s1 = s1 * t0;
+  s3 = s3 * t0;
+  return (s1 * (t2 - t1) + (T(0.5) * s3 - T(0.25)) * (t3 - t2)) / T(3);
+}
+
+// Eigenvectors at the Roe average of (qL, qR) in direction IXY (the same
+// formulas as the oracle's evec).  evl[ip][m] = row ip of R^-1, evr[m][ip].
+template <class RS, int IXY, typename T>
+struct Evec;
+
+template <int IXY, typename T>
+struct Evec<Shallow<T>, IXY, T> {
+  __device__ static __forceinline__ void get(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&evl)[3][3], T (&evr)[3][3]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    T h = (qL[0] + qR[0]) * T(0.50);
+    T hsqrtl = sqrt(qL[0]), hsqrtr = sqrt(qR[0]);
+    T hsq2 = hsqrtl + hsqrtr;
+    T u = (qL[mu] / hsqrtl + qR[mu] / hsqrtr) / hsq2;
+    T v = (qL[mv] / hsqrtl + qR[mv] / hsqrtr) / hsq2;
+    T c = sqrt(p.grav * h);
+    evr[0][0] = T(1); evr[mu][0] = u - c; evr[mv][0] = v;
+    evr[0][1] = T(0); evr[mu][1] = T(0); evr[mv][1] = T(1);
+    evr[0][2] = T(1); evr[mu][2] = u + c; evr[mv][2] = v;
+    T r2c = T(0.5) / c;
+    evl[0][0] = (u + c) * r2c; evl[0][mu] = -r2c; evl[0][mv] = T(0);
+    evl[1][0] = -v; evl[1][mu] = T(0); evl[1][mv] = T(1);
+    evl[2][0] = -(u - c) * r2c; evl[2][mu] = r2c; evl[2][mv] = T(0);
+  }
+};
+
+template <int IXY, typename T>
+struct Evec<Euler<T>, IXY, T> {
+  __device__ static __forceinline__ void get(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&evl)[4][4], T (&evr)[4][4]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T gamma1 = p.gamma1;
+    auto r = Euler<T>::template roe_avg<IXY>(gamma1, qL, qR);
+    T u = r.u, v = r.v, c = r.a, H = r.enth;
+    T q2 = T(0.5) * r.u2v2;
+    evr[0][0] = T(1); evr[mu][0] = u - c; evr[mv][0] = v; evr[3][0] = H - u * c;
+    evr[0][1] = T(0); evr[mu][1] = T(0); evr[mv][1] = T(1); evr[3][1] = v;
+    evr[0][2] = T(1); evr[mu][2] = u; evr[mv][2] = v; evr[3][2] = q2;
+    evr[0][3] = T(1); evr[mu][3] = u + c; evr[mv][3] = v; evr[3][3] = H + u * c;
+    T b1 = gamma1 / (c * c), b2 = q2 * b1;
+    T r2c = T(0.5) / c;
+    evl[0][0] = T(0.5) * b2 + u * r2c; evl[0][mu] = -(T(0.5) * b1 * u + r2c);
+    evl[0][mv] = -T(0.5) * b1 * v; evl[0][3] = T(0.5) * b1;
+    evl[1][0] = -v; evl[1][mu] = T(0); evl[1][mv] = T(1); evl[1][3] = T(0);
+    evl[2][0] = T(1) - b2; evl[2][mu] = b1 * u; evl[2][mv] = b1 * v; evl[2][3] = -b1;
+    evl[3][0] = T(0.5) * b2 - u * r2c; evl[3][mu] = -(T(0.5) * b1 * u - r2c);
+    evl[3][mv] = -T(0.5) * b1 * v; evl[3][3] = T(0.5) * b1;
+  }
+};
+
+// WENO5 with its linear weights (0.1, 0.6, 0.3): the 5th-order upwind-biased
+// reconstructions of both edges, component-wise (with linear weights the
+// characteristic projection cancels).  st[0..5] = cells c-3..c+2.
+template <int ME, typename T>
+__device__ __forceinline__ void recon_linear(const T (&st)[6][ME], T (&qrL)[ME], T (&qlR)[ME]) {
+#pragma unroll
+  for (int m = 0; m < ME; m++) {
+    qrL[m] = (T(2) * st[0][m] - T(13) * st[1][m] + T(47) * st[2][m] + T(27) * st[3][m] -
+              T(3) * st[4][m]) * T(1. / 60.);
+    qlR[m] = (-T(3) * st[1][m] + T(27) * st[2][m] + T(47) * st[3][m] - T(13) * st[4][m] +
+              T(2) * st[5][m]) * T(1. / 60.);
+  }
+}
+
+// Reconstruct both sides of the interface between stencil cells 2 and 3 of
+// st[0..5] (cells c-3..c+2): qrL = right edge of cell c-1, qlR = left edge of c.
+template <class RS, int IXY, typename T, bool CHAR>
+__device__ __forceinline__ void recon_interface(const RPParams<T>& p,
+                                                const T (&st)[6][RS::meqn],
+                                                T (&qrL)[RS::meqn], T (&qlR)[RS::meqn]) {
+  constexpr int ME = RS::meqn;
+  if constexpr (!CHAR) {
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      T a[5] = {st[0][m], st[1][m], st[2][m], st[3][m], st[4][m]};
+      T b[5] = {st[1][m], st[2][m], st[3][m], st[4][m], st[5][m]};
+      T f0, f1;
+      weno5_cell(a, f0, f1);
+      qrL[m] = f1;
+      weno5_cell(b, f0, f1);
+      qlR[m] = f0;
+    }
+  } else {
+    T evl[ME][ME], evr[ME][ME];
+    Evec<RS, IXY, T>::get(p, st[2], st[3], evl, evr);
+    T dq[5][ME];   // dq(m, c+m2) = q(c+m2) - q(c+m2-1), m2 = -2..2
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+      for (int m = 0; m < ME; m++) dq[k][m] = st[k + 1][m] - st[k][m];
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      T lin = (-st[1][m] + T(7) * (st[2][m] + st[3][m]) - st[4][m]) / T(12);
+      qrL[m] = lin;
+      qlR[m] = lin;
+    }
+#pragma unroll
+    for (int ip = 0; ip < ME; ip++) {
+      T hh[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        hh[k] = T(0);
+#pragma unroll
+        for (int m = 0; m < ME; m++) hh[k] = hh[k] + evl[ip][m] * dq[k][m];
+      }
+      T uu1 = weno_char_side(hh[0], hh[1], hh[2], hh[3], hh[4], 1);
+      T uu2 = weno_char_side(hh[0], hh[1], hh[2], hh[3], hh[4], -1);
+#pragma unroll
+      for (int m = 0; m < ME; m++) {
+        qrL[m] = qrL[m] + evr[m][ip] * uu1;
+        qlR[m] = qlR[m] + evr[m][ip] * uu2;
+      }
+    }
+  }
+}
+
+template <class RS, typename T, bool CHAR, bool FLAGS>
+__global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
+  constexpr int ME = RS::meqn, MW = RS::mwaves;
+  __shared__ T sQ[ME][LY][LX];
+  __shared__ T sL[ME][TY + 2][TX + 1];   // ql(c)  for x; reused for y
+  __shared__ T sR[ME][TY + 2][TX + 1];   // qr(c-1)
+  __shared__ T sAM[ME][TY + 2][TX + 1];
+  __shared__ T sAP[ME][TY + 2][TX + 1];
+
+  const int tid = threadIdx.x;
+  const int bx = blockIdx.x % a.ntx, by = blockIdx.x / a.ntx;
+  const int c0 = 1 + bx * TX, j0 = 1 + by * TY;
+  const int mx = a.g.mx, my = a.g.my;
+  const RPParams<T>& rp = a.rp;
+
+  // ---- stage Y (with halo) into LDS
+  for (int e = tid; e < LX * LY; e += 256) {
+    const int lx = e % LX, ly = e / LX;
+    const int c = c0 - HALO + lx, r = j0 - HALO + ly;
+    T y[ME];
+    load_cell2<ME>(a.P, a.g, a.bc, a.ng, c, r, y);
+    if (a.ymode != Y_ARRAY) {
+      T qv[ME];
+      load_cell2<ME>(a.Q, a.g, a.bc, a.ng, c, r, qv);
+#pragma unroll
+      for (int m = 0; m < ME; m++)
+        y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
+    }
+#pragma unroll
+    for (int m = 0; m < ME; m++) sQ[m][ly][lx] = y[m];
+  }
+  __syncthreads();
+
+  T smax_x = T(0), smax_y = T(0);
+  // rows of x-slices handled by this tile: [j0, j0+TY) plus rows 0 / my+1 (CFL only)
+  const int xr_lo = j0 == 1 ? 0 : j0;
+  const int xr_hi = min(j0 + TY - 1, my) == my ? my + 1 : j0 + TY - 1;
+  const int nxr = xr_hi - xr_lo + 1;   // <= TY + 2
+  // ---- x-interfaces c in [c0, c0+TX], rows xr_lo..xr_hi
+  for (int e = tid; e < nxr * (TX + 1); e += 256) {
+    const int k = e % (TX + 1), rr = e / (TX + 1);
+    const int c = c0 + k, r = xr_lo + rr;
+    const int ly = r - j0 + HALO;
+    T st[6][ME];
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+#pragma unroll
+      for (int m = 0; m < ME; m++) st[s][m] = sQ[m][ly][k + s];
+    T qrL[ME], qlR[ME];
+    bool smooth = false;
+    if (FLAGS && c <= mx + 1 && r >= 1 && r <= my) {
+      // smooth when this interface and its two x-neighbours are unflagged (the
+      // flags come from the step's first stage; a wave moves < 1 cell a step)
+      const unsigned char* f = a.flags + (long long)(r - 1) * (mx + 1);
+      smooth = (f[c - 1] | f[max(c - 2, 0)] | f[min(c, mx)]) == 0;
+    }
+    if (smooth) recon_linear<ME>(st, qrL, qlR);
+    else recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
+    T W[MW][ME], sp[MW], am[ME], ap[ME];
+    typename RS::Roe roe;
+    RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
+    if (c <= mx + 1) {
+#pragma unroll
+      for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, fabs(sp[w]));
+    }
+    const int ri = r - j0 + 1;   // 0 .. TY+1
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      sL[m][ri][k] = qlR[m];
+      sR[m][ri][k] = qrL[m];
+      sAM[m][ri][k] = am[m];
+      sAP[m][ri][k] = ap[m];
+    }
+  }
+  __syncthreads();
+  // ---- per-cell x contribution (thread = cell)
+  const int tx = tid % TX, ty = tid / TX;
+  const int cc = c0 + tx, cr = j0 + ty;
+  T dqx[ME];
+  {
+    const int ri = ty + 1;
+    T ql[ME], qr[ME];
+#pragma unroll
+    for (int m = 0; m < ME; m++) { ql[m] = sL[m][ri][tx]; qr[m] = sR[m][ri][tx + 1]; }
+    T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+    typename RS::Roe roe;
+    RS::template rpn<1>(rp, ql, qr, W, sp, am2, ap2, roe);
+#pragma unroll
+    for (int m = 0; m < ME; m++)
+      dqx[m] = T(0) - a.dtdx * (((sAM[m][ri][tx + 1] + sAP[m][ri][tx]) + am2[m]) + ap2[m]);
+  }
+  __syncthreads();
+  // ---- y-interfaces r in [j0, j0+TY], columns xc_lo..xc_hi
+  const int yc_lo = c0 == 1 ? 0 : c0;
+  const int yc_hi = min(c0 + TX - 1, mx) == mx ? mx + 1 : c0 + TX - 1;
+  const int nyc = yc_hi - yc_lo + 1;   // <= TX + 2
+  for (int e = tid; e < nyc * (TY + 1); e += 256) {
+    const int kc = e % nyc, k = e / nyc;
+    const int c = yc_lo + kc, r = j0 + k;
+    const int lx = c - c0 + HALO;
+    T st[6][ME];
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+#pragma unroll
+      for (int m = 0; m < ME; m++) st[s][m] = sQ[m][k + s][lx];
+    T qrL[ME], qlR[ME];
+    bool smooth = false;
+    if (FLAGS && c >= 1 && c <= mx && r <= my + 1) {
+      const unsigned char* f = a.flags + (long long)(mx + 1) * my + (c - 1);
+      smooth = (f[(long long)(r - 1) * mx] | f[(long long)max(r - 2, 0) * mx] |
+                f[(long long)min(r, my) * mx]) == 0;
+    }
+    if (smooth) recon_linear<ME>(st, qrL, qlR);
+    else recon_interface<RS, 2, T, CHAR>(rp, st, qrL, qlR);
+    T W[MW][ME], sp[MW], am[ME], ap[ME];
+    typename RS::Roe roe;
+    RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
+    if (r <= my + 1) {
+#pragma unroll
+      for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, fabs(sp[w]));
+    }
+    if (c >= c0 && c < c0 + TX) {
+      const int ci = c - c0;   // stored transposed: [ci][k]
+#pragma unroll
+      for (int m = 0; m < ME; m++) {
+        // layout reuse: first index = column within tile (< TX), second = interface k
+        (&sL[m][0][0])[ci * (TY + 1) + k] = qlR[m];
+        (&sR[m][0][0])[ci * (TY + 1) + k] = qrL[m];
+        (&sAM[m][0][0])[ci * (TY + 1) + k] = am[m];
+        (&sAP[m][0][0])[ci * (TY + 1) + k] = ap[m];
+      }
+    }
+  }
+  __syncthreads();
+  T dqv[ME];
+  {
+    T ql[ME], qr[ME];
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      ql[m] = (&sL[m][0][0])[tx * (TY + 1) + ty];
+      qr[m] = (&sR[m][0][0])[tx * (TY + 1) + ty + 1];
+    }
+    T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+    typename RS::Roe roe;
+    RS::template rpn<2>(rp, ql, qr, W, sp, am2, ap2, roe);
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      T dqy = T(0) - a.dtdy * ((((&sAM[m][0][0])[tx * (TY + 1) + ty + 1] +
+                                  (&sAP[m][0][0])[tx * (TY + 1) + ty]) + am2[m]) + ap2[m]);
+      dqv[m] = (T(0) + dqx[m]) + dqy;
+    }
+  }
+  // ---- CFL
+  {
+    double cx = (double)a.dtdx * (double)smax_x;
+    double cy = (double)a.dtdy * (double)smax_y;
+    wave_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
+  }
+  // ---- stage combination and store
+  if (cc <= mx && cr <= my) {
+    const long long off = (long long)(cc - 1) + (long long)(cr - 1) * a.g.rs;
+    const long long fs = a.g.fs;
+    T y[ME];
+#pragma unroll
+    for (int m = 0; m < ME; m++) y[m] = sQ[m][ty + HALO][tx + HALO];
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      const T D = dqv[m];
+      T o;
+      switch (a.omode) {
+        case OUT_DQDT: o = D / a.dt; break;
+        case OUT_SSP33_2: o = T(0.75) * a.A[off + m * fs] + T(0.25) * (y[m] + D); break;
+        case OUT_SSP33_3:
+          o = T(1. / 3.) * a.A[off + m * fs] + T(2. / 3.) * (y[m] + D);
+          break;
+        case OUT_ADD6: o = y[m] + D / T(6); break;
+        case OUT_SSP104_FIN: o = a.A[off + m * fs] + (T(0.6) * y[m] + T(0.1) * D); break;
+        case OUT_D: o = D; break;
+        case OUT_F32_S2: {
+          T e1 = a.s * a.B[off + m * fs];
+          T inc = e1 + D;
+          o = a.A[off + m * fs] + T(0.25) * inc;
+          a.aux_out[off + m * fs] = inc * T(1. / 6.);
+          break;
+        }
+        default: {   // OUT_F32_S3
+          T delta = a.aux_in[off + m * fs] + T(2. / 3.) * D;
+          T qa = a.A[off + m * fs];
+          T yk = delta - a.comp_in[off + m * fs];
+          T tq = qa + yk;
+          a.comp_out[off + m * fs] = (tq - qa) - yk;
+          o = tq;
+          break;
+        }
+      }
+      a.out[off + m * fs] = o;
+    }
+  }
+}
+
+// elementwise SSP104 middle update (solver.py:418-419):
+//   q = q/25. + 0.36*s1 ;  s1 = 15.*q - 5.*s1
+template <typename T>
+__global__ void k_ssp104_mid(T* q, T* s1, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T qn = q[i] / T(25) + T(0.36) * s1[i];
+  q[i] = qn;
+  s1[i] = T(15) * qn - T(5) * s1[i];
+}
+
+// ---------------------------------------------------------------- launchers
+namespace {
+template <class RS, typename T, bool CHAR>
+int run_stage(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
+              const int* refl, int ymode, int omode, double dt, double s, double dx, double dy,
+              const void* P, const void* Q, const void* A, const void* B, const void* aux_in,
+              const void* comp_in, void* out, void* aux_out, void* comp_out,
+              unsigned long long* cfl, const unsigned char* flags, hipStream_t stream) {
+  SharpArgs<T> a;
+  a.rp = to_params<T>(rp);
+  a.g = g;
+  a.bc = to_bc(lo, hi, refl, 2);
+  a.ng = 3;
+  a.char_decomp = CHAR ? 2 : 0;
+  a.ymode = ymode; a.omode = omode;
+  a.dt = T(dt); a.s = T(s);
+  a.dtdx = T(dt / dx); a.dtdy = T(dt / dy);
+  a.ntx = (g.mx + TX - 1) / TX; a.nty = (g.my + TY - 1) / TY;
+  a.P = (const T*)P; a.Q = (const T*)Q; a.A = (const T*)A; a.B = (const T*)B;
+  a.aux_in = (const T*)aux_in; a.comp_in = (const T*)comp_in;
+  a.out = (T*)out; a.aux_out = (T*)aux_out; a.comp_out = (T*)comp_out;
+  a.cfl_bits = cfl;
+  a.flags = flags;
+  if (flags)
+    hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, true>), dim3(a.ntx * a.nty), dim3(256), 0,
+                       stream, a);
+  else
+    hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, false>), dim3(a.ntx * a.nty), dim3(256), 0,
+                       stream, a);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+template <template <typename> class RSF, bool CHAR>
+int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
+                   const int* refl, int ymode, int omode, double dt, double s, double dx,
+                   double dy, const void* P, const void* Q, const void* A, const void* B,
+                   const void* aux_in, const void* comp_in, void* out, void* aux_out,
+                   void* comp_out, unsigned long long* cfl, const unsigned char* flags,
+                   hipStream_t stream) {
+  if (dtype == CLAWHIP_F64)
+    return run_stage<RSF<double>, double, CHAR>(rp, g, lo, hi, refl, ymode, omode, dt, s, dx,
+                                                dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
+                                                comp_out, cfl, flags, stream);
+  return run_stage<RSF<float>, float, CHAR>(rp, g, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
+                                            Q, A, B, aux_in, comp_in, out, aux_out, comp_out,
+                                            cfl, flags, stream);
+}
+}  // namespace
+
+int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
+                       const int* lo, const int* hi, const int* refl, int ymode, int omode,
+                       double dt, double s, double dx, double dy, const void* P, const void* Q,
+                       const void* A, const void* B, const void* aux_in, const void* comp_in,
+                       void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
+                       const unsigned char* flags, hipStream_t stream) {
+#define ARGS dtype, rp, g, lo, hi, refl, ymode, omode, dt, s, dx, dy, P, Q, A, B, aux_in, \
+             comp_in, out, aux_out, comp_out, cfl, flags, stream
+  if (char_decomp == 2) {
+    switch (rp.kind) {
+      case CLAWHIP_RP_SHALLOW: return dispatch_stage<Shallow, true>(ARGS);
+      case CLAWHIP_RP_EULER: return dispatch_stage<Euler, true>(ARGS);
+    }
+    return CLAWHIP_E_KIND;
+  }
+  switch (rp.kind) {
+    case CLAWHIP_RP_SHALLOW: return dispatch_stage<Shallow, false>(ARGS);
+    case CLAWHIP_RP_EULER: return dispatch_stage<Euler, false>(ARGS);
+    case CLAWHIP_RP_ACOUSTICS: return dispatch_stage<Acoustics, false>(ARGS);
+    case CLAWHIP_RP_ADVECTION: return dispatch_stage<Advection, false>(ARGS);
+  }
+#undef ARGS
+  return CLAWHIP_E_KIND;
+}
+
+int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream) {
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (dtype == CLAWHIP_F64)
+    hipLaunchKernelGGL(k_ssp104_mid<double>, dim3(blocks), dim3(256), 0, stream, (double*)q,
+                       (double*)s1, n);
+  else
+    hipLaunchKernelGGL(k_ssp104_mid<float>, dim3(blocks), dim3(256), 0, stream, (float*)q,
+                       (float*)s1, n);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+}  // namespace claw

@@ -1,0 +1,233 @@
+"""Learned troubled-interface indicator for SharpClaw's WENO5 path.
+
+A 6 -> 32 -> 32 -> 1 MLP (BF16 MFMA, FP32 accumulation; pyclaw_amd/csrc/
+indicator.hip) looks at the min-max normalised 6-cell WENO5 stencil of every
+(interface, conserved component) and predicts whether the WENO5 weights of that
+component depart from the linear weights.  Interfaces with no flagged component
+(and no flagged neighbour along the normal) are reconstructed with WENO5's
+linear 5th-order weights; flagged ones keep the full characteristic WENO5 of
+the reference (sharpclaw/reconstruct.f90:201-299), which stays the judge of
+every flagged interface.  There is no counterpart in the reference.
+
+Training runs on the GPU (clawhip_mlp_train_step): labels come from full-WENO
+SharpClaw runs of seeded shallow-water problems (generate_training_data).
+"""
+import os
+
+import numpy as np
+import torch
+
+from . import kernels as K
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "indicator_swe.npy")
+
+
+def init_params(seed=0):
+    """He-style initialisation in the kernel's parameter layout; W1 rows 7..15
+    (padding) and the biases start at zero."""
+    nf, nh = 16, 32
+    rng = np.random.default_rng(seed)
+    p = np.zeros(K.mlp_param_count(), np.float32)
+    w1 = rng.normal(0, np.sqrt(2.0 / 7.0), (nf, nh)).astype(np.float32)
+    w1[6:] = 0.0
+    p[:nf * nh] = w1.ravel()
+    p[nf * nh:nf * nh + nh * nh] = rng.normal(0, np.sqrt(2.0 / nh), nh * nh)
+    o = nf * nh + nh * nh + nh
+    p[o:o + nh] = rng.normal(0, np.sqrt(1.0 / nh), nh)
+    return p
+
+
+class TroubledInterfaceIndicator(object):
+    """Per-interface WENO5 selector.
+
+    params: FP32 parameter vector (clawhip_mlp_param_count floats) or None for
+    the shipped shallow-water weights.  threshold: logit above which a component
+    is troubled.  flat: stencils whose range is at most `flat` are never troubled.
+    """
+
+    def __init__(self, params=None, threshold=None, flat=1e-6, device="cuda"):
+        meta = {}
+        if params is None:
+            params, meta = load(DEFAULT_WEIGHTS)
+        self.params = torch.as_tensor(np.asarray(params, np.float32), device=device).contiguous()
+        self.threshold = float(meta.get("threshold", 0.0) if threshold is None else threshold)
+        self.flat = float(meta.get("flat", flat) if "flat" in meta else flat)
+        self._flags = None
+        self.last_fraction = None
+
+    def flags(self, q, bc_lower, bc_upper, reflect=(1, 2)):
+        """uint8 flags of every interface of the planar state q (meqn, my, mx)."""
+        meqn, my, mx = q.shape
+        n = K.mlp_num_interfaces(mx, my)
+        if self._flags is None or self._flags.numel() != n or self._flags.device != q.device:
+            self._flags = torch.empty(n, dtype=torch.uint8, device=q.device)
+        K.mlp_flags(q, bc_lower, bc_upper, reflect, self.flat, self.params, self.threshold,
+                    self._flags)
+        return self._flags
+
+    def troubled_fraction(self):
+        """Fraction of interfaces flagged by the last flags() call (syncs)."""
+        return float(self._flags.float().mean()) if self._flags is not None else None
+
+
+def save(path, params, threshold, flat):
+    """Weights + (threshold, flat) as one float32 .npy (no pickle)."""
+    arr = np.concatenate([np.asarray(params, np.float32), np.float32([threshold, flat])])
+    np.save(path, arr, allow_pickle=False)
+
+
+def load(path):
+    arr = np.load(path, allow_pickle=False)
+    n = arr.size - 2
+    return arr[:n].astype(np.float32), {"threshold": float(arr[n]), "flat": float(arr[n + 1])}
+
+
+# ---------------------------------------------------------------- training
+def features(q, bc_lower, bc_upper, flat, wtol, reflect=(1, 2)):
+    """(feat (n, 8) float32, label (n,) uint8) tensors of a device state."""
+    meqn, my, mx = q.shape
+    n = K.mlp_num_interfaces(mx, my) * meqn
+    feat = torch.empty((n, 8), dtype=torch.float32, device=q.device)
+    label = torch.empty(n, dtype=torch.uint8, device=q.device)
+    K.mlp_features(q, bc_lower, bc_upper, reflect, flat, wtol, feat, label)
+    return feat, label
+
+
+def train(feat, label, params=None, steps=2000, batch=4096, lr=3e-3, pos_weight=4.0, seed=0,
+          log_every=0):
+    """Adam on the device; returns (params tensor, loss history)."""
+    dev = feat.device
+    n = label.numel()
+    if params is None:
+        params = init_params(seed)
+    P = torch.as_tensor(np.asarray(params, np.float32), device=dev).clone()
+    grad = torch.zeros_like(P)
+    m1 = torch.zeros_like(P)
+    m2 = torch.zeros_like(P)
+    loss = torch.zeros(steps, dtype=torch.float32, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    for step in range(1, steps + 1):
+        idx = torch.randint(0, n, (batch,), device=dev, dtype=torch.int32, generator=gen)
+        K.mlp_train_step(feat, label, idx, P, grad, m1, m2, step, lr, pos_weight,
+                         loss[step - 1:step])
+        if log_every and step % log_every == 0:
+            print(f"step {step} loss {loss[step - 1].item():.4f}", flush=True)
+    return P, loss.cpu().numpy()
+
+
+def logits(feat, params):
+    n = feat.shape[0]
+    out = torch.empty(n, dtype=torch.float32, device=feat.device)
+    K.mlp_logits(feat, n, params, out)
+    return out
+
+
+def pick_threshold(z, label, recall=0.995):
+    """Largest logit threshold that keeps the given recall of troubled samples."""
+    zp = torch.sort(z[label.bool()]).values
+    if zp.numel() == 0:
+        return 0.0
+    k = int(np.floor((1.0 - recall) * zp.numel()))
+    return float(zp[k].item()) - 1e-3
+
+
+def swe_problem(seed, mx, my):
+    """A seeded shallow-water initial state (host, Fortran order (3, mx, my)):
+    a circular or planar dam break or a smooth bump on a random background."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(mx) + 0.5) / mx * 2 - 1
+    y = (np.arange(my) + 0.5) / my * 2 - 1
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    kind = seed % 3
+    h0 = rng.uniform(0.5, 1.5)
+    h1 = h0 + rng.uniform(0.2, 1.5) * rng.choice([-1, 1]) * (0.3 if kind == 2 else 1.0)
+    h1 = max(h1, 0.1)
+    cx, cy = rng.uniform(-0.4, 0.4, 2)
+    if kind == 0:
+        r = np.hypot(X - cx, Y - cy)
+        h = np.where(r < rng.uniform(0.15, 0.5), h1, h0)
+    elif kind == 1:
+        th = rng.uniform(0, np.pi)
+        h = np.where((X - cx) * np.cos(th) + (Y - cy) * np.sin(th) < 0, h1, h0)
+    else:
+        w = rng.uniform(0.1, 0.3)
+        h = h0 + (h1 - h0) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (w * w))
+    u = rng.uniform(-0.3, 0.3)
+    v = rng.uniform(-0.3, 0.3)
+    q = np.empty((3, mx, my), order="F")
+    q[0] = h
+    q[1] = h * u
+    q[2] = h * v
+    return q
+
+
+def generate_training_data(seeds, mx=256, my=256, nsteps=60, snapshots=(10, 30, 60),
+                           flat=1e-6, wtol=0.05, keep_negative=0.25, device="cuda"):
+    """Run full characteristic WENO5 SharpClaw (FP64, SSP33) on seeded SWE
+    problems and collect features/labels at the given step counts.  All troubled
+    samples are kept, a `keep_negative` fraction of the others."""
+    from . import BC, Domain, SharpClawSolver2D, Solution, riemann
+    feats, labels = [], []
+    gen = torch.Generator(device=device)
+    gen.manual_seed(12345)
+    for seed in seeds:
+        solver = SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
+        solver.time_integrator = "SSP33"
+        solver.cfl_max = 1.0
+        solver.cfl_desired = 0.9
+        solver.char_decomp = 2
+        solver.all_bcs = BC.extrap
+        domain = Domain([-1.0, -1.0], [1.0, 1.0], [mx, my])
+        sol = Solution(3, domain)
+        sol.problem_data["grav"] = 1.0
+        sol.problem_data["efix"] = True
+        sol.q = swe_problem(seed, mx, my)
+        solver.dt = 1e-4
+        solver.setup(sol)
+        for step in range(1, nsteps + 1):
+            solver.evolve_to_time(sol)
+            if step in snapshots:
+                q, _ = sol.state.device_q(K.L.F64)
+                f, l = features(q, [1, 1], [1, 1], flat, wtol)
+                keep = l.bool() | (torch.rand(l.shape, device=l.device, generator=gen)
+                                   < keep_negative)
+                feats.append(f[keep])
+                labels.append(l[keep])
+    return torch.cat(feats), torch.cat(labels)
+
+
+def main(argv=None):
+    """python -m pyclaw_amd.indicator --out FILE: train the shipped weights."""
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--runs", type=int, default=48)
+    ap.add_argument("--mx", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--recall", type=float, default=0.999)
+    args = ap.parse_args(argv)
+    feat, label = generate_training_data(range(args.runs), args.mx, args.mx)
+    n = label.numel()
+    perm = torch.randperm(n, device=feat.device)
+    ntr = int(0.9 * n)
+    tr, va = perm[:ntr], perm[ntr:]
+    print(f"samples {n}, troubled fraction {label.float().mean().item():.3f}", flush=True)
+    P, hist = train(feat[tr].contiguous(), label[tr].contiguous(), steps=args.steps,
+                    log_every=500)
+    z = logits(feat[va].contiguous(), P)
+    lv = label[va]
+    thr = pick_threshold(z, lv, args.recall)
+    pred = z > thr
+    recall = (pred & lv.bool()).sum().item() / max(1, lv.sum().item())
+    flagged = pred.float().mean().item()
+    print(f"threshold {thr:.3f} recall {recall:.4f} flagged {flagged:.3f} "
+          f"(troubled {lv.float().mean().item():.3f})", flush=True)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    save(args.out, P.cpu().numpy(), thr, 1e-6)
+    print(f"saved {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -107,3 +107,97 @@ def step3_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthli
         float(dt), _ptr(cfl), _stream())
     L.check(rc, "clawhip_step3_split")
     return (w, cw)
+
+
+# SharpClaw stage modes (include/clawhip.h, clawhip_sharp2_stage)
+Y_ARRAY, Y_AXPY, Y_AXPY6 = 0, 1, 2
+OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F32_S3, OUT_D = range(8)
+
+
+def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx, dy, P,
+                 Q=None, s=0.0, y_mode=Y_ARRAY, out_mode=OUT_D, A=None, B=None, aux_in=None,
+                 comp_in=None, aux_out=None, comp_out=None, flags=None):
+    """D = dt*L(Y) (SharpClaw flux2) fused with one RK stage; planar (meqn, my, mx).
+    flags: uint8 troubled-interface flags (mlp_flags) selecting full WENO5 per
+    interface; None = full WENO5 everywhere."""
+    meqn, my, mx = out.shape
+    g = L.geom2(mx, my)
+    code = _dtype_code(out)
+    rc = L.lib().clawhip_sharp2_stage(
+        ctypes.byref(rp), code, int(char_decomp), ctypes.byref(g), L.ints(bc_lower),
+        L.ints(bc_upper), L.ints(reflect), int(y_mode), int(out_mode), float(dt), float(s),
+        float(dx), float(dy), _ptr(P), _ptr(Q), _ptr(A), _ptr(B), _ptr(aux_in), _ptr(comp_in),
+        _ptr(out), _ptr(aux_out), _ptr(comp_out), _ptr(flags), _ptr(cfl), _stream())
+    L.check(rc, "clawhip_sharp2_stage")
+
+
+def ssp104_mid(q, s1):
+    rc = L.lib().clawhip_ssp104_mid(_dtype_code(q), _ptr(q), _ptr(s1), int(q.numel()), _stream())
+    L.check(rc, "clawhip_ssp104_mid")
+
+
+def step2_unsplit_run(rp, bufs, comps, ctl_dev, bc_lower, bc_upper, reflect, method, mthlim,
+                      dx, dy, nsteps, num_ghost=2):
+    """nsteps device-controlled attempts (clawhip_step2_unsplit_run)."""
+    meqn, my, mx = bufs[0].shape
+    g = L.geom2(mx, my)
+    code = _dtype_code(bufs[0])
+    rc = L.lib().clawhip_step2_unsplit_run(
+        ctypes.byref(rp), code, ctypes.byref(g), L.ints(bc_lower), L.ints(bc_upper),
+        L.ints(reflect), int(num_ghost), L.ints(method),
+        L.ints(list(mthlim) + [0] * (4 - len(mthlim))), float(dx), float(dy), _ptr(bufs[0]),
+        _ptr(bufs[1]), _ptr(comps[0]), _ptr(comps[1]), _ptr(ctl_dev), int(nsteps), _stream())
+    L.check(rc, "clawhip_step2_unsplit_run")
+
+
+# ---- learned troubled-interface indicator
+def mlp_param_count():
+    return int(L.lib().clawhip_mlp_param_count())
+
+
+def mlp_num_interfaces(mx, my):
+    return (mx + 1) * my + mx * (my + 1)
+
+
+def mlp_features(q, bc_lower, bc_upper, reflect, flat, wtol, feat, label):
+    """Features (n, 8) float32 and labels (n,) uint8 of every (interface, component)
+    of the planar state q (meqn, my, mx)."""
+    meqn, my, mx = q.shape
+    n = mlp_num_interfaces(mx, my) * meqn
+    if feat.numel() < n * 8 or label.numel() < n or feat.dtype != torch.float32 \
+            or label.dtype != torch.uint8:
+        raise ValueError("feat must hold (n, 8) float32 and label (n,) uint8")
+    g = L.geom2(mx, my, 0)
+    rc = L.lib().clawhip_mlp_features(_dtype_code(q), _ptr(q), ctypes.byref(g), L.ints(bc_lower),
+                                      L.ints(bc_upper), L.ints(reflect), meqn, float(flat),
+                                      float(wtol), _ptr(feat), _ptr(label), _stream())
+    L.check(rc, "clawhip_mlp_features")
+    return n
+
+
+def mlp_train_step(feat, label, batch_idx, params, grad, adam_m, adam_v, step, lr, pos_weight,
+                   loss):
+    if batch_idx.dtype != torch.int32 or label.numel() * 8 > feat.numel():
+        raise ValueError("batch_idx must be int32 and feat hold 8 floats per label")
+    rc = L.lib().clawhip_mlp_train_step(_ptr(feat), int(label.numel()), _ptr(label),
+                                        _ptr(batch_idx),
+                                        int(batch_idx.numel()), _ptr(params), _ptr(grad),
+                                        _ptr(adam_m), _ptr(adam_v), int(step), float(lr),
+                                        float(pos_weight), _ptr(loss), _stream())
+    L.check(rc, "clawhip_mlp_train_step")
+
+
+def mlp_logits(feat, n, params, logits):
+    rc = L.lib().clawhip_mlp_logits(_ptr(feat), int(n), _ptr(params), _ptr(logits), _stream())
+    L.check(rc, "clawhip_mlp_logits")
+
+
+def mlp_flags(q, bc_lower, bc_upper, reflect, flat, params, threshold, flags):
+    meqn, my, mx = q.shape
+    if flags.numel() < mlp_num_interfaces(mx, my) or flags.dtype != torch.uint8:
+        raise ValueError("flags must hold (mx+1)*my + mx*(my+1) uint8")
+    g = L.geom2(mx, my, 0)
+    rc = L.lib().clawhip_mlp_flags(_dtype_code(q), _ptr(q), ctypes.byref(g), L.ints(bc_lower),
+                                   L.ints(bc_upper), L.ints(reflect), meqn, float(flat),
+                                   _ptr(params), float(threshold), _ptr(flags), _stream())
+    L.check(rc, "clawhip_mlp_flags")

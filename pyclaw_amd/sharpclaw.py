@@ -1,0 +1,185 @@
+"""SharpClaw 2D solver (pyclaw/sharpclaw/solver.py:28-983) on the GPU.
+
+The semi-discrete operator dq = dt*L(q) (ghost fill + WENO5 reconstruction +
+Riemann solves) runs in clawhip_sharp2_stage, fused with the Runge-Kutta stage
+combination; SSP33 and SSP104 keep the reference's stage sequence and
+arithmetic (sharpclaw/solver.py:283-430), so the FP64 path equals the reference
+step bit for bit.  precision = "fp32k" runs SSP33 with FP32 stages and a
+Kahan-compensated final update.
+"""
+import torch
+
+from . import kernels as K
+from .solver import BC, Solver
+
+
+class SharpClawSolver(Solver):
+    _sspcoeff = {"Euler": 1.0, "SSP22": 1.0, "SSP33": 1.0, "SSP104": 6.0}
+    _cfl_default = {"SSP104": [2.45, 2.5]}
+
+    def __init__(self, riemann_solver=None, claw_package=None):
+        self.limiters = [1]
+        self.lim_type = 2
+        self.weno_order = 5
+        self.time_integrator = "SSP104"
+        self.char_decomp = 0
+        self.tfluct_solver = False
+        self.tfluct = None
+        self.aux_time_dep = False
+        self.kernel_language = "Fortran"
+        self.num_ghost = 3
+        self.fwave = False
+        self.cfl_desired = None
+        self.cfl_max = None
+        self.dq_src = None
+        self.call_before_step_each_stage = False
+        self._mthlim = self.limiters
+        self._method = None
+        self._registers = None
+        self.dq_dt = None
+        self.dt_old = None
+        self._rp_struct = None
+        self._bufs = {}
+        self.indicator = None
+        self._flags = None
+        super(SharpClawSolver, self).__init__(riemann_solver, claw_package)
+
+    # ---- sharpclaw/solver.py:202-265
+    def setup(self, solution):
+        if self.lim_type == 2:
+            self.num_ghost = (self.weno_order + 1) // 2
+        if self.lim_type != 2 or self.weno_order != 5:
+            raise NotImplementedError("pyclaw_amd implements lim_type=2 with weno_order=5")
+        if self.char_decomp not in (0, 2):
+            raise NotImplementedError("char_decomp must be 0 (component) or 2 (characteristic)")
+        if self.tfluct_solver or self.fwave or self.dq_src is not None:
+            raise NotImplementedError("tfluct solvers, f-waves and source terms are not supported")
+        if self.time_integrator not in ("SSP33", "SSP104"):
+            raise NotImplementedError("time_integrator must be 'SSP33' or 'SSP104'")
+        if self.precision == "fp32k" and self.time_integrator != "SSP33":
+            raise NotImplementedError("the FP32 path implements SSP33")
+        try:
+            if self.cfl_max is None:
+                self.cfl_desired = self._cfl_default[self.time_integrator][0]
+                self.cfl_max = self._cfl_default[self.time_integrator][1]
+            if self.cfl_desired is None:
+                self.cfl_desired = 0.9 * self.cfl_max
+        except KeyError:
+            raise KeyError("Maximum CFL number is not provided.")
+        self._set_mthlim()
+        state = solution.states[0]
+        if state.num_aux > 0 or state.index_capa >= 0:
+            raise NotImplementedError("aux arrays and capacity functions are not supported on the device")
+        if self.rp is None or not hasattr(self.rp, "kind"):
+            raise Exception("the solver needs one of the pyclaw_amd.riemann modules")
+        self._rp_struct = K.make_rp(self.rp.kind, state.problem_data)
+        self._is_set_up = True
+
+    def _set_mthlim(self):
+        self._mthlim = self.limiters
+        if not isinstance(self.limiters, list):
+            self._mthlim = [self._mthlim]
+        n = self.num_eqn if self.char_decomp == 0 else self.num_waves
+        if len(self._mthlim) == 1:
+            self._mthlim = self._mthlim * n
+        if len(self._mthlim) != n:
+            raise Exception("Length of solver.limiters is not equal to 1 or to solver.num_eqn/num_waves")
+
+    # ---- sharpclaw/solver.py:653-685 (Runge-Kutta branch)
+    def get_dt_new(self):
+        self.dt_old = self.dt
+        cfl = self.cfl.get_cached_max()
+        self.dt = self.dt * self.cfl_desired / cfl
+
+    def accept_reject_step(self, state):
+        return not (self.cfl.get_cached_max() > self.cfl_max)
+
+    def _buf(self, name, like):
+        t = self._bufs.get(name)
+        if t is None or t.shape != like.shape or t.dtype != like.dtype:
+            t = torch.empty_like(like)
+            self._bufs[name] = t
+        return t
+
+    def _stage(self, state_q, out, P, **kw):
+        """One operator evaluation (dq) fused with a stage; updates the CFL."""
+        lo, hi = self._bc_codes()
+        if BC.custom in lo or BC.custom in hi:
+            raise NotImplementedError("custom boundary conditions are not supported by SharpClaw on the device")
+        dx, dy = self._grid.delta
+        cfl = self._cfl_buffer()
+        K.cfl_reset(cfl)
+        K.sharp2_stage(self._rp_struct, self.char_decomp, out, cfl, lo, hi, self.reflect_index,
+                       self.dt, dx, dy, P, flags=self._flags, **kw)
+        self.cfl.update_global_max(float(cfl.item()))
+
+    # ---- sharpclaw/solver.py:283-430
+    def step(self, solution, take_one_step, tstart, tend):
+        state = solution.states[0]
+        self._grid = state.grid
+        q, comp = state.device_q(self._dtype_code)
+        self._flags = None
+        if self.indicator is not None:
+            lo, hi = self._bc_codes()
+            self._flags = self.indicator.flags(q, lo, hi, self.reflect_index)
+        if self.accept_step:
+            self.cfl.set_global_max(0.)
+            dqdt = self._buf("dq_dt_new", q)
+            self._stage(q, dqdt, q, out_mode=K.OUT_DQDT)
+            # keep the previous dq_dt buffer free for the next evaluation
+            self._bufs["dq_dt_new"], self._bufs["dq_dt"] = self._bufs.get("dq_dt"), dqdt
+            self.dq_dt = dqdt
+        self.get_dt(solution.t, tstart, tend, take_one_step)
+        cfl = self.cfl.get_cached_max()
+        self.cfl.set_global_max(self.dt / self.dt_old * cfl)
+        self.dt_old = self.dt
+        dt = self.dt
+        out_q = self._take_scratch((q, comp))
+        if self.time_integrator == "SSP33":
+            r2 = self._buf("r2", q)
+            if self.precision == "fp64":
+                self._stage(q, r2, q, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
+                            out_mode=K.OUT_SSP33_2, A=q)
+                self._stage(q, out_q[0], r2, out_mode=K.OUT_SSP33_3, A=q)
+            else:
+                aux = self._buf("aux", q)
+                self._stage(q, r2, q, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
+                            out_mode=K.OUT_F32_S2, A=q, B=self.dq_dt, aux_out=aux)
+                self._stage(q, out_q[0], r2, out_mode=K.OUT_F32_S3, A=q, aux_in=aux,
+                            comp_in=comp, comp_out=out_q[1])
+        else:   # SSP104
+            s1a, s1b, qm = self._buf("s1a", q), self._buf("s1b", q), self._buf("qm", q)
+            self._stage(q, s1a, q, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY6, out_mode=K.OUT_ADD6)
+            cur, nxt = s1a, s1b
+            for _ in range(3):
+                self._stage(q, nxt, cur, out_mode=K.OUT_ADD6)
+                cur, nxt = nxt, cur
+            qm.copy_(q)
+            K.ssp104_mid(qm, cur)
+            for _ in range(4):
+                self._stage(q, nxt, cur, out_mode=K.OUT_ADD6)
+                cur, nxt = nxt, cur
+            self._stage(q, out_q[0], cur, out_mode=K.OUT_SSP104_FIN, A=qm)
+        state.set_device_q(*out_q)
+        self._scratch = (q, comp)
+        self._flags = None
+        return True
+
+    def dq(self, state):
+        """dt*L(q) on the interior as a host array (sharpclaw/solver.py:532-542)."""
+        from .device import q_from_device
+        self._grid = state.grid
+        q, _ = state.device_q(self._dtype_code)
+        out = torch.empty_like(q)
+        self._stage(q, out, q, out_mode=K.OUT_D)
+        return q_from_device(out)
+
+
+class SharpClawSolver2D(SharpClawSolver):
+    """sharpclaw/solver.py:829-898"""
+
+    def __init__(self, riemann_solver=None, claw_package=None):
+        self.num_dim = 2
+        self.reflect_index = [1, 2]
+        self._grid = None
+        super(SharpClawSolver2D, self).__init__(riemann_solver, claw_package)

@@ -1,0 +1,233 @@
+"""GPU parity of the learned troubled-interface indicator (indicator.hip)
+against its CPU restatement (oracle/mlp.py), of the flag-selected SharpClaw
+reconstruction, and the accuracy/throughput gain of the FP32 + indicator path
+over FP64 full WENO5."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as M
+from oracle import oracle as O
+from tests._cases import seeded_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _swe_state(rng, mx, my):
+    """Smooth background plus two jumps (one along x, one along y)."""
+    q = seeded_state(rng, O.SHALLOW, (mx, my))
+    q[0, mx // 3:, :] += 0.7
+    q[:, :, 2 * my // 3:] *= 1.3
+    return q
+
+
+def _dev(q):
+    from pyclaw_amd.device import q_to_device
+    return q_to_device(q)
+
+
+def _planar(q):
+    return np.ascontiguousarray(np.transpose(q, (0, 2, 1)))
+
+
+def _params(seed):
+    p = M.init_params(seed)
+    rng = np.random.default_rng(seed + 100)
+    p[M.P_W1 + 6 * M.NH:M.P_W1 + 7 * M.NH] = rng.normal(0, 0.3, M.NH)
+    p[M.P_B2:M.P_W3] = rng.normal(0, 0.3, M.NH)
+    p[M.P_B3] = 0.1
+    return p
+
+
+@pytest.mark.parametrize("bcl,bcu", [([1, 1], [1, 1]), ([2, 3], [2, 1]), ([3, 2], [3, 2])])
+def test_features_labels_match_oracle(bcl, bcu):
+    from pyclaw_amd import indicator as I
+    rng = np.random.default_rng(sum(bcl) * 10 + sum(bcu))
+    mx, my = 37, 29
+    q = _swe_state(rng, mx, my)
+    flat, wtol = 1e-6, 0.05
+    f_ref, l_ref = M.features(_planar(q), bcl, bcu, flat, wtol)
+    f, l = I.features(_dev(q), bcl, bcu, flat, wtol)
+    assert np.array_equal(f.cpu().numpy(), f_ref)
+    assert np.array_equal(l.cpu().numpy(), l_ref)
+    assert 0 < l_ref.sum() < l_ref.size
+
+
+def test_logits_match_oracle():
+    # checks the MFMA operand/result lane layout of every layer
+    from pyclaw_amd import indicator as I
+    rng = np.random.default_rng(1)
+    feat = np.zeros((1000, 8), np.float32)       # not a multiple of 16: padded tile
+    feat[:, :6] = rng.uniform(-1, 1, (1000, 6))
+    feat[:, 6] = 1.0
+    p = _params(2)
+    z_ref, _ = M.forward(p, feat)
+    z = I.logits(torch.tensor(feat, device="cuda"),
+                 torch.tensor(p, device="cuda")).cpu().numpy()
+    assert np.abs(z - z_ref).max() <= 2e-3 * (1 + np.abs(z_ref).max())
+    # a transposed or shifted layout would decorrelate the two completely
+    assert np.corrcoef(z, z_ref)[0, 1] > 0.9999
+
+
+def _train_call(feat, label, idx, p, step, lr=1e-3, pw=3.0):
+    from pyclaw_amd import kernels as K
+    dev = "cuda"
+    P = torch.tensor(p, device=dev)
+    G = torch.zeros_like(P)
+    m1 = torch.zeros_like(P)
+    m2 = torch.zeros_like(P)
+    loss = torch.zeros(1, device=dev)
+    K.mlp_train_step(torch.tensor(feat, device=dev), torch.tensor(label, device=dev),
+                     torch.tensor(idx, dtype=torch.int32, device=dev), P, G, m1, m2, step, lr,
+                     pw, loss)
+    torch.cuda.synchronize()
+    return P.cpu().numpy(), G.cpu().numpy(), float(loss.item())
+
+
+def test_gradients_match_oracle():
+    rng = np.random.default_rng(3)
+    n = 5000
+    feat = np.zeros((n, 8), np.float32)
+    feat[:, :6] = rng.uniform(-1, 1, (n, 6))
+    feat[:, 6] = 1.0
+    label = (rng.uniform(size=n) < 0.3).astype(np.uint8)
+    idx = rng.integers(0, n, 1000)
+    idx[:3] = [-1, n, n + 7]                      # out-of-range entries are skipped
+    p = _params(4)
+    _, g, loss = _train_call(feat, label, idx, p, 0)
+    valid = idx[(idx >= 0) & (idx < n)]
+    # the kernel averages over nbatch (all entries); the oracle over its batch
+    lref, gref = M.loss_grad(p, feat, label, valid, 3.0)
+    scale = len(valid) / len(idx)
+    assert abs(loss - lref * scale) < 1e-4 * abs(lref)
+    groups = {"W1": (M.P_W1, M.P_W2), "W2": (M.P_W2, M.P_B2), "b2": (M.P_B2, M.P_W3),
+              "w3": (M.P_W3, M.P_B3), "b3": (M.P_B3, M.P_N)}
+    for name, (a, b) in groups.items():
+        ga, gr = g[a:b], gref[a:b] * scale
+        err = np.linalg.norm(ga - gr) / max(np.linalg.norm(gr), 1e-30)
+        assert err < 1e-2, (name, err)
+    assert np.all(g[M.P_W1 + 7 * M.NH:M.P_W2] == 0)
+
+
+def test_adam_step_matches_torch():
+    rng = np.random.default_rng(5)
+    n = 16                                         # one tile: one wavefront
+    feat = np.zeros((n, 8), np.float32)
+    feat[:, :6] = rng.uniform(-1, 1, (n, 6))
+    feat[:, 6] = 1.0
+    label = (np.arange(n) % 3 == 0).astype(np.uint8)
+    idx = np.arange(n)
+    p = _params(6)
+    _, g, _ = _train_call(feat, label, idx, p, 0)
+    p1, g1, _ = _train_call(feat, label, idx, p, 1, lr=1e-3)
+    assert np.all(g1 == 0)                         # Adam clears the gradient
+    t = torch.tensor(p.astype(np.float64), requires_grad=True)
+    opt = torch.optim.Adam([t], lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    t.grad = torch.tensor(g.astype(np.float64))
+    opt.step()
+    ref = t.detach().numpy()
+    ref[M.P_W1 + 7 * M.NH:M.P_W2] = 0.0
+    assert np.abs(p1 - ref).max() < 2e-5
+
+
+def test_flags_match_oracle():
+    from pyclaw_amd import indicator as I
+    rng = np.random.default_rng(8)
+    mx, my = 33, 26
+    q = _swe_state(rng, mx, my)
+    p = _params(9)
+    ind = I.TroubledInterfaceIndicator(p, threshold=0.2, flat=1e-6)
+    fl = ind.flags(_dev(q), [1, 2], [3, 2]).cpu().numpy()
+    ref, margin = M.flags(_planar(q), [1, 2], [3, 2], 1e-6, p, 0.2)
+    sure = margin > 1e-2
+    assert sure.mean() > 0.9
+    assert np.array_equal(fl[sure], ref[sure])
+    assert 0 < ref.sum() < ref.size
+
+
+def _advection_dq(q, flags, dt, dx, dy, u, v):
+    """dt*L(q) of periodic advection with u, v > 0 where each interface uses
+    the linear WENO5 weights unless it or a normal neighbour is flagged."""
+    mx, my = q.shape[1], q.shape[2]
+    ng = 3
+    qbc = O.with_ghosts(q, ng)
+    O.bc2(qbc, ng, [2, 2], [2, 2])
+    nxi = (mx + 1) * my
+    fx = flags[:nxi].reshape(my, mx + 1)
+    fy = flags[nxi:].reshape(my + 1, mx)
+
+    def lin(st):   # right edge of cell 2 of the stencil st[0..4]
+        return (2 * st[0] - 13 * st[1] + 47 * st[2] + 27 * st[3] - 3 * st[4]) / 60.0
+
+    dq = np.zeros((1, mx, my))
+    # x: upwind edge of interface c (1..mx+1) is the right edge of cell c-1
+    for j in range(my):
+        row = qbc[:, :, j + ng]
+        _, qr = O.weno5(row, ng)
+        QR = np.zeros(mx + 2)
+        for c in range(1, mx + 2):
+            f = fx[j, c - 1] | fx[j, max(c - 2, 0)] | fx[j, min(c, mx)]
+            a = c - 1 + ng                          # 0-based array index of cell c-1
+            QR[c] = qr[0, a] if f else lin(row[0, a - 2:a + 3])
+        dq[0, :, j] -= dt / dx * u * (QR[2:mx + 2] - QR[1:mx + 1])
+    for i in range(mx):
+        col = qbc[:, i + ng, :]
+        _, qt = O.weno5(col, ng)
+        QT = np.zeros(my + 2)
+        for r in range(1, my + 2):
+            f = fy[r - 1, i] | fy[max(r - 2, 0), i] | fy[min(r, my), i]
+            a = r - 1 + ng
+            QT[r] = qt[0, a] if f else lin(col[0, a - 2:a + 3])
+        dq[0, i, :] -= dt / dy * v * (QT[2:my + 2] - QT[1:my + 1])
+    return dq
+
+
+@pytest.mark.parametrize("pattern", ["none", "ones", "zeros", "random"])
+def test_stage_flag_selection(pattern):
+    from pyclaw_amd import kernels as K
+    from pyclaw_amd.device import q_from_device
+    mx, my = 41, 35
+    rng = np.random.default_rng(11)
+    x = (np.arange(mx) + 0.5) / mx
+    y = (np.arange(my) + 0.5) / my
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    q = np.asfortranarray((np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y) + (X > 0.5))[None])
+    n = K.mlp_num_interfaces(mx, my)
+    fl = {"none": None, "ones": np.ones(n, np.uint8), "zeros": np.zeros(n, np.uint8),
+          "random": (rng.uniform(size=n) < 0.2).astype(np.uint8)}[pattern]
+    u, v = 0.5, 0.7
+    pd = dict(u=u, v=v)
+    dx, dy, dt = 1.0 / mx, 1.0 / my, 0.3 / max(mx, my)
+    qi = _dev(q)
+    out = torch.empty_like(qi)
+    cfl = torch.zeros(1, dtype=torch.float64, device="cuda")
+    fdev = None if fl is None else torch.tensor(fl, device="cuda")
+    K.sharp2_stage(K.make_rp(O.ADVECTION, pd), 0, out, cfl, [2, 2], [2, 2], [1, 2], dt, dx, dy,
+                   qi, flags=fdev)
+    got = q_from_device(out)
+    if pattern in ("none", "ones"):
+        rp = O.make_rp(O.ADVECTION, u=u, v=v)
+        qbc = O.with_ghosts(q, 3)
+        O.bc2(qbc, 3, [2, 2], [2, 2])
+        ref = O.interior(O.sharp_flux2(rp, qbc, 3, dx, dy, dt, 0)[0], 3)
+        assert np.array_equal(got, ref)
+    else:
+        ref = _advection_dq(q, fl, dt, dx, dy, u, v)
+        assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def test_stage_all_flags_equals_char_weno():
+    from pyclaw_amd import kernels as K
+    from pyclaw_amd.device import q_from_device
+    rng = np.random.default_rng(12)
+    mx, my = 50, 21
+    q = _swe_state(rng, mx, my)
+    qi = _dev(q)
+    outs = []
+    for fl in (None, torch.ones(K.mlp_num_interfaces(mx, my), dtype=torch.uint8, device="cuda")):
+        out = torch.empty_like(qi)
+        cfl = torch.zeros(1, dtype=torch.float64, device="cuda")
+        K.sharp2_stage(K.make_rp(O.SHALLOW, dict(grav=1.0)), 2, out, cfl, [1, 3], [3, 1],
+                       [1, 2], 1e-3, 1.0 / mx, 1.0 / my, qi, flags=fl)
+        outs.append(q_from_device(out))
+    assert np.array_equal(outs[0], outs[1])
