@@ -7,8 +7,10 @@ extrapolation boundaries, CFL 0.8 (cfl_max 1.0), 400 steps.  Initial data: a
 four-quadrant Riemann problem whose quadrant states are drawn from
 numpy.random.default_rng(2): rho ~ U[0.5,1.5], p ~ U[0.1,1.0], u, v ~ U[-0.5,0.5].
 
-A step is one full ClawSolver2D time step through the PyClaw API (dt from the
-previous step's CFL, accept/reject), state resident in HBM.  `value` is the FP64
+A step is one full ClawSolver2D time step (dt from the previous step's CFL,
+accept/reject) in the device-controlled loop (ClawSolver2D.run_steps: the
+step kernel applies the controller itself; 16 steps per captured graph),
+state resident in HBM.  `value` is the FP64
 path (bitwise equal to the reference step); the FP32 Kahan-compensated path runs
 the same steps beside it and its error against the FP64 run is reported.
 
@@ -69,54 +71,55 @@ def make_problem(precision, mx=MX, my=MY):
     return solver, solution
 
 
-class KernelTimer(object):
-    """HIP events around every step launch, on the stream the kernel runs on."""
+class ChunkTimer(object):
+    """HIP events around every replay of the captured 16-step graph, on the
+    stream the graph runs on; kernel_ms = mean event time per step kernel."""
 
-    def __init__(self):
+    def __init__(self, solver):
         import torch
         self.torch = torch
+        self.solver = solver
         self.pairs = []
+        self.orig = solver._replay_graph
 
-    def wrap(self, solver):
-        from pyclaw_amd import kernels as K
-        orig = K.step2_unsplit
-        timer = self
-
-        def timed(*a, **kw):
-            s = timer.torch.cuda.current_stream()
-            e0 = timer.torch.cuda.Event(enable_timing=True)
-            e1 = timer.torch.cuda.Event(enable_timing=True)
+        def timed(bufs, args, n):
+            s = torch.cuda.current_stream()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
             e0.record(s)
-            orig(*a, **kw)
+            self.orig(bufs, args, n)
             e1.record(s)
-            timer.pairs.append((e0, e1))
-        K.step2_unsplit = timed
-        return orig
+            self.pairs.append((e0, e1, n))
+        solver._replay_graph = timed
 
-    def mean_ms(self):
+    def close(self):
+        self.solver._replay_graph = self.orig
+
+    def kernel_ms(self):
         self.torch.cuda.synchronize()
-        return float(np.mean([a.elapsed_time(b) for a, b in self.pairs]))
+        tot = sum(a.elapsed_time(b) for a, b, _ in self.pairs)
+        n = sum(k for _, _, k in self.pairs)
+        return tot / max(n, 1), n
 
 
 def run_gpu(precision, steps, warmup):
+    """warmup steps (they include the graph capture), then `steps` timed
+    device-loop steps (graph replays of 16 steps)."""
     import torch
     solver, solution = make_problem(precision)
-    for _ in range(warmup):
-        solver.evolve_to_time(solution)
+    solver.run_steps(solution, warmup)
     torch.cuda.synchronize()
-    timer = KernelTimer()
-    from pyclaw_amd import kernels as K
-    orig = timer.wrap(solver)
+    timer = ChunkTimer(solver)
     try:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            solver.evolve_to_time(solution)
+        host = solver.run_steps(solution, steps)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
     finally:
-        K.step2_unsplit = orig
-    return wall, timer.mean_ms(), solver, solution
+        timer.close()
+    kms, nk = timer.kernel_ms()
+    return wall, kms, nk, host, solution
 
 
 def cpu_baseline(sample_steps=3):
@@ -155,17 +158,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X in this build "
               "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
         sys.exit(2)
-    import torch
     cells = MX * MY
-    wall64, k64, solver64, sol64 = run_gpu("fp64", args.steps, args.warmup)
-    wall32, k32, solver32, sol32 = run_gpu("fp32k", args.steps, args.warmup)
+    wall64, k64, n64, host64, sol64 = run_gpu("fp64", args.steps, args.warmup)
+    wall32, k32, n32, host32, sol32 = run_gpu("fp32k", args.steps, args.warmup)
     q64 = sol64.state.q
     q32 = sol32.state.q
     err_l1 = float(np.abs(q32 - q64).sum() / np.abs(q64).sum())
@@ -173,7 +175,8 @@ def main():
     value = cells * args.steps / wall64
     achieved = BYTES_PER_CELL_F64 * cells / (k64 * 1e-3) / 1e9
     rec = {
-        "metric": "cell-updates/s, 2D Euler/SWE 2048²: FP64 plain vs FP32+learned-WENO; % HBM peak",
+        "metric": "cell-updates/s, 2D Euler/SWE 2048\u00b2: FP64 plain vs FP32+learned-WENO; "
+                  "% HBM peak",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -187,13 +190,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": "euler2d_quadrants_roe_efix_unsplit_transcor_mc_1024x1024",
                    "grid": [MX, MY], "steps": args.steps, "cfl": 0.8, "bc": "extrap",
-                   "seed": 2, "t_end": sol64.t},
+                   "seed": 2, "t_end": sol64.t, "accepted": host64.accepted,
+                   "rejected": host64.nrejected},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_unsplit<Euler<double>>", "kernel_ms": k64,
-                     "bytes_per_cell": BYTES_PER_CELL_F64},
-        "fp32_kahan": {"value": cells * args.steps / wall32, "ms_per_step": 1e3 * wall32 / args.steps,
-                       "kernel_ms": k32,
+                     "kernel": "k_unsplit_ctl<Euler<double>>", "kernel_ms": k64,
+                     "kernels_timed": n64, "bytes_per_cell": BYTES_PER_CELL_F64},
+        "fp32_kahan": {"value": cells * args.steps / wall32,
+                       "ms_per_step": 1e3 * wall32 / args.steps, "kernel_ms": k32,
                        "hbm_frac": BYTES_PER_CELL_F32K * cells / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "rel_l1_vs_fp64": err_l1, "rel_max_vs_fp64": err_max,
                        "t_end": sol32.t},

@@ -175,6 +175,38 @@ int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
  *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */
 int clawhip_ssp104_mid(int dtype, void *q, void *s1, long long n, void *stream);
 
+/* ---- entry points on ghost-storing arrays: one-to-one with the reference's
+ * f2py kernels, for a `solver.fmod` drop-in (pyclaw_amd/fmod).  The caller has
+ * filled the ghost cells (the reference's _apply_bcs); every side is BC_GIVEN.
+ * Arrays are planar with `num_ghost` stored ghost layers (gin/gout->ghosts ==
+ * num_ghost), FP64 only.
+ *
+ * One dimension-split sweep, replacing step2ds (classic/step2ds.f90:2-264;
+ * called at classic/solver.py:527-533).  ids 1: x-sweep q_in -> q_out (distinct);
+ * every cell of q_out is written: interior columns of all rows 1-ng..my+ng
+ * updated, ghost columns copied.  ids 2: y-sweep in place (q_in == q_out),
+ * columns 1-ng..mx+ng swept, interior cells updated.  *cfl_dev = max(*cfl_dev,
+ * CFL of the sweep). */
+int clawhip_step2_sweep(const clawhip_rp *rp, int dtype, int ids, const void *q_in,
+                        const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
+                        int num_ghost, const int *method, const int *mthlim, double dx, double dy,
+                        double dt, double *cfl_dev, void *stream);
+
+/* One sweep of step3ds (classic/step3ds.f90:2-378 with flux3.f90; called at
+ * classic/solver.py:680-690).  idir 1 out of place (every cell written),
+ * idir 2 and 3 in place; slices 0..m+1 as the reference. */
+int clawhip_step3_sweep(const clawhip_rp *rp, int dtype, int idir, const void *q_in,
+                        const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
+                        int num_ghost, const int *method, const int *mthlim, double dx, double dy,
+                        double dz, double dt, double *cfl_dev, void *stream);
+
+/* SharpClaw's flux2 (sharpclaw/flux2.f90:2-74; called at sharpclaw/solver.py:893):
+ * dq = dt*L(q) on the interior (gout compact) from q with >= 3 stored, filled
+ * ghost layers; char_decomp 0 (component WENO5) or 2 (characteristic). */
+int clawhip_sharp2_flux(const clawhip_rp *rp, int dtype, int char_decomp, const void *q,
+                        const clawhip_geom *gin, void *dq, const clawhip_geom *gout, double dt,
+                        double dx, double dy, double *cfl_dev, void *stream);
+
 /* ---- learned troubled-interface indicator (no counterpart in the reference;
  * the reference always runs the full WENO5 of reconstruct.f90:201-299).
  * A 6->32->32->1 MLP in BF16 MFMA with FP32 accumulation.  Parameters: one FP32

@@ -86,6 +86,12 @@ _SIGS = {
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_step2_unsplit_run": [ctypes.POINTER(RP), _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                                   _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _I, _P],
+    "clawhip_step2_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
+                            ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _P, _P],
+    "clawhip_step3_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
+                            ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _D, _P, _P],
+    "clawhip_sharp2_flux": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
+                            ctypes.POINTER(Geom), _D, _D, _D, _P, _P],
     "clawhip_mlp_param_count": [],
     "clawhip_mlp_features": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _F, _P, _P,
                              _P],

@@ -178,7 +178,16 @@ class ClawSolver2D(ClawSolver):
         self.run_device(solution, tend, max_steps)
         return self.status
 
-    def run_device(self, solution, tend, max_steps):
+    def run_steps(self, solution, nsteps):
+        """Attempt exactly nsteps steps (no end time) in the device loop."""
+        if not self._is_set_up:
+            self.setup(solution)
+        if not self._device_loop_ok(solution):
+            raise ValueError("run_steps needs the device loop: unsplit, no step callbacks, "
+                             "built-in boundary conditions")
+        return self.run_device(solution, float("inf"), int(nsteps), check_max_steps=False)
+
+    def run_device(self, solution, tend, max_steps, check_max_steps=True):
         """Up to max_steps attempted steps towards tend, controlled on the device.
         Same arithmetic and decisions as Solver.evolve_to_time (solver.py:554-666)."""
         import torch
@@ -240,7 +249,8 @@ class ClawSolver2D(ClawSolver):
         self.accept_step = host.done != 2
         if host.done == 2:
             raise Exception("CFL too large, giving up!")
-        if self.dt_variable and solution.t < tend and host.accepted == max_steps:
+        if (check_max_steps and self.dt_variable and solution.t < tend
+                and host.accepted == max_steps):
             raise Exception("Maximum number of timesteps have been taken")
         return host
 

@@ -162,7 +162,7 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
   for (int d = 0; d < 2; d++)
     if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
       return CLAWHIP_E_ARG;
-  return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), bc_lower, bc_upper, reflect,
+  return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), to_geom(g), bc_lower, bc_upper, reflect,
                             y_mode, out_mode, dt, s, dx, dy, P, Q, A, B, aux_in, comp_in, out,
                             aux_out, comp_out, (unsigned long long*)cfl_dev, flags,
                             (hipStream_t)stream);
@@ -199,6 +199,77 @@ int clawhip_step2_unsplit_run(const clawhip_rp* rp, int dtype, const clawhip_geo
   b.q[0] = q0; b.q[1] = q1; b.comp[0] = comp0; b.comp[1] = comp1; b.dx = dx; b.dy = dy;
   return launch_unsplit_ctl(*rp, dtype, to_geom(g), bc_lower, bc_upper, reflect, num_ghost,
                             method, mthlim, (StepCtl*)ctl_dev, b, nsteps, (hipStream_t)stream);
+}
+
+// ---- sweeps and operator on ghost-storing arrays (the f2py entry points) ----
+static int check_sweep(const clawhip_rp* rp, int dtype, int ndim, int dir, const void* q_in,
+                       const clawhip_geom* gin, const void* q_out, const clawhip_geom* gout,
+                       int num_ghost, const int* method, const int* mthlim, double* cfl_dev) {
+  if (!rp || !q_in || !q_out || !gin || !gout || !method || !mthlim || !cfl_dev)
+    return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64) return CLAWHIP_E_ARG;
+  if (dir < 1 || dir > ndim) return CLAWHIP_E_ARG;
+  if (!geom_ok(gin, ndim) || !geom_ok(gout, ndim)) return CLAWHIP_E_ARG;
+  if (num_ghost != 2 || gin->ghosts != num_ghost || gout->ghosts != num_ghost) return CLAWHIP_E_ARG;
+  if (gin->mx != gout->mx || gin->my != gout->my || (ndim == 3 && gin->mz != gout->mz))
+    return CLAWHIP_E_ARG;
+  if (gout->mx < num_ghost || gout->my < num_ghost || (ndim == 3 && gout->mz < num_ghost))
+    return CLAWHIP_E_ARG;
+  if ((dir == 1) == (q_in == q_out)) return CLAWHIP_E_ARG;   // sweep 1 out of place, 2/3 in place
+  if (dir > 1 && (gin->field_stride != gout->field_stride || gin->row_stride != gout->row_stride ||
+                  gin->slab_stride != gout->slab_stride))
+    return CLAWHIP_E_ARG;
+  if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
+  if (method[2] != -1 || method[5] != 0 || method[6] != 0) return CLAWHIP_E_METHOD;
+  return 0;
+}
+
+int clawhip_step2_sweep(const clawhip_rp* rp, int dtype, int ids, const void* q_in,
+                        const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
+                        int num_ghost, const int* method, const int* mthlim, double dx, double dy,
+                        double dt, double* cfl_dev, void* stream) {
+  int rc = check_sweep(rp, dtype, 2, ids, q_in, gin, q_out, gout, num_ghost, method, mthlim,
+                       cfl_dev);
+  if (rc) return rc;
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
+  Geom a = to_geom(gin), b = to_geom(gout);
+  a.ss = 0;   // 2D: the sweep kernels add (k-1+g)*ss with k = 1
+  b.ss = 0;
+  return launch_sweep2(*rp, ids, q_in, a, q_out, b, num_ghost, method, mthlim,
+                       ids == 1 ? dt / dx : dt / dy, (unsigned long long*)cfl_dev,
+                       (hipStream_t)stream);
+}
+
+int clawhip_step3_sweep(const clawhip_rp* rp, int dtype, int idir, const void* q_in,
+                        const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
+                        int num_ghost, const int* method, const int* mthlim, double dx, double dy,
+                        double dz, double dt, double* cfl_dev, void* stream) {
+  int rc = check_sweep(rp, dtype, 3, idir, q_in, gin, q_out, gout, num_ghost, method, mthlim,
+                       cfl_dev);
+  if (rc) return rc;
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
+  const double dtdx = idir == 1 ? dt / dx : (idir == 2 ? dt / dy : dt / dz);
+  return launch_sweep3(*rp, idir, q_in, to_geom(gin), q_out, to_geom(gout), num_ghost, method,
+                       mthlim, dtdx, (unsigned long long*)cfl_dev, (hipStream_t)stream);
+}
+
+int clawhip_sharp2_flux(const clawhip_rp* rp, int dtype, int char_decomp, const void* q,
+                        const clawhip_geom* gin, void* dq, const clawhip_geom* gout, double dt,
+                        double dx, double dy, double* cfl_dev, void* stream) {
+  if (!rp || !q || !gin || !dq || !gout || !cfl_dev) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_ARG;
+  if (!geom_ok(gin, 2) || !geom_ok(gout, 2) || gin->ghosts < 3 || gout->ghosts != 0)
+    return CLAWHIP_E_ARG;
+  if (gin->mx != gout->mx || gin->my != gout->my || gout->mx < 3 || gout->my < 3)
+    return CLAWHIP_E_ARG;
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
+  const int given[2] = {CLAWHIP_BC_GIVEN, CLAWHIP_BC_GIVEN};
+  const int refl[2] = {1, 2};
+  return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(gout), to_geom(gin), given, given,
+                            refl, 0, 7, dt, 0.0, dx, dy, q, nullptr, nullptr, nullptr, nullptr,
+                            nullptr, dq, nullptr, nullptr, (unsigned long long*)cfl_dev, nullptr,
+                            (hipStream_t)stream);
 }
 
 int clawhip_mlp_param_count(void) { return 16 * 32 + 32 * 32 + 32 + 32 + 1; }

@@ -395,6 +395,38 @@ int run_split3(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_
   return launch_march<RS, T, KAHAN, 3, true>(az, stream);
 }
 
+// One sweep of step2ds / step3ds between arrays that store their ghost cells
+// (the reference's qbc, filled by the caller: BC_GIVEN on every side).  Sweep 1
+// reads q_in and writes every cell of q_out (interior columns updated on all
+// rows, ghost columns copied), as step2ds/step3ds with ids/idir = 1 do; sweeps
+// 2 and 3 update q_out in place (q_in == q_out), marching whole pencils.
+template <class RS>
+int run_sweep(const clawhip_rp& rp, int ndim, int dir, const void* q_in, const Geom& gin,
+              void* q_out, const Geom& gout, int ng, const int* method, const int* mthlim,
+              double dtdx, unsigned long long* cfl, hipStream_t stream) {
+  const int given[3] = {BC_GIVEN, BC_GIVEN, BC_GIVEN};
+  const int refl[3] = {1, 2, 3};
+  auto a = base_args<double>(rp, gin, gout, given, given, refl, ndim, ng, method, mthlim,
+                             RS::mwaves, dtdx, q_in, q_out, nullptr, nullptr, cfl);
+  if (dir == 1) {
+    if (ndim == 2) return launch_x<RS, double, false, 2, false>(a, stream);
+    return launch_x<RS, double, false, 3, true>(a, stream);
+  }
+  if (ndim == 2) {   // step2ds ids = 2 (step2ds.f90:180): columns 1-ng..mx+ng
+    a.i_lo = 1 - ng; a.i_hi = gout.mx + ng; a.p_lo = a.p_hi = 1;
+    a.w_ilo = 1; a.w_ihi = gout.mx; a.w_plo = 1; a.w_phi = 1;
+    return launch_march<RS, double, false, 2, false>(a, stream);
+  }
+  if (dir == 2) {    // step3ds idir = 2: i, k in 0..m+1
+    a.i_lo = 0; a.i_hi = gout.mx + 1; a.p_lo = 0; a.p_hi = gout.mz + 1;
+    a.w_ilo = 0; a.w_ihi = gout.mx + 1; a.w_plo = 0; a.w_phi = gout.mz + 1;
+    return launch_march<RS, double, false, 2, true>(a, stream);
+  }
+  a.i_lo = 0; a.i_hi = gout.mx + 1; a.p_lo = 0; a.p_hi = gout.my + 1;   // idir = 3
+  a.w_ilo = 0; a.w_ihi = gout.mx + 1; a.w_plo = 0; a.w_phi = gout.my + 1;
+  return launch_march<RS, double, false, 3, true>(a, stream);
+}
+
 template <template <typename> class RSF>
 int dispatch_split2(int dtype, const clawhip_rp& rp, const void* q_in, const Geom& gin,
                     void* q_out, const Geom& gout, const void* cin, void* cout, void* work,
@@ -447,6 +479,34 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
   return run_split3<Euler3<float>, float, true>(rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                                 work, cwork, lo, hi, refl, ng, method, mthlim,
                                                 dx, dy, dz, dt, cfl, s);
+}
+
+int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& gin, void* q_out,
+                  const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
+                  unsigned long long* cfl, hipStream_t s) {
+  switch (rp.kind) {
+    case CLAWHIP_RP_ACOUSTICS:
+      return run_sweep<Acoustics<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
+                                          dtdx, cfl, s);
+    case CLAWHIP_RP_EULER:
+      return run_sweep<Euler<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
+                                      dtdx, cfl, s);
+    case CLAWHIP_RP_SHALLOW:
+      return run_sweep<Shallow<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
+                                        dtdx, cfl, s);
+    case CLAWHIP_RP_ADVECTION:
+      return run_sweep<Advection<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
+                                          dtdx, cfl, s);
+  }
+  return CLAWHIP_E_KIND;
+}
+
+int launch_sweep3(const clawhip_rp& rp, int idir, const void* q_in, const Geom& gin, void* q_out,
+                  const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
+                  unsigned long long* cfl, hipStream_t s) {
+  if (rp.kind != CLAWHIP_RP_EULER3) return CLAWHIP_E_KIND;
+  return run_sweep<Euler3<double>>(rp, 3, idir, q_in, gin, q_out, gout, ng, method, mthlim, dtdx,
+                                   cfl, s);
 }
 
 }  // namespace claw

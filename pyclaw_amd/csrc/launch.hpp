@@ -47,7 +47,7 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
                   double dz, double dt, unsigned long long* cfl, hipStream_t s);
 
 int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
-                       const int* lo, const int* hi, const int* refl, int ymode, int omode,
+                       const Geom& gin, const int* lo, const int* hi, const int* refl, int ymode, int omode,
                        double dt, double s, double dx, double dy, const void* P, const void* Q,
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
@@ -58,6 +58,13 @@ int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int
                        const int* hi, const int* refl, int ng, const int* method,
                        const int* mthlim, StepCtl* ctl, const CtlBuffers& bufs, int nsteps,
                        hipStream_t stream);
+
+int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& gin, void* q_out,
+                  const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
+                  unsigned long long* cfl, hipStream_t s);
+int launch_sweep3(const clawhip_rp& rp, int idir, const void* q_in, const Geom& gin, void* q_out,
+                  const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
+                  unsigned long long* cfl, hipStream_t s);
 
 int launch_features_labels(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
                            const int* refl, int meqn, float flat, float wtol, float* feat,

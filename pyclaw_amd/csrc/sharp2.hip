@@ -48,7 +48,8 @@ enum StageY {
 template <typename T>
 struct SharpArgs {
   RPParams<T> rp;
-  Geom g;               // all arrays: compact, this geometry
+  Geom g;               // A, B, aux, comp and the outputs: compact
+  Geom gin;             // P and Q (the stage input); may store ghost cells (BC_GIVEN)
   BCSpec bc;
   int ng;
   int char_decomp;
@@ -293,10 +294,10 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
     const int lx = e % LX, ly = e / LX;
     const int c = c0 - HALO + lx, r = j0 - HALO + ly;
     T y[ME];
-    load_cell2<ME>(a.P, a.g, a.bc, a.ng, c, r, y);
+    load_cell2<ME>(a.P, a.gin, a.bc, a.ng, c, r, y);
     if (a.ymode != Y_ARRAY) {
       T qv[ME];
-      load_cell2<ME>(a.Q, a.g, a.bc, a.ng, c, r, qv);
+      load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, qv);
 #pragma unroll
       for (int m = 0; m < ME; m++)
         y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
@@ -487,7 +488,7 @@ __global__ void k_ssp104_mid(T* q, T* s1, long long n) {
 // ---------------------------------------------------------------- launchers
 namespace {
 template <class RS, typename T, bool CHAR>
-int run_stage(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
+int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* lo, const int* hi,
               const int* refl, int ymode, int omode, double dt, double s, double dx, double dy,
               const void* P, const void* Q, const void* A, const void* B, const void* aux_in,
               const void* comp_in, void* out, void* aux_out, void* comp_out,
@@ -495,6 +496,7 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
   SharpArgs<T> a;
   a.rp = to_params<T>(rp);
   a.g = g;
+  a.gin = gin;
   a.bc = to_bc(lo, hi, refl, 2);
   a.ng = 3;
   a.char_decomp = CHAR ? 2 : 0;
@@ -518,29 +520,30 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
 }
 
 template <template <typename> class RSF, bool CHAR>
-int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
+int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* lo,
+                   const int* hi,
                    const int* refl, int ymode, int omode, double dt, double s, double dx,
                    double dy, const void* P, const void* Q, const void* A, const void* B,
                    const void* aux_in, const void* comp_in, void* out, void* aux_out,
                    void* comp_out, unsigned long long* cfl, const unsigned char* flags,
                    hipStream_t stream) {
   if (dtype == CLAWHIP_F64)
-    return run_stage<RSF<double>, double, CHAR>(rp, g, lo, hi, refl, ymode, omode, dt, s, dx,
+    return run_stage<RSF<double>, double, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx,
                                                 dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
                                                 comp_out, cfl, flags, stream);
-  return run_stage<RSF<float>, float, CHAR>(rp, g, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
+  return run_stage<RSF<float>, float, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
                                             Q, A, B, aux_in, comp_in, out, aux_out, comp_out,
                                             cfl, flags, stream);
 }
 }  // namespace
 
 int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
-                       const int* lo, const int* hi, const int* refl, int ymode, int omode,
+                       const Geom& gin, const int* lo, const int* hi, const int* refl, int ymode, int omode,
                        double dt, double s, double dx, double dy, const void* P, const void* Q,
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
                        const unsigned char* flags, hipStream_t stream) {
-#define ARGS dtype, rp, g, lo, hi, refl, ymode, omode, dt, s, dx, dy, P, Q, A, B, aux_in, \
+#define ARGS dtype, rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P, Q, A, B, aux_in, \
              comp_in, out, aux_out, comp_out, cfl, flags, stream
   if (char_decomp == 2) {
     switch (rp.kind) {

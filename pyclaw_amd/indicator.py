@@ -134,63 +134,78 @@ def pick_threshold(z, label, recall=0.995):
 
 
 def swe_problem(seed, mx, my):
-    """A seeded shallow-water initial state (host, Fortran order (3, mx, my)):
-    a circular or planar dam break or a smooth bump on a random background."""
+    """Seeded shallow-water initial state on the periodic unit square (host,
+    Fortran order (3, mx, my)): h = 1 + 8 Gaussian humps (amplitude U[0.05, 0.3],
+    width U[0.02, 0.1]) + 2 circular dam breaks (radius U[0.05, 0.15], jump
+    U[0.1, 0.5]), hu = hv = 0.  Distances are periodic."""
     rng = np.random.default_rng(seed)
-    x = (np.arange(mx) + 0.5) / mx * 2 - 1
-    y = (np.arange(my) + 0.5) / my * 2 - 1
+    x = (np.arange(mx) + 0.5) / mx
+    y = (np.arange(my) + 0.5) / my
     X, Y = np.meshgrid(x, y, indexing="ij")
-    kind = seed % 3
-    h0 = rng.uniform(0.5, 1.5)
-    h1 = h0 + rng.uniform(0.2, 1.5) * rng.choice([-1, 1]) * (0.3 if kind == 2 else 1.0)
-    h1 = max(h1, 0.1)
-    cx, cy = rng.uniform(-0.4, 0.4, 2)
-    if kind == 0:
-        r = np.hypot(X - cx, Y - cy)
-        h = np.where(r < rng.uniform(0.15, 0.5), h1, h0)
-    elif kind == 1:
-        th = rng.uniform(0, np.pi)
-        h = np.where((X - cx) * np.cos(th) + (Y - cy) * np.sin(th) < 0, h1, h0)
-    else:
-        w = rng.uniform(0.1, 0.3)
-        h = h0 + (h1 - h0) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (w * w))
-    u = rng.uniform(-0.3, 0.3)
-    v = rng.uniform(-0.3, 0.3)
-    q = np.empty((3, mx, my), order="F")
+
+    def r2(cx, cy):
+        ddx = (X - cx + 0.5) % 1.0 - 0.5
+        ddy = (Y - cy + 0.5) % 1.0 - 0.5
+        return ddx * ddx + ddy * ddy
+
+    h = np.ones((mx, my))
+    for _ in range(8):
+        amp, width = rng.uniform(0.05, 0.3), rng.uniform(0.02, 0.1)
+        cx, cy = rng.uniform(0, 1, 2)
+        h += amp * np.exp(-r2(cx, cy) / (width * width))
+    for _ in range(2):
+        rad, jump = rng.uniform(0.05, 0.15), rng.uniform(0.1, 0.5)
+        cx, cy = rng.uniform(0, 1, 2)
+        h += np.where(r2(cx, cy) < rad * rad, jump, 0.0)
+    q = np.zeros((3, mx, my), order="F")
     q[0] = h
-    q[1] = h * u
-    q[2] = h * v
     return q
 
 
-def generate_training_data(seeds, mx=256, my=256, nsteps=60, snapshots=(10, 30, 60),
-                           flat=1e-6, wtol=0.05, keep_negative=0.25, device="cuda"):
-    """Run full characteristic WENO5 SharpClaw (FP64, SSP33) on seeded SWE
-    problems and collect features/labels at the given step counts.  All troubled
-    samples are kept, a `keep_negative` fraction of the others."""
-    from . import BC, Domain, SharpClawSolver2D, Solution, riemann
+def swe_solver(precision="fp64", indicator=None, cfl=0.5, cfl_max=0.55):
+    """SharpClawSolver2D for the periodic shallow-water family: characteristic
+    WENO5, SSP33, CFL 0.5 (cfl_max 0.55)."""
+    from . import BC, SharpClawSolver2D, riemann
+    solver = SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
+    solver.time_integrator = "SSP33"
+    solver.cfl_desired = cfl
+    solver.cfl_max = cfl_max
+    solver.char_decomp = 2
+    solver.all_bcs = BC.periodic
+    solver.precision = precision
+    solver.indicator = indicator
+    return solver
+
+
+def swe_solution(seed, mx, my):
+    from . import Domain, Solution
+    domain = Domain([0.0, 0.0], [1.0, 1.0], [mx, my])
+    sol = Solution(3, domain)
+    sol.problem_data["grav"] = 1.0
+    sol.problem_data["efix"] = True
+    sol.q = swe_problem(seed, mx, my)
+    return sol
+
+
+def generate_training_data(seeds, mx=512, my=512, nsteps=150, snapshots=(30, 90, 150),
+                           flat=1e-6, wtol=0.05, keep_negative=0.1, device="cuda"):
+    """Run full characteristic WENO5 SharpClaw (FP64, SSP33) on the seeded
+    shallow-water family and collect features/labels after the given step
+    counts.  All troubled samples are kept, a `keep_negative` fraction of the
+    others."""
     feats, labels = [], []
     gen = torch.Generator(device=device)
     gen.manual_seed(12345)
     for seed in seeds:
-        solver = SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
-        solver.time_integrator = "SSP33"
-        solver.cfl_max = 1.0
-        solver.cfl_desired = 0.9
-        solver.char_decomp = 2
-        solver.all_bcs = BC.extrap
-        domain = Domain([-1.0, -1.0], [1.0, 1.0], [mx, my])
-        sol = Solution(3, domain)
-        sol.problem_data["grav"] = 1.0
-        sol.problem_data["efix"] = True
-        sol.q = swe_problem(seed, mx, my)
-        solver.dt = 1e-4
+        solver = swe_solver("fp64")
+        sol = swe_solution(seed, mx, my)
+        solver.dt = 0.5 / mx / 1.5
         solver.setup(sol)
         for step in range(1, nsteps + 1):
             solver.evolve_to_time(sol)
             if step in snapshots:
                 q, _ = sol.state.device_q(K.L.F64)
-                f, l = features(q, [1, 1], [1, 1], flat, wtol)
+                f, l = features(q, [2, 2], [2, 2], flat, wtol)
                 keep = l.bool() | (torch.rand(l.shape, device=l.device, generator=gen)
                                    < keep_negative)
                 feats.append(f[keep])
@@ -203,12 +218,12 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
-    ap.add_argument("--runs", type=int, default=48)
-    ap.add_argument("--mx", type=int, default=256)
+    ap.add_argument("--runs", type=int, default=256)
+    ap.add_argument("--mx", type=int, default=512)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--recall", type=float, default=0.999)
     args = ap.parse_args(argv)
-    feat, label = generate_training_data(range(args.runs), args.mx, args.mx)
+    feat, label = generate_training_data(range(1000, 1000 + args.runs), args.mx, args.mx)
     n = label.numel()
     perm = torch.randperm(n, device=feat.device)
     ntr = int(0.9 * n)

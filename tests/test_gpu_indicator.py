@@ -8,16 +8,23 @@ import torch
 
 from oracle import mlp as M
 from oracle import oracle as O
-from tests._cases import seeded_state
 
 pytestmark = pytest.mark.gpu
 
 
 def _swe_state(rng, mx, my):
-    """Smooth background plus two jumps (one along x, one along y)."""
-    q = seeded_state(rng, O.SHALLOW, (mx, my))
-    q[0, mx // 3:, :] += 0.7
-    q[:, :, 2 * my // 3:] *= 1.3
+    """Smooth shallow-water state with seeded phases plus two jumps (one along
+    x, one along y): some interfaces troubled, most not."""
+    x = (np.arange(mx) + 0.5) / mx
+    y = (np.arange(my) + 0.5) / my
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    a, b, c = rng.uniform(0, 2 * np.pi, 3)
+    h = 1.0 + 0.2 * np.sin(2 * np.pi * X + a) * np.cos(2 * np.pi * Y + b)
+    h = h + 0.5 * (X > 0.37) + 0.3 * (Y > 0.61)
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = h
+    q[1] = h * 0.2 * np.cos(2 * np.pi * Y + c)
+    q[2] = h * 0.1 * np.sin(2 * np.pi * X + c)
     return q
 
 
@@ -167,7 +174,7 @@ def _advection_dq(q, flags, dt, dx, dy, u, v):
         QR = np.zeros(mx + 2)
         for c in range(1, mx + 2):
             f = fx[j, c - 1] | fx[j, max(c - 2, 0)] | fx[j, min(c, mx)]
-            a = c - 1 + ng                          # 0-based array index of cell c-1
+            a = c - 2 + ng                          # 0-based array index of cell c-1
             QR[c] = qr[0, a] if f else lin(row[0, a - 2:a + 3])
         dq[0, :, j] -= dt / dx * u * (QR[2:mx + 2] - QR[1:mx + 1])
     for i in range(mx):
@@ -176,7 +183,7 @@ def _advection_dq(q, flags, dt, dx, dy, u, v):
         QT = np.zeros(my + 2)
         for r in range(1, my + 2):
             f = fy[r - 1, i] | fy[max(r - 2, 0), i] | fy[min(r, my), i]
-            a = r - 1 + ng
+            a = r - 2 + ng
             QT[r] = qt[0, a] if f else lin(col[0, a - 2:a + 3])
         dq[0, i, :] -= dt / dy * v * (QT[2:my + 2] - QT[1:my + 1])
     return dq

@@ -35,6 +35,11 @@ def test_operator_bit_exact(kind, cd):
         if kind == O.ADVECTION:
             bcl, bcu = [2, 1], [2, 1]
         q = seeded_state(rng, kind, (mx, my))
+        if kind == O.EULER:
+            # cell-to-cell noise of the full admissible range lets WENO5 edge
+            # states go negative in pressure (NaN sound speeds); keep it milder
+            q = 0.85 * seeded_state(np.random.default_rng(0), O.EULER, (1, 1)).mean(
+                axis=(1, 2))[:, None, None] + 0.15 * q
         ng = 3
         qbc = O.with_ghosts(q, ng)
         O.bc2(qbc, ng, bcl, bcu)
@@ -42,6 +47,7 @@ def test_operator_bit_exact(kind, cd):
         dt = 0.2 * min(dx, dy)
         dq, cref = O.sharp_flux2(_orp(kind), qbc, ng, dx, dy, dt, cd)
         ref = O.interior(dq, ng)
+        assert np.isfinite(ref).all() and np.isfinite(cref)
         qi = q_to_device(q)
         out = torch.empty_like(qi)
         cfl = torch.zeros(1, dtype=torch.float64, device="cuda")
