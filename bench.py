@@ -52,6 +52,7 @@ def quadrant_state(mx=MX, my=MY, seed=2, gamma=1.4):
 
 
 def make_problem(precision, mx=MX, my=MY):
+    """ClawSolver2D + Solution of the headline workload."""
     import pyclaw_amd as pyclaw
     from pyclaw_amd import riemann
     solver = pyclaw.ClawSolver2D(riemann.euler_4wave_2D)
@@ -154,12 +155,139 @@ def cpu_baseline(sample_steps=3):
                       f"{wall:.1f} s"}
 
 
+def ensemble_leg(steps=32, members=512, seed=4):
+    """512 independent 2D Euler Roe/MC unsplit problems (sides log-uniform in
+    [64, 512], independent in x and y; seeded quadrant states as the headline
+    workload), FP32 + Kahan, each with its own CFL-controlled dt, advanced by
+    shared graph-captured launches; slices per wavefront autotuned."""
+    import torch
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd.ensemble import Ensemble
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(64), np.log(512), (members, 2)))).astype(int)
+    sols = []
+    for k, (mx, my) in enumerate(sides):
+        sol = pyclaw.Solution(4, pyclaw.Domain([0., 0.], [1., 1.], [int(mx), int(my)]))
+        sol.problem_data["gamma"] = 1.4
+        sol.q = quadrant_state(int(mx), int(my), seed=1000 + k)
+        sols.append(sol)
+    solver, _ = make_problem("fp32k", 64, 64)
+    solver.dt_initial = 0.8 * (1.0 / 512) / 3.0
+    ens = Ensemble(solver, sols)
+    timing = ens.autotune()
+    ens.run_steps(16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run_steps(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = ens.cells
+    acc = sum(c.accepted for c in ens.controllers())
+    return {"workload": "ensemble_512_euler2d_roe_mc_unsplit_fp32k_loguniform_64_512",
+            "value": cells * steps / wall, "unit": "cell-updates/s", "members": members,
+            "cells": cells, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+            "rows_per_wavefront": ens.rows,
+            "autotune_ms": {str(k): 1e3 * v for k, v in timing.items()},
+            "accepted_fraction": acc / float(members * (steps + 16)),
+            "hbm_frac": BYTES_PER_CELL_F32K * cells / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+
+
+def euler3d_leg(steps=10, n=384, seed=5):
+    """3D Euler, Roe, dimension-split, MC, 384^3, extrapolation BCs: seeded
+    spherical blast (radius U[0.1,0.2], inner p U[5,10], outer p 0.1, rho 1 with
+    1e-3 seeded noise) on [-1,1]^3; FP64 and FP32 + Kahan."""
+    import torch
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    rng = np.random.default_rng(seed)
+    r0, pin = rng.uniform(0.1, 0.2), rng.uniform(5.0, 10.0)
+    x = -1 + (np.arange(n) + 0.5) * (2.0 / n)
+    out = {}
+    q = None
+    for prec in ("fp64", "fp32k"):
+        solver = pyclaw.ClawSolver3D(riemann.euler_3D)
+        solver.limiters = pyclaw.limiters.tvd.MC
+        solver.all_bcs = pyclaw.BC.extrap
+        solver.cfl_desired, solver.cfl_max = 0.8, 1.0
+        solver.precision = prec
+        sol = pyclaw.Solution(5, pyclaw.Domain([-1.] * 3, [1.] * 3, [n] * 3))
+        sol.problem_data["gamma"] = 1.4
+        if q is None:
+            X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+            p = np.where(X * X + Y * Y + Z * Z < r0 * r0, pin, 0.1)
+            rho = 1.0 + 1e-3 * rng.uniform(-1, 1, X.shape)
+            q = np.zeros((5, n, n, n), order="F")
+            q[0] = rho
+            q[4] = p / 0.4
+            del X, Y, Z, p, rho
+        sol.q = q
+        solver.dt = 0.8 * (2.0 / n) / np.sqrt(1.4 * pin)
+        solver.setup(sol)
+        solver.evolve_to_time(sol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solver.evolve_to_time(sol)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[prec] = {"value": n ** 3 * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "hbm_frac": 240.0 * n ** 3 / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+        out[prec + "_q"] = sol.state.q
+    qa, qb = out.pop("fp64_q"), out.pop("fp32k_q")
+    out["fp32k"]["rel_l1_vs_fp64"] = float(np.abs(qb - qa).sum() / np.abs(qa).sum())
+    out.update({"workload": "euler3d_blast_roe_mc_split_384cubed", "unit": "cell-updates/s",
+                "steps": steps, "bytes_per_cell_step": 240})
+    return out
+
+
+def sharpclaw_leg(steps=20, n=2048, seed=3):
+    """2D shallow water on the periodic unit square, 2048^2: h = 1 + 8 seeded
+    Gaussian humps + 2 seeded circular dam breaks, hu = hv = 0; SharpClaw
+    characteristic WENO5 + SSP33, CFL 0.5.  FP64 full WENO5 against FP32 + Kahan
+    with the learned troubled-interface indicator (shipped weights)."""
+    import torch
+    from pyclaw_amd import indicator as I
+    res = {}
+    qs = {}
+    for name, prec, ind in (("fp64_weno", "fp64", None), ("fp32k_weno", "fp32k", None),
+                            ("fp32k_mlp", "fp32k", "shipped")):
+        indicator = I.TroubledInterfaceIndicator() if ind else None
+        solver = I.swe_solver(prec, indicator)
+        sol = I.swe_solution(seed, n, n)
+        solver.dt = 0.5 / n / 1.6
+        solver.setup(sol)
+        solver.evolve_to_time(sol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        frac = []
+        for k in range(steps):
+            solver.evolve_to_time(sol)
+            if indicator is not None and k % 5 == 0:
+                frac.append(indicator.troubled_fraction())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res[name] = {"value": n * n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "t_end": sol.t}
+        if frac:
+            res[name]["troubled_fraction"] = float(np.mean(frac))
+        qs[name] = sol.state.q
+    ref = qs["fp64_weno"]
+    for name in ("fp32k_weno", "fp32k_mlp"):
+        res[name]["rel_l1_vs_fp64"] = float(np.abs(qs[name] - ref).sum() / np.abs(ref).sum())
+    res["speedup_fp32k_mlp_vs_fp64_weno"] = res["fp32k_mlp"]["value"] / res["fp64_weno"]["value"]
+    res.update({"workload": "swe2d_periodic_humps_dambreaks_sharpclaw_weno5char_ssp33_2048",
+                "unit": "cell-updates/s", "steps": steps})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extras", default="ensemble,sharpclaw,euler3d",
+                    help="comma list of extra workloads to time beside the headline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X in this build "
@@ -202,6 +330,13 @@ def main():
                        "rel_l1_vs_fp64": err_l1, "rel_max_vs_fp64": err_max,
                        "t_end": sol32.t},
     }
+    legs = {"ensemble": ensemble_leg, "sharpclaw": sharpclaw_leg, "euler3d": euler3d_leg}
+    for name in [x for x in args.extras.split(",") if x]:
+        try:
+            rec[name] = legs[name]()
+        except Exception as e:   # an extra leg never hides the headline line
+            rec[name] = {"error": repr(e)}
+        print(f"[bench] {name} done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline()
     print(json.dumps(rec))

@@ -95,11 +95,15 @@ int clawhip_cfl_reset(double *cfl_dev, void *stream);
  *   mthlim[num_waves] as _set_mthlim (classic/solver.py:162-171);
  *   q_in (geometry gin) -> q_out (geometry gout, ghosts = 0); distinct buffers;
  *   comp_in -> comp_out: Kahan compensation (CLAWHIP_F32K only; may alias);
+ *   work: a third buffer shaped like q_out (the step runs as an x-sweep into
+ *         work and a y-sweep into q_out: work holds the reference's qnew
+ *         between its two sweep loops);
  *   *cfl_dev = max(*cfl_dev, Courant number of this step) (the `cfl` output).
  * reflect[d] is solver.reflect_index[d] (classic/solver.py:445). */
 int clawhip_step2_unsplit(const clawhip_rp *rp, int dtype, const void *q_in,
                           const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
-                          const void *comp_in, void *comp_out, const int *bc_lower, const int *bc_upper,
+                          const void *comp_in, void *comp_out, void *work,
+                          const int *bc_lower, const int *bc_upper,
                           const int *reflect, int num_ghost, const int *method,
                           const int *mthlim, double dx, double dy, double dt,
                           double *cfl_dev, void *stream);
@@ -111,12 +115,12 @@ int clawhip_step2_unsplit(const clawhip_rp *rp, int dtype, const void *q_in,
  * evolve_to_time(solution, tend).  Launches after ctl->done != 0 exit at once:
  * 1 reached tend, 2 CFL too large with fixed dt, 3 max_steps attempts made.
  * The launches contain no host synchronisation and may be captured in a graph.
- * Boundary codes 1..3. */
+ * Boundary codes 1..3; work as in clawhip_step2_unsplit. */
 int clawhip_step2_unsplit_run(const clawhip_rp *rp, int dtype, const clawhip_geom *g,
                               const int *bc_lower, const int *bc_upper, const int *reflect,
                               int num_ghost, const int *method, const int *mthlim, double dx,
                               double dy, void *q0, void *q1, void *comp0, void *comp1,
-                              clawhip_stepctl *ctl_dev, int nsteps, void *stream);
+                              void *work, clawhip_stepctl *ctl_dev, int nsteps, void *stream);
 
 /* One dimension-split 2D step: both sweeps of ClawSolver2D.step_hyperbolic with
  * dimensional_split=True (pyclaw/classic/solver.py:523-535), i.e. fmod.step2ds
@@ -174,6 +178,39 @@ int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
 /* The elementwise middle of SSP104 (sharpclaw/solver.py:418-419):
  *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */
 int clawhip_ssp104_mid(int dtype, void *q, void *s1, long long n, void *stream);
+
+/* ---- ensembles: many independent unsplit 2D problems of uneven sizes (one
+ * Riemann solver and method), each with its own dt and step controller,
+ * advanced by the same launches.  No counterpart in the reference (which steps
+ * one grid per Solver); each member's arithmetic is that of
+ * clawhip_step2_unsplit_run on its own. */
+typedef struct {
+  int mx, my;                  /* compact planar (meqn, my, mx) arrays            */
+  void *q0, *q1;               /* ping-pong state (current one: ctl->cur)         */
+  void *comp0, *comp1;         /* Kahan compensation (CLAWHIP_F32K), else NULL    */
+  void *work;                  /* shaped like q0                                  */
+  double dx, dy;
+  clawhip_stepctl *ctl;        /* this member's device controller                 */
+} clawhip_ens_member;
+
+/* Bytes of the device table clawhip_ensemble_prepare writes for n members. */
+long long clawhip_ensemble_table_bytes(int n);
+
+/* Validates the members and writes the launch table (member order, wavefront
+ * offsets, method) into table_dev (device memory of clawhip_ensemble_table_bytes
+ * bytes; a synchronous host-to-device copy).  rows: slices per wavefront (0 =
+ * chosen from the occupancy and the total work).  waves[2] receives the grid
+ * sizes of the two sweeps, to pass to clawhip_ensemble_run.  Boundary codes
+ * 1..3, num_ghost 2, method/mthlim as clawhip_step2_unsplit. */
+int clawhip_ensemble_prepare(const clawhip_rp *rp, int dtype, const clawhip_ens_member *members,
+                             int n, const int *bc_lower, const int *bc_upper, const int *reflect,
+                             int num_ghost, const int *method, const int *mthlim, int rows,
+                             void *table_dev, long long *waves);
+
+/* nsteps attempted steps of every member (2 launches per step; members whose
+ * controller is done skip).  Capturable in a graph. */
+int clawhip_ensemble_run(const clawhip_rp *rp, int dtype, const void *table_dev,
+                         const long long *waves, int nsteps, void *stream);
 
 /* ---- entry points on ghost-storing arrays: one-to-one with the reference's
  * f2py kernels, for a `solver.fmod` drop-in (pyclaw_amd/fmod).  The caller has

@@ -44,6 +44,15 @@ class StepCtl(ctypes.Structure):
                 ("cfl_bits", ctypes.c_ulonglong)]
 
 
+class EnsMember(ctypes.Structure):
+    """clawhip_ens_member"""
+    _fields_ = [("mx", ctypes.c_int), ("my", ctypes.c_int),
+                ("q0", ctypes.c_void_p), ("q1", ctypes.c_void_p),
+                ("comp0", ctypes.c_void_p), ("comp1", ctypes.c_void_p),
+                ("work", ctypes.c_void_p), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("ctl", ctypes.c_void_p)]
+
+
 class Geom(ctypes.Structure):
     _fields_ = [("mx", ctypes.c_int), ("my", ctypes.c_int), ("mz", ctypes.c_int),
                 ("ghosts", ctypes.c_int), ("field_stride", ctypes.c_longlong),
@@ -72,7 +81,7 @@ _SIGS = {
     "clawhip_device_count": [_IP],
     "clawhip_cfl_reset": [_P, _P],
     "clawhip_step2_unsplit": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
-                              ctypes.POINTER(Geom), _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
+                              ctypes.POINTER(Geom), _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                               _D, _D, _D, _P, _P],
     "clawhip_step2_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
@@ -85,13 +94,19 @@ _SIGS = {
                              _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_step2_unsplit_run": [ctypes.POINTER(RP), _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
-                                  _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _I, _P],
+                                  _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _P, _I, _P],
     "clawhip_step2_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _P, _P],
     "clawhip_step3_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _D, _P, _P],
     "clawhip_sharp2_flux": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _D, _D, _D, _P, _P],
+    "clawhip_ensemble_table_bytes": [_I],
+    "clawhip_ensemble_prepare": [ctypes.POINTER(RP), _I, ctypes.POINTER(EnsMember), _I, _IP,
+                                 _IP, _IP, _I, _IP, _IP, _I, _P,
+                                 ctypes.POINTER(ctypes.c_longlong)],
+    "clawhip_ensemble_run": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(ctypes.c_longlong), _I,
+                             _P],
     "clawhip_mlp_param_count": [],
     "clawhip_mlp_features": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _F, _P, _P,
                              _P],
@@ -100,6 +115,8 @@ _SIGS = {
     "clawhip_mlp_logits": [_P, ctypes.c_longlong, _P, _P, _P],
     "clawhip_mlp_flags": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _P, _F, _P, _P],
 }
+
+_LONG_RESULT = {"clawhip_ensemble_table_bytes"}
 
 _lib = None
 
@@ -116,7 +133,7 @@ def lib():
         for name, argtypes in _SIGS.items():
             f = getattr(L, name)
             f.argtypes = argtypes
-            f.restype = ctypes.c_int
+            f.restype = ctypes.c_longlong if name in _LONG_RESULT else ctypes.c_int
         _lib = L
     return _lib
 

@@ -1,297 +1,102 @@
-// classic2.hip -- 2D classic Clawpack wave-propagation step for gfx950.
+// classic2.hip -- 2D classic Clawpack unsplit wave-propagation step for gfx950.
 //
-// k_unsplit: one fused kernel for the unsplit step (classic/step2.f90 +
-// flux2.f90 + limiter.f90 + philim.f90) including the ghost-cell fill
-// (solver.py:351-519, applied as an index map on load) and the CFL max-reduction.
+// The unsplit step (classic/step2.f90 + flux2.f90 + limiter.f90 + philim.f90,
+// with the ghost-cell fill of solver.py:351-519 applied as an index map on load
+// and the CFL max-reduction) runs as two launches of one sweep kernel:
 //
-// Work decomposition (wave64):
-//   * a wavefront owns a strip of 64 columns c0-2 .. c0+61 and writes the 60
-//     columns c0 .. c0+59; lanes 0,1,62,63 only supply neighbour data.
-//   * it marches down rows j0 .. j0+H-1.  x-interfaces of a row are computed
-//     lane-parallel (neighbour waves arrive by cross-lane moves); y-interfaces
-//     are lane-local, kept in a register window so each one is solved once.
-//   * every cell update applies the six contributions in the reference's order:
-//     x-slice j-1 (+dtdy*g2), x-slice j (main), x-slice j+1 (-dtdy*g1), then
-//     y-slice i-1, y-slice i, y-slice i+1 (step2.f90:136-145, 218-228).
+//   k_usweep<AX = 1>: the x-slices j = 0..my+1 (step2.f90:96-163).  Their
+//       updates of each cell are summed, in the reference's order, into `work`:
+//       exactly the reference's qnew after its x-sweep.
+//   k_usweep<AX = 2>: the y-slices i = 0..mx+1 (step2.f90:169-247), continuing
+//       each cell's sum from `work` and writing q_new.
+//
+// Both sweeps are the same code with the roles of x and y exchanged.  A
+// wavefront owns 64 consecutive cells of a slice (lanes 0,1,62,63 only supply
+// neighbour data, so 60 cells are written) and marches H slices across.  All
+// interfaces of a slice are solved lane-parallel: neighbour states, waves and
+// fluctuations arrive by cross-lane moves, and no state is carried between
+// slices except the three running sums of the cells the slice touches
+// (slices s-1, s, s+1 update cell s).  With only the slice's temporaries live,
+// the kernel needs far fewer registers than a fused x+y march, so several
+// wavefronts share each SIMD and hide the FP64 latency of the Riemann solvers.
+//
+// Every cell sums its six contributions in the reference's order and
+// parenthesisation:
+//   after x:  ((((q + dtdy*g2[j-1]) + qadd) - dtdx*dF) - dtdy*dG) - dtdy*g1[j+1]
+//   after y:  ((w + dtdx*g2[i-1]) + ((qadd - dtdy*dF) - dtdx*dG)) - dtdx*g1[i+1]
 // FP64: bitwise equal to the reference step (built with -ffp-contract=off).
-// FP32 (KAHAN): the same increment summed in FP32 and applied with a
-// Kahan-compensated update (compensation array alongside q).
+// FP32 (KAHAN): the same sums start from 0 instead of q, and the total increment
+// is applied with a Kahan-compensated update (compensation array alongside q).
+#include <stdlib.h>
+
 #include "launch.hpp"
+#include "usweep.hpp"
 
 namespace claw {
 
-template <typename T>
-struct UnsplitArgs {
-  RPParams<T> rp;
-  Geom gin, gout;        // input (any ghosts) / output (compact) geometry
-  BCSpec bc;
-  int ng, order, trans;
-  int mthlim[4];
-  T dtdx, dtdy;
-  int H, nstrips, nrb;
-  const T* qin;
-  T* qout;
-  const T* comp_in;              // Kahan compensation (FP32 path), output geometry
-  T* comp_out;
-  unsigned long long* cfl_bits;  // FP64 max CFL (bit pattern)
-};
-
-// Per-row result of the x-pass for the cell owned by this lane.
-template <int ME, typename T>
-struct XRow {
-  T qadd[ME], Fl[ME], Fr[ME], g1[ME], g2[ME];
-};
-
-// x-interfaces of one row (flux2.f90 with ixy = 1) for the cell owned by the lane.
-// Qc = state of this lane's cell; the left neighbour comes from lane-1.
-template <class RS, typename T>
-__device__ __forceinline__ void xpass(const UnsplitArgs<T>& a, const T (&Qc)[RS::meqn],
-                                      bool cfl_ok, T& smax, XRow<RS::meqn, T>& o) {
-  constexpr int ME = RS::meqn, MW = RS::mwaves;
-  T QL[ME];
-  from_left(Qc, QL);
-  T W[MW][ME], s[MW], am[ME], ap[ME];
-  typename RS::Roe roe;
-  RS::template rpn<1>(a.rp, QL, Qc, W, s, am, ap, roe);
-  if (cfl_ok) {
-#pragma unroll
-    for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(s[w]));
-  }
-  T F[ME], cq[ME];
-#pragma unroll
-  for (int m = 0; m < ME; m++) { F[m] = T(0); cq[m] = T(0); }
-  if (a.order != 1) {
-    T Wn[MW][ME];
-#pragma unroll
-    for (int w = 0; w < MW; w++) from_right(W[w], Wn[w]);
-    T dotr[MW], dotl[MW];
-    wave_dots<ME, MW>(W, Wn, dotr);
-    from_left(dotr, dotl);
-    limit_waves<ME, MW>(W, s, dotl, dotr, a.mthlim);
-    second_order<ME, MW>(W, s, a.dtdx, cq, F);
-  }
-  T amn[ME];
-  from_right(am, amn);
-  from_right(F, o.Fr);
-#pragma unroll
-  for (int m = 0; m < ME; m++) {
-    o.Fl[m] = F[m];
-    o.qadd[m] = (T(0) - a.dtdx * ap[m]) - a.dtdx * amn[m];
-    o.g1[m] = T(0);
-    o.g2[m] = T(0);
-  }
-  if (a.trans > 0) {
-    T amm[ME], apm[ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      bool cor = a.order > 1 && a.trans == 2;
-      amm[m] = cor ? am[m] + cq[m] : am[m];
-      apm[m] = cor ? ap[m] - cq[m] : ap[m];
-    }
-    T bmA[ME], bpA[ME], bmP[ME], bpP[ME];
-    RS::template rpt<1>(a.rp, roe, amm, bmA, bpA);
-    RS::template rpt<1>(a.rp, roe, apm, bmP, bpP);
-    T bmAn[ME], bpAn[ME];
-    from_right(bmA, bmAn);
-    from_right(bpA, bpAn);
-    const T h = T(0.5) * a.dtdx;
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      o.g1[m] = (T(0) - h * bmAn[m]) - h * bmP[m];
-      o.g2[m] = (T(0) - h * bpAn[m]) - h * bpP[m];
-    }
-  }
+// 3 wavefronts per SIMD: the FP64 Euler sweep fits 168 VGPRs (smaller solvers
+// need fewer and run more wavefronts).
+template <class RS, typename T, bool KAHAN, int AX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_usweep(const USweepArgs<T> a) {
+  usweep_body<RS, T, KAHAN, AX>(a, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
-template <class RS, typename T, bool KAHAN>
-__device__ __forceinline__ void unsplit_body(const UnsplitArgs<T>& a) {
-  constexpr int ME = RS::meqn, MW = RS::mwaves;
-  using Roe = typename RS::Roe;
-  const int lane = threadIdx.x & 63;
-  const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wid >= a.nstrips * a.nrb) return;   // uniform per wavefront
-  const int strip = wid % a.nstrips, rb = wid / a.nstrips;
-  const int mx = a.gout.mx, my = a.gout.my;
-  const int c = 1 + strip * 60 - 2 + lane;
-  const int j0 = 1 + rb * a.H;
-  const int jend = min(j0 + a.H, my + 1);
-  const bool out_col = lane >= 2 && lane < 62 && c >= 1 && c <= mx;
-  const bool cflx_col = lane >= 1 && c >= 1 && c <= mx + 1;   // lane 0 has no left state
-  const bool cfly_col = c >= 0 && c <= mx + 1;
-  const T dtdx = a.dtdx, dtdy = a.dtdy, hy = T(0.5) * a.dtdy;
-  T smx = T(0), smy = T(0);
-
-  T Qm[ME], Q0[ME], Q1[ME];
-  XRow<ME, T> xr;
-  // ---- warm-up: rows j0-2, j0-1, j0; x-pass of row j0-1; y-interfaces j0-1, j0
-  load_cell2<ME>(a.qin, a.gin, a.bc, a.ng, c, j0 - 2, Qm);
-  load_cell2<ME>(a.qin, a.gin, a.bc, a.ng, c, j0 - 1, Q0);
-  load_cell2<ME>(a.qin, a.gin, a.bc, a.ng, c, j0, Q1);
-  xpass<RS>(a, Q0, cflx_col && j0 - 1 >= 0, smx, xr);
-  T acc_prev[ME], acc_cur[ME], qold_prev[ME], qold_cur[ME];
-#pragma unroll
-  for (int m = 0; m < ME; m++) {
-    acc_cur[m] = (KAHAN ? T(0) : Q1[m]) + dtdy * xr.g2[m];
-    qold_cur[m] = Q1[m];
-    acc_prev[m] = T(0);
-    qold_prev[m] = T(0);
-  }
-  T Wt[MW][ME], st[MW], amt[ME], apt[ME], dotl[MW];
-  Roe roet;
-  {
-    T Wp[MW][ME], sp[MW], amp[ME], app[ME];
-    Roe roep;
-    RS::template rpn<2>(a.rp, Qm, Q0, Wp, sp, amp, app, roep);
-    if (cfly_col && j0 - 1 >= 1) {
-#pragma unroll
-      for (int w = 0; w < MW; w++) smy = dmax(smy, fabs(sp[w]));
-    }
-    RS::template rpn<2>(a.rp, Q0, Q1, Wt, st, amt, apt, roet);
-    if (cfly_col && j0 <= my + 1) {
-#pragma unroll
-      for (int w = 0; w < MW; w++) smy = dmax(smy, fabs(st[w]));
-    }
-    wave_dots<ME, MW>(Wp, Wt, dotl);
-  }
-#pragma unroll
-  for (int m = 0; m < ME; m++) Q0[m] = Q1[m];
-  T qaddy_p[ME], hbm_p[ME], hbp_p[ME], Fy_p[ME];
-#pragma unroll
-  for (int m = 0; m < ME; m++) { qaddy_p[m] = T(0); hbm_p[m] = T(0); hbp_p[m] = T(0); Fy_p[m] = T(0); }
-
-  for (int t = j0; t <= jend; t++) {
-    load_cell2<ME>(a.qin, a.gin, a.bc, a.ng, c, t + 1, Q1);
-    // ---- x-slice t
-    xpass<RS>(a, Q0, cflx_col && t <= my + 1, smx, xr);
-    T acc_next[ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      acc_prev[m] = acc_prev[m] - dtdy * xr.g1[m];
-      acc_cur[m] = ((acc_cur[m] + xr.qadd[m]) - dtdx * (xr.Fr[m] - xr.Fl[m])) - dtdy * (xr.g2[m] - xr.g1[m]);
-      acc_next[m] = (KAHAN ? T(0) : Q1[m]) + dtdy * xr.g2[m];
-    }
-    // ---- y-interface t+1 (rows t, t+1); limit interface t
-    T Wn[MW][ME], sn[MW], amn[ME], apn[ME], dotn[MW];
-    Roe roen;
-    RS::template rpn<2>(a.rp, Q0, Q1, Wn, sn, amn, apn, roen);
-    if (cfly_col && t + 1 <= my + 1) {
-#pragma unroll
-      for (int w = 0; w < MW; w++) smy = dmax(smy, fabs(sn[w]));
-    }
-    T Fy[ME], cq[ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) { Fy[m] = T(0); cq[m] = T(0); }
-    if (a.order != 1) {
-      wave_dots<ME, MW>(Wt, Wn, dotn);
-      limit_waves<ME, MW>(Wt, st, dotl, dotn, a.mthlim);
-      second_order<ME, MW>(Wt, st, a.dtdy, cq, Fy);
-    }
-    T bmA[ME], bpA[ME], bmP[ME], bpP[ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) { bmA[m] = T(0); bpA[m] = T(0); bmP[m] = T(0); bpP[m] = T(0); }
-    if (a.trans > 0) {
-      T amm[ME], apm[ME];
-#pragma unroll
-      for (int m = 0; m < ME; m++) {
-        bool cor = a.order > 1 && a.trans == 2;
-        amm[m] = cor ? amt[m] + cq[m] : amt[m];
-        apm[m] = cor ? apt[m] - cq[m] : apt[m];
-      }
-      RS::template rpt<2>(a.rp, roet, amm, bmA, bpA);
-      RS::template rpt<2>(a.rp, roet, apm, bmP, bpP);
-    }
-    if (t > j0) {
-      // row t-1 is complete: interface t-1 (apdq side, carried) and t (amdq side)
-      T g1y[ME], g2y[ME], g2l[ME], g1r[ME];
-#pragma unroll
-      for (int m = 0; m < ME; m++) {
-        g1y[m] = (T(0) - hy * bmA[m]) - hbm_p[m];
-        g2y[m] = (T(0) - hy * bpA[m]) - hbp_p[m];
-      }
-      from_left(g2y, g2l);
-      from_right(g1y, g1r);
-      T upd[ME];
-#pragma unroll
-      for (int m = 0; m < ME; m++) {
-        T qaddy = qaddy_p[m] - dtdy * amt[m];
-        T v = acc_prev[m] + dtdx * g2l[m];
-        v = v + ((qaddy - dtdy * (Fy[m] - Fy_p[m])) - dtdx * (g2y[m] - g1y[m]));
-        upd[m] = v - dtdx * g1r[m];
-      }
-      if (out_col) {
-        T* po = cell_ptr2(a.qout, a.gout, c, t - 1);
-        if (KAHAN) {
-          const T* pci = cell_ptr2(a.comp_in, a.gout, c, t - 1);
-          T* pco = cell_ptr2(a.comp_out, a.gout, c, t - 1);
-#pragma unroll
-          for (int m = 0; m < ME; m++) {
-            T y = upd[m] - pci[(long long)m * a.gout.fs];
-            T tq = qold_prev[m] + y;
-            pco[(long long)m * a.gout.fs] = (tq - qold_prev[m]) - y;
-            po[(long long)m * a.gout.fs] = tq;
-          }
-        } else {
-#pragma unroll
-          for (int m = 0; m < ME; m++) po[(long long)m * a.gout.fs] = upd[m];
-        }
-      }
-    }
-    // ---- carry: row t partials from interface t (apdq side); shift windows
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      qaddy_p[m] = T(0) - dtdy * apt[m];
-      hbm_p[m] = hy * bmP[m];
-      hbp_p[m] = hy * bpP[m];
-      Fy_p[m] = Fy[m];
-      amt[m] = amn[m];
-      apt[m] = apn[m];
-      acc_prev[m] = acc_cur[m];
-      acc_cur[m] = acc_next[m];
-      qold_prev[m] = qold_cur[m];
-      qold_cur[m] = Q1[m];
-      Q0[m] = Q1[m];
-    }
-#pragma unroll
-    for (int w = 0; w < MW; w++) {
-      st[w] = sn[w];
-      dotl[w] = dotn[w];
-#pragma unroll
-      for (int m = 0; m < ME; m++) Wt[w][m] = Wn[w][m];
-    }
-    roet = roen;
-  }
-  double cfl = (double)a.dtdx * (double)smx;
-  double cy = (double)a.dtdy * (double)smy;
-  if (cy > cfl) cfl = cy;
-  wave_atomic_max(cfl, a.cfl_bits);
-}
-
-template <class RS, typename T, bool KAHAN>
-__global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
-  unsplit_body<RS, T, KAHAN>(a);
-}
-
-// Device-resident stepping: one launch = one attempted step of
+// Device-resident stepping: the two sweeps of one attempted step of
 // Solver.evolve_to_time.  The state ping-pongs between two buffers; `ctl->cur`
-// names the current one.  The last workgroup to finish applies the CFL check,
-// accepts or rejects the step and computes the next dt (step_ctl_finish).
-template <class RS, typename T, bool KAHAN>
-__global__ void __launch_bounds__(256) k_unsplit_ctl(UnsplitArgs<T> a, StepCtl* ctl,
-                                                     CtlBuffers bufs) {
+// names the current one.  The last wavefront of the y-sweep applies the CFL
+// check, accepts or rejects the step and computes the next dt (step_ctl_finish).
+template <class RS, typename T, bool KAHAN, int AX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_usweep_ctl(USweepArgs<T> a, StepCtl* ctl, CtlBuffers bufs) {
   if (ctl->done) return;   // uniform across the grid
   const int cur = ctl->cur;
   const double dt = ctl->dt;
   a.qin = (const T*)bufs.q[cur];
-  a.qout = (T*)bufs.q[1 - cur];
+  if (AX == 1) {
+    a.wout = (T*)bufs.work;
+    a.dn = T(dt / bufs.dx);
+    a.dtr = T(dt / bufs.dy);
+  } else {
+    a.win = (const T*)bufs.work;
+    a.wout = (T*)bufs.q[1 - cur];
+    a.comp_in = (const T*)bufs.comp[cur];
+    a.comp_out = (T*)bufs.comp[1 - cur];
+    a.dn = T(dt / bufs.dy);
+    a.dtr = T(dt / bufs.dx);
+  }
+  a.cfl_bits = &ctl->cfl_bits;
+  const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  usweep_body<RS, T, KAHAN, AX>(a, wid);
+  if (AX == 2) {
+    const int total = a.nstrips * a.nrb;
+    if (wid < total) step_ctl_finish(ctl, (unsigned)total);
+  }
+}
+
+template <class RS, typename T, bool KAHAN>
+__global__ void __launch_bounds__(256) k_ywin(const USweepArgs<T> a) {
+  ywin_body<RS, T, KAHAN>(a, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
+
+template <class RS, typename T, bool KAHAN>
+__global__ void __launch_bounds__(256) k_ywin_ctl(USweepArgs<T> a, StepCtl* ctl,
+                                                  CtlBuffers bufs) {
+  if (ctl->done) return;   // uniform across the grid
+  const int cur = ctl->cur;
+  const double dt = ctl->dt;
+  a.qin = (const T*)bufs.q[cur];
+  a.win = (const T*)bufs.work;
+  a.wout = (T*)bufs.q[1 - cur];
   a.comp_in = (const T*)bufs.comp[cur];
   a.comp_out = (T*)bufs.comp[1 - cur];
-  a.dtdx = T(dt / bufs.dx);
-  a.dtdy = T(dt / bufs.dy);
+  a.dn = T(dt / bufs.dy);
+  a.dtr = T(dt / bufs.dx);
   a.cfl_bits = &ctl->cfl_bits;
-  unsplit_body<RS, T, KAHAN>(a);
-  const unsigned total = (unsigned)(a.nstrips * a.nrb);
-  const unsigned wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wid < total) step_ctl_finish(ctl, total);
+  const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  ywin_body<RS, T, KAHAN>(a, wid);
+  const int total = a.nstrips * a.nrb;
+  if (wid < total) step_ctl_finish(ctl, (unsigned)total);
 }
 
 }  // namespace claw
@@ -300,29 +105,120 @@ __global__ void __launch_bounds__(256) k_unsplit_ctl(UnsplitArgs<T> a, StepCtl* 
 namespace claw {
 
 namespace {
-constexpr int kUnsplitRows = 16;   // rows per wavefront (march length)
+// Slices per wavefront (march length H).  Each wavefront recomputes 2 halo
+// slices, so long marches waste less; but the grid must also fill the device:
+// with R wavefronts resident at once (from the kernel's occupancy), a launch of
+// W wavefronts runs in ceil(W / R) rounds of H + 2 slice steps each.  H
+// minimises that product (ties: the longer march).  CLAWHIP_UNSPLIT_ROWS
+// overrides the choice (tuning sweeps).
+template <class Kernel>
+int choose_rows(Kernel kernel, int nstrips, int mm) {
+  static long long resident = 0;   // per kernel instantiation
+  if (resident == 0) {
+    int dev = 0, ncu = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess ||
+        ncu < 1 || nb < 1)
+      return 16;
+    resident = (long long)ncu * nb * 4;
+  }
+  if (const char* env = getenv("CLAWHIP_UNSPLIT_ROWS")) {
+    int h = atoi(env);
+    if (h >= 1 && h <= 4096) return h;
+  }
+  int best = 16;
+  long long best_cost = -1;
+  for (int h = 2; h <= 64; h++) {
+    const long long waves = (long long)nstrips * ((mm + h - 1) / h);
+    const long long cost = ((waves + resident - 1) / resident) * (long long)(h + 2);
+    if (best_cost < 0 || cost < best_cost || (cost == best_cost && h > best)) {
+      best = h;
+      best_cost = cost;
+    }
+    if ((mm + h - 1) / h == 1) break;
+  }
+  return best;
+}
+
+// y-sweep variant: lanes across the columns marching down (default) or lanes
+// along the slices (CLAWHIP_YSWEEP=lane).
+bool ysweep_lane() {
+  const char* e = getenv("CLAWHIP_YSWEEP");
+  return e && e[0] == 'l';
+}
 
 template <class RS, typename T, bool KAHAN>
-int run_unsplit(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_out,
-                const Geom& gout, const void* comp_in, void* comp_out, const int* lo, const int* hi, const int* refl,
-                int ng, const int* method, const int* mthlim, double dx, double dy, double dt,
-                unsigned long long* cfl_bits, hipStream_t stream) {
-  UnsplitArgs<T> a;
+USweepArgs<T> ywin_args(const clawhip_rp& rp, const Geom& gin, const Geom& gout, const int* lo,
+                        const int* hi, const int* refl, int ng, const int* method,
+                        const int* mthlim, bool ctl) {
+  USweepArgs<T> a;
   a.rp = to_params<T>(rp);
   a.gin = gin; a.gout = gout;
   a.bc = to_bc(lo, hi, refl, 2);
   a.ng = ng; a.order = method[1]; a.trans = method[2];
   for (int w = 0; w < 4; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
-  a.dtdx = T(dt / dx); a.dtdy = T(dt / dy);
-  a.H = kUnsplitRows;
+  a.dn = T(0); a.dtr = T(0);
   a.nstrips = (gout.mx + 59) / 60;
+  a.H = ctl ? choose_rows(k_ywin_ctl<RS, T, KAHAN>, a.nstrips, gout.my)
+            : choose_rows(k_ywin<RS, T, KAHAN>, a.nstrips, gout.my);
   a.nrb = (gout.my + a.H - 1) / a.H;
-  a.qin = (const T*)q_in; a.qout = (T*)q_out;
-  a.comp_in = (const T*)comp_in; a.comp_out = (T*)comp_out;
-  a.cfl_bits = cfl_bits;
-  long long waves = (long long)a.nstrips * a.nrb;
-  int blocks = (int)((waves + 3) / 4);
-  hipLaunchKernelGGL((k_unsplit<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, stream, a);
+  a.qin = nullptr; a.win = nullptr; a.wout = nullptr;
+  a.comp_in = nullptr; a.comp_out = nullptr; a.cfl_bits = nullptr;
+  return a;
+}
+
+template <class RS, typename T, bool KAHAN, int AX>
+USweepArgs<T> sweep_args(const clawhip_rp& rp, const Geom& gin, const Geom& gout,
+                         const int* lo, const int* hi, const int* refl, int ng,
+                         const int* method, const int* mthlim, bool ctl) {
+  USweepArgs<T> a;
+  a.rp = to_params<T>(rp);
+  a.gin = gin; a.gout = gout;
+  a.bc = to_bc(lo, hi, refl, 2);
+  a.ng = ng; a.order = method[1]; a.trans = method[2];
+  for (int w = 0; w < 4; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
+  a.dn = T(0); a.dtr = T(0);
+  const int ms = AX == 1 ? gout.mx : gout.my, mm = AX == 1 ? gout.my : gout.mx;
+  a.nstrips = (ms + 59) / 60;
+  a.H = ctl ? choose_rows(k_usweep_ctl<RS, T, KAHAN, AX>, a.nstrips, mm)
+            : choose_rows(k_usweep<RS, T, KAHAN, AX>, a.nstrips, mm);
+  a.nrb = (mm + a.H - 1) / a.H;
+  a.qin = nullptr; a.win = nullptr; a.wout = nullptr;
+  a.comp_in = nullptr; a.comp_out = nullptr; a.cfl_bits = nullptr;
+  return a;
+}
+
+template <typename T>
+unsigned sweep_blocks(const USweepArgs<T>& a) {
+  return (unsigned)(((long long)a.nstrips * a.nrb + 3) / 4);
+}
+
+template <class RS, typename T, bool KAHAN>
+int run_unsplit(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_out,
+                const Geom& gout, const void* comp_in, void* comp_out, void* work,
+                const int* lo, const int* hi, const int* refl, int ng, const int* method,
+                const int* mthlim, double dx, double dy, double dt,
+                unsigned long long* cfl_bits, hipStream_t stream) {
+  auto ax = sweep_args<RS, T, KAHAN, 1>(rp, gin, gout, lo, hi, refl, ng, method, mthlim, false);
+  ax.dn = T(dt / dx); ax.dtr = T(dt / dy);
+  ax.qin = (const T*)q_in; ax.wout = (T*)work; ax.cfl_bits = cfl_bits;
+  hipLaunchKernelGGL((k_usweep<RS, T, KAHAN, 1>), dim3(sweep_blocks(ax)), dim3(256), 0, stream,
+                     ax);
+  const bool lane = ysweep_lane();
+  auto ay = lane ? sweep_args<RS, T, KAHAN, 2>(rp, gin, gout, lo, hi, refl, ng, method, mthlim,
+                                               false)
+                 : ywin_args<RS, T, KAHAN>(rp, gin, gout, lo, hi, refl, ng, method, mthlim,
+                                           false);
+  ay.dn = T(dt / dy); ay.dtr = T(dt / dx);
+  ay.qin = (const T*)q_in; ay.win = (const T*)work; ay.wout = (T*)q_out;
+  ay.comp_in = (const T*)comp_in; ay.comp_out = (T*)comp_out; ay.cfl_bits = cfl_bits;
+  if (lane)
+    hipLaunchKernelGGL((k_usweep<RS, T, KAHAN, 2>), dim3(sweep_blocks(ay)), dim3(256), 0,
+                       stream, ay);
+  else
+    hipLaunchKernelGGL((k_ywin<RS, T, KAHAN>), dim3(sweep_blocks(ay)), dim3(256), 0, stream,
+                       ay);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
 }
@@ -331,23 +227,20 @@ template <class RS, typename T, bool KAHAN>
 int run_unsplit_ctl(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
                     const int* refl, int ng, const int* method, const int* mthlim,
                     StepCtl* ctl, const CtlBuffers& bufs, int nsteps, hipStream_t stream) {
-  UnsplitArgs<T> a;
-  a.rp = to_params<T>(rp);
-  a.gin = g; a.gout = g;
-  a.bc = to_bc(lo, hi, refl, 2);
-  a.ng = ng; a.order = method[1]; a.trans = method[2];
-  for (int w = 0; w < 4; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
-  a.dtdx = T(0); a.dtdy = T(0);
-  a.H = kUnsplitRows;
-  a.nstrips = (g.mx + 59) / 60;
-  a.nrb = (g.my + a.H - 1) / a.H;
-  a.qin = nullptr; a.qout = nullptr; a.comp_in = nullptr; a.comp_out = nullptr;
-  a.cfl_bits = nullptr;
-  long long waves = (long long)a.nstrips * a.nrb;
-  int blocks = (int)((waves + 3) / 4);
-  for (int n = 0; n < nsteps; n++)
-    hipLaunchKernelGGL((k_unsplit_ctl<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, stream, a,
-                       ctl, bufs);
+  auto ax = sweep_args<RS, T, KAHAN, 1>(rp, g, g, lo, hi, refl, ng, method, mthlim, true);
+  const bool lane = ysweep_lane();
+  auto ay = lane ? sweep_args<RS, T, KAHAN, 2>(rp, g, g, lo, hi, refl, ng, method, mthlim, true)
+                 : ywin_args<RS, T, KAHAN>(rp, g, g, lo, hi, refl, ng, method, mthlim, true);
+  for (int n = 0; n < nsteps; n++) {
+    hipLaunchKernelGGL((k_usweep_ctl<RS, T, KAHAN, 1>), dim3(sweep_blocks(ax)), dim3(256), 0,
+                       stream, ax, ctl, bufs);
+    if (lane)
+      hipLaunchKernelGGL((k_usweep_ctl<RS, T, KAHAN, 2>), dim3(sweep_blocks(ay)), dim3(256), 0,
+                         stream, ay, ctl, bufs);
+    else
+      hipLaunchKernelGGL((k_ywin_ctl<RS, T, KAHAN>), dim3(sweep_blocks(ay)), dim3(256), 0,
+                         stream, ay, ctl, bufs);
+  }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
 }
@@ -366,36 +259,34 @@ int dispatch_unsplit_ctl(int dtype, const clawhip_rp& rp, const Geom& g, const i
 
 template <template <typename> class RSF>
 int dispatch_unsplit(int dtype, const clawhip_rp& rp, const void* q_in, const Geom& gin,
-                     void* q_out, const Geom& gout, const void* comp_in, void* comp_out, const int* lo, const int* hi,
-                     const int* refl, int ng, const int* method, const int* mthlim, double dx,
-                     double dy, double dt, unsigned long long* cfl_bits, hipStream_t stream) {
+                     void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
+                     void* work, const int* lo, const int* hi, const int* refl, int ng,
+                     const int* method, const int* mthlim, double dx, double dy, double dt,
+                     unsigned long long* cfl_bits, hipStream_t stream) {
   if (dtype == CLAWHIP_F64)
-    return run_unsplit<RSF<double>, double, false>(rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi,
-                                                   refl, ng, method, mthlim, dx, dy, dt,
-                                                   cfl_bits, stream);
-  return run_unsplit<RSF<float>, float, true>(rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
-                                              ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
+    return run_unsplit<RSF<double>, double, false>(rp, q_in, gin, q_out, gout, comp_in,
+                                                   comp_out, work, lo, hi, refl, ng, method,
+                                                   mthlim, dx, dy, dt, cfl_bits, stream);
+  return run_unsplit<RSF<float>, float, true>(rp, q_in, gin, q_out, gout, comp_in, comp_out,
+                                              work, lo, hi, refl, ng, method, mthlim, dx, dy,
+                                              dt, cfl_bits, stream);
 }
 }  // namespace
 
 int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
-                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out, const int* lo, const int* hi,
-                   const int* refl, int ng, const int* method, const int* mthlim, double dx,
-                   double dy, double dt, unsigned long long* cfl_bits, hipStream_t stream) {
+                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
+                   void* work, const int* lo, const int* hi, const int* refl, int ng,
+                   const int* method, const int* mthlim, double dx, double dy, double dt,
+                   unsigned long long* cfl_bits, hipStream_t stream) {
+#define ARGS dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, work, lo, hi, refl, ng, \
+             method, mthlim, dx, dy, dt, cfl_bits, stream
   switch (rp.kind) {
-    case CLAWHIP_RP_ACOUSTICS:
-      return dispatch_unsplit<Acoustics>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
-                                         ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
-    case CLAWHIP_RP_EULER:
-      return dispatch_unsplit<Euler>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl, ng,
-                                     method, mthlim, dx, dy, dt, cfl_bits, stream);
-    case CLAWHIP_RP_SHALLOW:
-      return dispatch_unsplit<Shallow>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
-                                       ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
-    case CLAWHIP_RP_ADVECTION:
-      return dispatch_unsplit<Advection>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
-                                         ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
+    case CLAWHIP_RP_ACOUSTICS: return dispatch_unsplit<Acoustics>(ARGS);
+    case CLAWHIP_RP_EULER: return dispatch_unsplit<Euler>(ARGS);
+    case CLAWHIP_RP_SHALLOW: return dispatch_unsplit<Shallow>(ARGS);
+    case CLAWHIP_RP_ADVECTION: return dispatch_unsplit<Advection>(ARGS);
   }
+#undef ARGS
   return CLAWHIP_E_KIND;
 }
 
@@ -403,20 +294,14 @@ int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int
                        const int* hi, const int* refl, int ng, const int* method,
                        const int* mthlim, StepCtl* ctl, const CtlBuffers& bufs, int nsteps,
                        hipStream_t stream) {
+#define ARGS dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl, bufs, nsteps, stream
   switch (rp.kind) {
-    case CLAWHIP_RP_ACOUSTICS:
-      return dispatch_unsplit_ctl<Acoustics>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
-                                             bufs, nsteps, stream);
-    case CLAWHIP_RP_EULER:
-      return dispatch_unsplit_ctl<Euler>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
-                                         bufs, nsteps, stream);
-    case CLAWHIP_RP_SHALLOW:
-      return dispatch_unsplit_ctl<Shallow>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
-                                           bufs, nsteps, stream);
-    case CLAWHIP_RP_ADVECTION:
-      return dispatch_unsplit_ctl<Advection>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
-                                             bufs, nsteps, stream);
+    case CLAWHIP_RP_ACOUSTICS: return dispatch_unsplit_ctl<Acoustics>(ARGS);
+    case CLAWHIP_RP_EULER: return dispatch_unsplit_ctl<Euler>(ARGS);
+    case CLAWHIP_RP_SHALLOW: return dispatch_unsplit_ctl<Shallow>(ARGS);
+    case CLAWHIP_RP_ADVECTION: return dispatch_unsplit_ctl<Advection>(ARGS);
   }
+#undef ARGS
   return CLAWHIP_E_KIND;
 }
 

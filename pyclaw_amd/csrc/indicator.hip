@@ -405,58 +405,72 @@ __global__ void __launch_bounds__(256) k_mlp_logits(const float* feat, long long
 }
 
 // Interface flags of a state: flag = 1 when any component whose stencil range
-// exceeds `flat` has a logit above `zthr` (troubled -> full WENO).  Flags: x-interfaces (mx+1)*my, then
-// y-interfaces mx*(my+1), row-major as in gather_stencil.
+// exceeds `flat` has a logit above `zthr` (troubled -> full WENO).  Flags:
+// x-interfaces (mx+1)*my, then y-interfaces mx*(my+1), row-major as in
+// gather_stencil.  A wavefront takes 64/meqn interfaces per pass: every lane
+// gathers one (interface, component) sample, the 64 rows go through the MLP
+// as four 16-row MFMA tiles, and the first lanes OR their interface's components.
 template <typename T>
 __global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc, int meqn,
                                                    float flat, const float* master, float zthr,
                                                    unsigned char* flags) {
   __shared__ short sP[P_N + 1];
-  __shared__ short sX[4][16 * NF];
+  __shared__ short sX[4][64 * NF];
   __shared__ short sH[4][16 * NH];
-  __shared__ unsigned char sRough[4][16];
+  __shared__ unsigned char sHit[4][64];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   for (int i = tid; i < P_N; i += 256) sP[i] = f2bf(master[i]);
   __syncthreads();
   const long long nint = (long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1);
-  const long long ntiles = (nint + 15) / 16;
+  const int ni = 64 / meqn;                 // interfaces per pass
+  const long long npass = (nint + ni - 1) / ni;
   const int col = lane & 15, kq = 4 * (lane >> 4);
-  for (long long tile = (long long)blockIdx.x * 4 + wv; tile < ntiles; tile += (long long)gridDim.x * 4) {
-    unsigned char any[4] = {0, 0, 0, 0};
-    for (int m = 0; m < meqn; m++) {
-      // lanes 0..15 build one sample row each
-      if (lane < 16) {
-        long long idx = tile * 16 + lane;
-        float f[6] = {0, 0, 0, 0, 0, 0};
-        unsigned char rough = 0;
-        if (idx < nint) {
-          double v[6];
-          gather_stencil(q, g, bc, idx, m, v);
-          rough = norm_stencil(v, flat, f) > flat ? 1 : 0;
-        }
-        sRough[wv][lane] = rough;
+  for (long long pass = (long long)blockIdx.x * 4 + wv; pass < npass;
+       pass += (long long)gridDim.x * 4) {
+    const long long idx = pass * ni + lane / meqn;
+    const int m = lane % meqn;
+    const bool live = lane < ni * meqn && idx < nint;
+    float f[6] = {0, 0, 0, 0, 0, 0};
+    unsigned char rough = 0;
+    if (live) {
+      double v[6];
+      gather_stencil(q, g, bc, idx, m, v);
+      rough = norm_stencil(v, flat, f) > flat ? 1 : 0;
+    }
+    short* x = sX[wv] + lane * NF;
 #pragma unroll
-        for (int k = 0; k < 6; k++) sX[wv][lane * NF + k] = f2bf(f[k]);
-        sX[wv][lane * NF + 6] = f2bf(1.f);
+    for (int k = 0; k < 6; k++) x[k] = f2bf(f[k]);
+    x[6] = f2bf(1.f);
 #pragma unroll
-        for (int k = 7; k < NF; k++) sX[wv][lane * NF + k] = 0;
+    for (int k = 7; k < NF; k++) x[k] = 0;
+    sHit[wv][lane] = 0;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+    for (int t = 0; t < 4; t++) {
+      Fwd fw;
+      mlp_forward(sP, sX[wv] + t * 16 * NF, sH[wv], fw);
+      if (col == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) sHit[wv][t * 16 + kq + e] = fw.z3[e] > zthr ? 1 : 0;
       }
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_wave_barrier();
-      Fwd f;
-      mlp_forward(sP, sX[wv], sH[wv], f);
-#pragma unroll
-      for (int e = 0; e < 4; e++) any[e] |= (sRough[wv][kq + e] && f.z3[e] > zthr) ? 1 : 0;
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_wave_barrier();
     }
-    if (col == 0) {
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        long long idx = tile * 16 + kq + e;
-        if (idx < nint) flags[idx] = any[e];
+    // gate by the stencil range, then OR the components of each interface
+    sHit[wv][lane] = sHit[wv][lane] & rough;
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < ni) {
+      const long long fi = pass * ni + lane;
+      if (fi < nint) {
+        unsigned char any = 0;
+        for (int c = 0; c < meqn; c++) any |= sHit[wv][lane * meqn + c];
+        flags[fi] = any;
       }
     }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -509,8 +523,9 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
                      unsigned char* flags, hipStream_t s) {
   BCSpec bc = to_bc(lo, hi, refl, 2);
   long long nint = (long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1);
-  long long tiles = (nint + 15) / 16;
-  long long blocks = (tiles + 3) / 4;
+  if (meqn < 1 || meqn > 64) return CLAWHIP_E_ARG;
+  long long passes = (nint + 64 / meqn - 1) / (64 / meqn);
+  long long blocks = (passes + 3) / 4;
   if (blocks > 8192) blocks = 8192;
   if (dtype == CLAWHIP_F64)
     hipLaunchKernelGGL(k_mlp_flags<double>, dim3((unsigned)blocks), dim3(256), 0, s,

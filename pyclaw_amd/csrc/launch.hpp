@@ -29,9 +29,8 @@ inline BCSpec to_bc(const int* lo, const int* hi, const int* refl, int ndim) {
 
 int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
                    void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
-                   const int* bc_lower,
-                   const int* bc_upper, const int* reflect, int ng, const int* method,
-                   const int* mthlim, double dx, double dy, double dt,
+                   void* work, const int* lo, const int* hi, const int* refl, int ng,
+                   const int* method, const int* mthlim, double dx, double dy, double dt,
                    unsigned long long* cfl_bits, hipStream_t stream);
 
 int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
@@ -58,6 +57,13 @@ int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int
                        const int* hi, const int* refl, int ng, const int* method,
                        const int* mthlim, StepCtl* ctl, const CtlBuffers& bufs, int nsteps,
                        hipStream_t stream);
+
+long long ensemble_table_bytes(int n);
+int ensemble_prepare(const clawhip_rp& rp, const clawhip_ens_member* members, int n,
+                     const int* lo, const int* hi, const int* refl, int ng, const int* method,
+                     const int* mthlim, int rows, void* table_dev, long long* waves);
+int ensemble_run(const clawhip_rp& rp, int dtype, const void* table_dev, const long long* waves,
+                 int nsteps, hipStream_t s);
 
 int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& gin, void* q_out,
                   const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,

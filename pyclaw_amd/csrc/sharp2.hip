@@ -21,12 +21,17 @@
 namespace claw {
 
 // The WENO epsilon.  FP64: the reference's REAL(4) literal 1.e-36 widened to
-// double (reconstruct.f90:11, weno.f90:52-57).  FP32: (1e-36)^2 underflows to 0,
-// and flat data would give 0/0, so the FP32 path uses 1e-18 (square 1e-36 is a
-// normal float); the weights of any stencil whose smoothness indicators exceed
-// ~1e-18 are unchanged by this.
+// double (reconstruct.f90:11, weno.f90:52-57).  FP32 keeps flat stencils away
+// from 0/0: the component form divides by (eps + beta)^2, so eps = 1e-18 keeps
+// that square a normal float; the characteristic form divides by sums of
+// products of two such squares, (eps + beta)^4, so it takes eps = 1e-9.  The
+// weights of stencils whose smoothness indicators exceed eps are unchanged.
 template <typename T>
 __device__ __forceinline__ T weno_eps() { return sizeof(T) == 8 ? T((double)1.0e-36f) : T(1.0e-18f); }
+template <typename T>
+__device__ __forceinline__ T weno_eps_char() {
+  return sizeof(T) == 8 ? T((double)1.0e-36f) : T(1.0e-9f);
+}
 
 // ---------------------------------------------------------------- stage modes
 enum StageOut {
@@ -135,7 +140,7 @@ __device__ __forceinline__ T weno_char_side(T hm2, T hm1, T h0, T hp1, T hp2, in
   T a2 = im == 1 ? hm2 : hp2;   // hh(intwo)
   T a1 = im == 1 ? hm1 : hp1;   // hh(inone)
   T b1 = im == 1 ? hp1 : hm1;   // hh(ione)
-  const T eps = weno_eps<T>();
+  const T eps = weno_eps_char<T>();
   T fim = T(im);
   T t1 = fim * (a2 - a1);
   T t2 = fim * (a1 - h0);

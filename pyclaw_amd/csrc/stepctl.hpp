@@ -26,6 +26,7 @@ struct StepCtl {
 struct CtlBuffers {
   const void* q[2];
   void* comp[2];
+  void* work;       // the x-sweep's result (compact, like q)
   double dx, dy;
 };
 

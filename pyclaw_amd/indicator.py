@@ -43,16 +43,17 @@ class TroubledInterfaceIndicator(object):
 
     params: FP32 parameter vector (clawhip_mlp_param_count floats) or None for
     the shipped shallow-water weights.  threshold: logit above which a component
-    is troubled.  flat: stencils whose range is at most `flat` are never troubled.
+    is troubled; flat: stencils whose range is at most `flat` are never troubled
+    (None: the values stored with the shipped weights).
     """
 
-    def __init__(self, params=None, threshold=None, flat=1e-6, device="cuda"):
+    def __init__(self, params=None, threshold=None, flat=None, device="cuda"):
         meta = {}
         if params is None:
             params, meta = load(DEFAULT_WEIGHTS)
         self.params = torch.as_tensor(np.asarray(params, np.float32), device=device).contiguous()
         self.threshold = float(meta.get("threshold", 0.0) if threshold is None else threshold)
-        self.flat = float(meta.get("flat", flat) if "flat" in meta else flat)
+        self.flat = float(meta.get("flat", 1e-6) if flat is None else flat)
         self._flags = None
         self.last_fraction = None
 
@@ -187,7 +188,7 @@ def swe_solution(seed, mx, my):
     return sol
 
 
-def generate_training_data(seeds, mx=512, my=512, nsteps=150, snapshots=(30, 90, 150),
+def generate_training_data(seeds, mx=512, my=512, nsteps=300, snapshots=(50, 150, 300),
                            flat=1e-6, wtol=0.05, keep_negative=0.1, device="cuda"):
     """Run full characteristic WENO5 SharpClaw (FP64, SSP33) on the seeded
     shallow-water family and collect features/labels after the given step
@@ -222,8 +223,14 @@ def main(argv=None):
     ap.add_argument("--mx", type=int, default=512)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--recall", type=float, default=0.999)
+    ap.add_argument("--sim-steps", type=int, default=300)
+    ap.add_argument("--flat", type=float, default=1e-6)
+    ap.add_argument("--wtol", type=float, default=0.05)
     args = ap.parse_args(argv)
-    feat, label = generate_training_data(range(1000, 1000 + args.runs), args.mx, args.mx)
+    snaps = tuple(sorted({max(1, args.sim_steps // 6), args.sim_steps // 2, args.sim_steps}))
+    feat, label = generate_training_data(range(1000, 1000 + args.runs), args.mx, args.mx,
+                                         nsteps=args.sim_steps, snapshots=snaps, flat=args.flat,
+                                         wtol=args.wtol)
     n = label.numel()
     perm = torch.randperm(n, device=feat.device)
     ntr = int(0.9 * n)
@@ -240,7 +247,7 @@ def main(argv=None):
     print(f"threshold {thr:.3f} recall {recall:.4f} flagged {flagged:.3f} "
           f"(troubled {lv.float().mean().item():.3f})", flush=True)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-    save(args.out, P.cpu().numpy(), thr, 1e-6)
+    save(args.out, P.cpu().numpy(), thr, args.flat)
     print(f"saved {args.out}", flush=True)
 
 

@@ -238,3 +238,4 @@ def test_stage_all_flags_equals_char_weno():
                        [1, 2], 1e-3, 1.0 / mx, 1.0 / my, qi, flags=fl)
         outs.append(q_from_device(out))
     assert np.array_equal(outs[0], outs[1])
+

@@ -145,3 +145,23 @@ def test_fp32_kahan_ssp33_close_to_fp64():
     assert np.abs(qa - qb).sum() / np.abs(qa).sum() < 1e-6
     # the Kahan-compensated update keeps the totals at FP64-level drift
     np.testing.assert_allclose(qb.sum(axis=(1, 2)), q.sum(axis=(1, 2)), rtol=1e-6, atol=1e-5)
+
+
+def test_fp32_kahan_char_weno_flat_regions_stay_finite():
+    """Exactly flat regions (h = 1, hu = hv = 0 away from the humps): the FP32
+    characteristic weights must not form 0/0 (their (eps + beta)^4 products)."""
+    from pyclaw_amd import indicator as I
+    out = {}
+    for prec in ("fp64", "fp32k"):
+        solver = I.swe_solver(prec)
+        sol = I.swe_solution(3, 96, 96)
+        solver.dt = 0.5 / 96 / 1.6
+        solver.setup(sol)
+        for _ in range(30):
+            solver.evolve_to_time(sol)
+        out[prec] = (sol.state.q.copy(), sol.t)
+    qa, ta = out["fp64"]
+    qb, tb = out["fp32k"]
+    assert np.isfinite(qb).all()
+    assert abs(tb - ta) < 1e-6 * ta
+    assert np.abs(qb - qa).sum() / np.abs(qa).sum() < 1e-5
