@@ -95,14 +95,11 @@ int clawhip_cfl_reset(double *cfl_dev, void *stream);
  *   mthlim[num_waves] as _set_mthlim (classic/solver.py:162-171);
  *   q_in (geometry gin) -> q_out (geometry gout, ghosts = 0); distinct buffers;
  *   comp_in -> comp_out: Kahan compensation (CLAWHIP_F32K only; may alias);
- *   work: a third buffer shaped like q_out (the step runs as an x-sweep into
- *         work and a y-sweep into q_out: work holds the reference's qnew
- *         between its two sweep loops);
  *   *cfl_dev = max(*cfl_dev, Courant number of this step) (the `cfl` output).
  * reflect[d] is solver.reflect_index[d] (classic/solver.py:445). */
 int clawhip_step2_unsplit(const clawhip_rp *rp, int dtype, const void *q_in,
                           const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
-                          const void *comp_in, void *comp_out, void *work,
+                          const void *comp_in, void *comp_out,
                           const int *bc_lower, const int *bc_upper,
                           const int *reflect, int num_ghost, const int *method,
                           const int *mthlim, double dx, double dy, double dt,
@@ -115,12 +112,12 @@ int clawhip_step2_unsplit(const clawhip_rp *rp, int dtype, const void *q_in,
  * evolve_to_time(solution, tend).  Launches after ctl->done != 0 exit at once:
  * 1 reached tend, 2 CFL too large with fixed dt, 3 max_steps attempts made.
  * The launches contain no host synchronisation and may be captured in a graph.
- * Boundary codes 1..3; work as in clawhip_step2_unsplit. */
+ * Boundary codes 1..3. */
 int clawhip_step2_unsplit_run(const clawhip_rp *rp, int dtype, const clawhip_geom *g,
                               const int *bc_lower, const int *bc_upper, const int *reflect,
                               int num_ghost, const int *method, const int *mthlim, double dx,
                               double dy, void *q0, void *q1, void *comp0, void *comp1,
-                              void *work, clawhip_stepctl *ctl_dev, int nsteps, void *stream);
+                              clawhip_stepctl *ctl_dev, int nsteps, void *stream);
 
 /* One dimension-split 2D step: both sweeps of ClawSolver2D.step_hyperbolic with
  * dimensional_split=True (pyclaw/classic/solver.py:523-535), i.e. fmod.step2ds
@@ -188,7 +185,6 @@ typedef struct {
   int mx, my;                  /* compact planar (meqn, my, mx) arrays            */
   void *q0, *q1;               /* ping-pong state (current one: ctl->cur)         */
   void *comp0, *comp1;         /* Kahan compensation (CLAWHIP_F32K), else NULL    */
-  void *work;                  /* shaped like q0                                  */
   double dx, dy;
   clawhip_stepctl *ctl;        /* this member's device controller                 */
 } clawhip_ens_member;
@@ -198,16 +194,16 @@ long long clawhip_ensemble_table_bytes(int n);
 
 /* Validates the members and writes the launch table (member order, wavefront
  * offsets, method) into table_dev (device memory of clawhip_ensemble_table_bytes
- * bytes; a synchronous host-to-device copy).  rows: slices per wavefront (0 =
+ * bytes; a synchronous host-to-device copy).  rows: rows per wavefront (0 =
  * chosen from the occupancy and the total work).  waves[2] receives the grid
- * sizes of the two sweeps, to pass to clawhip_ensemble_run.  Boundary codes
+ * size of the step launch (waves[1] is 0), to pass to clawhip_ensemble_run.  Boundary codes
  * 1..3, num_ghost 2, method/mthlim as clawhip_step2_unsplit. */
 int clawhip_ensemble_prepare(const clawhip_rp *rp, int dtype, const clawhip_ens_member *members,
                              int n, const int *bc_lower, const int *bc_upper, const int *reflect,
                              int num_ghost, const int *method, const int *mthlim, int rows,
                              void *table_dev, long long *waves);
 
-/* nsteps attempted steps of every member (2 launches per step; members whose
+/* nsteps attempted steps of every member (one launch per step; members whose
  * controller is done skip).  Capturable in a graph. */
 int clawhip_ensemble_run(const clawhip_rp *rp, int dtype, const void *table_dev,
                          const long long *waves, int nsteps, void *stream);

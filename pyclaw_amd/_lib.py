@@ -49,7 +49,7 @@ class EnsMember(ctypes.Structure):
     _fields_ = [("mx", ctypes.c_int), ("my", ctypes.c_int),
                 ("q0", ctypes.c_void_p), ("q1", ctypes.c_void_p),
                 ("comp0", ctypes.c_void_p), ("comp1", ctypes.c_void_p),
-                ("work", ctypes.c_void_p), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double),
                 ("ctl", ctypes.c_void_p)]
 
 
@@ -81,7 +81,7 @@ _SIGS = {
     "clawhip_device_count": [_IP],
     "clawhip_cfl_reset": [_P, _P],
     "clawhip_step2_unsplit": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
-                              ctypes.POINTER(Geom), _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
+                              ctypes.POINTER(Geom), _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                               _D, _D, _D, _P, _P],
     "clawhip_step2_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
@@ -94,7 +94,7 @@ _SIGS = {
                              _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_step2_unsplit_run": [ctypes.POINTER(RP), _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
-                                  _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _P, _I, _P],
+                                  _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _I, _P],
     "clawhip_step2_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _P, _P],
     "clawhip_step3_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,

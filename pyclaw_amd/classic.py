@@ -111,7 +111,6 @@ class ClawSolver2D(ClawSolver):
         self.use_graph = True
         self.graph_chunk = 16
         self._ctl = None
-        self._uwork = None
         self._graphs = {}
         super(ClawSolver2D, self).__init__(riemann_solver, claw_package)
 
@@ -145,10 +144,9 @@ class ClawSolver2D(ClawSolver):
                                        self.dt, comp_in=src[1], comp_out=out[1], gin=gin,
                                        num_ghost=self.num_ghost, work=self._work)
         else:
-            self._work = K.step2_unsplit(self._rp_struct, src[0], out[0], cfl_dev, lo, hi,
-                                         self.reflect_index, self._method, self._mthlim, dx, dy,
-                                         self.dt, comp_in=src[1], comp_out=out[1], gin=gin,
-                                         num_ghost=self.num_ghost, work=self._work)
+            K.step2_unsplit(self._rp_struct, src[0], out[0], cfl_dev, lo, hi, self.reflect_index,
+                            self._method, self._mthlim, dx, dy, self.dt, comp_in=src[1],
+                            comp_out=out[1], gin=gin, num_ghost=self.num_ghost)
         cfl = float(cfl_dev.item())
         self.cfl.update_global_max(cfl)
         state.set_device_q(*out)
@@ -223,12 +221,8 @@ class ClawSolver2D(ClawSolver):
         grid = state.grid
         dx, dy = grid.delta
         lo, hi = self._bc_codes()
-        if self._uwork is None or self._uwork.shape != qin[0].shape or \
-                self._uwork.dtype != qin[0].dtype:
-            self._uwork = torch.empty_like(qin[0])
-            self._graphs = {}   # captured graphs hold the old buffer's address
-        args = (self._rp_struct, (bufs[0][0], bufs[1][0]), (bufs[0][1], bufs[1][1]), self._uwork,
-                self._ctl, lo, hi, self.reflect_index, self._method, self._mthlim, dx, dy)
+        args = (self._rp_struct, (bufs[0][0], bufs[1][0]), (bufs[0][1], bufs[1][1]), self._ctl,
+                lo, hi, self.reflect_index, self._method, self._mthlim, dx, dy)
         launched = 0
         host = L.StepCtl()
         while True:

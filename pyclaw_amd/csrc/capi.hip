@@ -47,18 +47,18 @@ int clawhip_cfl_reset(double* cfl_dev, void* stream) {
 
 int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
                           const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
-                          const void* comp_in, void* comp_out, void* work,
+                          const void* comp_in, void* comp_out,
                           const int* bc_lower, const int* bc_upper,
                           const int* reflect, int num_ghost, const int* method,
                           const int* mthlim, double dx, double dy, double dt,
                           double* cfl_dev, void* stream) {
-  if (!rp || !q_in || !q_out || !work || !cfl_dev || !bc_lower || !bc_upper || !reflect ||
-      !method || !mthlim)
+  if (!rp || !q_in || !q_out || !cfl_dev || !bc_lower || !bc_upper || !reflect || !method ||
+      !mthlim)
     return CLAWHIP_E_ARG;
   if (!geom_ok(gin, 2) || !geom_ok(gout, 2) || gout->ghosts != 0) return CLAWHIP_E_ARG;
   if (gin->mx != gout->mx || gin->my != gout->my) return CLAWHIP_E_ARG;
   if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
-  if (q_in == q_out || work == q_in || work == q_out) return CLAWHIP_E_ARG;
+  if (q_in == q_out) return CLAWHIP_E_ARG;
   if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
@@ -72,7 +72,7 @@ int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
       return CLAWHIP_E_ARG;
   }
   return launch_unsplit(*rp, dtype, q_in, to_geom(gin), q_out, to_geom(gout), comp_in, comp_out,
-                        work, bc_lower,
+                        bc_lower,
                         bc_upper, reflect, num_ghost, method, mthlim, dx, dy, dt,
                         (unsigned long long*)cfl_dev, (hipStream_t)stream);
 }
@@ -179,13 +179,13 @@ int clawhip_step2_unsplit_run(const clawhip_rp* rp, int dtype, const clawhip_geo
                               const int* bc_lower, const int* bc_upper, const int* reflect,
                               int num_ghost, const int* method, const int* mthlim, double dx,
                               double dy, void* q0, void* q1, void* comp0, void* comp1,
-                              void* work, clawhip_stepctl* ctl_dev, int nsteps, void* stream) {
+                              clawhip_stepctl* ctl_dev, int nsteps, void* stream) {
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !method || !mthlim || !q0 || !q1 ||
-      !work || !ctl_dev || nsteps < 0)
+      !ctl_dev || nsteps < 0)
     return CLAWHIP_E_ARG;
   if (!geom_ok(g, 2) || g->ghosts != 0 || num_ghost != 2 || g->mx < 2 || g->my < 2)
     return CLAWHIP_E_ARG;
-  if (q0 == q1 || work == q0 || work == q1) return CLAWHIP_E_ARG;
+  if (q0 == q1) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (dtype == CLAWHIP_F32K && (!comp0 || !comp1 || comp0 == comp1)) return CLAWHIP_E_ARG;
   if (!(dx > 0.0) || !(dy > 0.0)) return CLAWHIP_E_ARG;
@@ -197,7 +197,7 @@ int clawhip_step2_unsplit_run(const clawhip_rp* rp, int dtype, const clawhip_geo
       return CLAWHIP_E_ARG;
   static_assert(sizeof(clawhip_stepctl) == sizeof(StepCtl), "clawhip_stepctl layout");
   CtlBuffers b;
-  b.q[0] = q0; b.q[1] = q1; b.comp[0] = comp0; b.comp[1] = comp1; b.work = work;
+  b.q[0] = q0; b.q[1] = q1; b.comp[0] = comp0; b.comp[1] = comp1;
   b.dx = dx; b.dy = dy;
   return launch_unsplit_ctl(*rp, dtype, to_geom(g), bc_lower, bc_upper, reflect, num_ghost,
                             method, mthlim, (StepCtl*)ctl_dev, b, nsteps, (hipStream_t)stream);
@@ -224,8 +224,8 @@ int clawhip_ensemble_prepare(const clawhip_rp* rp, int dtype, const clawhip_ens_
       return CLAWHIP_E_ARG;
   for (int k = 0; k < n; k++) {
     const clawhip_ens_member& m = members[k];
-    if (m.mx < 2 || m.my < 2 || !m.q0 || !m.q1 || !m.work || !m.ctl) return CLAWHIP_E_ARG;
-    if (m.q0 == m.q1 || m.work == m.q0 || m.work == m.q1) return CLAWHIP_E_ARG;
+    if (m.mx < 2 || m.my < 2 || !m.q0 || !m.q1 || !m.ctl) return CLAWHIP_E_ARG;
+    if (m.q0 == m.q1) return CLAWHIP_E_ARG;
     if (dtype == CLAWHIP_F32K && (!m.comp0 || !m.comp1 || m.comp0 == m.comp1))
       return CLAWHIP_E_ARG;
     if (!(m.dx > 0.0) || !(m.dy > 0.0)) return CLAWHIP_E_ARG;

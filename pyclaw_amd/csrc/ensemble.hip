@@ -2,13 +2,13 @@
 // Riemann solver and method) advanced together: every launch sweeps all
 // members, each with its own dt and device step controller.
 //
-// The members' wavefronts are concatenated into one grid per sweep (largest
-// members first, so the small ones fill the tail); a wavefront finds its member
-// by binary search over the members' first-wavefront offsets and then runs the
-// same sweep body as the single-grid step (usweep.hpp).  The last wavefront of a
-// member's y-sweep applies that member's accept/reject and next dt
-// (stepctl.hpp).  A step of the whole ensemble is two launches with no host
-// synchronisation, so runs of steps are captured in a HIP graph.
+// The members' wavefronts are concatenated into one grid (largest members
+// first, so the small ones fill the tail); a wavefront finds its member by
+// binary search over the members' first-wavefront offsets and then runs the
+// same step body as the single-grid kernel (unsplit.hpp).  The last wavefront of
+// a member applies that member's accept/reject and next dt (stepctl.hpp).  A
+// step of the whole ensemble is one launch with no host synchronisation, so
+// runs of steps are captured in a HIP graph.
 #include <stdlib.h>
 #include <string.h>
 
@@ -16,17 +16,16 @@
 #include <vector>
 
 #include "launch.hpp"
-#include "usweep.hpp"
+#include "unsplit.hpp"
 
 namespace claw {
 
 struct EnsMember {
   int mx, my;
-  int nstrips_x, nrb_x, nstrips_y, nrb_y;
-  long long base_x, base_y;   // first wavefront of this member in each sweep
+  int nstrips, nrb;
+  long long base;             // first wavefront of this member
   const void* q[2];
   void* comp[2];
-  void* work;
   double dx, dy;
   StepCtl* ctl;
 };
@@ -37,7 +36,7 @@ struct EnsHeader {
   BCSpec bc;
   RPParams<double> rp64;
   RPParams<float> rp32;
-  long long waves_x, waves_y;
+  long long waves;
 };
 
 constexpr long long kEnsAlign = 256;
@@ -56,18 +55,16 @@ __device__ __forceinline__ const RPParams<float>& ens_rp<float>(const EnsHeader&
   return h.rp32;
 }
 
-template <class RS, typename T, bool KAHAN, int AX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-k_ens_sweep(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem) {
+template <class RS, typename T, bool KAHAN>
+__global__ void __launch_bounds__(256)
+k_ens_step(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem) {
   const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const long long waves = AX == 1 ? hdr->waves_x : hdr->waves_y;
-  if (gw >= waves) return;
+  if (gw >= hdr->waves) return;
   // member k: the last one whose first wavefront is <= gw
   int lo = 0, hi = hdr->n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    const long long b = AX == 1 ? mem[mid].base_x : mem[mid].base_y;
-    if (b <= gw) lo = mid;
+    if (mem[mid].base <= gw) lo = mid;
     else hi = mid - 1;
   }
   const EnsMember& m = mem[lo];
@@ -75,7 +72,7 @@ k_ens_sweep(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem
   if (ctl->done) return;   // uniform across the member
   const int cur = ctl->cur;
   const double dt = ctl->dt;
-  USweepArgs<T> a;
+  UnsplitArgs<T> a;
   a.rp = ens_rp<T>(*hdr);
   a.gin.mx = m.mx; a.gin.my = m.my; a.gin.mz = 1; a.gin.g = 0;
   a.gin.fs = (long long)m.mx * m.my; a.gin.rs = m.mx; a.gin.ss = 0;
@@ -83,27 +80,17 @@ k_ens_sweep(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem
   a.bc = hdr->bc;
   a.ng = hdr->ng; a.order = hdr->order; a.trans = hdr->trans;
   for (int w = 0; w < 4; w++) a.mthlim[w] = hdr->mthlim[w];
-  a.H = hdr->H;
+  a.H = hdr->H; a.nstrips = m.nstrips; a.nrb = m.nrb;
   a.qin = (const T*)m.q[cur];
+  a.qout = (T*)m.q[1 - cur];
+  a.comp_in = (const T*)m.comp[cur];
+  a.comp_out = (T*)m.comp[1 - cur];
+  a.dtdx = T(dt / m.dx);
+  a.dtdy = T(dt / m.dy);
   a.cfl_bits = &ctl->cfl_bits;
-  a.win = nullptr; a.comp_in = nullptr; a.comp_out = nullptr;
-  long long local;
-  if (AX == 1) {
-    a.nstrips = m.nstrips_x; a.nrb = m.nrb_x;
-    a.wout = (T*)m.work;
-    a.dn = T(dt / m.dx); a.dtr = T(dt / m.dy);
-    local = gw - m.base_x;
-  } else {
-    a.nstrips = m.nstrips_y; a.nrb = m.nrb_y;
-    a.win = (const T*)m.work;
-    a.wout = (T*)m.q[1 - cur];
-    a.comp_in = (const T*)m.comp[cur];
-    a.comp_out = (T*)m.comp[1 - cur];
-    a.dn = T(dt / m.dy); a.dtr = T(dt / m.dx);
-    local = gw - m.base_y;
-  }
-  usweep_body<RS, T, KAHAN, AX>(a, (int)local);
-  if (AX == 2) step_ctl_finish(ctl, (unsigned)(m.nstrips_y * m.nrb_y));
+  const int local = (int)(gw - m.base);
+  unsplit_body<RS, T, KAHAN>(a, local);
+  step_ctl_finish(ctl, (unsigned)(m.nstrips * m.nrb));
 }
 
 namespace {
@@ -115,7 +102,7 @@ int ens_resident_waves() {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, k_ens_sweep<Euler<float>, float, true, 2>, 256, 0) != hipSuccess ||
+            &nb, k_ens_step<Euler<float>, float, true>, 256, 0) != hipSuccess ||
         ncu < 1 || nb < 1)
       return 1024;
     resident = ncu * nb * 4;
@@ -123,24 +110,20 @@ int ens_resident_waves() {
   return resident;
 }
 
-long long ens_waves(const std::vector<EnsMember>& v, int H, int ax) {
+long long ens_waves(const std::vector<EnsMember>& v, int H) {
   long long w = 0;
-  for (const auto& m : v) {
-    const int ms = ax == 1 ? m.mx : m.my, mm = ax == 1 ? m.my : m.mx;
-    w += (long long)((ms + 59) / 60) * ((mm + H - 1) / H);
-  }
+  for (const auto& m : v) w += (long long)((m.mx + 59) / 60) * ((m.my + H - 1) / H);
   return w;
 }
 
 // Rows per wavefront for the whole ensemble: the single-grid rule
-// (rounds x (H + 2)) over the total wavefront count of both sweeps.
+// (rounds x (H + 2)) over the total wavefront count.
 int ens_choose_rows(const std::vector<EnsMember>& v) {
   const long long R = ens_resident_waves();
   int best = 16;
   long long best_cost = -1;
   for (int h = 2; h <= 64; h++) {
-    long long cost = 0;
-    for (int ax = 1; ax <= 2; ax++) cost += ((ens_waves(v, h, ax) + R - 1) / R) * (h + 2);
+    const long long cost = ((ens_waves(v, h) + R - 1) / R) * (h + 2);
     if (best_cost < 0 || cost < best_cost || (cost == best_cost && h > best)) {
       best = h;
       best_cost = cost;
@@ -152,11 +135,9 @@ int ens_choose_rows(const std::vector<EnsMember>& v) {
 template <class RS, typename T, bool KAHAN>
 void ens_launch(const EnsHeader* h, const EnsMember* m, const long long* waves, int nsteps,
                 hipStream_t s) {
-  const unsigned bx = (unsigned)((waves[0] + 3) / 4), by = (unsigned)((waves[1] + 3) / 4);
-  for (int n = 0; n < nsteps; n++) {
-    hipLaunchKernelGGL((k_ens_sweep<RS, T, KAHAN, 1>), dim3(bx), dim3(256), 0, s, h, m);
-    hipLaunchKernelGGL((k_ens_sweep<RS, T, KAHAN, 2>), dim3(by), dim3(256), 0, s, h, m);
-  }
+  const unsigned blocks = (unsigned)((waves[0] + 3) / 4);
+  for (int n = 0; n < nsteps; n++)
+    hipLaunchKernelGGL((k_ens_step<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, s, h, m);
 }
 
 template <template <typename> class RSF>
@@ -180,7 +161,7 @@ int ensemble_prepare(const clawhip_rp& rp, const clawhip_ens_member* members, in
     const clawhip_ens_member& s = members[k];
     EnsMember& m = v[k];
     m.mx = s.mx; m.my = s.my;
-    m.q[0] = s.q0; m.q[1] = s.q1; m.comp[0] = s.comp0; m.comp[1] = s.comp1; m.work = s.work;
+    m.q[0] = s.q0; m.q[1] = s.q1; m.comp[0] = s.comp0; m.comp[1] = s.comp1;
     m.dx = s.dx; m.dy = s.dy;
     m.ctl = (StepCtl*)s.ctl;
   }
@@ -190,13 +171,12 @@ int ensemble_prepare(const clawhip_rp& rp, const clawhip_ens_member* members, in
     return (long long)a.mx * a.my > (long long)b.mx * b.my;
   });
   const int H = rows > 0 ? rows : ens_choose_rows(v);
-  long long bx = 0, by = 0;
+  long long base = 0;
   for (auto& m : v) {
-    m.nstrips_x = (m.mx + 59) / 60; m.nrb_x = (m.my + H - 1) / H;
-    m.nstrips_y = (m.my + 59) / 60; m.nrb_y = (m.mx + H - 1) / H;
-    m.base_x = bx; m.base_y = by;
-    bx += (long long)m.nstrips_x * m.nrb_x;
-    by += (long long)m.nstrips_y * m.nrb_y;
+    m.nstrips = (m.mx + 59) / 60;
+    m.nrb = (m.my + H - 1) / H;
+    m.base = base;
+    base += (long long)m.nstrips * m.nrb;
   }
   EnsHeader h;
   h.n = n; h.H = H; h.ng = ng; h.order = method[1]; h.trans = method[2];
@@ -204,13 +184,13 @@ int ensemble_prepare(const clawhip_rp& rp, const clawhip_ens_member* members, in
   h.bc = to_bc(lo, hi, refl, 2);
   h.rp64 = to_params<double>(rp);
   h.rp32 = to_params<float>(rp);
-  h.waves_x = bx; h.waves_y = by;
+  h.waves = base;
   std::vector<char> buf((size_t)ensemble_table_bytes(n), 0);
   ::memcpy(buf.data(), &h, sizeof(h));
   ::memcpy(buf.data() + ens_members_offset(), v.data(), (size_t)n * sizeof(EnsMember));
   hipError_t e = hipMemcpy(table_dev, buf.data(), buf.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) return (int)e;
-  waves[0] = bx; waves[1] = by;
+  waves[0] = base; waves[1] = 0;
   return 0;
 }
 

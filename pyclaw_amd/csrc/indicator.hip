@@ -41,19 +41,21 @@ __device__ __forceinline__ short f2bf(float f) {
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
 
 // ---------------------------------------------------------------- features
-// stencil v[0..5] -> normalised features f[0..5]; returns the range.
-__device__ __forceinline__ float norm_stencil(const double (&v)[6], float flat, float (&f)[6]) {
-  double lo = v[0], hi = v[0];
+// stencil v[0..5] -> normalised features f[0..5]; returns the range.  Computed in
+// the state's precision (features are rounded to BF16 afterwards).
+template <typename T>
+__device__ __forceinline__ float norm_stencil(const T (&v)[6], float flat, float (&f)[6]) {
+  T lo = v[0], hi = v[0];
 #pragma unroll
   for (int k = 1; k < 6; k++) { lo = v[k] < lo ? v[k] : lo; hi = v[k] > hi ? v[k] : hi; }
-  double rng = hi - lo;
-  if (rng <= (double)flat) {
+  T rng = hi - lo;
+  if (rng <= (T)flat) {
 #pragma unroll
     for (int k = 0; k < 6; k++) f[k] = 0.f;
   } else {
-    double inv = 2.0 / rng;
+    T inv = T(2) / rng;
 #pragma unroll
-    for (int k = 0; k < 6; k++) f[k] = (float)((v[k] - lo) * inv - 1.0);
+    for (int k = 0; k < 6; k++) f[k] = (float)((v[k] - lo) * inv - T(1));
   }
   return (float)rng;
 }
@@ -91,9 +93,9 @@ __device__ __forceinline__ double weno_departure(const double (&v)[6]) {
 
 // Gather the stencil of interface `idx` (x-interfaces first, then y) for field m.
 // x-interface (c, r): c in 1..mx+1, r in 1..my; y-interface (c, r): r in 1..my+1.
-template <typename T>
+template <typename T, typename V>
 __device__ __forceinline__ void gather_stencil(const T* q, const Geom& g, const BCSpec& bc,
-                                               long long idx, int m, double (&v)[6]) {
+                                               long long idx, int m, V (&v)[6]) {
   const long long nx = (long long)(g.mx + 1) * g.my;
   int c, r, dir;
   if (idx < nx) {
@@ -109,7 +111,35 @@ __device__ __forceinline__ void gather_stencil(const T* q, const Geom& g, const 
     bool fx = false, fy = false;
     int ci = bc_map(cc, g.mx, 3, bc.lo[0], bc.hi[0], fx);
     int ri = bc_map(rr, g.my, 3, bc.lo[1], bc.hi[1], fy);
-    double x = (double)q[(long long)m * g.fs + (ci - 1) + (long long)(ri - 1) * g.rs];
+    V x = (V)q[(long long)m * g.fs + (ci - 1) + (long long)(ri - 1) * g.rs];
+    if ((fx && m == bc.refl[0]) != (fy && m == bc.refl[1])) x = -x;
+    v[s] = x;
+  }
+}
+
+// Stencil of the interface (c, r) in direction dir (0: x-interface between cells
+// (c-1, r) and (c, r); 1: y-interface between (c, r-1) and (c, r)), field m.
+// Interior stencils load directly; others go through the boundary map.
+template <typename T, typename V>
+__device__ __forceinline__ void gather_stencil_at(const T* q, const Geom& g, const BCSpec& bc,
+                                                  int dir, int c, int r, int m, V (&v)[6]) {
+  const T* base = q + (long long)m * g.fs;
+  const bool interior = dir == 0 ? (c - 3 >= 1 && c + 2 <= g.mx) : (r - 3 >= 1 && r + 2 <= g.my);
+  if (interior) {
+    const T* p0 = base + (long long)(r - 1) * g.rs + (c - 1);
+    const long long step = dir == 0 ? 1 : g.rs;
+#pragma unroll
+    for (int s = 0; s < 6; s++) v[s] = (V)p0[(s - 3) * step];
+    return;
+  }
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int cc = dir == 0 ? c - 3 + s : c;
+    const int rr = dir == 0 ? r : r - 3 + s;
+    bool fx = false, fy = false;
+    int ci = bc_map(cc, g.mx, 3, bc.lo[0], bc.hi[0], fx);
+    int ri = bc_map(rr, g.my, 3, bc.lo[1], bc.hi[1], fy);
+    V x = (V)base[(ci - 1) + (long long)(ri - 1) * g.rs];
     if ((fx && m == bc.refl[0]) != (fy && m == bc.refl[1])) x = -x;
     v[s] = x;
   }
@@ -199,6 +229,75 @@ __device__ __forceinline__ void mlp_forward(const short* P, const short* sX, sho
     for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
     o.z3[e] = s + bf2f(P[P_B3]);
   }
+}
+
+// Inference form, computed transposed: Z1^T = W1^T X^T, Z2^T = W2^T H1^T + b2,
+// z = w3 . relu(Z2^T) + b3.  With the samples as the MFMA's N index, a layer's
+// D fragment (rows 4*(lane/16)+e = hidden units, column lane%16 = sample) is
+// exactly the next layer's B fragment, so the activations never leave the
+// registers; the weight fragments (the A operands) are loaded once per kernel.
+struct MlpRegs {
+  bf16x4 w1[2];       // W1^T rows nb*16+col, k = kq..kq+3  (= W1[kq+i][nb*16+col])
+  bf16x4 w2[2][2];    // W2^T rows nb*16+col, k = kb*16+kq+i (= W2[kb*16+kq+i][nb*16+col])
+  float b2[2][4], w3[2][4], b3;   // entries of hidden units nb*16+kq+e
+};
+
+__device__ __forceinline__ void mlp_load_regs(const float* master, MlpRegs& r) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+  for (int nb = 0; nb < 2; nb++) {
+    bf16x4 b;
+    b.x = f2bf(master[P_W1 + (kq + 0) * NH + nb * 16 + col]);
+    b.y = f2bf(master[P_W1 + (kq + 1) * NH + nb * 16 + col]);
+    b.z = f2bf(master[P_W1 + (kq + 2) * NH + nb * 16 + col]);
+    b.w = f2bf(master[P_W1 + (kq + 3) * NH + nb * 16 + col]);
+    r.w1[nb] = b;
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) {
+      const int k0 = kb * 16 + kq;
+      bf16x4 c;
+      c.x = f2bf(master[P_W2 + (k0 + 0) * NH + nb * 16 + col]);
+      c.y = f2bf(master[P_W2 + (k0 + 1) * NH + nb * 16 + col]);
+      c.z = f2bf(master[P_W2 + (k0 + 2) * NH + nb * 16 + col]);
+      c.w = f2bf(master[P_W2 + (k0 + 3) * NH + nb * 16 + col]);
+      r.w2[kb][nb] = c;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      r.b2[nb][e] = bf2f(f2bf(master[P_B2 + nb * 16 + kq + e]));
+      r.w3[nb][e] = bf2f(f2bf(master[P_W3 + nb * 16 + kq + e]));
+    }
+  }
+  r.b3 = bf2f(f2bf(master[P_B3]));
+}
+
+// 16 samples; xT = this lane's B fragment of X^T (features kq..kq+3 of sample
+// lane%16).  Returns the logit of sample lane%16 (in every lane of that column).
+__device__ __forceinline__ float mlp_forward_regs(const MlpRegs& r, bf16x4 xT) {
+  bf16x4 h[2];
+#pragma unroll
+  for (int nb = 0; nb < 2; nb++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[nb], xT, acc, 0, 0, 0);
+    h[nb].x = f2bf(fmaxf(acc[0], 0.f));
+    h[nb].y = f2bf(fmaxf(acc[1], 0.f));
+    h[nb].z = f2bf(fmaxf(acc[2], 0.f));
+    h[nb].w = f2bf(fmaxf(acc[3], 0.f));
+  }
+  float v = 0.f;
+#pragma unroll
+  for (int nb = 0; nb < 2; nb++) {
+    f32x4 acc = {r.b2[nb][0], r.b2[nb][1], r.b2[nb][2], r.b2[nb][3]};
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[kb][nb], h[kb], acc, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < 4; e++) v += fmaxf(acc[e], 0.f) * r.w3[nb][e];
+  }
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v + r.b3;
 }
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.f / (1.f + __expf(-z)); }
@@ -410,64 +509,73 @@ __global__ void __launch_bounds__(256) k_mlp_logits(const float* feat, long long
 // gather_stencil.  A wavefront takes 64/meqn interfaces per pass: every lane
 // gathers one (interface, component) sample, the 64 rows go through the MLP
 // as four 16-row MFMA tiles, and the first lanes OR their interface's components.
+// A pass covers 64/meqn consecutive interfaces of one row (x-interfaces c =
+// 1..mx+1 of rows 1..my, then y-interfaces c = 1..mx of rows 1..my+1), so the
+// row and column block are wavefront-uniform and the stencil loads coalesce.
 template <typename T>
 __global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc, int meqn,
                                                    float flat, const float* master, float zthr,
                                                    unsigned char* flags) {
-  __shared__ short sP[P_N + 1];
-  __shared__ short sX[4][64 * NF];
-  __shared__ short sH[4][16 * NH];
+  __shared__ __attribute__((aligned(16))) short sX[4][64 * 8];
   __shared__ unsigned char sHit[4][64];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  for (int i = tid; i < P_N; i += 256) sP[i] = f2bf(master[i]);
-  __syncthreads();
-  const long long nint = (long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1);
+  MlpRegs reg;
+  mlp_load_regs(master, reg);
+  const int mx = g.mx, my = g.my;
   const int ni = 64 / meqn;                 // interfaces per pass
-  const long long npass = (nint + ni - 1) / ni;
-  const int col = lane & 15, kq = 4 * (lane >> 4);
+  const int npx = (mx + 1 + ni - 1) / ni, npy = (mx + ni - 1) / ni;
+  const long long nxp = (long long)npx * my;
+  const long long npass = nxp + (long long)npy * (my + 1);
+  const int col = lane & 15, grp = lane >> 4;
+  const int li = lane / meqn, m = lane % meqn;
   for (long long pass = (long long)blockIdx.x * 4 + wv; pass < npass;
        pass += (long long)gridDim.x * 4) {
-    const long long idx = pass * ni + lane / meqn;
-    const int m = lane % meqn;
-    const bool live = lane < ni * meqn && idx < nint;
+    int dir, r, cb;
+    if (pass < nxp) {
+      dir = 0; r = 1 + (int)(pass / npx); cb = (int)(pass % npx);
+    } else {
+      const long long p2 = pass - nxp;
+      dir = 1; r = 1 + (int)(p2 / npy); cb = (int)(p2 % npy);
+    }
+    const int c = 1 + cb * ni + li;
+    const int cmax = dir == 0 ? mx + 1 : mx;
+    const bool live = lane < ni * meqn && c <= cmax;
     float f[6] = {0, 0, 0, 0, 0, 0};
     unsigned char rough = 0;
     if (live) {
-      double v[6];
-      gather_stencil(q, g, bc, idx, m, v);
+      T v[6];
+      gather_stencil_at(q, g, bc, dir, c, r, m, v);
       rough = norm_stencil(v, flat, f) > flat ? 1 : 0;
     }
-    short* x = sX[wv] + lane * NF;
-#pragma unroll
-    for (int k = 0; k < 6; k++) x[k] = f2bf(f[k]);
-    x[6] = f2bf(1.f);
-#pragma unroll
-    for (int k = 7; k < NF; k++) x[k] = 0;
-    sHit[wv][lane] = 0;
+    // this lane's sample row: features 0..5, bias input 1, pad
+    bf16x4 lo, hi;
+    lo.x = f2bf(f[0]); lo.y = f2bf(f[1]); lo.z = f2bf(f[2]); lo.w = f2bf(f[3]);
+    hi.x = f2bf(f[4]); hi.y = f2bf(f[5]); hi.z = f2bf(1.f); hi.w = 0;
+    bf16x4* x = reinterpret_cast<bf16x4*>(sX[wv] + lane * 8);
+    x[0] = lo; x[1] = hi;
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-#pragma unroll 1
+    // tile t = samples 16t..16t+15: lane group 0 holds features 0-3 of sample
+    // col, group 1 features 4-7, groups 2-3 the zero padding (features 8-15);
+    // lane l's own sample (l) is in tile l/16
+    unsigned char hit = 0;
+#pragma unroll
     for (int t = 0; t < 4; t++) {
-      Fwd fw;
-      mlp_forward(sP, sX[wv] + t * 16 * NF, sH[wv], fw);
-      if (col == 0) {
-#pragma unroll
-        for (int e = 0; e < 4; e++) sHit[wv][t * 16 + kq + e] = fw.z3[e] > zthr ? 1 : 0;
-      }
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_wave_barrier();
+      bf16x4 xT = {0, 0, 0, 0};
+      if (grp < 2) xT = *reinterpret_cast<const bf16x4*>(sX[wv] + (t * 16 + col) * 8 + grp * 4);
+      const float z = mlp_forward_regs(reg, xT);
+      if (t == grp) hit = z > zthr ? 1 : 0;
     }
-    // gate by the stencil range, then OR the components of each interface
-    sHit[wv][lane] = sHit[wv][lane] & rough;
+    sHit[wv][lane] = hit & rough;
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    if (lane < ni) {
-      const long long fi = pass * ni + lane;
-      if (fi < nint) {
-        unsigned char any = 0;
-        for (int c = 0; c < meqn; c++) any |= sHit[wv][lane * meqn + c];
-        flags[fi] = any;
-      }
+    const int ci = 1 + cb * ni + lane;
+    if (lane < ni && ci <= cmax) {
+      unsigned char any = 0;
+      for (int k = 0; k < meqn; k++) any |= sHit[wv][lane * meqn + k];
+      const long long fi = dir == 0 ? (long long)(r - 1) * (mx + 1) + (ci - 1)
+                                    : (long long)(mx + 1) * my + (long long)(r - 1) * mx + (ci - 1);
+      flags[fi] = any;
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
@@ -522,9 +630,10 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
                      const int* refl, int meqn, float flat, const float* master, float zthr,
                      unsigned char* flags, hipStream_t s) {
   BCSpec bc = to_bc(lo, hi, refl, 2);
-  long long nint = (long long)(g.mx + 1) * g.my + (long long)g.mx * (g.my + 1);
   if (meqn < 1 || meqn > 64) return CLAWHIP_E_ARG;
-  long long passes = (nint + 64 / meqn - 1) / (64 / meqn);
+  const int ni = 64 / meqn;
+  long long passes = (long long)((g.mx + 1 + ni - 1) / ni) * g.my +
+                     (long long)((g.mx + ni - 1) / ni) * (g.my + 1);
   long long blocks = (passes + 3) / 4;
   if (blocks > 8192) blocks = 8192;
   if (dtype == CLAWHIP_F64)

@@ -29,8 +29,8 @@ inline BCSpec to_bc(const int* lo, const int* hi, const int* refl, int ndim) {
 
 int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
                    void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
-                   void* work, const int* lo, const int* hi, const int* refl, int ng,
-                   const int* method, const int* mthlim, double dx, double dy, double dt,
+                   const int* lo, const int* hi, const int* refl, int ng, const int* method,
+                   const int* mthlim, double dx, double dy, double dt,
                    unsigned long long* cfl_bits, hipStream_t stream);
 
 int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,

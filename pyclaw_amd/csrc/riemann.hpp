@@ -71,6 +71,14 @@ struct Acoustics {
     bm[0] = cc * a1 * zz; bm[mu] = T(0); bm[mv] = -cc * a1;
     bp[0] = cc * a2 * zz; bp[mu] = T(0); bp[mv] = cc * a2;
   }
+  // normal flux A q (bulk = rho c^2): amdq + apdq of rpn equals flux(qR) - flux(qL)
+  template <int IXY>
+  __device__ static __forceinline__ void flux(const RPParams<T>& p, const T* q, T (&f)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    f[0] = p.bulk * q[mu];
+    f[mu] = q[0] / p.rho;
+    f[mv] = T(0);
+  }
 };
 
 // ---------------------------------------------------------------- 2D Euler
@@ -78,6 +86,17 @@ template <typename T>
 struct Euler {
   static constexpr int meqn = 4, mwaves = 4;
   struct Roe { T u, v, enth, u2v2, a, g1a2, euv; };
+  // normal flux; the Roe linearisation gives amdq + apdq = flux(qR) - flux(qL)
+  template <int IXY>
+  __device__ static __forceinline__ void flux(const RPParams<T>& p, const T* q, T (&f)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T un = q[mu] / q[0];
+    const T pr = p.gamma1 * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    f[0] = q[mu];
+    f[mu] = q[mu] * un + pr;
+    f[mv] = q[mv] * un;
+    f[3] = (q[3] + pr) * un;
+  }
 
   template <int IXY>
   __device__ static __forceinline__ Roe roe_avg(T gamma1, const T* qL, const T* qR) {
@@ -218,6 +237,15 @@ template <typename T>
 struct Shallow {
   static constexpr int meqn = 3, mwaves = 3;
   struct Roe { T u, v, a; };
+  // normal flux; the Roe linearisation gives amdq + apdq = flux(qR) - flux(qL)
+  template <int IXY>
+  __device__ static __forceinline__ void flux(const RPParams<T>& p, const T* q, T (&f)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T un = q[mu] / q[0];
+    f[0] = q[mu];
+    f[mu] = q[mu] * un + T(0.5) * p.grav * q[0] * q[0];
+    f[mv] = q[mv] * un;
+  }
 
   template <int IXY>
   __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
@@ -340,6 +368,10 @@ struct Advection {
     T stran = IXY == 1 ? p.v : p.u;
     bm[0] = dmin(stran, T(0)) * asdq[0];
     bp[0] = dmax(stran, T(0)) * asdq[0];
+  }
+  template <int IXY>
+  __device__ static __forceinline__ void flux(const RPParams<T>& p, const T* q, T (&f)[meqn]) {
+    f[0] = (IXY == 1 ? p.u : p.v) * q[0];
   }
 };
 

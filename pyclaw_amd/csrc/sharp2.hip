@@ -16,6 +16,8 @@
 // direction live in LDS, per-cell x results in registers.  The stage combination
 // of SharpClawSolver.step (sharpclaw/solver.py:312-323, 400-430) is applied when
 // the tile's dq is complete, so each RK stage is a single pass over HBM.
+#include <stdlib.h>
+
 #include "launch.hpp"
 
 namespace claw {
@@ -287,6 +289,10 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
   __shared__ T sR[ME][TY + 2][TX + 1];   // qr(c-1)
   __shared__ T sAM[ME][TY + 2][TX + 1];
   __shared__ T sAP[ME][TY + 2][TX + 1];
+  // troubled interfaces of one direction (flagged path); sizes bound
+  // nx_int <= (TY + 2) * (TX + 1) and ny_int <= (TX + 2) * (TY + 1)
+  __shared__ short sTrouble[FLAGS ? (TX + 2) * (TY + 2) : 1];
+  __shared__ int sNTrouble;
 
   const int tid = threadIdx.x;
   const int bx = blockIdx.x % a.ntx, by = blockIdx.x / a.ntx;
@@ -318,7 +324,8 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
   const int xr_hi = min(j0 + TY - 1, my) == my ? my + 1 : j0 + TY - 1;
   const int nxr = xr_hi - xr_lo + 1;   // <= TY + 2
   // ---- x-interfaces c in [c0, c0+TX], rows xr_lo..xr_hi
-  for (int e = tid; e < nxr * (TX + 1); e += 256) {
+  const int nx_int = nxr * (TX + 1);
+  auto x_interface = [&](int e, bool weno) {
     const int k = e % (TX + 1), rr = e / (TX + 1);
     const int c = c0 + k, r = xr_lo + rr;
     const int ly = r - j0 + HALO;
@@ -328,21 +335,14 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < ME; m++) st[s][m] = sQ[m][ly][k + s];
     T qrL[ME], qlR[ME];
-    bool smooth = false;
-    if (FLAGS && c <= mx + 1 && r >= 1 && r <= my) {
-      // smooth when this interface and its two x-neighbours are unflagged (the
-      // flags come from the step's first stage; a wave moves < 1 cell a step)
-      const unsigned char* f = a.flags + (long long)(r - 1) * (mx + 1);
-      smooth = (f[c - 1] | f[max(c - 2, 0)] | f[min(c, mx)]) == 0;
-    }
-    if (smooth) recon_linear<ME>(st, qrL, qlR);
-    else recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
+    if (weno) recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
+    else recon_linear<ME>(st, qrL, qlR);
     T W[MW][ME], sp[MW], am[ME], ap[ME];
     typename RS::Roe roe;
     RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
     if (c <= mx + 1) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, fabs(sp[w]));
+      for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
     }
     const int ri = r - j0 + 1;   // 0 .. TY+1
 #pragma unroll
@@ -352,6 +352,30 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
       sAM[m][ri][k] = am[m];
       sAP[m][ri][k] = ap[m];
     }
+  };
+  if (!FLAGS) {
+    for (int e = tid; e < nx_int; e += 256) x_interface(e, true);
+  } else {
+    // smooth interfaces (this one and its two x-neighbours unflagged; the flags
+    // come from the step's first stage, and a wave moves < 1 cell a step) take
+    // the linear weights now; troubled ones are listed and then reconstructed
+    // by the whole block, so the WENO path runs on dense wavefronts.
+    if (tid == 0) sNTrouble = 0;
+    __syncthreads();
+    for (int e = tid; e < nx_int; e += 256) {
+      const int k = e % (TX + 1), rr = e / (TX + 1);
+      const int c = c0 + k, r = xr_lo + rr;
+      bool smooth = false;
+      if (c <= mx + 1 && r >= 1 && r <= my) {
+        const unsigned char* f = a.flags + (long long)(r - 1) * (mx + 1);
+        smooth = (f[c - 1] | f[max(c - 2, 0)] | f[min(c, mx)]) == 0;
+      }
+      if (smooth) x_interface(e, false);
+      else sTrouble[atomicAdd(&sNTrouble, 1)] = (short)e;
+    }
+    __syncthreads();
+    const int nt = sNTrouble;
+    for (int i = tid; i < nt; i += 256) x_interface(sTrouble[i], true);
   }
   __syncthreads();
   // ---- per-cell x contribution (thread = cell)
@@ -375,7 +399,8 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
   const int yc_lo = c0 == 1 ? 0 : c0;
   const int yc_hi = min(c0 + TX - 1, mx) == mx ? mx + 1 : c0 + TX - 1;
   const int nyc = yc_hi - yc_lo + 1;   // <= TX + 2
-  for (int e = tid; e < nyc * (TY + 1); e += 256) {
+  const int ny_int = nyc * (TY + 1);
+  auto y_interface = [&](int e, bool weno) {
     const int kc = e % nyc, k = e / nyc;
     const int c = yc_lo + kc, r = j0 + k;
     const int lx = c - c0 + HALO;
@@ -385,20 +410,14 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < ME; m++) st[s][m] = sQ[m][k + s][lx];
     T qrL[ME], qlR[ME];
-    bool smooth = false;
-    if (FLAGS && c >= 1 && c <= mx && r <= my + 1) {
-      const unsigned char* f = a.flags + (long long)(mx + 1) * my + (c - 1);
-      smooth = (f[(long long)(r - 1) * mx] | f[(long long)max(r - 2, 0) * mx] |
-                f[(long long)min(r, my) * mx]) == 0;
-    }
-    if (smooth) recon_linear<ME>(st, qrL, qlR);
-    else recon_interface<RS, 2, T, CHAR>(rp, st, qrL, qlR);
+    if (weno) recon_interface<RS, 2, T, CHAR>(rp, st, qrL, qlR);
+    else recon_linear<ME>(st, qrL, qlR);
     T W[MW][ME], sp[MW], am[ME], ap[ME];
     typename RS::Roe roe;
     RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
     if (r <= my + 1) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, fabs(sp[w]));
+      for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
     }
     if (c >= c0 && c < c0 + TX) {
       const int ci = c - c0;   // stored transposed: [ci][k]
@@ -411,6 +430,27 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
         (&sAP[m][0][0])[ci * (TY + 1) + k] = ap[m];
       }
     }
+  };
+  if (!FLAGS) {
+    for (int e = tid; e < ny_int; e += 256) y_interface(e, true);
+  } else {
+    if (tid == 0) sNTrouble = 0;
+    __syncthreads();
+    for (int e = tid; e < ny_int; e += 256) {
+      const int kc = e % nyc, k = e / nyc;
+      const int c = yc_lo + kc, r = j0 + k;
+      bool smooth = false;
+      if (c >= 1 && c <= mx && r <= my + 1) {
+        const unsigned char* f = a.flags + (long long)(mx + 1) * my + (c - 1);
+        smooth = (f[(long long)(r - 1) * mx] | f[(long long)max(r - 2, 0) * mx] |
+                  f[(long long)min(r, my) * mx]) == 0;
+      }
+      if (smooth) y_interface(e, false);
+      else sTrouble[atomicAdd(&sNTrouble, 1)] = (short)e;
+    }
+    __syncthreads();
+    const int nt = sNTrouble;
+    for (int i = tid; i < nt; i += 256) y_interface(sTrouble[i], true);
   }
   __syncthreads();
   T dqv[ME];
@@ -479,6 +519,216 @@ __global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------- march form
+// The same stage as k_sharp_stage, without LDS or barriers: a wavefront owns a
+// strip of 64 columns c0-3 .. c0+60 (lanes 3..60 write, the others supply
+// stencil neighbours) and marches down H rows.  Per row, the x-interfaces are
+// reconstructed lane-parallel (the 6-cell stencil arrives by cross-lane moves)
+// and each lane's y-interface is reconstructed from a register window of the
+// 6 rows around it; the interface results of the previous row are carried, so
+// every interface is solved once.  Arithmetic per interface and per cell is that
+// of k_sharp_stage (bitwise the same results).
+template <typename T>
+struct MarchArgs {
+  SharpArgs<T> s;
+  int H, nstrips, nrb;
+};
+
+template <int ME, typename T>
+__device__ __forceinline__ void lane_shift(const T (&v)[ME], int d, T (&o)[ME]) {
+#pragma unroll
+  for (int m = 0; m < ME; m++) o[m] = __shfl(v[m], (int)(threadIdx.x & 63) + d, 64);
+}
+
+// FP32 (fp32k) path: the fluctuation inside a cell, amdq + apdq of the Riemann
+// problem between its two reconstructed edges (flux1.f90:174-186), is the flux
+// difference f(qr) - f(ql), which the Roe solvers reproduce exactly in exact
+// arithmetic (their linearisation satisfies A_hat (qr - ql) = f(qr) - f(ql), and
+// the entropy fixes only move fluctuation between amdq and apdq).  FP64 keeps the
+// reference's Riemann solve, bit for bit.
+template <class RS, typename T, bool CHAR, bool FLAGS>
+__global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
+  constexpr int ME = RS::meqn, MW = RS::mwaves;
+  constexpr bool TFLUCT = sizeof(T) == 4;
+  const SharpArgs<T>& a = ma.s;
+  const int lane = threadIdx.x & 63;
+  const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wid >= ma.nstrips * ma.nrb) return;   // uniform per wavefront
+  const int strip = wid % ma.nstrips, rb = wid / ma.nstrips;
+  const int mx = a.g.mx, my = a.g.my;
+  const int c0 = 1 + strip * 58;
+  const int c = c0 - 3 + lane;                 // this lane's column
+  const int r0 = 1 + rb * ma.H;
+  const int rend = min(r0 + ma.H - 1, my);     // last output row
+  const bool out_lane = lane >= 3 && lane <= 60 && c >= 1 && c <= mx;
+  const bool xint_lane = lane >= 3 && lane <= 61 && c >= 1 && c <= mx + 1;
+  const bool ycfl_lane = c >= 0 && c <= mx + 1;
+  const RPParams<T>& rp = a.rp;
+  T smax_x = T(0), smax_y = T(0);
+
+  auto load_y = [&](int r, T (&y)[ME]) {
+    load_cell2<ME>(a.P, a.gin, a.bc, a.ng, c, r, y);
+    if (a.ymode != Y_ARRAY) {
+      T qv[ME];
+      load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, qv);
+#pragma unroll
+      for (int m = 0; m < ME; m++)
+        y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
+    }
+  };
+  // x-slice of row r: interface c (left of this lane's cell) -> qlR, qrL, am, ap
+  auto x_interface = [&](const T (&row)[ME], int r, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME],
+                         T (&ap)[ME]) {
+    T st[6][ME];
+#pragma unroll
+    for (int k = 0; k < 6; k++) lane_shift<ME>(row, k - 3, st[k]);
+    bool weno = true;
+    if (FLAGS && c >= 1 && c <= mx + 1 && r >= 1 && r <= my) {
+      const unsigned char* f = a.flags + (long long)(r - 1) * (mx + 1);
+      weno = (f[c - 1] | f[max(c - 2, 0)] | f[min(c, mx)]) != 0;
+    }
+    if (weno) recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
+    else recon_linear<ME>(st, qrL, qlR);
+    T W[MW][ME], sp[MW];
+    typename RS::Roe roe;
+    RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
+    if (xint_lane) {
+#pragma unroll
+      for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
+    }
+  };
+  // y-interface t (rows t-1 | t) of this lane's column from the window win[0..5]
+  // = rows t-3..t+2
+  auto y_interface = [&](const T (&win)[6][ME], int t, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME],
+                         T (&ap)[ME]) {
+    bool weno = true;
+    if (FLAGS && c >= 1 && c <= mx && t <= my + 1) {
+      const unsigned char* f = a.flags + (long long)(mx + 1) * my + (c - 1);
+      weno = (f[(long long)(t - 1) * mx] | f[(long long)max(t - 2, 0) * mx] |
+              f[(long long)min(t, my) * mx]) != 0;
+    }
+    if (weno) recon_interface<RS, 2, T, CHAR>(rp, win, qrL, qlR);
+    else recon_linear<ME>(win, qrL, qlR);
+    T W[MW][ME], sp[MW];
+    typename RS::Roe roe;
+    RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
+    if (ycfl_lane && t <= my + 1) {
+#pragma unroll
+      for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
+    }
+  };
+  // x contribution of this lane's cell in row r (a whole x-slice pass)
+  auto x_cell = [&](const T (&row)[ME], int r, T (&dqx)[ME]) {
+    T qrL[ME], qlR[ME], am[ME], ap[ME];
+    x_interface(row, r, qrL, qlR, am, ap);
+    T qr[ME], amn[ME];
+    lane_shift<ME>(qrL, 1, qr);     // right edge of this cell: interface c+1
+    lane_shift<ME>(am, 1, amn);     // amdq at interface c+1
+    if (TFLUCT) {
+      T fl[ME], fr[ME];
+      RS::template flux<1>(rp, qlR, fl);
+      RS::template flux<1>(rp, qr, fr);
+#pragma unroll
+      for (int m = 0; m < ME; m++) dqx[m] = T(0) - a.dtdx * ((amn[m] + ap[m]) + (fr[m] - fl[m]));
+    } else {
+      T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+      typename RS::Roe roe;
+      RS::template rpn<1>(rp, qlR, qr, W, sp, am2, ap2, roe);
+#pragma unroll
+      for (int m = 0; m < ME; m++)
+        dqx[m] = T(0) - a.dtdx * (((amn[m] + ap[m]) + am2[m]) + ap2[m]);
+    }
+  };
+
+  // ---- window rows r0-3 .. r0+2; y-interface r0
+  T win[6][ME];
+#pragma unroll
+  for (int k = 0; k < 6; k++) load_y(r0 - 3 + k, win[k]);
+  if (r0 == 1) {   // x-slice of row 0 (CFL only)
+    T d[ME];
+    x_cell(win[2], 0, d);
+  }
+  T qlY[ME], apY[ME];   // interface t-1 carried: lower edge of row t-1, apdq
+  {
+    T qrL[ME], am[ME];
+    y_interface(win, r0, qrL, qlY, am, apY);
+  }
+  for (int t = r0 + 1; t <= rend + 1; t++) {
+    // row t-1 = win[3]; shift the window to rows t-3..t+2
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+      for (int m = 0; m < ME; m++) win[k][m] = win[k + 1][m];
+    load_y(t + 2, win[5]);
+    // y-interface t closes row t-1
+    T qrL[ME], qlR[ME], am[ME], ap[ME];
+    y_interface(win, t, qrL, qlR, am, ap);
+    T dqy[ME];
+    if (TFLUCT) {
+      T fl[ME], fr[ME];
+      RS::template flux<2>(rp, qlY, fl);
+      RS::template flux<2>(rp, qrL, fr);
+#pragma unroll
+      for (int m = 0; m < ME; m++) dqy[m] = T(0) - a.dtdy * ((am[m] + apY[m]) + (fr[m] - fl[m]));
+    } else {
+      T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+      typename RS::Roe roe;
+      RS::template rpn<2>(rp, qlY, qrL, W, sp, am2, ap2, roe);
+#pragma unroll
+      for (int m = 0; m < ME; m++)
+        dqy[m] = T(0) - a.dtdy * (((am[m] + apY[m]) + am2[m]) + ap2[m]);
+    }
+    // x-slice of row t-1
+    T dqx[ME];
+    x_cell(win[2], t - 1, dqx);
+    const int r = t - 1;
+    if (out_lane) {
+      const long long off = (long long)(c - 1) + (long long)(r - 1) * a.g.rs;
+      const long long fs = a.g.fs;
+#pragma unroll
+      for (int m = 0; m < ME; m++) {
+        const T D = (T(0) + dqx[m]) + dqy[m];
+        const T yv = win[2][m];
+        T o;
+        switch (a.omode) {
+          case OUT_DQDT: o = D / a.dt; break;
+          case OUT_SSP33_2: o = T(0.75) * a.A[off + m * fs] + T(0.25) * (yv + D); break;
+          case OUT_SSP33_3: o = T(1. / 3.) * a.A[off + m * fs] + T(2. / 3.) * (yv + D); break;
+          case OUT_ADD6: o = yv + D / T(6); break;
+          case OUT_SSP104_FIN: o = a.A[off + m * fs] + (T(0.6) * yv + T(0.1) * D); break;
+          case OUT_D: o = D; break;
+          case OUT_F32_S2: {
+            T e1 = a.s * a.B[off + m * fs];
+            T inc = e1 + D;
+            o = a.A[off + m * fs] + T(0.25) * inc;
+            a.aux_out[off + m * fs] = inc * T(1. / 6.);
+            break;
+          }
+          default: {   // OUT_F32_S3
+            T delta = a.aux_in[off + m * fs] + T(2. / 3.) * D;
+            T qa = a.A[off + m * fs];
+            T yk = delta - a.comp_in[off + m * fs];
+            T tq = qa + yk;
+            a.comp_out[off + m * fs] = (tq - qa) - yk;
+            o = tq;
+            break;
+          }
+        }
+        a.out[off + m * fs] = o;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < ME; m++) { qlY[m] = qlR[m]; apY[m] = ap[m]; }
+  }
+  if (rend == my) {   // x-slice of row my+1 (CFL only) = window row index 3 now
+    T d[ME];
+    x_cell(win[3], my + 1, d);
+  }
+  double cx = (double)a.dtdx * (double)smax_x;
+  double cy = (double)a.dtdy * (double)smax_y;
+  wave_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
+}
+
 // elementwise SSP104 middle update (solver.py:418-419):
 //   q = q/25. + 0.36*s1 ;  s1 = 15.*q - 5.*s1
 template <typename T>
@@ -492,6 +742,40 @@ __global__ void k_ssp104_mid(T* q, T* s1, long long n) {
 
 // ---------------------------------------------------------------- launchers
 namespace {
+// Rows per wavefront of the march form: with R wavefronts resident, W in the
+// launch, minimise ceil(W / R) * (H + 6) (6 = the window rows loaded and the
+// interface reconstructed before the first output row).  CLAWHIP_SHARP_ROWS
+// overrides it.
+template <class Kernel>
+int march_rows(Kernel kernel, int nstrips, int my) {
+  static long long resident = 0;   // per kernel instantiation
+  if (resident == 0) {
+    int dev = 0, ncu = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess ||
+        ncu < 1 || nb < 1)
+      return 16;
+    resident = (long long)ncu * nb * 4;
+  }
+  if (const char* env = getenv("CLAWHIP_SHARP_ROWS")) {
+    int h = atoi(env);
+    if (h >= 1 && h <= 65536) return h;
+  }
+  int best = 16;
+  long long best_cost = -1;
+  for (int h = 2; h <= 256; h++) {
+    const long long waves = (long long)nstrips * ((my + h - 1) / h);
+    const long long cost = ((waves + resident - 1) / resident) * (long long)(h + 6);
+    if (best_cost < 0 || cost < best_cost || (cost == best_cost && h > best)) {
+      best = h;
+      best_cost = cost;
+    }
+    if ((my + h - 1) / h == 1) break;
+  }
+  return best;
+}
+
 template <class RS, typename T, bool CHAR>
 int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* lo, const int* hi,
               const int* refl, int ymode, int omode, double dt, double s, double dx, double dy,
@@ -514,12 +798,29 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
   a.out = (T*)out; a.aux_out = (T*)aux_out; a.comp_out = (T*)comp_out;
   a.cfl_bits = cfl;
   a.flags = flags;
-  if (flags)
-    hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, true>), dim3(a.ntx * a.nty), dim3(256), 0,
-                       stream, a);
-  else
-    hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, false>), dim3(a.ntx * a.nty), dim3(256), 0,
-                       stream, a);
+  const char* tile = getenv("CLAWHIP_SHARP_TILE");
+  if (tile && tile[0] == '1') {
+    if (flags)
+      hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, true>), dim3(a.ntx * a.nty), dim3(256), 0,
+                         stream, a);
+    else
+      hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, false>), dim3(a.ntx * a.nty), dim3(256),
+                         0, stream, a);
+  } else {
+    MarchArgs<T> ma;
+    ma.s = a;
+    ma.nstrips = (g.mx + 57) / 58;
+    ma.H = flags ? march_rows(k_sharp_march<RS, T, CHAR, true>, ma.nstrips, g.my)
+                 : march_rows(k_sharp_march<RS, T, CHAR, false>, ma.nstrips, g.my);
+    ma.nrb = (g.my + ma.H - 1) / ma.H;
+    const unsigned blocks = (unsigned)(((long long)ma.nstrips * ma.nrb + 3) / 4);
+    if (flags)
+      hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
+                         ma);
+    else
+      hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, false>), dim3(blocks), dim3(256), 0, stream,
+                         ma);
+  }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
 }

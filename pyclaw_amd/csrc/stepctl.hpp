@@ -26,7 +26,6 @@ struct StepCtl {
 struct CtlBuffers {
   const void* q[2];
   void* comp[2];
-  void* work;       // the x-sweep's result (compact, like q)
   double dx, dy;
 };
 

@@ -60,14 +60,13 @@ class Ensemble(object):
             st = sol.state
             q0 = q_to_device(np.asarray(st.q), self._code)
             q1 = torch.empty_like(q0)
-            work = torch.empty_like(q0)
             comp = (torch.zeros_like(q0), torch.zeros_like(q0)) if self._code == L.F32K \
                 else (None, None)
-            self._bufs.append((q0, q1, comp[0], comp[1], work))
+            self._bufs.append((q0, q1, comp[0], comp[1]))
             dx, dy = st.grid.delta
             m = self._members[k]
             m.mx, m.my = q0.shape[2], q0.shape[1]
-            m.q0, m.q1, m.work = q0.data_ptr(), q1.data_ptr(), work.data_ptr()
+            m.q0, m.q1 = q0.data_ptr(), q1.data_ptr()
             m.comp0 = comp[0].data_ptr() if comp[0] is not None else None
             m.comp1 = comp[1].data_ptr() if comp[1] is not None else None
             m.dx, m.dy = dx, dy

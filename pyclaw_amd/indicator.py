@@ -55,7 +55,8 @@ class TroubledInterfaceIndicator(object):
         self.threshold = float(meta.get("threshold", 0.0) if threshold is None else threshold)
         self.flat = float(meta.get("flat", 1e-6) if flat is None else flat)
         self._flags = None
-        self.last_fraction = None
+        self.every = 1          # steps between flag refreshes (SharpClaw SSP33)
+        self._age = None
 
     def flags(self, q, bc_lower, bc_upper, reflect=(1, 2)):
         """uint8 flags of every interface of the planar state q (meqn, my, mx)."""
@@ -65,6 +66,17 @@ class TroubledInterfaceIndicator(object):
             self._flags = torch.empty(n, dtype=torch.uint8, device=q.device)
         K.mlp_flags(q, bc_lower, bc_upper, reflect, self.flat, self.params, self.threshold,
                     self._flags)
+        return self._flags
+
+    def step_flags(self, q, bcs, reflect=(1, 2)):
+        """Flags for the next time step: recomputed from q every `every` steps
+        (with CFL <= 0.5 a wave moves at most one cell in two steps, which the
+        stage kernel's one-interface dilation of the flags covers)."""
+        if self._age is None or self._age >= self.every or self._flags is None \
+                or self._flags.numel() != K.mlp_num_interfaces(q.shape[2], q.shape[1]):
+            self.flags(q, bcs[0], bcs[1], reflect)
+            self._age = 0
+        self._age += 1
         return self._flags
 
     def troubled_fraction(self):

@@ -43,25 +43,21 @@ def cfl_reset(cfl):
 
 
 def step2_unsplit(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim,
-                  dx, dy, dt, comp_in=None, comp_out=None, num_ghost=2, gin=None, work=None):
-    """One unsplit step q_in -> q_out (planar (meqn, my, mx) tensors).  work: a
-    buffer shaped like q_out (allocated when None); returned for reuse."""
+                  dx, dy, dt, comp_in=None, comp_out=None, num_ghost=2, gin=None):
+    """One unsplit step q_in -> q_out (planar (meqn, my, mx) tensors)."""
     meqn, my, mx = q_out.shape
     gout = L.geom2(mx, my)
     if gin is None:
         gin = gout
     code = _dtype_code(q_out)
     assert q_in.dtype == q_out.dtype and q_in.is_contiguous() and q_out.is_contiguous()
-    if work is None or work.shape != q_out.shape or work.dtype != q_out.dtype:
-        work = torch.empty_like(q_out)
     rc = L.lib().clawhip_step2_unsplit(
         ctypes.byref(rp), code, _ptr(q_in), ctypes.byref(gin), _ptr(q_out), ctypes.byref(gout),
-        _ptr(comp_in), _ptr(comp_out), _ptr(work), L.ints(bc_lower), L.ints(bc_upper),
+        _ptr(comp_in), _ptr(comp_out), L.ints(bc_lower), L.ints(bc_upper),
         L.ints(reflect), int(num_ghost), L.ints(method),
         L.ints(list(mthlim) + [0] * (4 - len(mthlim))), float(dx), float(dy), float(dt),
         _ptr(cfl), _stream())
     L.check(rc, "clawhip_step2_unsplit")
-    return work
 
 
 def _work_like(q_out, ng, work):
@@ -141,10 +137,9 @@ def ssp104_mid(q, s1):
     L.check(rc, "clawhip_ssp104_mid")
 
 
-def step2_unsplit_run(rp, bufs, comps, work, ctl_dev, bc_lower, bc_upper, reflect, method,
-                      mthlim, dx, dy, nsteps, num_ghost=2):
-    """nsteps device-controlled attempts (clawhip_step2_unsplit_run); work is a
-    buffer shaped like the state."""
+def step2_unsplit_run(rp, bufs, comps, ctl_dev, bc_lower, bc_upper, reflect, method, mthlim,
+                      dx, dy, nsteps, num_ghost=2):
+    """nsteps device-controlled attempts (clawhip_step2_unsplit_run)."""
     meqn, my, mx = bufs[0].shape
     g = L.geom2(mx, my)
     code = _dtype_code(bufs[0])
@@ -152,8 +147,7 @@ def step2_unsplit_run(rp, bufs, comps, work, ctl_dev, bc_lower, bc_upper, reflec
         ctypes.byref(rp), code, ctypes.byref(g), L.ints(bc_lower), L.ints(bc_upper),
         L.ints(reflect), int(num_ghost), L.ints(method),
         L.ints(list(mthlim) + [0] * (4 - len(mthlim))), float(dx), float(dy), _ptr(bufs[0]),
-        _ptr(bufs[1]), _ptr(comps[0]), _ptr(comps[1]), _ptr(work), _ptr(ctl_dev), int(nsteps),
-        _stream())
+        _ptr(bufs[1]), _ptr(comps[0]), _ptr(comps[1]), _ptr(ctl_dev), int(nsteps), _stream())
     L.check(rc, "clawhip_step2_unsplit_run")
 
 
