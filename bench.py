@@ -103,11 +103,12 @@ class ChunkTimer(object):
         return tot / max(n, 1), n
 
 
-def run_gpu(precision, steps, warmup):
+def run_gpu(precision, steps, warmup, use_graph=True):
     """warmup steps (they include the graph capture), then `steps` timed
     device-loop steps (graph replays of 16 steps)."""
     import torch
     solver, solution = make_problem(precision)
+    solver.use_graph = use_graph
     solver.run_steps(solution, warmup)
     torch.cuda.synchronize()
     timer = ChunkTimer(solver)
@@ -120,7 +121,21 @@ def run_gpu(precision, steps, warmup):
     finally:
         timer.close()
     kms, nk = timer.kernel_ms()
+    if nk == 0:          # eager launches: the wall time per step stands in
+        kms = 1e3 * wall / steps
     return wall, kms, nk, host, solution
+
+
+def measured_traffic():
+    """HBM bytes per k_unsplit_ctl<Euler<double>> launch on this workload, from
+    the rocprofv3 FETCH_SIZE and WRITE_SIZE runs summarised in profiles/ (each
+    scaled by the same counters' reading for a 256 MiB copy); None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_unsplit_ctl_euler2d_1024.traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_baseline(sample_steps=3):
@@ -288,14 +303,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extras", default="ensemble,sharpclaw,euler3d",
                     help="comma list of extra workloads to time beside the headline")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the steps one by one (for per-dispatch counter runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with a plain 256 MiB device copy (known bytes for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X in this build "
               "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
         sys.exit(2)
     cells = MX * MY
-    wall64, k64, n64, host64, sol64 = run_gpu("fp64", args.steps, args.warmup)
-    wall32, k32, n32, host32, sol32 = run_gpu("fp32k", args.steps, args.warmup)
+    wall64, k64, n64, host64, sol64 = run_gpu("fp64", args.steps, args.warmup, not args.no_graph)
+    wall32, k32, n32, host32, sol32 = run_gpu("fp32k", args.steps, args.warmup,
+                                              not args.no_graph)
     q64 = sol64.state.q
     q32 = sol32.state.q
     err_l1 = float(np.abs(q32 - q64).sum() / np.abs(q64).sum())
@@ -321,7 +341,7 @@ def main():
                    "seed": 2, "t_end": sol64.t, "accepted": host64.accepted,
                    "rejected": host64.nrejected},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_unsplit_ctl<Euler<double>>", "kernel_ms": k64,
                      "kernels_timed": n64, "bytes_per_cell": BYTES_PER_CELL_F64},
         "fp32_kahan": {"value": cells * args.steps / wall32,
@@ -339,6 +359,12 @@ def main():
         print(f"[bench] {name} done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
     print(json.dumps(rec))
 
 

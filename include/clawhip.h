@@ -161,16 +161,20 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
  *            7: out = D                         (fmod.flux2's dq on the interior)
  * All arrays compact (ghosts = 0) with geometry g; num_ghost is 3 (WENO5).
  * Boundary codes 1..3 only.  *cfl_dev = max(*cfl_dev, CFL of this evaluation).
- * flags (may be NULL): per-interface troubled flags from
- * clawhip_mlp_flags; interfaces flagged 0 use WENO5's linear 5th-order
- * reconstruction instead of the characteristic WENO5.  NULL = full WENO5. */
+ * flags (may be NULL): per-interface troubled flags from clawhip_mlp_flags.
+ * An interface whose flag and whose two neighbours' flags (along its normal) are
+ * 0 uses WENO5's linear 5th-order weights; the others use the WENO5 selected by
+ * char_decomp.  troubled_cells / troubled_count: the cells next to a flagged
+ * interface, from clawhip_mlp_troubled_cells (required with flags).  NULL flags
+ * = WENO5 everywhere (the reference's operator). */
 int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
                          const clawhip_geom *g, const int *bc_lower, const int *bc_upper,
                          const int *reflect, int y_mode, int out_mode, double dt, double s,
                          double dx, double dy, const void *P, const void *Q, const void *A,
                          const void *B, const void *aux_in, const void *comp_in, void *out,
                          void *aux_out, void *comp_out, const unsigned char *flags,
-                         double *cfl_dev, void *stream);
+                         const int *troubled_cells, const int *troubled_count, double *cfl_dev,
+                         void *stream);
 
 /* The elementwise middle of SSP104 (sharpclaw/solver.py:418-419):
  *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */
@@ -270,7 +274,14 @@ int clawhip_mlp_train_step(const float *feat, long long nsamples, const unsigned
 int clawhip_mlp_logits(const float *feat, long long n, const float *params, float *logits,
                        void *stream);
 
-/* Per-interface flags of a state: 1 when any component's logit > logit_threshold. */
+/* Per-interface flags of a state: 1 when any component whose stencil range exceeds
+ * flat has a logit > logit_threshold. */
+/* The cells (index (r-1)*mx + (c-1)) with at least one flagged interface among
+ * their four, flags dilated as clawhip_sharp2_stage applies them; *count = their
+ * number (cells holds up to mx*my entries, in no particular order). */
+int clawhip_mlp_troubled_cells(const unsigned char *flags, int mx, int my, int *cells,
+                               int *count, void *stream);
+
 int clawhip_mlp_flags(int dtype, const void *q, const clawhip_geom *g, const int *bc_lower,
                       const int *bc_upper, const int *reflect, int meqn, float flat,
                       const float *params, float logit_threshold, unsigned char *flags,

@@ -91,7 +91,7 @@ _SIGS = {
                             _D, _D, _D, _D, _P, _P],
     "clawhip_sharp2_stage": [ctypes.POINTER(RP), _I, _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                              _I, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                             _P],
+                             _P, _P, _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_step2_unsplit_run": [ctypes.POINTER(RP), _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                                   _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _I, _P],
@@ -113,6 +113,7 @@ _SIGS = {
     "clawhip_mlp_train_step": [_P, ctypes.c_longlong, _P, _P, _I, _P, _P, _P, _P, _I, _F, _F,
                                _P, _P],
     "clawhip_mlp_logits": [_P, ctypes.c_longlong, _P, _P, _P],
+    "clawhip_mlp_troubled_cells": [_P, _I, _I, _P, _P, _P],
     "clawhip_mlp_flags": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _P, _F, _P, _P],
 }
 

@@ -145,9 +145,11 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
                          double dx, double dy, const void* P, const void* Q, const void* A,
                          const void* B, const void* aux_in, const void* comp_in, void* out,
                          void* aux_out, void* comp_out, const unsigned char* flags,
-                         double* cfl_dev, void* stream) {
+                         const int* troubled_cells, const int* troubled_count, double* cfl_dev,
+                         void* stream) {
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !P || !out || !cfl_dev)
     return CLAWHIP_E_ARG;
+  if (flags && (!troubled_cells || !troubled_count)) return CLAWHIP_E_ARG;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
@@ -166,7 +168,7 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
   return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), to_geom(g), bc_lower, bc_upper, reflect,
                             y_mode, out_mode, dt, s, dx, dy, P, Q, A, B, aux_in, comp_in, out,
                             aux_out, comp_out, (unsigned long long*)cfl_dev, flags,
-                            (hipStream_t)stream);
+                            troubled_cells, troubled_count, (hipStream_t)stream);
 }
 
 int clawhip_ssp104_mid(int dtype, void* q, void* s1, long long n, void* stream) {
@@ -310,7 +312,7 @@ int clawhip_sharp2_flux(const clawhip_rp* rp, int dtype, int char_decomp, const 
   return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(gout), to_geom(gin), given, given,
                             refl, 0, 7, dt, 0.0, dx, dy, q, nullptr, nullptr, nullptr, nullptr,
                             nullptr, dq, nullptr, nullptr, (unsigned long long*)cfl_dev, nullptr,
-                            (hipStream_t)stream);
+                            nullptr, nullptr, (hipStream_t)stream);
 }
 
 int clawhip_mlp_param_count(void) { return 16 * 32 + 32 * 32 + 32 + 32 + 1; }
@@ -344,6 +346,12 @@ int clawhip_mlp_logits(const float* feat, long long n, const float* params, floa
   if (!feat || !params || !logits || n < 0) return CLAWHIP_E_ARG;
   if (n == 0) return 0;
   return launch_mlp_logits(feat, n, params, logits, (hipStream_t)stream);
+}
+
+int clawhip_mlp_troubled_cells(const unsigned char* flags, int mx, int my, int* cells,
+                               int* count, void* stream) {
+  if (!flags || !cells || !count || mx < 3 || my < 3) return CLAWHIP_E_ARG;
+  return launch_troubled_cells(flags, mx, my, cells, count, (hipStream_t)stream);
 }
 
 int clawhip_mlp_flags(int dtype, const void* q, const clawhip_geom* g, const int* bc_lower,

@@ -50,7 +50,8 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
                        double dt, double s, double dx, double dy, const void* P, const void* Q,
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
-                       const unsigned char* flags, hipStream_t stream);
+                       const unsigned char* flags, const int* cells, const int* ncells,
+                       hipStream_t stream);
 int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream);
 
 int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int* lo,
@@ -80,6 +81,8 @@ int launch_mlp_train(const float* feat, long long nsamples, const unsigned char*
                      float* m2, int step, float lr, float pos_weight, float* loss, hipStream_t s);
 int launch_mlp_logits(const float* feat, long long n, const float* master, float* logits,
                       hipStream_t s);
+int launch_troubled_cells(const unsigned char* flags, int mx, int my, int* cells, int* count,
+                          hipStream_t s);
 int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
                      const int* refl, int meqn, float flat, const float* master, float zthr,
                      unsigned char* flags, hipStream_t s);

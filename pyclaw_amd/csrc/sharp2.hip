@@ -11,11 +11,12 @@
 // (dq = 0 + dqx + dqy).  The x-slices cover rows 0..my+1 and the y-slices
 // columns 0..mx+1; their ghost-row / ghost-column parts only enter the CFL.
 //
-// One workgroup owns a TX x TY tile; the input state plus a 3-cell halo (ghost
-// values from the boundary map) is staged in LDS; interface results for one
-// direction live in LDS, per-cell x results in registers.  The stage combination
-// of SharpClawSolver.step (sharpclaw/solver.py:312-323, 400-430) is applied when
-// the tile's dq is complete, so each RK stage is a single pass over HBM.
+// k_sharp_march computes a stage in one pass over HBM: each wavefront marches a
+// 64-column strip down its rows (details below), and the stage combination of
+// SharpClawSolver.step (sharpclaw/solver.py:312-323, 400-430) is applied as each
+// cell's dq is complete.  With the learned indicator's flags, k_sharp_march
+// takes the linear weights everywhere and k_sharp_patch recomputes the cells
+// next to a flagged interface.
 #include <stdlib.h>
 
 #include "launch.hpp"
@@ -64,7 +65,6 @@ struct SharpArgs {
   T dt;                 // dt passed to the operator (dq = dt*L)
   T s;                  // scalar of the Y / e1 axpy
   T dtdx, dtdy;
-  int ntx, nty;
   const T* P;           // Y source
   const T* Q;           // axpy partner of P
   const T* A;           // stage anchor (q at the start of the step)
@@ -76,10 +76,10 @@ struct SharpArgs {
   T* comp_out;
   unsigned long long* cfl_bits;
   const unsigned char* flags;   // troubled-interface flags (learned indicator) or null
+  const int* cells;             // with flags: cells next to a flagged interface
+  const int* ncells;            //   and their number (device)
 };
 
-constexpr int TX = 32, TY = 8, HALO = 3;
-constexpr int LX = TX + 2 * HALO, LY = TY + 2 * HALO;
 
 // ---------------------------------------------------------------- WENO5
 // weno.f90:32-79 for one cell: ql = left edge (fs0), qr = right edge (fs1).
@@ -281,253 +281,63 @@ __device__ __forceinline__ void recon_interface(const RPParams<T>& p,
   }
 }
 
-template <class RS, typename T, bool CHAR, bool FLAGS>
-__global__ void __launch_bounds__(256) k_sharp_stage(const SharpArgs<T> a) {
-  constexpr int ME = RS::meqn, MW = RS::mwaves;
-  __shared__ T sQ[ME][LY][LX];
-  __shared__ T sL[ME][TY + 2][TX + 1];   // ql(c)  for x; reused for y
-  __shared__ T sR[ME][TY + 2][TX + 1];   // qr(c-1)
-  __shared__ T sAM[ME][TY + 2][TX + 1];
-  __shared__ T sAP[ME][TY + 2][TX + 1];
-  // troubled interfaces of one direction (flagged path); sizes bound
-  // nx_int <= (TY + 2) * (TX + 1) and ny_int <= (TX + 2) * (TY + 1)
-  __shared__ short sTrouble[FLAGS ? (TX + 2) * (TY + 2) : 1];
-  __shared__ int sNTrouble;
+// Stage combination of cell offset `off`, field m (SharpClawSolver.step,
+// sharpclaw/solver.py:312-323, 400-430; the FP32 stages keep a compensated sum).
+template <typename T>
+__device__ __forceinline__ void stage_store(const SharpArgs<T>& a, long long off, int m, T D,
+                                            T yv) {
+  const long long fs = a.g.fs;
+  T o;
+  switch (a.omode) {
+    case OUT_DQDT: o = D / a.dt; break;
+    case OUT_SSP33_2: o = T(0.75) * a.A[off + m * fs] + T(0.25) * (yv + D); break;
+    case OUT_SSP33_3: o = T(1. / 3.) * a.A[off + m * fs] + T(2. / 3.) * (yv + D); break;
+    case OUT_ADD6: o = yv + D / T(6); break;
+    case OUT_SSP104_FIN: o = a.A[off + m * fs] + (T(0.6) * yv + T(0.1) * D); break;
+    case OUT_D: o = D; break;
+    case OUT_F32_S2: {
+      T e1 = a.s * a.B[off + m * fs];
+      T inc = e1 + D;
+      o = a.A[off + m * fs] + T(0.25) * inc;
+      a.aux_out[off + m * fs] = inc * T(1. / 6.);
+      break;
+    }
+    default: {   // OUT_F32_S3
+      T delta = a.aux_in[off + m * fs] + T(2. / 3.) * D;
+      T qa = a.A[off + m * fs];
+      T yk = delta - a.comp_in[off + m * fs];
+      T tq = qa + yk;
+      a.comp_out[off + m * fs] = (tq - qa) - yk;
+      o = tq;
+      break;
+    }
+  }
+  a.out[off + m * fs] = o;
+}
 
-  const int tid = threadIdx.x;
-  const int bx = blockIdx.x % a.ntx, by = blockIdx.x / a.ntx;
-  const int c0 = 1 + bx * TX, j0 = 1 + by * TY;
-  const int mx = a.g.mx, my = a.g.my;
-  const RPParams<T>& rp = a.rp;
-
-  // ---- stage Y (with halo) into LDS
-  for (int e = tid; e < LX * LY; e += 256) {
-    const int lx = e % LX, ly = e / LX;
-    const int c = c0 - HALO + lx, r = j0 - HALO + ly;
-    T y[ME];
-    load_cell2<ME>(a.P, a.gin, a.bc, a.ng, c, r, y);
-    if (a.ymode != Y_ARRAY) {
-      T qv[ME];
-      load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, qv);
-#pragma unroll
-      for (int m = 0; m < ME; m++)
-        y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
-    }
-#pragma unroll
-    for (int m = 0; m < ME; m++) sQ[m][ly][lx] = y[m];
-  }
-  __syncthreads();
-
-  T smax_x = T(0), smax_y = T(0);
-  // rows of x-slices handled by this tile: [j0, j0+TY) plus rows 0 / my+1 (CFL only)
-  const int xr_lo = j0 == 1 ? 0 : j0;
-  const int xr_hi = min(j0 + TY - 1, my) == my ? my + 1 : j0 + TY - 1;
-  const int nxr = xr_hi - xr_lo + 1;   // <= TY + 2
-  // ---- x-interfaces c in [c0, c0+TX], rows xr_lo..xr_hi
-  const int nx_int = nxr * (TX + 1);
-  auto x_interface = [&](int e, bool weno) {
-    const int k = e % (TX + 1), rr = e / (TX + 1);
-    const int c = c0 + k, r = xr_lo + rr;
-    const int ly = r - j0 + HALO;
-    T st[6][ME];
-#pragma unroll
-    for (int s = 0; s < 6; s++)
-#pragma unroll
-      for (int m = 0; m < ME; m++) st[s][m] = sQ[m][ly][k + s];
-    T qrL[ME], qlR[ME];
-    if (weno) recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
-    else recon_linear<ME>(st, qrL, qlR);
-    T W[MW][ME], sp[MW], am[ME], ap[ME];
-    typename RS::Roe roe;
-    RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
-    if (c <= mx + 1) {
-#pragma unroll
-      for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
-    }
-    const int ri = r - j0 + 1;   // 0 .. TY+1
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      sL[m][ri][k] = qlR[m];
-      sR[m][ri][k] = qrL[m];
-      sAM[m][ri][k] = am[m];
-      sAP[m][ri][k] = ap[m];
-    }
-  };
-  if (!FLAGS) {
-    for (int e = tid; e < nx_int; e += 256) x_interface(e, true);
-  } else {
-    // smooth interfaces (this one and its two x-neighbours unflagged; the flags
-    // come from the step's first stage, and a wave moves < 1 cell a step) take
-    // the linear weights now; troubled ones are listed and then reconstructed
-    // by the whole block, so the WENO path runs on dense wavefronts.
-    if (tid == 0) sNTrouble = 0;
-    __syncthreads();
-    for (int e = tid; e < nx_int; e += 256) {
-      const int k = e % (TX + 1), rr = e / (TX + 1);
-      const int c = c0 + k, r = xr_lo + rr;
-      bool smooth = false;
-      if (c <= mx + 1 && r >= 1 && r <= my) {
-        const unsigned char* f = a.flags + (long long)(r - 1) * (mx + 1);
-        smooth = (f[c - 1] | f[max(c - 2, 0)] | f[min(c, mx)]) == 0;
-      }
-      if (smooth) x_interface(e, false);
-      else sTrouble[atomicAdd(&sNTrouble, 1)] = (short)e;
-    }
-    __syncthreads();
-    const int nt = sNTrouble;
-    for (int i = tid; i < nt; i += 256) x_interface(sTrouble[i], true);
-  }
-  __syncthreads();
-  // ---- per-cell x contribution (thread = cell)
-  const int tx = tid % TX, ty = tid / TX;
-  const int cc = c0 + tx, cr = j0 + ty;
-  T dqx[ME];
-  {
-    const int ri = ty + 1;
-    T ql[ME], qr[ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) { ql[m] = sL[m][ri][tx]; qr[m] = sR[m][ri][tx + 1]; }
-    T W[MW][ME], sp[MW], am2[ME], ap2[ME];
-    typename RS::Roe roe;
-    RS::template rpn<1>(rp, ql, qr, W, sp, am2, ap2, roe);
-#pragma unroll
-    for (int m = 0; m < ME; m++)
-      dqx[m] = T(0) - a.dtdx * (((sAM[m][ri][tx + 1] + sAP[m][ri][tx]) + am2[m]) + ap2[m]);
-  }
-  __syncthreads();
-  // ---- y-interfaces r in [j0, j0+TY], columns xc_lo..xc_hi
-  const int yc_lo = c0 == 1 ? 0 : c0;
-  const int yc_hi = min(c0 + TX - 1, mx) == mx ? mx + 1 : c0 + TX - 1;
-  const int nyc = yc_hi - yc_lo + 1;   // <= TX + 2
-  const int ny_int = nyc * (TY + 1);
-  auto y_interface = [&](int e, bool weno) {
-    const int kc = e % nyc, k = e / nyc;
-    const int c = yc_lo + kc, r = j0 + k;
-    const int lx = c - c0 + HALO;
-    T st[6][ME];
-#pragma unroll
-    for (int s = 0; s < 6; s++)
-#pragma unroll
-      for (int m = 0; m < ME; m++) st[s][m] = sQ[m][k + s][lx];
-    T qrL[ME], qlR[ME];
-    if (weno) recon_interface<RS, 2, T, CHAR>(rp, st, qrL, qlR);
-    else recon_linear<ME>(st, qrL, qlR);
-    T W[MW][ME], sp[MW], am[ME], ap[ME];
-    typename RS::Roe roe;
-    RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
-    if (r <= my + 1) {
-#pragma unroll
-      for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
-    }
-    if (c >= c0 && c < c0 + TX) {
-      const int ci = c - c0;   // stored transposed: [ci][k]
-#pragma unroll
-      for (int m = 0; m < ME; m++) {
-        // layout reuse: first index = column within tile (< TX), second = interface k
-        (&sL[m][0][0])[ci * (TY + 1) + k] = qlR[m];
-        (&sR[m][0][0])[ci * (TY + 1) + k] = qrL[m];
-        (&sAM[m][0][0])[ci * (TY + 1) + k] = am[m];
-        (&sAP[m][0][0])[ci * (TY + 1) + k] = ap[m];
-      }
-    }
-  };
-  if (!FLAGS) {
-    for (int e = tid; e < ny_int; e += 256) y_interface(e, true);
-  } else {
-    if (tid == 0) sNTrouble = 0;
-    __syncthreads();
-    for (int e = tid; e < ny_int; e += 256) {
-      const int kc = e % nyc, k = e / nyc;
-      const int c = yc_lo + kc, r = j0 + k;
-      bool smooth = false;
-      if (c >= 1 && c <= mx && r <= my + 1) {
-        const unsigned char* f = a.flags + (long long)(mx + 1) * my + (c - 1);
-        smooth = (f[(long long)(r - 1) * mx] | f[(long long)max(r - 2, 0) * mx] |
-                  f[(long long)min(r, my) * mx]) == 0;
-      }
-      if (smooth) y_interface(e, false);
-      else sTrouble[atomicAdd(&sNTrouble, 1)] = (short)e;
-    }
-    __syncthreads();
-    const int nt = sNTrouble;
-    for (int i = tid; i < nt; i += 256) y_interface(sTrouble[i], true);
-  }
-  __syncthreads();
-  T dqv[ME];
-  {
-    T ql[ME], qr[ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      ql[m] = (&sL[m][0][0])[tx * (TY + 1) + ty];
-      qr[m] = (&sR[m][0][0])[tx * (TY + 1) + ty + 1];
-    }
-    T W[MW][ME], sp[MW], am2[ME], ap2[ME];
-    typename RS::Roe roe;
-    RS::template rpn<2>(rp, ql, qr, W, sp, am2, ap2, roe);
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      T dqy = T(0) - a.dtdy * ((((&sAM[m][0][0])[tx * (TY + 1) + ty + 1] +
-                                  (&sAP[m][0][0])[tx * (TY + 1) + ty]) + am2[m]) + ap2[m]);
-      dqv[m] = (T(0) + dqx[m]) + dqy;
-    }
-  }
-  // ---- CFL
-  {
-    double cx = (double)a.dtdx * (double)smax_x;
-    double cy = (double)a.dtdy * (double)smax_y;
-    wave_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
-  }
-  // ---- stage combination and store
-  if (cc <= mx && cr <= my) {
-    const long long off = (long long)(cc - 1) + (long long)(cr - 1) * a.g.rs;
-    const long long fs = a.g.fs;
-    T y[ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) y[m] = sQ[m][ty + HALO][tx + HALO];
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      const T D = dqv[m];
-      T o;
-      switch (a.omode) {
-        case OUT_DQDT: o = D / a.dt; break;
-        case OUT_SSP33_2: o = T(0.75) * a.A[off + m * fs] + T(0.25) * (y[m] + D); break;
-        case OUT_SSP33_3:
-          o = T(1. / 3.) * a.A[off + m * fs] + T(2. / 3.) * (y[m] + D);
-          break;
-        case OUT_ADD6: o = y[m] + D / T(6); break;
-        case OUT_SSP104_FIN: o = a.A[off + m * fs] + (T(0.6) * y[m] + T(0.1) * D); break;
-        case OUT_D: o = D; break;
-        case OUT_F32_S2: {
-          T e1 = a.s * a.B[off + m * fs];
-          T inc = e1 + D;
-          o = a.A[off + m * fs] + T(0.25) * inc;
-          a.aux_out[off + m * fs] = inc * T(1. / 6.);
-          break;
-        }
-        default: {   // OUT_F32_S3
-          T delta = a.aux_in[off + m * fs] + T(2. / 3.) * D;
-          T qa = a.A[off + m * fs];
-          T yk = delta - a.comp_in[off + m * fs];
-          T tq = qa + yk;
-          a.comp_out[off + m * fs] = (tq - qa) - yk;
-          o = tq;
-          break;
-        }
-      }
-      a.out[off + m * fs] = o;
-    }
-  }
+// Flags of the interfaces (dilated by one interface along the normal).
+__device__ __forceinline__ bool xflag(const unsigned char* f, int mx, int c, int r) {
+  const unsigned char* fr = f + (long long)(r - 1) * (mx + 1);
+  return (fr[c - 1] | fr[max(c - 2, 0)] | fr[min(c, mx)]) != 0;
+}
+__device__ __forceinline__ bool yflag(const unsigned char* f, int mx, int my, int c, int t) {
+  const unsigned char* fc = f + (long long)(mx + 1) * my + (c - 1);
+  return (fc[(long long)(t - 1) * mx] | fc[(long long)max(t - 2, 0) * mx] |
+          fc[(long long)min(t, my) * mx]) != 0;
 }
 
 // ---------------------------------------------------------------- march form
-// The same stage as k_sharp_stage, without LDS or barriers: a wavefront owns a
-// strip of 64 columns c0-3 .. c0+60 (lanes 3..60 write, the others supply
-// stencil neighbours) and marches down H rows.  Per row, the x-interfaces are
+// The stage kernel: a wavefront owns a strip of 64 columns c0-3 .. c0+60
+// (lanes 3..60 write, the others supply stencil neighbours) and marches down H
+// rows, with no LDS and no barriers.  Per row, the x-interfaces are
 // reconstructed lane-parallel (the 6-cell stencil arrives by cross-lane moves)
 // and each lane's y-interface is reconstructed from a register window of the
 // 6 rows around it; the interface results of the previous row are carried, so
-// every interface is solved once.  Arithmetic per interface and per cell is that
-// of k_sharp_stage (bitwise the same results).
+// every interface is solved once.  An LDS-tiled form (a 32x8 tile with a 3-cell
+// halo staged in LDS, interfaces then cells in barrier-separated passes) was
+// measured at 0.91 ms (FP64) / 0.80 ms (FP32) per 2048^2 characteristic stage
+// against 0.48 / 0.32 ms for this march: its partial second passes and barriers
+// left the SIMDs idle.
 template <typename T>
 struct MarchArgs {
   SharpArgs<T> s;
@@ -582,17 +392,19 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     T st[6][ME];
 #pragma unroll
     for (int k = 0; k < 6; k++) lane_shift<ME>(row, k - 3, st[k]);
-    bool weno = true;
-    if (FLAGS && c >= 1 && c <= mx + 1 && r >= 1 && r <= my) {
-      const unsigned char* f = a.flags + (long long)(r - 1) * (mx + 1);
-      weno = (f[c - 1] | f[max(c - 2, 0)] | f[min(c, mx)]) != 0;
+    // flagged path: every interface takes the linear weights here, and the
+    // cells next to a flagged interface are recomputed by k_sharp_patch
+    bool counted = true;
+    if (FLAGS) {
+      if (c >= 1 && c <= mx + 1 && r >= 1 && r <= my) counted = !xflag(a.flags, mx, c, r);
+      recon_linear<ME>(st, qrL, qlR);
+    } else {
+      recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
     }
-    if (weno) recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
-    else recon_linear<ME>(st, qrL, qlR);
     T W[MW][ME], sp[MW];
     typename RS::Roe roe;
     RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
-    if (xint_lane) {
+    if (xint_lane && counted) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
     }
@@ -601,18 +413,17 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   // = rows t-3..t+2
   auto y_interface = [&](const T (&win)[6][ME], int t, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME],
                          T (&ap)[ME]) {
-    bool weno = true;
-    if (FLAGS && c >= 1 && c <= mx && t <= my + 1) {
-      const unsigned char* f = a.flags + (long long)(mx + 1) * my + (c - 1);
-      weno = (f[(long long)(t - 1) * mx] | f[(long long)max(t - 2, 0) * mx] |
-              f[(long long)min(t, my) * mx]) != 0;
+    bool counted = true;
+    if (FLAGS) {
+      if (c >= 1 && c <= mx && t <= my + 1) counted = !yflag(a.flags, mx, my, c, t);
+      recon_linear<ME>(win, qrL, qlR);
+    } else {
+      recon_interface<RS, 2, T, CHAR>(rp, win, qrL, qlR);
     }
-    if (weno) recon_interface<RS, 2, T, CHAR>(rp, win, qrL, qlR);
-    else recon_linear<ME>(win, qrL, qlR);
     T W[MW][ME], sp[MW];
     typename RS::Roe roe;
     RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
-    if (ycfl_lane && t <= my + 1) {
+    if (ycfl_lane && t <= my + 1 && counted) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
     }
@@ -684,38 +495,8 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     const int r = t - 1;
     if (out_lane) {
       const long long off = (long long)(c - 1) + (long long)(r - 1) * a.g.rs;
-      const long long fs = a.g.fs;
 #pragma unroll
-      for (int m = 0; m < ME; m++) {
-        const T D = (T(0) + dqx[m]) + dqy[m];
-        const T yv = win[2][m];
-        T o;
-        switch (a.omode) {
-          case OUT_DQDT: o = D / a.dt; break;
-          case OUT_SSP33_2: o = T(0.75) * a.A[off + m * fs] + T(0.25) * (yv + D); break;
-          case OUT_SSP33_3: o = T(1. / 3.) * a.A[off + m * fs] + T(2. / 3.) * (yv + D); break;
-          case OUT_ADD6: o = yv + D / T(6); break;
-          case OUT_SSP104_FIN: o = a.A[off + m * fs] + (T(0.6) * yv + T(0.1) * D); break;
-          case OUT_D: o = D; break;
-          case OUT_F32_S2: {
-            T e1 = a.s * a.B[off + m * fs];
-            T inc = e1 + D;
-            o = a.A[off + m * fs] + T(0.25) * inc;
-            a.aux_out[off + m * fs] = inc * T(1. / 6.);
-            break;
-          }
-          default: {   // OUT_F32_S3
-            T delta = a.aux_in[off + m * fs] + T(2. / 3.) * D;
-            T qa = a.A[off + m * fs];
-            T yk = delta - a.comp_in[off + m * fs];
-            T tq = qa + yk;
-            a.comp_out[off + m * fs] = (tq - qa) - yk;
-            o = tq;
-            break;
-          }
-        }
-        a.out[off + m * fs] = o;
-      }
+      for (int m = 0; m < ME; m++) stage_store(a, off, m, (T(0) + dqx[m]) + dqy[m], win[2][m]);
     }
 #pragma unroll
     for (int m = 0; m < ME; m++) { qlY[m] = qlR[m]; apY[m] = ap[m]; }
@@ -723,6 +504,139 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   if (rend == my) {   // x-slice of row my+1 (CFL only) = window row index 3 now
     T d[ME];
     x_cell(win[3], my + 1, d);
+  }
+  double cx = (double)a.dtdx * (double)smax_x;
+  double cy = (double)a.dtdy * (double)smax_y;
+  wave_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
+}
+
+// Flagged path, second pass: every cell next to a flagged interface (listed by
+// clawhip_mlp_troubled_cells) is recomputed with the characteristic (or
+// component) WENO5 on its flagged interfaces and the linear weights on the
+// others -- per interface and per cell the arithmetic of k_sharp_march -- and
+// its stage output is rewritten.  The speeds of the flagged interfaces enter
+// the CFL here.  One thread per listed cell; the list length is read on the
+// device, so the launch needs no host synchronisation.
+template <class RS, typename T, bool CHAR>
+__global__ void __launch_bounds__(256) k_sharp_patch(const SharpArgs<T> a, const int* cells,
+                                                     const int* count) {
+  constexpr int ME = RS::meqn, MW = RS::mwaves;
+  constexpr bool TFLUCT = sizeof(T) == 4;
+  const int n = *count;
+  const int mx = a.g.mx, my = a.g.my;
+  const RPParams<T>& rp = a.rp;
+  T smax_x = T(0), smax_y = T(0);
+  auto load_y = [&](int c, int r, T (&y)[ME]) {
+    load_cell2<ME>(a.P, a.gin, a.bc, a.ng, c, r, y);
+    if (a.ymode != Y_ARRAY) {
+      T qv[ME];
+      load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, qv);
+#pragma unroll
+      for (int m = 0; m < ME; m++)
+        y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
+    }
+  };
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int cell = cells[i];
+    const int c = 1 + cell % mx, r = 1 + cell / mx;
+    T X[7][ME];   // cells c-3..c+3 of row r
+    T Y[7][ME];   // rows r-3..r+3 of column c
+#pragma unroll
+    for (int k = 0; k < 7; k++) load_y(c - 3 + k, r, X[k]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      if (k == 3) {
+#pragma unroll
+        for (int m = 0; m < ME; m++) Y[3][m] = X[3][m];
+      } else {
+        load_y(c, r - 3 + k, Y[k]);
+      }
+    }
+    T dqx[ME], dqy[ME];
+    {
+      // x-interfaces c (stencil X[0..5]) and c+1 (X[1..6])
+      T qrL0[ME], qlR0[ME], am0[ME], ap0[ME], qrL1[ME], qlR1[ME], am1[ME], ap1[ME];
+      for (int side = 0; side < 2; side++) {
+        T st[6][ME];
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+#pragma unroll
+          for (int m = 0; m < ME; m++) st[k][m] = X[k + side][m];
+        const bool weno = xflag(a.flags, mx, c + side, r);
+        T qrL[ME], qlR[ME], W[MW][ME], sp[MW], am[ME], ap[ME];
+        if (weno) recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
+        else recon_linear<ME>(st, qrL, qlR);
+        typename RS::Roe roe;
+        RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
+        if (weno) {
+#pragma unroll
+          for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
+        }
+#pragma unroll
+        for (int m = 0; m < ME; m++) {
+          if (side == 0) { qrL0[m] = qrL[m]; qlR0[m] = qlR[m]; am0[m] = am[m]; ap0[m] = ap[m]; }
+          else { qrL1[m] = qrL[m]; qlR1[m] = qlR[m]; am1[m] = am[m]; ap1[m] = ap[m]; }
+        }
+      }
+      if (TFLUCT) {
+        T fl[ME], fr[ME];
+        RS::template flux<1>(rp, qlR0, fl);
+        RS::template flux<1>(rp, qrL1, fr);
+#pragma unroll
+        for (int m = 0; m < ME; m++)
+          dqx[m] = T(0) - a.dtdx * ((am1[m] + ap0[m]) + (fr[m] - fl[m]));
+      } else {
+        T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+        typename RS::Roe roe;
+        RS::template rpn<1>(rp, qlR0, qrL1, W, sp, am2, ap2, roe);
+#pragma unroll
+        for (int m = 0; m < ME; m++)
+          dqx[m] = T(0) - a.dtdx * (((am1[m] + ap0[m]) + am2[m]) + ap2[m]);
+      }
+    }
+    {
+      T qrL0[ME], qlR0[ME], am0[ME], ap0[ME], qrL1[ME], qlR1[ME], am1[ME], ap1[ME];
+      for (int side = 0; side < 2; side++) {
+        T st[6][ME];
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+#pragma unroll
+          for (int m = 0; m < ME; m++) st[k][m] = Y[k + side][m];
+        const bool weno = yflag(a.flags, mx, my, c, r + side);
+        T qrL[ME], qlR[ME], W[MW][ME], sp[MW], am[ME], ap[ME];
+        if (weno) recon_interface<RS, 2, T, CHAR>(rp, st, qrL, qlR);
+        else recon_linear<ME>(st, qrL, qlR);
+        typename RS::Roe roe;
+        RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
+        if (weno) {
+#pragma unroll
+          for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
+        }
+#pragma unroll
+        for (int m = 0; m < ME; m++) {
+          if (side == 0) { qrL0[m] = qrL[m]; qlR0[m] = qlR[m]; am0[m] = am[m]; ap0[m] = ap[m]; }
+          else { qrL1[m] = qrL[m]; qlR1[m] = qlR[m]; am1[m] = am[m]; ap1[m] = ap[m]; }
+        }
+      }
+      if (TFLUCT) {
+        T fl[ME], fr[ME];
+        RS::template flux<2>(rp, qlR0, fl);
+        RS::template flux<2>(rp, qrL1, fr);
+#pragma unroll
+        for (int m = 0; m < ME; m++)
+          dqy[m] = T(0) - a.dtdy * ((am1[m] + ap0[m]) + (fr[m] - fl[m]));
+      } else {
+        T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+        typename RS::Roe roe;
+        RS::template rpn<2>(rp, qlR0, qrL1, W, sp, am2, ap2, roe);
+#pragma unroll
+        for (int m = 0; m < ME; m++)
+          dqy[m] = T(0) - a.dtdy * (((am1[m] + ap0[m]) + am2[m]) + ap2[m]);
+      }
+    }
+    const long long off = (long long)(c - 1) + (long long)(r - 1) * a.g.rs;
+#pragma unroll
+    for (int m = 0; m < ME; m++) stage_store(a, off, m, (T(0) + dqx[m]) + dqy[m], X[3][m]);
   }
   double cx = (double)a.dtdx * (double)smax_x;
   double cy = (double)a.dtdy * (double)smax_y;
@@ -781,7 +695,8 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
               const int* refl, int ymode, int omode, double dt, double s, double dx, double dy,
               const void* P, const void* Q, const void* A, const void* B, const void* aux_in,
               const void* comp_in, void* out, void* aux_out, void* comp_out,
-              unsigned long long* cfl, const unsigned char* flags, hipStream_t stream) {
+              unsigned long long* cfl, const unsigned char* flags, const int* cells,
+              const int* ncells, hipStream_t stream) {
   SharpArgs<T> a;
   a.rp = to_params<T>(rp);
   a.g = g;
@@ -792,34 +707,28 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
   a.ymode = ymode; a.omode = omode;
   a.dt = T(dt); a.s = T(s);
   a.dtdx = T(dt / dx); a.dtdy = T(dt / dy);
-  a.ntx = (g.mx + TX - 1) / TX; a.nty = (g.my + TY - 1) / TY;
   a.P = (const T*)P; a.Q = (const T*)Q; a.A = (const T*)A; a.B = (const T*)B;
   a.aux_in = (const T*)aux_in; a.comp_in = (const T*)comp_in;
   a.out = (T*)out; a.aux_out = (T*)aux_out; a.comp_out = (T*)comp_out;
   a.cfl_bits = cfl;
   a.flags = flags;
-  const char* tile = getenv("CLAWHIP_SHARP_TILE");
-  if (tile && tile[0] == '1') {
-    if (flags)
-      hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, true>), dim3(a.ntx * a.nty), dim3(256), 0,
-                         stream, a);
-    else
-      hipLaunchKernelGGL((k_sharp_stage<RS, T, CHAR, false>), dim3(a.ntx * a.nty), dim3(256),
-                         0, stream, a);
+  a.cells = cells;
+  a.ncells = ncells;
+  MarchArgs<T> ma;
+  ma.s = a;
+  ma.nstrips = (g.mx + 57) / 58;
+  ma.H = flags ? march_rows(k_sharp_march<RS, T, CHAR, true>, ma.nstrips, g.my)
+               : march_rows(k_sharp_march<RS, T, CHAR, false>, ma.nstrips, g.my);
+  ma.nrb = (g.my + ma.H - 1) / ma.H;
+  const unsigned blocks = (unsigned)(((long long)ma.nstrips * ma.nrb + 3) / 4);
+  if (flags) {
+    hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
+                       ma);
+    hipLaunchKernelGGL((k_sharp_patch<RS, T, CHAR>), dim3(1024), dim3(256), 0, stream, a,
+                       cells, ncells);
   } else {
-    MarchArgs<T> ma;
-    ma.s = a;
-    ma.nstrips = (g.mx + 57) / 58;
-    ma.H = flags ? march_rows(k_sharp_march<RS, T, CHAR, true>, ma.nstrips, g.my)
-                 : march_rows(k_sharp_march<RS, T, CHAR, false>, ma.nstrips, g.my);
-    ma.nrb = (g.my + ma.H - 1) / ma.H;
-    const unsigned blocks = (unsigned)(((long long)ma.nstrips * ma.nrb + 3) / 4);
-    if (flags)
-      hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
-                         ma);
-    else
-      hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, false>), dim3(blocks), dim3(256), 0, stream,
-                         ma);
+    hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, false>), dim3(blocks), dim3(256), 0, stream,
+                       ma);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
@@ -832,14 +741,14 @@ int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const Geom& g
                    double dy, const void* P, const void* Q, const void* A, const void* B,
                    const void* aux_in, const void* comp_in, void* out, void* aux_out,
                    void* comp_out, unsigned long long* cfl, const unsigned char* flags,
-                   hipStream_t stream) {
+                   const int* cells, const int* ncells, hipStream_t stream) {
   if (dtype == CLAWHIP_F64)
     return run_stage<RSF<double>, double, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx,
                                                 dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
-                                                comp_out, cfl, flags, stream);
+                                                comp_out, cfl, flags, cells, ncells, stream);
   return run_stage<RSF<float>, float, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
                                             Q, A, B, aux_in, comp_in, out, aux_out, comp_out,
-                                            cfl, flags, stream);
+                                            cfl, flags, cells, ncells, stream);
 }
 }  // namespace
 
@@ -848,9 +757,10 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
                        double dt, double s, double dx, double dy, const void* P, const void* Q,
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
-                       const unsigned char* flags, hipStream_t stream) {
+                       const unsigned char* flags, const int* cells, const int* ncells,
+                       hipStream_t stream) {
 #define ARGS dtype, rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P, Q, A, B, aux_in, \
-             comp_in, out, aux_out, comp_out, cfl, flags, stream
+             comp_in, out, aux_out, comp_out, cfl, flags, cells, ncells, stream
   if (char_decomp == 2) {
     switch (rp.kind) {
       case CLAWHIP_RP_SHALLOW: return dispatch_stage<Shallow, true>(ARGS);

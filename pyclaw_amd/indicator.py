@@ -55,6 +55,7 @@ class TroubledInterfaceIndicator(object):
         self.threshold = float(meta.get("threshold", 0.0) if threshold is None else threshold)
         self.flat = float(meta.get("flat", 1e-6) if flat is None else flat)
         self._flags = None
+        self._cells = None
         self.every = 1          # steps between flag refreshes (SharpClaw SSP33)
         self._age = None
 
@@ -66,7 +67,13 @@ class TroubledInterfaceIndicator(object):
             self._flags = torch.empty(n, dtype=torch.uint8, device=q.device)
         K.mlp_flags(q, bc_lower, bc_upper, reflect, self.flat, self.params, self.threshold,
                     self._flags)
+        self._cells = K.mlp_troubled_cells(self._flags, mx, my, self._cells)
         return self._flags
+
+    @property
+    def cells(self):
+        """(cells, count) of the last flags() call: the cells the stage recomputes."""
+        return self._cells
 
     def step_flags(self, q, bcs, reflect=(1, 2)):
         """Flags for the next time step: recomputed from q every `every` steps
@@ -209,7 +216,10 @@ def generate_training_data(seeds, mx=512, my=512, nsteps=300, snapshots=(50, 150
     feats, labels = [], []
     gen = torch.Generator(device=device)
     gen.manual_seed(12345)
-    for seed in seeds:
+    seeds = list(seeds)
+    for k, seed in enumerate(seeds):
+        if k % 16 == 0:
+            print(f"training data: run {k} of {len(seeds)}", flush=True)
         solver = swe_solver("fp64")
         sol = swe_solution(seed, mx, my)
         solver.dt = 0.5 / mx / 1.5

@@ -117,10 +117,13 @@ OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F3
 
 def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx, dy, P,
                  Q=None, s=0.0, y_mode=Y_ARRAY, out_mode=OUT_D, A=None, B=None, aux_in=None,
-                 comp_in=None, aux_out=None, comp_out=None, flags=None):
+                 comp_in=None, aux_out=None, comp_out=None, flags=None, cells=None):
     """D = dt*L(Y) (SharpClaw flux2) fused with one RK stage; planar (meqn, my, mx).
     flags: uint8 troubled-interface flags (mlp_flags) selecting full WENO5 per
-    interface; None = full WENO5 everywhere."""
+    interface, with cells = (list, count) from mlp_troubled_cells (built here when
+    None); flags None = full WENO5 everywhere."""
+    if flags is not None and cells is None:
+        cells = mlp_troubled_cells(flags, out.shape[2], out.shape[1])
     meqn, my, mx = out.shape
     g = L.geom2(mx, my)
     code = _dtype_code(out)
@@ -128,7 +131,9 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
         ctypes.byref(rp), code, int(char_decomp), ctypes.byref(g), L.ints(bc_lower),
         L.ints(bc_upper), L.ints(reflect), int(y_mode), int(out_mode), float(dt), float(s),
         float(dx), float(dy), _ptr(P), _ptr(Q), _ptr(A), _ptr(B), _ptr(aux_in), _ptr(comp_in),
-        _ptr(out), _ptr(aux_out), _ptr(comp_out), _ptr(flags), _ptr(cfl), _stream())
+        _ptr(out), _ptr(aux_out), _ptr(comp_out), _ptr(flags),
+        _ptr(cells[0]) if cells is not None else None,
+        _ptr(cells[1]) if cells is not None else None, _ptr(cfl), _stream())
     L.check(rc, "clawhip_sharp2_stage")
 
 
@@ -202,3 +207,15 @@ def mlp_flags(q, bc_lower, bc_upper, reflect, flat, params, threshold, flags):
                                    L.ints(bc_upper), L.ints(reflect), meqn, float(flat),
                                    _ptr(params), float(threshold), _ptr(flags), _stream())
     L.check(rc, "clawhip_mlp_flags")
+
+
+def mlp_troubled_cells(flags, mx, my, out=None):
+    """(cells int32 (mx*my,), count int32 (1,)) of the cells next to a flagged
+    interface (clawhip_mlp_troubled_cells)."""
+    if out is None or out[0].numel() < mx * my:
+        out = (torch.empty(mx * my, dtype=torch.int32, device=flags.device),
+               torch.zeros(1, dtype=torch.int32, device=flags.device))
+    rc = L.lib().clawhip_mlp_troubled_cells(_ptr(flags), int(mx), int(my), _ptr(out[0]),
+                                            _ptr(out[1]), _stream())
+    L.check(rc, "clawhip_mlp_troubled_cells")
+    return out

@@ -110,7 +110,8 @@ class SharpClawSolver(Solver):
         cfl = self._cfl_buffer()
         K.cfl_reset(cfl)
         K.sharp2_stage(self._rp_struct, self.char_decomp, out, cfl, lo, hi, self.reflect_index,
-                       self.dt, dx, dy, P, flags=self._flags, **kw)
+                       self.dt, dx, dy, P, flags=self._flags,
+                       cells=self.indicator.cells if self._flags is not None else None, **kw)
         self.cfl.update_global_max(float(cfl.item()))
 
     # ---- sharpclaw/solver.py:283-430
