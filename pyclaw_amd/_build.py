@@ -5,6 +5,12 @@ Flags:
                       match the reference's Fortran (no FMA contraction)
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       keep the compiler from emitting packed FP32 arithmetic
+  -mllvm -amdgpu-mfma-vgpr-form
+                      MFMA results in VGPRs (no accumulator-register copies)
+  -fno-hip-fp32-correctly-rounded-divide-sqrt
+                      FP32 division and square root by the hardware reciprocal
+                      and square-root instructions (the FP32 paths are checked
+                      against tolerances; FP64 division and sqrt stay IEEE)
 """
 import glob
 import os
@@ -19,6 +25,7 @@ ARCH = os.environ.get("CLAWHIP_ARCH", "gfx950")
 
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
          f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function"]
 
 

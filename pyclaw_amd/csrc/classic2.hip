@@ -66,7 +66,9 @@ int choose_rows(Kernel kernel, int nstrips, int my) {
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess ||
         ncu < 1 || nb < 1)
       return 16;
-    resident = (long long)ncu * nb * 4;
+    // beyond two wavefronts per SIMD the measured gain of a third is smaller
+    // than the halo rows the shorter marches add, so count at most two
+    resident = (long long)ncu * (nb < 2 ? nb : 2) * 4;
   }
   if (const char* env = getenv("CLAWHIP_UNSPLIT_ROWS")) {
     int h = atoi(env);

@@ -92,11 +92,28 @@ __device__ __forceinline__ T* cell_ptr3(T* base, const Geom& g, int c, int r, in
          (long long)(k - 1 + g.g) * g.ss;
 }
 
-// ---- cross-lane moves (ds_bpermute): value of lane-1 / lane+1 --------------
+// ---- cross-lane moves: value of lane-1 / lane+1 ------------------------------
+// DPP whole-wavefront shifts (wave_shr:1 = 0x138: lane l reads lane l-1;
+// wave_shl:1 = 0x130: lane l reads lane l+1), one VALU move per dword; the lane
+// at the end (0, resp. 63) keeps its own value.
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_wave_shift(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 4) {
+    const int x = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xf, 0xf, false));
+  } else {
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = (int)x, hi = (int)(x >> 32);
+    const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, ((long long)h2 << 32) | (long long)(unsigned)l2);
+  }
+}
 template <typename T>
-__device__ __forceinline__ T from_left(T v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ T from_left(T v) { return dpp_wave_shift<0x138>(v); }
 template <typename T>
-__device__ __forceinline__ T from_right(T v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ T from_right(T v) { return dpp_wave_shift<0x130>(v); }
 
 template <int N, typename T>
 __device__ __forceinline__ void from_left(const T (&v)[N], T (&o)[N]) {

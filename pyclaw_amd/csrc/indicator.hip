@@ -16,6 +16,7 @@
 // MFMA 16x16x16 bf16 operand layout (wave64): lane l holds A[l%16][4*(l/16)+0..3],
 // B[4*(l/16)+0..3][l%16]; the 4 FP32 results are D[4*(l/16)+0..3][l%16].
 #include <hip/hip_bf16.h>
+#include <stdlib.h>
 
 #include "launch.hpp"
 
@@ -32,12 +33,8 @@ constexpr int NH = 32;      // hidden width
 constexpr int P_W1 = 0, P_W2 = P_W1 + NF * NH, P_B2 = P_W2 + NH * NH, P_W3 = P_B2 + NH,
               P_B3 = P_W3 + NH, P_N = P_B3 + 1;
 
-__device__ __forceinline__ short f2bf(float f) {
-  // round to nearest even (finite inputs)
-  unsigned u = __float_as_uint(f);
-  unsigned r = u + 0x7fffu + ((u >> 16) & 1u);
-  return (short)(r >> 16);
-}
+// float -> BF16 bits, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
 
 // ---------------------------------------------------------------- features
@@ -231,73 +228,109 @@ __device__ __forceinline__ void mlp_forward(const short* P, const short* sX, sho
   }
 }
 
-// Inference form, computed transposed: Z1^T = W1^T X^T, Z2^T = W2^T H1^T + b2,
-// z = w3 . relu(Z2^T) + b3.  With the samples as the MFMA's N index, a layer's
-// D fragment (rows 4*(lane/16)+e = hidden units, column lane%16 = sample) is
-// exactly the next layer's B fragment, so the activations never leave the
-// registers; the weight fragments (the A operands) are loaded once per kernel.
+// Inference form on 32x32 MFMA tiles (32 samples per product), computed
+// transposed: Z1^T = W1^T X^T (K = 8: six features, the bias input 1 and a zero;
+// v_mfma_f32_32x32x8bf16_1k), Z2^T = W2^T relu(Z1^T) + b2 (K = 32 as two
+// v_mfma_f32_32x32x16_bf16), z = w3 . relu(Z2^T) + b3.  Result item i of lane l
+// of a 32x32 product is row d32_row(l/32, i), column l%32.  Those rows are the
+// hidden units the lane holds; they go unchanged into the lane's B items of the
+// next product, and the weights (the A operand) are loaded in the same unit
+// order, so the hardware's K order never matters: A item t of lane (row + 32h)
+// and B item t of lane (col + 32h) always meet.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int d32_row(int h, int i) { return 8 * (i >> 2) + 4 * h + (i & 3); }
+
+// two floats -> packed BF16 pair (round to nearest even; a in the low half)
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+  const f2 v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, b2));
+}
+// max(x, 0) of both halves of a BF16 pair: BF16 bit patterns order like int16
+// for the sign test, so a packed int16 max against 0 is the ReLU
+__device__ __forceinline__ unsigned relu_bf16x2(unsigned u) {
+  typedef short s2 __attribute__((ext_vector_type(2)));
+  const s2 z = {0, 0};
+  return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(s2, u), z));
+}
+__device__ __forceinline__ float bf16_round(float f) { return bf2f(f2bf(f)); }
+
 struct MlpRegs {
-  bf16x4 w1[2];       // W1^T rows nb*16+col, k = kq..kq+3  (= W1[kq+i][nb*16+col])
-  bf16x4 w2[2][2];    // W2^T rows nb*16+col, k = kb*16+kq+i (= W2[kb*16+kq+i][nb*16+col])
-  float b2[2][4], w3[2][4], b3;   // entries of hidden units nb*16+kq+e
+  bf16x4 w1;       // W1[4*(l/32)+t][l%32], t = 0..3 (features 7.. are zero)
+  bf16x8 w2[2];    // product b: W2[d32_row(l/32, 8b+t)][l%32], t = 0..7
+  f32x16 b2;       // b2[d32_row(l/32, i)]
+  float w3[16];    // w3[d32_row(l/32, i)]
+  float b3;
 };
 
 __device__ __forceinline__ void mlp_load_regs(const float* master, MlpRegs& r) {
   const int lane = threadIdx.x & 63;
-  const int col = lane & 15, kq = 4 * (lane >> 4);
+  const int col = lane & 31, h = lane >> 5;
 #pragma unroll
-  for (int nb = 0; nb < 2; nb++) {
-    bf16x4 b;
-    b.x = f2bf(master[P_W1 + (kq + 0) * NH + nb * 16 + col]);
-    b.y = f2bf(master[P_W1 + (kq + 1) * NH + nb * 16 + col]);
-    b.z = f2bf(master[P_W1 + (kq + 2) * NH + nb * 16 + col]);
-    b.w = f2bf(master[P_W1 + (kq + 3) * NH + nb * 16 + col]);
-    r.w1[nb] = b;
-#pragma unroll
-    for (int kb = 0; kb < 2; kb++) {
-      const int k0 = kb * 16 + kq;
-      bf16x4 c;
-      c.x = f2bf(master[P_W2 + (k0 + 0) * NH + nb * 16 + col]);
-      c.y = f2bf(master[P_W2 + (k0 + 1) * NH + nb * 16 + col]);
-      c.z = f2bf(master[P_W2 + (k0 + 2) * NH + nb * 16 + col]);
-      c.w = f2bf(master[P_W2 + (k0 + 3) * NH + nb * 16 + col]);
-      r.w2[kb][nb] = c;
-    }
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      r.b2[nb][e] = bf2f(f2bf(master[P_B2 + nb * 16 + kq + e]));
-      r.w3[nb][e] = bf2f(f2bf(master[P_W3 + nb * 16 + kq + e]));
-    }
+  for (int t = 0; t < 4; t++) {
+    const int k = 4 * h + t;
+    r.w1[t] = k < 7 ? f2bf(master[P_W1 + k * NH + col]) : (short)0;
   }
-  r.b3 = bf2f(f2bf(master[P_B3]));
+#pragma unroll
+  for (int b = 0; b < 2; b++)
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+      r.w2[b][t] = (__bf16)master[P_W2 + d32_row(h, 8 * b + t) * NH + col];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    r.b2[i] = bf16_round(master[P_B2 + d32_row(h, i)]);
+    r.w3[i] = bf16_round(master[P_W3 + d32_row(h, i)]);
+  }
+  r.b3 = bf16_round(master[P_B3]);
 }
 
-// 16 samples; xT = this lane's B fragment of X^T (features kq..kq+3 of sample
-// lane%16).  Returns the logit of sample lane%16 (in every lane of that column).
-__device__ __forceinline__ float mlp_forward_regs(const MlpRegs& r, bf16x4 xT) {
-  bf16x4 h[2];
+// 64 samples, one per lane.  lo / hi: this lane's own sample as BF16 features
+// 0-3 and 4-7 (4, 5, the bias input 1, 0).  Tile 0 is samples 0..31, tile 1
+// samples 32..63; returns the logit of the lane's own sample.
+__device__ __forceinline__ float mlp_forward_regs(const MlpRegs& r, u32x2 lo, u32x2 hi) {
+  const int lane = threadIdx.x & 63;
+  const bool low = lane < 32;
+  // B of tile 0: lanes 0-31 features 0-3 of sample l, lanes 32-63 features 4-7
+  // of sample l-32; tile 1: lanes 0-31 features 0-3 of sample l+32, lanes 32-63
+  // features 4-7 of sample l.  One exchange with lane l^32 supplies the rest.
+  const u32x2 send = low ? hi : lo;
+  u32x2 recv;
+  recv.x = (unsigned)__shfl_xor((int)send.x, 32, 64);
+  recv.y = (unsigned)__shfl_xor((int)send.y, 32, 64);
+  const u32x2 x[2] = {low ? lo : recv, low ? recv : hi};
+  float part[2];
 #pragma unroll
-  for (int nb = 0; nb < 2; nb++) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[nb], xT, acc, 0, 0, 0);
-    h[nb].x = f2bf(fmaxf(acc[0], 0.f));
-    h[nb].y = f2bf(fmaxf(acc[1], 0.f));
-    h[nb].z = f2bf(fmaxf(acc[2], 0.f));
-    h[nb].w = f2bf(fmaxf(acc[3], 0.f));
+  for (int tile = 0; tile < 2; tile++) {
+    const f32x16 zero = {};
+    const f32x16 z1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+        r.w1, __builtin_bit_cast(bf16x4, x[tile]), zero, 0, 0, 0);
+    u32x4 hb[2];
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        hb[b][j] = relu_bf16x2(pack_bf16(z1[8 * b + 2 * j], z1[8 * b + 2 * j + 1]));
+    f32x16 z2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        r.w2[0], __builtin_bit_cast(bf16x8, hb[0]), r.b2, 0, 0, 0);
+    z2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        r.w2[1], __builtin_bit_cast(bf16x8, hb[1]), z2, 0, 0, 0);
+    // ReLU as a signed-integer max of the bit pattern (negative floats have the
+    // sign bit set), then an explicit FMA: the products and sums of the FP32 dot
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      s = fmaf(__int_as_float(max(__float_as_int(z2[i]), 0)), r.w3[i], s);
+    part[tile] = s;
   }
-  float v = 0.f;
-#pragma unroll
-  for (int nb = 0; nb < 2; nb++) {
-    f32x4 acc = {r.b2[nb][0], r.b2[nb][1], r.b2[nb][2], r.b2[nb][3]};
-#pragma unroll
-    for (int kb = 0; kb < 2; kb++)
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[kb][nb], h[kb], acc, 0, 0, 0);
-#pragma unroll
-    for (int e = 0; e < 4; e++) v += fmaxf(acc[e], 0.f) * r.w3[nb][e];
-  }
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
-  return v + r.b3;
+  // lane l's own sample is column l%32 of tile l/32; its other half-sum sits in
+  // lane l^32
+  const float other = __shfl_xor(low ? part[1] : part[0], 32, 64);
+  return (low ? part[0] : part[1]) + other + r.b3;
 }
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.f / (1.f + __expf(-z)); }
@@ -506,79 +539,94 @@ __global__ void __launch_bounds__(256) k_mlp_logits(const float* feat, long long
 // Interface flags of a state: flag = 1 when any component whose stencil range
 // exceeds `flat` has a logit above `zthr` (troubled -> full WENO).  Flags:
 // x-interfaces (mx+1)*my, then y-interfaces mx*(my+1), row-major as in
-// gather_stencil.  A wavefront takes 64/meqn interfaces per pass: every lane
-// gathers one (interface, component) sample, the 64 rows go through the MLP
-// as four 16-row MFMA tiles, and the first lanes OR their interface's components.
-// A pass covers 64/meqn consecutive interfaces of one row (x-interfaces c =
-// 1..mx+1 of rows 1..my, then y-interfaces c = 1..mx of rows 1..my+1), so the
-// row and column block are wavefront-uniform and the stencil loads coalesce.
+// gather_stencil.  A wavefront takes ni = 64/meqn interfaces of one row per
+// pass (x-interfaces c = 1..mx+1 of rows 1..my, then y-interfaces c = 1..mx of
+// rows 1..my+1): lane l gathers component l%meqn of interface l/meqn (the
+// spare lanes repeat the last interface), the 64 samples go through the MLP as
+// two 32-sample tiles, and a ballot of the hits gives each interface its
+// components.  A wavefront loads its next pass's stencils before it evaluates
+// the current one; the pass position is wavefront-uniform, so the row, the
+// column block and whether the stencils need the boundary map are scalar.
+struct FlagPass {
+  int dir, r, cb;
+};
+
+template <typename T>
+__device__ __forceinline__ void flag_stencils(const T* q, const Geom& g, const BCSpec& bc,
+                                              const FlagPass& p, int ni, int li, int m,
+                                              T (&v)[6]) {
+  const int cmax = p.dir == 0 ? g.mx + 1 : g.mx;
+  const int c = min(1 + p.cb * ni + li, cmax);
+  const bool interior = p.dir == 0 ? (p.cb * ni >= 3 && p.cb * ni + ni + 2 <= g.mx)
+                                   : (p.r >= 4 && p.r + 2 <= g.my);
+  if (interior) {
+    const T* p0 = q + (long long)m * g.fs + (long long)(p.r - 1) * g.rs + (c - 1);
+    const long long step = p.dir == 0 ? 1 : g.rs;
+#pragma unroll
+    for (int s = 0; s < 6; s++) v[s] = p0[(s - 3) * step];
+  } else {
+    gather_stencil_at(q, g, bc, p.dir, c, p.r, m, v);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc, int meqn,
                                                    float flat, const float* master, float zthr,
                                                    unsigned char* flags) {
-  __shared__ __attribute__((aligned(16))) short sX[4][64 * 8];
-  __shared__ unsigned char sHit[4][64];
-  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int lane = threadIdx.x & 63;
+  const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int stride = gridDim.x * 4;
+  const int mx = g.mx, my = g.my;
+  const int ni = 64 / meqn;
+  const int npx = (mx + 1 + ni - 1) / ni, npy = (mx + ni - 1) / ni;
+  const int nxp = npx * my;
+  const int npass = nxp + npy * (my + 1);
+  if (wid >= npass) return;
   MlpRegs reg;
   mlp_load_regs(master, reg);
-  const int mx = g.mx, my = g.my;
-  const int ni = 64 / meqn;                 // interfaces per pass
-  const int npx = (mx + 1 + ni - 1) / ni, npy = (mx + ni - 1) / ni;
-  const long long nxp = (long long)npx * my;
-  const long long npass = nxp + (long long)npy * (my + 1);
-  const int col = lane & 15, grp = lane >> 4;
-  const int li = lane / meqn, m = lane % meqn;
-  for (long long pass = (long long)blockIdx.x * 4 + wv; pass < npass;
-       pass += (long long)gridDim.x * 4) {
-    int dir, r, cb;
+  const int li = min(lane / meqn, ni - 1), m = lane % meqn;
+  const unsigned long long cmask = meqn >= 64 ? ~0ull : (1ull << meqn) - 1ull;
+  auto locate = [&](int pass) {
+    FlagPass p;
     if (pass < nxp) {
-      dir = 0; r = 1 + (int)(pass / npx); cb = (int)(pass % npx);
+      p.dir = 0; p.r = 1 + pass / npx; p.cb = pass % npx;
     } else {
-      const long long p2 = pass - nxp;
-      dir = 1; r = 1 + (int)(p2 / npy); cb = (int)(p2 % npy);
+      p.dir = 1; p.r = 1 + (pass - nxp) / npy; p.cb = (pass - nxp) % npy;
     }
-    const int c = 1 + cb * ni + li;
-    const int cmax = dir == 0 ? mx + 1 : mx;
-    const bool live = lane < ni * meqn && c <= cmax;
-    float f[6] = {0, 0, 0, 0, 0, 0};
-    unsigned char rough = 0;
-    if (live) {
-      T v[6];
-      gather_stencil_at(q, g, bc, dir, c, r, m, v);
-      rough = norm_stencil(v, flat, f) > flat ? 1 : 0;
+    return p;
+  };
+  FlagPass cur = locate(wid);
+  T v[6];
+  flag_stencils(q, g, bc, cur, ni, li, m, v);
+  // passes are interleaved across the grid (wid, wid + stride, ...), which keeps
+  // the waves balanced where whole smooth passes skip the network
+  for (int pass = wid; pass < npass; pass += stride) {
+    const bool more = pass + stride < npass;
+    const FlagPass nxt = locate(more ? pass + stride : pass);
+    T vn[6];
+    if (more) flag_stencils(q, g, bc, nxt, ni, li, m, vn);
+    float f[6];
+    const bool rough = norm_stencil(v, flat, f) > flat;
+    unsigned long long hits = 0;
+    if (__ballot(rough) != 0ull) {
+      const u32x2 lo = {pack_bf16(f[0], f[1]), pack_bf16(f[2], f[3])};
+      const u32x2 hi = {pack_bf16(f[4], f[5]), 0x3f80u};   // bias input 1.0, then 0
+      const float z = mlp_forward_regs(reg, lo, hi);
+      hits = __ballot(rough && z > zthr);
     }
-    // this lane's sample row: features 0..5, bias input 1, pad
-    bf16x4 lo, hi;
-    lo.x = f2bf(f[0]); lo.y = f2bf(f[1]); lo.z = f2bf(f[2]); lo.w = f2bf(f[3]);
-    hi.x = f2bf(f[4]); hi.y = f2bf(f[5]); hi.z = f2bf(1.f); hi.w = 0;
-    bf16x4* x = reinterpret_cast<bf16x4*>(sX[wv] + lane * 8);
-    x[0] = lo; x[1] = hi;
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    // tile t = samples 16t..16t+15: lane group 0 holds features 0-3 of sample
-    // col, group 1 features 4-7, groups 2-3 the zero padding (features 8-15);
-    // lane l's own sample (l) is in tile l/16
-    unsigned char hit = 0;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      bf16x4 xT = {0, 0, 0, 0};
-      if (grp < 2) xT = *reinterpret_cast<const bf16x4*>(sX[wv] + (t * 16 + col) * 8 + grp * 4);
-      const float z = mlp_forward_regs(reg, xT);
-      if (t == grp) hit = z > zthr ? 1 : 0;
-    }
-    sHit[wv][lane] = hit & rough;
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
-    const int ci = 1 + cb * ni + lane;
+    const int cmax = cur.dir == 0 ? mx + 1 : mx;
+    const int ci = 1 + cur.cb * ni + lane;
     if (lane < ni && ci <= cmax) {
-      unsigned char any = 0;
-      for (int k = 0; k < meqn; k++) any |= sHit[wv][lane * meqn + k];
-      const long long fi = dir == 0 ? (long long)(r - 1) * (mx + 1) + (ci - 1)
-                                    : (long long)(mx + 1) * my + (long long)(r - 1) * mx + (ci - 1);
-      flags[fi] = any;
+      const long long fi = cur.dir == 0
+                               ? (long long)(cur.r - 1) * (mx + 1) + (ci - 1)
+                               : (long long)(mx + 1) * my + (long long)(cur.r - 1) * mx + (ci - 1);
+      flags[fi] = ((hits >> (lane * meqn)) & cmask) != 0ull ? 1 : 0;
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_wave_barrier();
+    if (more) {
+      cur = nxt;
+#pragma unroll
+      for (int s = 0; s < 6; s++) v[s] = vn[s];
+    }
   }
 }
 
@@ -681,8 +729,15 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
   const int ni = 64 / meqn;
   long long passes = (long long)((g.mx + 1 + ni - 1) / ni) * g.my +
                      (long long)((g.mx + ni - 1) / ni) * (g.my + 1);
+  if (passes + 4LL * 65536 >= (1LL << 31)) return CLAWHIP_E_ARG;   // 32-bit pass index
+  // grid-stride: a few resident rounds of wavefronts, each loading the weights once
+  long long cap = 2048;
+  if (const char* env = getenv("CLAWHIP_FLAGS_BLOCKS")) {
+    const long long b = atoll(env);
+    if (b >= 1 && b <= 65536) cap = b;
+  }
   long long blocks = (passes + 3) / 4;
-  if (blocks > 8192) blocks = 8192;
+  if (blocks > cap) blocks = cap;
   if (dtype == CLAWHIP_F64)
     hipLaunchKernelGGL(k_mlp_flags<double>, dim3((unsigned)blocks), dim3(256), 0, s,
                        (const double*)q, g, bc, meqn, flat, master, zthr, flags);

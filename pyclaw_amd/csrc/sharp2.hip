@@ -344,12 +344,6 @@ struct MarchArgs {
   int H, nstrips, nrb;
 };
 
-template <int ME, typename T>
-__device__ __forceinline__ void lane_shift(const T (&v)[ME], int d, T (&o)[ME]) {
-#pragma unroll
-  for (int m = 0; m < ME; m++) o[m] = __shfl(v[m], (int)(threadIdx.x & 63) + d, 64);
-}
-
 // FP32 (fp32k) path: the fluctuation inside a cell, amdq + apdq of the Riemann
 // problem between its two reconstructed edges (flux1.f90:174-186), is the flux
 // difference f(qr) - f(ql), which the Roe solvers reproduce exactly in exact
@@ -389,9 +383,15 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   // x-slice of row r: interface c (left of this lane's cell) -> qlR, qrL, am, ap
   auto x_interface = [&](const T (&row)[ME], int r, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME],
                          T (&ap)[ME]) {
+    // st[k] = the row value of lane l + k - 3 (chained one-lane moves)
     T st[6][ME];
 #pragma unroll
-    for (int k = 0; k < 6; k++) lane_shift<ME>(row, k - 3, st[k]);
+    for (int m = 0; m < ME; m++) st[3][m] = row[m];
+    from_left<ME>(st[3], st[2]);
+    from_left<ME>(st[2], st[1]);
+    from_left<ME>(st[1], st[0]);
+    from_right<ME>(st[3], st[4]);
+    from_right<ME>(st[4], st[5]);
     // flagged path: every interface takes the linear weights here, and the
     // cells next to a flagged interface are recomputed by k_sharp_patch
     bool counted = true;
@@ -433,8 +433,8 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     T qrL[ME], qlR[ME], am[ME], ap[ME];
     x_interface(row, r, qrL, qlR, am, ap);
     T qr[ME], amn[ME];
-    lane_shift<ME>(qrL, 1, qr);     // right edge of this cell: interface c+1
-    lane_shift<ME>(am, 1, amn);     // amdq at interface c+1
+    from_right<ME>(qrL, qr);     // right edge of this cell: interface c+1
+    from_right<ME>(am, amn);     // amdq at interface c+1
     if (TFLUCT) {
       T fl[ME], fr[ME];
       RS::template flux<1>(rp, qlR, fl);

@@ -101,18 +101,32 @@ class SharpClawSolver(Solver):
             self._bufs[name] = t
         return t
 
-    def _stage(self, state_q, out, P, **kw):
-        """One operator evaluation (dq) fused with a stage; updates the CFL."""
+    def _stage(self, state_q, out, P, cfl=None, **kw):
+        """One operator evaluation (dq) fused with a stage.  cfl=None: the stage's
+        Courant number goes into self.cfl at once; otherwise it is max-reduced
+        into the given device buffer, which the caller reads later."""
         lo, hi = self._bc_codes()
         if BC.custom in lo or BC.custom in hi:
             raise NotImplementedError("custom boundary conditions are not supported by SharpClaw on the device")
         dx, dy = self._grid.delta
-        cfl = self._cfl_buffer()
-        K.cfl_reset(cfl)
+        now = cfl is None
+        if now:
+            cfl = self._cfl_buffer()
+            K.cfl_reset(cfl)
         K.sharp2_stage(self._rp_struct, self.char_decomp, out, cfl, lo, hi, self.reflect_index,
                        self.dt, dx, dy, P, flags=self._flags,
                        cells=self.indicator.cells if self._flags is not None else None, **kw)
-        self.cfl.update_global_max(float(cfl.item()))
+        if now:
+            self.cfl.update_global_max(float(cfl.item()))
+
+    def _stage_cfl(self):
+        """Device buffer for the Courant number of a step's later stages."""
+        t = self._bufs.get("cfl_stages")
+        if t is None:
+            t = torch.zeros(1, dtype=torch.float64, device="cuda")
+            self._bufs["cfl_stages"] = t
+        K.cfl_reset(t)
+        return t
 
     # ---- sharpclaw/solver.py:283-430
     def step(self, solution, take_one_step, tstart, tend):
@@ -135,31 +149,36 @@ class SharpClawSolver(Solver):
         self.dt_old = self.dt
         dt = self.dt
         out_q = self._take_scratch((q, comp))
+        # the later stages max-reduce their Courant numbers on the device; one
+        # read at the end of the step (max is order-free, so self.cfl ends as the
+        # stage-by-stage updates of sharpclaw/solver.py:283-430 leave it)
+        cs = self._stage_cfl()
         if self.time_integrator == "SSP33":
             r2 = self._buf("r2", q)
             if self.precision == "fp64":
-                self._stage(q, r2, q, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
+                self._stage(q, r2, q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
                             out_mode=K.OUT_SSP33_2, A=q)
-                self._stage(q, out_q[0], r2, out_mode=K.OUT_SSP33_3, A=q)
+                self._stage(q, out_q[0], r2, cfl=cs, out_mode=K.OUT_SSP33_3, A=q)
             else:
                 aux = self._buf("aux", q)
-                self._stage(q, r2, q, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
+                self._stage(q, r2, q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
                             out_mode=K.OUT_F32_S2, A=q, B=self.dq_dt, aux_out=aux)
-                self._stage(q, out_q[0], r2, out_mode=K.OUT_F32_S3, A=q, aux_in=aux,
+                self._stage(q, out_q[0], r2, cfl=cs, out_mode=K.OUT_F32_S3, A=q, aux_in=aux,
                             comp_in=comp, comp_out=out_q[1])
         else:   # SSP104
             s1a, s1b, qm = self._buf("s1a", q), self._buf("s1b", q), self._buf("qm", q)
-            self._stage(q, s1a, q, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY6, out_mode=K.OUT_ADD6)
+            self._stage(q, s1a, q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY6, out_mode=K.OUT_ADD6)
             cur, nxt = s1a, s1b
             for _ in range(3):
-                self._stage(q, nxt, cur, out_mode=K.OUT_ADD6)
+                self._stage(q, nxt, cur, cfl=cs, out_mode=K.OUT_ADD6)
                 cur, nxt = nxt, cur
             qm.copy_(q)
             K.ssp104_mid(qm, cur)
             for _ in range(4):
-                self._stage(q, nxt, cur, out_mode=K.OUT_ADD6)
+                self._stage(q, nxt, cur, cfl=cs, out_mode=K.OUT_ADD6)
                 cur, nxt = nxt, cur
-            self._stage(q, out_q[0], cur, out_mode=K.OUT_SSP104_FIN, A=qm)
+            self._stage(q, out_q[0], cur, cfl=cs, out_mode=K.OUT_SSP104_FIN, A=qm)
+        self.cfl.update_global_max(float(cs.item()))
         state.set_device_q(*out_q)
         self._scratch = (q, comp)
         self._flags = None
