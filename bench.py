@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 MX = MY = 1024
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 BYTES_PER_CELL_F64 = 64     # read q + write q, 4 fields x 8 B
 BYTES_PER_CELL_F32K = 64    # q + compensation, read + write, 4 fields x 4 B x 2
 
@@ -126,16 +127,18 @@ def run_gpu(precision, steps, warmup, use_graph=True):
     return wall, kms, nk, host, solution
 
 
-def measured_traffic():
-    """HBM bytes per k_unsplit_ctl<Euler<double>> launch on this workload, from
-    the rocprofv3 FETCH_SIZE and WRITE_SIZE runs summarised in profiles/ (each
-    scaled by the same counters' reading for a 256 MiB copy); None if absent."""
+def measured_counts():
+    """Per-launch counter figures of k_unsplit_ctl<Euler<double>> on this
+    workload, from the rocprofv3 runs summarised in profiles/: HBM bytes
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's reading for a
+    256 MiB copy) and executed FP64 operations (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64
+    x 64 lanes, FMA counted twice).  {} if absent."""
     path = os.path.join(ROOT, "profiles", "k_unsplit_ctl_euler2d_1024.traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)["bytes_per_launch"]
-    except (OSError, ValueError, KeyError):
-        return None
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
 
 
 def cpu_baseline(sample_steps=3):
@@ -258,8 +261,10 @@ def euler3d_leg(steps=10, n=384, seed=5):
 def sharpclaw_leg(steps=20, n=2048, seed=3):
     """2D shallow water on the periodic unit square, 2048^2: h = 1 + 8 seeded
     Gaussian humps + 2 seeded circular dam breaks, hu = hv = 0; SharpClaw
-    characteristic WENO5 + SSP33, CFL 0.5.  FP64 full WENO5 against FP32 + Kahan
-    with the learned troubled-interface indicator (shipped weights)."""
+    characteristic WENO5 + SSP33, CFL 0.45 (max 0.5).  FP64 full WENO5 against
+    FP32 + Kahan, with and without the learned troubled-interface indicator
+    (shipped weights; flags refreshed every second step).  Each leg: one
+    untimed step, then `steps` timed evolve_to_time steps from the host."""
     import torch
     from pyclaw_amd import indicator as I
     res = {}
@@ -269,27 +274,27 @@ def sharpclaw_leg(steps=20, n=2048, seed=3):
         indicator = I.TroubledInterfaceIndicator() if ind else None
         solver = I.swe_solver(prec, indicator)
         sol = I.swe_solution(seed, n, n)
-        solver.dt = 0.5 / n / 1.6
+        solver.dt = 0.45 / n / 1.6
         solver.setup(sol)
         solver.evolve_to_time(sol)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        frac = []
-        for k in range(steps):
+        for _ in range(steps):
             solver.evolve_to_time(sol)
-            if indicator is not None and k % 5 == 0:
-                frac.append(indicator.troubled_fraction())
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         res[name] = {"value": n * n * steps / wall, "ms_per_step": 1e3 * wall / steps,
                      "t_end": sol.t}
-        if frac:
-            res[name]["troubled_fraction"] = float(np.mean(frac))
+        if indicator is not None:
+            res[name]["troubled_fraction"] = indicator.troubled_fraction()
+            res[name]["flag_refresh_every"] = indicator.refresh_interval(solver.cfl_max)
         qs[name] = sol.state.q
     ref = qs["fp64_weno"]
     for name in ("fp32k_weno", "fp32k_mlp"):
         res[name]["rel_l1_vs_fp64"] = float(np.abs(qs[name] - ref).sum() / np.abs(ref).sum())
     res["speedup_fp32k_mlp_vs_fp64_weno"] = res["fp32k_mlp"]["value"] / res["fp64_weno"]["value"]
+    res["speedup_fp32k_weno_vs_fp64_weno"] = (res["fp32k_weno"]["value"] /
+                                              res["fp64_weno"]["value"])
     res.update({"workload": "swe2d_periodic_humps_dambreaks_sharpclaw_weno5char_ssp33_2048",
                 "unit": "cell-updates/s", "steps": steps})
     return res
@@ -322,6 +327,8 @@ def main():
     err_max = float(np.abs(q32 - q64).max() / np.abs(q64).max())
     value = cells * args.steps / wall64
     achieved = BYTES_PER_CELL_F64 * cells / (k64 * 1e-3) / 1e9
+    counts = measured_counts()
+    flops = counts.get("fp64_flops_per_launch")
     rec = {
         "metric": "cell-updates/s, 2D Euler/SWE 2048\u00b2: FP64 plain vs FP32+learned-WENO; "
                   "% HBM peak",
@@ -341,9 +348,16 @@ def main():
                    "seed": 2, "t_end": sol64.t, "accepted": host64.accepted,
                    "rejected": host64.nrejected},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": counts.get("bytes_per_launch"),
                      "kernel": "k_unsplit_ctl<Euler<double>>", "kernel_ms": k64,
                      "kernels_timed": n64, "bytes_per_cell": BYTES_PER_CELL_F64},
+        # the same kernel against the FP64 vector peak: executed FP64 operations
+        # per launch (counter run) over the live kernel time
+        "fp64_valu": (None if flops is None else
+                      {"achieved": flops / (k64 * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s",
+                       "frac": flops / (k64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                       "flops_per_launch": flops}),
         "fp32_kahan": {"value": cells * args.steps / wall32,
                        "ms_per_step": 1e3 * wall32 / args.steps, "kernel_ms": k32,
                        "hbm_frac": BYTES_PER_CELL_F32K * cells / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS,

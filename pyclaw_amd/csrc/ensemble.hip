@@ -105,7 +105,7 @@ int ens_resident_waves() {
             &nb, k_ens_step<Euler<float>, float, true>, 256, 0) != hipSuccess ||
         ncu < 1 || nb < 1)
       return 1024;
-    resident = ncu * nb * 4;
+    resident = ncu * (nb < 2 ? nb : 2) * 4;   // as the single-grid row chooser
   }
   return resident;
 }

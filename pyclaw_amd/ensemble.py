@@ -132,7 +132,7 @@ class Ensemble(object):
                 self._launch(n)
             done += n
 
-    def autotune(self, candidates=(4, 6, 8, 10, 12, 16, 24), steps=4, repeat=2):
+    def autotune(self, candidates=(4, 6, 8, 10, 12, 16, 24, 32, 48, 64), steps=4, repeat=2):
         """Times `steps` ensemble steps per candidate slices-per-wavefront shape
         (on copies of the members' controllers, so the ensemble's state does not
         advance) and keeps the fastest.  Returns {rows: seconds per step}."""

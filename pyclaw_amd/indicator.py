@@ -56,7 +56,7 @@ class TroubledInterfaceIndicator(object):
         self.flat = float(meta.get("flat", 1e-6) if flat is None else flat)
         self._flags = None
         self._cells = None
-        self.every = 1          # steps between flag refreshes (SharpClaw SSP33)
+        self.every = None       # steps between flag refreshes; None: from the CFL bound
         self._age = None
 
     def flags(self, q, bc_lower, bc_upper, reflect=(1, 2)):
@@ -75,11 +75,20 @@ class TroubledInterfaceIndicator(object):
         """(cells, count) of the last flags() call: the cells the stage recomputes."""
         return self._cells
 
-    def step_flags(self, q, bcs, reflect=(1, 2)):
-        """Flags for the next time step: recomputed from q every `every` steps
-        (with CFL <= 0.5 a wave moves at most one cell in two steps, which the
-        stage kernel's one-interface dilation of the flags covers)."""
-        if self._age is None or self._age >= self.every or self._flags is None \
+    def refresh_interval(self, cfl_max=None):
+        """Steps between flag refreshes: `every` when set, else 2 when the
+        solver's CFL bound is at most 0.5 (a wave then moves at most one cell in
+        two steps, which the stage kernel's one-interface dilation of the flags
+        covers), else 1."""
+        if self.every is not None:
+            return int(self.every)
+        return 2 if cfl_max is not None and cfl_max <= 0.5 else 1
+
+    def step_flags(self, q, bcs, reflect=(1, 2), cfl_max=None):
+        """Flags for the next time step, recomputed from q every
+        refresh_interval(cfl_max) steps."""
+        every = self.refresh_interval(cfl_max)
+        if self._age is None or self._age >= every or self._flags is None \
                 or self._flags.numel() != K.mlp_num_interfaces(q.shape[2], q.shape[1]):
             self.flags(q, bcs[0], bcs[1], reflect)
             self._age = 0
@@ -182,9 +191,10 @@ def swe_problem(seed, mx, my):
     return q
 
 
-def swe_solver(precision="fp64", indicator=None, cfl=0.5, cfl_max=0.55):
+def swe_solver(precision="fp64", indicator=None, cfl=0.45, cfl_max=0.5):
     """SharpClawSolver2D for the periodic shallow-water family: characteristic
-    WENO5, SSP33, CFL 0.5 (cfl_max 0.55)."""
+    WENO5, SSP33, CFL 0.45 (cfl_max 0.5, so the indicator refreshes its flags
+    every second step)."""
     from . import BC, SharpClawSolver2D, riemann
     solver = SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
     solver.time_integrator = "SSP33"
@@ -220,7 +230,7 @@ def generate_training_data(seeds, mx=512, my=512, nsteps=300, snapshots=(50, 150
     for k, seed in enumerate(seeds):
         if k % 16 == 0:
             print(f"training data: run {k} of {len(seeds)}", flush=True)
-        solver = swe_solver("fp64")
+        solver = swe_solver("fp64", cfl=0.5, cfl_max=0.55)
         sol = swe_solution(seed, mx, my)
         solver.dt = 0.5 / mx / 1.5
         solver.setup(sol)

@@ -135,7 +135,8 @@ class SharpClawSolver(Solver):
         q, comp = state.device_q(self._dtype_code)
         self._flags = None
         if self.indicator is not None:
-            self._flags = self.indicator.step_flags(q, self._bc_codes(), self.reflect_index)
+            self._flags = self.indicator.step_flags(q, self._bc_codes(), self.reflect_index,
+                                                    self.cfl_max)
         if self.accept_step:
             self.cfl.set_global_max(0.)
             dqdt = self._buf("dq_dt_new", q)

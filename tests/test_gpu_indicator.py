@@ -239,3 +239,88 @@ def test_stage_all_flags_equals_char_weno():
         outs.append(q_from_device(out))
     assert np.array_equal(outs[0], outs[1])
 
+
+
+def _stage_ms(K, rp, q, flags=None, cells=None, reps=5, n=10):
+    out = torch.empty_like(q)
+    cfl = torch.zeros(1, dtype=torch.float64, device=q.device)
+    my, mx = q.shape[1], q.shape[2]
+    args = (rp, 2, out, cfl, [2, 2], [2, 2], [1, 2], 1e-4, 1.0 / mx, 1.0 / my, q)
+    K.sharp2_stage(*args, flags=flags, cells=cells)
+    return _best_ms(lambda: K.sharp2_stage(*args, flags=flags, cells=cells), reps, n)
+
+
+def _best_ms(fn, reps=5, n=10):
+    best = None
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / n
+        best = ms if best is None else min(best, ms)
+    return best
+
+
+def test_indicator_gain_accuracy_and_speed():
+    """The shipped network on the periodic shallow-water family (humps + dam
+    breaks, characteristic WENO5, SSP33, CFL 0.45 / max 0.5): FP32 + Kahan with
+    the learned selection (flags refreshed every second step) stays within 1e-4
+    relative L1 of FP64 full WENO5 over 100 steps at 256^2 while most interfaces
+    take the linear weights; at 2048^2 its step (three flagged FP32 stages plus
+    half a flags + troubled-cells pass) is at least 1.8x faster than the FP64
+    full-WENO step (three stages), and faster than the FP32 full-WENO step."""
+    from pyclaw_amd import indicator as I
+    from pyclaw_amd import kernels as K
+    res = {}
+    for name, prec, ind in (("fp64", "fp64", None), ("mlp", "fp32k", True)):
+        indicator = I.TroubledInterfaceIndicator() if ind else None
+        solver = I.swe_solver(prec, indicator)
+        sol = I.swe_solution(3, 256, 256)
+        solver.dt = 0.45 / 256 / 1.6
+        solver.setup(sol)
+        frac = []
+        for k in range(100):
+            solver.evolve_to_time(sol)
+            if indicator is not None and k % 10 == 0:
+                frac.append(indicator.troubled_fraction())
+        if indicator is not None:
+            assert indicator.refresh_interval(solver.cfl_max) == 2
+        res[name] = (sol.state.q.copy(), sol.t, np.mean(frac) if frac else None)
+    qa, ta, _ = res["fp64"]
+    qb, tb, frac = res["mlp"]
+    err = np.abs(qb - qa).sum() / np.abs(qa).sum()
+    print(f"256^2 x 100 steps: rel L1 {err:.2e}, troubled interfaces {frac:.3f}")
+    # the two runs choose their own dt from their own CFL numbers
+    assert abs(tb - ta) < 1e-4 * ta
+    assert err < 1e-4
+    assert frac < 0.5
+    # kernel-level step cost at 2048^2 on an evolved state
+    n = 2048
+    solver = I.swe_solver("fp64")
+    sol = I.swe_solution(3, n, n)
+    solver.dt = 0.45 / n / 1.6
+    solver.setup(sol)
+    for _ in range(10):
+        solver.evolve_to_time(sol)
+    q64 = sol.state.device_q(K.L.F64)[0].contiguous()
+    q32 = q64.float()
+    rp = K.make_rp(3, {"grav": 1.0, "efix": 1})
+    ind = I.TroubledInterfaceIndicator()
+    fl = ind.flags(q32, [2, 2], [2, 2])
+    torch.cuda.synchronize()
+    t64 = _stage_ms(K, rp, q64)
+    t32 = _stage_ms(K, rp, q32)
+    t32f = _stage_ms(K, rp, q32, fl, ind.cells)
+    tflags = _best_ms(lambda: ind.flags(q32, [2, 2], [2, 2]))
+    step64 = 3 * t64
+    step32 = 3 * t32
+    step_mlp = 3 * t32f + tflags / 2
+    print(f"2048^2 stage ms: fp64 {t64:.3f} fp32 {t32:.3f} fp32 flagged {t32f:.3f} "
+          f"flags+cells {tflags:.3f} troubled {ind.troubled_fraction():.4f}; step ratio "
+          f"fp64/mlp {step64 / step_mlp:.2f}")
+    assert step_mlp < step32
+    assert step64 / step_mlp >= 1.8
