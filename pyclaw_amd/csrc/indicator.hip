@@ -630,6 +630,76 @@ __global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc
   }
 }
 
+// Flags with one interface per lane and its ME components in registers (ME <=
+// 4): a pass takes 64 consecutive interfaces of one row, and the ME component
+// samples of the pass go through the network as ME independent 64-sample
+// evaluations, which the scheduler interleaves.  Same flags as k_mlp_flags.
+template <typename T, int ME>
+__global__ void __launch_bounds__(256) k_mlp_flags_row(const T* q, Geom g, BCSpec bc, float flat,
+                                                       const float* master, float zthr,
+                                                       unsigned char* flags) {
+  const int lane = threadIdx.x & 63;
+  const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int stride = gridDim.x * 4;
+  const int mx = g.mx, my = g.my;
+  const int npx = (mx + 1 + 63) / 64, npy = (mx + 63) / 64;
+  const int nxp = npx * my;
+  const int npass = nxp + npy * (my + 1);
+  if (wid >= npass) return;
+  MlpRegs reg;
+  mlp_load_regs(master, reg);
+  for (int pass = wid; pass < npass; pass += stride) {
+    int dir, r, cb;
+    if (pass < nxp) {
+      dir = 0; r = 1 + pass / npx; cb = pass % npx;
+    } else {
+      dir = 1; r = 1 + (pass - nxp) / npy; cb = (pass - nxp) % npy;
+    }
+    const int cmax = dir == 0 ? mx + 1 : mx;
+    const int ci = 1 + cb * 64 + lane;
+    const int c = min(ci, cmax);
+    const bool interior = dir == 0 ? (cb * 64 >= 3 && cb * 64 + 64 + 2 <= mx)
+                                   : (r >= 4 && r + 2 <= my);
+    T v[ME][6];
+    if (interior) {
+      const long long step = dir == 0 ? 1 : g.rs;
+#pragma unroll
+      for (int m = 0; m < ME; m++) {
+        const T* p0 = q + (long long)m * g.fs + (long long)(r - 1) * g.rs + (c - 1);
+#pragma unroll
+        for (int k = 0; k < 6; k++) v[m][k] = p0[(k - 3) * step];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < ME; m++) gather_stencil_at(q, g, bc, dir, c, r, m, v[m]);
+    }
+    bool rough[ME];
+    u32x2 lo[ME], hi[ME];
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      float f[6];
+      rough[m] = norm_stencil(v[m], flat, f) > flat;
+      lo[m] = u32x2{pack_bf16(f[0], f[1]), pack_bf16(f[2], f[3])};
+      hi[m] = u32x2{pack_bf16(f[4], f[5]), 0x3f80u};   // bias input 1.0, then 0
+      any = any || rough[m];
+    }
+    bool hit = false;
+    if (__ballot(any) != 0ull) {
+#pragma unroll
+      for (int m = 0; m < ME; m++) {
+        const float z = mlp_forward_regs(reg, lo[m], hi[m]);
+        hit = hit || (rough[m] && z > zthr);
+      }
+    }
+    if (ci <= cmax) {
+      const long long fi = dir == 0 ? (long long)(r - 1) * (mx + 1) + (ci - 1)
+                                    : (long long)(mx + 1) * my + (long long)(r - 1) * mx + (ci - 1);
+      flags[fi] = hit ? 1 : 0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- troubled cells
 __global__ void k_zero_count(int* count) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0;
@@ -738,12 +808,36 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
   }
   long long blocks = (passes + 3) / 4;
   if (blocks > cap) blocks = cap;
-  if (dtype == CLAWHIP_F64)
+  if (meqn <= 4) {
+    const long long rpass = (long long)((g.mx + 1 + 63) / 64) * g.my +
+                            (long long)((g.mx + 63) / 64) * (g.my + 1);
+    long long rblocks = (rpass + 3) / 4;
+    if (rblocks > cap) rblocks = cap;
+    const dim3 grid((unsigned)rblocks);
+    if (dtype == CLAWHIP_F64) {
+      const double* qd = (const double*)q;
+      switch (meqn) {
+        case 1: hipLaunchKernelGGL((k_mlp_flags_row<double, 1>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
+        case 2: hipLaunchKernelGGL((k_mlp_flags_row<double, 2>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
+        case 3: hipLaunchKernelGGL((k_mlp_flags_row<double, 3>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
+        default: hipLaunchKernelGGL((k_mlp_flags_row<double, 4>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
+      }
+    } else {
+      const float* qf = (const float*)q;
+      switch (meqn) {
+        case 1: hipLaunchKernelGGL((k_mlp_flags_row<float, 1>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
+        case 2: hipLaunchKernelGGL((k_mlp_flags_row<float, 2>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
+        case 3: hipLaunchKernelGGL((k_mlp_flags_row<float, 3>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
+        default: hipLaunchKernelGGL((k_mlp_flags_row<float, 4>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
+      }
+    }
+  } else if (dtype == CLAWHIP_F64) {
     hipLaunchKernelGGL(k_mlp_flags<double>, dim3((unsigned)blocks), dim3(256), 0, s,
                        (const double*)q, g, bc, meqn, flat, master, zthr, flags);
-  else
+  } else {
     hipLaunchKernelGGL(k_mlp_flags<float>, dim3((unsigned)blocks), dim3(256), 0, s,
                        (const float*)q, g, bc, meqn, flat, master, zthr, flags);
+  }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
 }

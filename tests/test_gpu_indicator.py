@@ -137,11 +137,20 @@ def test_adam_step_matches_torch():
     assert np.abs(p1 - ref).max() < 2e-5
 
 
-def test_flags_match_oracle():
+@pytest.mark.parametrize("meqn", [1, 3, 4, 5])
+def test_flags_match_oracle(meqn):
+    """meqn <= 4 runs the one-interface-per-lane kernel, larger meqn the
+    (interface, component)-per-lane kernel; both against the oracle."""
     from pyclaw_amd import indicator as I
     rng = np.random.default_rng(8)
     mx, my = 33, 26
     q = _swe_state(rng, mx, my)
+    if meqn < 3:
+        q = np.asfortranarray(q[:meqn])
+    elif meqn > 3:
+        extra = [q[0] * (1.0 + 0.1 * k) + 0.2 * k * np.roll(q[1], k, axis=0)
+                 for k in range(1, meqn - 2)]
+        q = np.asfortranarray(np.concatenate([q] + [e[None] for e in extra], axis=0))
     p = _params(9)
     ind = I.TroubledInterfaceIndicator(p, threshold=0.2, flat=1e-6)
     fl = ind.flags(_dev(q), [1, 2], [3, 2]).cpu().numpy()
