@@ -258,6 +258,8 @@ def main(argv=None):
     ap.add_argument("--sim-steps", type=int, default=300)
     ap.add_argument("--flat", type=float, default=1e-6)
     ap.add_argument("--wtol", type=float, default=0.05)
+    ap.add_argument("--ship-flat", type=float, default=None,
+                    help="the `flat` stored with the weights for inference (default: --flat)")
     args = ap.parse_args(argv)
     snaps = tuple(sorted({max(1, args.sim_steps // 6), args.sim_steps // 2, args.sim_steps}))
     feat, label = generate_training_data(range(1000, 1000 + args.runs), args.mx, args.mx,
@@ -279,7 +281,8 @@ def main(argv=None):
     print(f"threshold {thr:.3f} recall {recall:.4f} flagged {flagged:.3f} "
           f"(troubled {lv.float().mean().item():.3f})", flush=True)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-    save(args.out, P.cpu().numpy(), thr, args.flat)
+    save(args.out, P.cpu().numpy(), thr,
+         args.flat if args.ship_flat is None else args.ship_flat)
     print(f"saved {args.out}", flush=True)
 
 
