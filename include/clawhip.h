@@ -274,18 +274,36 @@ int clawhip_mlp_train_step(const float *feat, long long nsamples, const unsigned
 int clawhip_mlp_logits(const float *feat, long long n, const float *params, float *logits,
                        void *stream);
 
-/* Per-interface flags of a state: 1 when any component whose stencil range exceeds
- * flat has a logit > logit_threshold. */
 /* The cells (index (r-1)*mx + (c-1)) with at least one flagged interface among
  * their four, flags dilated as clawhip_sharp2_stage applies them; *count = their
  * number (cells holds up to mx*my entries, in no particular order). */
 int clawhip_mlp_troubled_cells(const unsigned char *flags, int mx, int my, int *cells,
                                int *count, void *stream);
 
+/* Per-interface flags of a state: 1 when any component whose stencil range exceeds
+ * flat has a logit > logit_threshold. */
 int clawhip_mlp_flags(int dtype, const void *q, const clawhip_geom *g, const int *bc_lower,
                       const int *bc_upper, const int *reflect, int meqn, float flat,
                       const float *params, float logit_threshold, unsigned char *flags,
                       void *stream);
+
+/* ---- FP64 reductions over a state (no counterpart in the reference's kernels:
+ * they replace the numpy sums of q PyClaw scripts take for conservation checks,
+ * e.g. sum(q)*dx*dy, and give the indicator path its positivity and L1 checks).
+ * Deterministic (fixed reduction order).  work: device scratch of
+ * clawhip_sums_work_doubles(nfields) doubles, nfields = meqn + 1 for
+ * clawhip_state_sums and 2 for clawhip_l1_diff. */
+long long clawhip_sums_work_doubles(int nfields);
+
+/* out[f] = sum over the interior cells of field f (f < meqn <= 7), in FP64;
+ * out[meqn] = number of cells where a field whose bit is set in positive_mask is
+ * <= 0 or not finite.  2D (mz = 1) or 3D geometry, any stored ghost width. */
+int clawhip_state_sums(int dtype, const void *q, const clawhip_geom *g, int meqn,
+                       unsigned positive_mask, double *out, double *work, void *stream);
+
+/* out[0] = sum |a[i] - b[i]|, out[1] = sum |b[i]| over n elements, in FP64. */
+int clawhip_l1_diff(int dtype, const void *a, const void *b, long long n, double *out,
+                    double *work, void *stream);
 
 #ifdef __cplusplus
 }

@@ -368,4 +368,24 @@ int clawhip_mlp_flags(int dtype, const void* q, const clawhip_geom* g, const int
                           logit_threshold, flags, (hipStream_t)stream);
 }
 
+long long clawhip_sums_work_doubles(int nfields) {
+  return nfields < 1 ? 0 : 1024LL * nfields;
+}
+
+int clawhip_state_sums(int dtype, const void* q, const clawhip_geom* g, int meqn,
+                       unsigned positive_mask, double* out, double* work, void* stream) {
+  if (!q || !g || !out || !work || meqn < 1 || meqn > 7) return CLAWHIP_E_ARG;
+  if (!geom_ok(g, g->mz > 1 ? 3 : 2) || g->mz < 1) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  return launch_state_sums(dtype, q, to_geom(g), meqn, positive_mask, out, work,
+                           (hipStream_t)stream);
+}
+
+int clawhip_l1_diff(int dtype, const void* a, const void* b, long long n, double* out,
+                    double* work, void* stream) {
+  if (!a || !b || !out || !work || n < 0) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  return launch_l1_diff(dtype, a, b, n, out, work, (hipStream_t)stream);
+}
+
 }  // extern "C"

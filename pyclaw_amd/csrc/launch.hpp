@@ -86,5 +86,9 @@ int launch_troubled_cells(const unsigned char* flags, int mx, int my, int* cells
 int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
                      const int* refl, int meqn, float flat, const float* master, float zthr,
                      unsigned char* flags, hipStream_t s);
+int launch_state_sums(int dtype, const void* q, const Geom& g, int meqn, unsigned positive_mask,
+                      double* out, double* work, hipStream_t s);
+int launch_l1_diff(int dtype, const void* a, const void* b, long long n, double* out,
+                   double* work, hipStream_t s);
 
 }  // namespace claw

@@ -58,6 +58,14 @@ class TroubledInterfaceIndicator(object):
         self._cells = None
         self.every = None       # steps between flag refreshes; None: from the CFL bound
         self._age = None
+        # full WENO5 judges the indicator's steps (SharpClawSolver._judge): a step
+        # that loses positivity is recomputed with full WENO5, and every
+        # audit_every steps the step is compared with the full-WENO5 one
+        self.audit_every = 100
+        self.audit_tol = 1e-4   # relative L1 between the two states, per step
+        self.judge_stats = {"steps": 0, "positivity_fallbacks": 0, "audits": 0,
+                            "audit_failures": 0, "last_audit_rel_l1": None,
+                            "max_audit_rel_l1": 0.0}
 
     def flags(self, q, bc_lower, bc_upper, reflect=(1, 2)):
         """uint8 flags of every interface of the planar state q (meqn, my, mx)."""

@@ -219,3 +219,48 @@ def mlp_troubled_cells(flags, mx, my, out=None):
                                             _ptr(out[1]), _stream())
     L.check(rc, "clawhip_mlp_troubled_cells")
     return out
+
+
+# ---- FP64 reductions over a state
+_sums_work = {}
+
+
+def _work(nfields, device):
+    n = int(L.lib().clawhip_sums_work_doubles(int(nfields)))
+    key = (str(device), n)
+    w = _sums_work.get(key)
+    if w is None:
+        w = torch.empty(n, dtype=torch.float64, device=device)
+        _sums_work[key] = w
+    return w
+
+
+def state_sums(q, positive=(), out=None):
+    """Device float64 tensor (meqn + 1,): the FP64 total of each field of the
+    planar state q ((meqn, my, mx) or (meqn, mz, my, mx)) over its cells, then the
+    number of cells where a field listed in `positive` is <= 0 or not finite
+    (clawhip_state_sums; fixed reduction order)."""
+    meqn = q.shape[0]
+    g = L.geom2(q.shape[2], q.shape[1]) if q.dim() == 3 else \
+        L.geom3(q.shape[3], q.shape[2], q.shape[1])
+    if out is None:
+        out = torch.empty(meqn + 1, dtype=torch.float64, device=q.device)
+    mask = 0
+    for f in positive:
+        mask |= 1 << int(f)
+    rc = L.lib().clawhip_state_sums(_dtype_code(q), _ptr(q), ctypes.byref(g), meqn, mask,
+                                    _ptr(out), _ptr(_work(meqn + 1, q.device)), _stream())
+    L.check(rc, "clawhip_state_sums")
+    return out
+
+
+def l1_diff(a, b, out=None):
+    """Device float64 tensor (2,): sum |a - b| and sum |b| (clawhip_l1_diff)."""
+    if a.shape != b.shape or a.dtype != b.dtype:
+        raise ValueError("l1_diff needs two arrays of one shape and dtype")
+    if out is None:
+        out = torch.empty(2, dtype=torch.float64, device=a.device)
+    rc = L.lib().clawhip_l1_diff(_dtype_code(a), _ptr(a), _ptr(b), int(a.numel()), _ptr(out),
+                                 _ptr(_work(2, a.device)), _stream())
+    L.check(rc, "clawhip_l1_diff")
+    return out

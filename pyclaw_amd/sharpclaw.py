@@ -148,27 +148,39 @@ class SharpClawSolver(Solver):
         cfl = self.cfl.get_cached_max()
         self.cfl.set_global_max(self.dt / self.dt_old * cfl)
         self.dt_old = self.dt
-        dt = self.dt
         out_q = self._take_scratch((q, comp))
         # the later stages max-reduce their Courant numbers on the device; one
         # read at the end of the step (max is order-free, so self.cfl ends as the
         # stage-by-stage updates of sharpclaw/solver.py:283-430 leave it)
         cs = self._stage_cfl()
+        self._rk_stages(q, comp, self.dq_dt, out_q, self.dt, cs)
+        if self._flags is not None:
+            out_q = self._judge(q, comp, out_q, cs)
+        else:
+            self.cfl.update_global_max(float(cs.item()))
+        state.set_device_q(*out_q)
+        self._scratch = (q, comp)
+        self._flags = None
+        return True
+
+    def _rk_stages(self, q, comp, dq_dt, out_q, dt, cs):
+        """The Runge-Kutta stages after the first operator evaluation
+        (sharpclaw/solver.py:312-323, 400-430), into out_q."""
         if self.time_integrator == "SSP33":
             r2 = self._buf("r2", q)
             if self.precision == "fp64":
-                self._stage(q, r2, q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
+                self._stage(q, r2, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
                             out_mode=K.OUT_SSP33_2, A=q)
                 self._stage(q, out_q[0], r2, cfl=cs, out_mode=K.OUT_SSP33_3, A=q)
             else:
                 aux = self._buf("aux", q)
-                self._stage(q, r2, q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
-                            out_mode=K.OUT_F32_S2, A=q, B=self.dq_dt, aux_out=aux)
+                self._stage(q, r2, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
+                            out_mode=K.OUT_F32_S2, A=q, B=dq_dt, aux_out=aux)
                 self._stage(q, out_q[0], r2, cfl=cs, out_mode=K.OUT_F32_S3, A=q, aux_in=aux,
                             comp_in=comp, comp_out=out_q[1])
         else:   # SSP104
             s1a, s1b, qm = self._buf("s1a", q), self._buf("s1b", q), self._buf("qm", q)
-            self._stage(q, s1a, q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY6, out_mode=K.OUT_ADD6)
+            self._stage(q, s1a, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY6, out_mode=K.OUT_ADD6)
             cur, nxt = s1a, s1b
             for _ in range(3):
                 self._stage(q, nxt, cur, cfl=cs, out_mode=K.OUT_ADD6)
@@ -179,11 +191,69 @@ class SharpClawSolver(Solver):
                 self._stage(q, nxt, cur, cfl=cs, out_mode=K.OUT_ADD6)
                 cur, nxt = nxt, cur
             self._stage(q, out_q[0], cur, cfl=cs, out_mode=K.OUT_SSP104_FIN, A=qm)
-        self.cfl.update_global_max(float(cs.item()))
-        state.set_device_q(*out_q)
-        self._scratch = (q, comp)
-        self._flags = None
-        return True
+
+    def _full_weno_step(self, q, comp, out_q, cs):
+        """The same step (same dt) with full WENO5 at every interface: the first
+        operator evaluation and the later stages, into out_q; Courant numbers
+        max-reduced into cs."""
+        flags, self._flags = self._flags, None
+        try:
+            dqdt = self._buf("dq_dt_full", q)
+            self._stage(q, dqdt, q, cfl=cs, out_mode=K.OUT_DQDT)
+            self._rk_stages(q, comp, dqdt, out_q, self.dt, cs)
+        finally:
+            self._flags = flags
+
+    def _positive_fields(self):
+        """Fields that must stay positive: depth (shallow water), density and
+        energy (Euler)."""
+        return {2: (0, 3), 3: (0,)}.get(getattr(self.rp, "kind", None), ())
+
+    def _judge(self, q, comp, out_q, cs):
+        """Full WENO5 judges the indicator's step.  The step is recomputed with
+        full WENO5 when it leaves a non-positive depth / density / energy (or a
+        non-finite value), and every indicator.audit_every steps it is compared
+        with the full-WENO5 step: above indicator.audit_tol (relative L1 of the
+        states) the full-WENO5 result is kept and the flags are refreshed every
+        step from then on.  Returns the buffers holding the accepted result."""
+        ind = self.indicator
+        ind.judge_stats["steps"] += 1
+        pos = self._positive_fields()
+        audit = ind.audit_every and ind.judge_stats["steps"] % int(ind.audit_every) == 0
+        if pos:
+            bad = K.state_sums(out_q[0], pos)[-1:]
+            cfl_stages, nbad = torch.cat([cs, bad]).tolist()
+        else:
+            cfl_stages, nbad = float(cs.item()), 0.0
+        self.cfl.update_global_max(cfl_stages)
+        if nbad > 0:
+            ind.judge_stats["positivity_fallbacks"] += 1
+            cs2 = self._stage_cfl()
+            self._full_weno_step(q, comp, out_q, cs2)
+            self.cfl.update_global_max(float(cs2.item()))
+            return out_q
+        if audit:
+            alt = (self._buf("audit_q", q), self._buf("audit_comp", comp) if comp is not None
+                   else None)
+            cs2 = self._stage_cfl()
+            self._full_weno_step(q, comp, alt, cs2)
+            d = K.l1_diff(out_q[0], alt[0])
+            cfl_full, diff, norm = torch.cat([cs2, d]).tolist()
+            rel = diff / norm if norm > 0 else diff
+            ind.judge_stats["audits"] += 1
+            ind.judge_stats["last_audit_rel_l1"] = rel
+            ind.judge_stats["max_audit_rel_l1"] = max(ind.judge_stats["max_audit_rel_l1"], rel)
+            if not rel <= ind.audit_tol:
+                ind.judge_stats["audit_failures"] += 1
+                ind.every = 1
+                self.cfl.update_global_max(cfl_full)
+                # the full-WENO5 result becomes the step's; its buffers swap
+                # places with the scratch ones
+                self._bufs["audit_q"] = out_q[0]
+                if comp is not None:
+                    self._bufs["audit_comp"] = out_q[1]
+                return alt
+        return out_q
 
     def dq(self, state):
         """dt*L(q) on the interior as a host array (sharpclaw/solver.py:532-542)."""
