@@ -173,6 +173,56 @@ def cpu_baseline(sample_steps=3):
                       f"{wall:.1f} s"}
 
 
+def acoustics_leg(n=256, seed=1, t_final=0.6):
+    """2D acoustics (rho 1, bulk 4, c 2) on [0,1]^2, 256^2, solid walls, classic
+    unsplit with transverse order 2, MC, CFL 0.9 (max 1.0), dt_initial 0.9 dx / c,
+    to t = 0.6: p = exp(-100 r^2) + 1e-3 U[-1,1) (seed 1), u = v = 0.  FP64 and
+    FP32 + Kahan through the device-controlled evolve_to_time.  The run to
+    t = 0.6 (untimed: it includes the graph capture) gives the errors; the rate
+    is timed on the continuation from t = 0.6 to 1.2."""
+    import torch
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    rng = np.random.default_rng(seed)
+    xc = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    q0 = np.zeros((3, n, n), order="F")
+    q0[0] = np.exp(-100.0 * ((X - 0.5) ** 2 + (Y - 0.5) ** 2)) + 1e-3 * rng.uniform(-1, 1, X.shape)
+    out, qs = {}, {}
+    for prec in ("fp64", "fp32k"):
+        solver = pyclaw.ClawSolver2D(riemann.acoustics_2D)
+        solver.dimensional_split = False
+        solver.transverse_waves = 2
+        solver.limiters = pyclaw.limiters.tvd.MC
+        solver.cfl_desired, solver.cfl_max = 0.9, 1.0
+        solver.all_bcs = pyclaw.BC.wall
+        solver.precision = prec
+        sol = pyclaw.Solution(3, pyclaw.Domain([0., 0.], [1., 1.], [n, n]))
+        sol.problem_data.update({"rho": 1.0, "bulk": 4.0, "cc": 2.0, "zz": 2.0})
+        sol.q = q0
+        solver.dt_initial = 0.9 * (1.0 / n) / 2.0
+        solver.dt = solver.dt_initial
+        solver.evolve_to_time(sol, t_final)
+        steps_to_tfinal = solver.status["numsteps"]
+        qs[prec] = sol.state.q.copy()
+        t_at = sol.t
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.evolve_to_time(sol, 2 * t_final)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        steps = solver.status["numsteps"] - steps_to_tfinal
+        out[prec] = {"value": n * n * steps / wall, "steps_to_t_final": steps_to_tfinal,
+                     "t_final": t_at, "timed_steps": steps, "ms_per_step": 1e3 * wall / steps,
+                     "hbm_frac": 48.0 * n * n / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+    qa, qb = qs["fp64"], qs["fp32k"]
+    out["fp32k"]["rel_l1_vs_fp64"] = float(np.abs(qb - qa).sum() / np.abs(qa).sum())
+    out["fp32k"]["rel_max_vs_fp64"] = float(np.abs(qb - qa).max() / np.abs(qa).max())
+    out.update({"workload": "acoustics2d_gaussian_walls_unsplit_mc_256_t0.6",
+                "unit": "cell-updates/s", "bytes_per_cell": 48})
+    return out
+
+
 def ensemble_leg(steps=32, members=512, seed=4):
     """512 independent 2D Euler Roe/MC unsplit problems (sides log-uniform in
     [64, 512], independent in x and y; seeded quadrant states as the headline
@@ -201,7 +251,26 @@ def ensemble_leg(steps=32, members=512, seed=4):
     wall = time.perf_counter() - t0
     cells = ens.cells
     acc = sum(c.accepted for c in ens.controllers())
+    # the same members and steps in FP64 (untimed): error of the FP32 + Kahan run
+    sols64 = []
+    for k, (mx, my) in enumerate(sides):
+        sol = pyclaw.Solution(4, pyclaw.Domain([0., 0.], [1., 1.], [int(mx), int(my)]))
+        sol.problem_data["gamma"] = 1.4
+        sol.q = quadrant_state(int(mx), int(my), seed=1000 + k)
+        sols64.append(sol)
+    solver64, _ = make_problem("fp64", 64, 64)
+    solver64.dt_initial = solver.dt_initial
+    ens64 = Ensemble(solver64, sols64)
+    ens64.run_steps(16 + steps)
+    diff = norm = mx_d = mx_n = 0.0
+    ctl64, ctl32 = ens64.controllers(), ens.controllers()
+    for k in range(members):
+        qa, qb = ens64.member_q(k, ctl64), ens.member_q(k, ctl32)
+        diff += float(np.abs(qb - qa).sum())
+        norm += float(np.abs(qa).sum())
+        mx_d, mx_n = max(mx_d, float(np.abs(qb - qa).max())), max(mx_n, float(np.abs(qa).max()))
     return {"workload": "ensemble_512_euler2d_roe_mc_unsplit_fp32k_loguniform_64_512",
+            "rel_l1_vs_fp64": diff / norm, "rel_max_vs_fp64": mx_d / mx_n,
             "value": cells * steps / wall, "unit": "cell-updates/s", "members": members,
             "cells": cells, "steps": steps, "ms_per_step": 1e3 * wall / steps,
             "rows_per_wavefront": ens.rows,
@@ -253,6 +322,7 @@ def euler3d_leg(steps=10, n=384, seed=5):
         out[prec + "_q"] = sol.state.q
     qa, qb = out.pop("fp64_q"), out.pop("fp32k_q")
     out["fp32k"]["rel_l1_vs_fp64"] = float(np.abs(qb - qa).sum() / np.abs(qa).sum())
+    out["fp32k"]["rel_max_vs_fp64"] = float(np.abs(qb - qa).max() / np.abs(qa).max())
     out.update({"workload": "euler3d_blast_roe_mc_split_384cubed", "unit": "cell-updates/s",
                 "steps": steps, "bytes_per_cell_step": 240})
     return out
@@ -288,10 +358,12 @@ def sharpclaw_leg(steps=20, n=2048, seed=3):
         if indicator is not None:
             res[name]["troubled_fraction"] = indicator.troubled_fraction()
             res[name]["flag_refresh_every"] = indicator.refresh_interval(solver.cfl_max)
+            res[name]["judge"] = dict(indicator.judge_stats)
         qs[name] = sol.state.q
     ref = qs["fp64_weno"]
     for name in ("fp32k_weno", "fp32k_mlp"):
         res[name]["rel_l1_vs_fp64"] = float(np.abs(qs[name] - ref).sum() / np.abs(ref).sum())
+        res[name]["rel_max_vs_fp64"] = float(np.abs(qs[name] - ref).max() / np.abs(ref).max())
     res["speedup_fp32k_mlp_vs_fp64_weno"] = res["fp32k_mlp"]["value"] / res["fp64_weno"]["value"]
     res["speedup_fp32k_weno_vs_fp64_weno"] = (res["fp32k_weno"]["value"] /
                                               res["fp64_weno"]["value"])
@@ -306,7 +378,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="ensemble,sharpclaw,euler3d",
+    ap.add_argument("--extras", default="acoustics,ensemble,sharpclaw,euler3d",
                     help="comma list of extra workloads to time beside the headline")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the steps one by one (for per-dispatch counter runs)")
@@ -364,7 +436,8 @@ def main():
                        "rel_l1_vs_fp64": err_l1, "rel_max_vs_fp64": err_max,
                        "t_end": sol32.t},
     }
-    legs = {"ensemble": ensemble_leg, "sharpclaw": sharpclaw_leg, "euler3d": euler3d_leg}
+    legs = {"acoustics": acoustics_leg, "ensemble": ensemble_leg, "sharpclaw": sharpclaw_leg,
+            "euler3d": euler3d_leg}
     for name in [x for x in args.extras.split(",") if x]:
         try:
             rec[name] = legs[name]()

@@ -369,7 +369,7 @@ int clawhip_mlp_flags(int dtype, const void* q, const clawhip_geom* g, const int
 }
 
 long long clawhip_sums_work_doubles(int nfields) {
-  return nfields < 1 ? 0 : 1024LL * nfields;
+  return nfields < 1 ? 0 : 2048LL * nfields;
 }
 
 int clawhip_state_sums(int dtype, const void* q, const clawhip_geom* g, int meqn,

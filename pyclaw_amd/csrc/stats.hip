@@ -10,7 +10,7 @@
 
 namespace claw {
 
-constexpr int SUMS_BLOCKS = 1024;
+constexpr int SUMS_BLOCKS = 2048;
 constexpr int SUMS_THREADS = 256;
 constexpr int SUMS_MAXF = 8;      // fields reduced at once (meqn + 1 <= 8)
 
@@ -30,6 +30,8 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 
 // partial[b * (nf) + f]: block b's sum of field f (f < meqn) and, at f = meqn,
 // its count of cells where a field in positive_mask is <= 0 or not finite.
+// Block b takes the rows (j, k) = b, b + SUMS_BLOCKS, ...; its threads stride
+// along each row.
 template <typename T>
 __global__ void __launch_bounds__(SUMS_THREADS) k_state_sums(const T* q, Geom g, int meqn,
                                                              unsigned positive_mask,
@@ -38,25 +40,24 @@ __global__ void __launch_bounds__(SUMS_THREADS) k_state_sums(const T* q, Geom g,
   double acc[SUMS_MAXF - 1], nbad = 0.0;
 #pragma unroll
   for (int f = 0; f < SUMS_MAXF - 1; f++) acc[f] = 0.0;
-  const long long ncell = (long long)g.mx * g.my * g.mz;
-  const long long stride = (long long)SUMS_BLOCKS * SUMS_THREADS;
-  for (long long i = (long long)blockIdx.x * SUMS_THREADS + threadIdx.x; i < ncell; i += stride) {
-    const int c = (int)(i % g.mx);
-    const long long rest = i / g.mx;
-    const int r = (int)(rest % g.my);
-    const int k = (int)(rest / g.my);
-    const long long off = (long long)(c + g.g) + (long long)(r + g.g) * g.rs +
-                          (g.mz > 1 ? (long long)(k + g.g) * g.ss : 0LL);
-    bool bad = false;
+  const int nrow = g.my * g.mz;
+  for (int row = blockIdx.x; row < nrow; row += SUMS_BLOCKS) {
+    const int r = row % g.my, k = row / g.my;
+    const T* base = q + (long long)g.g + (long long)(r + g.g) * g.rs +
+                    (g.mz > 1 ? (long long)(k + g.g) * g.ss : 0LL);
+#pragma unroll 4
+    for (int c = threadIdx.x; c < g.mx; c += SUMS_THREADS) {
+      bool bad = false;
 #pragma unroll
-    for (int f = 0; f < SUMS_MAXF - 1; f++) {
-      if (f < meqn) {
-        const double x = (double)q[(long long)f * g.fs + off];
-        acc[f] += x;
-        if ((positive_mask >> f) & 1u) bad = bad || !(x > 0.0) || !isfinite(x);
+      for (int f = 0; f < SUMS_MAXF - 1; f++) {
+        if (f < meqn) {
+          const double x = (double)base[(long long)f * g.fs + c];
+          acc[f] += x;
+          if ((positive_mask >> f) & 1u) bad = bad || !(x > 0.0) || !isfinite(x);
+        }
       }
+      if (bad) nbad += 1.0;
     }
-    if (bad) nbad += 1.0;
   }
 #pragma unroll
   for (int f = 0; f < SUMS_MAXF; f++) {

@@ -171,9 +171,10 @@ class Ensemble(object):
             out.append(c)
         return out
 
-    def member_q(self, k):
-        """Host copy of member k's current state, (meqn, mx, my) as Solution.q."""
-        c = self.controllers()[k]
+    def member_q(self, k, controllers=None):
+        """Host copy of member k's current state, (meqn, mx, my) as Solution.q
+        (controllers: a controllers() list already read, to skip the read)."""
+        c = (controllers if controllers is not None else self.controllers())[k]
         b = self._bufs[k]
         return q_from_device(b[c.cur] if c.cur in (0, 1) else b[0])
 

@@ -117,7 +117,16 @@ class SharpClawSolver(Solver):
                        self.dt, dx, dy, P, flags=self._flags,
                        cells=self.indicator.cells if self._flags is not None else None, **kw)
         if now:
-            self.cfl.update_global_max(float(cfl.item()))
+            self.cfl.update_global_max(self._read_back(cfl)[0])
+
+    def _step_buffer(self, meqn):
+        """Device float64 [Courant number of the later stages, totals of the
+        meqn fields, non-positive cell count], read back with one copy."""
+        t = self._bufs.get("step_buf")
+        if t is None or t.numel() != meqn + 2:
+            t = torch.zeros(meqn + 2, dtype=torch.float64, device="cuda")
+            self._bufs["step_buf"] = t
+        return t
 
     def _stage_cfl(self):
         """Device buffer for the Courant number of a step's later stages."""
@@ -137,6 +146,9 @@ class SharpClawSolver(Solver):
         if self.indicator is not None:
             self._flags = self.indicator.step_flags(q, self._bc_codes(), self.reflect_index,
                                                     self.cfl_max)
+        sb = self._step_buffer(q.shape[0])
+        cs = sb[0:1]
+        K.cfl_reset(cs)   # before stage 1: off the path between its CFL read and stage 2
         if self.accept_step:
             self.cfl.set_global_max(0.)
             dqdt = self._buf("dq_dt_new", q)
@@ -152,12 +164,11 @@ class SharpClawSolver(Solver):
         # the later stages max-reduce their Courant numbers on the device; one
         # read at the end of the step (max is order-free, so self.cfl ends as the
         # stage-by-stage updates of sharpclaw/solver.py:283-430 leave it)
-        cs = self._stage_cfl()
         self._rk_stages(q, comp, self.dq_dt, out_q, self.dt, cs)
         if self._flags is not None:
-            out_q = self._judge(q, comp, out_q, cs)
+            out_q = self._judge(q, comp, out_q, sb)
         else:
-            self.cfl.update_global_max(float(cs.item()))
+            self.cfl.update_global_max(self._read_back(cs)[0])
         state.set_device_q(*out_q)
         self._scratch = (q, comp)
         self._flags = None
@@ -209,7 +220,7 @@ class SharpClawSolver(Solver):
         energy (Euler)."""
         return {2: (0, 3), 3: (0,)}.get(getattr(self.rp, "kind", None), ())
 
-    def _judge(self, q, comp, out_q, cs):
+    def _judge(self, q, comp, out_q, sb):
         """Full WENO5 judges the indicator's step.  The step is recomputed with
         full WENO5 when it leaves a non-positive depth / density / energy (or a
         non-finite value), and every indicator.audit_every steps it is compared
@@ -221,16 +232,19 @@ class SharpClawSolver(Solver):
         pos = self._positive_fields()
         audit = ind.audit_every and ind.judge_stats["steps"] % int(ind.audit_every) == 0
         if pos:
-            bad = K.state_sums(out_q[0], pos)[-1:]
-            cfl_stages, nbad = torch.cat([cs, bad]).tolist()
+            # only the fields up to the last checked one are read
+            nf = max(pos) + 1
+            K.state_sums(out_q[0][:nf], pos, out=sb[1:nf + 2])
+            vals = self._read_back(sb)
+            cfl_stages, nbad = vals[0], vals[nf + 1]
         else:
-            cfl_stages, nbad = float(cs.item()), 0.0
+            cfl_stages, nbad = self._read_back(sb[0:1])[0], 0.0
         self.cfl.update_global_max(cfl_stages)
         if nbad > 0:
             ind.judge_stats["positivity_fallbacks"] += 1
             cs2 = self._stage_cfl()
             self._full_weno_step(q, comp, out_q, cs2)
-            self.cfl.update_global_max(float(cs2.item()))
+            self.cfl.update_global_max(self._read_back(cs2)[0])
             return out_q
         if audit:
             alt = (self._buf("audit_q", q), self._buf("audit_comp", comp) if comp is not None
@@ -238,7 +252,7 @@ class SharpClawSolver(Solver):
             cs2 = self._stage_cfl()
             self._full_weno_step(q, comp, alt, cs2)
             d = K.l1_diff(out_q[0], alt[0])
-            cfl_full, diff, norm = torch.cat([cs2, d]).tolist()
+            (cfl_full,), (diff, norm) = self._read_back(cs2), self._read_back(d)
             rel = diff / norm if norm > 0 else diff
             ind.judge_stats["audits"] += 1
             ind.judge_stats["last_audit_rel_l1"] = rel

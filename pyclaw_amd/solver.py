@@ -84,6 +84,7 @@ class Solver(object):
         self.precision = "fp64"
         self.compute_gauge_values = default_compute_gauge_values
         self._cfl_dev = None
+        self._pinned = None
         self._scratch = None
         if riemann_solver is not None:
             self.rp = riemann_solver
@@ -131,6 +132,21 @@ class Solver(object):
         if self._cfl_dev is None:
             self._cfl_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
         return self._cfl_dev
+
+    def _read_back(self, t):
+        """Host list of a small device tensor: one copy into pinned host memory on
+        the current stream, then a wait for that stream."""
+        import torch
+        if self._pinned is None:
+            self._pinned = {}
+        pinned = self._pinned
+        key = (t.numel(), t.dtype)
+        h = pinned.get(key)
+        if h is None:
+            h = pinned[key] = torch.empty(t.numel(), dtype=t.dtype, pin_memory=True)
+        h.copy_(t.reshape(-1), non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        return h.tolist()
 
     def _bc_codes(self):
         """Kernel boundary codes; raises for unset BCs as _bc_lower does (solver.py:474-478)."""
