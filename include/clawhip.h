@@ -42,7 +42,11 @@ enum {
   CLAWHIP_RP_EULER = 2,     /* euler_4wave_2D           */
   CLAWHIP_RP_SHALLOW = 3,   /* shallow_roe_with_efix_2D */
   CLAWHIP_RP_ADVECTION = 4, /* advection_2D             */
-  CLAWHIP_RP_EULER3 = 5     /* euler_3D                 */
+  CLAWHIP_RP_EULER3 = 5,    /* euler_3D                 */
+  CLAWHIP_RP_VC_ADVECTION = 6 /* variable-coefficient advection of rp->meqn (1..4) fields,
+                               * edge velocities in 2 aux fields
+                               * (examples/advection_reaction_2d/rpn2_vc_advection_vector.f90);
+                               * dimension-split 2D steps only */
 };
 
 /* Boundary codes: pyclaw.BC (pyclaw/solver.py:13-19) plus GIVEN. */
@@ -57,6 +61,7 @@ typedef struct {
   double gamma;             /* Euler */
   double grav;              /* shallow water */
   double u, v;              /* advection */
+  int meqn;                 /* CLAWHIP_RP_VC_ADVECTION: number of fields */
 } clawhip_rp;
 
 typedef struct {
@@ -125,13 +130,19 @@ int clawhip_step2_unsplit_run(const clawhip_rp *rp, int dtype, const clawhip_geo
  * step2ds.f90:96).  method[2] must be -1.
  *   work:      device buffer of num_eqn*(mx+2*num_ghost)*(my+2*num_ghost) elements
  *              of the state's type (the intermediate qbc between the sweeps);
- *   comp_work: the same for the Kahan compensation (CLAWHIP_F32K only). */
+ *   comp_work: the same for the Kahan compensation (CLAWHIP_F32K only);
+ *   aux:       the solver's aux fields (state.aux; required exactly when the
+ *              solver has them, method[6] = their number), planar like q in the
+ *              state's type, geometry gaux; aux_bc_lower/upper: the
+ *              solver.aux_bc codes (a wall mirrors without negating). */
 int clawhip_step2_split(const clawhip_rp *rp, int dtype, const void *q_in,
                         const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
                         const void *comp_in, void *comp_out, void *work, void *comp_work,
                         const int *bc_lower, const int *bc_upper, const int *reflect,
                         int num_ghost, const int *method, const int *mthlim, double dx,
-                        double dy, double dt, double *cfl_dev, void *stream);
+                        double dy, double dt, const void *aux, const clawhip_geom *gaux,
+                        const int *aux_bc_lower, const int *aux_bc_upper, double *cfl_dev,
+                        void *stream);
 
 /* One dimension-split 3D step: ClawSolver3D.step_hyperbolic with
  * dimensional_split=True (pyclaw/classic/solver.py:676-692), i.e. fmod.step3ds
@@ -223,11 +234,13 @@ int clawhip_ensemble_run(const clawhip_rp *rp, int dtype, const void *table_dev,
  * every cell of q_out is written: interior columns of all rows 1-ng..my+ng
  * updated, ghost columns copied.  ids 2: y-sweep in place (q_in == q_out),
  * columns 1-ng..mx+ng swept, interior cells updated.  *cfl_dev = max(*cfl_dev,
- * CFL of the sweep). */
+ * CFL of the sweep).  aux: the caller-filled auxbc (ghosts stored, as q), or NULL
+ * for solvers without aux fields. */
 int clawhip_step2_sweep(const clawhip_rp *rp, int dtype, int ids, const void *q_in,
                         const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
                         int num_ghost, const int *method, const int *mthlim, double dx, double dy,
-                        double dt, double *cfl_dev, void *stream);
+                        double dt, const void *aux, const clawhip_geom *gaux, double *cfl_dev,
+                        void *stream);
 
 /* One sweep of step3ds (classic/step3ds.f90:2-378 with flux3.f90; called at
  * classic/solver.py:680-690).  idir 1 out of place (every cell written),

@@ -23,7 +23,8 @@ class OracleRun:
                  dimensional_split=True, transverse=1, limiters=1, order=2,
                  cfl_desired=0.9, cfl_max=1.0, dt_initial=0.1, dt_max=1e99,
                  max_steps=10000, dt_variable=True, time_integrator="SSP104",
-                 char_decomp=0, num_ghost=None):
+                 char_decomp=0, num_ghost=None, aux=None, aux_bc_lower=None,
+                 aux_bc_upper=None, before_step=None, step_source=None, source_split=1):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -53,6 +54,11 @@ class OracleRun:
         self.dq_dt = None
         self.numsteps = 0
         self.dt_history = []
+        # aux fields (interior, (maux, mx, my)), their BCs (no negation), and the
+        # user callbacks of Solver.evolve_to_time / ClawSolver.step
+        self.aux = None if aux is None else np.asfortranarray(aux, dtype=np.float64).copy(order="F")
+        self.aux_bc_lower, self.aux_bc_upper = aux_bc_lower, aux_bc_upper
+        self.before_step, self.step_source, self.source_split = before_step, step_source, source_split
 
     # ---- boundary conditions: Solver._apply_bcs (solver.py:351-437)
     def _apply_bcs(self, q):
@@ -90,20 +96,34 @@ class OracleRun:
             if tend - t - self.dt < 1.e-14 * t:
                 self.dt = tend - t
 
+    def _auxbc(self):
+        ng = self.ng
+        abc = np.zeros((self.aux.shape[0],) + tuple(n + 2 * ng for n in self.aux.shape[1:]),
+                       order="F")
+        abc[(slice(None),) + (slice(ng, -ng),) * 2] = self.aux
+        O.bc2(abc, ng, self.aux_bc_lower, self.aux_bc_upper, reflect=(-1, -1))
+        return abc
+
     # ---- classic step (classic/solver.py:96-146, 498-549, 648-706)
     def _step_classic(self, t, tstart, tend, take_one_step):
         self._get_dt(t, tstart, tend, take_one_step)
         self.cfl = 0.0
+        if self.source_split == 2 and self.step_source is not None:
+            self.step_source(self, self.dt / 2.0)
         qbc = self._apply_bcs(self.q)
         qold = qbc.copy(order="F")
         ng, dt = self.ng, self.dt
         if self.ndim == 2:
             dx, dy = self.delta
             if self.split:
+                maux = 0 if self.aux is None else self.aux.shape[0]
+                abc = None if self.aux is None else self._auxbc()
                 meth = O.method_array(order=self.order, dimensional_split=True,
-                                      dt_variable=self.dt_variable)
-                c1 = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=1)
-                c2 = O.step2(self.rp, qbc, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=2)
+                                      dt_variable=self.dt_variable, num_aux=maux)
+                c1 = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=1,
+                             aux=abc)
+                c2 = O.step2(self.rp, qbc, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=2,
+                             aux=abc)
                 cfl = max(c1, c2)
             else:
                 meth = O.method_array(order=self.order, transverse=self.transverse,
@@ -119,6 +139,10 @@ class OracleRun:
             cfl = max(c1, c2, c3)
         self.cfl = max(self.cfl, cfl)
         self.q = self._interior(qbc)
+        if self.cfl >= self.cfl_max:
+            return
+        if self.step_source is not None:
+            self.step_source(self, self.dt / 2.0 if self.source_split == 2 else self.dt)
 
     # ---- SharpClaw dq (sharpclaw/solver.py:848-898), returns dt*L(q) interior
     def _dq(self, q):
@@ -168,6 +192,8 @@ class OracleRun:
         for n in range(max_steps):
             q_backup = self.q.copy(order="F")
             told = self.t
+            if self.before_step is not None:
+                self.before_step(self)
             if self.solver == "classic":
                 self._step_classic(self.t, tstart, tend, take_one_step)
             else:

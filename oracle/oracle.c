@@ -31,7 +31,7 @@ int orc_rp_meqn(int kind) {
     case ORC_RP_ADVECTION: return 1;
     case ORC_RP_EULER3: return 5;
   }
-  return 0;
+  return 0;   /* ORC_RP_VC_ADVECTION: any (orc_rp.meqn) */
 }
 
 int orc_rp_mwaves(int kind) {
@@ -41,6 +41,7 @@ int orc_rp_mwaves(int kind) {
     case ORC_RP_SHALLOW: return 3;
     case ORC_RP_ADVECTION: return 1;
     case ORC_RP_EULER3: return 3;
+    case ORC_RP_VC_ADVECTION: return 1;
   }
   return 0;
 }
@@ -433,6 +434,32 @@ void orc_rpn(const orc_rp *p, int ixy, const double *qL, const double *qR,
   }
 }
 
+/* rpn2_vc_advection_vector.f90:53-60: one wave qR - qL with speed
+ * auxl(ixy, i) (the right cell's edge velocity); its flux difference goes
+ * downwind: amdq = dmin1(s, 0) * wave, apdq = dmax1(s, 0) * wave. */
+static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                             const double *auxR, double *wave, double *s, double *amdq,
+                             double *apdq) {
+  const double u = auxR[ixy - 1];
+  const double um = u < 0.0 ? u : 0.0, up = u > 0.0 ? u : 0.0;
+  s[0] = u;
+  for (int m = 0; m < p->meqn; m++) {
+    wave[m] = qR[m] - qL[m];
+    amdq[m] = um * wave[m];
+    apdq[m] = up * wave[m];
+  }
+}
+
+void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                 const double *auxL, const double *auxR, double *wave, double *s,
+                 double *amdq, double *apdq) {
+  (void)auxL;
+  if (p->kind == ORC_RP_VC_ADVECTION)
+    rpn_vc_advection(p, ixy, qL, qR, auxR, wave, s, amdq, apdq);
+  else
+    orc_rpn(p, ixy, qL, qR, wave, s, amdq, apdq);
+}
+
 void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
              const double *asdq, double *bmasdq, double *bpasdq) {
   (void)imp;
@@ -554,6 +581,8 @@ static double philim(double a, double b, int meth) {
 typedef struct {
   int meqn, mwaves, ng, len;
   double *wave, *s, *amdq, *apdq, *cqxx, *bmasdq, *bpasdq, *qadd, *fadd, *gadd;
+  int maux;
+  double *aux2;   /* aux(:, slice), maux x len (the reference's aux2) */
 } slice_ws;
 
 static void ws_alloc(slice_ws *w, int meqn, int mwaves, int ng, int maxm) {
@@ -569,11 +598,14 @@ static void ws_alloc(slice_ws *w, int meqn, int mwaves, int ng, int maxm) {
   w->qadd = (double *)calloc(L * meqn, sizeof(double));
   w->fadd = (double *)calloc(L * meqn, sizeof(double));
   w->gadd = (double *)calloc(L * meqn * 2, sizeof(double));
+  w->maux = 0;
+  w->aux2 = NULL;
 }
 
 static void ws_free(slice_ws *w) {
   free(w->wave); free(w->s); free(w->amdq); free(w->apdq); free(w->cqxx);
   free(w->bmasdq); free(w->bpasdq); free(w->qadd); free(w->fadd); free(w->gadd);
+  free(w->aux2);
 }
 
 #define IX(i) ((i) + ng - 1)
@@ -609,9 +641,15 @@ static void limiter(slice_ws *w, int mx, const int *mthlim) {
 /* Riemann solves along a slice, i = 2-ng .. mx+ng (reference rp ABI loop). */
 static void slice_rpn(slice_ws *w, int ixy, int mx, const double *q1d, const orc_rp *p) {
   const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng;
-  for (int i = 2 - ng; i <= mx + ng; i++)
-    orc_rpn(p, ixy, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &WV(0, 0, i), &SV(0, i),
-            &V1(w->amdq, 0, i), &V1(w->apdq, 0, i));
+  for (int i = 2 - ng; i <= mx + ng; i++) {
+    if (w->maux > 0)   /* flux2.f90:101: auxl = auxr = aux2 */
+      orc_rpn_aux(p, ixy, &V1(q1d, 0, i - 1), &V1(q1d, 0, i),
+                  &w->aux2[(long)w->maux * IX(i - 1)], &w->aux2[(long)w->maux * IX(i)],
+                  &WV(0, 0, i), &SV(0, i), &V1(w->amdq, 0, i), &V1(w->apdq, 0, i));
+    else
+      orc_rpn(p, ixy, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &WV(0, 0, i), &SV(0, i),
+              &V1(w->amdq, 0, i), &V1(w->apdq, 0, i));
+  }
 }
 
 /* classic/flux2.f90:2-207.  dtdx1d is constant along the slice (no capacity). */
@@ -771,10 +809,16 @@ double orc_step2(int meqn, int mwaves, int ng, int mx, int my, const double *qol
  * including ghost rows; y-sweeps cover every column including ghost columns. */
 double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *qold,
                    double *qnew, double dx, double dy, double dt, const int *method,
-                   const int *mthlim, int ids, int use_fwave, const orc_rp *p) {
+                   const int *mthlim, int ids, int use_fwave, const orc_rp *p, int maux,
+                   const double *aux) {
   int maxm = mx > my ? mx : my;
   slice_ws W, *w = &W;
   ws_alloc(w, meqn, mwaves, ng, maxm);
+  if (maux > 0 && aux) {
+    w->maux = maux;
+    w->aux2 = (double *)calloc((long)maux * w->len, sizeof(double));
+  }
+#define AUX2(ma, i, j) aux[(ma) + (long)maux * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
   double *q1d = (double *)calloc((long)(maxm + 2 * ng) * meqn, sizeof(double));
   double cfl = 0.0;
   double dtdx = dt / dx, dtdy = dt / dy;
@@ -782,6 +826,9 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
     for (int j = 1 - ng; j <= my + ng; j++) {
       for (int i = 1 - ng; i <= mx + ng; i++)
         for (int m = 0; m < meqn; m++) V1(q1d, m, i) = Q2(qold, m, i, j);
+      if (w->maux > 0)   /* step2ds.f90:110-114: aux2 = aux(:, :, j) */
+        for (int i = 1 - ng; i <= mx + ng; i++)
+          for (int ma = 0; ma < maux; ma++) w->aux2[ma + (long)maux * IX(i)] = AUX2(ma, i, j);
       double cfl1d = flux2(w, 1, mx, q1d, dtdx, method, mthlim, use_fwave, p);
       cfl = dmax(cfl, cfl1d);
       for (int i = 1; i <= mx; i++)
@@ -793,6 +840,9 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
     for (int i = 1 - ng; i <= mx + ng; i++) {
       for (int j = 1 - ng; j <= my + ng; j++)
         for (int m = 0; m < meqn; m++) V1(q1d, m, j) = Q2(qold, m, i, j);
+      if (w->maux > 0)   /* step2ds.f90:195-199: aux2 = aux(:, i, :) */
+        for (int j = 1 - ng; j <= my + ng; j++)
+          for (int ma = 0; ma < maux; ma++) w->aux2[ma + (long)maux * IX(j)] = AUX2(ma, i, j);
       double cfl1d = flux2(w, 2, my, q1d, dtdy, method, mthlim, use_fwave, p);
       cfl = dmax(cfl, cfl1d);
       for (int j = 1; j <= my; j++)
@@ -805,6 +855,7 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
   ws_free(w);
   return cfl;
 }
+#undef AUX2
 #undef Q2
 
 #define Q3(a, m, i, j, k)                                                              \

@@ -23,7 +23,9 @@ enum {
   ORC_RP_EULER = 2,      /* euler_4wave_2D:          meqn 4, mwaves 4 */
   ORC_RP_SHALLOW = 3,    /* shallow_roe_with_efix_2D: meqn 3, mwaves 3 */
   ORC_RP_ADVECTION = 4,  /* advection_2D:            meqn 1, mwaves 1 */
-  ORC_RP_EULER3 = 5      /* euler_3D:                meqn 5, mwaves 3 */
+  ORC_RP_EULER3 = 5,     /* euler_3D:                meqn 5, mwaves 3 */
+  ORC_RP_VC_ADVECTION = 6 /* examples/advection_reaction_2d/rpn2_vc_advection_vector.f90:
+                           * meqn = orc_rp.meqn, mwaves 1, 2 aux fields (edge u, v) */
 };
 
 /* Problem parameters: the reference passes these through the Fortran /cparam/
@@ -35,6 +37,7 @@ typedef struct {
   double gamma;                 /* Euler */
   double grav;                  /* shallow water */
   double u, v;                  /* advection velocities */
+  int meqn;                     /* ORC_RP_VC_ADVECTION: number of fields */
 } orc_rp;
 
 /* Boundary condition codes: pyclaw.BC (solver.py:13-19). */
@@ -47,6 +50,12 @@ void orc_rpn(const orc_rp *p, int ixy, const double *qL, const double *qR,
              double *wave, double *s, double *amdq, double *apdq);
 void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
              const double *asdq, double *bmasdq, double *bpasdq);
+/* The normal solve with the aux values of the left and right cells (auxL =
+ * auxr(:,i-1), auxR = auxl(:,i) in the reference ABI); solvers without aux
+ * fields ignore them. */
+void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                 const double *auxL, const double *auxR, double *wave, double *s,
+                 double *amdq, double *apdq);
 int orc_rp_meqn(int kind);
 int orc_rp_mwaves(int kind);
 
@@ -66,10 +75,11 @@ void orc_bc3(double *qbc, int meqn, int mx, int my, int mz, int ng,
 double orc_step2(int meqn, int mwaves, int ng, int mx, int my,
                  const double *qold, double *qnew, double dx, double dy, double dt,
                  const int *method, const int *mthlim, int use_fwave, const orc_rp *p);
+/* aux: NULL (maux = 0) or aux(maux, 1-ng:mx+ng, 1-ng:my+ng), ghosts filled. */
 double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my,
                    const double *qold, double *qnew, double dx, double dy, double dt,
                    const int *method, const int *mthlim, int ids, int use_fwave,
-                   const orc_rp *p);
+                   const orc_rp *p, int maux, const double *aux);
 double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz,
                    const double *qold, double *qnew, double dx, double dy, double dz,
                    double dt, const int *method, const int *mthlim, int idir,

@@ -22,7 +22,8 @@ ACOUSTICS, EULER, SHALLOW, ADVECTION, EULER3 = 1, 2, 3, 4, 5
 CUSTOM, EXTRAP, PERIODIC, WALL = 0, 1, 2, 3
 
 MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5}
-MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3}
+MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1}
+VC_ADVECTION = 6
 
 
 class RP(ctypes.Structure):
@@ -31,12 +32,12 @@ class RP(ctypes.Structure):
                 ("rho", ctypes.c_double), ("bulk", ctypes.c_double),
                 ("cc", ctypes.c_double), ("zz", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("grav", ctypes.c_double),
-                ("u", ctypes.c_double), ("v", ctypes.c_double)]
+                ("u", ctypes.c_double), ("v", ctypes.c_double), ("meqn", ctypes.c_int)]
 
 
-def make_rp(kind, efix=1, rho=1.0, bulk=4.0, gamma=1.4, grav=1.0, u=0.5, v=1.0):
+def make_rp(kind, efix=1, rho=1.0, bulk=4.0, gamma=1.4, grav=1.0, u=0.5, v=1.0, meqn=0):
     cc = float(np.sqrt(bulk / rho))
-    return RP(kind, efix, rho, bulk, cc, rho * cc, gamma, grav, u, v)
+    return RP(kind, efix, rho, bulk, cc, rho * cc, gamma, grav, u, v, meqn)
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -97,9 +98,9 @@ def ref_sharp():
     return _ref_sharp
 
 
-def method_array(order=2, transverse=2, dimensional_split=False, dt_variable=True):
-    """ClawSolver._set_method (classic/solver.py:173-193) for 2D/3D, no aux."""
-    m = [int(dt_variable), order, -1 if dimensional_split else transverse, 0, 0, 0, 0]
+def method_array(order=2, transverse=2, dimensional_split=False, dt_variable=True, num_aux=0):
+    """ClawSolver._set_method (classic/solver.py:173-193) for 2D/3D, no capacity."""
+    m = [int(dt_variable), order, -1 if dimensional_split else transverse, 0, 0, 0, int(num_aux)]
     return m
 
 
@@ -133,8 +134,9 @@ def interior(qbc, ng):
 
 
 # ---------------------------------------------------------------- classic
-def step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0, use_fwave=False):
-    """ids=0: unsplit step2; ids=1/2: step2ds sweep.  qnew updated in place."""
+def step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0, use_fwave=False, aux=None):
+    """ids=0: unsplit step2; ids=1/2: step2ds sweep.  qnew updated in place.
+    aux: aux(maux, 1-ng:mx+ng, 1-ng:my+ng) with ghosts filled (split sweeps)."""
     meqn, nx, ny = qold.shape
     mx, my = nx - 2 * ng, ny - 2 * ng
     mw = MWAVES[rp.kind]
@@ -142,9 +144,11 @@ def step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0, use_fwave=False
         return lib().orc_step2(meqn, mw, ng, mx, my, _dp(qold), _dp(qnew), ctypes.c_double(dx),
                                ctypes.c_double(dy), ctypes.c_double(dt), _ip(method),
                                _ip(mthlim), int(use_fwave), ctypes.byref(rp))
+    maux = 0 if aux is None else aux.shape[0]
     return lib().orc_step2ds(meqn, mw, ng, mx, my, _dp(qold), _dp(qnew), ctypes.c_double(dx),
                              ctypes.c_double(dy), ctypes.c_double(dt), _ip(method), _ip(mthlim),
-                             int(ids), int(use_fwave), ctypes.byref(rp))
+                             int(ids), int(use_fwave), ctypes.byref(rp), int(maux),
+                             _dp(aux) if aux is not None else None)
 
 
 def step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir, use_fwave=False):
@@ -213,6 +217,28 @@ def ref_step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0):
                                    nx - 2 * ng, ny - 2 * ng, _dp(qold), _dp(qnew),
                                    ctypes.c_double(dx), ctypes.c_double(dy),
                                    ctypes.c_double(dt), _ip(method), _ip(mthlim))
+
+
+_ref_vcadv = None
+REF_VCADV_PATH = os.path.join(os.path.dirname(REF_CLASSIC_PATH), "libref_vcadv.so")
+
+
+def ref_vcadv_rpn2():
+    """Address of the advection_reaction_2d example's rpn2 (built from its source)."""
+    global _ref_vcadv
+    if _ref_vcadv is None:
+        _ref_vcadv = ctypes.CDLL(REF_VCADV_PATH)
+    return ctypes.cast(_ref_vcadv.rpn2_, ctypes.c_void_p)
+
+
+def ref_step2ds_aux(rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, ids, mwaves=1):
+    """The reference's step2ds sweep with aux fields and the given rpn2 address."""
+    meqn, nx, ny = qold.shape
+    f = ref_classic().ref_step2ds_aux
+    f.restype = ctypes.c_double
+    return f(rpn2, int(ids), meqn, int(mwaves), aux.shape[0], ng, nx - 2 * ng, ny - 2 * ng,
+             _dp(qold), _dp(qnew), _dp(aux), ctypes.c_double(dx), ctypes.c_double(dy),
+             ctypes.c_double(dt), _ip(method), _ip(mthlim))
 
 
 def ref_step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir):

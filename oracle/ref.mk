@@ -18,7 +18,17 @@ SHARP = $(REF)/sharpclaw/ClawParams.f90 $(REF)/sharpclaw/workspace.f90 \
         $(REF)/sharpclaw/weno.f90 $(REF)/sharpclaw/reconstruct.f90 \
         $(REF)/sharpclaw/flux1.f90 $(REF)/sharpclaw/flux2.f90
 
-all: $(OUT)/libref_classic.so $(OUT)/libref_sharp.so
+EX ?= /root/reference/examples
+VCADV = $(EX)/advection_reaction_2d/rpn2_vc_advection_vector.f90
+
+all: $(OUT)/libref_classic.so $(OUT)/libref_sharp.so $(OUT)/libref_vcadv.so
+
+# the advection_reaction_2d example's own Riemann solver (rpn2 only), in a
+# library of its own: its symbol is rpn2, and the step takes it as an argument
+$(OUT)/libref_vcadv.so: $(VCADV)
+	mkdir -p $(OUT)/vcadv
+	cd $(OUT)/vcadv && $(FC) $(FFLAGS) -c $(VCADV)
+	$(FC) -shared -fPIC -o $@ $(OUT)/vcadv/*.o
 
 $(OUT)/libref_classic.so: $(CLASSIC) $(HERE)ref_shim.c $(HERE)oracle.c $(HERE)oracle.h
 	mkdir -p $(OUT)/classic

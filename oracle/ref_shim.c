@@ -127,6 +127,29 @@ double ref_step2(const orc_rp *p, int ids, int meqn, int mwaves, int ng, int mx,
   return cfl;
 }
 
+/* step2ds (ids 1, 2) with aux fields and a caller-supplied rpn2 -- the
+ * reference example's own Riemann solver, compiled from its source by ref.mk
+ * (examples/advection_reaction_2d/rpn2_vc_advection_vector.f90).
+ * aux(maux, 1-ng:mx+ng, 1-ng:my+ng), ghosts filled. */
+double ref_step2ds_aux(rp_fn rpn2, int ids, int meqn, int mwaves, int maux, int ng, int mx,
+                       int my, const double *qold, double *qnew, const double *aux, double dx,
+                       double dy, double dt, const int *method, const int *mthlim) {
+  int maxm = mx > my ? mx : my, use_fwave = 0;
+  long L = maxm + 2 * ng;
+  int mwork = (int)(L * (5 * meqn + mwaves + meqn * mwaves));
+  double *buf = (double *)calloc(L * (6 * meqn + 2 + 3 * maux) + mwork + 16, sizeof(double));
+  double *qadd = buf, *fadd = qadd + L * meqn, *gadd = fadd + L * meqn,
+         *q1d = gadd + 2 * L * meqn, *dtdx1d = q1d + L * meqn, *dtdy1d = dtdx1d + L,
+         *aux1 = dtdy1d + L, *aux2 = aux1 + L * maux, *aux3 = aux2 + L * maux,
+         *work = aux3 + L * maux;
+  double cfl = 0.0;
+  step2ds_(&maxm, &meqn, &mwaves, &maux, &ng, &mx, &my, qold, qnew, aux, &dx, &dy, &dt,
+           method, mthlim, &cfl, qadd, fadd, gadd, q1d, dtdx1d, dtdy1d, aux1, aux2, aux3,
+           work, &mwork, &ids, &use_fwave, rpn2, (rp_fn)shim_rpt2_);
+  free(buf);
+  return cfl;
+}
+
 double ref_step3ds(const orc_rp *p, int idir, int meqn, int mwaves, int ng, int mx, int my,
                    int mz, const double *qold, double *qnew, double dx, double dy, double dz,
                    double dt, const int *method, const int *mthlim) {

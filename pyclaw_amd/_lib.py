@@ -27,7 +27,7 @@ class RP(ctypes.Structure):
                 ("rho", ctypes.c_double), ("bulk", ctypes.c_double),
                 ("cc", ctypes.c_double), ("zz", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("grav", ctypes.c_double),
-                ("u", ctypes.c_double), ("v", ctypes.c_double)]
+                ("u", ctypes.c_double), ("v", ctypes.c_double), ("meqn", ctypes.c_int)]
 
 
 class StepCtl(ctypes.Structure):
@@ -85,7 +85,7 @@ _SIGS = {
                               _D, _D, _D, _P, _P],
     "clawhip_step2_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
-                            _D, _D, _D, _P, _P],
+                            _D, _D, _D, _P, ctypes.POINTER(Geom), _IP, _IP, _P, _P],
     "clawhip_step3_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                             _D, _D, _D, _D, _P, _P],
@@ -96,7 +96,8 @@ _SIGS = {
     "clawhip_step2_unsplit_run": [ctypes.POINTER(RP), _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                                   _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _I, _P],
     "clawhip_step2_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
-                            ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _P, _P],
+                            ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _P,
+                            ctypes.POINTER(Geom), _P, _P],
     "clawhip_step3_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _D, _P, _P],
     "clawhip_sharp2_flux": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,

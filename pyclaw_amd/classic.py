@@ -80,12 +80,18 @@ class ClawSolver(Solver):
         if self.fwave:
             raise NotImplementedError("f-wave Riemann solvers are not available on the device")
         state = solution.states[0]
-        if state.num_aux > 0 or state.index_capa >= 0:
-            raise NotImplementedError("aux arrays and capacity functions are not supported on the device")
-        self._set_method(state)
         if self.rp is None or not hasattr(self.rp, "kind"):
             raise Exception("the solver needs one of the pyclaw_amd.riemann modules")
-        self._rp_struct = K.make_rp(self.rp.kind, state.problem_data)
+        if state.index_capa >= 0:
+            raise NotImplementedError("capacity functions are not supported on the device")
+        rp_aux = getattr(self.rp, "num_aux", 0)
+        if state.num_aux != rp_aux:
+            raise NotImplementedError(
+                "the Riemann solver takes %d aux fields, the state has %d" % (rp_aux, state.num_aux))
+        if rp_aux and not (self.num_dim == 2 and self.dimensional_split):
+            raise NotImplementedError("aux fields are supported by the dimension-split 2D step")
+        self._set_method(state)
+        self._rp_struct = K.make_rp(self.rp.kind, state.problem_data, meqn=state.num_eqn)
         self._is_set_up = True
 
 
@@ -139,10 +145,15 @@ class ClawSolver2D(ClawSolver):
             src, gin = self._custom_bc_input(state, lo, hi)
             lo, hi = [4, 4], [4, 4]
         if self.dimensional_split:
+            aux = alo = ahi = None
+            if state.num_aux > 0:
+                aux = state.device_aux(self._dtype_code)
+                alo, ahi = self._aux_bc_codes()
             self._work = K.step2_split(self._rp_struct, src[0], out[0], cfl_dev, lo, hi,
                                        self.reflect_index, self._method, self._mthlim, dx, dy,
                                        self.dt, comp_in=src[1], comp_out=out[1], gin=gin,
-                                       num_ghost=self.num_ghost, work=self._work)
+                                       num_ghost=self.num_ghost, work=self._work, aux=aux,
+                                       aux_bc_lower=alo, aux_bc_upper=ahi)
         else:
             K.step2_unsplit(self._rp_struct, src[0], out[0], cfl_dev, lo, hi, self.reflect_index,
                             self._method, self._mthlim, dx, dy, self.dt, comp_in=src[1],

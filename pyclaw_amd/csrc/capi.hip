@@ -81,7 +81,7 @@ static int check_common(const clawhip_rp* rp, int dtype, const void* q_in, const
                         const void* comp_in, const void* comp_out, double* cfl_dev,
                         const int* bc_lower, const int* bc_upper, const int* reflect,
                         const int* method, const int* mthlim, int ndim, int num_ghost,
-                        const clawhip_geom* gin, const clawhip_geom* gout) {
+                        const clawhip_geom* gin, const clawhip_geom* gout, int num_aux = 0) {
   if (!rp || !q_in || !q_out || !cfl_dev || !bc_lower || !bc_upper || !reflect || !method ||
       !mthlim)
     return CLAWHIP_E_ARG;
@@ -94,7 +94,7 @@ static int check_common(const clawhip_rp* rp, int dtype, const void* q_in, const
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
-  if (method[5] != 0 || method[6] != 0) return CLAWHIP_E_METHOD;
+  if (method[5] != 0 || method[6] != num_aux) return CLAWHIP_E_METHOD;
   for (int d = 0; d < ndim; d++) {
     int lo = bc_lower[d], hi = bc_upper[d];
     if (lo < 1 || lo > 4 || hi < 1 || hi > 4) return CLAWHIP_E_ARG;
@@ -104,21 +104,52 @@ static int check_common(const clawhip_rp* rp, int dtype, const void* q_in, const
   return 0;
 }
 
+// aux: NULL, or the solver's aux fields (planar like q, geometry gaux with the
+// interior of q; stored ghosts only with CLAWHIP_BC_GIVEN aux codes).
+static int aux_in(const void* aux, const clawhip_geom* gaux, const int* alo, const int* ahi,
+                  const clawhip_geom* gq, int num_ghost, AuxIn& out) {
+  if (!aux) return 0;
+  if (!gaux || !alo || !ahi || !geom_ok(gaux, 2)) return CLAWHIP_E_ARG;
+  if (gaux->mx != gq->mx || gaux->my != gq->my) return CLAWHIP_E_ARG;
+  for (int d = 0; d < 2; d++) {
+    if (alo[d] < 1 || alo[d] > 4 || ahi[d] < 1 || ahi[d] > 4) return CLAWHIP_E_ARG;
+    if ((alo[d] == CLAWHIP_BC_GIVEN || ahi[d] == CLAWHIP_BC_GIVEN) && gaux->ghosts < num_ghost)
+      return CLAWHIP_E_ARG;
+    out.lo[d] = alo[d];
+    out.hi[d] = ahi[d];
+  }
+  out.p = aux;
+  out.g = to_geom(gaux);
+  out.g.ss = 0;
+  return 0;
+}
+
+static int rp_num_aux(const clawhip_rp* rp) {
+  return rp->kind == CLAWHIP_RP_VC_ADVECTION ? 2 : 0;
+}
+
 int clawhip_step2_split(const clawhip_rp* rp, int dtype, const void* q_in,
                         const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
                         const void* comp_in, void* comp_out, void* work, void* comp_work,
                         const int* bc_lower, const int* bc_upper, const int* reflect,
                         int num_ghost, const int* method, const int* mthlim, double dx,
-                        double dy, double dt, double* cfl_dev, void* stream) {
+                        double dy, double dt, const void* aux, const clawhip_geom* gaux,
+                        const int* aux_bc_lower, const int* aux_bc_upper, double* cfl_dev,
+                        void* stream) {
+  if (!rp) return CLAWHIP_E_ARG;
   int rc = check_common(rp, dtype, q_in, q_out, comp_in, comp_out, cfl_dev, bc_lower, bc_upper,
-                        reflect, method, mthlim, 2, num_ghost, gin, gout);
+                        reflect, method, mthlim, 2, num_ghost, gin, gout, rp_num_aux(rp));
   if (rc) return rc;
   if (!work || (dtype == CLAWHIP_F32K && !comp_work)) return CLAWHIP_E_ARG;
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
   if (method[2] != -1) return CLAWHIP_E_METHOD;
+  AuxIn ax;
+  rc = aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, gout, num_ghost, ax);
+  if (rc) return rc;
+  if ((rp_num_aux(rp) > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   return launch_split2(*rp, dtype, q_in, to_geom(gin), q_out, to_geom(gout), comp_in, comp_out,
                        work, comp_work, bc_lower, bc_upper, reflect, num_ghost, method, mthlim,
-                       dx, dy, dt, (unsigned long long*)cfl_dev, (hipStream_t)stream);
+                       dx, dy, dt, ax, (unsigned long long*)cfl_dev, (hipStream_t)stream);
 }
 
 int clawhip_step3_split(const clawhip_rp* rp, int dtype, const void* q_in,
@@ -247,7 +278,8 @@ int clawhip_ensemble_run(const clawhip_rp* rp, int dtype, const void* table_dev,
 // ---- sweeps and operator on ghost-storing arrays (the f2py entry points) ----
 static int check_sweep(const clawhip_rp* rp, int dtype, int ndim, int dir, const void* q_in,
                        const clawhip_geom* gin, const void* q_out, const clawhip_geom* gout,
-                       int num_ghost, const int* method, const int* mthlim, double* cfl_dev) {
+                       int num_ghost, const int* method, const int* mthlim, double* cfl_dev,
+                       int num_aux = 0) {
   if (!rp || !q_in || !q_out || !gin || !gout || !method || !mthlim || !cfl_dev)
     return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64) return CLAWHIP_E_ARG;
@@ -263,23 +295,30 @@ static int check_sweep(const clawhip_rp* rp, int dtype, int ndim, int dir, const
                   gin->slab_stride != gout->slab_stride))
     return CLAWHIP_E_ARG;
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
-  if (method[2] != -1 || method[5] != 0 || method[6] != 0) return CLAWHIP_E_METHOD;
+  if (method[2] != -1 || method[5] != 0 || method[6] != num_aux) return CLAWHIP_E_METHOD;
   return 0;
 }
 
 int clawhip_step2_sweep(const clawhip_rp* rp, int dtype, int ids, const void* q_in,
                         const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
                         int num_ghost, const int* method, const int* mthlim, double dx, double dy,
-                        double dt, double* cfl_dev, void* stream) {
+                        double dt, const void* aux, const clawhip_geom* gaux, double* cfl_dev,
+                        void* stream) {
+  if (!rp) return CLAWHIP_E_ARG;
   int rc = check_sweep(rp, dtype, 2, ids, q_in, gin, q_out, gout, num_ghost, method, mthlim,
-                       cfl_dev);
+                       cfl_dev, rp_num_aux(rp));
   if (rc) return rc;
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
+  const int given[2] = {CLAWHIP_BC_GIVEN, CLAWHIP_BC_GIVEN};
+  AuxIn ax;
+  rc = aux_in(aux, gaux, given, given, gout, num_ghost, ax);
+  if (rc) return rc;
+  if ((rp_num_aux(rp) > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   Geom a = to_geom(gin), b = to_geom(gout);
   a.ss = 0;   // 2D: the sweep kernels add (k-1+g)*ss with k = 1
   b.ss = 0;
   return launch_sweep2(*rp, ids, q_in, a, q_out, b, num_ghost, method, mthlim,
-                       ids == 1 ? dt / dx : dt / dy, (unsigned long long*)cfl_dev,
+                       ids == 1 ? dt / dx : dt / dy, ax, (unsigned long long*)cfl_dev,
                        (hipStream_t)stream);
 }
 

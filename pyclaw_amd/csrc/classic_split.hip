@@ -69,6 +69,11 @@ struct SweepArgs {
   const T* comp_in;
   T* comp_out;
   unsigned long long* cfl_bits;
+  // aux fields (solvers with RS::maux > 0; 2D): compact or with stored ghosts,
+  // boundary values by bcaux (aux BCs never negate)
+  const T* aux;
+  Geom gaux;
+  BCSpec bcaux;
 };
 
 // ---------------------------------------------------------------- x-sweeps
@@ -92,6 +97,13 @@ __global__ void __launch_bounds__(256) k_xsweep(const SweepArgs<T> a) {
   T Q[ME];
   if (DIM == 2) load_cell2<ME>(a.qin, a.gin, a.bc, ng, c, j, Q);
   else load_cell3<ME>(a.qin, a.gin, a.bc, ng, c, j, k, Q);
+  constexpr int MA = RS::maux > 0 ? RS::maux : 1;
+  T A[MA], AL[MA];
+  if constexpr (RS::maux > 0) {
+    static_assert(DIM == 2, "aux fields: 2D sweeps");
+    load_cell2<MA>(a.aux, a.gaux, a.bcaux, ng, c, j, A);
+    from_left(A, AL);
+  }
   T upd[ME];
 #pragma unroll
   for (int m = 0; m < ME; m++) upd[m] = Q[m];
@@ -101,7 +113,7 @@ __global__ void __launch_bounds__(256) k_xsweep(const SweepArgs<T> a) {
     from_left(Q, QL);
     T W[MW][ME], s[MW], am[ME], ap[ME];
     typename RS::Roe roe;
-    RS::template rpn<1>(a.rp, QL, Q, W, s, am, ap, roe);
+    rpn_call<RS, 1>(a.rp, QL, Q, AL, A, W, s, am, ap, roe);
     T smax = T(0);
     if (lane >= 1 && c >= 1 && c <= mx + 1) {
 #pragma unroll
@@ -201,22 +213,34 @@ __global__ void __launch_bounds__(256) k_march(const SweepArgs<T> a) {
 #pragma unroll
     for (int m = 0; m < ME; m++) out[m] = ptr[(long long)m * a.gin.fs];
   };
+  constexpr int MA = RS::maux > 0 ? RS::maux : 1;
+  auto load_aux = [&](int s, T (&out)[MA]) {
+    if constexpr (RS::maux > 0) {
+      static_assert(AXIS == 2, "aux fields: 2D sweeps");
+      load_cell2<MA>(a.aux, a.gaux, a.bcaux, ng, ic, s, out);
+    } else {
+      (void)s; out[0] = T(0);
+    }
+  };
 
-  T Qm[ME], Q0[ME], Q1[ME];
+  T Qm[ME], Q0[ME], Q1[ME], Am[MA], A0[MA], A1[MA];
   load(j0 - 2, Qm);
   load(j0 - 1, Q0);
   load(j0, Q1);
+  load_aux(j0 - 2, Am);
+  load_aux(j0 - 1, A0);
+  load_aux(j0, A1);
   T Wt[MW][ME], st[MW], amt[ME], apt[ME], dotl[MW];
   Roe roet;
   {
     T Wp[MW][ME], sp[MW], amp[ME], app[ME];
     Roe roep;
-    RS::template rpn<IXY>(a.rp, Qm, Q0, Wp, sp, amp, app, roep);
+    rpn_call<RS, IXY>(a.rp, Qm, Q0, Am, A0, Wp, sp, amp, app, roep);
     if (j0 - 1 >= 1) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(sp[w]));
     }
-    RS::template rpn<IXY>(a.rp, Q0, Q1, Wt, st, amt, apt, roet);
+    rpn_call<RS, IXY>(a.rp, Q0, Q1, A0, A1, Wt, st, amt, apt, roet);
 #pragma unroll
     for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(st[w]));
     wave_dots<ME, MW>(Wp, Wt, dotl);
@@ -224,12 +248,15 @@ __global__ void __launch_bounds__(256) k_march(const SweepArgs<T> a) {
   T qold_prev[ME], qadd_p[ME], F_p[ME];
 #pragma unroll
   for (int m = 0; m < ME; m++) { Q0[m] = Q1[m]; qold_prev[m] = T(0); qadd_p[m] = T(0); F_p[m] = T(0); }
+#pragma unroll
+  for (int m = 0; m < MA; m++) A0[m] = A1[m];
 
   for (int t = j0; t <= jend; t++) {
     load(t + 1, Q1);
+    load_aux(t + 1, A1);
     T Wn[MW][ME], sn[MW], amn[ME], apn[ME], dotn[MW];
     Roe roen;
-    RS::template rpn<IXY>(a.rp, Q0, Q1, Wn, sn, amn, apn, roen);
+    rpn_call<RS, IXY>(a.rp, Q0, Q1, A0, A1, Wn, sn, amn, apn, roen);
     if (t + 1 <= m_ax + 1) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(sn[w]));
@@ -278,6 +305,8 @@ __global__ void __launch_bounds__(256) k_march(const SweepArgs<T> a) {
       Q0[m] = Q1[m];
     }
 #pragma unroll
+    for (int m = 0; m < MA; m++) A0[m] = A1[m];
+#pragma unroll
     for (int w = 0; w < MW; w++) {
       st[w] = sn[w];
       dotl[w] = dotn[w];
@@ -312,7 +341,21 @@ SweepArgs<T> base_args(const clawhip_rp& rp, const Geom& gin, const Geom& gout, 
   a.nstrips = 0; a.axis = 0; a.i_lo = a.i_hi = a.p_lo = a.p_hi = 0;
   a.w_ilo = a.w_ihi = a.w_plo = a.w_phi = 0;
   a.H = kMarchRows; a.nrb = a.npencil = a.nistrips = 0;
+  a.aux = nullptr;
+  a.gaux = gout;
+  a.bcaux = a.bc;
   return a;
+}
+
+// The aux fields of a 2D split step: bcaux from the aux BC codes, never negated.
+template <typename T>
+void set_aux(SweepArgs<T>& a, const void* aux, const Geom* gaux, const int* alo,
+             const int* ahi) {
+  if (!aux) return;
+  const int norefl[3] = {-1, -1, -1};
+  a.aux = (const T*)aux;
+  a.gaux = *gaux;
+  a.bcaux = to_bc(alo, ahi, norefl, 2);
 }
 
 template <class RS, typename T, bool KAHAN, int DIM, bool HALF>
@@ -349,18 +392,20 @@ template <class RS, typename T, bool KAHAN>
 int run_split2(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_out,
                const Geom& gout, const void* cin, void* cout, void* work, void* cwork,
                const int* lo, const int* hi, const int* refl, int ng, const int* method,
-               const int* mthlim, double dx, double dy, double dt, unsigned long long* cfl,
-               hipStream_t stream) {
+               const int* mthlim, double dx, double dy, double dt, const AuxIn& aux,
+               unsigned long long* cfl, hipStream_t stream) {
   Geom gw;
   gw.mx = gout.mx; gw.my = gout.my; gw.mz = 1; gw.g = ng;
   gw.rs = gout.mx + 2 * ng; gw.fs = (long long)gw.rs * (gout.my + 2 * ng);
   gw.ss = 0;   // 2D: no k index (cell_off adds (k-1+g)*ss with k = 1)
   auto ax = base_args<T>(rp, gin, gw, lo, hi, refl, 2, ng, method, mthlim, RS::mwaves, dt / dx,
                          q_in, work, cin, cwork, cfl);
+  set_aux(ax, aux.p, &aux.g, aux.lo, aux.hi);
   int rc = launch_x<RS, T, KAHAN, 2, false>(ax, stream);
   if (rc) return rc;
   auto ay = base_args<T>(rp, gw, gout, lo, hi, refl, 2, ng, method, mthlim, RS::mwaves, dt / dy,
                          work, q_out, cwork, cout, cfl);
+  set_aux(ay, aux.p, &aux.g, aux.lo, aux.hi);
   ay.i_lo = 1 - ng; ay.i_hi = gout.mx + ng; ay.p_lo = ay.p_hi = 1;
   ay.w_ilo = 1; ay.w_ihi = gout.mx; ay.w_plo = 1; ay.w_phi = 1;
   return launch_march<RS, T, KAHAN, 2, false>(ay, stream);
@@ -403,28 +448,32 @@ int run_split3(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_
 template <class RS>
 int run_sweep(const clawhip_rp& rp, int ndim, int dir, const void* q_in, const Geom& gin,
               void* q_out, const Geom& gout, int ng, const int* method, const int* mthlim,
-              double dtdx, unsigned long long* cfl, hipStream_t stream) {
+              double dtdx, const AuxIn& aux, unsigned long long* cfl, hipStream_t stream) {
   const int given[3] = {BC_GIVEN, BC_GIVEN, BC_GIVEN};
   const int refl[3] = {1, 2, 3};
   auto a = base_args<double>(rp, gin, gout, given, given, refl, ndim, ng, method, mthlim,
                              RS::mwaves, dtdx, q_in, q_out, nullptr, nullptr, cfl);
-  if (dir == 1) {
-    if (ndim == 2) return launch_x<RS, double, false, 2, false>(a, stream);
-    return launch_x<RS, double, false, 3, true>(a, stream);
-  }
-  if (ndim == 2) {   // step2ds ids = 2 (step2ds.f90:180): columns 1-ng..mx+ng
+  set_aux(a, aux.p, &aux.g, given, given);
+  if (ndim == 2) {
+    if (dir == 1) return launch_x<RS, double, false, 2, false>(a, stream);
+    // step2ds ids = 2 (step2ds.f90:180): columns 1-ng..mx+ng
     a.i_lo = 1 - ng; a.i_hi = gout.mx + ng; a.p_lo = a.p_hi = 1;
     a.w_ilo = 1; a.w_ihi = gout.mx; a.w_plo = 1; a.w_phi = 1;
     return launch_march<RS, double, false, 2, false>(a, stream);
   }
-  if (dir == 2) {    // step3ds idir = 2: i, k in 0..m+1
-    a.i_lo = 0; a.i_hi = gout.mx + 1; a.p_lo = 0; a.p_hi = gout.mz + 1;
-    a.w_ilo = 0; a.w_ihi = gout.mx + 1; a.w_plo = 0; a.w_phi = gout.mz + 1;
-    return launch_march<RS, double, false, 2, true>(a, stream);
+  if constexpr (RS::maux > 0) {
+    return CLAWHIP_E_KIND;   // aux fields: 2D only
+  } else {
+    if (dir == 1) return launch_x<RS, double, false, 3, true>(a, stream);
+    if (dir == 2) {    // step3ds idir = 2: i, k in 0..m+1
+      a.i_lo = 0; a.i_hi = gout.mx + 1; a.p_lo = 0; a.p_hi = gout.mz + 1;
+      a.w_ilo = 0; a.w_ihi = gout.mx + 1; a.w_plo = 0; a.w_phi = gout.mz + 1;
+      return launch_march<RS, double, false, 2, true>(a, stream);
+    }
+    a.i_lo = 0; a.i_hi = gout.mx + 1; a.p_lo = 0; a.p_hi = gout.my + 1;   // idir = 3
+    a.w_ilo = 0; a.w_ihi = gout.mx + 1; a.w_plo = 0; a.w_phi = gout.my + 1;
+    return launch_march<RS, double, false, 3, true>(a, stream);
   }
-  a.i_lo = 0; a.i_hi = gout.mx + 1; a.p_lo = 0; a.p_hi = gout.my + 1;   // idir = 3
-  a.w_ilo = 0; a.w_ihi = gout.mx + 1; a.w_plo = 0; a.w_phi = gout.my + 1;
-  return launch_march<RS, double, false, 3, true>(a, stream);
 }
 
 template <template <typename> class RSF>
@@ -432,13 +481,14 @@ int dispatch_split2(int dtype, const clawhip_rp& rp, const void* q_in, const Geo
                     void* q_out, const Geom& gout, const void* cin, void* cout, void* work,
                     void* cwork, const int* lo, const int* hi, const int* refl, int ng,
                     const int* method, const int* mthlim, double dx, double dy, double dt,
-                    unsigned long long* cfl, hipStream_t s) {
+                    const AuxIn& aux, unsigned long long* cfl, hipStream_t s) {
   if (dtype == CLAWHIP_F64)
     return run_split2<RSF<double>, double, false>(rp, q_in, gin, q_out, gout, cin, cout, work,
                                                   cwork, lo, hi, refl, ng, method, mthlim, dx,
-                                                  dy, dt, cfl, s);
+                                                  dy, dt, aux, cfl, s);
   return run_split2<RSF<float>, float, true>(rp, q_in, gin, q_out, gout, cin, cout, work, cwork,
-                                             lo, hi, refl, ng, method, mthlim, dx, dy, dt, cfl, s);
+                                             lo, hi, refl, ng, method, mthlim, dx, dy, dt, aux,
+                                             cfl, s);
 }
 }  // namespace
 
@@ -446,22 +496,42 @@ int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
                   void* work, void* cwork, const int* lo, const int* hi, const int* refl,
                   int ng, const int* method, const int* mthlim, double dx, double dy,
-                  double dt, unsigned long long* cfl, hipStream_t s) {
+                  double dt, const AuxIn& aux, unsigned long long* cfl, hipStream_t s) {
+  if (rp.kind == CLAWHIP_RP_VC_ADVECTION) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    switch (rp.meqn) {
+      case 1: return dispatch_split2<VCAdvection1>(dtype, rp, q_in, gin, q_out, gout, comp_in,
+                                                   comp_out, work, cwork, lo, hi, refl, ng,
+                                                   method, mthlim, dx, dy, dt, aux, cfl, s);
+      case 2: return dispatch_split2<VCAdvection2>(dtype, rp, q_in, gin, q_out, gout, comp_in,
+                                                   comp_out, work, cwork, lo, hi, refl, ng,
+                                                   method, mthlim, dx, dy, dt, aux, cfl, s);
+      case 3: return dispatch_split2<VCAdvection3>(dtype, rp, q_in, gin, q_out, gout, comp_in,
+                                                   comp_out, work, cwork, lo, hi, refl, ng,
+                                                   method, mthlim, dx, dy, dt, aux, cfl, s);
+      case 4: return dispatch_split2<VCAdvection4>(dtype, rp, q_in, gin, q_out, gout, comp_in,
+                                                   comp_out, work, cwork, lo, hi, refl, ng,
+                                                   method, mthlim, dx, dy, dt, aux, cfl, s);
+    }
+    return CLAWHIP_E_ARG;
+  }
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:
       return dispatch_split2<Acoustics>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                         work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
-                                        dt, cfl, s);
+                                        dt, aux, cfl, s);
     case CLAWHIP_RP_EULER:
       return dispatch_split2<Euler>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, work,
-                                    cwork, lo, hi, refl, ng, method, mthlim, dx, dy, dt, cfl, s);
+                                    cwork, lo, hi, refl, ng, method, mthlim, dx, dy, dt, aux, cfl,
+                                    s);
     case CLAWHIP_RP_SHALLOW:
       return dispatch_split2<Shallow>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, work,
-                                      cwork, lo, hi, refl, ng, method, mthlim, dx, dy, dt, cfl, s);
+                                      cwork, lo, hi, refl, ng, method, mthlim, dx, dy, dt, aux,
+                                      cfl, s);
     case CLAWHIP_RP_ADVECTION:
       return dispatch_split2<Advection>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                         work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
-                                        dt, cfl, s);
+                                        dt, aux, cfl, s);
   }
   return CLAWHIP_E_KIND;
 }
@@ -483,20 +553,34 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
 
 int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& gin, void* q_out,
                   const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
-                  unsigned long long* cfl, hipStream_t s) {
+                  const AuxIn& aux, unsigned long long* cfl, hipStream_t s) {
+  if (rp.kind == CLAWHIP_RP_VC_ADVECTION) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    switch (rp.meqn) {
+      case 1: return run_sweep<VCAdvection1<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng,
+                                                     method, mthlim, dtdx, aux, cfl, s);
+      case 2: return run_sweep<VCAdvection2<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng,
+                                                     method, mthlim, dtdx, aux, cfl, s);
+      case 3: return run_sweep<VCAdvection3<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng,
+                                                     method, mthlim, dtdx, aux, cfl, s);
+      case 4: return run_sweep<VCAdvection4<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng,
+                                                     method, mthlim, dtdx, aux, cfl, s);
+    }
+    return CLAWHIP_E_ARG;
+  }
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:
       return run_sweep<Acoustics<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
-                                          dtdx, cfl, s);
+                                          dtdx, aux, cfl, s);
     case CLAWHIP_RP_EULER:
       return run_sweep<Euler<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
-                                      dtdx, cfl, s);
+                                      dtdx, aux, cfl, s);
     case CLAWHIP_RP_SHALLOW:
       return run_sweep<Shallow<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
-                                        dtdx, cfl, s);
+                                        dtdx, aux, cfl, s);
     case CLAWHIP_RP_ADVECTION:
       return run_sweep<Advection<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
-                                          dtdx, cfl, s);
+                                          dtdx, aux, cfl, s);
   }
   return CLAWHIP_E_KIND;
 }
@@ -505,8 +589,9 @@ int launch_sweep3(const clawhip_rp& rp, int idir, const void* q_in, const Geom& 
                   const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
                   unsigned long long* cfl, hipStream_t s) {
   if (rp.kind != CLAWHIP_RP_EULER3) return CLAWHIP_E_KIND;
+  const AuxIn none;
   return run_sweep<Euler3<double>>(rp, 3, idir, q_in, gin, q_out, gout, ng, method, mthlim, dtdx,
-                                   cfl, s);
+                                   none, cfl, s);
 }
 
 }  // namespace claw

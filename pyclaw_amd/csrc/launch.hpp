@@ -33,11 +33,19 @@ int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom
                    const int* mthlim, double dx, double dy, double dt,
                    unsigned long long* cfl_bits, hipStream_t stream);
 
+// Aux fields of a 2D split step (p == nullptr: none): planar like q, their
+// boundary codes per side (never negating).
+struct AuxIn {
+  const void* p = nullptr;
+  Geom g{};
+  int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+};
+
 int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
                   void* work, void* cwork, const int* lo, const int* hi, const int* refl,
                   int ng, const int* method, const int* mthlim, double dx, double dy,
-                  double dt, unsigned long long* cfl, hipStream_t s);
+                  double dt, const AuxIn& aux, unsigned long long* cfl, hipStream_t s);
 
 int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
@@ -68,7 +76,7 @@ int ensemble_run(const clawhip_rp& rp, int dtype, const void* table_dev, const l
 
 int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& gin, void* q_out,
                   const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
-                  unsigned long long* cfl, hipStream_t s);
+                  const AuxIn& aux, unsigned long long* cfl, hipStream_t s);
 int launch_sweep3(const clawhip_rp& rp, int idir, const void* q_in, const Geom& gin, void* q_out,
                   const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
                   unsigned long long* cfl, hipStream_t s);

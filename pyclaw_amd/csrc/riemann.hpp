@@ -41,7 +41,7 @@ __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
 // ---------------------------------------------------------------- acoustics
 template <typename T>
 struct Acoustics {
-  static constexpr int meqn = 3, mwaves = 2;
+  static constexpr int meqn = 3, mwaves = 2, maux = 0;
   struct Roe {};
   template <int IXY>
   __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
@@ -84,7 +84,7 @@ struct Acoustics {
 // ---------------------------------------------------------------- 2D Euler
 template <typename T>
 struct Euler {
-  static constexpr int meqn = 4, mwaves = 4;
+  static constexpr int meqn = 4, mwaves = 4, maux = 0;
   struct Roe { T u, v, enth, u2v2, a, g1a2, euv; };
   // normal flux; the Roe linearisation gives amdq + apdq = flux(qR) - flux(qL)
   template <int IXY>
@@ -235,7 +235,7 @@ struct Euler {
 // ---------------------------------------------------------------- shallow water
 template <typename T>
 struct Shallow {
-  static constexpr int meqn = 3, mwaves = 3;
+  static constexpr int meqn = 3, mwaves = 3, maux = 0;
   struct Roe { T u, v, a; };
   // normal flux; the Roe linearisation gives amdq + apdq = flux(qR) - flux(qL)
   template <int IXY>
@@ -351,7 +351,7 @@ struct Shallow {
 // ---------------------------------------------------------------- advection
 template <typename T>
 struct Advection {
-  static constexpr int meqn = 1, mwaves = 1;
+  static constexpr int meqn = 1, mwaves = 1, maux = 0;
   struct Roe {};
   template <int IXY>
   __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
@@ -378,7 +378,7 @@ struct Advection {
 // ---------------------------------------------------------------- 3D Euler
 template <typename T>
 struct Euler3 {
-  static constexpr int meqn = 5, mwaves = 3;
+  static constexpr int meqn = 5, mwaves = 3, maux = 0;
   struct Roe {};
   // IXY = 1, 2, 3 selects x, y, z as the normal direction.
   template <int IXY>
@@ -488,5 +488,55 @@ struct Euler3 {
     for (int m = 0; m < meqn; m++) { bm[m] = T(0); bp[m] = T(0); }
   }
 };
+
+// ---------------------------------------------------------------- variable-coefficient advection
+// examples/advection_reaction_2d/rpn2_vc_advection_vector.f90:53-60: the
+// non-conservative transport q_t + u q_x + v q_y = 0 of ME fields, velocities
+// given at cell edges in aux (aux[0] = u at the cell's left edge, aux[1] = v
+// at its lower edge).  Interface i (left cell i-1, right cell i): one wave
+// q_i - q_{i-1} with speed the right cell's edge velocity; its flux
+// difference goes downwind.  Dimension-split steps only (no transverse solver).
+template <typename T, int ME>
+struct VCAdvection {
+  static constexpr int meqn = ME, mwaves = 1, maux = 2;
+  struct Roe {};
+  template <int IXY>
+  __device__ static __forceinline__ void rpn_aux(const RPParams<T>&, const T* qL, const T* qR,
+                                                 const T* auxL, const T* auxR,
+                                                 T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                                 T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+    (void)auxL;
+    const T u = auxR[IXY - 1];
+    const T um = u < T(0) ? u : T(0);   // dmin1(u, 0)
+    const T up = u > T(0) ? u : T(0);   // dmax1(u, 0)
+    s[0] = u;
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      wave[0][m] = qR[m] - qL[m];
+      amdq[m] = um * wave[0][m];
+      apdq[m] = up * wave[0][m];
+    }
+  }
+};
+template <typename T> using VCAdvection1 = VCAdvection<T, 1>;
+template <typename T> using VCAdvection2 = VCAdvection<T, 2>;
+template <typename T> using VCAdvection3 = VCAdvection<T, 3>;
+template <typename T> using VCAdvection4 = VCAdvection<T, 4>;
+
+// The normal solve with the aux values of the left and right cells; solvers
+// without aux fields take the plain call.
+template <class RS, int IXY, typename T>
+__device__ __forceinline__ void rpn_call(const RPParams<T>& p, const T* qL, const T* qR,
+                                         const T* auxL, const T* auxR,
+                                         T (&wave)[RS::mwaves][RS::meqn], T (&s)[RS::mwaves],
+                                         T (&amdq)[RS::meqn], T (&apdq)[RS::meqn],
+                                         typename RS::Roe& roe) {
+  if constexpr (RS::maux > 0) {
+    RS::template rpn_aux<IXY>(p, qL, qR, auxL, auxR, wave, s, amdq, apdq, roe);
+  } else {
+    (void)auxL; (void)auxR;
+    RS::template rpn<IXY>(p, qL, qR, wave, s, amdq, apdq, roe);
+  }
+}
 
 }  // namespace claw

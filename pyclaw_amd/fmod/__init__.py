@@ -28,20 +28,23 @@ class CParam(object):
     (state.py:206-231) copies problem_data onto it."""
 
 
-def rp_struct(rpn, cparam):
-    """Device Riemann-solver parameters from the solver tag and cparam."""
+def rp_struct(rpn, cparam, meqn=0):
+    """Device Riemann-solver parameters from the solver tag and cparam (meqn: the
+    number of fields, for the solvers that take any)."""
     if isinstance(rpn, bool) or not isinstance(rpn, (int, np.integer)):
         raise TypeError("pyclaw_amd.fmod runs the device Riemann solvers: use a "
                         "pyclaw_amd.riemann module as the solver's riemann_solver")
     pd = {k: v for k, v in vars(cparam).items() if not k.startswith("_")}
-    return K.make_rp(int(rpn), pd)
+    return K.make_rp(int(rpn), pd, meqn=meqn)
 
 
-def check_method(method, use_fwave):
+def check_method(method, use_fwave, allow_aux=False):
     if use_fwave:
         raise NotImplementedError("f-wave Riemann solvers are not supported")
-    if int(method[5]) != 0 or int(method[6]) != 0:
-        raise NotImplementedError("capacity functions and aux arrays are not supported")
+    if int(method[5]) != 0:
+        raise NotImplementedError("capacity functions are not supported")
+    if int(method[6]) != 0 and not allow_aux:
+        raise NotImplementedError("aux arrays are supported by the dimension-split 2D step only")
 
 
 def mthlim4(mthlim):

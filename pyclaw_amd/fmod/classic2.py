@@ -31,15 +31,18 @@ def step2ds(maxm, num_ghost, mx, my, qold, qnew, aux, dx, dy, dt, method, mthlim
             aux3, work, ids, use_fwave, rpn2, rpt2):
     """One dimension-split sweep (classic/step2ds.f90:2-264): ids 1 = x, 2 = y.
     Returns (qnew, cfl)."""
-    check_method(method, use_fwave)
-    rp = rp_struct(rpn2, cparam)
+    check_method(method, use_fwave, allow_aux=True)
+    rp = rp_struct(rpn2, cparam, meqn=qold.shape[0])
     g = L.geom2(mx, my, num_ghost)
     qin = to_device2(qold)
     qout = torch.empty_like(qin) if ids == 1 else qin
     cfl = new_cfl()
+    auxd = to_device2(aux) if (aux is not None and method[6] > 0) else None
     rc = L.lib().clawhip_step2_sweep(ctypes.byref(rp), L.F64, int(ids), ptr(qin), ctypes.byref(g),
                                      ptr(qout), ctypes.byref(g), int(num_ghost), L.ints(method),
                                      L.ints(mthlim4(mthlim)), float(dx), float(dy), float(dt),
+                                     ptr(auxd) if auxd is not None else None,
+                                     ctypes.byref(g) if auxd is not None else None,
                                      ptr(cfl), stream())
     L.check(rc, "clawhip_step2_sweep")
     qnew[...] = from_device2(qout)

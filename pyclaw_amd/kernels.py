@@ -26,8 +26,9 @@ def _dtype_code(t):
     raise L.ClawhipError(f"unsupported dtype {t.dtype}")
 
 
-def make_rp(kind, problem_data, efix=1):
-    """Parameters of the Riemann solver from problem_data (state.py:206-231)."""
+def make_rp(kind, problem_data, efix=1, meqn=0):
+    """Parameters of the Riemann solver from problem_data (state.py:206-231);
+    meqn: the number of fields, for solvers that take any (vc advection)."""
     pd = problem_data
     rho = float(pd.get("rho", 1.0))
     bulk = float(pd.get("bulk", 1.0))
@@ -35,7 +36,7 @@ def make_rp(kind, problem_data, efix=1):
     zz = float(pd.get("zz", rho * cc))
     return L.RP(kind, int(pd.get("efix", efix)), rho, bulk, cc, zz,
                 float(pd.get("gamma", 1.4)), float(pd.get("grav", 1.0)),
-                float(pd.get("u", 0.0)), float(pd.get("v", 0.0)))
+                float(pd.get("u", 0.0)), float(pd.get("v", 0.0)), int(meqn))
 
 
 def cfl_reset(cfl):
@@ -69,8 +70,10 @@ def _work_like(q_out, ng, work):
 
 
 def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim, dx, dy, dt,
-                comp_in=None, comp_out=None, num_ghost=2, gin=None, work=None):
-    """Both dimension-split sweeps (step2ds ids=1, 2); returns the work buffers."""
+                comp_in=None, comp_out=None, num_ghost=2, gin=None, work=None, aux=None,
+                aux_bc_lower=None, aux_bc_upper=None):
+    """Both dimension-split sweeps (step2ds ids=1, 2); returns the work buffers.
+    aux: planar (maux, my, mx) tensor of q's dtype for solvers with aux fields."""
     meqn, my, mx = q_out.shape
     gout = L.geom2(mx, my)
     if gin is None:
@@ -85,7 +88,9 @@ def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthli
         _ptr(comp_in), _ptr(comp_out), _ptr(w), _ptr(cw), L.ints(bc_lower), L.ints(bc_upper),
         L.ints(reflect), int(num_ghost), L.ints(method),
         L.ints(list(mthlim) + [0] * (4 - len(mthlim))), float(dx), float(dy), float(dt),
-        _ptr(cfl), _stream())
+        _ptr(aux), ctypes.byref(L.geom2(mx, my)) if aux is not None else None,
+        L.ints(aux_bc_lower) if aux is not None else None,
+        L.ints(aux_bc_upper) if aux is not None else None, _ptr(cfl), _stream())
     L.check(rc, "clawhip_step2_split")
     return (w, cw)
 

@@ -164,6 +164,22 @@ class Solver(object):
                 out.append(int(b))
         return lo, hi
 
+    def _aux_bc_codes(self):
+        """Kernel codes of solver.aux_bc_lower/upper (extrap, periodic, wall: a wall
+        mirrors aux without negation, solver.py:439-519)."""
+        lo, hi = [], []
+        for d in range(self.num_dim):
+            for side, codes, out in (("Lower", self.aux_bc_lower, lo),
+                                     ("Upper", self.aux_bc_upper, hi)):
+                b = codes[d]
+                if b is None:
+                    raise Exception("%s boundary condition not specified for either q or aux." % side)
+                if b not in (BC.extrap, BC.periodic, BC.wall):
+                    raise NotImplementedError("aux boundary condition %s is not supported on the "
+                                              "device" % b)
+                out.append(int(b))
+        return lo, hi
+
     # ---- time-step control (solver.py:524-552)
     def accept_reject_step(self, state):
         cfl = self.cfl.get_cached_max()

@@ -32,6 +32,7 @@ class State(object):
         self.aux = self.new_array(num_aux) if num_aux > 0 else None
         self._dev = None          # (q_tensor, comp_tensor_or_None) when the device is newer
         self._dev_stale = True    # device copy must be re-uploaded before use
+        self._aux_dev = None      # device copy of aux (uploaded at every step that uses it)
 
     # ---- geometry passthrough
     @property
@@ -89,6 +90,14 @@ class State(object):
         self._dev = (q, comp)
         self._dev_stale = False
         return self._dev
+
+    def device_aux(self, dtype_code):
+        """Planar device copy of aux in the state's arithmetic type.  aux is a
+        plain host array that users (and before_step callbacks) change in place,
+        so every call uploads it again."""
+        from .device import q_to_device
+        self._aux_dev = q_to_device(np.asarray(self.aux), dtype_code)
+        return self._aux_dev
 
     def set_device_q(self, q, comp=None):
         """The solver hands back the newest state (device tensors)."""

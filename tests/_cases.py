@@ -21,6 +21,40 @@ def advection_2d():
     return q, d
 
 
+def advection_reaction_2d():
+    """examples/advection_reaction_2d/advection_reaction.py: source_step 43-47,
+    psi 50-52, set_velocities 54-63, setup 66-107 (100^2 on [0,1]^2, t_period 4,
+    epsilon 0.1).  Returns (q, aux, delta, nodes, set_velocities(t, dt),
+    source(q, dt)): the edge-velocity field for a time and the reaction step on
+    a host q (meqn, mx, my)."""
+    t_period, epsilon = 4.0, 0.1
+    (Xc, Yc), d = grid([0, 0], [1, 1], [100, 100])
+    xe = np.linspace(0.0, 1.0, 101)
+    Xe, Ye = np.meshgrid(xe, xe, indexing="ij")
+    dx, dy = d
+
+    def psi(x, y):
+        return (np.sin(np.pi * x)) ** 2 * (np.sin(np.pi * y)) ** 2 / np.pi
+
+    def velocities(v_t=1.0):
+        aux = np.empty((2, 100, 100), order="F")
+        aux[0] = -(psi(Xe[:-1, 1:], Ye[:-1, 1:]) - psi(Xe[:-1, :-1], Ye[:-1, :-1])) / dy
+        aux[1] = (psi(Xe[1:, :-1], Ye[1:, :-1]) - psi(Xe[:-1, :-1], Ye[:-1, :-1])) / dx
+        aux[:] = v_t * aux[:]
+        return aux
+
+    def set_velocities(t, dt):
+        return velocities(np.cos(2 * np.pi * (t + dt / 2) / t_period))
+
+    def source(q, dt):
+        qq = q.copy()
+        q[0, :, :] = q[0, :, :] + epsilon * dt * qq[1, :, :]
+        q[1, :, :] = q[1, :, :] + epsilon * dt * qq[0, :, :]
+
+    q = np.array([(Xc <= 0.5) * 1.0, (Yc <= 0.5) * 1.0])
+    return q, velocities(), d, set_velocities, source
+
+
 def euler_quadrants():
     """examples/euler_2d/quadrants.py:80-110."""
     (X, Y), d = grid([0, 0], [1, 1], [100, 100])

@@ -92,6 +92,10 @@ def main():
     dst = os.path.join(HERE, "acoustics_2d_verify_classic.npz")
     np.savez_compressed(dst, pressure=p)
     print(dst, p.shape)
+    rho = np.loadtxt(os.path.join(REF, "advection_reaction_2d", "advection_reaction.txt"))
+    dst = os.path.join(HERE, "advection_reaction_2d_expected.npz")
+    np.savez_compressed(dst, density=rho)
+    print(dst, rho.shape)
 
 
 if __name__ == "__main__":

@@ -10,7 +10,10 @@ reference BC rules) and the reference routine's outputs:
   classic/step3ds.f90    split sweeps 1, 2, 3
   sharpclaw/flux2.f90    char_decomp 0 (weno.f90 weno5) and 2 (weno5_char)
 
-Output: tests/golden/ref_step_vectors.npz
+and, with aux fields, step2ds sweeps 1 and 2 driving the advection_reaction_2d
+example's own Riemann solver (rpn2_vc_advection_vector.f90, built by ref.mk).
+
+Output: tests/golden/ref_step_vectors.npz, tests/golden/ref_step_vectors_aux.npz
 """
 import os
 import sys
@@ -79,6 +82,43 @@ def main():
         out[key + "_qout"] = dq
         n += 1
     dst = os.path.join(HERE, "ref_step_vectors.npz")
+    np.savez_compressed(dst, **out)
+    print(dst, n, "cases")
+    aux_vectors()
+
+
+def aux_vectors():
+    """step2ds with edge-velocity aux fields: the example's rpn2, meqn 1..3,
+    velocities of both signs, aux BCs (never negated) as the q BCs."""
+    rng = np.random.default_rng(20240612)
+    rpn2 = O.ref_vcadv_rpn2()
+    out = {}
+    n = 0
+    for meqn, ids, lim, bcl, bcu in [(2, 1, 4, [O.EXTRAP, O.WALL], [O.PERIODIC, O.EXTRAP]),
+                                     (2, 2, 4, [O.EXTRAP, O.WALL], [O.PERIODIC, O.EXTRAP]),
+                                     (1, 1, 1, [O.WALL, O.PERIODIC], [O.WALL, O.PERIODIC]),
+                                     (3, 2, 3, [O.PERIODIC, O.EXTRAP], [O.PERIODIC, O.WALL])]:
+        ng, mx, my = 2, 13, 10
+        q = rng.uniform(-1.0, 2.0, (meqn, mx, my))
+        aux = rng.uniform(-1.5, 1.5, (2, mx, my))
+        qbc = O.with_ghosts(np.asfortranarray(q), ng)
+        abc = O.with_ghosts(np.asfortranarray(aux), ng)
+        O.bc2(qbc, ng, bcl, bcu)
+        O.bc2(abc, ng, bcl, bcu, reflect=(-1, -1))
+        meth = O.method_array(dimensional_split=True, num_aux=2)
+        dx, dy, dt = 0.1, 0.13, 0.021
+        qnew = qbc.copy(order="F")
+        cfl = O.ref_step2ds_aux(rpn2, qbc, qnew, abc, ng, dx, dy, dt, meth, [lim], ids)
+        key = f"a{n:02d}"
+        out[key + "_meta"] = np.array([O.VC_ADVECTION, ids, -1, lim, ng, mx, my, meqn],
+                                      dtype=np.int64)
+        out[key + "_real"] = np.array([dx, dy, dt, cfl, 0.0])
+        out[key + "_bc"] = np.array(bcl + bcu, dtype=np.int64)
+        out[key + "_qin"] = qbc
+        out[key + "_aux"] = abc
+        out[key + "_qout"] = qnew
+        n += 1
+    dst = os.path.join(HERE, "ref_step_vectors_aux.npz")
     np.savez_compressed(dst, **out)
     print(dst, n, "cases")
 

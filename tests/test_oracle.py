@@ -12,6 +12,8 @@
    asdq), WENO5 exactness on linear data, evl*evr = I, conservation under
    periodic boundaries, and the limiter closed forms.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -87,6 +89,52 @@ def test_live_reference_bit_exact():
         assert np.array_equal(d1[:, ng:-ng, ng:-ng], d2[:, ng:-ng, ng:-ng]) and c1 == c2
 
 
+def _aux_case(vec, key):
+    meta = vec[key + "_meta"]
+    kind, ids, _, lim, ng, mx, my, meqn = [int(x) for x in meta]
+    dx, dy, dt, cfl_ref, _ = vec[key + "_real"]
+    qin, aux = vec[key + "_qin"], vec[key + "_aux"]
+    rp = O.make_rp(kind, meqn=meqn)
+    out = np.asfortranarray(qin.copy())
+    cfl = O.step2(rp, np.asfortranarray(qin), out, ng, dx, dy, dt,
+                  O.method_array(dimensional_split=True, num_aux=2), [lim], ids=ids,
+                  aux=np.asfortranarray(aux))
+    return out, cfl, cfl_ref
+
+
+def test_golden_aux_vectors_bit_exact(golden_dir):
+    """step2ds with aux fields, the advection_reaction_2d example's Riemann
+    solver (golden vectors from the reference's own Fortran)."""
+    vec = np.load(f"{golden_dir}/ref_step_vectors_aux.npz")
+    keys = sorted({k.split("_")[0] for k in vec.files})
+    assert len(keys) == 4
+    for key in keys:
+        out, cfl, cfl_ref = _aux_case(vec, key)
+        ng = int(vec[key + "_meta"][4])
+        sl = (slice(None),) + (slice(ng, -ng),) * 2
+        assert np.array_equal(out[sl], vec[key + "_qout"][sl]), key
+        assert cfl == cfl_ref, key
+
+
+@pytest.mark.skipif(not (O.ref_available() and os.path.exists(O.REF_VCADV_PATH)),
+                    reason="reference build (oracle/_ref) not present")
+def test_live_reference_aux_bit_exact():
+    rng = np.random.default_rng(11)
+    rpn2 = O.ref_vcadv_rpn2()
+    for meqn, ids in [(2, 1), (2, 2), (4, 1)]:
+        ng, mx, my = 2, 9, 12
+        qbc = O.with_ghosts(np.asfortranarray(rng.uniform(0, 1, (meqn, mx, my))), ng)
+        abc = O.with_ghosts(np.asfortranarray(rng.uniform(-1, 1, (2, mx, my))), ng)
+        O.bc2(qbc, ng, [O.EXTRAP, O.WALL], [O.WALL, O.PERIODIC])
+        O.bc2(abc, ng, [O.EXTRAP, O.WALL], [O.WALL, O.PERIODIC], reflect=(-1, -1))
+        meth = O.method_array(dimensional_split=True, num_aux=2)
+        a, b = qbc.copy(order="F"), qbc.copy(order="F")
+        c1 = O.step2(O.make_rp(O.VC_ADVECTION, meqn=meqn), qbc, a, ng, 0.1, 0.08, 0.03, meth,
+                     [4], ids=ids, aux=abc)
+        c2 = O.ref_step2ds_aux(rpn2, qbc, b, abc, ng, 0.1, 0.08, 0.03, meth, [4], ids)
+        assert np.array_equal(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) and c1 == c2
+
+
 # ---------------------------------------------------------------- fixtures
 def _l1(frame, expected, d):
     return np.prod(d) * np.abs(frame.reshape(-1) - expected).sum()
@@ -151,6 +199,28 @@ def test_fixture_acoustics_verify_classic(golden_dir):
     exp = np.load(f"{golden_dir}/acoustics_2d_verify_classic.npz")["pressure"]
     dd = np.prod(d)
     assert dd * np.linalg.norm(exp - fr[-1][0]) / (dd * np.linalg.norm(exp)) < 1e-3
+
+
+def test_fixture_advection_reaction(golden_dir):
+    """examples/advection_reaction_2d/test_advection_reaction.py: vc advection
+    (the example's Riemann solver, edge velocities in aux, reset every step by
+    before_step), dimension-split, reaction by step_source (Godunov split),
+    t = 4; density against the stored result, check_diff reltol 1e-5 (2-norm)."""
+    q, aux, d, set_vel, source = C.advection_reaction_2d()
+
+    def before(run):
+        run.aux = set_vel(run.t, run.dt)
+
+    def src(run, dt):
+        source(run.q, dt)
+
+    run = OracleRun(O.make_rp(O.VC_ADVECTION, meqn=2), q, d, [O.EXTRAP] * 2, [O.EXTRAP] * 2,
+                    dimensional_split=True, limiters=1, aux=aux, aux_bc_lower=[O.EXTRAP] * 2,
+                    aux_bc_upper=[O.EXTRAP] * 2, before_step=before, step_source=src)
+    fr = run.run(4.0, 10)
+    exp = np.load(f"{golden_dir}/advection_reaction_2d_expected.npz")["density"]
+    err = np.linalg.norm(exp - fr[-1][0]) / np.linalg.norm(exp)
+    assert err < 1e-5, err
 
 
 def test_fixture_shocktube_3d(golden_dir):
