@@ -43,10 +43,14 @@ enum {
   CLAWHIP_RP_SHALLOW = 3,   /* shallow_roe_with_efix_2D */
   CLAWHIP_RP_ADVECTION = 4, /* advection_2D             */
   CLAWHIP_RP_EULER3 = 5,    /* euler_3D                 */
-  CLAWHIP_RP_VC_ADVECTION = 6 /* variable-coefficient advection of rp->meqn (1..4) fields,
-                               * edge velocities in 2 aux fields
-                               * (examples/advection_reaction_2d/rpn2_vc_advection_vector.f90);
-                               * dimension-split 2D steps only */
+  CLAWHIP_RP_VC_ADVECTION = 6, /* variable-coefficient advection of rp->meqn (1..4) fields,
+                                * edge velocities in 2 aux fields
+                                * (examples/advection_reaction_2d/rpn2_vc_advection_vector.f90);
+                                * its rpt2 is a dummy: dimension-split steps, or unsplit
+                                * with transverse_waves = 0 */
+  CLAWHIP_RP_VC_ADVECTION_2D = 7 /* vc_advection_2D (clawpack/riemann): scalar variable-
+                                  * coefficient advection, the same rpn2 with a transverse
+                                  * solver (rpt2_vc_advection.f90); 2D steps */
 };
 
 /* Boundary codes: pyclaw.BC (pyclaw/solver.py:13-19) plus GIVEN. */
@@ -101,13 +105,20 @@ int clawhip_cfl_reset(double *cfl_dev, void *stream);
  *   q_in (geometry gin) -> q_out (geometry gout, ghosts = 0); distinct buffers;
  *   comp_in -> comp_out: Kahan compensation (CLAWHIP_F32K only; may alias);
  *   *cfl_dev = max(*cfl_dev, Courant number of this step) (the `cfl` output).
- * reflect[d] is solver.reflect_index[d] (classic/solver.py:445). */
+ * reflect[d] is solver.reflect_index[d] (classic/solver.py:445).
+ * aux (geometry gaux, method[6] fields, codes aux_bc_lower/upper as for
+ * clawhip_step2_split) for the solvers that read aux (kinds 6 and 7, at least
+ * their 2 velocity fields, at most 3 fields), else NULL.  method[5] =
+ * index_capa + 1 > 0 names a capacity field among them: the capa branches of
+ * step2.f90:101-105,152-163,183-187,232-244 with flux2.f90's per-cell dtdx1d. */
 int clawhip_step2_unsplit(const clawhip_rp *rp, int dtype, const void *q_in,
                           const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
                           const void *comp_in, void *comp_out,
                           const int *bc_lower, const int *bc_upper,
                           const int *reflect, int num_ghost, const int *method,
                           const int *mthlim, double dx, double dy, double dt,
+                          const void *aux, const clawhip_geom *gaux,
+                          const int *aux_bc_lower, const int *aux_bc_upper,
                           double *cfl_dev, void *stream);
 
 /* Up to nsteps attempted unsplit steps with the control on the device: each

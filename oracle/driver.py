@@ -24,7 +24,8 @@ class OracleRun:
                  cfl_desired=0.9, cfl_max=1.0, dt_initial=0.1, dt_max=1e99,
                  max_steps=10000, dt_variable=True, time_integrator="SSP104",
                  char_decomp=0, num_ghost=None, aux=None, aux_bc_lower=None,
-                 aux_bc_upper=None, before_step=None, step_source=None, source_split=1):
+                 aux_bc_upper=None, before_step=None, step_source=None, source_split=1,
+                 index_capa=-1, auxbc_fn=None):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -59,6 +60,9 @@ class OracleRun:
         self.aux = None if aux is None else np.asfortranarray(aux, dtype=np.float64).copy(order="F")
         self.aux_bc_lower, self.aux_bc_upper = aux_bc_lower, aux_bc_upper
         self.before_step, self.step_source, self.source_split = before_step, step_source, source_split
+        # capacity field (0-based aux index, -1: none); auxbc_fn(run) -> the aux
+        # array with its ghosts filled, for custom aux boundary conditions
+        self.index_capa, self.auxbc_fn = index_capa, auxbc_fn
 
     # ---- boundary conditions: Solver._apply_bcs (solver.py:351-437)
     def _apply_bcs(self, q):
@@ -97,6 +101,8 @@ class OracleRun:
                 self.dt = tend - t
 
     def _auxbc(self):
+        if self.auxbc_fn is not None:
+            return np.asfortranarray(self.auxbc_fn(self), dtype=np.float64)
         ng = self.ng
         abc = np.zeros((self.aux.shape[0],) + tuple(n + 2 * ng for n in self.aux.shape[1:]),
                        order="F")
@@ -126,9 +132,13 @@ class OracleRun:
                              aux=abc)
                 cfl = max(c1, c2)
             else:
+                maux = 0 if self.aux is None else self.aux.shape[0]
+                abc = None if self.aux is None else self._auxbc()
                 meth = O.method_array(order=self.order, transverse=self.transverse,
-                                      dt_variable=self.dt_variable)
-                cfl = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=0)
+                                      dt_variable=self.dt_variable, num_aux=maux,
+                                      index_capa=self.index_capa)
+                cfl = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=0,
+                              aux=abc)
         else:
             dx, dy, dz = self.delta
             meth = O.method_array(order=self.order, dimensional_split=True,

@@ -30,6 +30,7 @@ int orc_rp_meqn(int kind) {
     case ORC_RP_SHALLOW: return 3;
     case ORC_RP_ADVECTION: return 1;
     case ORC_RP_EULER3: return 5;
+    case ORC_RP_VC_ADVECTION_2D: return 1;
   }
   return 0;   /* ORC_RP_VC_ADVECTION: any (orc_rp.meqn) */
 }
@@ -42,6 +43,7 @@ int orc_rp_mwaves(int kind) {
     case ORC_RP_ADVECTION: return 1;
     case ORC_RP_EULER3: return 3;
     case ORC_RP_VC_ADVECTION: return 1;
+    case ORC_RP_VC_ADVECTION_2D: return 1;
   }
   return 0;
 }
@@ -437,13 +439,15 @@ void orc_rpn(const orc_rp *p, int ixy, const double *qL, const double *qR,
 /* rpn2_vc_advection_vector.f90:53-60: one wave qR - qL with speed
  * auxl(ixy, i) (the right cell's edge velocity); its flux difference goes
  * downwind: amdq = dmin1(s, 0) * wave, apdq = dmax1(s, 0) * wave. */
+static int vc_meqn(const orc_rp *p) { return p->kind == ORC_RP_VC_ADVECTION_2D ? 1 : p->meqn; }
+
 static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const double *qR,
                              const double *auxR, double *wave, double *s, double *amdq,
                              double *apdq) {
   const double u = auxR[ixy - 1];
   const double um = u < 0.0 ? u : 0.0, up = u > 0.0 ? u : 0.0;
   s[0] = u;
-  for (int m = 0; m < p->meqn; m++) {
+  for (int m = 0; m < vc_meqn(p); m++) {
     wave[m] = qR[m] - qL[m];
     amdq[m] = um * wave[m];
     apdq[m] = up * wave[m];
@@ -454,10 +458,46 @@ void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
                  const double *auxL, const double *auxR, double *wave, double *s,
                  double *amdq, double *apdq) {
   (void)auxL;
-  if (p->kind == ORC_RP_VC_ADVECTION)
+  if (p->kind == ORC_RP_VC_ADVECTION || p->kind == ORC_RP_VC_ADVECTION_2D)
     rpn_vc_advection(p, ixy, qL, qR, auxR, wave, s, amdq, apdq);
   else
     orc_rpn(p, ixy, qL, qR, wave, s, amdq, apdq);
+}
+
+/* vc_advection_2D rpt2 (clawpack.riemann rpt2_vc_advection.f90, published form;
+ * the package is not in the reference tree): kv = 3-ixy is the transverse
+ * velocity; cell i1 = i-2+imp receives asdq; the down-going part moves with
+ * that cell's lower-edge velocity (aux2), the up-going part with its upper-edge
+ * velocity (aux3, the next slice's lower edge).  Applied to every field (the
+ * vector solver's own rpt2 in examples/advection_reaction_2d is a dummy). */
+/* A planted mistake for the test that shows the annulus fixture pins the
+ * choice of velocity component (orc_set_plant(1): the normal velocity instead
+ * of the transverse one; 0 = none).  Which edge (aux2 or aux3) and which cell
+ * (i1) the published solver reads is NOT pinned by any fixture: the annulus
+ * field has zero radial velocity and a theta velocity that depends on r only,
+ * so those choices leave its solution unchanged (parity unpinned). */
+static int g_plant = 0;
+void orc_set_plant(int k) { g_plant = k; }
+
+static void rpt_vc_advection(const orc_rp *p, int ixy, int imp, int maux, const double *a2,
+                             const double *a3, const double *asdq, double *bm, double *bp) {
+  const int kv = g_plant == 1 ? ixy : 3 - ixy;   /* 1-based aux index */
+  const long i1 = imp == 1 ? 0 : maux;           /* a2, a3 point at cell i-1 */
+  const double vm = dmin(a2[i1 + kv - 1], 0.0), vp = dmax(a3[i1 + kv - 1], 0.0);
+  for (int m = 0; m < vc_meqn(p); m++) {
+    bm[m] = vm * asdq[m];
+    bp[m] = vp * asdq[m];
+  }
+}
+
+void orc_rpt_aux(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
+                 int maux, const double *a1, const double *a2, const double *a3,
+                 const double *asdq, double *bmasdq, double *bpasdq) {
+  (void)a1;
+  if (p->kind == ORC_RP_VC_ADVECTION || p->kind == ORC_RP_VC_ADVECTION_2D)
+    rpt_vc_advection(p, ixy, imp, maux, a2, a3, asdq, bmasdq, bpasdq);
+  else
+    orc_rpt(p, ixy, imp, qL, qR, asdq, bmasdq, bpasdq);
 }
 
 void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
@@ -582,7 +622,8 @@ typedef struct {
   int meqn, mwaves, ng, len;
   double *wave, *s, *amdq, *apdq, *cqxx, *bmasdq, *bpasdq, *qadd, *fadd, *gadd;
   int maux;
-  double *aux2;   /* aux(:, slice), maux x len (the reference's aux2) */
+  double *aux1, *aux2, *aux3;   /* aux of the previous, this, next slice (maux x len) */
+  double *dtdx1d;               /* dt/dx per cell of the slice (len) */
 } slice_ws;
 
 static void ws_alloc(slice_ws *w, int meqn, int mwaves, int ng, int maxm) {
@@ -599,13 +640,21 @@ static void ws_alloc(slice_ws *w, int meqn, int mwaves, int ng, int maxm) {
   w->fadd = (double *)calloc(L * meqn, sizeof(double));
   w->gadd = (double *)calloc(L * meqn * 2, sizeof(double));
   w->maux = 0;
-  w->aux2 = NULL;
+  w->aux1 = w->aux2 = w->aux3 = NULL;
+  w->dtdx1d = (double *)calloc(L, sizeof(double));
+}
+
+static void ws_aux(slice_ws *w, int maux) {
+  w->maux = maux;
+  w->aux1 = (double *)calloc((long)maux * w->len, sizeof(double));
+  w->aux2 = (double *)calloc((long)maux * w->len, sizeof(double));
+  w->aux3 = (double *)calloc((long)maux * w->len, sizeof(double));
 }
 
 static void ws_free(slice_ws *w) {
   free(w->wave); free(w->s); free(w->amdq); free(w->apdq); free(w->cqxx);
   free(w->bmasdq); free(w->bpasdq); free(w->qadd); free(w->fadd); free(w->gadd);
-  free(w->aux2);
+  free(w->aux1); free(w->aux2); free(w->aux3); free(w->dtdx1d);
 }
 
 #define IX(i) ((i) + ng - 1)
@@ -652,10 +701,13 @@ static void slice_rpn(slice_ws *w, int ixy, int mx, const double *q1d, const orc
   }
 }
 
-/* classic/flux2.f90:2-207.  dtdx1d is constant along the slice (no capacity). */
-static double flux2(slice_ws *w, int ixy, int mx, const double *q1d, double dtdx,
-                    const int *method, const int *mthlim, int use_fwave, const orc_rp *p) {
-  const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng;
+/* classic/flux2.f90:2-207 with dtdx1d(i) = w->dtdx1d (dtdx for every cell
+ * without a capacity function) and, for solvers with aux, the slices
+ * aux1/aux2/aux3 (auxl = auxr = aux2 for rpn2, flux2.f90:101). */
+#define DT1(i) w->dtdx1d[IX(i)]
+static double flux2(slice_ws *w, int ixy, int mx, const double *q1d, const int *method,
+                    const int *mthlim, int use_fwave, const orc_rp *p) {
+  const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng, maux = w->maux;
   int limit = 0;
   for (int mw = 0; mw < mwaves; mw++)
     if (mthlim[mw] > 0) limit = 1;
@@ -669,18 +721,19 @@ static double flux2(slice_ws *w, int ixy, int mx, const double *q1d, double dtdx
   slice_rpn(w, ixy, mx, q1d, p);
   /* forall: the first statement completes for all i before the second runs */
   for (int i = 1; i <= mx + 1; i++)
-    for (int m = 0; m < meqn; m++) V1(w->qadd, m, i) = V1(w->qadd, m, i) - dtdx * V1(w->apdq, m, i);
+    for (int m = 0; m < meqn; m++)
+      V1(w->qadd, m, i) = V1(w->qadd, m, i) - DT1(i) * V1(w->apdq, m, i);
   for (int i = 1; i <= mx + 1; i++)
     for (int m = 0; m < meqn; m++)
-      V1(w->qadd, m, i - 1) = V1(w->qadd, m, i - 1) - dtdx * V1(w->amdq, m, i);
+      V1(w->qadd, m, i - 1) = V1(w->qadd, m, i - 1) - DT1(i - 1) * V1(w->amdq, m, i);
   double cfl1d = 0.0;
   for (int mw = 0; mw < mwaves; mw++)
     for (int i = 1; i <= mx + 1; i++)
-      cfl1d = dmax(dmax(cfl1d, dtdx * SV(mw, i)), -dtdx * SV(mw, i));
+      cfl1d = dmax(dmax(cfl1d, DT1(i) * SV(mw, i)), -DT1(i - 1) * SV(mw, i));
   if (method[1] != 1) {
     if (limit) limiter(w, mx, mthlim);
     for (int i = 2 - ng; i <= mx + ng; i++) {
-      double dtdxave = 0.5 * (dtdx + dtdx);
+      double dtdxave = 0.5 * (DT1(i - 1) + DT1(i));
       for (int m = 0; m < meqn; m++) {
         V1(w->cqxx, m, i) = 0.0;
         for (int mw = 0; mw < mwaves; mw++) {
@@ -700,24 +753,29 @@ static double flux2(slice_ws *w, int ixy, int mx, const double *q1d, double dtdx
         V1(w->apdq, m, i) = V1(w->apdq, m, i) - V1(w->cqxx, m, i);
       }
   }
-  for (int i = 2 - ng; i <= mx + ng; i++)
-    orc_rpt(p, ixy, 1, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &V1(w->amdq, 0, i),
-            &V1(w->bmasdq, 0, i), &V1(w->bpasdq, 0, i));
-  for (int i = 1; i <= mx + 1; i++)
-    for (int m = 0; m < meqn; m++) {
-      GA(m, 0, i - 1) = GA(m, 0, i - 1) - 0.5 * dtdx * V1(w->bmasdq, m, i);
-      GA(m, 1, i - 1) = GA(m, 1, i - 1) - 0.5 * dtdx * V1(w->bpasdq, m, i);
+  for (int imp = 1; imp <= 2; imp++) {
+    double *asdq = imp == 1 ? w->amdq : w->apdq;
+    for (int i = 2 - ng; i <= mx + ng; i++) {
+      if (maux > 0)
+        orc_rpt_aux(p, ixy, imp, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), maux,
+                    &w->aux1[(long)maux * IX(i - 1)], &w->aux2[(long)maux * IX(i - 1)],
+                    &w->aux3[(long)maux * IX(i - 1)], &V1(asdq, 0, i), &V1(w->bmasdq, 0, i),
+                    &V1(w->bpasdq, 0, i));
+      else
+        orc_rpt(p, ixy, imp, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &V1(asdq, 0, i),
+                &V1(w->bmasdq, 0, i), &V1(w->bpasdq, 0, i));
     }
-  for (int i = 2 - ng; i <= mx + ng; i++)
-    orc_rpt(p, ixy, 2, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &V1(w->apdq, 0, i),
-            &V1(w->bmasdq, 0, i), &V1(w->bpasdq, 0, i));
-  for (int i = 1; i <= mx + 1; i++)
-    for (int m = 0; m < meqn; m++) {
-      GA(m, 0, i) = GA(m, 0, i) - 0.5 * dtdx * V1(w->bmasdq, m, i);
-      GA(m, 1, i) = GA(m, 1, i) - 0.5 * dtdx * V1(w->bpasdq, m, i);
-    }
+    /* imp 1 modifies the fluxes of cell i-1, imp 2 those of cell i */
+    const int off = imp == 1 ? 1 : 0;
+    for (int i = 1; i <= mx + 1; i++)
+      for (int m = 0; m < meqn; m++) {
+        GA(m, 0, i - off) = GA(m, 0, i - off) - 0.5 * DT1(i - off) * V1(w->bmasdq, m, i);
+        GA(m, 1, i - off) = GA(m, 1, i - off) - 0.5 * DT1(i - off) * V1(w->bpasdq, m, i);
+      }
+  }
   return cfl1d;
 }
+#undef DT1
 
 /* flux3.f90:168-263 with method(3) < 0 (dimensional splitting): the 0.5 is
  * folded into cqxx and fadd += cqxx (flux3.f90:244,257). */
@@ -762,44 +820,89 @@ static double flux3_split(slice_ws *w, int ixyz, int mx, const double *q1d, doub
 /* 2D array access q(m, i, j) with ghosts, Fortran order. */
 #define Q2(a, m, i, j) (a)[(m) + (long)meqn * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
 
-/* classic/step2.f90:2-250 (index_capa = 0) */
+/* classic/step2.f90:2-250, with the capacity branches (index_capa =
+ * method[5] > 0: dtdx1d = dtdx/kappa per cell, 101-105 and 183-187; updates
+ * divided by the target cell's kappa, 152-163 and 232-244) and the aux slices
+ * aux1/aux2/aux3 = aux(:,:,j-1), aux(:,:,j), aux(:,:,j+1) (107-115; y-slices
+ * aux(:,i-1,:), aux(:,i,:), aux(:,i+1,:), 189-197). */
 double orc_step2(int meqn, int mwaves, int ng, int mx, int my, const double *qold,
                  double *qnew, double dx, double dy, double dt, const int *method,
-                 const int *mthlim, int use_fwave, const orc_rp *p) {
+                 const int *mthlim, int use_fwave, const orc_rp *p, int maux,
+                 const double *aux) {
   int maxm = mx > my ? mx : my;
   slice_ws W, *w = &W;
   ws_alloc(w, meqn, mwaves, ng, maxm);
+  if (maux > 0 && aux) ws_aux(w, maux);
+  const int capa = method[5];
+#define AUX2(ma, i, j) aux[(ma) + (long)maux * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
+#define KAP(i, j) AUX2(capa - 1, i, j)
   double *q1d = (double *)calloc((long)(maxm + 2 * ng) * meqn, sizeof(double));
   double cfl = 0.0;
   double dtdx = dt / dx, dtdy = dt / dy;
   for (int j = 0; j <= my + 1; j++) {
     for (int i = 1 - ng; i <= mx + ng; i++)
       for (int m = 0; m < meqn; m++) V1(q1d, m, i) = Q2(qold, m, i, j);
-    double cfl1d = flux2(w, 1, mx, q1d, dtdx, method, mthlim, use_fwave, p);
+    for (int i = 1 - ng; i <= mx + ng; i++)
+      w->dtdx1d[IX(i)] = capa > 0 ? dtdx / KAP(i, j) : dtdx;
+    if (w->maux > 0)
+      for (int i = 1 - ng; i <= mx + ng; i++)
+        for (int ma = 0; ma < maux; ma++) {
+          w->aux1[ma + (long)maux * IX(i)] = AUX2(ma, i, j - 1);
+          w->aux2[ma + (long)maux * IX(i)] = AUX2(ma, i, j);
+          w->aux3[ma + (long)maux * IX(i)] = AUX2(ma, i, j + 1);
+        }
+    double cfl1d = flux2(w, 1, mx, q1d, method, mthlim, use_fwave, p);
     cfl = dmax(cfl, cfl1d);
     for (int m = 0; m < meqn; m++)
       for (int i = 1; i <= mx; i++) {
-        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, i)
-                            - dtdx * (V1(w->fadd, m, i + 1) - V1(w->fadd, m, i))
-                            - dtdy * (GA(m, 1, i) - GA(m, 0, i));
-        Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) - dtdy * GA(m, 0, i);
-        Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) + dtdy * GA(m, 1, i);
+        if (capa == 0) {
+          Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, i)
+                              - dtdx * (V1(w->fadd, m, i + 1) - V1(w->fadd, m, i))
+                              - dtdy * (GA(m, 1, i) - GA(m, 0, i));
+          Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) - dtdy * GA(m, 0, i);
+          Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) + dtdy * GA(m, 1, i);
+        } else {
+          Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, i)
+                              - (dtdx * (V1(w->fadd, m, i + 1) - V1(w->fadd, m, i))
+                                 + dtdy * (GA(m, 1, i) - GA(m, 0, i))) / KAP(i, j);
+          Q2(qnew, m, i, j - 1) = Q2(qnew, m, i, j - 1) - dtdy * GA(m, 0, i) / KAP(i, j - 1);
+          Q2(qnew, m, i, j + 1) = Q2(qnew, m, i, j + 1) + dtdy * GA(m, 1, i) / KAP(i, j + 1);
+        }
       }
   }
   for (int i = 0; i <= mx + 1; i++) {
     for (int j = 1 - ng; j <= my + ng; j++)
       for (int m = 0; m < meqn; m++) V1(q1d, m, j) = Q2(qold, m, i, j);
-    double cfl1d = flux2(w, 2, my, q1d, dtdy, method, mthlim, use_fwave, p);
+    for (int j = 1 - ng; j <= my + ng; j++)
+      w->dtdx1d[IX(j)] = capa > 0 ? dtdy / KAP(i, j) : dtdy;
+    if (w->maux > 0)
+      for (int j = 1 - ng; j <= my + ng; j++)
+        for (int ma = 0; ma < maux; ma++) {
+          w->aux1[ma + (long)maux * IX(j)] = AUX2(ma, i - 1, j);
+          w->aux2[ma + (long)maux * IX(j)] = AUX2(ma, i, j);
+          w->aux3[ma + (long)maux * IX(j)] = AUX2(ma, i + 1, j);
+        }
+    double cfl1d = flux2(w, 2, my, q1d, method, mthlim, use_fwave, p);
     cfl = dmax(cfl, cfl1d);
     for (int m = 0; m < meqn; m++)
       for (int j = 1; j <= my; j++) {
-        Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + (V1(w->qadd, m, j)
-                            - dtdy * (V1(w->fadd, m, j + 1) - V1(w->fadd, m, j))
-                            - dtdx * (GA(m, 1, j) - GA(m, 0, j)));
-        Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) - dtdx * GA(m, 0, j);
-        Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) + dtdx * GA(m, 1, j);
+        if (capa == 0) {
+          Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + (V1(w->qadd, m, j)
+                              - dtdy * (V1(w->fadd, m, j + 1) - V1(w->fadd, m, j))
+                              - dtdx * (GA(m, 1, j) - GA(m, 0, j)));
+          Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) - dtdx * GA(m, 0, j);
+          Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) + dtdx * GA(m, 1, j);
+        } else {
+          Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, j)
+                              - (dtdy * (V1(w->fadd, m, j + 1) - V1(w->fadd, m, j))
+                                 + dtdx * (GA(m, 1, j) - GA(m, 0, j))) / KAP(i, j);
+          Q2(qnew, m, i - 1, j) = Q2(qnew, m, i - 1, j) - dtdx * GA(m, 0, j) / KAP(i - 1, j);
+          Q2(qnew, m, i + 1, j) = Q2(qnew, m, i + 1, j) + dtdx * GA(m, 1, j) / KAP(i + 1, j);
+        }
       }
   }
+#undef KAP
+#undef AUX2
   free(q1d);
   ws_free(w);
   return cfl;
@@ -814,14 +917,12 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
   int maxm = mx > my ? mx : my;
   slice_ws W, *w = &W;
   ws_alloc(w, meqn, mwaves, ng, maxm);
-  if (maux > 0 && aux) {
-    w->maux = maux;
-    w->aux2 = (double *)calloc((long)maux * w->len, sizeof(double));
-  }
+  if (maux > 0 && aux) ws_aux(w, maux);
 #define AUX2(ma, i, j) aux[(ma) + (long)maux * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
   double *q1d = (double *)calloc((long)(maxm + 2 * ng) * meqn, sizeof(double));
   double cfl = 0.0;
   double dtdx = dt / dx, dtdy = dt / dy;
+  for (int i = 0; i < w->len; i++) w->dtdx1d[i] = ids == 1 ? dtdx : dtdy;
   if (ids == 1) {
     for (int j = 1 - ng; j <= my + ng; j++) {
       for (int i = 1 - ng; i <= mx + ng; i++)
@@ -829,7 +930,7 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
       if (w->maux > 0)   /* step2ds.f90:110-114: aux2 = aux(:, :, j) */
         for (int i = 1 - ng; i <= mx + ng; i++)
           for (int ma = 0; ma < maux; ma++) w->aux2[ma + (long)maux * IX(i)] = AUX2(ma, i, j);
-      double cfl1d = flux2(w, 1, mx, q1d, dtdx, method, mthlim, use_fwave, p);
+      double cfl1d = flux2(w, 1, mx, q1d, method, mthlim, use_fwave, p);
       cfl = dmax(cfl, cfl1d);
       for (int i = 1; i <= mx; i++)
         for (int m = 0; m < meqn; m++)
@@ -843,7 +944,7 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
       if (w->maux > 0)   /* step2ds.f90:195-199: aux2 = aux(:, i, :) */
         for (int j = 1 - ng; j <= my + ng; j++)
           for (int ma = 0; ma < maux; ma++) w->aux2[ma + (long)maux * IX(j)] = AUX2(ma, i, j);
-      double cfl1d = flux2(w, 2, my, q1d, dtdy, method, mthlim, use_fwave, p);
+      double cfl1d = flux2(w, 2, my, q1d, method, mthlim, use_fwave, p);
       cfl = dmax(cfl, cfl1d);
       for (int j = 1; j <= my; j++)
         for (int m = 0; m < meqn; m++)

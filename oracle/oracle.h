@@ -24,8 +24,10 @@ enum {
   ORC_RP_SHALLOW = 3,    /* shallow_roe_with_efix_2D: meqn 3, mwaves 3 */
   ORC_RP_ADVECTION = 4,  /* advection_2D:            meqn 1, mwaves 1 */
   ORC_RP_EULER3 = 5,     /* euler_3D:                meqn 5, mwaves 3 */
-  ORC_RP_VC_ADVECTION = 6 /* examples/advection_reaction_2d/rpn2_vc_advection_vector.f90:
-                           * meqn = orc_rp.meqn, mwaves 1, 2 aux fields (edge u, v) */
+  ORC_RP_VC_ADVECTION = 6, /* examples/advection_reaction_2d/rpn2_vc_advection_vector.f90:
+                            * meqn = orc_rp.meqn, mwaves 1, 2 aux fields (edge u, v) */
+  ORC_RP_VC_ADVECTION_2D = 7 /* clawpack.riemann vc_advection_2D: the same rpn2 with
+                              * meqn 1, and its transverse solver rpt2 */
 };
 
 /* Problem parameters: the reference passes these through the Fortran /cparam/
@@ -56,6 +58,15 @@ void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *
 void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
                  const double *auxL, const double *auxR, double *wave, double *s,
                  double *amdq, double *apdq);
+/* The transverse split with aux (the rpt2 ABI's aux1/aux2/aux3 slices):
+ * a1, a2, a3 point at aux(:, i-1) of the previous, this and the next slice
+ * (maux values per cell); the solver reads cell i1 = i-2+imp of them. */
+void orc_rpt_aux(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
+                 int maux, const double *a1, const double *a2, const double *a3,
+                 const double *asdq, double *bmasdq, double *bpasdq);
+/* Test hook: k = 1 plants a mistake in the vc_advection_2D transverse solver
+ * (the normal velocity in place of the transverse one), 0 removes it. */
+void orc_set_plant(int k);
 int orc_rp_meqn(int kind);
 int orc_rp_mwaves(int kind);
 
@@ -72,9 +83,12 @@ void orc_bc3(double *qbc, int meqn, int mx, int my, int mz, int ng,
  * method[7] and mthlim[mwaves] as built by ClawSolver._set_method /
  * _set_mthlim (classic/solver.py:162-193).  qold and qnew may alias for
  * step2ds(ids=2) and step3ds(idir=2,3), exactly as the reference calls them. */
+/* aux: NULL (maux = 0) or aux(maux, 1-ng:mx+ng, 1-ng:my+ng), ghosts filled;
+ * method[5] = index_capa (1-based, 0: none) names the capacity field. */
 double orc_step2(int meqn, int mwaves, int ng, int mx, int my,
                  const double *qold, double *qnew, double dx, double dy, double dt,
-                 const int *method, const int *mthlim, int use_fwave, const orc_rp *p);
+                 const int *method, const int *mthlim, int use_fwave, const orc_rp *p,
+                 int maux, const double *aux);
 /* aux: NULL (maux = 0) or aux(maux, 1-ng:mx+ng, 1-ng:my+ng), ghosts filled. */
 double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my,
                    const double *qold, double *qnew, double dx, double dy, double dt,

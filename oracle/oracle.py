@@ -22,8 +22,9 @@ ACOUSTICS, EULER, SHALLOW, ADVECTION, EULER3 = 1, 2, 3, 4, 5
 CUSTOM, EXTRAP, PERIODIC, WALL = 0, 1, 2, 3
 
 MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5}
-MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1}
+MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1, 7: 1}
 VC_ADVECTION = 6
+VC_ADVECTION_2D = 7
 
 
 class RP(ctypes.Structure):
@@ -98,9 +99,12 @@ def ref_sharp():
     return _ref_sharp
 
 
-def method_array(order=2, transverse=2, dimensional_split=False, dt_variable=True, num_aux=0):
-    """ClawSolver._set_method (classic/solver.py:173-193) for 2D/3D, no capacity."""
-    m = [int(dt_variable), order, -1 if dimensional_split else transverse, 0, 0, 0, int(num_aux)]
+def method_array(order=2, transverse=2, dimensional_split=False, dt_variable=True, num_aux=0,
+                 index_capa=-1):
+    """ClawSolver._set_method (classic/solver.py:173-193) for 2D/3D; index_capa is
+    the 0-based capacity field (-1: none), stored 1-based in method(6)."""
+    m = [int(dt_variable), order, -1 if dimensional_split else transverse, 0, 0,
+         int(index_capa) + 1, int(num_aux)]
     return m
 
 
@@ -136,15 +140,17 @@ def interior(qbc, ng):
 # ---------------------------------------------------------------- classic
 def step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0, use_fwave=False, aux=None):
     """ids=0: unsplit step2; ids=1/2: step2ds sweep.  qnew updated in place.
-    aux: aux(maux, 1-ng:mx+ng, 1-ng:my+ng) with ghosts filled (split sweeps)."""
+    aux: aux(maux, 1-ng:mx+ng, 1-ng:my+ng) with ghosts filled; method[5] =
+    index_capa (1-based) names a capacity field (unsplit)."""
     meqn, nx, ny = qold.shape
     mx, my = nx - 2 * ng, ny - 2 * ng
     mw = MWAVES[rp.kind]
+    maux = 0 if aux is None else aux.shape[0]
     if ids == 0:
         return lib().orc_step2(meqn, mw, ng, mx, my, _dp(qold), _dp(qnew), ctypes.c_double(dx),
                                ctypes.c_double(dy), ctypes.c_double(dt), _ip(method),
-                               _ip(mthlim), int(use_fwave), ctypes.byref(rp))
-    maux = 0 if aux is None else aux.shape[0]
+                               _ip(mthlim), int(use_fwave), ctypes.byref(rp), int(maux),
+                               _dp(aux) if aux is not None else None)
     return lib().orc_step2ds(meqn, mw, ng, mx, my, _dp(qold), _dp(qnew), ctypes.c_double(dx),
                              ctypes.c_double(dy), ctypes.c_double(dt), _ip(method), _ip(mthlim),
                              int(ids), int(use_fwave), ctypes.byref(rp), int(maux),
@@ -239,6 +245,17 @@ def ref_step2ds_aux(rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, ids, 
     return f(rpn2, int(ids), meqn, int(mwaves), aux.shape[0], ng, nx - 2 * ng, ny - 2 * ng,
              _dp(qold), _dp(qnew), _dp(aux), ctypes.c_double(dx), ctypes.c_double(dy),
              ctypes.c_double(dt), _ip(method), _ip(mthlim))
+
+
+def ref_step2_aux(rp, rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, mwaves=1):
+    """The reference's unsplit step2 with aux fields (and a capacity field when
+    method[5] > 0), the given rpn2 address and the oracle's rpt2 for rp."""
+    meqn, nx, ny = qold.shape
+    f = ref_classic().ref_step2_aux
+    f.restype = ctypes.c_double
+    return f(ctypes.byref(rp), rpn2, meqn, int(mwaves), aux.shape[0], ng, nx - 2 * ng,
+             ny - 2 * ng, _dp(qold), _dp(qnew), _dp(aux), ctypes.c_double(dx),
+             ctypes.c_double(dy), ctypes.c_double(dt), _ip(method), _ip(mthlim))
 
 
 def ref_step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir):

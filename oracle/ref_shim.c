@@ -56,6 +56,22 @@ void shim_rpt2_(const int *ixy, const int *imp, const int *maxm, const int *meqn
   }
 }
 
+/* rpt2 for solvers with aux: aux1/aux2/aux3 are the three slices
+ * (maux values per cell); problem i's solver reads cell i1 = i-2+imp. */
+void shim_rpt2_aux_(const int *ixy, const int *imp, const int *maxm, const int *meqn,
+                    const int *mwaves, const int *maux, const int *mbc, const int *mx,
+                    const double *ql, const double *qr, const double *aux1, const double *aux2,
+                    const double *aux3, const double *asdq, double *bmasdq, double *bpasdq) {
+  (void)maxm; (void)mwaves;
+  const int me = *meqn, ng = *mbc, ma = *maux;
+  for (int i = 2 - ng; i <= *mx + ng; i++) {
+    long ci = i + ng - 1, cm = ci - 1;
+    orc_rpt_aux(&g_rp, *ixy, *imp, qr + (long)me * cm, ql + (long)me * ci, ma,
+                aux1 + (long)ma * cm, aux2 + (long)ma * cm, aux3 + (long)ma * cm,
+                asdq + (long)me * ci, bmasdq + (long)me * ci, bpasdq + (long)me * ci);
+  }
+}
+
 /* rpn3 has the rpn2 argument list (flux3.f90:201-202). */
 void shim_rpn3_(const int *ixyz, const int *maxm, const int *meqn, const int *mwaves,
                 const int *maux, const int *mbc, const int *mx, const double *ql,
@@ -123,6 +139,29 @@ double ref_step2(const orc_rp *p, int ids, int meqn, int mwaves, int ng, int mx,
     step2ds_(&maxm, &meqn, &mwaves, &maux, &ng, &mx, &my, qold, qnew, aux, &dx, &dy, &dt,
              method, mthlim, &cfl, qadd, fadd, gadd, q1d, dtdx1d, dtdy1d, aux1, aux2, aux3,
              work, &mwork, &ids, &use_fwave, (rp_fn)shim_rpn2_, (rp_fn)shim_rpt2_);
+  free(buf);
+  return cfl;
+}
+
+/* The unsplit step2 with aux fields (method[6] = maux; method[5] = index_capa
+ * for a capacity field) and a caller-supplied rpn2 (the advection_reaction_2d
+ * example's, compiled from its source); rpt2 from the oracle's solver p. */
+double ref_step2_aux(const orc_rp *p, rp_fn rpn2, int meqn, int mwaves, int maux, int ng,
+                     int mx, int my, const double *qold, double *qnew, const double *aux,
+                     double dx, double dy, double dt, const int *method, const int *mthlim) {
+  shim_set_rp(p);
+  int maxm = mx > my ? mx : my, use_fwave = 0;
+  long L = maxm + 2 * ng;
+  int mwork = (int)(L * (5 * meqn + mwaves + meqn * mwaves));
+  double *buf = (double *)calloc(L * (6 * meqn + 2 + 3 * maux) + mwork + 16, sizeof(double));
+  double *qadd = buf, *fadd = qadd + L * meqn, *gadd = fadd + L * meqn,
+         *q1d = gadd + 2 * L * meqn, *dtdx1d = q1d + L * meqn, *dtdy1d = dtdx1d + L,
+         *aux1 = dtdy1d + L, *aux2 = aux1 + L * maux, *aux3 = aux2 + L * maux,
+         *work = aux3 + L * maux;
+  double cfl = 0.0;
+  step2_(&maxm, &meqn, &mwaves, &maux, &ng, &mx, &my, qold, qnew, aux, &dx, &dy, &dt, method,
+         mthlim, &cfl, qadd, fadd, gadd, q1d, dtdx1d, dtdy1d, aux1, aux2, aux3, work, &mwork,
+         &use_fwave, rpn2, (rp_fn)shim_rpt2_aux_);
   free(buf);
   return cfl;
 }

@@ -82,7 +82,7 @@ _SIGS = {
     "clawhip_cfl_reset": [_P, _P],
     "clawhip_step2_unsplit": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                               ctypes.POINTER(Geom), _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
-                              _D, _D, _D, _P, _P],
+                              _D, _D, _D, _P, ctypes.POINTER(Geom), _IP, _IP, _P, _P],
     "clawhip_step2_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                             _D, _D, _D, _P, ctypes.POINTER(Geom), _IP, _IP, _P, _P],

@@ -82,14 +82,26 @@ class ClawSolver(Solver):
         state = solution.states[0]
         if self.rp is None or not hasattr(self.rp, "kind"):
             raise Exception("the solver needs one of the pyclaw_amd.riemann modules")
-        if state.index_capa >= 0:
-            raise NotImplementedError("capacity functions are not supported on the device")
         rp_aux = getattr(self.rp, "num_aux", 0)
-        if state.num_aux != rp_aux:
-            raise NotImplementedError(
-                "the Riemann solver takes %d aux fields, the state has %d" % (rp_aux, state.num_aux))
-        if rp_aux and not (self.num_dim == 2 and self.dimensional_split):
-            raise NotImplementedError("aux fields are supported by the dimension-split 2D step")
+        if rp_aux == 0 and state.num_aux != 0:
+            raise NotImplementedError("aux fields are supported with the variable-coefficient "
+                                      "advection solvers")
+        if rp_aux and not rp_aux <= state.num_aux <= 3:
+            raise NotImplementedError("the Riemann solver reads %d aux fields; the device steps "
+                                      "take %d to 3, the state has %d"
+                                      % (rp_aux, rp_aux, state.num_aux))
+        if rp_aux and self.num_dim != 2:
+            raise NotImplementedError("aux fields are supported by the 2D steps")
+        if state.index_capa >= 0:
+            if self.num_dim != 2 or self.dimensional_split or not rp_aux:
+                raise NotImplementedError("capacity functions are supported by the unsplit 2D "
+                                          "step with the variable-coefficient advection solvers")
+            if state.index_capa >= state.num_aux:
+                raise ValueError("index_capa %d is not an aux field" % state.index_capa)
+        if (getattr(self.rp, "kind", 0) == 6 and self.num_dim == 2 and not self.dimensional_split
+                and self.transverse_waves != 0):
+            raise NotImplementedError("vc_advection_vector_2D has no transverse solver: use "
+                                      "dimensional splitting or transverse_waves = 0")
         self._set_method(state)
         self._rp_struct = K.make_rp(self.rp.kind, state.problem_data, meqn=state.num_eqn)
         self._is_set_up = True
@@ -141,14 +153,13 @@ class ClawSolver2D(ClawSolver):
         K.cfl_reset(cfl_dev)
         gin = None
         src = qin
+        aux = alo = ahi = None
+        if state.num_aux > 0:
+            aux, alo, ahi = self._aux_input(state)
         if BC.custom in lo or BC.custom in hi:
             src, gin = self._custom_bc_input(state, lo, hi)
             lo, hi = [4, 4], [4, 4]
         if self.dimensional_split:
-            aux = alo = ahi = None
-            if state.num_aux > 0:
-                aux = state.device_aux(self._dtype_code)
-                alo, ahi = self._aux_bc_codes()
             self._work = K.step2_split(self._rp_struct, src[0], out[0], cfl_dev, lo, hi,
                                        self.reflect_index, self._method, self._mthlim, dx, dy,
                                        self.dt, comp_in=src[1], comp_out=out[1], gin=gin,
@@ -157,7 +168,8 @@ class ClawSolver2D(ClawSolver):
         else:
             K.step2_unsplit(self._rp_struct, src[0], out[0], cfl_dev, lo, hi, self.reflect_index,
                             self._method, self._mthlim, dx, dy, self.dt, comp_in=src[1],
-                            comp_out=out[1], gin=gin, num_ghost=self.num_ghost)
+                            comp_out=out[1], gin=gin, num_ghost=self.num_ghost, aux=aux,
+                            aux_bc_lower=alo, aux_bc_upper=ahi)
         cfl = float(cfl_dev.item())
         self.cfl.update_global_max(cfl)
         state.set_device_q(*out)
@@ -167,7 +179,8 @@ class ClawSolver2D(ClawSolver):
     def _device_loop_ok(self, solution):
         lo, hi = self._bc_codes()
         return (self.device_loop and not self.dimensional_split and self.before_step is None
-                and self.step_source is None and BC.custom not in lo and BC.custom not in hi)
+                and self.step_source is None and BC.custom not in lo and BC.custom not in hi
+                and solution.states[0].num_aux == 0)
 
     def evolve_to_time(self, solution, tend=None):
         if not self._is_set_up:
@@ -281,6 +294,22 @@ class ClawSolver2D(ClawSolver):
             self._graphs[key] = ent
         ent["graph"].replay()
 
+    def _aux_input(self, state):
+        """Device aux and its boundary codes.  With a custom aux BC the ghosts are
+        filled on the host in the reference's order (solver.py:371-437; the user
+        function gets qbc = None: q lives on the device) and stored (code 4)."""
+        from .device import q_to_device
+        alo, ahi = self._aux_bc_codes()
+        if BC.custom not in alo and BC.custom not in ahi:
+            return state.device_aux(self._dtype_code), alo, ahi
+        ng = self.num_ghost
+        aux = np.asarray(state.aux)
+        auxbc = np.zeros((aux.shape[0],) + tuple(n + 2 * ng for n in aux.shape[1:]), order="F")
+        auxbc[(slice(None),) + (slice(ng, -ng),) * self.num_dim] = aux
+        self.auxbc = auxbc
+        _fill_bcs_host(self, state, auxbc, alo, ahi, aux=True)
+        return q_to_device(auxbc, self._dtype_code), [4] * self.num_dim, [4] * self.num_dim
+
     def _custom_bc_input(self, state, lo, hi):
         """User BC functions act on a host qbc (solver.py:393-398); the kernel then
         reads the stored ghosts (BC_GIVEN)."""
@@ -348,18 +377,23 @@ def ctypes_memmove(struct, raw):
     ctypes.memmove(ctypes.addressof(struct), raw, ctypes.sizeof(struct))
 
 
-def _fill_bcs_host(solver, state, qbc, lo, hi):
-    """Solver._apply_bcs on a host qbc (solver.py:351-519), for custom BCs."""
+def _fill_bcs_host(solver, state, qbc, lo, hi, aux=False):
+    """Solver._apply_bcs on a host qbc (solver.py:351-519), for custom BCs; with
+    aux=True the array is auxbc: the solver's aux functions, walls not negated."""
     ng = solver.num_ghost
     for idim, dim in enumerate(state.grid.dimensions):
         for upper, codes in ((False, lo), (True, hi)):
             b = codes[idim]
             if b == BC.custom:
-                fun = solver.user_bc_upper if upper else solver.user_bc_lower
-                fun(state, dim, state.t, qbc, solver.auxbc, ng)
+                if aux:
+                    fun = solver.user_aux_bc_upper if upper else solver.user_aux_bc_lower
+                    fun(state, dim, state.t, None, qbc, ng)
+                else:
+                    fun = solver.user_bc_upper if upper else solver.user_bc_lower
+                    fun(state, dim, state.t, qbc, solver.auxbc, ng)
                 continue
             arr = np.moveaxis(qbc, idim + 1, 1)
-            r = solver.reflect_index[idim]
+            r = -1 if aux else solver.reflect_index[idim]
             for i in range(ng):
                 if not upper:
                     if b == BC.extrap:
@@ -368,7 +402,8 @@ def _fill_bcs_host(solver, state, qbc, lo, hi):
                         arr[:, i, ...] = arr[:, -2 * ng + i, ...]
                     elif b == BC.wall:
                         arr[:, i, ...] = arr[:, 2 * ng - 1 - i, ...]
-                        arr[r, i, ...] = -arr[r, 2 * ng - 1 - i, ...]
+                        if r >= 0:
+                            arr[r, i, ...] = -arr[r, 2 * ng - 1 - i, ...]
                 else:
                     if b == BC.extrap:
                         arr[:, -i - 1, ...] = arr[:, -ng - 1, ...]
@@ -376,4 +411,5 @@ def _fill_bcs_host(solver, state, qbc, lo, hi):
                         arr[:, -ng + i, ...] = arr[:, ng + i, ...]
                     elif b == BC.wall:
                         arr[:, -i - 1, ...] = arr[:, -2 * ng + i, ...]
-                        arr[r, -i - 1, ...] = -arr[r, -2 * ng + i, ...]
+                        if r >= 0:
+                            arr[r, -i - 1, ...] = -arr[r, -2 * ng + i, ...]

@@ -45,65 +45,6 @@ int clawhip_cfl_reset(double* cfl_dev, void* stream) {
   return e == hipSuccess ? 0 : (int)e;
 }
 
-int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
-                          const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
-                          const void* comp_in, void* comp_out,
-                          const int* bc_lower, const int* bc_upper,
-                          const int* reflect, int num_ghost, const int* method,
-                          const int* mthlim, double dx, double dy, double dt,
-                          double* cfl_dev, void* stream) {
-  if (!rp || !q_in || !q_out || !cfl_dev || !bc_lower || !bc_upper || !reflect || !method ||
-      !mthlim)
-    return CLAWHIP_E_ARG;
-  if (!geom_ok(gin, 2) || !geom_ok(gout, 2) || gout->ghosts != 0) return CLAWHIP_E_ARG;
-  if (gin->mx != gout->mx || gin->my != gout->my) return CLAWHIP_E_ARG;
-  if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
-  if (q_in == q_out) return CLAWHIP_E_ARG;
-  if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
-  if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
-  if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
-  if (method[2] < 0 || method[2] > 2) return CLAWHIP_E_METHOD;   // unsplit only
-  if (method[5] != 0 || method[6] != 0) return CLAWHIP_E_METHOD;  // no capa / aux
-  for (int d = 0; d < 2; d++) {
-    int lo = bc_lower[d], hi = bc_upper[d];
-    if (lo < 1 || lo > 4 || hi < 1 || hi > 4) return CLAWHIP_E_ARG;
-    if ((lo == CLAWHIP_BC_GIVEN || hi == CLAWHIP_BC_GIVEN) && gin->ghosts < num_ghost)
-      return CLAWHIP_E_ARG;
-  }
-  return launch_unsplit(*rp, dtype, q_in, to_geom(gin), q_out, to_geom(gout), comp_in, comp_out,
-                        bc_lower,
-                        bc_upper, reflect, num_ghost, method, mthlim, dx, dy, dt,
-                        (unsigned long long*)cfl_dev, (hipStream_t)stream);
-}
-
-static int check_common(const clawhip_rp* rp, int dtype, const void* q_in, const void* q_out,
-                        const void* comp_in, const void* comp_out, double* cfl_dev,
-                        const int* bc_lower, const int* bc_upper, const int* reflect,
-                        const int* method, const int* mthlim, int ndim, int num_ghost,
-                        const clawhip_geom* gin, const clawhip_geom* gout, int num_aux = 0) {
-  if (!rp || !q_in || !q_out || !cfl_dev || !bc_lower || !bc_upper || !reflect || !method ||
-      !mthlim)
-    return CLAWHIP_E_ARG;
-  if (!geom_ok(gin, ndim) || !geom_ok(gout, ndim) || gout->ghosts != 0) return CLAWHIP_E_ARG;
-  if (gin->mx != gout->mx || gin->my != gout->my) return CLAWHIP_E_ARG;
-  if (ndim == 3 && gin->mz != gout->mz) return CLAWHIP_E_ARG;
-  if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
-  if (ndim == 3 && gout->mz < num_ghost) return CLAWHIP_E_ARG;
-  if (q_in == q_out) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
-  if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
-  if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
-  if (method[5] != 0 || method[6] != num_aux) return CLAWHIP_E_METHOD;
-  for (int d = 0; d < ndim; d++) {
-    int lo = bc_lower[d], hi = bc_upper[d];
-    if (lo < 1 || lo > 4 || hi < 1 || hi > 4) return CLAWHIP_E_ARG;
-    if ((lo == CLAWHIP_BC_GIVEN || hi == CLAWHIP_BC_GIVEN) && gin->ghosts < num_ghost)
-      return CLAWHIP_E_ARG;
-  }
-  return 0;
-}
-
 // aux: NULL, or the solver's aux fields (planar like q, geometry gaux with the
 // interior of q; stored ghosts only with CLAWHIP_BC_GIVEN aux codes).
 static int aux_in(const void* aux, const clawhip_geom* gaux, const int* alo, const int* ahi,
@@ -125,7 +66,78 @@ static int aux_in(const void* aux, const clawhip_geom* gaux, const int* alo, con
 }
 
 static int rp_num_aux(const clawhip_rp* rp) {
-  return rp->kind == CLAWHIP_RP_VC_ADVECTION ? 2 : 0;
+  return rp->kind == CLAWHIP_RP_VC_ADVECTION || rp->kind == CLAWHIP_RP_VC_ADVECTION_2D ? 2 : 0;
+}
+
+int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
+                          const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
+                          const void* comp_in, void* comp_out,
+                          const int* bc_lower, const int* bc_upper,
+                          const int* reflect, int num_ghost, const int* method,
+                          const int* mthlim, double dx, double dy, double dt,
+                          const void* aux, const clawhip_geom* gaux, const int* aux_bc_lower,
+                          const int* aux_bc_upper, double* cfl_dev, void* stream) {
+  if (!rp || !q_in || !q_out || !cfl_dev || !bc_lower || !bc_upper || !reflect || !method ||
+      !mthlim)
+    return CLAWHIP_E_ARG;
+  if (!geom_ok(gin, 2) || !geom_ok(gout, 2) || gout->ghosts != 0) return CLAWHIP_E_ARG;
+  if (gin->mx != gout->mx || gin->my != gout->my) return CLAWHIP_E_ARG;
+  if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
+  if (q_in == q_out) return CLAWHIP_E_ARG;
+  if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
+  if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
+  if (method[2] < 0 || method[2] > 2) return CLAWHIP_E_METHOD;   // unsplit only
+  // aux fields: the solver's own (velocities first) and at most one more; a
+  // capacity field (method[5] = index_capa + 1) is one of them
+  const int na = rp_num_aux(rp);
+  if (na == 0 ? method[6] != 0 : (method[6] < na || method[6] > 3)) return CLAWHIP_E_METHOD;
+  if (method[5] < 0 || method[5] > method[6]) return CLAWHIP_E_METHOD;
+  if (rp->kind == CLAWHIP_RP_VC_ADVECTION && method[2] != 0) return CLAWHIP_E_METHOD;
+  for (int d = 0; d < 2; d++) {
+    int lo = bc_lower[d], hi = bc_upper[d];
+    if (lo < 1 || lo > 4 || hi < 1 || hi > 4) return CLAWHIP_E_ARG;
+    if ((lo == CLAWHIP_BC_GIVEN || hi == CLAWHIP_BC_GIVEN) && gin->ghosts < num_ghost)
+      return CLAWHIP_E_ARG;
+  }
+  AuxIn ax;
+  int rc = aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, gout, num_ghost, ax);
+  if (rc) return rc;
+  if ((na > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
+  return launch_unsplit(*rp, dtype, q_in, to_geom(gin), q_out, to_geom(gout), comp_in, comp_out,
+                        bc_lower, bc_upper, reflect, num_ghost, method, mthlim, dx, dy, dt,
+                        (unsigned long long*)cfl_dev, (hipStream_t)stream, ax);
+}
+
+static int check_common(const clawhip_rp* rp, int dtype, const void* q_in, const void* q_out,
+                        const void* comp_in, const void* comp_out, double* cfl_dev,
+                        const int* bc_lower, const int* bc_upper, const int* reflect,
+                        const int* method, const int* mthlim, int ndim, int num_ghost,
+                        const clawhip_geom* gin, const clawhip_geom* gout, int num_aux = 0) {
+  if (!rp || !q_in || !q_out || !cfl_dev || !bc_lower || !bc_upper || !reflect || !method ||
+      !mthlim)
+    return CLAWHIP_E_ARG;
+  if (!geom_ok(gin, ndim) || !geom_ok(gout, ndim) || gout->ghosts != 0) return CLAWHIP_E_ARG;
+  if (gin->mx != gout->mx || gin->my != gout->my) return CLAWHIP_E_ARG;
+  if (ndim == 3 && gin->mz != gout->mz) return CLAWHIP_E_ARG;
+  if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
+  if (ndim == 3 && gout->mz < num_ghost) return CLAWHIP_E_ARG;
+  if (q_in == q_out) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
+  if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
+  // no capacity; a solver with aux fields may be given more than it reads
+  if (method[5] != 0) return CLAWHIP_E_METHOD;
+  if (num_aux == 0 ? method[6] != 0 : (method[6] < num_aux || method[6] > 3))
+    return CLAWHIP_E_METHOD;
+  for (int d = 0; d < ndim; d++) {
+    int lo = bc_lower[d], hi = bc_upper[d];
+    if (lo < 1 || lo > 4 || hi < 1 || hi > 4) return CLAWHIP_E_ARG;
+    if ((lo == CLAWHIP_BC_GIVEN || hi == CLAWHIP_BC_GIVEN) && gin->ghosts < num_ghost)
+      return CLAWHIP_E_ARG;
+  }
+  return 0;
 }
 
 int clawhip_step2_split(const clawhip_rp* rp, int dtype, const void* q_in,
@@ -295,7 +307,9 @@ static int check_sweep(const clawhip_rp* rp, int dtype, int ndim, int dir, const
                   gin->slab_stride != gout->slab_stride))
     return CLAWHIP_E_ARG;
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
-  if (method[2] != -1 || method[5] != 0 || method[6] != num_aux) return CLAWHIP_E_METHOD;
+  if (method[2] != -1 || method[5] != 0) return CLAWHIP_E_METHOD;
+  if (num_aux == 0 ? method[6] != 0 : (method[6] < num_aux || method[6] > 3))
+    return CLAWHIP_E_METHOD;
   return 0;
 }
 

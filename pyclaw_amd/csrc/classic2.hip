@@ -16,9 +16,9 @@
 
 namespace claw {
 
-template <class RS, typename T, bool KAHAN>
+template <class RS, typename T, bool KAHAN, bool AUX = false, bool CAPA = false>
 __global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
-  unsplit_body<RS, T, KAHAN>(a, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  unsplit_body<RS, T, KAHAN, AUX, CAPA>(a, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
 // Device-resident stepping: one launch = one attempted step of
@@ -88,12 +88,23 @@ int choose_rows(Kernel kernel, int nstrips, int my) {
   return best;
 }
 
-template <class RS, typename T, bool KAHAN>
+template <class RS, typename T, bool KAHAN, bool AUX = false, bool CAPA = false>
 int run_unsplit(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_out,
                 const Geom& gout, const void* comp_in, void* comp_out, const int* lo, const int* hi, const int* refl,
                 int ng, const int* method, const int* mthlim, double dx, double dy, double dt,
-                unsigned long long* cfl_bits, hipStream_t stream) {
+                unsigned long long* cfl_bits, hipStream_t stream, const AuxIn* aux = nullptr) {
   UnsplitArgs<T> a;
+  a.aux = nullptr;
+  a.naux = 0;
+  a.capa = -1;
+  if constexpr (AUX) {
+    a.aux = (const T*)aux->p;
+    a.gaux = aux->g;
+    const int never[3] = {-1, -1, -1};
+    a.bcaux = to_bc(aux->lo, aux->hi, never, 2);
+    a.naux = method[6];
+    a.capa = method[5] - 1;
+  }
   a.rp = to_params<T>(rp);
   a.gin = gin; a.gout = gout;
   a.bc = to_bc(lo, hi, refl, 2);
@@ -101,14 +112,14 @@ int run_unsplit(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q
   for (int w = 0; w < 4; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
   a.dtdx = T(dt / dx); a.dtdy = T(dt / dy);
   a.nstrips = (gout.mx + 59) / 60;
-  a.H = choose_rows(k_unsplit<RS, T, KAHAN>, a.nstrips, gout.my);
+  a.H = choose_rows(k_unsplit<RS, T, KAHAN, AUX, CAPA>, a.nstrips, gout.my);
   a.nrb = (gout.my + a.H - 1) / a.H;
   a.qin = (const T*)q_in; a.qout = (T*)q_out;
   a.comp_in = (const T*)comp_in; a.comp_out = (T*)comp_out;
   a.cfl_bits = cfl_bits;
   long long waves = (long long)a.nstrips * a.nrb;
   int blocks = (int)((waves + 3) / 4);
-  hipLaunchKernelGGL((k_unsplit<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL((k_unsplit<RS, T, KAHAN, AUX, CAPA>), dim3(blocks), dim3(256), 0, stream, a);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
 }
@@ -162,12 +173,65 @@ int dispatch_unsplit(int dtype, const clawhip_rp& rp, const void* q_in, const Ge
   return run_unsplit<RSF<float>, float, true>(rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
                                               ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
 }
+// Solvers with aux fields: the aux window, with or without a capacity field.
+template <class RS>
+int dispatch_unsplit_aux(int dtype, const clawhip_rp& rp, const void* q_in, const Geom& gin,
+                         void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
+                         const int* lo, const int* hi, const int* refl, int ng,
+                         const int* method, const int* mthlim, double dx, double dy, double dt,
+                         unsigned long long* cfl_bits, hipStream_t stream, const AuxIn& aux) {
+  (void)rp;
+  const bool capa = method[5] > 0;
+  if (dtype == CLAWHIP_F64) {
+    using R = typename RS::template with<double>;
+    return capa ? run_unsplit<R, double, false, true, true>(rp, q_in, gin, q_out, gout, comp_in,
+                                                            comp_out, lo, hi, refl, ng, method,
+                                                            mthlim, dx, dy, dt, cfl_bits, stream,
+                                                            &aux)
+                : run_unsplit<R, double, false, true, false>(rp, q_in, gin, q_out, gout, comp_in,
+                                                             comp_out, lo, hi, refl, ng, method,
+                                                             mthlim, dx, dy, dt, cfl_bits, stream,
+                                                             &aux);
+  }
+  using R = typename RS::template with<float>;
+  return capa ? run_unsplit<R, float, true, true, true>(rp, q_in, gin, q_out, gout, comp_in,
+                                                        comp_out, lo, hi, refl, ng, method, mthlim,
+                                                        dx, dy, dt, cfl_bits, stream, &aux)
+              : run_unsplit<R, float, true, true, false>(rp, q_in, gin, q_out, gout, comp_in,
+                                                         comp_out, lo, hi, refl, ng, method,
+                                                         mthlim, dx, dy, dt, cfl_bits, stream,
+                                                         &aux);
+}
+
+template <int ME>
+struct VCAdvectionOf {
+  template <typename T>
+  using with = VCAdvection<T, ME>;
+};
 }  // namespace
 
 int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
                    void* q_out, const Geom& gout, const void* comp_in, void* comp_out, const int* lo, const int* hi,
                    const int* refl, int ng, const int* method, const int* mthlim, double dx,
-                   double dy, double dt, unsigned long long* cfl_bits, hipStream_t stream) {
+                   double dy, double dt, unsigned long long* cfl_bits, hipStream_t stream,
+                   const AuxIn& aux) {
+  if (rp.kind == CLAWHIP_RP_VC_ADVECTION || rp.kind == CLAWHIP_RP_VC_ADVECTION_2D) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    const int me = rp.kind == CLAWHIP_RP_VC_ADVECTION_2D ? 1 : rp.meqn;
+#define CLAWHIP_VCA(ME)                                                                         \
+  return dispatch_unsplit_aux<VCAdvectionOf<ME>>(dtype, rp, q_in, gin, q_out, gout, comp_in,   \
+                                                 comp_out, lo, hi, refl, ng, method, mthlim,   \
+                                                 dx, dy, dt, cfl_bits, stream, aux)
+    switch (me) {
+      case 1: CLAWHIP_VCA(1);
+      case 2: CLAWHIP_VCA(2);
+      case 3: CLAWHIP_VCA(3);
+      case 4: CLAWHIP_VCA(4);
+    }
+#undef CLAWHIP_VCA
+    return CLAWHIP_E_ARG;
+  }
+  if (aux.p) return CLAWHIP_E_ARG;
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:
       return dispatch_unsplit<Acoustics>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,

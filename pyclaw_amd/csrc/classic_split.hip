@@ -497,9 +497,9 @@ int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
                   void* work, void* cwork, const int* lo, const int* hi, const int* refl,
                   int ng, const int* method, const int* mthlim, double dx, double dy,
                   double dt, const AuxIn& aux, unsigned long long* cfl, hipStream_t s) {
-  if (rp.kind == CLAWHIP_RP_VC_ADVECTION) {
+  if (rp.kind == CLAWHIP_RP_VC_ADVECTION || rp.kind == CLAWHIP_RP_VC_ADVECTION_2D) {
     if (!aux.p) return CLAWHIP_E_ARG;
-    switch (rp.meqn) {
+    switch (rp.kind == CLAWHIP_RP_VC_ADVECTION_2D ? 1 : rp.meqn) {
       case 1: return dispatch_split2<VCAdvection1>(dtype, rp, q_in, gin, q_out, gout, comp_in,
                                                    comp_out, work, cwork, lo, hi, refl, ng,
                                                    method, mthlim, dx, dy, dt, aux, cfl, s);
@@ -554,9 +554,9 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
 int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& gin, void* q_out,
                   const Geom& gout, int ng, const int* method, const int* mthlim, double dtdx,
                   const AuxIn& aux, unsigned long long* cfl, hipStream_t s) {
-  if (rp.kind == CLAWHIP_RP_VC_ADVECTION) {
+  if (rp.kind == CLAWHIP_RP_VC_ADVECTION || rp.kind == CLAWHIP_RP_VC_ADVECTION_2D) {
     if (!aux.p) return CLAWHIP_E_ARG;
-    switch (rp.meqn) {
+    switch (rp.kind == CLAWHIP_RP_VC_ADVECTION_2D ? 1 : rp.meqn) {
       case 1: return run_sweep<VCAdvection1<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng,
                                                      method, mthlim, dtdx, aux, cfl, s);
       case 2: return run_sweep<VCAdvection2<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng,

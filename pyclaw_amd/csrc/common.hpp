@@ -182,11 +182,11 @@ __device__ __forceinline__ void wave_dots(const T (&A)[MW][ME], const T (&B)[MW]
   }
 }
 
-// flux2.f90:134-146: cqxx from limited waves, fadd = 0.5*cqxx.
+// flux2.f90:134-146: cqxx from limited waves, fadd = 0.5*cqxx.  dtdxave is
+// 0.5*(dtdx1d(i-1) + dtdx1d(i)), the mean over the interface's two cells.
 template <int ME, int MW, typename T>
-__device__ __forceinline__ void second_order(const T (&W)[MW][ME], const T (&s)[MW], T dtdx,
-                                             T (&cq)[ME], T (&F)[ME]) {
-  T dtdxave = T(0.5) * (dtdx + dtdx);
+__device__ __forceinline__ void second_order_ave(const T (&W)[MW][ME], const T (&s)[MW],
+                                                 T dtdxave, T (&cq)[ME], T (&F)[ME]) {
 #pragma unroll
   for (int m = 0; m < ME; m++) {
     cq[m] = T(0);
@@ -197,6 +197,13 @@ __device__ __forceinline__ void second_order(const T (&W)[MW][ME], const T (&s)[
     }
     F[m] = T(0) + T(0.5) * cq[m];
   }
+}
+
+// The same with one dtdx for every cell (no capacity function).
+template <int ME, int MW, typename T>
+__device__ __forceinline__ void second_order(const T (&W)[MW][ME], const T (&s)[MW], T dtdx,
+                                             T (&cq)[ME], T (&F)[ME]) {
+  second_order_ave<ME, MW>(W, s, T(0.5) * (dtdx + dtdx), cq, F);
 }
 
 }  // namespace claw

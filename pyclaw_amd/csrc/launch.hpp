@@ -27,19 +27,19 @@ inline BCSpec to_bc(const int* lo, const int* hi, const int* refl, int ndim) {
   return b;
 }
 
-int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
-                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
-                   const int* lo, const int* hi, const int* refl, int ng, const int* method,
-                   const int* mthlim, double dx, double dy, double dt,
-                   unsigned long long* cfl_bits, hipStream_t stream);
-
-// Aux fields of a 2D split step (p == nullptr: none): planar like q, their
+// Aux fields of a 2D step (p == nullptr: none): planar like q, their
 // boundary codes per side (never negating).
 struct AuxIn {
   const void* p = nullptr;
   Geom g{};
   int lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
 };
+
+int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
+                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
+                   const int* lo, const int* hi, const int* refl, int ng, const int* method,
+                   const int* mthlim, double dx, double dy, double dt,
+                   unsigned long long* cfl_bits, hipStream_t stream, const AuxIn& aux);
 
 int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,

@@ -4,6 +4,8 @@
 // device functions:
 //   rpn<IXY>(p, qL, qR, wave, s, amdq, apdq, roe)   normal solve at one interface
 //   rpt<IXY>(p, roe, asdq, bmasdq, bpasdq)           transverse split of asdq
+// Solvers with aux fields (maux > 0) take rpn_aux / rpt_aux instead, with the
+// aux values of the cells involved; rpn_call / rpt_call pick the right form.
 // qL is the state left of the interface (the reference ABI's qr(:,i-1)), qR the
 // state right of it (ql(:,i)).  IXY selects the normal momentum component.  The
 // Roe averages rpn computes are handed to rpt in `roe`; the reference's rpt2
@@ -495,7 +497,18 @@ struct Euler3 {
 // given at cell edges in aux (aux[0] = u at the cell's left edge, aux[1] = v
 // at its lower edge).  Interface i (left cell i-1, right cell i): one wave
 // q_i - q_{i-1} with speed the right cell's edge velocity; its flux
-// difference goes downwind.  Dimension-split steps only (no transverse solver).
+// difference goes downwind.
+//
+// rpt_aux is the transverse split of the scalar solver vc_advection_2D of the
+// clawpack/riemann package (rpt2_vc_advection.f90; that package is not part of
+// the reference tree, so this restates its published form): the transverse
+// velocity kv (v for x-slices, u for y-slices) of the cell i1 that receives
+// asdq, taken at i1's lower edge (aux2, in the slice) for the down-going part
+// and at its upper edge (aux3, the next slice) for the up-going part:
+//   bmasdq = min(aux2(kv, i1), 0) * asdq,  bpasdq = max(aux3(kv, i1), 0) * asdq.
+// Applied to each of the ME fields.  The vector solver of
+// examples/advection_reaction_2d pairs rpn2 with a dummy rpt2, so kind 6 is
+// used with dimension splitting or with transverse_waves = 0.
 template <typename T, int ME>
 struct VCAdvection {
   static constexpr int meqn = ME, mwaves = 1, maux = 2;
@@ -517,6 +530,21 @@ struct VCAdvection {
       apdq[m] = up * wave[0][m];
     }
   }
+  // a1, a2, a3: aux of cell i1 in the previous, this and the next slice.
+  template <int IXY>
+  __device__ static __forceinline__ void rpt_aux(const RPParams<T>&, const Roe&, const T* a1,
+                                                 const T* a2, const T* a3, const T (&asdq)[ME],
+                                                 T (&bm)[ME], T (&bp)[ME]) {
+    (void)a1;
+    constexpr int kv = 2 - IXY;   // x-slices: v (aux[1]); y-slices: u (aux[0])
+    const T vm = a2[kv] < T(0) ? a2[kv] : T(0);
+    const T vp = a3[kv] > T(0) ? a3[kv] : T(0);
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      bm[m] = vm * asdq[m];
+      bp[m] = vp * asdq[m];
+    }
+  }
 };
 template <typename T> using VCAdvection1 = VCAdvection<T, 1>;
 template <typename T> using VCAdvection2 = VCAdvection<T, 2>;
@@ -536,6 +564,21 @@ __device__ __forceinline__ void rpn_call(const RPParams<T>& p, const T* qL, cons
   } else {
     (void)auxL; (void)auxR;
     RS::template rpn<IXY>(p, qL, qR, wave, s, amdq, apdq, roe);
+  }
+}
+
+// The transverse split with the aux values of cell i1 in the three slices
+// (previous, this, next); solvers without aux fields take the plain call.
+template <class RS, int IXY, typename T>
+__device__ __forceinline__ void rpt_call(const RPParams<T>& p, const typename RS::Roe& roe,
+                                         const T* a1, const T* a2, const T* a3,
+                                         const T (&asdq)[RS::meqn], T (&bm)[RS::meqn],
+                                         T (&bp)[RS::meqn]) {
+  if constexpr (RS::maux > 0) {
+    RS::template rpt_aux<IXY>(p, roe, a1, a2, a3, asdq, bm, bp);
+  } else {
+    (void)a1; (void)a2; (void)a3;
+    RS::template rpt<IXY>(p, roe, asdq, bm, bp);
   }
 }
 

@@ -38,13 +38,13 @@ def rp_struct(rpn, cparam, meqn=0):
     return K.make_rp(int(rpn), pd, meqn=meqn)
 
 
-def check_method(method, use_fwave, allow_aux=False):
+def check_method(method, use_fwave, allow_aux=False, allow_capa=False):
     if use_fwave:
         raise NotImplementedError("f-wave Riemann solvers are not supported")
-    if int(method[5]) != 0:
-        raise NotImplementedError("capacity functions are not supported")
+    if int(method[5]) != 0 and not allow_capa:
+        raise NotImplementedError("capacity functions are supported by the unsplit 2D step only")
     if int(method[6]) != 0 and not allow_aux:
-        raise NotImplementedError("aux arrays are supported by the dimension-split 2D step only")
+        raise NotImplementedError("aux arrays are supported by the 2D steps only")
 
 
 def mthlim4(mthlim):

@@ -15,14 +15,16 @@ cparam = CParam()
 def step2(maxm, num_ghost, mx, my, qold, qnew, aux, dx, dy, dt, method, mthlim, aux1, aux2,
           aux3, work, use_fwave, rpn2, rpt2):
     """Unsplit step (classic/step2.f90:2-250): qnew's interior <- step of qold
-    (ghost cells filled by the caller).  Returns (qnew, cfl)."""
-    check_method(method, use_fwave)
-    rp = rp_struct(rpn2, cparam)
+    (ghost cells filled by the caller, in aux too).  Returns (qnew, cfl)."""
+    check_method(method, use_fwave, allow_aux=True, allow_capa=True)
+    rp = rp_struct(rpn2, cparam, meqn=qold.shape[0])
     qin = to_device2(qold)
     out = qin.new_empty((qold.shape[0], my, mx))
     cfl = new_cfl()
+    auxd = to_device2(aux) if (aux is not None and method[6] > 0) else None
     K.step2_unsplit(rp, qin, out, cfl, [4, 4], [4, 4], [1, 2], method, mthlim4(mthlim), dx, dy,
-                    dt, num_ghost=num_ghost, gin=L.geom2(mx, my, num_ghost))
+                    dt, num_ghost=num_ghost, gin=L.geom2(mx, my, num_ghost), aux=auxd,
+                    aux_bc_lower=[4, 4], aux_bc_upper=[4, 4])
     qnew[:, num_ghost:num_ghost + mx, num_ghost:num_ghost + my] = from_device2(out)
     return qnew, float(cfl.item())
 

@@ -40,12 +40,18 @@ class Dimension(object):
         return np.array([self.lower + (i + 0.5) * self.delta for i in range(self.num_cells)])
 
     def centers_with_ghost(self, num_ghost):
+        """geometry.py:583-589: ghost centres below from lower, above from upper."""
         d = self.delta
-        return np.array([self.lower + (i + 0.5) * d for i in range(-num_ghost, self.num_cells + num_ghost)])
+        below = self.lower + (np.arange(-num_ghost, 0) + 0.5) * d
+        above = self.upper + d * (np.arange(num_ghost) + 0.5)
+        return np.concatenate((below, self.centers, above))
 
     def nodes_with_ghost(self, num_ghost):
+        """geometry.py:591-597: ghost nodes as evenly spaced runs outside each end."""
         d = self.delta
-        return np.array([self.lower + i * d for i in range(-num_ghost, self.num_cells + num_ghost + 1)])
+        below = np.linspace(self.lower - num_ghost * d, self.lower - d, num_ghost)
+        above = np.linspace(self.upper + d, self.upper + num_ghost * d, num_ghost)
+        return np.concatenate((below, self.nodes, above))
 
     def __len__(self):
         return self.num_cells
@@ -55,8 +61,14 @@ class Dimension(object):
             self.name, self.num_cells, self.delta, self.lower, self.upper)
 
 
+def _identity_map(*xc):
+    """geometry.py:13-25: the default mapping, physical = computational."""
+    return list(xc)
+
+
 class Grid(object):
-    """geometry.py:27-470 (no mapped grids, no gauges)."""
+    """geometry.py:27-470 (no gauges).  Physical coordinates are the
+    computational ones mapped by grid.mapc2p (identity by default)."""
 
     def __init__(self, dimensions):
         if isinstance(dimensions, Dimension):
@@ -67,7 +79,7 @@ class Grid(object):
         self.gauges = []
         self.gauge_files = []
         self.gauge_file_names = []
-        self.mapc2p = None
+        self.mapc2p = _identity_map
 
     def add_dimension(self, dimension):
         if dimension.name in self.get_dim_attribute("name"):
@@ -104,19 +116,33 @@ class Grid(object):
     def c_nodes(self):
         return np.meshgrid(*[d.nodes for d in self.dimensions], indexing="ij")
 
+    def _map(self, xc):
+        f = self.mapc2p if self.mapc2p is not None else _identity_map
+        return list(f(*xc))
+
     @property
     def p_centers(self):
-        return self.c_centers
+        """geometry.py:303-312: mapc2p of the computational centres."""
+        return self._map(self.c_centers)
 
     @property
     def p_nodes(self):
-        return self.c_nodes
+        """geometry.py:314-323: mapc2p of the computational nodes."""
+        return self._map(self.c_nodes)
 
     def c_centers_with_ghost(self, num_ghost):
         return np.meshgrid(*[d.centers_with_ghost(num_ghost) for d in self.dimensions], indexing="ij")
 
     def p_centers_with_ghost(self, num_ghost):
-        return self.c_centers_with_ghost(num_ghost)
+        return self._map(self.c_centers_with_ghost(num_ghost))
+
+    def c_nodes_with_ghost(self, num_ghost):
+        """geometry.py:350-366"""
+        return np.meshgrid(*[d.nodes_with_ghost(num_ghost) for d in self.dimensions], indexing="ij")
+
+    def p_nodes_with_ghost(self, num_ghost):
+        """geometry.py:368-369"""
+        return self._map(self.c_nodes_with_ghost(num_ghost))
 
     def __copy__(self):
         return self.__class__(copy.deepcopy(self.dimensions))

@@ -43,21 +43,55 @@ def cfl_reset(cfl):
     L.check(L.lib().clawhip_cfl_reset(_ptr(cfl), _stream()), "clawhip_cfl_reset")
 
 
+_RP_MEQN = {1: 3, 2: 4, 3: 3, 4: 1, 5: 5, 7: 1}
+
+
+def _check_fields(rp, meqn, aux, method):
+    """The kernels read as many fields as the solver has (kind 6: rp.meqn) and
+    method[6] aux fields: the tensors must hold exactly those."""
+    want = rp.meqn if rp.kind == 6 else _RP_MEQN.get(rp.kind)
+    if want is not None and meqn != want:
+        raise ValueError("q has %d fields; the Riemann solver takes %d" % (meqn, want))
+    if aux is not None and aux.shape[0] != int(method[6]):
+        raise ValueError("aux has %d fields, method[6] says %d" % (aux.shape[0], int(method[6])))
+
+
+def _aux_geom(aux, mx, my):
+    """Geometry of a planar aux tensor: compact (maux, my, mx) or with stored
+    ghosts (maux, my + 2g, mx + 2g)."""
+    if aux is None:
+        return None
+    g = (aux.shape[2] - mx) // 2
+    if aux.shape[1] != my + 2 * g or aux.shape[2] != mx + 2 * g or g < 0:
+        raise ValueError("aux shape %s does not fit a %d x %d grid" % (tuple(aux.shape), mx, my))
+    return L.geom2(mx, my, g)
+
+
 def step2_unsplit(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim,
-                  dx, dy, dt, comp_in=None, comp_out=None, num_ghost=2, gin=None):
-    """One unsplit step q_in -> q_out (planar (meqn, my, mx) tensors)."""
+                  dx, dy, dt, comp_in=None, comp_out=None, num_ghost=2, gin=None, aux=None,
+                  aux_bc_lower=None, aux_bc_upper=None):
+    """One unsplit step q_in -> q_out (planar (meqn, my, mx) tensors).
+    aux: planar aux fields of q's dtype, compact or with num_ghost stored ghosts
+    (then with aux codes 4 = given), for solvers that read aux; method[5] > 0
+    names a capacity field among them."""
     meqn, my, mx = q_out.shape
     gout = L.geom2(mx, my)
     if gin is None:
         gin = gout
     code = _dtype_code(q_out)
     assert q_in.dtype == q_out.dtype and q_in.is_contiguous() and q_out.is_contiguous()
+    if aux is not None:
+        assert aux.dtype == q_out.dtype and aux.is_contiguous()
+    _check_fields(rp, meqn, aux, method)
+    gaux = _aux_geom(aux, mx, my)
     rc = L.lib().clawhip_step2_unsplit(
         ctypes.byref(rp), code, _ptr(q_in), ctypes.byref(gin), _ptr(q_out), ctypes.byref(gout),
         _ptr(comp_in), _ptr(comp_out), L.ints(bc_lower), L.ints(bc_upper),
         L.ints(reflect), int(num_ghost), L.ints(method),
         L.ints(list(mthlim) + [0] * (4 - len(mthlim))), float(dx), float(dy), float(dt),
-        _ptr(cfl), _stream())
+        _ptr(aux), ctypes.byref(gaux) if aux is not None else None,
+        L.ints(aux_bc_lower) if aux is not None else None,
+        L.ints(aux_bc_upper) if aux is not None else None, _ptr(cfl), _stream())
     L.check(rc, "clawhip_step2_unsplit")
 
 
@@ -79,6 +113,7 @@ def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthli
     if gin is None:
         gin = gout
     code = _dtype_code(q_out)
+    _check_fields(rp, meqn, aux, method)
     w, cw = work if work is not None else (None, None)
     w = _work_like(q_out, num_ghost, w)
     if code == L.F32K:
@@ -88,7 +123,7 @@ def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthli
         _ptr(comp_in), _ptr(comp_out), _ptr(w), _ptr(cw), L.ints(bc_lower), L.ints(bc_upper),
         L.ints(reflect), int(num_ghost), L.ints(method),
         L.ints(list(mthlim) + [0] * (4 - len(mthlim))), float(dx), float(dy), float(dt),
-        _ptr(aux), ctypes.byref(L.geom2(mx, my)) if aux is not None else None,
+        _ptr(aux), ctypes.byref(_aux_geom(aux, mx, my)) if aux is not None else None,
         L.ints(aux_bc_lower) if aux is not None else None,
         L.ints(aux_bc_upper) if aux is not None else None, _ptr(cfl), _stream())
     L.check(rc, "clawhip_step2_split")

@@ -166,7 +166,8 @@ class Solver(object):
 
     def _aux_bc_codes(self):
         """Kernel codes of solver.aux_bc_lower/upper (extrap, periodic, wall: a wall
-        mirrors aux without negation, solver.py:439-519)."""
+        mirrors aux without negation, solver.py:439-519; custom: the solver fills
+        the ghosts on the host)."""
         lo, hi = [], []
         for d in range(self.num_dim):
             for side, codes, out in (("Lower", self.aux_bc_lower, lo),
@@ -174,9 +175,11 @@ class Solver(object):
                 b = codes[d]
                 if b is None:
                     raise Exception("%s boundary condition not specified for either q or aux." % side)
-                if b not in (BC.extrap, BC.periodic, BC.wall):
-                    raise NotImplementedError("aux boundary condition %s is not supported on the "
-                                              "device" % b)
+                if b not in (BC.custom, BC.extrap, BC.periodic, BC.wall):
+                    raise NotImplementedError("Boundary condition %s not implemented" % b)
+                if b == BC.custom and (self.user_aux_bc_upper if side == "Upper"
+                                       else self.user_aux_bc_lower) is None:
+                    raise Exception("%s custom aux BC function has not been set." % side)
                 out.append(int(b))
         return lo, hi
 

@@ -55,6 +55,87 @@ def advection_reaction_2d():
     return q, velocities(), d, set_velocities, source
 
 
+def advection_annulus():
+    """examples/advection_2d_annulus/advection_annulus.py: the polar map
+    mapc2p_annulus 22-38, two Gaussian pulses qinit 41-61, ghost edge
+    velocities 64-95, edge velocities and area ratio from the stream function
+    pi*(x^2 + y^2) 98-147, setup 150-213 (r in [0.2, 1] x 40 cells, theta in
+    [0, 2 pi] x 120, index_capa 2, vanleer, trans_cor, cfl 0.4/0.5, tfinal 1).
+
+    Returns a dict: q (1, 40, 120), aux (3, 40, 120), auxbc (aux with its ghosts
+    filled as _apply_bcs orders it: custom r sides, then periodic theta), delta,
+    and the example's callbacks for the PyClaw API (mapc2p, edge_aux,
+    ghost_lower / ghost_upper with the reference's custom-BC signature)."""
+    lo, hi, n = (0.2, 0.0), (1.0, np.pi * 2.0), (40, 120)
+    d = [(h - l) / float(m) for l, h, m in zip(lo, hi, n)]
+    ng = 2
+
+    def nodes(k, ghosts=0):
+        # Dimension.nodes / nodes_with_ghost (geometry.py:538-545, 591-597)
+        inner = np.array([lo[k] + i * d[k] for i in range(n[k] + 1)])
+        if ghosts == 0:
+            return inner
+        below = np.linspace(lo[k] - ghosts * d[k], lo[k] - d[k], ghosts)
+        above = np.linspace(hi[k] + d[k], hi[k] + ghosts * d[k], ghosts)
+        return np.concatenate((below, inner, above))
+
+    def mapc2p(xc, yc):
+        return [xc * np.cos(yc), xc * np.sin(yc)]
+
+    def stream(x, y):
+        return np.pi * (x ** 2 + y ** 2)
+
+    def edge_aux(X, Y, dx, dy):
+        """aux of the cells spanned by corner arrays X, Y (nx+1, ny+1): u at the
+        left edge, v at the bottom edge (stream-function differences) and the
+        physical area over dx*dy (shoelace over the four corners)."""
+        x0, y0 = X[:-1, :-1], Y[:-1, :-1]    # lower left
+        x1, y1 = X[:-1, 1:], Y[:-1, 1:]      # upper left
+        x2, y2 = X[1:, 1:], Y[1:, 1:]        # upper right
+        x3, y3 = X[1:, :-1], Y[1:, :-1]      # lower right
+        a = np.empty((3,) + x0.shape, order="F")
+        a[0] = (stream(x1, y1) - stream(x0, y0)) / dy
+        a[1] = -(stream(x3, y3) - stream(x0, y0)) / dx
+        area = 0.5 * ((y0 + y1) * (x1 - x0) + (y1 + y2) * (x2 - x1) +
+                      (y2 + y3) * (x3 - x2) + (y3 + y0) * (x0 - x3))
+        a[2] = area / (dx * dy)
+        return a
+
+    def ghost_lower(state, dim, t, qbc, auxbc, num_ghost):
+        grid = state.grid
+        if dim != grid.dimensions[0]:
+            raise Exception("custom aux BC only on the inner radius")
+        dx, dy = grid.delta
+        X, Y = grid.p_nodes_with_ghost(num_ghost=2)
+        auxbc[:, 0:num_ghost, :] = edge_aux(X[0:num_ghost + 1, :], Y[0:num_ghost + 1, :], dx, dy)
+
+    def ghost_upper(state, dim, t, qbc, auxbc, num_ghost):
+        grid = state.grid
+        if dim != grid.dimensions[0]:
+            raise Exception("custom aux BC only on the outer radius")
+        dx, dy = grid.delta
+        X, Y = grid.p_nodes_with_ghost(num_ghost=2)
+        auxbc[:, -num_ghost:, :] = edge_aux(X[-num_ghost - 1:, :], Y[-num_ghost - 1:, :], dx, dy)
+
+    Xn, Yn = mapc2p(*np.meshgrid(nodes(0), nodes(1), indexing="ij"))
+    aux = edge_aux(Xn, Yn, d[0], d[1])
+    Xg, Yg = mapc2p(*np.meshgrid(nodes(0, ng), nodes(1, ng), indexing="ij"))
+    auxbc = np.zeros((3, n[0] + 2 * ng, n[1] + 2 * ng), order="F")
+    auxbc[:, ng:-ng, ng:-ng] = aux
+    auxbc[:, 0:ng, :] = edge_aux(Xg[0:ng + 1, :], Yg[0:ng + 1, :], d[0], d[1])
+    auxbc[:, -ng:, :] = edge_aux(Xg[-ng - 1:, :], Yg[-ng - 1:, :], d[0], d[1])
+    auxbc[:, :, 0:ng] = auxbc[:, :, -2 * ng:-ng]       # theta periodic (solver.py:461-463)
+    auxbc[:, :, -ng:] = auxbc[:, :, ng:2 * ng]
+
+    centers = [np.array([lo[k] + (i + 0.5) * d[k] for i in range(n[k])]) for k in range(2)]
+    X, Y = mapc2p(*np.meshgrid(*centers, indexing="ij"))
+    q = (1.0 * np.exp(-40.0 * (np.square(X + 0.5) + np.square(Y - 0.0))) +
+         -1.0 * np.exp(-40.0 * (np.square(X - 0.5) + np.square(Y - 0.0))))[None]
+    return dict(q=np.asfortranarray(q), aux=aux, auxbc=auxbc, delta=d, lower=lo, upper=hi,
+                num_cells=n, mapc2p=mapc2p, edge_aux=edge_aux, ghost_lower=ghost_lower,
+                ghost_upper=ghost_upper)
+
+
 def euler_quadrants():
     """examples/euler_2d/quadrants.py:80-110."""
     (X, Y), d = grid([0, 0], [1, 1], [100, 100])

@@ -19,6 +19,7 @@ Sources (reference test that consumes each fixture):
   examples/shallow_2d/test_shallow2d.py:25-36
   examples/euler_3d/test_euler3d.py:26-57
   examples/acoustics_2d_homogeneous/test_2d_acoustics.py:19-30
+  examples/advection_2d_annulus/test_2d_advection_annulus.py:10-32
 """
 import os
 import pickletools
@@ -37,6 +38,7 @@ WANT = {
     "euler_2d": ["quadrants_roe"],
     "shallow_2d": ["radialdambreak_classic_roe", "radialdambreak_sharpclaw_roe"],
     "euler_3d": ["shocktube", "shock_bubble"],
+    "advection_2d_annulus": ["classic"],
 }
 
 _DTYPE_CODES = {"O8", "f8", "|", "<", ">", "=", "i8", "f4", "i4"}

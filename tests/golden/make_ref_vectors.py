@@ -89,7 +89,8 @@ def main():
 
 def aux_vectors():
     """step2ds with edge-velocity aux fields: the example's rpn2, meqn 1..3,
-    velocities of both signs, aux BCs (never negated) as the q BCs."""
+    velocities of both signs, aux BCs (never negated) as the q BCs; then the
+    unsplit step2 with aux and with a capacity field."""
     rng = np.random.default_rng(20240612)
     rpn2 = O.ref_vcadv_rpn2()
     out = {}
@@ -112,6 +113,36 @@ def aux_vectors():
         key = f"a{n:02d}"
         out[key + "_meta"] = np.array([O.VC_ADVECTION, ids, -1, lim, ng, mx, my, meqn],
                                       dtype=np.int64)
+        out[key + "_real"] = np.array([dx, dy, dt, cfl, 0.0])
+        out[key + "_bc"] = np.array(bcl + bcu, dtype=np.int64)
+        out[key + "_qin"] = qbc
+        out[key + "_aux"] = abc
+        out[key + "_qout"] = qnew
+        n += 1
+    # the unsplit step2 with aux (vc_advection_2D's rpt2 from the oracle) and a
+    # capacity field: order, transverse, limiter, index_capa, num_aux
+    for kind, meqn, order, trans, lim, capa, maux, bcl, bcu in [
+            (7, 1, 2, 2, 3, 2, 3, [O.EXTRAP, O.PERIODIC], [O.EXTRAP, O.PERIODIC]),
+            (7, 1, 2, 1, 4, -1, 2, [O.WALL, O.EXTRAP], [O.PERIODIC, O.WALL]),
+            (7, 1, 1, 2, 1, 2, 3, [O.PERIODIC, O.WALL], [O.PERIODIC, O.EXTRAP]),
+            (6, 2, 2, 0, 4, 2, 3, [O.EXTRAP, O.EXTRAP], [O.WALL, O.PERIODIC])]:
+        ng, mx, my = 2, 11, 14
+        q = rng.uniform(-1.0, 2.0, (meqn, mx, my))
+        aux = rng.uniform(-1.5, 1.5, (maux, mx, my))
+        if capa >= 0:
+            aux[capa] = rng.uniform(0.5, 2.0, (mx, my))
+        qbc = O.with_ghosts(np.asfortranarray(q), ng)
+        abc = O.with_ghosts(np.asfortranarray(aux), ng)
+        O.bc2(qbc, ng, bcl, bcu)
+        O.bc2(abc, ng, bcl, bcu, reflect=(-1, -1))
+        meth = O.method_array(order=order, transverse=trans, num_aux=maux, index_capa=capa)
+        dx, dy, dt = 0.1, 0.13, 0.021
+        qnew = qbc.copy(order="F")
+        cfl = O.ref_step2_aux(O.make_rp(kind, meqn=meqn), rpn2, qbc, qnew, abc, ng, dx, dy, dt,
+                              meth, [lim])
+        key = f"a{n:02d}"
+        out[key + "_meta"] = np.array([kind, 0, trans, lim, ng, mx, my, meqn], dtype=np.int64)
+        out[key + "_ext"] = np.array([order, capa, maux], dtype=np.int64)
         out[key + "_real"] = np.array([dx, dy, dt, cfl, 0.0])
         out[key + "_bc"] = np.array(bcl + bcu, dtype=np.int64)
         out[key + "_qin"] = qbc

@@ -91,23 +91,28 @@ def test_live_reference_bit_exact():
 
 def _aux_case(vec, key):
     meta = vec[key + "_meta"]
-    kind, ids, _, lim, ng, mx, my, meqn = [int(x) for x in meta]
+    kind, ids, trans, lim, ng, mx, my, meqn = [int(x) for x in meta]
     dx, dy, dt, cfl_ref, _ = vec[key + "_real"]
     qin, aux = vec[key + "_qin"], vec[key + "_aux"]
     rp = O.make_rp(kind, meqn=meqn)
     out = np.asfortranarray(qin.copy())
-    cfl = O.step2(rp, np.asfortranarray(qin), out, ng, dx, dy, dt,
-                  O.method_array(dimensional_split=True, num_aux=2), [lim], ids=ids,
+    if ids == 0:     # unsplit, with its order, index_capa and aux count
+        order, capa, maux = [int(x) for x in vec[key + "_ext"]]
+        meth = O.method_array(order=order, transverse=trans, num_aux=maux, index_capa=capa)
+    else:
+        meth = O.method_array(dimensional_split=True, num_aux=2)
+    cfl = O.step2(rp, np.asfortranarray(qin), out, ng, dx, dy, dt, meth, [lim], ids=ids,
                   aux=np.asfortranarray(aux))
     return out, cfl, cfl_ref
 
 
 def test_golden_aux_vectors_bit_exact(golden_dir):
-    """step2ds with aux fields, the advection_reaction_2d example's Riemann
-    solver (golden vectors from the reference's own Fortran)."""
+    """step2ds and the unsplit step2 with aux fields (and a capacity field), the
+    advection_reaction_2d example's rpn2 (golden vectors from the reference's
+    own Fortran step with that solver)."""
     vec = np.load(f"{golden_dir}/ref_step_vectors_aux.npz")
     keys = sorted({k.split("_")[0] for k in vec.files})
-    assert len(keys) == 4
+    assert len(keys) == 8
     for key in keys:
         out, cfl, cfl_ref = _aux_case(vec, key)
         ng = int(vec[key + "_meta"][4])
@@ -133,6 +138,74 @@ def test_live_reference_aux_bit_exact():
                      [4], ids=ids, aux=abc)
         c2 = O.ref_step2ds_aux(rpn2, qbc, b, abc, ng, 0.1, 0.08, 0.03, meth, [4], ids)
         assert np.array_equal(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) and c1 == c2
+
+
+@pytest.mark.skipif(not (O.ref_available() and os.path.exists(O.REF_VCADV_PATH)),
+                    reason="reference build (oracle/_ref) not present")
+def test_live_reference_unsplit_capa_bit_exact():
+    """The unsplit step2 with aux fields and a capacity field (step2.f90:101-105,
+    152-163, 183-187, 232-244; flux2.f90's per-cell dtdx1d) against the
+    reference's own Fortran on random velocities of both signs and kappa in
+    [0.4, 2.5]."""
+    rng = np.random.default_rng(12)
+    rpn2 = O.ref_vcadv_rpn2()
+    for kind, meqn, order, trans, lim, capa, maux in [(7, 1, 2, 2, 3, 2, 3), (7, 1, 2, 1, 1, 0, 3),
+                                                      (7, 1, 1, 2, 4, -1, 2),
+                                                      (6, 3, 2, 0, 2, 2, 3)]:
+        ng, mx, my = 2, 10, 13
+        qbc = O.with_ghosts(np.asfortranarray(rng.uniform(0, 1, (meqn, mx, my))), ng)
+        aux = rng.uniform(-1, 1, (maux, mx, my))
+        if capa >= 0:
+            aux[capa] = rng.uniform(0.4, 2.5, (mx, my))
+        abc = O.with_ghosts(np.asfortranarray(aux), ng)
+        O.bc2(qbc, ng, [O.EXTRAP, O.WALL], [O.WALL, O.PERIODIC])
+        O.bc2(abc, ng, [O.EXTRAP, O.WALL], [O.WALL, O.PERIODIC], reflect=(-1, -1))
+        meth = O.method_array(order=order, transverse=trans, num_aux=maux, index_capa=capa)
+        a, b = qbc.copy(order="F"), qbc.copy(order="F")
+        rp = O.make_rp(kind, meqn=meqn)
+        c1 = O.step2(rp, qbc, a, ng, 0.1, 0.08, 0.03, meth, [lim], aux=abc)
+        c2 = O.ref_step2_aux(rp, rpn2, qbc, b, abc, ng, 0.1, 0.08, 0.03, meth, [lim])
+        assert np.array_equal(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) and c1 == c2
+
+
+def annulus_run(c):
+    """examples/advection_2d_annulus set-up on the oracle (classic/solver.py
+    unsplit, trans_cor, vanleer, cfl 0.4 / 0.5, dt_initial 0.1, 10 outputs)."""
+    return OracleRun(O.make_rp(O.VC_ADVECTION_2D), c["q"], c["delta"], [1, 2], [1, 2],
+                     dimensional_split=False, transverse=2, limiters=3, order=2,
+                     cfl_desired=0.4, cfl_max=0.5, dt_initial=0.1, aux=c["aux"], index_capa=2,
+                     auxbc_fn=lambda run: c["auxbc"])
+
+
+def annulus_l1(c, q, golden_dir):
+    """test_2d_advection_annulus.py:10-26: sum |area (q - expected)| over cells."""
+    exp = np.load(f"{golden_dir}/advection_2d_annulus_expected.npz")["classic"]
+    area = c["aux"][2].reshape(-1) * c["delta"][0] * c["delta"][1]
+    return float(np.sum(np.abs(area * (q.reshape(-1) - exp))))
+
+
+def test_fixture_advection_annulus(golden_dir):
+    """examples/advection_2d_annulus/test_2d_advection_annulus.py (L1 < 1e-6): a
+    mapped grid, capacity aux[2], custom aux ghosts, vc_advection_2D."""
+    c = C.advection_annulus()
+    run = annulus_run(c)
+    run.run(1.0, 10)
+    err = annulus_l1(c, run.q, golden_dir)
+    print("annulus L1", err, "steps", run.numsteps)
+    assert err < 1e-6
+
+
+def test_fixture_annulus_catches_planted_velocity_component(golden_dir):
+    """The fixture distinguishes the transverse velocity (kv = 3 - ixy) from the
+    normal one: the same run with that mistake planted in the oracle fails."""
+    c = C.advection_annulus()
+    O.lib().orc_set_plant(1)
+    try:
+        run = annulus_run(c)
+        run.run(1.0, 10)
+    finally:
+        O.lib().orc_set_plant(0)
+    assert annulus_l1(c, run.q, golden_dir) > 1e-3
 
 
 # ---------------------------------------------------------------- fixtures
