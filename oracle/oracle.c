@@ -31,6 +31,7 @@ int orc_rp_meqn(int kind) {
     case ORC_RP_ADVECTION: return 1;
     case ORC_RP_EULER3: return 5;
     case ORC_RP_VC_ADVECTION_2D: return 1;
+    case ORC_RP_VC_ACOUSTICS: return 3;
   }
   return 0;   /* ORC_RP_VC_ADVECTION: any (orc_rp.meqn) */
 }
@@ -44,6 +45,7 @@ int orc_rp_mwaves(int kind) {
     case ORC_RP_EULER3: return 3;
     case ORC_RP_VC_ADVECTION: return 1;
     case ORC_RP_VC_ADVECTION_2D: return 1;
+    case ORC_RP_VC_ACOUSTICS: return 2;
   }
   return 0;
 }
@@ -454,9 +456,54 @@ static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const d
   }
 }
 
+/* Planted mistakes for the tests that show a fixture pins a choice of the
+ * restated transverse solvers (orc_set_plant(k), 0 = none):
+ *   1  vc_advection_2D: the normal velocity instead of the transverse one
+ *      (caught by the annulus fixture);
+ *   2  vc_acoustics_2D: the cells below and above swapped (aux1 <-> aux3);
+ *   3  vc_acoustics_2D: cell i for both imp (caught by the variable-acoustics
+ *      io fixture, as is 2).
+ * Which edge (aux2 or aux3) and which cell (i1) vc_advection_2D's rpt2 reads is
+ * NOT pinned by any fixture: the annulus field has zero radial velocity and a
+ * theta velocity that depends on r only, so those choices leave its solution
+ * unchanged (parity unpinned). */
+static int g_plant = 0;
+void orc_set_plant(int k) { g_plant = k; }
+
+/* vc_acoustics_2D rpn2 (clawpack.riemann rpn2_vc_acoustics.f90, published
+ * form; not in the reference tree).  aux = (rho, c); impedance Z = rho*c.
+ * Interface i: cells i-1 (auxL) | i (auxR); waves at -c(i-1) and +c(i). */
+static void rpn_vc_acoustics(int ixy, const double *qL, const double *qR, const double *auxL,
+                             const double *auxR, double *wave, double *s, double *amdq,
+                             double *apdq) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double zi = auxR[0] * auxR[1];
+  const double zim = auxL[0] * auxL[1];
+  const double delta0 = qR[0] - qL[0], delta1 = qR[mu] - qL[mu];
+  const double a1 = (-delta0 + zi * delta1) / (zim + zi);
+  const double a2 = (delta0 + zim * delta1) / (zim + zi);
+  double *w1 = wave, *w2 = wave + 3;
+  w1[0] = -a1 * zim;
+  w1[mu] = a1;
+  w1[mv] = 0.0;
+  s[0] = -auxL[1];
+  w2[0] = a2 * zi;
+  w2[mu] = a2;
+  w2[mv] = 0.0;
+  s[1] = auxR[1];
+  for (int m = 0; m < 3; m++) {
+    amdq[m] = s[0] * w1[m];
+    apdq[m] = s[1] * w2[m];
+  }
+}
+
 void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
                  const double *auxL, const double *auxR, double *wave, double *s,
                  double *amdq, double *apdq) {
+  if (p->kind == ORC_RP_VC_ACOUSTICS) {
+    rpn_vc_acoustics(ixy, qL, qR, auxL, auxR, wave, s, amdq, apdq);
+    return;
+  }
   (void)auxL;
   if (p->kind == ORC_RP_VC_ADVECTION || p->kind == ORC_RP_VC_ADVECTION_2D)
     rpn_vc_advection(p, ixy, qL, qR, auxR, wave, s, amdq, apdq);
@@ -470,15 +517,6 @@ void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
  * that cell's lower-edge velocity (aux2), the up-going part with its upper-edge
  * velocity (aux3, the next slice's lower edge).  Applied to every field (the
  * vector solver's own rpt2 in examples/advection_reaction_2d is a dummy). */
-/* A planted mistake for the test that shows the annulus fixture pins the
- * choice of velocity component (orc_set_plant(1): the normal velocity instead
- * of the transverse one; 0 = none).  Which edge (aux2 or aux3) and which cell
- * (i1) the published solver reads is NOT pinned by any fixture: the annulus
- * field has zero radial velocity and a theta velocity that depends on r only,
- * so those choices leave its solution unchanged (parity unpinned). */
-static int g_plant = 0;
-void orc_set_plant(int k) { g_plant = k; }
-
 static void rpt_vc_advection(const orc_rp *p, int ixy, int imp, int maux, const double *a2,
                              const double *a3, const double *asdq, double *bm, double *bp) {
   const int kv = g_plant == 1 ? ixy : 3 - ixy;   /* 1-based aux index */
@@ -490,9 +528,34 @@ static void rpt_vc_advection(const orc_rp *p, int ixy, int imp, int maux, const 
   }
 }
 
+/* vc_acoustics_2D rpt2 (clawpack.riemann rpt2_vc_acoustics.f90, published
+ * form): cell i1 = i-2+imp; the down-going part is transmitted into the cell
+ * below (aux1, speed -c there), the up-going part into the cell above (aux3). */
+static void rpt_vc_acoustics(int ixy, int imp, int maux, const double *aux1, const double *aux2,
+                             const double *aux3, const double *asdq, double *bm, double *bp) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const long i1 = (imp == 1 && g_plant != 3) ? 0 : maux;
+  const double *am = (g_plant == 2 ? aux3 : aux1) + i1, *a0 = aux2 + i1,
+               *ap = (g_plant == 2 ? aux1 : aux3) + i1;
+  const double cm = am[1], cp = ap[1];
+  const double zm = am[0] * am[1], zz = a0[0] * a0[1], zp = ap[0] * ap[1];
+  const double a1 = (-asdq[0] + asdq[mv] * zz) / (zm + zz);
+  const double a2 = (asdq[0] + asdq[mv] * zz) / (zz + zp);
+  bm[0] = cm * a1 * zm;
+  bm[mu] = 0.0;
+  bm[mv] = -cm * a1;
+  bp[0] = cp * a2 * zp;
+  bp[mu] = 0.0;
+  bp[mv] = cp * a2;
+}
+
 void orc_rpt_aux(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
                  int maux, const double *a1, const double *a2, const double *a3,
                  const double *asdq, double *bmasdq, double *bpasdq) {
+  if (p->kind == ORC_RP_VC_ACOUSTICS) {
+    rpt_vc_acoustics(ixy, imp, maux, a1, a2, a3, asdq, bmasdq, bpasdq);
+    return;
+  }
   (void)a1;
   if (p->kind == ORC_RP_VC_ADVECTION || p->kind == ORC_RP_VC_ADVECTION_2D)
     rpt_vc_advection(p, ixy, imp, maux, a2, a3, asdq, bmasdq, bpasdq);

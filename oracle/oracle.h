@@ -26,8 +26,10 @@ enum {
   ORC_RP_EULER3 = 5,     /* euler_3D:                meqn 5, mwaves 3 */
   ORC_RP_VC_ADVECTION = 6, /* examples/advection_reaction_2d/rpn2_vc_advection_vector.f90:
                             * meqn = orc_rp.meqn, mwaves 1, 2 aux fields (edge u, v) */
-  ORC_RP_VC_ADVECTION_2D = 7 /* clawpack.riemann vc_advection_2D: the same rpn2 with
-                              * meqn 1, and its transverse solver rpt2 */
+  ORC_RP_VC_ADVECTION_2D = 7, /* clawpack.riemann vc_advection_2D: the same rpn2 with
+                               * meqn 1, and its transverse solver rpt2 */
+  ORC_RP_VC_ACOUSTICS = 8      /* clawpack.riemann vc_acoustics_2D: meqn 3, mwaves 2,
+                               * aux = (density, sound speed) per cell */
 };
 
 /* Problem parameters: the reference passes these through the Fortran /cparam/
@@ -64,8 +66,9 @@ void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
 void orc_rpt_aux(const orc_rp *p, int ixy, int imp, const double *qL, const double *qR,
                  int maux, const double *a1, const double *a2, const double *a3,
                  const double *asdq, double *bmasdq, double *bpasdq);
-/* Test hook: k = 1 plants a mistake in the vc_advection_2D transverse solver
- * (the normal velocity in place of the transverse one), 0 removes it. */
+/* Test hook: plant mistake k in a restated transverse solver (1: vc_advection
+ * normal velocity; 2: vc_acoustics cells below/above swapped; 3: vc_acoustics
+ * cell i for both imp); 0 removes it. */
 void orc_set_plant(int k);
 int orc_rp_meqn(int kind);
 int orc_rp_mwaves(int kind);

@@ -22,9 +22,10 @@ ACOUSTICS, EULER, SHALLOW, ADVECTION, EULER3 = 1, 2, 3, 4, 5
 CUSTOM, EXTRAP, PERIODIC, WALL = 0, 1, 2, 3
 
 MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5}
-MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1, 7: 1}
+MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1, 7: 1, 8: 2}
 VC_ADVECTION = 6
 VC_ADVECTION_2D = 7
+VC_ACOUSTICS = 8
 
 
 class RP(ctypes.Structure):
@@ -237,9 +238,13 @@ def ref_vcadv_rpn2():
     return ctypes.cast(_ref_vcadv.rpn2_, ctypes.c_void_p)
 
 
-def ref_step2ds_aux(rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, ids, mwaves=1):
-    """The reference's step2ds sweep with aux fields and the given rpn2 address."""
+def ref_step2ds_aux(rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, ids, mwaves=1,
+                    rp=None):
+    """The reference's step2ds sweep with aux fields and the given rpn2 address
+    (None: the oracle's solver rp)."""
     meqn, nx, ny = qold.shape
+    if rp is not None:
+        ref_classic().shim_set_rp(ctypes.byref(rp))
     f = ref_classic().ref_step2ds_aux
     f.restype = ctypes.c_double
     return f(rpn2, int(ids), meqn, int(mwaves), aux.shape[0], ng, nx - 2 * ng, ny - 2 * ng,

@@ -56,6 +56,22 @@ void shim_rpt2_(const int *ixy, const int *imp, const int *maxm, const int *meqn
   }
 }
 
+/* rpn2 for solvers with aux, from the oracle: problem i takes the aux of
+ * cell i-1 from auxr and of cell i from auxl (both are flux2's aux2). */
+void shim_rpn2_aux_(const int *ixy, const int *maxm, const int *meqn, const int *mwaves,
+                    const int *maux, const int *mbc, const int *mx, const double *ql,
+                    const double *qr, const double *auxl, const double *auxr, double *wave,
+                    double *s, double *amdq, double *apdq) {
+  (void)maxm;
+  const int me = *meqn, mw = *mwaves, ng = *mbc, ma = *maux;
+  for (int i = 2 - ng; i <= *mx + ng; i++) {
+    long ci = i + ng - 1, cm = ci - 1;
+    orc_rpn_aux(&g_rp, *ixy, qr + (long)me * cm, ql + (long)me * ci, auxr + (long)ma * cm,
+                auxl + (long)ma * ci, wave + (long)me * mw * ci, s + (long)mw * ci,
+                amdq + (long)me * ci, apdq + (long)me * ci);
+  }
+}
+
 /* rpt2 for solvers with aux: aux1/aux2/aux3 are the three slices
  * (maux values per cell); problem i's solver reads cell i1 = i-2+imp. */
 void shim_rpt2_aux_(const int *ixy, const int *imp, const int *maxm, const int *meqn,
@@ -145,7 +161,8 @@ double ref_step2(const orc_rp *p, int ids, int meqn, int mwaves, int ng, int mx,
 
 /* The unsplit step2 with aux fields (method[6] = maux; method[5] = index_capa
  * for a capacity field) and a caller-supplied rpn2 (the advection_reaction_2d
- * example's, compiled from its source); rpt2 from the oracle's solver p. */
+ * example's, compiled from its source; NULL: the oracle's solver p); rpt2 from
+ * the oracle's solver p. */
 double ref_step2_aux(const orc_rp *p, rp_fn rpn2, int meqn, int mwaves, int maux, int ng,
                      int mx, int my, const double *qold, double *qnew, const double *aux,
                      double dx, double dy, double dt, const int *method, const int *mthlim) {
@@ -161,14 +178,15 @@ double ref_step2_aux(const orc_rp *p, rp_fn rpn2, int meqn, int mwaves, int maux
   double cfl = 0.0;
   step2_(&maxm, &meqn, &mwaves, &maux, &ng, &mx, &my, qold, qnew, aux, &dx, &dy, &dt, method,
          mthlim, &cfl, qadd, fadd, gadd, q1d, dtdx1d, dtdy1d, aux1, aux2, aux3, work, &mwork,
-         &use_fwave, rpn2, (rp_fn)shim_rpt2_aux_);
+         &use_fwave, rpn2 ? rpn2 : (rp_fn)shim_rpn2_aux_, (rp_fn)shim_rpt2_aux_);
   free(buf);
   return cfl;
 }
 
 /* step2ds (ids 1, 2) with aux fields and a caller-supplied rpn2 -- the
  * reference example's own Riemann solver, compiled from its source by ref.mk
- * (examples/advection_reaction_2d/rpn2_vc_advection_vector.f90).
+ * (examples/advection_reaction_2d/rpn2_vc_advection_vector.f90), or NULL for
+ * the oracle's solver set by shim_set_rp.
  * aux(maux, 1-ng:mx+ng, 1-ng:my+ng), ghosts filled. */
 double ref_step2ds_aux(rp_fn rpn2, int ids, int meqn, int mwaves, int maux, int ng, int mx,
                        int my, const double *qold, double *qnew, const double *aux, double dx,
@@ -184,7 +202,8 @@ double ref_step2ds_aux(rp_fn rpn2, int ids, int meqn, int mwaves, int maux, int 
   double cfl = 0.0;
   step2ds_(&maxm, &meqn, &mwaves, &maux, &ng, &mx, &my, qold, qnew, aux, &dx, &dy, &dt,
            method, mthlim, &cfl, qadd, fadd, gadd, q1d, dtdx1d, dtdy1d, aux1, aux2, aux3,
-           work, &mwork, &ids, &use_fwave, rpn2, (rp_fn)shim_rpt2_);
+           work, &mwork, &ids, &use_fwave, rpn2 ? rpn2 : (rp_fn)shim_rpn2_aux_,
+           (rp_fn)shim_rpt2_);
   free(buf);
   return cfl;
 }

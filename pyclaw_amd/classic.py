@@ -85,7 +85,7 @@ class ClawSolver(Solver):
         rp_aux = getattr(self.rp, "num_aux", 0)
         if rp_aux == 0 and state.num_aux != 0:
             raise NotImplementedError("aux fields are supported with the variable-coefficient "
-                                      "advection solvers")
+                                      "solvers (vc_advection, vc_acoustics)")
         if rp_aux and not rp_aux <= state.num_aux <= 3:
             raise NotImplementedError("the Riemann solver reads %d aux fields; the device steps "
                                       "take %d to 3, the state has %d"
@@ -95,7 +95,7 @@ class ClawSolver(Solver):
         if state.index_capa >= 0:
             if self.num_dim != 2 or self.dimensional_split or not rp_aux:
                 raise NotImplementedError("capacity functions are supported by the unsplit 2D "
-                                          "step with the variable-coefficient advection solvers")
+                                          "step with the variable-coefficient solvers")
             if state.index_capa >= state.num_aux:
                 raise ValueError("index_capa %d is not an aux field" % state.index_capa)
         if (getattr(self.rp, "kind", 0) == 6 and self.num_dim == 2 and not self.dimensional_split

@@ -1,7 +1,8 @@
 """Controller (pyclaw/controller.py:275-395): the output-time loop around
-Solver.evolve_to_time, keeping in-memory frames.  Writing frames to disk is not
-part of this package; output_format is accepted and ignored."""
+Solver.evolve_to_time, keeping in-memory frames (keep_copy) and writing each
+frame to outdir in output_format (ascii; None writes nothing)."""
 import copy
+import os
 
 import numpy as np
 
@@ -63,8 +64,16 @@ class Controller(object):
         if len(output_times) == 0:
             print("No valid output times; halting.")
             return None
+        frame = self.start_frame
         if self.keep_copy:
             self.frames.append(copy.deepcopy(self.solution))
+        if self.output_format is not None:
+            # controller.py:329-350: refuse to overwrite unless allowed, then frame 0
+            if os.path.exists(self.outdir) and self.overwrite is False:
+                raise Exception("Refusing to overwrite existing output data. \n"
+                                "Either delete/move the directory or set "
+                                "controller.overwrite=True.")
+            self._write(frame, self.write_aux_always or self.write_aux_init)
         status = None
         for t in output_times[1:]:
             if self.output_style < 3:
@@ -72,7 +81,14 @@ class Controller(object):
             else:
                 for _ in range(self.nstepout):
                     status = self.solver.evolve_to_time(self.solution)
+            frame += 1
             if self.keep_copy:
                 self.frames.append(copy.deepcopy(self.solution))
+            if self.output_format is not None:
+                self._write(frame, self.write_aux_always)
         self.solution.__dict__["_start_frame"] = len(self.frames)
         return status
+
+    def _write(self, frame, write_aux):
+        self.solution.write(frame, self.outdir, self.output_format, self.output_file_prefix,
+                            write_aux, self.output_options)

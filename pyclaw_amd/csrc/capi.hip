@@ -66,7 +66,10 @@ static int aux_in(const void* aux, const clawhip_geom* gaux, const int* alo, con
 }
 
 static int rp_num_aux(const clawhip_rp* rp) {
-  return rp->kind == CLAWHIP_RP_VC_ADVECTION || rp->kind == CLAWHIP_RP_VC_ADVECTION_2D ? 2 : 0;
+  return rp->kind == CLAWHIP_RP_VC_ADVECTION || rp->kind == CLAWHIP_RP_VC_ADVECTION_2D ||
+                 rp->kind == CLAWHIP_RP_VC_ACOUSTICS
+             ? 2
+             : 0;
 }
 
 int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,

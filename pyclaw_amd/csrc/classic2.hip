@@ -208,6 +208,11 @@ struct VCAdvectionOf {
   template <typename T>
   using with = VCAdvection<T, ME>;
 };
+template <template <typename> class RSF>
+struct SolverOf {
+  template <typename T>
+  using with = RSF<T>;
+};
 }  // namespace
 
 int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
@@ -230,6 +235,12 @@ int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom
     }
 #undef CLAWHIP_VCA
     return CLAWHIP_E_ARG;
+  }
+  if (rp.kind == CLAWHIP_RP_VC_ACOUSTICS) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    return dispatch_unsplit_aux<SolverOf<VCAcoustics>>(dtype, rp, q_in, gin, q_out, gout, comp_in,
+                                                      comp_out, lo, hi, refl, ng, method, mthlim,
+                                                      dx, dy, dt, cfl_bits, stream, aux);
   }
   if (aux.p) return CLAWHIP_E_ARG;
   switch (rp.kind) {

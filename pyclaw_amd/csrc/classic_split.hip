@@ -515,6 +515,12 @@ int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
     }
     return CLAWHIP_E_ARG;
   }
+  if (rp.kind == CLAWHIP_RP_VC_ACOUSTICS) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    return dispatch_split2<VCAcoustics>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
+                                        work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
+                                        dt, aux, cfl, s);
+  }
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:
       return dispatch_split2<Acoustics>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
@@ -567,6 +573,11 @@ int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& g
                                                      method, mthlim, dtdx, aux, cfl, s);
     }
     return CLAWHIP_E_ARG;
+  }
+  if (rp.kind == CLAWHIP_RP_VC_ACOUSTICS) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    return run_sweep<VCAcoustics<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
+                                          dtdx, aux, cfl, s);
   }
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:

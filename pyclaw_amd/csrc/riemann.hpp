@@ -19,6 +19,8 @@
 //   shallow_roe_with_efix_2D  Roe, 3 waves (u-c, u, u+c), Harten-Hyman fix
 //   advection_2D              1 wave
 //   euler_3D                  Roe, 3 waves (u-a, lumped entropy+shear, u+a), fix
+//   vc_advection(_vector)_2D  1 wave, edge velocities in aux
+//   vc_acoustics_2D           2 waves (-c left, +c right), rho and c in aux
 // Every expression keeps the reference's evaluation order; the library is
 // compiled with -ffp-contract=off so FP64 results match the CPU oracle bitwise.
 #pragma once
@@ -550,6 +552,64 @@ template <typename T> using VCAdvection1 = VCAdvection<T, 1>;
 template <typename T> using VCAdvection2 = VCAdvection<T, 2>;
 template <typename T> using VCAdvection3 = VCAdvection<T, 3>;
 template <typename T> using VCAdvection4 = VCAdvection<T, 4>;
+
+// ---------------------------------------------------------------- variable-coefficient acoustics
+// vc_acoustics_2D of the clawpack/riemann package (rpn2_vc_acoustics.f90 and
+// rpt2_vc_acoustics.f90, not part of the reference tree: restated from their
+// published form), as examples/acoustics_2d_variable uses it.  q = (p, u, v);
+// aux[0] = density rho, aux[1] = sound speed c of each cell; impedance Z = rho*c.
+// Normal solve at interface i (cells i-1 | i): the two waves move at -c(i-1)
+// and +c(i) with the transmission coefficients of the impedance jump.
+// Transverse split of asdq into cell i1's neighbours across the slice: the
+// down-going part moves at -c of the cell below (aux1), the up-going part at +c
+// of the cell above (aux3), each weighted by the impedance jump it crosses.
+template <typename T>
+struct VCAcoustics {
+  static constexpr int meqn = 3, mwaves = 2, maux = 2;
+  struct Roe {};
+  template <int IXY>
+  __device__ static __forceinline__ void rpn_aux(const RPParams<T>&, const T* qL, const T* qR,
+                                                 const T* auxL, const T* auxR,
+                                                 T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                                 T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T zi = auxR[0] * auxR[1];     // impedance of cell i
+    const T zim = auxL[0] * auxL[1];    // impedance of cell i-1
+    const T d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu];
+    const T a1 = (-d0 + zi * d1) / (zim + zi);
+    const T a2 = (d0 + zim * d1) / (zim + zi);
+    wave[0][0] = -(a1 * zim);
+    wave[0][mu] = a1;
+    wave[0][mv] = T(0);
+    s[0] = -auxL[1];
+    wave[1][0] = a2 * zi;
+    wave[1][mu] = a2;
+    wave[1][mv] = T(0);
+    s[1] = auxR[1];
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      amdq[m] = s[0] * wave[0][m];
+      apdq[m] = s[1] * wave[1][m];
+    }
+  }
+  // a1, a2, a3: aux of cell i1 in the previous, this and the next slice.
+  template <int IXY>
+  __device__ static __forceinline__ void rpt_aux(const RPParams<T>&, const Roe&, const T* a1,
+                                                 const T* a2, const T* a3, const T (&asdq)[meqn],
+                                                 T (&bm)[meqn], T (&bp)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T cm = a1[1], cp = a3[1];
+    const T zm = a1[0] * a1[1], zz = a2[0] * a2[1], zp = a3[0] * a3[1];
+    const T b1 = (-asdq[0] + asdq[mv] * zz) / (zm + zz);   // transmitted down-going
+    const T b2 = (asdq[0] + asdq[mv] * zz) / (zz + zp);    // transmitted up-going
+    bm[0] = cm * b1 * zm;
+    bm[mu] = T(0);
+    bm[mv] = -(cm * b1);
+    bp[0] = cp * b2 * zp;
+    bp[mu] = T(0);
+    bp[mv] = cp * b2;
+  }
+};
 
 // The normal solve with the aux values of the left and right cells; solvers
 // without aux fields take the plain call.

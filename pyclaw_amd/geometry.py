@@ -163,13 +163,18 @@ class Patch(object):
         if key in ("num_dim", "dimensions", "num_cells", "lower", "upper", "delta",
                    "centers", "nodes", "p_centers", "c_centers", "p_nodes", "c_nodes"):
             return getattr(self.grid, key)
+        if key in ("num_cells_global", "lower_global", "upper_global"):
+            return getattr(self.grid, key[:-len("_global")])
         grid = self.__dict__.get("grid")
         if grid is not None and key in grid._dimensions:
             return getattr(grid, key)
         raise AttributeError(key)
 
     def __deepcopy__(self, memo={}):
-        return Patch(copy.deepcopy(self.grid.dimensions))
+        result = Patch(copy.deepcopy(self.grid.dimensions))
+        result.level, result.patch_index = self.level, self.patch_index
+        result.grid.mapc2p = self.grid.mapc2p
+        return result
 
 
 class Domain(object):

@@ -136,6 +136,25 @@ def advection_annulus():
                 ghost_upper=ghost_upper)
 
 
+def acoustics_2d_variable(num_cells=(50, 50)):
+    """examples/acoustics_2d_variable/acoustics_2d_interface.py:22-96: [-1, 1]^2,
+    density 4 | 1 and bulk modulus 4 | 4 left | right of x = 0 (aux = rho, c),
+    a ring pulse of pressure centred at (-0.5, 0); walls at the lower sides,
+    extrapolation at the upper ones (q and aux); tfinal 0.6, 20 outputs.
+    Returns (q, aux, delta)."""
+    (X, Y), d = grid([-1.0, -1.0], [1.0, 1.0], list(num_cells))
+    rho_l, rho_r, bulk_l, bulk_r = 4.0, 1.0, 4.0, 4.0
+    c_l, c_r = np.sqrt(bulk_l / rho_l), np.sqrt(bulk_r / rho_r)
+    aux = np.empty((2,) + X.shape, order="F")
+    aux[0] = rho_l * (X < 0.) + rho_r * (X >= 0.)
+    aux[1] = c_l * (X < 0.) + c_r * (X >= 0.)
+    r = np.sqrt((X + 0.5) ** 2 + (Y - 0.) ** 2)
+    width, rad = 0.1, 0.25
+    q = np.zeros((3,) + X.shape, order="F")
+    q[0] = (np.abs(r - rad) <= width) * (1. + np.cos(np.pi * (r - rad) / width))
+    return q, aux, d
+
+
 def euler_quadrants():
     """examples/euler_2d/quadrants.py:80-110."""
     (X, Y), d = grid([0, 0], [1, 1], [100, 100])

@@ -20,9 +20,12 @@ Sources (reference test that consumes each fixture):
   examples/euler_3d/test_euler3d.py:26-57
   examples/acoustics_2d_homogeneous/test_2d_acoustics.py:19-30
   examples/advection_2d_annulus/test_2d_advection_annulus.py:10-32
+  examples/acoustics_2d_variable/test_acoustics_2d_variable.py:6-23 (pressure_classic.txt)
+  examples/acoustics_2d_variable/test_acoustics_2d_variable_io.py:1-60 (io_test_verification)
 """
 import os
 import pickletools
+import shutil
 import sys
 
 import numpy as np
@@ -94,6 +97,17 @@ def main():
     dst = os.path.join(HERE, "acoustics_2d_verify_classic.npz")
     np.savez_compressed(dst, pressure=p)
     print(dst, p.shape)
+    p = np.loadtxt(os.path.join(REF, "acoustics_2d_variable", "pressure_classic.txt"))
+    dst = os.path.join(HERE, "acoustics_2d_variable_pressure_classic.npz")
+    np.savez_compressed(dst, pressure=p)
+    print(dst, p.shape)
+    # the io test's verification frames, kept verbatim (they are ascii data)
+    io_src = os.path.join(REF, "acoustics_2d_variable", "io_test_verification")
+    io_dst = os.path.join(HERE, "acoustics_2d_variable_io")
+    os.makedirs(io_dst, exist_ok=True)
+    for name in ("fort.t0000", "fort.q0000", "fort.a0000", "fort.t0020", "fort.q0020"):
+        shutil.copyfile(os.path.join(io_src, name), os.path.join(io_dst, name))
+    print(io_dst)
     rho = np.loadtxt(os.path.join(REF, "advection_reaction_2d", "advection_reaction.txt"))
     dst = os.path.join(HERE, "advection_reaction_2d_expected.npz")
     np.savez_compressed(dst, density=rho)

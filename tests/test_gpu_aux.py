@@ -173,6 +173,9 @@ _UNSPLIT_CASES = [
     (7, 1, 2, 0, 2, 0, 3, [1, 1], [3, 3]),
     (6, 2, 2, 0, 4, 2, 3, [1, 1], [3, 2]),
     (6, 4, 2, 0, 3, -1, 2, [2, 2], [2, 2]),
+    (8, 3, 2, 1, 4, -1, 2, [3, 3], [1, 1]),
+    (8, 3, 2, 2, 1, 2, 3, [3, 2], [1, 2]),
+    (8, 3, 1, 1, 4, -1, 2, [2, 1], [2, 3]),
 ]
 
 
@@ -187,6 +190,8 @@ def test_unsplit_step_aux_capa_bit_exact(case):
     mx, my, ng = 77, 50, 2
     q = np.asfortranarray(rng.uniform(-1.0, 2.0, (meqn, mx, my)))
     aux = rng.uniform(-1.5, 1.5, (maux, mx, my))
+    if kind == 8:      # density and sound speed
+        aux[:2] = rng.uniform(0.5, 2.0, (2, mx, my))
     if capa >= 0:
         aux[capa] = rng.uniform(0.5, 2.0, (mx, my))
     aux = np.asfortranarray(aux)
@@ -344,3 +349,87 @@ def test_fmod_step2_aux_capa():
                               riemann.vc_advection_2D.rpt2._cpointer)
     sl = (slice(None), slice(ng, -ng), slice(ng, -ng))
     assert np.array_equal(out[sl], ref[sl]) and cfl == c_ref
+
+
+@pytest.mark.parametrize("bcl,bcu", [([3, 3], [1, 1]), ([2, 1], [2, 3])])
+def test_split_step_vc_acoustics_bit_exact(bcl, bcu):
+    """vc_acoustics_2D in the dimension-split step: both sweeps equal the
+    oracle's bit for bit, CFL included."""
+    from pyclaw_amd import kernels as K
+    from pyclaw_amd.device import q_from_device
+    rng = np.random.default_rng(77)
+    mx, my, ng = 66, 41, 2
+    q = np.asfortranarray(rng.uniform(-1.0, 1.0, (3, mx, my)))
+    aux = np.asfortranarray(rng.uniform(0.5, 2.0, (2, mx, my)))
+    dx, dy, dt = 1.0 / mx, 1.0 / my, 0.003
+    meth = O.method_array(dimensional_split=True, num_aux=2)
+    qbc, abc = O.with_ghosts(q, ng), O.with_ghosts(aux, ng)
+    O.bc2(qbc, ng, bcl, bcu)
+    O.bc2(abc, ng, bcl, bcu, reflect=(-1, -1))
+    rp_o = O.make_rp(O.VC_ACOUSTICS)
+    qold = qbc.copy(order="F")
+    c1 = O.step2(rp_o, qold, qbc, ng, dx, dy, dt, meth, [4, 4], ids=1, aux=abc)
+    c2 = O.step2(rp_o, qbc, qbc, ng, dx, dy, dt, meth, [4, 4], ids=2, aux=abc)
+    ref = O.interior(qbc, ng)
+    out = torch.empty_like(_dev(q))
+    cfl = torch.zeros(1, dtype=torch.float64, device="cuda")
+    K.step2_split(K.make_rp(8, {}), _dev(q), out, cfl, bcl, bcu, [1, 2], meth, [4, 4], dx, dy,
+                  dt, aux=_dev(aux), aux_bc_lower=bcl, aux_bc_upper=bcu)
+    torch.cuda.synchronize()
+    got = q_from_device(out)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    assert float(cfl.item()) == max(c1, c2)
+
+
+def test_api_acoustics_2d_variable_io_fixture(golden_dir, tmp_path):
+    """examples/acoustics_2d_variable through the PyClaw API with ascii output:
+    test_acoustics_2d_variable_io.py (frames 0 and 20 written, read back and
+    compared with io_test_verification at relative 1e-6, aux included) and
+    test_acoustics_2d_variable.py (pressure, abstol 0.1); the run equals the
+    oracle's bit for bit."""
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    from tests.test_oracle import vc_acoustics_run
+    q0, aux0, d = C.acoustics_2d_variable()
+    solver = pyclaw.ClawSolver2D(riemann.vc_acoustics_2D)
+    solver.dimensional_split = False
+    solver.limiters = pyclaw.limiters.tvd.MC
+    solver.bc_lower[0] = pyclaw.BC.wall
+    solver.bc_upper[0] = pyclaw.BC.extrap
+    solver.bc_lower[1] = pyclaw.BC.wall
+    solver.bc_upper[1] = pyclaw.BC.extrap
+    solver.aux_bc_lower[0] = pyclaw.BC.wall
+    solver.aux_bc_upper[0] = pyclaw.BC.extrap
+    solver.aux_bc_lower[1] = pyclaw.BC.wall
+    solver.aux_bc_upper[1] = pyclaw.BC.extrap
+    x = pyclaw.Dimension(-1.0, 1.0, 50, name="x")
+    y = pyclaw.Dimension(-1.0, 1.0, 50, name="y")
+    domain = pyclaw.Domain([x, y])
+    state = pyclaw.State(domain, 3, 2)
+    state.aux[...] = aux0
+    state.q[...] = q0
+    claw = pyclaw.Controller()
+    claw.keep_copy = True
+    claw.solution = pyclaw.Solution(state, domain)
+    claw.solver = solver
+    claw.outdir = str(tmp_path)
+    claw.tfinal = 0.6
+    claw.num_output_times = 20
+    claw.write_aux_init = True
+    claw.run()
+    ver = f"{golden_dir}/acoustics_2d_variable_io"
+    exp_aux = pyclaw.Solution(0, path=ver, read_aux=True).state.aux
+    exp_q = pyclaw.Solution(20, path=ver, read_aux=False).state.q
+    got_aux = pyclaw.Solution(0, path=str(tmp_path), read_aux=True).state.get_aux_global()
+    got_q = pyclaw.Solution(20, path=str(tmp_path), read_aux=False).state.get_q_global()
+    rel_q = np.linalg.norm(exp_q - got_q) / np.linalg.norm(exp_q)
+    rel_aux = np.linalg.norm(exp_aux - got_aux) / np.linalg.norm(exp_aux)
+    print("variable acoustics io: q", rel_q, "aux", rel_aux)
+    assert rel_q < 1e-6 and rel_aux < 1e-6
+    p = np.load(f"{golden_dir}/acoustics_2d_variable_pressure_classic.npz")["pressure"]
+    final = claw.frames[-1].state.q
+    assert np.abs(p.reshape(-1) - final[0].reshape(-1)).max() < 1e-1
+    run = vc_acoustics_run(q0, aux0, d)
+    run.run(0.6, 20)
+    assert claw.solver.status["numsteps"] == run.numsteps
+    assert np.array_equal(final, run.q), np.abs(final - run.q).max()

@@ -168,6 +168,96 @@ def test_live_reference_unsplit_capa_bit_exact():
         assert np.array_equal(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) and c1 == c2
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_live_reference_vc_acoustics_bit_exact():
+    """vc_acoustics_2D through the reference's own step2 (unsplit, with and
+    without a capacity field) and step2ds (both sweeps), the solver supplied by
+    the oracle: the step arithmetic around an aux solver that reads both cells."""
+    rng = np.random.default_rng(13)
+    rp = O.make_rp(O.VC_ACOUSTICS)
+    ng, mx, my = 2, 12, 9
+    for order, trans, capa, maux, ids in [(2, 1, -1, 2, 0), (2, 2, 2, 3, 0), (1, 1, -1, 2, 0),
+                                          (2, 0, -1, 2, 1), (2, 0, -1, 2, 2)]:
+        qbc = O.with_ghosts(np.asfortranarray(rng.uniform(-1, 1, (3, mx, my))), ng)
+        abc = O.with_ghosts(np.asfortranarray(rng.uniform(0.5, 2.5, (maux, mx, my))), ng)
+        O.bc2(qbc, ng, [O.WALL, O.EXTRAP], [O.PERIODIC, O.WALL])
+        O.bc2(abc, ng, [O.WALL, O.EXTRAP], [O.PERIODIC, O.WALL], reflect=(-1, -1))
+        if ids == 0:
+            meth = O.method_array(order=order, transverse=trans, num_aux=maux, index_capa=capa)
+        else:
+            meth = O.method_array(order=order, dimensional_split=True, num_aux=maux)
+        a, b = qbc.copy(order="F"), qbc.copy(order="F")
+        c1 = O.step2(rp, qbc, a, ng, 0.1, 0.08, 0.01, meth, [4, 4], ids=ids, aux=abc)
+        if ids == 0:
+            c2 = O.ref_step2_aux(rp, None, qbc, b, abc, ng, 0.1, 0.08, 0.01, meth, [4, 4],
+                                 mwaves=2)
+        else:
+            c2 = O.ref_step2ds_aux(None, qbc, b, abc, ng, 0.1, 0.08, 0.01, meth, [4, 4], ids,
+                                   mwaves=2, rp=rp)
+        assert np.array_equal(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) and c1 == c2
+
+
+def vc_acoustics_run(q, aux, d):
+    """examples/acoustics_2d_variable classic set-up on the oracle: unsplit,
+    trans_inc, MC, walls low / extrapolation high (q and aux), 20 outputs."""
+    return OracleRun(O.make_rp(O.VC_ACOUSTICS), q, d, [3, 3], [1, 1], dimensional_split=False,
+                     transverse=1, limiters=4, order=2, aux=aux, aux_bc_lower=[3, 3],
+                     aux_bc_upper=[1, 1])
+
+
+def test_fileio_ascii_reads_and_rewrites_reference_frames(golden_dir, tmp_path):
+    """pyclaw_amd.fileio.ascii reads the reference's own ascii frames
+    (examples/acoustics_2d_variable/io_test_verification) and writes them back
+    byte for byte (ascii.py:17-126 format)."""
+    import filecmp
+    import pyclaw_amd as pyclaw
+    src = f"{golden_dir}/acoustics_2d_variable_io"
+    s20 = pyclaw.Solution(20, path=src, file_format="ascii", file_prefix=None, read_aux=False)
+    assert s20.t == 0.6 and s20.state.q.shape == (3, 50, 50)
+    assert s20.domain.grid.delta == [0.04, 0.04] and s20.domain.grid.lower == [-1.0, -1.0]
+    s0 = pyclaw.Solution()
+    s0.read(0, path=src, file_format="ascii", file_prefix=None, read_aux=True)
+    assert s0.state.aux.shape == (2, 50, 50)
+    s20.write(20, str(tmp_path))
+    s0.write(0, str(tmp_path), write_aux=True)
+    for name in ("fort.q0020", "fort.q0000", "fort.a0000"):
+        assert filecmp.cmp(f"{src}/{name}", str(tmp_path / name), shallow=False), name
+    again = pyclaw.Solution(20, path=str(tmp_path))
+    np.testing.assert_array_equal(again.state.q, s20.state.q)
+
+
+def test_fixture_acoustics_2d_variable(golden_dir):
+    """examples/acoustics_2d_variable: test_acoustics_2d_variable.py (classic,
+    50x50, max |p - pressure_classic| < 0.1) and the io test's frame 20 (all
+    fields, relative 2-norm < 1e-6), with vc_acoustics_2D restated."""
+    import pyclaw_amd as pyclaw
+    q, aux, d = C.acoustics_2d_variable()
+    run = vc_acoustics_run(q, aux, d)
+    run.run(0.6, 20)
+    p = np.load(f"{golden_dir}/acoustics_2d_variable_pressure_classic.npz")["pressure"]
+    perr = np.abs(p.reshape(-1) - run.q[0].reshape(-1)).max()
+    exp = pyclaw.Solution(20, path=f"{golden_dir}/acoustics_2d_variable_io").state.q
+    rel = np.linalg.norm(exp - run.q) / np.linalg.norm(exp)
+    print("variable acoustics: max pressure error", perr, "frame-20 relative error", rel)
+    assert perr < 1e-1 and rel < 1e-6
+
+
+@pytest.mark.parametrize("plant", [2, 3])
+def test_fixture_acoustics_variable_catches_planted_transverse(golden_dir, plant):
+    """The io fixture pins vc_acoustics_2D's transverse split: swapping the
+    cells below and above (2) or reading cell i for both imp (3) fails it."""
+    import pyclaw_amd as pyclaw
+    q, aux, d = C.acoustics_2d_variable()
+    O.lib().orc_set_plant(plant)
+    try:
+        run = vc_acoustics_run(q, aux, d)
+        run.run(0.6, 20)
+    finally:
+        O.lib().orc_set_plant(0)
+    exp = pyclaw.Solution(20, path=f"{golden_dir}/acoustics_2d_variable_io").state.q
+    assert np.linalg.norm(exp - run.q) / np.linalg.norm(exp) > 1e-4
+
+
 def annulus_run(c):
     """examples/advection_2d_annulus set-up on the oracle (classic/solver.py
     unsplit, trans_cor, vanleer, cfl 0.4 / 0.5, dt_initial 0.1, 10 outputs)."""
