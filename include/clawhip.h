@@ -51,9 +51,12 @@ enum {
   CLAWHIP_RP_VC_ADVECTION_2D = 7, /* vc_advection_2D (clawpack/riemann): scalar variable-
                                    * coefficient advection, the same rpn2 with a transverse
                                    * solver (rpt2_vc_advection.f90); 2D steps */
-  CLAWHIP_RP_VC_ACOUSTICS = 8      /* vc_acoustics_2D (clawpack/riemann): acoustics with
+  CLAWHIP_RP_VC_ACOUSTICS = 8,     /* vc_acoustics_2D (clawpack/riemann): acoustics with
                                    * density and sound speed per cell in 2 aux fields;
                                    * 2D steps */
+  CLAWHIP_RP_SHALLOW_HLLE = 9      /* shallow_hlle_2D (clawpack/riemann): HLLE shallow
+                                   * water; no transverse solver (split or unsplit with
+                                   * transverse_waves = 0; SharpClaw) */
 };
 
 /* Boundary codes: pyclaw.BC (pyclaw/solver.py:13-19) plus GIVEN. */

@@ -32,6 +32,7 @@ int orc_rp_meqn(int kind) {
     case ORC_RP_EULER3: return 5;
     case ORC_RP_VC_ADVECTION_2D: return 1;
     case ORC_RP_VC_ACOUSTICS: return 3;
+    case ORC_RP_SHALLOW_HLLE: return 3;
   }
   return 0;   /* ORC_RP_VC_ADVECTION: any (orc_rp.meqn) */
 }
@@ -46,6 +47,7 @@ int orc_rp_mwaves(int kind) {
     case ORC_RP_VC_ADVECTION: return 1;
     case ORC_RP_VC_ADVECTION_2D: return 1;
     case ORC_RP_VC_ACOUSTICS: return 2;
+    case ORC_RP_SHALLOW_HLLE: return 2;
   }
   return 0;
 }
@@ -427,9 +429,51 @@ static void rpn_euler3(const orc_rp *p, int ixyz, const double *qL, const double
   }
 }
 
+/* shallow_hlle_2D rpn2 (clawpack.riemann rpn2_shallow_hlle.f90, published
+ * form; not in the reference tree): two waves at the Einfeldt speeds
+ * s1 = min(u_l - c_l, ubar - cbar), s2 = max(u_r + c_r, ubar + cbar) around
+ * the HLL middle state q* = (f(q_r) - f(q_l) - s2 q_r + s1 q_l) / (s1 - s2);
+ * ubar the Roe average, cbar = sqrt(g (h_l + h_r) / 2).  Fluctuations by the
+ * sign of each speed. */
+static void rpn_shallow_hlle(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                             double *wave, double *s, double *amdq, double *apdq) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double grav = p->grav;
+  const double h_l = qL[0], h_r = qR[0];
+  const double u_l = qL[mu] / qL[0], u_r = qR[mu] / qR[0];
+  const double hsqrt_l = sqrt(qL[0]), hsqrt_r = sqrt(qR[0]);
+  const double hsq2 = hsqrt_l + hsqrt_r;
+  const double ubar = (u_l * hsqrt_l + u_r * hsqrt_r) / hsq2;
+  const double cbar = sqrt(0.5 * grav * (h_l + h_r));
+  const double s1 = dmin(u_l - sqrt(grav * h_l), ubar - cbar);
+  const double s2 = dmax(u_r + sqrt(grav * h_r), ubar + cbar);
+  const double hm = (qR[mu] - qL[mu] - s2 * qR[0] + s1 * qL[0]) / (s1 - s2);
+  const double hum = (qR[mu] * u_r - qL[mu] * u_l + 0.5 * grav * (h_r * h_r - h_l * h_l)
+                      - s2 * qR[mu] + s1 * qL[mu]) / (s1 - s2);
+  const double hvm = (qR[mv] * u_r - qL[mv] * u_l - s2 * qR[mv] + s1 * qL[mv]) / (s1 - s2);
+  double *w1 = wave, *w2 = wave + 3;
+  w1[0] = hm - h_l;
+  w1[mu] = hum - qL[mu];
+  w1[mv] = hvm - qL[mv];
+  s[0] = s1;
+  w2[0] = h_r - hm;
+  w2[mu] = qR[mu] - hum;
+  w2[mv] = qR[mv] - hvm;
+  s[1] = s2;
+  for (int m = 0; m < 3; m++) {
+    amdq[m] = 0.0;
+    apdq[m] = 0.0;
+    for (int mw = 0; mw < 2; mw++) {
+      if (s[mw] < 0.0) amdq[m] = amdq[m] + s[mw] * wave[3 * mw + m];
+      else apdq[m] = apdq[m] + s[mw] * wave[3 * mw + m];
+    }
+  }
+}
+
 void orc_rpn(const orc_rp *p, int ixy, const double *qL, const double *qR,
              double *wave, double *s, double *amdq, double *apdq) {
   switch (p->kind) {
+    case ORC_RP_SHALLOW_HLLE: rpn_shallow_hlle(p, ixy, qL, qR, wave, s, amdq, apdq); break;
     case ORC_RP_ACOUSTICS: rpn_acoustics(p, ixy, qL, qR, wave, s, amdq, apdq); break;
     case ORC_RP_EULER: rpn_euler(p, ixy, qL, qR, wave, s, amdq, apdq); break;
     case ORC_RP_SHALLOW: rpn_shallow(p, ixy, qL, qR, wave, s, amdq, apdq); break;
@@ -1224,7 +1268,7 @@ void orc_evec(const orc_rp *p, int ixy, int meqn, int n2, const double *q, doubl
     const double *qL = &B1(q, 0, i - 1), *qR = &B1(q, 0, i);
     for (int a = 0; a < meqn; a++)
       for (int b = 0; b < meqn; b++) { EVL(a, b, i) = 0.0; EVR(a, b, i) = 0.0; }
-    if (p->kind == ORC_RP_SHALLOW) {
+    if (p->kind == ORC_RP_SHALLOW || p->kind == ORC_RP_SHALLOW_HLLE) {
       double h = (qL[0] + qR[0]) * 0.50;
       double hsqrtl = sqrt(qL[0]), hsqrtr = sqrt(qR[0]);
       double hsq2 = hsqrtl + hsqrtr;

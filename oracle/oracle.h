@@ -28,8 +28,9 @@ enum {
                             * meqn = orc_rp.meqn, mwaves 1, 2 aux fields (edge u, v) */
   ORC_RP_VC_ADVECTION_2D = 7, /* clawpack.riemann vc_advection_2D: the same rpn2 with
                                * meqn 1, and its transverse solver rpt2 */
-  ORC_RP_VC_ACOUSTICS = 8      /* clawpack.riemann vc_acoustics_2D: meqn 3, mwaves 2,
+  ORC_RP_VC_ACOUSTICS = 8,     /* clawpack.riemann vc_acoustics_2D: meqn 3, mwaves 2,
                                * aux = (density, sound speed) per cell */
+  ORC_RP_SHALLOW_HLLE = 9      /* clawpack.riemann shallow_hlle_2D: meqn 3, mwaves 2 */
 };
 
 /* Problem parameters: the reference passes these through the Fortran /cparam/

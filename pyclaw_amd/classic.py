@@ -98,10 +98,11 @@ class ClawSolver(Solver):
                                           "step with the variable-coefficient solvers")
             if state.index_capa >= state.num_aux:
                 raise ValueError("index_capa %d is not an aux field" % state.index_capa)
-        if (getattr(self.rp, "kind", 0) == 6 and self.num_dim == 2 and not self.dimensional_split
-                and self.transverse_waves != 0):
-            raise NotImplementedError("vc_advection_vector_2D has no transverse solver: use "
-                                      "dimensional splitting or transverse_waves = 0")
+        if (getattr(self.rp, "kind", 0) in (6, 9) and self.num_dim == 2
+                and not self.dimensional_split and self.transverse_waves != 0):
+            raise NotImplementedError("%s has no transverse solver here: use dimensional "
+                                      "splitting or transverse_waves = 0"
+                                      % self.rp.__name__.split(".")[-1])
         self._set_method(state)
         self._rp_struct = K.make_rp(self.rp.kind, state.problem_data, meqn=state.num_eqn)
         self._is_set_up = True

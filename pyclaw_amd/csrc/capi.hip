@@ -97,7 +97,10 @@ int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
   const int na = rp_num_aux(rp);
   if (na == 0 ? method[6] != 0 : (method[6] < na || method[6] > 3)) return CLAWHIP_E_METHOD;
   if (method[5] < 0 || method[5] > method[6]) return CLAWHIP_E_METHOD;
-  if (rp->kind == CLAWHIP_RP_VC_ADVECTION && method[2] != 0) return CLAWHIP_E_METHOD;
+  // solvers without a transverse solver
+  if ((rp->kind == CLAWHIP_RP_VC_ADVECTION || rp->kind == CLAWHIP_RP_SHALLOW_HLLE) &&
+      method[2] != 0)
+    return CLAWHIP_E_METHOD;
   for (int d = 0; d < 2; d++) {
     int lo = bc_lower[d], hi = bc_upper[d];
     if (lo < 1 || lo > 4 || hi < 1 || hi > 4) return CLAWHIP_E_ARG;
@@ -240,6 +243,7 @@ int clawhip_step2_unsplit_run(const clawhip_rp* rp, int dtype, const clawhip_geo
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
   if (method[2] < 0 || method[2] > 2) return CLAWHIP_E_METHOD;
   if (method[5] != 0 || method[6] != 0) return CLAWHIP_E_METHOD;
+  if (rp->kind == CLAWHIP_RP_SHALLOW_HLLE && method[2] != 0) return CLAWHIP_E_METHOD;
   for (int d = 0; d < 2; d++)
     if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
       return CLAWHIP_E_ARG;

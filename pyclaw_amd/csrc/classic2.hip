@@ -256,6 +256,10 @@ int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom
     case CLAWHIP_RP_ADVECTION:
       return dispatch_unsplit<Advection>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
                                          ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
+    case CLAWHIP_RP_SHALLOW_HLLE:
+      return dispatch_unsplit<ShallowHLLE>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
+                                           lo, hi, refl, ng, method, mthlim, dx, dy, dt,
+                                           cfl_bits, stream);
   }
   return CLAWHIP_E_KIND;
 }
@@ -277,6 +281,9 @@ int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int
     case CLAWHIP_RP_ADVECTION:
       return dispatch_unsplit_ctl<Advection>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
                                              bufs, nsteps, stream);
+    case CLAWHIP_RP_SHALLOW_HLLE:
+      return dispatch_unsplit_ctl<ShallowHLLE>(dtype, rp, g, lo, hi, refl, ng, method, mthlim,
+                                               ctl, bufs, nsteps, stream);
   }
   return CLAWHIP_E_KIND;
 }

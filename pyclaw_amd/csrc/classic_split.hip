@@ -538,6 +538,10 @@ int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
       return dispatch_split2<Advection>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                         work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
                                         dt, aux, cfl, s);
+    case CLAWHIP_RP_SHALLOW_HLLE:
+      return dispatch_split2<ShallowHLLE>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
+                                          work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
+                                          dt, aux, cfl, s);
   }
   return CLAWHIP_E_KIND;
 }
@@ -592,6 +596,9 @@ int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& g
     case CLAWHIP_RP_ADVECTION:
       return run_sweep<Advection<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
                                           dtdx, aux, cfl, s);
+    case CLAWHIP_RP_SHALLOW_HLLE:
+      return run_sweep<ShallowHLLE<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method,
+                                            mthlim, dtdx, aux, cfl, s);
   }
   return CLAWHIP_E_KIND;
 }

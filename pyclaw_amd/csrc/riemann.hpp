@@ -21,6 +21,7 @@
 //   euler_3D                  Roe, 3 waves (u-a, lumped entropy+shear, u+a), fix
 //   vc_advection(_vector)_2D  1 wave, edge velocities in aux
 //   vc_acoustics_2D           2 waves (-c left, +c right), rho and c in aux
+//   shallow_hlle_2D           HLLE, 2 waves at the Einfeldt speeds
 // Every expression keeps the reference's evaluation order; the library is
 // compiled with -ffp-contract=off so FP64 results match the CPU oracle bitwise.
 #pragma once
@@ -485,6 +486,70 @@ struct Euler3 {
       apdq[m] = df - amdq[m];
     }
   }
+  template <int IXY>
+  __device__ static __forceinline__ void rpt(const RPParams<T>&, const Roe&, const T (&)[meqn],
+                                             T (&bm)[meqn], T (&bp)[meqn]) {
+#pragma unroll
+    for (int m = 0; m < meqn; m++) { bm[m] = T(0); bp[m] = T(0); }
+  }
+};
+
+// ---------------------------------------------------------------- shallow water, HLLE
+// shallow_hlle_2D of the clawpack/riemann package (rpn2_shallow_hlle.f90, not
+// part of the reference tree: restated from its published form).  Two waves
+// at the Einfeldt speeds s1 = min(u_l - c_l, ubar - cbar) and
+// s2 = max(u_r + c_r, ubar + cbar) (ubar the Roe average, cbar =
+// sqrt(g (h_l + h_r) / 2)) around the HLL middle state
+// q* = (f(q_r) - f(q_l) - s2 q_r + s1 q_l) / (s1 - s2); each wave's flux
+// difference goes by the sign of its speed.  No transverse solver is paired
+// with it here: unsplit steps take it with transverse_waves = 0.
+template <typename T>
+struct ShallowHLLE {
+  static constexpr int meqn = 3, mwaves = 2, maux = 0;
+  struct Roe {};
+  // normal flux; the HLL waves give amdq + apdq = flux(qR) - flux(qL)
+  template <int IXY>
+  __device__ static __forceinline__ void flux(const RPParams<T>& p, const T* q, T (&f)[meqn]) {
+    Shallow<T>::template flux<IXY>(p, q, f);
+  }
+  template <int IXY>
+  __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
+                                             T (&wave)[mwaves][meqn], T (&s)[mwaves],
+                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T grav = p.grav;
+    const T h_l = qL[0], h_r = qR[0];
+    const T u_l = qL[mu] / qL[0], u_r = qR[mu] / qR[0];
+    const T hsqrt_l = sqrt(qL[0]), hsqrt_r = sqrt(qR[0]);
+    const T hsq2 = hsqrt_l + hsqrt_r;
+    const T ubar = (u_l * hsqrt_l + u_r * hsqrt_r) / hsq2;
+    const T cbar = sqrt(T(0.5) * grav * (h_l + h_r));
+    const T s1 = dmin(u_l - sqrt(grav * h_l), ubar - cbar);
+    const T s2 = dmax(u_r + sqrt(grav * h_r), ubar + cbar);
+    const T hm = (qR[mu] - qL[mu] - s2 * qR[0] + s1 * qL[0]) / (s1 - s2);
+    const T hum = (qR[mu] * u_r - qL[mu] * u_l + T(0.5) * grav * (h_r * h_r - h_l * h_l) -
+                   s2 * qR[mu] + s1 * qL[mu]) / (s1 - s2);
+    const T hvm = (qR[mv] * u_r - qL[mv] * u_l - s2 * qR[mv] + s1 * qL[mv]) / (s1 - s2);
+    wave[0][0] = hm - h_l;
+    wave[0][mu] = hum - qL[mu];
+    wave[0][mv] = hvm - qL[mv];
+    s[0] = s1;
+    wave[1][0] = h_r - hm;
+    wave[1][mu] = qR[mu] - hum;
+    wave[1][mv] = qR[mv] - hvm;
+    s[1] = s2;
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      amdq[m] = T(0);
+      apdq[m] = T(0);
+#pragma unroll
+      for (int w = 0; w < mwaves; w++) {
+        if (s[w] < T(0)) amdq[m] = amdq[m] + s[w] * wave[w][m];
+        else apdq[m] = apdq[m] + s[w] * wave[w][m];
+      }
+    }
+  }
+  // Not reached: the entry points refuse transverse_waves > 0 for this solver.
   template <int IXY>
   __device__ static __forceinline__ void rpt(const RPParams<T>&, const Roe&, const T (&)[meqn],
                                              T (&bm)[meqn], T (&bp)[meqn]) {

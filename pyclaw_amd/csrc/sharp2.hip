@@ -191,6 +191,10 @@ struct Evec<Shallow<T>, IXY, T> {
   }
 };
 
+// The shallow-water eigenvectors do not depend on the Riemann solver.
+template <int IXY, typename T>
+struct Evec<ShallowHLLE<T>, IXY, T> : Evec<Shallow<T>, IXY, T> {};
+
 template <int IXY, typename T>
 struct Evec<Euler<T>, IXY, T> {
   __device__ static __forceinline__ void get(const RPParams<T>& p, const T* qL, const T* qR,
@@ -764,12 +768,14 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
   if (char_decomp == 2) {
     switch (rp.kind) {
       case CLAWHIP_RP_SHALLOW: return dispatch_stage<Shallow, true>(ARGS);
+      case CLAWHIP_RP_SHALLOW_HLLE: return dispatch_stage<ShallowHLLE, true>(ARGS);
       case CLAWHIP_RP_EULER: return dispatch_stage<Euler, true>(ARGS);
     }
     return CLAWHIP_E_KIND;
   }
   switch (rp.kind) {
     case CLAWHIP_RP_SHALLOW: return dispatch_stage<Shallow, false>(ARGS);
+    case CLAWHIP_RP_SHALLOW_HLLE: return dispatch_stage<ShallowHLLE, false>(ARGS);
     case CLAWHIP_RP_EULER: return dispatch_stage<Euler, false>(ARGS);
     case CLAWHIP_RP_ACOUSTICS: return dispatch_stage<Acoustics, false>(ARGS);
     case CLAWHIP_RP_ADVECTION: return dispatch_stage<Advection, false>(ARGS);

@@ -218,7 +218,7 @@ class SharpClawSolver(Solver):
     def _positive_fields(self):
         """Fields that must stay positive: depth (shallow water), density and
         energy (Euler)."""
-        return {2: (0, 3), 3: (0,)}.get(getattr(self.rp, "kind", None), ())
+        return {2: (0, 3), 3: (0,), 9: (0,)}.get(getattr(self.rp, "kind", None), ())
 
     def _judge(self, q, comp, out_q, sb):
         """Full WENO5 judges the indicator's step.  The step is recomputed with

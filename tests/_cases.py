@@ -207,7 +207,7 @@ def seeded_state(rng, kind, shape):
         for d in range(nd):
             q[1 + d] = rho * vel[d]
         q[nd + 1] = p / 0.4 + 0.5 * rho * sum(v * v for v in vel)
-    elif kind == O.SHALLOW:
+    elif kind in (O.SHALLOW, O.SHALLOW_HLLE):
         h = rng.uniform(0.5, 1.5, shape)
         q[0] = h
         q[1] = h * rng.uniform(-0.5, 0.5, shape)

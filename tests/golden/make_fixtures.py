@@ -39,7 +39,8 @@ WANT = {
                      "classic_unsplit_trans_inc", "classic_unsplit_trans_cor",
                      "sharpclaw"],
     "euler_2d": ["quadrants_roe"],
-    "shallow_2d": ["radialdambreak_classic_roe", "radialdambreak_sharpclaw_roe"],
+    "shallow_2d": ["radialdambreak_classic_roe", "radialdambreak_sharpclaw_roe",
+                   "radialdambreak_classic_hlle", "radialdambreak_sharpclaw_hlle"],
     "euler_3d": ["shocktube", "shock_bubble"],
     "advection_2d_annulus": ["classic"],
 }

@@ -342,6 +342,28 @@ def test_fixture_radial_dam_break_classic(golden_dir):
     assert _l1(fr[-1][0], exp, d) < 1e-5
 
 
+def test_fixture_radial_dam_break_classic_hlle(golden_dir):
+    """examples/shallow_2d/test_shallow2d.py:25-33 with riemann_solver='hlle'
+    (split, MC; L1 < 1e-5): shallow_hlle_2D restated."""
+    q, d = C.radial_dam_break()
+    run = OracleRun(O.make_rp(O.SHALLOW_HLLE, grav=1.0), q, d, [1, 1], [3, 3], limiters=4)
+    fr = run.run(2.5, 10)
+    exp = np.load(f"{golden_dir}/shallow_2d_expected.npz")["radialdambreak_classic_hlle"]
+    err = _l1(fr[-1][0], exp, d)
+    print("radial dam break classic hlle L1", err)
+    assert err < 1e-5
+
+
+@pytest.mark.slow
+def test_fixture_radial_dam_break_sharpclaw_hlle(golden_dir):
+    q, d = C.radial_dam_break()
+    run = OracleRun(O.make_rp(O.SHALLOW_HLLE, grav=1.0), q, d, [1, 1], [3, 3],
+                    solver="sharp", cfl_desired=2.45, cfl_max=2.5, time_integrator="SSP104")
+    fr = run.run(2.5, 10)
+    exp = np.load(f"{golden_dir}/shallow_2d_expected.npz")["radialdambreak_sharpclaw_hlle"]
+    assert _l1(fr[-1][0], exp, d) < 1e-5
+
+
 @pytest.mark.slow
 def test_fixture_radial_dam_break_sharpclaw(golden_dir):
     q, d = C.radial_dam_break()
