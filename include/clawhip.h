@@ -187,6 +187,7 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
  *            5: FP32 SSP33 stage 2: inc = s*B + D; out = A + 0.25*inc; aux_out = inc/6
  *            6: FP32 SSP33 stage 3: Kahan update of A by aux_in + 2/3*D (comp_in->comp_out)
  *            7: out = D                         (fmod.flux2's dq on the interior)
+ *            8: out = 0.5*((A + Y) + D)         (SSP22, solver.py:388-393)
  * All arrays compact (ghosts = 0) with geometry g; num_ghost is 3 (WENO5).
  * Boundary codes 1..3 only.  *cfl_dev = max(*cfl_dev, CFL of this evaluation).
  * flags (may be NULL): per-interface troubled flags from clawhip_mlp_flags.
@@ -207,6 +208,17 @@ int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
 /* The elementwise middle of SSP104 (sharpclaw/solver.py:418-419):
  *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */
 int clawhip_ssp104_mid(int dtype, void *q, void *s1, long long n, void *stream);
+
+/* The time-integrator updates that take no operator evaluation, over n
+ * elements (SharpClawSolver.step, sharpclaw/solver.py:306-307 Euler and
+ * 351-374 SSPLMMk2/k3), with the reference's grouping; c[5] are the FP64
+ * scalars the solver computed:
+ *   mode 0: out = q + c1*u                                 (Euler: c1 = dt, u = dq_dt)
+ *   mode 1: out = c0*(c1*q + c2*u) + c3*R                  (SSPLMMk2)
+ *   mode 2: out = (c0*(c1*q + c2*u) + c3*R) + c4*v         (SSPLMMk3)
+ * R and v may be NULL where the mode does not read them; out may alias q. */
+int clawhip_lmm_update(int dtype, int mode, long long n, const void *q, const void *u,
+                       const void *R, const void *v, const double *c, void *out, void *stream);
 
 /* ---- ensembles: many independent unsplit 2D problems of uneven sizes (one
  * Riemann solver and method), each with its own dt and step controller,

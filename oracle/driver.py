@@ -7,7 +7,10 @@ TEST INFRASTRUCTURE ONLY (never imported by ``pyclaw_amd``).  It follows:
   Solver.get_dt / get_dt_new           solver.py:531-552
   ClawSolver.step                      classic/solver.py:96-146
   ClawSolver2D/3D.step_hyperbolic      classic/solver.py:498-549, 648-706
-  SharpClawSolver.step (SSP33, SSP104) sharpclaw/solver.py:283-323, 400-430
+  SharpClawSolver.step (Euler, SSP22, SSP33, SSP104, SSPLMMk2/k3)
+                                       sharpclaw/solver.py:283-430
+  SharpClawSolver.update_saved_values  sharpclaw/solver.py:432-490
+  SharpClawSolver.accept_reject_step   sharpclaw/solver.py:622-650
   SharpClawSolver.get_dt_new           sharpclaw/solver.py:653-685
   SharpClawSolver2D.dq_hyperbolic      sharpclaw/solver.py:848-898
 """
@@ -25,7 +28,7 @@ class OracleRun:
                  max_steps=10000, dt_variable=True, time_integrator="SSP104",
                  char_decomp=0, num_ghost=None, aux=None, aux_bc_lower=None,
                  aux_bc_upper=None, before_step=None, step_source=None, source_split=1,
-                 index_capa=-1, auxbc_fn=None):
+                 index_capa=-1, auxbc_fn=None, lmm_steps=4):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -63,6 +66,16 @@ class OracleRun:
         # capacity field (0-based aux index, -1: none); auxbc_fn(run) -> the aux
         # array with its ghosts filled, for custom aux boundary conditions
         self.index_capa, self.auxbc_fn = index_capa, auxbc_fn
+        # multistep state (SSPLMMk2 / k3)
+        self.lmm_steps = lmm_steps
+        self.sspcoeff = None
+        if time_integrator == "SSPLMMk2":
+            self.sspcoeff = (lmm_steps - 2.) / (lmm_steps - 1.)
+        elif time_integrator == "SSPLMMk3":
+            self.sspcoeff = (lmm_steps - 3.) / (lmm_steps - 1.)
+        self.sspcoeff0 = None
+        self.registers = [None] * lmm_steps
+        self.prev_dq_dt, self.prev_dt, self.prev_dtFE = [], [], []
 
     # ---- boundary conditions: Solver._apply_bcs (solver.py:351-437)
     def _apply_bcs(self, q):
@@ -88,8 +101,7 @@ class OracleRun:
                 if self.solver == "classic":
                     self.dt = min(self.dt_max, self.dt * self.cfl_desired / cfl)
                 else:
-                    self.dt_old = self.dt
-                    self.dt = self.dt * self.cfl_desired / cfl
+                    self._get_dt_new_sharp(cfl)
             else:
                 self.dt = self.dt_max
         else:
@@ -99,6 +111,59 @@ class OracleRun:
                 self.dt = tend - t
             if tend - t - self.dt < 1.e-14 * t:
                 self.dt = tend - t
+
+    # ---- SharpClawSolver.get_dt_new (sharpclaw/solver.py:653-685)
+    def _get_dt_new_sharp(self, cfl):
+        self.dt_old = self.dt
+        if "SSPLMM" in self.time_integrator:
+            step_index = self.numsteps + 1
+            if step_index < self.lmm_steps:
+                sspcoeff_ratio = self.sspcoeff0 / self.sspcoeff
+                self.dt = sspcoeff_ratio * self.dt * self.cfl_desired / cfl
+            elif self.accept_step:
+                s = self.lmm_steps
+                p = int(self.time_integrator[-1])
+                mu = min([self.prev_dtFE[i] for i in range(s)])
+                H = sum(self.prev_dt[1:])
+                self.dt = H * mu / (H + (p - 1) * mu)
+        else:
+            self.dt = self.dt * self.cfl_desired / cfl
+
+    # ---- SharpClawSolver.accept_reject_step (sharpclaw/solver.py:622-650)
+    def _accept(self):
+        if self.solver == "sharp" and "LMM" in self.time_integrator:
+            if self.numsteps + 1 < self.lmm_steps:
+                return not (self.cfl > self.sspcoeff0 / self.sspcoeff * self.cfl_max)
+            return True
+        return not (self.cfl > self.cfl_max)
+
+    # ---- SharpClawSolver.update_saved_values (sharpclaw/solver.py:432-490)
+    def _update_saved_values(self, step_index):
+        if self.prev_dt == []:
+            self.registers[-1] = self.q.copy(order="F")
+            self.prev_dq_dt.append(self.dq_dt)
+            self.prev_dt.append(0.)
+            dtFE = self.dt / self.cfl * self.cfl_max / self.sspcoeff
+            self.prev_dtFE.append(dtFE)
+            self.sspcoeff0 = 1.0   # _sspcoeff['SSP22']
+        elif self.accept_step:
+            dtFE = self.dt / self.cfl * self.cfl_max / self.sspcoeff
+            if step_index <= self.lmm_steps:
+                if self.time_integrator == "SSPLMMk3":
+                    self.prev_dq_dt.append(self.dq_dt)
+                self.prev_dt.append(self.dt_old)
+                self.prev_dtFE.append(dtFE)
+            else:
+                if self.time_integrator == "SSPLMMk3":
+                    self.prev_dq_dt = self.prev_dq_dt[1:] + self.prev_dq_dt[:1]
+                    self.prev_dq_dt[-1] = self.dq_dt
+                self.prev_dt = self.prev_dt[1:] + self.prev_dt[:1]
+                self.prev_dt[-1] = self.dt_old
+                self.prev_dtFE = self.prev_dtFE[1:] + self.prev_dtFE[:1]
+                self.prev_dtFE[-1] = dtFE
+            self.registers = self.registers[1:] + self.registers[:1]
+            self.registers[-1] = self.q.copy(order="F")
+        return step_index
 
     def _auxbc(self):
         if self.auxbc_fn is not None:
@@ -164,15 +229,40 @@ class OracleRun:
 
     # ---- SharpClawSolver.step (sharpclaw/solver.py:283-430)
     def _step_sharp(self, t, tstart, tend, take_one_step):
+        step_index = self.numsteps + 1
         if self.accept_step:
             self.cfl = 0.0
             self.dq_dt = self._dq(self.q) / self.dt
+        if "LMM" in self.time_integrator:
+            step_index = self._update_saved_values(step_index)
         self._get_dt(t, tstart, tend, take_one_step)
         cfl = self.cfl
         self.cfl = self.dt / self.dt_old * cfl
         self.dt_old = self.dt
         q, dt = self.q, self.dt
-        if self.time_integrator == "SSP33":
+        if self.time_integrator == "Euler":
+            self.q = q + dt * self.dq_dt
+        elif self.time_integrator == "SSP22" or ("SSPLMM" in self.time_integrator
+                                                   and step_index < self.lmm_steps):
+            r = q + dt * self.dq_dt
+            self.q = 0.5 * (q + r + self._dq(r))
+        elif self.time_integrator == "SSPLMMk2":
+            om = sum(self.prev_dt[1:]) / dt
+            r = (om - 1.) / om
+            delta = 1. / om ** 2
+            beta = (om + 1.) / om
+            self.q = beta * (r * q + dt * self.dq_dt) + delta * self.registers[-self.lmm_steps]
+        elif self.time_integrator == "SSPLMMk3":
+            om = sum(self.prev_dt[1:]) / dt
+            omk = om + 1.
+            r = (om - 2.) / om
+            delta0 = (4 * omk - om ** 2) / om ** 3
+            beta0 = omk / om ** 2
+            bk = omk ** 2 / om ** 2
+            self.q = (bk * (r * q + dt * self.prev_dq_dt[-1]) +
+                      (r * beta0 + delta0) * self.registers[-self.lmm_steps] +
+                      beta0 * dt * self.prev_dq_dt[-self.lmm_steps])
+        elif self.time_integrator == "SSP33":
             r = q + dt * self.dq_dt
             r = 0.75 * q + 0.25 * (r + self._dq(r))
             self.q = 1. / 3. * q + 2. / 3. * (r + self._dq(r))
@@ -208,7 +298,7 @@ class OracleRun:
                 self._step_classic(self.t, tstart, tend, take_one_step)
             else:
                 self._step_sharp(self.t, tstart, tend, take_one_step)
-            self.accept_step = not (self.cfl > self.cfl_max)
+            self.accept_step = self._accept()
             if self.accept_step:
                 if self.dt_variable:
                     self.t += self.dt

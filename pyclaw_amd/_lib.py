@@ -93,6 +93,8 @@ _SIGS = {
                              _I, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P, _P, _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
+    "clawhip_lmm_update": [_I, _I, ctypes.c_longlong, _P, _P, _P, _P,
+                           ctypes.POINTER(ctypes.c_double), _P, _P],
     "clawhip_step2_unsplit_run": [ctypes.POINTER(RP), _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                                   _I, _IP, _IP, _D, _D, _P, _P, _P, _P, _P, _I, _P],
     "clawhip_step2_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,

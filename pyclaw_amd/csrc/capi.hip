@@ -202,9 +202,9 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
-  if (y_mode < 0 || y_mode > 2 || out_mode < 0 || out_mode > 7) return CLAWHIP_E_ARG;
+  if (y_mode < 0 || y_mode > 2 || out_mode < 0 || out_mode > 8) return CLAWHIP_E_ARG;
   if (y_mode != 0 && !Q) return CLAWHIP_E_ARG;
-  if ((out_mode == 1 || out_mode == 2 || (out_mode >= 4 && out_mode <= 6)) && !A)
+  if ((out_mode == 1 || out_mode == 2 || (out_mode >= 4 && out_mode <= 6) || out_mode == 8) && !A)
     return CLAWHIP_E_ARG;
   if (out_mode == 5 && (!B || !aux_out || dtype != CLAWHIP_F32K)) return CLAWHIP_E_ARG;
   if (out_mode == 6 && (!aux_in || !comp_in || !comp_out || dtype != CLAWHIP_F32K))
@@ -218,6 +218,16 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
                             y_mode, out_mode, dt, s, dx, dy, P, Q, A, B, aux_in, comp_in, out,
                             aux_out, comp_out, (unsigned long long*)cfl_dev, flags,
                             troubled_cells, troubled_count, (hipStream_t)stream);
+}
+
+int clawhip_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
+                       const void* R, const void* v, const double* c, void* out,
+                       void* stream) {
+  if (!q || !u || !c || !out || n < 0 || mode < 0 || mode > 2) return CLAWHIP_E_ARG;
+  if (mode >= 1 && !R) return CLAWHIP_E_ARG;
+  if (mode == 2 && !v) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  return launch_lmm_update(dtype, mode, n, q, u, R, v, c, out, (hipStream_t)stream);
 }
 
 int clawhip_ssp104_mid(int dtype, void* q, void* s1, long long n, void* stream) {

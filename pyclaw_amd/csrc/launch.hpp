@@ -61,6 +61,9 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
                        const unsigned char* flags, const int* cells, const int* ncells,
                        hipStream_t stream);
 int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream);
+int launch_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
+                      const void* R, const void* v, const double* c, void* out,
+                      hipStream_t stream);
 
 int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int* lo,
                        const int* hi, const int* refl, int ng, const int* method,

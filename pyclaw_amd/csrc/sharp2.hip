@@ -45,7 +45,8 @@ enum StageOut {
   OUT_SSP104_FIN = 4,// out = A + (0.6*Y + 0.1*D)            (solver.py:430)
   OUT_F32_S2 = 5,    // FP32: out = A + 0.25*(e1 + D), aux = (e1 + D)/6, e1 = s*B
   OUT_F32_S3 = 6,    // FP32: Kahan(A, aux + 2/3*D, comp)
-  OUT_D = 7          // out = D (the operator itself, fmod.flux2's dq)
+  OUT_D = 7,         // out = D (the operator itself, fmod.flux2's dq)
+  OUT_SSP22 = 8      // out = 0.5*((A + Y) + D)              (solver.py:393)
 };
 enum StageY {
   Y_ARRAY = 0,       // Y = P
@@ -299,6 +300,7 @@ __device__ __forceinline__ void stage_store(const SharpArgs<T>& a, long long off
     case OUT_ADD6: o = yv + D / T(6); break;
     case OUT_SSP104_FIN: o = a.A[off + m * fs] + (T(0.6) * yv + T(0.1) * D); break;
     case OUT_D: o = D; break;
+    case OUT_SSP22: o = T(0.5) * ((a.A[off + m * fs] + yv) + D); break;
     case OUT_F32_S2: {
       T e1 = a.s * a.B[off + m * fs];
       T inc = e1 + D;
@@ -658,6 +660,28 @@ __global__ void k_ssp104_mid(T* q, T* s1, long long n) {
   s1[i] = T(15) * qn - T(5) * s1[i];
 }
 
+// The multistep updates that need no operator evaluation (SharpClawSolver.step,
+// sharpclaw/solver.py:306-307, 351-374), elementwise with the reference's
+// grouping; c = the scalar coefficients the solver computed in FP64:
+//   mode 0 (Euler):    out = q + c1*u                       (c1 = dt)
+//   mode 1 (SSPLMMk2): out = c0*(c1*q + c2*u) + c3*R        (beta, r, dt, delta)
+//   mode 2 (SSPLMMk3): out = (c0*(c1*q + c2*u) + c3*R) + c4*v
+//                      (beta_{k-1}, r, dt, r*beta0 + delta0, beta0*dt)
+template <typename T>
+__global__ void k_lmm_update(int mode, long long n, const T* q, const T* u, const T* R,
+                             const T* v, T c0, T c1, T c2, T c3, T c4, T* out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T o;
+  if (mode == 0) {
+    o = q[i] + c1 * u[i];
+  } else {
+    o = c0 * (c1 * q[i] + c2 * u[i]) + c3 * R[i];
+    if (mode == 2) o = o + c4 * v[i];
+  }
+  out[i] = o;
+}
+
 // ---------------------------------------------------------------- launchers
 namespace {
 // Rows per wavefront of the march form: with R wavefronts resident, W in the
@@ -782,6 +806,23 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
   }
 #undef ARGS
   return CLAWHIP_E_KIND;
+}
+
+int launch_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
+                      const void* R, const void* v, const double* c, void* out,
+                      hipStream_t stream) {
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (dtype == CLAWHIP_F64)
+    hipLaunchKernelGGL(k_lmm_update<double>, dim3(blocks), dim3(256), 0, stream, mode, n,
+                       (const double*)q, (const double*)u, (const double*)R, (const double*)v,
+                       c[0], c[1], c[2], c[3], c[4], (double*)out);
+  else
+    hipLaunchKernelGGL(k_lmm_update<float>, dim3(blocks), dim3(256), 0, stream, mode, n,
+                       (const float*)q, (const float*)u, (const float*)R, (const float*)v,
+                       (float)c[0], (float)c[1], (float)c[2], (float)c[3], (float)c[4],
+                       (float*)out);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
 }
 
 int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream) {

@@ -152,7 +152,8 @@ def step3_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthli
 
 # SharpClaw stage modes (include/clawhip.h, clawhip_sharp2_stage)
 Y_ARRAY, Y_AXPY, Y_AXPY6 = 0, 1, 2
-OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F32_S3, OUT_D = range(8)
+OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F32_S3, OUT_D, \
+    OUT_SSP22 = range(9)
 
 
 def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx, dy, P,
@@ -175,6 +176,16 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
         _ptr(cells[0]) if cells is not None else None,
         _ptr(cells[1]) if cells is not None else None, _ptr(cfl), _stream())
     L.check(rc, "clawhip_sharp2_stage")
+
+
+def lmm_update(mode, q, u, coeffs, out, R=None, v=None):
+    """Euler / SSPLMMk2 / SSPLMMk3 update (include/clawhip.h, clawhip_lmm_update)."""
+    c = (ctypes.c_double * 5)(*[float(x) for x in (list(coeffs) + [0.0] * 5)[:5]])
+    for t in (u, R, v, out):
+        assert t is None or (t.shape == q.shape and t.dtype == q.dtype and t.is_contiguous())
+    rc = L.lib().clawhip_lmm_update(_dtype_code(q), int(mode), int(q.numel()), _ptr(q), _ptr(u),
+                                    _ptr(R), _ptr(v), c, _ptr(out), _stream())
+    L.check(rc, "clawhip_lmm_update")
 
 
 def ssp104_mid(q, s1):

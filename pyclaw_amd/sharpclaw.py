@@ -2,8 +2,9 @@
 
 The semi-discrete operator dq = dt*L(q) (ghost fill + WENO5 reconstruction +
 Riemann solves) runs in clawhip_sharp2_stage, fused with the Runge-Kutta stage
-combination; SSP33 and SSP104 keep the reference's stage sequence and
-arithmetic (sharpclaw/solver.py:283-430), so the FP64 path equals the reference
+combination; Euler, SSP22, SSP33, SSP104 and the multistep SSPLMMk2/k3 keep the
+reference's stage sequence, step-size control and arithmetic
+(sharpclaw/solver.py:283-490, 595-690), so the FP64 path equals the reference
 step bit for bit.  precision = "fp32k" runs SSP33 with FP32 stages and a
 Kahan-compensated final update.
 """
@@ -14,8 +15,11 @@ from .solver import BC, Solver
 
 
 class SharpClawSolver(Solver):
-    _sspcoeff = {"Euler": 1.0, "SSP22": 1.0, "SSP33": 1.0, "SSP104": 6.0}
-    _cfl_default = {"SSP104": [2.45, 2.5]}
+    _sspcoeff = {"Euler": 1.0, "SSP22": 1.0, "SSP33": 1.0, "SSP104": 6.0,
+                 "SSPLMM32": 0.5, "SSPLMM43": 1. / 3., "SSPLMM53": 0.5}
+    _cfl_default = {"SSP104": [2.45, 2.5], "SSPLMM32": [0.24, 0.25],
+                    "SSPLMM43": [0.15, 1. / 6.], "SSPLMM53": [0.24, 0.25]}
+    _integrators = ("Euler", "SSP22", "SSP33", "SSP104", "SSPLMMk2", "SSPLMMk3")
 
     def __init__(self, riemann_solver=None, claw_package=None):
         self.limiters = [1]
@@ -42,6 +46,15 @@ class SharpClawSolver(Solver):
         self._bufs = {}
         self.indicator = None
         self._flags = None
+        # multistep methods (sharpclaw/solver.py:186-195)
+        self.sspcoeff0 = None
+        self.lmm_steps = 4
+        self.sspcoeff = None
+        self.prev_dq_dt_values = []
+        self.prev_dt_values = []
+        self.prev_dtFE_values = []
+        self.check_lmm_cond = False
+        self.lmm_cond = True
         super(SharpClawSolver, self).__init__(riemann_solver, claw_package)
 
     # ---- sharpclaw/solver.py:202-265
@@ -54,18 +67,40 @@ class SharpClawSolver(Solver):
             raise NotImplementedError("char_decomp must be 0 (component) or 2 (characteristic)")
         if self.tfluct_solver or self.fwave or self.dq_src is not None:
             raise NotImplementedError("tfluct solvers, f-waves and source terms are not supported")
-        if self.time_integrator not in ("SSP33", "SSP104"):
-            raise NotImplementedError("time_integrator must be 'SSP33' or 'SSP104'")
+        if self.time_integrator not in self._integrators:
+            raise NotImplementedError("time_integrator must be one of %s" % (self._integrators,))
         if self.precision == "fp32k" and self.time_integrator != "SSP33":
             raise NotImplementedError("the FP32 path implements SSP33")
+        if self.indicator is not None and self.time_integrator not in ("SSP33", "SSP104"):
+            raise NotImplementedError("the indicator runs with SSP33 or SSP104")
+        if self.check_lmm_cond:
+            raise NotImplementedError("check_lmm_cond (the third-order SSPLMM a-posteriori "
+                                      "conditions) is not supported")
         try:
-            if self.cfl_max is None:
+            if "SSPLMM" in self.time_integrator:
+                if self.time_integrator == "SSPLMMk2":
+                    assert 3 <= self.lmm_steps, \
+                        "Must set solver.lmm_steps greater than 2 for 2nd order SSPLMM integrator."
+                    self.sspcoeff = (self.lmm_steps - 2.) / (self.lmm_steps - 1.)
+                    if self.cfl_max is None:
+                        self.cfl_desired = 0.14 * self.sspcoeff
+                        self.cfl_max = 0.15 * self.sspcoeff
+                else:
+                    assert 4 <= self.lmm_steps <= 5, \
+                        "Must set solver.lmm_steps equal to 4 or 5 for 3rd order SSPLMM integrator."
+                    self.sspcoeff = (self.lmm_steps - 3.) / (self.lmm_steps - 1.)
+                    if self.cfl_max is None:
+                        self.cfl_desired = 0.48 * self.sspcoeff
+                        self.cfl_max = 0.5 * self.sspcoeff
+            elif self.cfl_max is None:
                 self.cfl_desired = self._cfl_default[self.time_integrator][0]
                 self.cfl_max = self._cfl_default[self.time_integrator][1]
             if self.cfl_desired is None:
                 self.cfl_desired = 0.9 * self.cfl_max
         except KeyError:
             raise KeyError("Maximum CFL number is not provided.")
+        self._registers = None
+        self.prev_dq_dt_values, self.prev_dt_values, self.prev_dtFE_values = [], [], []
         self._set_mthlim()
         state = solution.states[0]
         if state.num_aux > 0 or state.index_capa >= 0:
@@ -85,14 +120,35 @@ class SharpClawSolver(Solver):
         if len(self._mthlim) != n:
             raise Exception("Length of solver.limiters is not equal to 1 or to solver.num_eqn/num_waves")
 
-    # ---- sharpclaw/solver.py:653-685 (Runge-Kutta branch)
+    # ---- sharpclaw/solver.py:653-685
     def get_dt_new(self):
         self.dt_old = self.dt
         cfl = self.cfl.get_cached_max()
-        self.dt = self.dt * self.cfl_desired / cfl
+        if "SSPLMM" in self.time_integrator:
+            step_index = self.status["numsteps"] + 1
+            if step_index < self.lmm_steps:
+                # step size of the starting method
+                sspcoeff_ratio = self.sspcoeff0 / self.sspcoeff
+                self.dt = sspcoeff_ratio * self.dt * self.cfl_desired / cfl
+            elif self.accept_step:
+                # this step size keeps the multistep method within its CFL bound
+                s = self.lmm_steps
+                p = int(self.time_integrator[-1])
+                mu = min([self.prev_dtFE_values[i] for i in range(s)])
+                H = sum(self.prev_dt_values[1:])
+                self.dt = H * mu / (H + (p - 1) * mu)
+        else:
+            self.dt = self.dt * self.cfl_desired / cfl
 
+    # ---- sharpclaw/solver.py:622-650
     def accept_reject_step(self, state):
-        return not (self.cfl.get_cached_max() > self.cfl_max)
+        cfl = self.cfl.get_cached_max()
+        if "LMM" in self.time_integrator:
+            step_index = self.status["numsteps"] + 1
+            if step_index < self.lmm_steps:   # only the starting steps are checked
+                return not (cfl > self.sspcoeff0 / self.sspcoeff * self.cfl_max)
+            return True
+        return not (cfl > self.cfl_max)
 
     def _buf(self, name, like):
         t = self._bufs.get(name)
@@ -149,6 +205,7 @@ class SharpClawSolver(Solver):
         sb = self._step_buffer(q.shape[0])
         cs = sb[0:1]
         K.cfl_reset(cs)   # before stage 1: off the path between its CFL read and stage 2
+        step_index = self.status["numsteps"] + 1
         if self.accept_step:
             self.cfl.set_global_max(0.)
             dqdt = self._buf("dq_dt_new", q)
@@ -156,6 +213,8 @@ class SharpClawSolver(Solver):
             # keep the previous dq_dt buffer free for the next evaluation
             self._bufs["dq_dt_new"], self._bufs["dq_dt"] = self._bufs.get("dq_dt"), dqdt
             self.dq_dt = dqdt
+        if "LMM" in self.time_integrator:
+            step_index = self._update_saved_values(q, state, step_index)
         self.get_dt(solution.t, tstart, tend, take_one_step)
         cfl = self.cfl.get_cached_max()
         self.cfl.set_global_max(self.dt / self.dt_old * cfl)
@@ -164,7 +223,10 @@ class SharpClawSolver(Solver):
         # the later stages max-reduce their Courant numbers on the device; one
         # read at the end of the step (max is order-free, so self.cfl ends as the
         # stage-by-stage updates of sharpclaw/solver.py:283-430 leave it)
-        self._rk_stages(q, comp, self.dq_dt, out_q, self.dt, cs)
+        if "LMM" in self.time_integrator:
+            self._lmm_step(q, out_q, step_index, cs)
+        else:
+            self._rk_stages(q, comp, self.dq_dt, out_q, self.dt, cs)
         if self._flags is not None:
             out_q = self._judge(q, comp, out_q, sb)
         else:
@@ -174,10 +236,76 @@ class SharpClawSolver(Solver):
         self._flags = None
         return True
 
+    def _update_saved_values(self, q, state, step_index):
+        """sharpclaw/solver.py:432-490: the saved states, step sizes and
+        forward-Euler step sizes of the multistep method; the oldest saved state
+        is rolled to the end and overwritten with the current one."""
+        if self._registers is None:
+            self._registers = [torch.empty_like(q) for _ in range(self.lmm_steps)]
+        if self.prev_dt_values == []:
+            # the very first step
+            self._registers[-1].copy_(q)
+            self.prev_dq_dt_values.append(self.dq_dt.clone())
+            self.prev_dt_values.append(0.)   # not used
+            cfl = self.cfl.get_cached_max()
+            dtFE = self.dt / cfl * self.cfl_max / self.sspcoeff
+            self.prev_dtFE_values.append(dtFE)
+            self.sspcoeff0 = self._sspcoeff["SSP22"]
+        elif self.accept_step:
+            cfl = self.cfl.get_cached_max()
+            dtFE = self.dt / cfl * self.cfl_max / self.sspcoeff
+            if step_index <= self.lmm_steps:   # start-up
+                if self.time_integrator == "SSPLMMk3":
+                    self.prev_dq_dt_values.append(self.dq_dt.clone())
+                self.prev_dt_values.append(self.dt_old)
+                self.prev_dtFE_values.append(dtFE)
+            else:
+                if self.time_integrator == "SSPLMMk3":
+                    self.prev_dq_dt_values = self.prev_dq_dt_values[1:] + self.prev_dq_dt_values[:1]
+                    self.prev_dq_dt_values[-1].copy_(self.dq_dt)
+                self.prev_dt_values = self.prev_dt_values[1:] + self.prev_dt_values[:1]
+                self.prev_dt_values[-1] = self.dt_old
+                self.prev_dtFE_values = self.prev_dtFE_values[1:] + self.prev_dtFE_values[:1]
+                self.prev_dtFE_values[-1] = dtFE
+            self._registers = self._registers[1:] + self._registers[:1]
+            self._registers[-1].copy_(q)
+        return step_index
+
+    def _lmm_step(self, q, out_q, step_index, cs):
+        """sharpclaw/solver.py:351-374: SSP22 for the starting values, then the
+        SSPLMMk2 / k3 combination (no operator evaluation beyond dq_dt)."""
+        dt = self.dt
+        num_steps = self.lmm_steps
+        if step_index < num_steps:
+            self._stage(q, out_q[0], q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
+                        out_mode=K.OUT_SSP22, A=q)
+            return
+        omega_k_minus_1 = sum(self.prev_dt_values[1:]) / dt
+        if self.time_integrator == "SSPLMMk2":
+            r = (omega_k_minus_1 - 1.) / omega_k_minus_1
+            delta = 1. / omega_k_minus_1 ** 2
+            beta = (omega_k_minus_1 + 1.) / omega_k_minus_1
+            K.lmm_update(1, q, self.dq_dt, (beta, r, dt, delta), out_q[0],
+                         R=self._registers[-num_steps])
+        else:
+            omega_k = omega_k_minus_1 + 1.
+            r = (omega_k_minus_1 - 2.) / omega_k_minus_1
+            delta0 = (4 * omega_k - omega_k_minus_1 ** 2) / omega_k_minus_1 ** 3
+            beta0 = omega_k / omega_k_minus_1 ** 2
+            beta_k_minus_1 = omega_k ** 2 / omega_k_minus_1 ** 2
+            K.lmm_update(2, q, self.prev_dq_dt_values[-1],
+                         (beta_k_minus_1, r, dt, r * beta0 + delta0, beta0 * dt), out_q[0],
+                         R=self._registers[-num_steps], v=self.prev_dq_dt_values[-num_steps])
+
     def _rk_stages(self, q, comp, dq_dt, out_q, dt, cs):
         """The Runge-Kutta stages after the first operator evaluation
-        (sharpclaw/solver.py:312-323, 400-430), into out_q."""
-        if self.time_integrator == "SSP33":
+        (sharpclaw/solver.py:306-323, 388-430), into out_q."""
+        if self.time_integrator == "Euler":
+            K.lmm_update(0, q, dq_dt, (0.0, dt), out_q[0])
+        elif self.time_integrator == "SSP22":
+            self._stage(q, out_q[0], q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
+                        out_mode=K.OUT_SSP22, A=q)
+        elif self.time_integrator == "SSP33":
             r2 = self._buf("r2", q)
             if self.precision == "fp64":
                 self._stage(q, r2, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,

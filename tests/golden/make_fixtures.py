@@ -18,7 +18,8 @@ Sources (reference test that consumes each fixture):
   examples/euler_2d/test_euler2d.py:28-37
   examples/shallow_2d/test_shallow2d.py:25-36
   examples/euler_3d/test_euler3d.py:26-57
-  examples/acoustics_2d_homogeneous/test_2d_acoustics.py:19-30
+  examples/acoustics_2d_homogeneous/test_2d_acoustics.py:19-30 (verify_classic.txt,
+    verify_sharpclaw.txt, verify_sharpclaw_lmm.txt)
   examples/advection_2d_annulus/test_2d_advection_annulus.py:10-32
   examples/acoustics_2d_variable/test_acoustics_2d_variable.py:6-23 (pressure_classic.txt)
   examples/acoustics_2d_variable/test_acoustics_2d_variable_io.py:1-60 (io_test_verification)
@@ -98,6 +99,11 @@ def main():
     dst = os.path.join(HERE, "acoustics_2d_verify_classic.npz")
     np.savez_compressed(dst, pressure=p)
     print(dst, p.shape)
+    sharp = {k: np.loadtxt(os.path.join(REF, "acoustics_2d_homogeneous", f"verify_{k}.txt"))
+             for k in ("sharpclaw", "sharpclaw_lmm")}
+    dst = os.path.join(HERE, "acoustics_2d_verify_sharpclaw.npz")
+    np.savez_compressed(dst, **sharp)
+    print(dst, {k: v.shape for k, v in sharp.items()})
     p = np.loadtxt(os.path.join(REF, "acoustics_2d_variable", "pressure_classic.txt"))
     dst = os.path.join(HERE, "acoustics_2d_variable_pressure_classic.npz")
     np.savez_compressed(dst, pressure=p)

@@ -165,3 +165,55 @@ def test_fp32_kahan_char_weno_flat_regions_stay_finite():
     assert np.isfinite(qb).all()
     assert abs(tb - ta) < 1e-6 * ta
     assert np.abs(qb - qa).sum() / np.abs(qa).sum() < 1e-5
+
+
+def test_ssplmm_k2_acoustics_fixture(golden_dir):
+    """examples/acoustics_2d_homogeneous test_sharpclaw_multistep through the
+    PyClaw API: SSPLMMk2 with lmm_steps 3 (SSP22 starting steps, then the
+    multistep update), equal to the oracle run bit for bit and to the reference's
+    verify_sharpclaw_lmm.txt to round-off (bar: relative 1e-3)."""
+    from pyclaw_amd import riemann
+    q, d = C.acoustics_2d()
+    pd = {"rho": 1.0, "bulk": 4.0, "cc": 2.0, "zz": 2.0}
+    claw = _run_sharp(riemann.acoustics_2D, q, ([-1, -1], [1, 1]), [1, 1], [1, 1], 0.12, 10, pd,
+                      time_integrator="SSPLMMk2", lmm_steps=3, cfl_max=0.25, cfl_desired=0.24,
+                      dt_initial=min(d) / 2.0 * 0.24)
+    rp = O.make_rp(O.ACOUSTICS, rho=1.0, bulk=4.0)
+    run = OracleRun(rp, q, d, [1, 1], [1, 1], solver="sharp", cfl_desired=0.24, cfl_max=0.25,
+                    time_integrator="SSPLMMk2", lmm_steps=3, dt_initial=min(d) / rp.cc * 0.24)
+    run.run(0.12, 10)
+    got = claw.frames[-1].state.q
+    assert claw.solver.status["numsteps"] == run.numsteps
+    assert np.array_equal(got, run.q), np.abs(got - run.q).max()
+    exp = np.load(f"{golden_dir}/acoustics_2d_verify_sharpclaw.npz")["sharpclaw_lmm"]
+    rel = np.linalg.norm(exp - got[0]) / np.linalg.norm(exp)
+    print("SSPLMMk2 relative error", rel)
+    assert rel < 1e-12
+
+
+@pytest.mark.parametrize("ti,steps", [("SSPLMMk3", 4), ("SSPLMMk3", 5), ("SSPLMMk2", 6),
+                                      ("SSP22", 0), ("Euler", 0)])
+def test_more_integrators_match_oracle(ti, steps):
+    """SSPLMMk3 (4 and 5 steps), SSPLMMk2 with more steps (their default CFL
+    limits), SSP22 and forward Euler: whole runs on shallow water equal the
+    oracle's bit for bit (same accepted steps, same state)."""
+    from pyclaw_amd import riemann
+    rng = np.random.default_rng(55)
+    q = seeded_state(rng, O.SHALLOW, (40, 30))
+    opts = dict(time_integrator=ti, dt_initial=0.0005)
+    kw = dict(solver="sharp", time_integrator=ti, dt_initial=0.0005)
+    if steps:
+        opts["lmm_steps"] = kw["lmm_steps"] = steps
+        sc = (steps - 2.) / (steps - 1.) if ti == "SSPLMMk2" else (steps - 3.) / (steps - 1.)
+        cd, cm = (0.14 * sc, 0.15 * sc) if ti == "SSPLMMk2" else (0.48 * sc, 0.5 * sc)
+    else:
+        cd, cm = (0.45, 0.5) if ti == "SSP22" else (0.2, 0.25)
+        opts.update(cfl_desired=cd, cfl_max=cm)
+    claw = _run_sharp(riemann.shallow_roe_with_efix_2D, q, ([0, 0], [1, 1]), [2, 1], [2, 3],
+                      0.03, 2, {"grav": 1.0}, **opts)
+    run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, [1.0 / 40, 1.0 / 30], [2, 1],
+                    [2, 3], cfl_desired=cd, cfl_max=cm, **kw)
+    run.run(0.03, 2)
+    got = claw.frames[-1].state.q
+    assert claw.solver.status["numsteps"] == run.numsteps
+    assert np.array_equal(got, run.q), np.abs(got - run.q).max()

@@ -374,6 +374,25 @@ def test_fixture_radial_dam_break_sharpclaw(golden_dir):
     assert _l1(fr[-1][0], exp, d) < 1e-5
 
 
+@pytest.mark.parametrize("ti,key,cd,cm", [("SSPLMMk2", "sharpclaw_lmm", 0.24, 0.25),
+                                         ("SSP104", "sharpclaw", 0.45, 0.5)])
+def test_fixture_acoustics_verify_sharpclaw(golden_dir, ti, key, cd, cm):
+    """examples/acoustics_2d_homogeneous/test_2d_acoustics.py test_sharpclaw and
+    test_sharpclaw_multistep (SSPLMMk2, lmm_steps 3): relative 2-norm < 1e-3.
+    The multistep run meets its file to round-off."""
+    q, d = C.acoustics_2d()
+    rp = O.make_rp(O.ACOUSTICS, rho=1.0, bulk=4.0)
+    run = OracleRun(rp, q, d, [1, 1], [1, 1], solver="sharp", cfl_desired=cd, cfl_max=cm,
+                    time_integrator=ti, lmm_steps=3, dt_initial=min(d) / rp.cc * cd)
+    fr = run.run(0.12, 10)
+    exp = np.load(f"{golden_dir}/acoustics_2d_verify_sharpclaw.npz")[key]
+    rel = np.linalg.norm(exp - fr[-1][0]) / np.linalg.norm(exp)
+    print(ti, "relative error", rel, "steps", run.numsteps)
+    assert rel < 1e-3
+    if ti == "SSPLMMk2":
+        assert rel < 1e-12
+
+
 def test_fixture_acoustics_verify_classic(golden_dir):
     """examples/acoustics_2d_homogeneous/test_2d_acoustics.py:8-20 (rel L2 < 1e-3)."""
     q, d = C.acoustics_2d()
