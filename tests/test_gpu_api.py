@@ -260,3 +260,24 @@ def test_device_loop_equals_host_loop(fixed_dt):
         for fa, fb in zip(claw.frames, ref.frames):
             assert fa.t == fb.t
             assert np.array_equal(fa.state.q, fb.state.q)
+
+
+@pytest.mark.parametrize("module", ["acoustics_2D", "acoustics_2D_ptwise"])
+def test_api_acoustics_2d_verify_classic(golden_dir, module):
+    """examples/acoustics_2d_homogeneous test_classic and test_classic_ptwise
+    (split, MC, cfl 0.45 / 0.5, relative 2-norm of the pressure < 1e-3): the
+    pointwise module is the same device solver; both runs equal the oracle's."""
+    from pyclaw_amd import riemann
+    q, d = C.acoustics_2d()
+    rp = O.make_rp(O.ACOUSTICS, rho=1.0, bulk=4.0)
+    pd = {"rho": 1.0, "bulk": 4.0, "zz": rp.zz, "cc": rp.cc}
+    claw = _run_api(getattr(riemann, module), q, ([-1, -1], [1, 1]), [1, 1], [1, 1], 0.12,
+                    problem_data=pd, limiters=4, cfl_desired=0.45, cfl_max=0.5,
+                    dimensional_split=True, dt_initial=min(d) / rp.cc * 0.45)
+    run = OracleRun(rp, q, d, [1, 1], [1, 1], limiters=4, cfl_desired=0.45, cfl_max=0.5,
+                    dt_initial=min(d) / rp.cc * 0.45)
+    run.run(0.12, 10)
+    _check(claw, run, None, None, d)
+    exp = np.load(f"{golden_dir}/acoustics_2d_verify_classic.npz")["pressure"]
+    final = claw.frames[-1].state.q
+    assert np.linalg.norm(exp - final[0]) / np.linalg.norm(exp) < 1e-3
