@@ -195,15 +195,21 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
  * 0 uses WENO5's linear 5th-order weights; the others use the WENO5 selected by
  * char_decomp.  troubled_cells / troubled_count: the cells next to a flagged
  * interface, from clawhip_mlp_troubled_cells (required with flags).  NULL flags
- * = WENO5 everywhere (the reference's operator). */
+ * = WENO5 everywhere (the reference's operator).
+ * aux (geometry gaux, num_aux fields, codes aux_bc_lower/upper 1..4 as for
+ * clawhip_step2_split; 4 needs >= 3 stored ghosts): the solver's aux fields for
+ * the solvers that read them (kinds 6, 7, 8; char_decomp 0, no flags), else
+ * NULL.  The solve at an interface takes the aux of its two cells, the solve
+ * inside a cell that cell's aux (sharpclaw/flux1.f90:131, 174-181). */
 int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
                          const clawhip_geom *g, const int *bc_lower, const int *bc_upper,
                          const int *reflect, int y_mode, int out_mode, double dt, double s,
                          double dx, double dy, const void *P, const void *Q, const void *A,
                          const void *B, const void *aux_in, const void *comp_in, void *out,
                          void *aux_out, void *comp_out, const unsigned char *flags,
-                         const int *troubled_cells, const int *troubled_count, double *cfl_dev,
-                         void *stream);
+                         const int *troubled_cells, const int *troubled_count, const void *aux,
+                         const clawhip_geom *gaux, const int *aux_bc_lower,
+                         const int *aux_bc_upper, int num_aux, double *cfl_dev, void *stream);
 
 /* The elementwise middle of SSP104 (sharpclaw/solver.py:418-419):
  *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */
@@ -281,10 +287,13 @@ int clawhip_step3_sweep(const clawhip_rp *rp, int dtype, int idir, const void *q
 
 /* SharpClaw's flux2 (sharpclaw/flux2.f90:2-74; called at sharpclaw/solver.py:893):
  * dq = dt*L(q) on the interior (gout compact) from q with >= 3 stored, filled
- * ghost layers; char_decomp 0 (component WENO5) or 2 (characteristic). */
+ * ghost layers; char_decomp 0 (component WENO5) or 2 (characteristic).  aux
+ * (geometry gaux with >= 3 filled ghost layers): the solver's aux fields for the
+ * solvers that read them, else NULL. */
 int clawhip_sharp2_flux(const clawhip_rp *rp, int dtype, int char_decomp, const void *q,
                         const clawhip_geom *gin, void *dq, const clawhip_geom *gout, double dt,
-                        double dx, double dy, double *cfl_dev, void *stream);
+                        double dx, double dy, const void *aux, const clawhip_geom *gaux,
+                        double *cfl_dev, void *stream);
 
 /* ---- learned troubled-interface indicator (no counterpart in the reference;
  * the reference always runs the full WENO5 of reconstruct.f90:201-299).

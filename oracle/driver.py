@@ -223,7 +223,8 @@ class OracleRun:
     def _dq(self, q):
         qbc = self._apply_bcs(q)
         dx, dy = self.delta
-        dq, cfl = O.sharp_flux2(self.rp, qbc, self.ng, dx, dy, self.dt, self.char_decomp)
+        dq, cfl = O.sharp_flux2(self.rp, qbc, self.ng, dx, dy, self.dt, self.char_decomp,
+                                aux=None if self.aux is None else self._auxbc())
         self.cfl = max(self.cfl, cfl)
         return self._interior(dq)
 

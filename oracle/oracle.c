@@ -1308,9 +1308,14 @@ void orc_evec(const orc_rp *p, int ixy, int meqn, int n2, const double *q, doubl
 #undef EVR
 }
 
-/* sharpclaw/flux1.f90:76-268 with lim_type = 2, tfluct_solver = False, no capacity. */
+/* sharpclaw/flux1.f90:76-268 with lim_type = 2, tfluct_solver = False, no capacity.
+ * aux1d(maux, n2) (NULL: maux = 0): the Riemann solve at interface c takes the
+ * aux of cells c-1 and c, the solve inside cell c that cell's aux on both sides
+ * (the pointer shifts of flux1.f90:174-181). */
 double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy, const double *q1d,
-                       double *dq1d, double dtdx, int char_decomp, const orc_rp *p) {
+                       double *dq1d, double dtdx, int char_decomp, const orc_rp *p, int maux,
+                       const double *aux1d) {
+#define AX(k) (aux1d + (long)maux * ((k) - 1))   /* 1-based slice index k = c + ng */
   const int n2 = n + 2 * ng;
   double *ql = (double *)calloc((long)meqn * n2, sizeof(double));
   double *qr = (double *)calloc((long)meqn * n2, sizeof(double));
@@ -1331,7 +1336,11 @@ double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy, const doubl
   double *amdq_i = (double *)calloc((long)meqn * (n + 2), sizeof(double));
   double *apdq_i = (double *)calloc((long)meqn * (n + 2), sizeof(double));
   for (int c = 1; c <= n + 1; c++) {
-    orc_rpn(p, ixy, &B1(qr, 0, c - 1 + ng), &B1(ql, 0, c + ng), wave, s, amdq, apdq);
+    if (maux > 0)
+      orc_rpn_aux(p, ixy, &B1(qr, 0, c - 1 + ng), &B1(ql, 0, c + ng), AX(c - 1 + ng),
+                  AX(c + ng), wave, s, amdq, apdq);
+    else
+      orc_rpn(p, ixy, &B1(qr, 0, c - 1 + ng), &B1(ql, 0, c + ng), wave, s, amdq, apdq);
     for (int m = 0; m < meqn; m++) {
       amdq_i[m + (long)meqn * c] = amdq[m];
       apdq_i[m + (long)meqn * c] = apdq[m];
@@ -1342,7 +1351,11 @@ double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy, const doubl
   /* the CFL loop in the reference is mw-outer; max is exact so the order is free */
   for (int c = 1; c <= n; c++) {
     double amdq2[5], apdq2[5];
-    orc_rpn(p, ixy, &B1(ql, 0, c + ng), &B1(qr, 0, c + ng), wave, s, amdq2, apdq2);
+    if (maux > 0)
+      orc_rpn_aux(p, ixy, &B1(ql, 0, c + ng), &B1(qr, 0, c + ng), AX(c + ng), AX(c + ng),
+                  wave, s, amdq2, apdq2);
+    else
+      orc_rpn(p, ixy, &B1(ql, 0, c + ng), &B1(qr, 0, c + ng), wave, s, amdq2, apdq2);
     for (int m = 0; m < meqn; m++) {
       amdq_n[m] = amdq_i[m + (long)meqn * (c + 1)];
       B1(dq1d, m, c + ng) = B1(dq1d, m, c + ng) -
@@ -1351,14 +1364,18 @@ double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy, const doubl
   }
   free(amdq_i); free(apdq_i); free(ql); free(qr);
   return cfl;
+#undef AX
 }
 
 #define Q2(a, m, i, j) (a)[(m) + (long)meqn * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
-/* sharpclaw/flux2.f90:2-74; dq is zeroed first (f2py optional intent(in,out)). */
+/* sharpclaw/flux2.f90:2-74; dq is zeroed first (f2py optional intent(in,out)).
+ * aux(maux, 1-ng:mx+ng, 1-ng:my+ng) (NULL: maux = 0), sliced like q (46, 64). */
 double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my, const double *q,
                        double *dq, double dx, double dy, double dt, int char_decomp,
-                       const orc_rp *p) {
+                       const orc_rp *p, int maux, const double *aux) {
   int maxm = mx > my ? mx : my;
+  double *a1d = maux > 0 ? (double *)calloc((long)maux * (maxm + 2 * ng), sizeof(double)) : NULL;
+#define AUX2(ma, i, j) aux[(ma) + (long)maux * (((i) + ng - 1) + (long)(mx + 2 * ng) * ((j) + ng - 1))]
   long nq = (long)meqn * (mx + 2 * ng) * (my + 2 * ng);
   for (long k = 0; k < nq; k++) dq[k] = 0.0;
   double *q1d = (double *)calloc((long)meqn * (maxm + 2 * ng), sizeof(double));
@@ -1368,7 +1385,10 @@ double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my, const doubl
   for (int j = 0; j <= my + 1; j++) {
     for (int i = 1 - ng; i <= mx + ng; i++)
       for (int m = 0; m < meqn; m++) B1(q1d, m, i + ng) = Q2(q, m, i, j);
-    double c1 = orc_sharp_flux1(meqn, mwaves, ng, mx, 1, q1d, dq1d, dtdx, char_decomp, p);
+    for (int i = 1 - ng; i <= mx + ng && maux > 0; i++)
+      for (int ma = 0; ma < maux; ma++) a1d[ma + (long)maux * (i + ng - 1)] = AUX2(ma, i, j);
+    double c1 = orc_sharp_flux1(meqn, mwaves, ng, mx, 1, q1d, dq1d, dtdx, char_decomp, p, maux,
+                                a1d);
     cfl = dmax(cfl, c1);
     for (int i = 1; i <= mx; i++)
       for (int m = 0; m < meqn; m++) Q2(dq, m, i, j) = Q2(dq, m, i, j) + B1(dq1d, m, i + ng);
@@ -1376,12 +1396,16 @@ double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my, const doubl
   for (int i = 0; i <= mx + 1; i++) {
     for (int j = 1 - ng; j <= my + ng; j++)
       for (int m = 0; m < meqn; m++) B1(q1d, m, j + ng) = Q2(q, m, i, j);
-    double c1 = orc_sharp_flux1(meqn, mwaves, ng, my, 2, q1d, dq1d, dtdy, char_decomp, p);
+    for (int j = 1 - ng; j <= my + ng && maux > 0; j++)
+      for (int ma = 0; ma < maux; ma++) a1d[ma + (long)maux * (j + ng - 1)] = AUX2(ma, i, j);
+    double c1 = orc_sharp_flux1(meqn, mwaves, ng, my, 2, q1d, dq1d, dtdy, char_decomp, p, maux,
+                                a1d);
     cfl = dmax(cfl, c1);
     for (int j = 1; j <= my; j++)
       for (int m = 0; m < meqn; m++) Q2(dq, m, i, j) = Q2(dq, m, i, j) + B1(dq1d, m, j + ng);
   }
-  free(q1d); free(dq1d);
+  free(q1d); free(dq1d); free(a1d);
+#undef AUX2
   return cfl;
 }
 

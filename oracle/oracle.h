@@ -110,14 +110,14 @@ double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz,
  * orc_evec.  Returns the CFL number. */
 double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my,
                        const double *q, double *dq, double dx, double dy, double dt,
-                       int char_decomp, const orc_rp *p);
+                       int char_decomp, const orc_rp *p, int maux, const double *aux);
 
 /* One slice of the SharpClaw operator (sharpclaw/flux1.f90) for a slice of n
  * cells plus ng ghosts each side: q1d(meqn, 1-ng:n+ng) -> dq1d(meqn, 1-ng:n+ng).
  * Returns the slice CFL. */
 double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy,
                        const double *q1d, double *dq1d, double dtdx,
-                       int char_decomp, const orc_rp *p);
+                       int char_decomp, const orc_rp *p, int maux, const double *aux1d);
 
 /* Eigenvectors at each interface of a slice (evec ABI of
  * examples/euler_1d/evec.f90:2-21, with a direction argument added):

@@ -168,12 +168,15 @@ def step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir, use_fwave=
 
 
 # ---------------------------------------------------------------- SharpClaw
-def sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp):
+def sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp, aux=None):
+    """aux: aux(maux, 1-ng:mx+ng, 1-ng:my+ng) with ghosts filled, or None."""
     meqn, nx, ny = q.shape
     dq = np.zeros_like(q, order="F")
+    maux = 0 if aux is None else aux.shape[0]
     cfl = lib().orc_sharp_flux2(meqn, MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, _dp(q),
                                 _dp(dq), ctypes.c_double(dx), ctypes.c_double(dy),
-                                ctypes.c_double(dt), int(char_decomp), ctypes.byref(rp))
+                                ctypes.c_double(dt), int(char_decomp), ctypes.byref(rp),
+                                int(maux), _dp(aux) if aux is not None else None)
     return dq, cfl
 
 
@@ -182,7 +185,7 @@ def sharp_flux1(rp, q1d, ng, ixy, dtdx, char_decomp):
     dq1d = np.zeros_like(q1d, order="F")
     cfl = lib().orc_sharp_flux1(meqn, MWAVES[rp.kind], ng, n2 - 2 * ng, ixy, _dp(q1d),
                                 _dp(dq1d), ctypes.c_double(dtdx), int(char_decomp),
-                                ctypes.byref(rp))
+                                ctypes.byref(rp), 0, None)
     return dq1d, cfl
 
 
@@ -273,13 +276,17 @@ def ref_step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir):
                                      _ip(mthlim))
 
 
-def ref_sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp):
+def ref_sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp, aux=None):
+    """The reference's SharpClaw flux2 (aux: ghost-filled aux fields or None; the
+    Riemann solver with aux is the oracle's)."""
     meqn, nx, ny = q.shape
     dq = np.zeros_like(q, order="F")
+    maux = 0 if aux is None else aux.shape[0]
     cfl = ref_sharp().ref_sharp_flux2(ctypes.byref(rp), int(char_decomp), meqn,
                                       MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, _dp(q),
                                       _dp(dq), ctypes.c_double(dx), ctypes.c_double(dy),
-                                      ctypes.c_double(dt))
+                                      ctypes.c_double(dt), int(maux),
+                                      _dp(aux) if aux is not None else None)
     return dq, cfl
 
 

@@ -249,18 +249,19 @@ extern void flux1_(const double *q1d, double *dq1d, const double *aux, const dou
 /* SharpClawSolver2D.dq_hyperbolic's call (sharpclaw/solver.py:893) with the
  * module set-up of _set_fortran_parameters (sharpclaw/solver.py:563-588). */
 double ref_sharp_flux2(const orc_rp *p, int char_decomp, int meqn, int mwaves, int ng, int mx,
-                       int my, const double *q, double *dq, double dx, double dy, double dt) {
+                       int my, const double *q, double *dq, double dx, double dy, double dt,
+                       int num_aux, const double *auxq) {
   shim_set_rp(p);
-  int maxm = mx > my ? mx : my, num_aux = 0;
+  int maxm = mx > my ? mx : my;
   double dxs[2] = {dx, dy};
   int mthlim[8] = {1, 1, 1, 1, 1, 1, 1, 1};
   ref_sharp_init(2, 2, 5, char_decomp, mwaves, dxs, mthlim, maxm + 2 * ng, ng, meqn);
   if (char_decomp == 2) shim_set_evec_count(my + 2);
   long nq = (long)meqn * (mx + 2 * ng) * (my + 2 * ng);
   memset(dq, 0, sizeof(double) * nq);
-  double aux[1] = {0.0}, t = 0.0, cfl = 0.0;
-  flux2_(q, dq, aux, &dt, &cfl, &t, &num_aux, &meqn, &ng, &maxm, &mx, &my, (rp_fn)shim_rpn2_,
-         (rp_fn)shim_unused_);
+  double aux0[1] = {0.0}, t = 0.0, cfl = 0.0;
+  flux2_(q, dq, num_aux > 0 ? auxq : aux0, &dt, &cfl, &t, &num_aux, &meqn, &ng, &maxm, &mx, &my,
+         num_aux > 0 ? (rp_fn)shim_rpn2_aux_ : (rp_fn)shim_rpn2_, (rp_fn)shim_unused_);
   return cfl;
 }
 

@@ -91,7 +91,7 @@ _SIGS = {
                             _D, _D, _D, _D, _P, _P],
     "clawhip_sharp2_stage": [ctypes.POINTER(RP), _I, _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                              _I, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                             _P, _P, _P],
+                             _P, _P, ctypes.POINTER(Geom), _IP, _IP, _I, _P, _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_lmm_update": [_I, _I, ctypes.c_longlong, _P, _P, _P, _P,
                            ctypes.POINTER(ctypes.c_double), _P, _P],
@@ -103,7 +103,7 @@ _SIGS = {
     "clawhip_step3_sweep": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _I, _IP, _IP, _D, _D, _D, _D, _P, _P],
     "clawhip_sharp2_flux": [ctypes.POINTER(RP), _I, _I, _P, ctypes.POINTER(Geom), _P,
-                            ctypes.POINTER(Geom), _D, _D, _D, _P, _P],
+                            ctypes.POINTER(Geom), _D, _D, _D, _P, ctypes.POINTER(Geom), _P, _P],
     "clawhip_ensemble_table_bytes": [_I],
     "clawhip_ensemble_prepare": [ctypes.POINTER(RP), _I, ctypes.POINTER(EnsMember), _I, _IP,
                                  _IP, _IP, _I, _IP, _IP, _I, _P,

@@ -47,7 +47,7 @@ int clawhip_cfl_reset(double* cfl_dev, void* stream) {
 
 // aux: NULL, or the solver's aux fields (planar like q, geometry gaux with the
 // interior of q; stored ghosts only with CLAWHIP_BC_GIVEN aux codes).
-static int aux_in(const void* aux, const clawhip_geom* gaux, const int* alo, const int* ahi,
+static int make_aux_in(const void* aux, const clawhip_geom* gaux, const int* alo, const int* ahi,
                   const clawhip_geom* gq, int num_ghost, AuxIn& out) {
   if (!aux) return 0;
   if (!gaux || !alo || !ahi || !geom_ok(gaux, 2)) return CLAWHIP_E_ARG;
@@ -108,7 +108,7 @@ int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
       return CLAWHIP_E_ARG;
   }
   AuxIn ax;
-  int rc = aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, gout, num_ghost, ax);
+  int rc = make_aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, gout, num_ghost, ax);
   if (rc) return rc;
   if ((na > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   return launch_unsplit(*rp, dtype, q_in, to_geom(gin), q_out, to_geom(gout), comp_in, comp_out,
@@ -162,7 +162,7 @@ int clawhip_step2_split(const clawhip_rp* rp, int dtype, const void* q_in,
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
   if (method[2] != -1) return CLAWHIP_E_METHOD;
   AuxIn ax;
-  rc = aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, gout, num_ghost, ax);
+  rc = make_aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, gout, num_ghost, ax);
   if (rc) return rc;
   if ((rp_num_aux(rp) > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   return launch_split2(*rp, dtype, q_in, to_geom(gin), q_out, to_geom(gout), comp_in, comp_out,
@@ -194,8 +194,9 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
                          double dx, double dy, const void* P, const void* Q, const void* A,
                          const void* B, const void* aux_in, const void* comp_in, void* out,
                          void* aux_out, void* comp_out, const unsigned char* flags,
-                         const int* troubled_cells, const int* troubled_count, double* cfl_dev,
-                         void* stream) {
+                         const int* troubled_cells, const int* troubled_count, const void* aux,
+                         const clawhip_geom* gaux, const int* aux_bc_lower,
+                         const int* aux_bc_upper, int num_aux, double* cfl_dev, void* stream) {
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !P || !out || !cfl_dev)
     return CLAWHIP_E_ARG;
   if (flags && (!troubled_cells || !troubled_count)) return CLAWHIP_E_ARG;
@@ -214,10 +215,17 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
   for (int d = 0; d < 2; d++)
     if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
       return CLAWHIP_E_ARG;
+  // aux: the solver's fields (at least those it reads, at most 3)
+  const int na = rp_num_aux(rp);
+  if (na > 0 && (num_aux < na || num_aux > 3)) return CLAWHIP_E_ARG;
+  AuxIn ax;
+  int rc = make_aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, g, 3, ax);
+  if (rc) return rc;
+  if ((na > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), to_geom(g), bc_lower, bc_upper, reflect,
                             y_mode, out_mode, dt, s, dx, dy, P, Q, A, B, aux_in, comp_in, out,
                             aux_out, comp_out, (unsigned long long*)cfl_dev, flags,
-                            troubled_cells, troubled_count, (hipStream_t)stream);
+                            troubled_cells, troubled_count, (hipStream_t)stream, ax);
 }
 
 int clawhip_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
@@ -342,7 +350,7 @@ int clawhip_step2_sweep(const clawhip_rp* rp, int dtype, int ids, const void* q_
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
   const int given[2] = {CLAWHIP_BC_GIVEN, CLAWHIP_BC_GIVEN};
   AuxIn ax;
-  rc = aux_in(aux, gaux, given, given, gout, num_ghost, ax);
+  rc = make_aux_in(aux, gaux, given, given, gout, num_ghost, ax);
   if (rc) return rc;
   if ((rp_num_aux(rp) > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   Geom a = to_geom(gin), b = to_geom(gout);
@@ -368,7 +376,8 @@ int clawhip_step3_sweep(const clawhip_rp* rp, int dtype, int idir, const void* q
 
 int clawhip_sharp2_flux(const clawhip_rp* rp, int dtype, int char_decomp, const void* q,
                         const clawhip_geom* gin, void* dq, const clawhip_geom* gout, double dt,
-                        double dx, double dy, double* cfl_dev, void* stream) {
+                        double dx, double dy, const void* aux, const clawhip_geom* gaux,
+                        double* cfl_dev, void* stream) {
   if (!rp || !q || !gin || !dq || !gout || !cfl_dev) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_ARG;
@@ -379,10 +388,14 @@ int clawhip_sharp2_flux(const clawhip_rp* rp, int dtype, int char_decomp, const 
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
   const int given[2] = {CLAWHIP_BC_GIVEN, CLAWHIP_BC_GIVEN};
   const int refl[2] = {1, 2};
+  AuxIn ax;
+  int rc = make_aux_in(aux, gaux, given, given, gout, 3, ax);
+  if (rc) return rc;
+  if ((rp_num_aux(rp) > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(gout), to_geom(gin), given, given,
                             refl, 0, 7, dt, 0.0, dx, dy, q, nullptr, nullptr, nullptr, nullptr,
                             nullptr, dq, nullptr, nullptr, (unsigned long long*)cfl_dev, nullptr,
-                            nullptr, nullptr, (hipStream_t)stream);
+                            nullptr, nullptr, (hipStream_t)stream, ax);
 }
 
 int clawhip_mlp_param_count(void) { return 16 * 32 + 32 * 32 + 32 + 32 + 1; }

@@ -79,6 +79,9 @@ struct SharpArgs {
   const unsigned char* flags;   // troubled-interface flags (learned indicator) or null
   const int* cells;             // with flags: cells next to a flagged interface
   const int* ncells;            //   and their number (device)
+  const T* qaux;                // the solver's aux fields (solvers with maux > 0), geometry
+  Geom gaux;                    //   gaux, boundary codes bcaux (never negated)
+  BCSpec bcaux;
 };
 
 
@@ -356,10 +359,16 @@ struct MarchArgs {
 // arithmetic (their linearisation satisfies A_hat (qr - ql) = f(qr) - f(ql), and
 // the entropy fixes only move fluctuation between amdq and apdq).  FP64 keeps the
 // reference's Riemann solve, bit for bit.
+//
+// Solvers with aux fields (flux1.f90:131, 174-181): the solve at an interface
+// takes the aux of its two cells, the solve inside a cell that cell's aux on
+// both sides; they keep that solve in FP32 too (no flux-difference shortcut).
 template <class RS, typename T, bool CHAR, bool FLAGS>
 __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   constexpr int ME = RS::meqn, MW = RS::mwaves;
-  constexpr bool TFLUCT = sizeof(T) == 4;
+  constexpr bool AUX = RS::maux > 0;
+  constexpr int MA = AUX ? RS::maux : 1;
+  constexpr bool TFLUCT = sizeof(T) == 4 && !AUX;
   const SharpArgs<T>& a = ma.s;
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -386,9 +395,14 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
         y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
     }
   };
-  // x-slice of row r: interface c (left of this lane's cell) -> qlR, qrL, am, ap
-  auto x_interface = [&](const T (&row)[ME], int r, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME],
-                         T (&ap)[ME]) {
+  auto load_a = [&](int r, T (&A)[MA]) {
+    if constexpr (AUX) load_cell2<MA>(a.qaux, a.gaux, a.bcaux, a.ng, c, r, A);
+    else A[0] = T(0);
+  };
+  // x-slice of row r (aux Ar of this lane's cell): interface c (left of this
+  // lane's cell) -> qlR, qrL, am, ap
+  auto x_interface = [&](const T (&row)[ME], const T (&Ar)[MA], int r, T (&qrL)[ME],
+                         T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
     // st[k] = the row value of lane l + k - 3 (chained one-lane moves)
     T st[6][ME];
 #pragma unroll
@@ -409,16 +423,18 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     }
     T W[MW][ME], sp[MW];
     typename RS::Roe roe;
-    RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
+    T AL[MA];
+    if constexpr (AUX) from_left<MA>(Ar, AL);
+    rpn_call<RS, 1>(rp, qrL, qlR, AL, Ar, W, sp, am, ap, roe);
     if (xint_lane && counted) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
     }
   };
-  // y-interface t (rows t-1 | t) of this lane's column from the window win[0..5]
-  // = rows t-3..t+2
-  auto y_interface = [&](const T (&win)[6][ME], int t, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME],
-                         T (&ap)[ME]) {
+  // y-interface t (rows t-1 | t, aux Alo | Ahi) of this lane's column from the
+  // window win[0..5] = rows t-3..t+2
+  auto y_interface = [&](const T (&win)[6][ME], const T (&Alo)[MA], const T (&Ahi)[MA], int t,
+                         T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
     bool counted = true;
     if (FLAGS) {
       if (c >= 1 && c <= mx && t <= my + 1) counted = !yflag(a.flags, mx, my, c, t);
@@ -428,20 +444,20 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     }
     T W[MW][ME], sp[MW];
     typename RS::Roe roe;
-    RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
+    rpn_call<RS, 2>(rp, qrL, qlR, Alo, Ahi, W, sp, am, ap, roe);
     if (ycfl_lane && t <= my + 1 && counted) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
     }
   };
   // x contribution of this lane's cell in row r (a whole x-slice pass)
-  auto x_cell = [&](const T (&row)[ME], int r, T (&dqx)[ME]) {
+  auto x_cell = [&](const T (&row)[ME], const T (&Ar)[MA], int r, T (&dqx)[ME]) {
     T qrL[ME], qlR[ME], am[ME], ap[ME];
-    x_interface(row, r, qrL, qlR, am, ap);
+    x_interface(row, Ar, r, qrL, qlR, am, ap);
     T qr[ME], amn[ME];
     from_right<ME>(qrL, qr);     // right edge of this cell: interface c+1
     from_right<ME>(am, amn);     // amdq at interface c+1
-    if (TFLUCT) {
+    if constexpr (TFLUCT) {
       T fl[ME], fr[ME];
       RS::template flux<1>(rp, qlR, fl);
       RS::template flux<1>(rp, qr, fr);
@@ -450,25 +466,28 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     } else {
       T W[MW][ME], sp[MW], am2[ME], ap2[ME];
       typename RS::Roe roe;
-      RS::template rpn<1>(rp, qlR, qr, W, sp, am2, ap2, roe);
+      rpn_call<RS, 1>(rp, qlR, qr, Ar, Ar, W, sp, am2, ap2, roe);
 #pragma unroll
       for (int m = 0; m < ME; m++)
         dqx[m] = T(0) - a.dtdx * (((amn[m] + ap[m]) + am2[m]) + ap2[m]);
     }
   };
 
-  // ---- window rows r0-3 .. r0+2; y-interface r0
+  // ---- window rows r0-3 .. r0+2; y-interface r0; aux of rows t-1 (Alo), t (Ahi)
   T win[6][ME];
 #pragma unroll
   for (int k = 0; k < 6; k++) load_y(r0 - 3 + k, win[k]);
+  T Alo[MA], Ahi[MA];
+  load_a(r0 - 1, Alo);
+  load_a(r0, Ahi);
   if (r0 == 1) {   // x-slice of row 0 (CFL only)
     T d[ME];
-    x_cell(win[2], 0, d);
+    x_cell(win[2], Alo, 0, d);
   }
   T qlY[ME], apY[ME];   // interface t-1 carried: lower edge of row t-1, apdq
   {
     T qrL[ME], am[ME];
-    y_interface(win, r0, qrL, qlY, am, apY);
+    y_interface(win, Alo, Ahi, r0, qrL, qlY, am, apY);
   }
   for (int t = r0 + 1; t <= rend + 1; t++) {
     // row t-1 = win[3]; shift the window to rows t-3..t+2
@@ -477,11 +496,14 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
 #pragma unroll
       for (int m = 0; m < ME; m++) win[k][m] = win[k + 1][m];
     load_y(t + 2, win[5]);
+#pragma unroll
+    for (int k = 0; k < MA; k++) Alo[k] = Ahi[k];
+    load_a(t, Ahi);
     // y-interface t closes row t-1
     T qrL[ME], qlR[ME], am[ME], ap[ME];
-    y_interface(win, t, qrL, qlR, am, ap);
+    y_interface(win, Alo, Ahi, t, qrL, qlR, am, ap);
     T dqy[ME];
-    if (TFLUCT) {
+    if constexpr (TFLUCT) {
       T fl[ME], fr[ME];
       RS::template flux<2>(rp, qlY, fl);
       RS::template flux<2>(rp, qrL, fr);
@@ -490,14 +512,14 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     } else {
       T W[MW][ME], sp[MW], am2[ME], ap2[ME];
       typename RS::Roe roe;
-      RS::template rpn<2>(rp, qlY, qrL, W, sp, am2, ap2, roe);
+      rpn_call<RS, 2>(rp, qlY, qrL, Alo, Alo, W, sp, am2, ap2, roe);
 #pragma unroll
       for (int m = 0; m < ME; m++)
         dqy[m] = T(0) - a.dtdy * (((am[m] + apY[m]) + am2[m]) + ap2[m]);
     }
     // x-slice of row t-1
     T dqx[ME];
-    x_cell(win[2], t - 1, dqx);
+    x_cell(win[2], Alo, t - 1, dqx);
     const int r = t - 1;
     if (out_lane) {
       const long long off = (long long)(c - 1) + (long long)(r - 1) * a.g.rs;
@@ -509,7 +531,7 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   }
   if (rend == my) {   // x-slice of row my+1 (CFL only) = window row index 3 now
     T d[ME];
-    x_cell(win[3], my + 1, d);
+    x_cell(win[3], Ahi, my + 1, d);
   }
   double cx = (double)a.dtdx * (double)smax_x;
   double cy = (double)a.dtdy * (double)smax_y;
@@ -724,7 +746,7 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
               const void* P, const void* Q, const void* A, const void* B, const void* aux_in,
               const void* comp_in, void* out, void* aux_out, void* comp_out,
               unsigned long long* cfl, const unsigned char* flags, const int* cells,
-              const int* ncells, hipStream_t stream) {
+              const int* ncells, hipStream_t stream, const AuxIn& ax) {
   SharpArgs<T> a;
   a.rp = to_params<T>(rp);
   a.g = g;
@@ -742,18 +764,29 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
   a.flags = flags;
   a.cells = cells;
   a.ncells = ncells;
+  a.qaux = (const T*)ax.p;
+  a.gaux = ax.g;
+  const int never[3] = {-1, -1, -1};
+  a.bcaux = to_bc(ax.lo, ax.hi, never, 2);
+  if ((RS::maux > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
+  if constexpr (RS::maux > 0) {
+    if (flags) return CLAWHIP_E_ARG;   // the indicator path takes solvers without aux
+  }
   MarchArgs<T> ma;
   ma.s = a;
   ma.nstrips = (g.mx + 57) / 58;
-  ma.H = flags ? march_rows(k_sharp_march<RS, T, CHAR, true>, ma.nstrips, g.my)
-               : march_rows(k_sharp_march<RS, T, CHAR, false>, ma.nstrips, g.my);
+  ma.H = (flags && RS::maux == 0)
+             ? march_rows(k_sharp_march<RS, T, CHAR, RS::maux == 0>, ma.nstrips, g.my)
+             : march_rows(k_sharp_march<RS, T, CHAR, false>, ma.nstrips, g.my);
   ma.nrb = (g.my + ma.H - 1) / ma.H;
   const unsigned blocks = (unsigned)(((long long)ma.nstrips * ma.nrb + 3) / 4);
   if (flags) {
-    hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
-                       ma);
-    hipLaunchKernelGGL((k_sharp_patch<RS, T, CHAR>), dim3(1024), dim3(256), 0, stream, a,
-                       cells, ncells);
+    if constexpr (RS::maux == 0) {
+      hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
+                         ma);
+      hipLaunchKernelGGL((k_sharp_patch<RS, T, CHAR>), dim3(1024), dim3(256), 0, stream, a,
+                         cells, ncells);
+    }
   } else {
     hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, false>), dim3(blocks), dim3(256), 0, stream,
                        ma);
@@ -769,14 +802,14 @@ int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const Geom& g
                    double dy, const void* P, const void* Q, const void* A, const void* B,
                    const void* aux_in, const void* comp_in, void* out, void* aux_out,
                    void* comp_out, unsigned long long* cfl, const unsigned char* flags,
-                   const int* cells, const int* ncells, hipStream_t stream) {
+                   const int* cells, const int* ncells, hipStream_t stream, const AuxIn& ax) {
   if (dtype == CLAWHIP_F64)
     return run_stage<RSF<double>, double, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx,
                                                 dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
-                                                comp_out, cfl, flags, cells, ncells, stream);
+                                                comp_out, cfl, flags, cells, ncells, stream, ax);
   return run_stage<RSF<float>, float, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
                                             Q, A, B, aux_in, comp_in, out, aux_out, comp_out,
-                                            cfl, flags, cells, ncells, stream);
+                                            cfl, flags, cells, ncells, stream, ax);
 }
 }  // namespace
 
@@ -786,9 +819,23 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
                        const unsigned char* flags, const int* cells, const int* ncells,
-                       hipStream_t stream) {
+                       hipStream_t stream, const AuxIn& ax) {
 #define ARGS dtype, rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P, Q, A, B, aux_in, \
-             comp_in, out, aux_out, comp_out, cfl, flags, cells, ncells, stream
+             comp_in, out, aux_out, comp_out, cfl, flags, cells, ncells, stream, ax
+  if (rp.kind == CLAWHIP_RP_VC_ACOUSTICS || rp.kind == CLAWHIP_RP_VC_ADVECTION ||
+      rp.kind == CLAWHIP_RP_VC_ADVECTION_2D) {
+    if (char_decomp != 0) return CLAWHIP_E_KIND;   // component WENO5 only
+    switch (rp.kind == CLAWHIP_RP_VC_ACOUSTICS ? 0
+            : rp.kind == CLAWHIP_RP_VC_ADVECTION_2D ? 1 : rp.meqn) {
+      case 0: return dispatch_stage<VCAcoustics, false>(ARGS);
+      case 1: return dispatch_stage<VCAdvection1, false>(ARGS);
+      case 2: return dispatch_stage<VCAdvection2, false>(ARGS);
+      case 3: return dispatch_stage<VCAdvection3, false>(ARGS);
+      case 4: return dispatch_stage<VCAdvection4, false>(ARGS);
+    }
+    return CLAWHIP_E_ARG;
+  }
+  if (ax.p) return CLAWHIP_E_ARG;
   if (char_decomp == 2) {
     switch (rp.kind) {
       case CLAWHIP_RP_SHALLOW: return dispatch_stage<Shallow, true>(ARGS);

@@ -67,16 +67,20 @@ def flux2(q, aux, dt, t, num_ghost, maxm, mx, my, rpn2, tfluct2):
         raise NotImplementedError("char_decomp 0 or 2, no capacity function")
     if num_ghost < 3:
         raise ValueError("WENO5 needs 3 ghost cells")
-    rp = rp_struct(rpn2, cparam)
+    rp = rp_struct(rpn2, cparam, meqn=q.shape[0])
     dx, dy = [float(d) for d in p.dx]
     gin = L.geom2(mx, my, num_ghost)
     gout = L.geom2(mx, my, 0)
     qin = to_device2(q)
     out = qin.new_empty((q.shape[0], my, mx))
     cfl = new_cfl()
+    auxd = to_device2(aux) if (aux is not None and np.size(aux) > 1 and
+                               getattr(rp, "kind", 0) in (6, 7, 8)) else None
     rc = L.lib().clawhip_sharp2_flux(ctypes.byref(rp), L.F64, int(p.char_decomp), ptr(qin),
                                      ctypes.byref(gin), ptr(out), ctypes.byref(gout), float(dt),
-                                     dx, dy, ptr(cfl), stream())
+                                     dx, dy, ptr(auxd) if auxd is not None else None,
+                                     ctypes.byref(gin) if auxd is not None else None,
+                                     ptr(cfl), stream())
     L.check(rc, "clawhip_sharp2_flux")
     dq = np.zeros_like(q, order="F")
     dq[:, num_ghost:num_ghost + mx, num_ghost:num_ghost + my] = np.transpose(out.cpu().numpy(),

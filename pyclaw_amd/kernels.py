@@ -158,23 +158,36 @@ OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F3
 
 def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx, dy, P,
                  Q=None, s=0.0, y_mode=Y_ARRAY, out_mode=OUT_D, A=None, B=None, aux_in=None,
-                 comp_in=None, aux_out=None, comp_out=None, flags=None, cells=None):
+                 comp_in=None, aux_out=None, comp_out=None, flags=None, cells=None, qaux=None,
+                 qaux_bc_lower=None, qaux_bc_upper=None):
     """D = dt*L(Y) (SharpClaw flux2) fused with one RK stage; planar (meqn, my, mx).
     flags: uint8 troubled-interface flags (mlp_flags) selecting full WENO5 per
     interface, with cells = (list, count) from mlp_troubled_cells (built here when
-    None); flags None = full WENO5 everywhere."""
+    None); flags None = full WENO5 everywhere.  qaux: the solver's aux fields
+    (planar, compact or with 3 stored ghosts and codes 4) for the solvers that
+    read them."""
     if flags is not None and cells is None:
         cells = mlp_troubled_cells(flags, out.shape[2], out.shape[1])
     meqn, my, mx = out.shape
+    _check_fields(rp, meqn, None, None)
+    for t in (P, Q, A, B):
+        assert t is None or (t.dtype == out.dtype and t.numel() >= out.numel())
     g = L.geom2(mx, my)
     code = _dtype_code(out)
+    gaux = _aux_geom(qaux, mx, my)
+    if qaux is not None:
+        assert qaux.dtype == out.dtype and qaux.is_contiguous()
     rc = L.lib().clawhip_sharp2_stage(
         ctypes.byref(rp), code, int(char_decomp), ctypes.byref(g), L.ints(bc_lower),
         L.ints(bc_upper), L.ints(reflect), int(y_mode), int(out_mode), float(dt), float(s),
         float(dx), float(dy), _ptr(P), _ptr(Q), _ptr(A), _ptr(B), _ptr(aux_in), _ptr(comp_in),
         _ptr(out), _ptr(aux_out), _ptr(comp_out), _ptr(flags),
         _ptr(cells[0]) if cells is not None else None,
-        _ptr(cells[1]) if cells is not None else None, _ptr(cfl), _stream())
+        _ptr(cells[1]) if cells is not None else None, _ptr(qaux),
+        ctypes.byref(gaux) if qaux is not None else None,
+        L.ints(qaux_bc_lower) if qaux is not None else None,
+        L.ints(qaux_bc_upper) if qaux is not None else None,
+        int(qaux.shape[0]) if qaux is not None else 0, _ptr(cfl), _stream())
     L.check(rc, "clawhip_sharp2_stage")
 
 

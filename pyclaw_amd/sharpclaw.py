@@ -46,6 +46,7 @@ class SharpClawSolver(Solver):
         self._bufs = {}
         self.indicator = None
         self._flags = None
+        self._qaux = None     # (device aux, lower codes, upper codes) for the current step
         # multistep methods (sharpclaw/solver.py:186-195)
         self.sspcoeff0 = None
         self.lmm_steps = 4
@@ -103,11 +104,22 @@ class SharpClawSolver(Solver):
         self.prev_dq_dt_values, self.prev_dt_values, self.prev_dtFE_values = [], [], []
         self._set_mthlim()
         state = solution.states[0]
-        if state.num_aux > 0 or state.index_capa >= 0:
-            raise NotImplementedError("aux arrays and capacity functions are not supported on the device")
         if self.rp is None or not hasattr(self.rp, "kind"):
             raise Exception("the solver needs one of the pyclaw_amd.riemann modules")
-        self._rp_struct = K.make_rp(self.rp.kind, state.problem_data)
+        rp_aux = getattr(self.rp, "num_aux", 0)
+        if state.index_capa >= 0:
+            raise NotImplementedError("capacity functions are not supported by SharpClaw on the device")
+        if rp_aux == 0 and state.num_aux != 0:
+            raise NotImplementedError("aux fields are supported with the variable-coefficient "
+                                      "solvers (vc_advection, vc_acoustics)")
+        if rp_aux and not rp_aux <= state.num_aux <= 3:
+            raise NotImplementedError("the Riemann solver reads %d aux fields; the device "
+                                      "operator takes %d to 3, the state has %d"
+                                      % (rp_aux, rp_aux, state.num_aux))
+        if rp_aux and (self.char_decomp != 0 or self.indicator is not None):
+            raise NotImplementedError("solvers with aux fields run with char_decomp = 0 and "
+                                      "no indicator")
+        self._rp_struct = K.make_rp(self.rp.kind, state.problem_data, meqn=state.num_eqn)
         self._is_set_up = True
 
     def _set_mthlim(self):
@@ -169,9 +181,11 @@ class SharpClawSolver(Solver):
         if now:
             cfl = self._cfl_buffer()
             K.cfl_reset(cfl)
+        qaux, alo, ahi = self._qaux if self._qaux is not None else (None, None, None)
         K.sharp2_stage(self._rp_struct, self.char_decomp, out, cfl, lo, hi, self.reflect_index,
                        self.dt, dx, dy, P, flags=self._flags,
-                       cells=self.indicator.cells if self._flags is not None else None, **kw)
+                       cells=self.indicator.cells if self._flags is not None else None,
+                       qaux=qaux, qaux_bc_lower=alo, qaux_bc_upper=ahi, **kw)
         if now:
             self.cfl.update_global_max(self._read_back(cfl)[0])
 
@@ -198,6 +212,9 @@ class SharpClawSolver(Solver):
         state = solution.states[0]
         self._grid = state.grid
         q, comp = state.device_q(self._dtype_code)
+        # aux and its ghosts once per step (dq_hyperbolic applies the aux BCs of
+        # the step's state at every stage, sharpclaw/solver.py:848-898)
+        self._qaux = self._aux_input(state) if state.num_aux > 0 else None
         self._flags = None
         if self.indicator is not None:
             self._flags = self.indicator.step_flags(q, self._bc_codes(), self.reflect_index,
@@ -402,6 +419,7 @@ class SharpClawSolver(Solver):
         from .device import q_from_device
         self._grid = state.grid
         q, _ = state.device_q(self._dtype_code)
+        self._qaux = self._aux_input(state) if state.num_aux > 0 else None
         out = torch.empty_like(q)
         self._stage(q, out, q, out_mode=K.OUT_D)
         return q_from_device(out)

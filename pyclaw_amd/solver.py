@@ -183,6 +183,23 @@ class Solver(object):
                 out.append(int(b))
         return lo, hi
 
+    def _aux_input(self, state):
+        """Device aux and its boundary codes.  With a custom aux BC the ghosts are
+        filled on the host in the reference's order (solver.py:371-437; the user
+        function gets qbc = None: q lives on the device) and stored (code 4)."""
+        from .device import q_to_device
+        alo, ahi = self._aux_bc_codes()
+        if BC.custom not in alo and BC.custom not in ahi:
+            return state.device_aux(self._dtype_code), alo, ahi
+        ng = self.num_ghost
+        aux = np.asarray(state.aux)
+        auxbc = np.zeros((aux.shape[0],) + tuple(n + 2 * ng for n in aux.shape[1:]), order="F")
+        auxbc[(slice(None),) + (slice(ng, -ng),) * self.num_dim] = aux
+        self.auxbc = auxbc
+        from .classic import _fill_bcs_host
+        _fill_bcs_host(self, state, auxbc, alo, ahi, aux=True)
+        return q_to_device(auxbc, self._dtype_code), [4] * self.num_dim, [4] * self.num_dim
+
     # ---- time-step control (solver.py:524-552)
     def accept_reject_step(self, state):
         cfl = self.cfl.get_cached_max()

@@ -108,6 +108,10 @@ def main():
     dst = os.path.join(HERE, "acoustics_2d_variable_pressure_classic.npz")
     np.savez_compressed(dst, pressure=p)
     print(dst, p.shape)
+    p = np.loadtxt(os.path.join(REF, "acoustics_2d_variable", "pressure_sharpclaw.txt"))
+    dst = os.path.join(HERE, "acoustics_2d_variable_pressure_sharpclaw.npz")
+    np.savez_compressed(dst, pressure=p)
+    print(dst, p.shape)
     # the io test's verification frames, kept verbatim (they are ascii data)
     io_src = os.path.join(REF, "acoustics_2d_variable", "io_test_verification")
     io_dst = os.path.join(HERE, "acoustics_2d_variable_io")

@@ -217,3 +217,94 @@ def test_more_integrators_match_oracle(ti, steps):
     got = claw.frames[-1].state.q
     assert claw.solver.status["numsteps"] == run.numsteps
     assert np.array_equal(got, run.q), np.abs(got - run.q).max()
+
+
+@pytest.mark.parametrize("kind,meqn,maux", [(O.VC_ACOUSTICS, 3, 2), (O.VC_ACOUSTICS, 3, 3),
+                                            (O.VC_ADVECTION_2D, 1, 2), (O.VC_ADVECTION, 2, 2)])
+def test_operator_aux_bit_exact(kind, meqn, maux):
+    """The SharpClaw operator with aux fields (interface solves with both cells'
+    aux, in-cell solves with the cell's own) equals the oracle's, whose aux
+    operator equals the reference's flux2 (tests/test_oracle.py); compact aux
+    with boundary codes and aux with stored ghosts; FP32 close."""
+    from pyclaw_amd import kernels as K
+    from pyclaw_amd.device import q_from_device, q_to_device
+    rng = np.random.default_rng(61 + kind + maux)
+    lo, hi = (0.5, 2.0) if kind == O.VC_ACOUSTICS else (-1.5, 1.5)
+    for (mx, my), (bcl, bcu) in zip([(45, 37), (64, 9)], [([3, 1], [2, 3]), ([1, 2], [1, 2])]):
+        print("sharp aux", kind, meqn, maux, mx, my, flush=True)
+        q = np.asfortranarray(rng.uniform(-1, 1, (meqn, mx, my)))
+        aux = np.asfortranarray(rng.uniform(lo, hi, (maux, mx, my)))
+        ng = 3
+        qbc, abc = O.with_ghosts(q, ng), O.with_ghosts(aux, ng)
+        O.bc2(qbc, ng, bcl, bcu)
+        O.bc2(abc, ng, bcl, bcu, reflect=(-1, -1))
+        dx, dy = 1.0 / mx, 0.9 / my
+        dt = 0.1 * min(dx, dy)
+        rp_o = O.make_rp(kind, meqn=meqn)
+        dq, cref = O.sharp_flux2(rp_o, qbc, ng, dx, dy, dt, 0, aux=abc)
+        ref = O.interior(dq, ng)
+        rp = K.make_rp(kind, {}, meqn=meqn)
+        for given in (False, True):
+            out = torch.empty_like(q_to_device(q))
+            cfl = torch.zeros(1, dtype=torch.float64, device="cuda")
+            K.sharp2_stage(rp, 0, out, cfl, bcl, bcu, [1, 2], dt, dx, dy, q_to_device(q),
+                           qaux=q_to_device(abc if given else aux),
+                           qaux_bc_lower=[4, 4] if given else bcl,
+                           qaux_bc_upper=[4, 4] if given else bcu)
+            torch.cuda.synchronize()
+            got = q_from_device(out)
+            assert np.array_equal(got, ref), (given, np.abs(got - ref).max())
+            assert float(cfl.item()) == cref
+        out32 = torch.empty_like(q_to_device(q, 1))
+        cfl32 = torch.zeros(1, dtype=torch.float64, device="cuda")
+        K.sharp2_stage(rp, 0, out32, cfl32, bcl, bcu, [1, 2], dt, dx, dy, q_to_device(q, 1),
+                       qaux=q_to_device(aux, 1), qaux_bc_lower=bcl, qaux_bc_upper=bcu)
+        torch.cuda.synchronize()
+        assert np.abs(q_from_device(out32) - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-30)
+
+
+@pytest.mark.parametrize("ti,cd,cm", [("SSP104", 2.45, 2.5), ("SSPLMMk2", 0.24, 0.25)])
+def test_acoustics_2d_variable_sharpclaw_fixture(golden_dir, ti, cd, cm):
+    """examples/acoustics_2d_variable test_sharpclaw / test_sharpclaw_multistep
+    through the PyClaw API (vc_acoustics_2D with aux density and sound speed,
+    walls / extrapolation for q and aux): equal to the oracle run bit for bit,
+    and within the fixture's 0.1 of pressure_sharpclaw.txt."""
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    q0, aux0, d = C.acoustics_2d_variable()
+    solver = pyclaw.SharpClawSolver2D(riemann.vc_acoustics_2D)
+    solver.time_integrator = ti
+    if ti == "SSPLMMk2":
+        solver.lmm_steps = 3
+        solver.cfl_max = cm
+        solver.cfl_desired = cd
+    solver.bc_lower[0] = pyclaw.BC.wall
+    solver.bc_upper[0] = pyclaw.BC.extrap
+    solver.bc_lower[1] = pyclaw.BC.wall
+    solver.bc_upper[1] = pyclaw.BC.extrap
+    solver.aux_bc_lower[0] = pyclaw.BC.wall
+    solver.aux_bc_upper[0] = pyclaw.BC.extrap
+    solver.aux_bc_lower[1] = pyclaw.BC.wall
+    solver.aux_bc_upper[1] = pyclaw.BC.extrap
+    domain = pyclaw.Domain([pyclaw.Dimension(-1.0, 1.0, 50, name="x"),
+                            pyclaw.Dimension(-1.0, 1.0, 50, name="y")])
+    state = pyclaw.State(domain, 3, 2)
+    state.aux[...] = aux0
+    state.q[...] = q0
+    claw = pyclaw.Controller()
+    claw.keep_copy = True
+    claw.output_format = None
+    claw.solution = pyclaw.Solution(state, domain)
+    claw.solver = solver
+    claw.tfinal = 0.6
+    claw.num_output_times = 20
+    claw.run()
+    run = OracleRun(O.make_rp(O.VC_ACOUSTICS), q0, d, [3, 3], [1, 1], solver="sharp",
+                    time_integrator=ti, cfl_desired=cd, cfl_max=cm, lmm_steps=3, aux=aux0,
+                    aux_bc_lower=[3, 3], aux_bc_upper=[1, 1])
+    run.run(0.6, 20)
+    final = claw.frames[-1].state.q
+    assert claw.solver.status["numsteps"] == run.numsteps
+    assert np.array_equal(final, run.q), np.abs(final - run.q).max()
+    p = np.load(f"{golden_dir}/acoustics_2d_variable_pressure_sharpclaw.npz")["pressure"]
+    assert np.abs(p.reshape(-1) - final[0].reshape(-1)).max() < 1e-1

@@ -197,6 +197,41 @@ def test_live_reference_vc_acoustics_bit_exact():
         assert np.array_equal(a[:, ng:-ng, ng:-ng], b[:, ng:-ng, ng:-ng]) and c1 == c2
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_live_reference_sharp_aux_bit_exact():
+    """The SharpClaw operator with aux fields (sharpclaw/flux1.f90:131, 174-181:
+    interface solves with both cells' aux, in-cell solves with the cell's own)
+    against the reference's own flux2, the solvers supplied by the oracle."""
+    rng = np.random.default_rng(17)
+    ng, mx, my = 3, 11, 9
+    for kind, meqn, maux in [(O.VC_ACOUSTICS, 3, 2), (O.VC_ADVECTION_2D, 1, 3),
+                             (O.VC_ADVECTION, 2, 2)]:
+        rp = O.make_rp(kind, meqn=meqn)
+        qbc = O.with_ghosts(np.asfortranarray(rng.uniform(-1, 1, (meqn, mx, my))), ng)
+        lo, hi = (0.5, 2.5) if kind == O.VC_ACOUSTICS else (-1.5, 1.5)
+        abc = O.with_ghosts(np.asfortranarray(rng.uniform(lo, hi, (maux, mx, my))), ng)
+        O.bc2(qbc, ng, [3, 1], [2, 3])
+        O.bc2(abc, ng, [3, 1], [2, 3], reflect=(-1, -1))
+        d1, c1 = O.sharp_flux2(rp, qbc, ng, 0.1, 0.08, 0.01, 0, aux=abc)
+        d2, c2 = O.ref_sharp_flux2(rp, qbc, ng, 0.1, 0.08, 0.01, 0, aux=abc)
+        assert np.array_equal(d1, d2) and c1 == c2, kind
+
+
+@pytest.mark.parametrize("ti,cd,cm", [("SSP104", 2.45, 2.5), ("SSPLMMk2", 0.24, 0.25)])
+def test_fixture_acoustics_2d_variable_sharpclaw(golden_dir, ti, cd, cm):
+    """examples/acoustics_2d_variable test_sharpclaw and test_sharpclaw_multistep
+    (max |p - pressure_sharpclaw| < 0.1)."""
+    q, aux, d = C.acoustics_2d_variable()
+    run = OracleRun(O.make_rp(O.VC_ACOUSTICS), q, d, [3, 3], [1, 1], solver="sharp",
+                    time_integrator=ti, cfl_desired=cd, cfl_max=cm, lmm_steps=3, aux=aux,
+                    aux_bc_lower=[3, 3], aux_bc_upper=[1, 1])
+    run.run(0.6, 20)
+    p = np.load(f"{golden_dir}/acoustics_2d_variable_pressure_sharpclaw.npz")["pressure"]
+    err = np.abs(p.reshape(-1) - run.q[0].reshape(-1)).max()
+    print(ti, "max pressure error", err)
+    assert err < 1e-1
+
+
 def vc_acoustics_run(q, aux, d):
     """examples/acoustics_2d_variable classic set-up on the oracle: unsplit,
     trans_inc, MC, walls low / extrapolation high (q and aux), 20 outputs."""
