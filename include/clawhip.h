@@ -54,9 +54,14 @@ enum {
   CLAWHIP_RP_VC_ACOUSTICS = 8,     /* vc_acoustics_2D (clawpack/riemann): acoustics with
                                    * density and sound speed per cell in 2 aux fields;
                                    * 2D steps */
-  CLAWHIP_RP_SHALLOW_HLLE = 9      /* shallow_hlle_2D (clawpack/riemann): HLLE shallow
+  CLAWHIP_RP_SHALLOW_HLLE = 9,     /* shallow_hlle_2D (clawpack/riemann): HLLE shallow
                                    * water; no transverse solver (split or unsplit with
                                    * transverse_waves = 0; SharpClaw) */
+  CLAWHIP_RP_EULER5 = 10           /* euler_5wave_2D (clawpack/riemann; in-tree copy
+                                   * development/rp_approaches/rpn2_euler_5wave.f,
+                                   * rpt2_euler_5wave.f): euler_4wave_2D plus a passive
+                                   * tracer q[4]; classic 2D steps, SharpClaw with
+                                   * char_decomp 0 */
 };
 
 /* Boundary codes: pyclaw.BC (pyclaw/solver.py:13-19) plus GIVEN. */

@@ -28,7 +28,7 @@ class OracleRun:
                  max_steps=10000, dt_variable=True, time_integrator="SSP104",
                  char_decomp=0, num_ghost=None, aux=None, aux_bc_lower=None,
                  aux_bc_upper=None, before_step=None, step_source=None, source_split=1,
-                 index_capa=-1, auxbc_fn=None, lmm_steps=4):
+                 index_capa=-1, auxbc_fn=None, lmm_steps=4, user_bc_lower=None):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -66,6 +66,9 @@ class OracleRun:
         # capacity field (0-based aux index, -1: none); auxbc_fn(run) -> the aux
         # array with its ghosts filled, for custom aux boundary conditions
         self.index_capa, self.auxbc_fn = index_capa, auxbc_fn
+        # user_bc_lower(run, dim, qbc): Solver.user_bc_lower for BC.custom lower
+        # sides, called in the reference's per-dimension order
+        self.user_bc_lower = user_bc_lower
         # multistep state (SSPLMMk2 / k3)
         self.lmm_steps = lmm_steps
         self.sspcoeff = None
@@ -82,7 +85,15 @@ class OracleRun:
         ng = self.ng
         sl = (slice(None),) + tuple(slice(ng, -ng) for _ in range(self.ndim))
         self.qbc[sl] = q
-        if self.ndim == 2:
+        if self.ndim == 2 and self.user_bc_lower is not None:
+            # dimension by dimension, lower side then upper (solver.py:351-367)
+            for d in range(2):
+                lo, hi = [O.CUSTOM] * 2, [O.CUSTOM] * 2
+                lo[d], hi[d] = self.bc_lower[d], self.bc_upper[d]
+                if lo[d] == O.CUSTOM:
+                    self.user_bc_lower(self, d, self.qbc)
+                O.bc2(self.qbc, ng, lo, hi)
+        elif self.ndim == 2:
             O.bc2(self.qbc, ng, self.bc_lower, self.bc_upper)
         else:
             O.bc3(self.qbc, ng, self.bc_lower, self.bc_upper)

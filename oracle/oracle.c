@@ -33,6 +33,7 @@ int orc_rp_meqn(int kind) {
     case ORC_RP_VC_ADVECTION_2D: return 1;
     case ORC_RP_VC_ACOUSTICS: return 3;
     case ORC_RP_SHALLOW_HLLE: return 3;
+    case ORC_RP_EULER5: return 5;
   }
   return 0;   /* ORC_RP_VC_ADVECTION: any (orc_rp.meqn) */
 }
@@ -48,6 +49,7 @@ int orc_rp_mwaves(int kind) {
     case ORC_RP_VC_ADVECTION_2D: return 1;
     case ORC_RP_VC_ACOUSTICS: return 2;
     case ORC_RP_SHALLOW_HLLE: return 2;
+    case ORC_RP_EULER5: return 5;
   }
   return 0;
 }
@@ -104,11 +106,16 @@ static roe2 roe_euler2(double gamma1, int mu, int mv, const double *qL, const do
   return r;
 }
 
+static int g_plant;   /* planted mistakes, listed at orc_set_plant */
+
 /* euler_4wave_2D rpn2: Roe solver, waves (u-a, shear u, entropy u, u+a), with the
- * Harten-Hyman entropy fix of rpn2_euler_5wave.f:137-234 (tracer wave dropped). */
+ * Harten-Hyman entropy fix of rpn2_euler_5wave.f:137-234 (tracer wave dropped).
+ * tr = 1: euler_5wave_2D, the same with the tracer q[4] and its wave (the jump
+ * of q[4], speed u; rpn2_euler_5wave.f:96-103, 242-254). */
 static void rpn_euler(const orc_rp *p, int ixy, const double *qL, const double *qR,
-                      double *wave, double *s, double *amdq, double *apdq) {
+                      double *wave, double *s, double *amdq, double *apdq, int tr) {
   const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const int ME = 4 + tr, MW = 4 + tr;
   const double gamma = p->gamma, gamma1 = p->gamma - 1.0;
   roe2 r = roe_euler2(gamma1, mu, mv, qL, qR);
   double delta[4];
@@ -120,7 +127,7 @@ static void rpn_euler(const orc_rp *p, int ixy, const double *qL, const double *
   double a2 = delta[2] - r.v * delta[0];
   double a4 = (delta[1] + (r.a - r.u) * delta[0] - r.a * a3) / (2.0 * r.a);
   double a1 = delta[0] - a3 - a4;
-  double *w1 = wave, *w2 = wave + 4, *w3 = wave + 8, *w4 = wave + 12;
+  double *w1 = wave, *w2 = wave + ME, *w3 = wave + 2 * ME, *w4 = wave + 3 * ME;
   w1[0] = a1; w1[mu] = a1 * (r.u - r.a); w1[mv] = a1 * r.v; w1[3] = a1 * (r.enth - r.u * r.a);
   s[0] = r.u - r.a;
   w2[0] = 0.0; w2[mu] = 0.0; w2[mv] = a2; w2[3] = a2 * r.v;
@@ -129,13 +136,20 @@ static void rpn_euler(const orc_rp *p, int ixy, const double *qL, const double *
   s[2] = r.u;
   w4[0] = a4; w4[mu] = a4 * (r.u + r.a); w4[mv] = a4 * r.v; w4[3] = a4 * (r.enth + r.u * r.a);
   s[3] = r.u + r.a;
+  double *w5 = wave + 4 * ME;
+  if (tr) {
+    w1[4] = 0.0; w2[4] = 0.0; w3[4] = 0.0; w4[4] = 0.0;
+    w5[0] = 0.0; w5[mu] = 0.0; w5[mv] = 0.0; w5[3] = 0.0;
+    w5[4] = qR[4] - qL[4];
+    s[4] = r.u;
+  }
 
   if (!p->efix) {
-    for (int m = 0; m < 4; m++) {
+    for (int m = 0; m < ME; m++) {
       amdq[m] = 0.0; apdq[m] = 0.0;
-      for (int mw = 0; mw < 4; mw++) {
-        if (s[mw] < 0.0) amdq[m] = amdq[m] + s[mw] * wave[mw * 4 + m];
-        else apdq[m] = apdq[m] + s[mw] * wave[mw * 4 + m];
+      for (int mw = 0; mw < MW; mw++) {
+        if (s[mw] < 0.0) amdq[m] = amdq[m] + s[mw] * wave[mw * ME + m];
+        else apdq[m] = apdq[m] + s[mw] * wave[mw * ME + m];
       }
     }
     return;
@@ -147,7 +161,7 @@ static void rpn_euler(const orc_rp *p, int ixy, const double *qL, const double *
     double cim1 = sqrt(gamma * pim1 / rhoim1);
     double s0 = qL[mu] / rhoim1 - cim1;
     if (s0 >= 0.0 && s[0] > 0.0) {
-      for (int m = 0; m < 4; m++) amdq[m] = 0.0;
+      for (int m = 0; m < ME; m++) amdq[m] = 0.0;
       break;
     }
     double rho1 = qL[0] + w1[0];
@@ -161,11 +175,12 @@ static void rpn_euler(const orc_rp *p, int ixy, const double *qL, const double *
     if (s0 < 0.0 && s1 > 0.0) sfract = s0 * (s1 - s[0]) / (s1 - s0);
     else if (s[0] < 0.0) sfract = s[0];
     else sfract = 0.0;
-    for (int m = 0; m < 4; m++) amdq[m] = sfract * w1[m];
+    for (int m = 0; m < ME; m++) amdq[m] = sfract * w1[m];
     if (s[1] >= 0.0) break;
-    for (int m = 0; m < 4; m++) {
+    for (int m = 0; m < ME; m++) {
       amdq[m] = amdq[m] + s[1] * w2[m];
       amdq[m] = amdq[m] + s[2] * w3[m];
+      if (tr && g_plant != 4) amdq[m] = amdq[m] + s[4] * w5[m];
     }
     double rhoi = qR[0];
     double pi = gamma1 * (qR[3] - 0.5 * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
@@ -181,19 +196,21 @@ static void rpn_euler(const orc_rp *p, int ixy, const double *qL, const double *
     if (s2 < 0.0 && s3 > 0.0) sfract = s2 * (s3 - s[3]) / (s3 - s2);
     else if (s[3] < 0.0) sfract = s[3];
     else break;
-    for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + sfract * w4[m];
+    for (int m = 0; m < ME; m++) amdq[m] = amdq[m] + sfract * w4[m];
   } while (0);
-  for (int m = 0; m < 4; m++) {
+  for (int m = 0; m < ME; m++) {
     double df = 0.0;
-    for (int mw = 0; mw < 4; mw++) df = df + s[mw] * wave[mw * 4 + m];
+    for (int mw = 0; mw < MW; mw++) df = df + s[mw] * wave[mw * ME + m];
     apdq[m] = df - amdq[m];
   }
 }
 
 /* euler_4wave_2D rpt2: split asdq into waves of the transverse Roe matrix at the
- * Roe average of the slice's interface states (rpt2_euler_5wave.f:47-95). */
+ * Roe average of the slice's interface states (rpt2_euler_5wave.f:47-95); tr = 1
+ * adds the tracer wave (asdq[4] at speed v, rpt2_euler_5wave.f:83-88). */
 static void rpt_euler(const orc_rp *p, int ixy, const double *qL, const double *qR,
-                      const double *asdq, double *bm, double *bp) {
+                      const double *asdq, double *bm, double *bp, int tr) {
+  const int ME = 4 + tr, MWB = 3 + tr;
   const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
   const double gamma1 = p->gamma - 1.0;
   roe2 r = roe_euler2(gamma1, mu, mv, qL, qR);
@@ -201,16 +218,21 @@ static void rpt_euler(const orc_rp *p, int ixy, const double *qL, const double *
   double a2 = asdq[mu] - r.u * asdq[0];
   double a4 = (asdq[mv] + (r.a - r.v) * asdq[0] - r.a * a3) / (2.0 * r.a);
   double a1 = asdq[0] - a3 - a4;
-  double wb[3][4], sb[3];
+  double wb[4][5], sb[4];
   wb[0][0] = a1; wb[0][mu] = a1 * r.u; wb[0][mv] = a1 * (r.v - r.a);
   wb[0][3] = a1 * (r.enth - r.v * r.a); sb[0] = r.v - r.a;
   wb[1][0] = a3; wb[1][mu] = a3 * r.u + a2; wb[1][mv] = a3 * r.v;
   wb[1][3] = a3 * 0.5 * r.u2v2 + a2 * r.u; sb[1] = r.v;
   wb[2][0] = a4; wb[2][mu] = a4 * r.u; wb[2][mv] = a4 * (r.v + r.a);
   wb[2][3] = a4 * (r.enth + r.v * r.a); sb[2] = r.v + r.a;
-  for (int m = 0; m < 4; m++) {
+  if (tr) {
+    wb[0][4] = 0.0; wb[1][4] = 0.0; wb[2][4] = 0.0;
+    wb[3][0] = 0.0; wb[3][mu] = 0.0; wb[3][mv] = 0.0; wb[3][3] = 0.0; wb[3][4] = asdq[4];
+    sb[3] = g_plant == 5 ? r.u : r.v;
+  }
+  for (int m = 0; m < ME; m++) {
     bm[m] = 0.0; bp[m] = 0.0;
-    for (int mw = 0; mw < 3; mw++) {
+    for (int mw = 0; mw < MWB; mw++) {
       bm[m] = bm[m] + dmin(sb[mw], 0.0) * wb[mw][m];
       bp[m] = bp[m] + dmax(sb[mw], 0.0) * wb[mw][m];
     }
@@ -475,7 +497,8 @@ void orc_rpn(const orc_rp *p, int ixy, const double *qL, const double *qR,
   switch (p->kind) {
     case ORC_RP_SHALLOW_HLLE: rpn_shallow_hlle(p, ixy, qL, qR, wave, s, amdq, apdq); break;
     case ORC_RP_ACOUSTICS: rpn_acoustics(p, ixy, qL, qR, wave, s, amdq, apdq); break;
-    case ORC_RP_EULER: rpn_euler(p, ixy, qL, qR, wave, s, amdq, apdq); break;
+    case ORC_RP_EULER: rpn_euler(p, ixy, qL, qR, wave, s, amdq, apdq, 0); break;
+    case ORC_RP_EULER5: rpn_euler(p, ixy, qL, qR, wave, s, amdq, apdq, 1); break;
     case ORC_RP_SHALLOW: rpn_shallow(p, ixy, qL, qR, wave, s, amdq, apdq); break;
     case ORC_RP_ADVECTION: rpn_advection(p, ixy, qL, qR, wave, s, amdq, apdq); break;
     case ORC_RP_EULER3: rpn_euler3(p, ixy, qL, qR, wave, s, amdq, apdq); break;
@@ -506,7 +529,12 @@ static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const d
  *      (caught by the annulus fixture);
  *   2  vc_acoustics_2D: the cells below and above swapped (aux1 <-> aux3);
  *   3  vc_acoustics_2D: cell i for both imp (caught by the variable-acoustics
- *      io fixture, as is 2).
+ *      io fixture, as is 2);
+ *   4  euler_5wave_2D rpn2: the tracer wave left out of the entropy-fixed
+ *      left-going fluctuation;
+ *   5  euler_5wave_2D rpt2: the tracer's transverse wave at the normal velocity
+ *      (4 and 5 are caught by the uniform-flow tracer test, where the tracer
+ *      must move exactly as advection_2D moves its scalar).
  * Which edge (aux2 or aux3) and which cell (i1) vc_advection_2D's rpt2 reads is
  * NOT pinned by any fixture: the annulus field has zero radial velocity and a
  * theta velocity that depends on r only, so those choices leave its solution
@@ -612,7 +640,8 @@ void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *
   (void)imp;
   switch (p->kind) {
     case ORC_RP_ACOUSTICS: rpt_acoustics(p, ixy, asdq, bmasdq, bpasdq); break;
-    case ORC_RP_EULER: rpt_euler(p, ixy, qL, qR, asdq, bmasdq, bpasdq); break;
+    case ORC_RP_EULER: rpt_euler(p, ixy, qL, qR, asdq, bmasdq, bpasdq, 0); break;
+    case ORC_RP_EULER5: rpt_euler(p, ixy, qL, qR, asdq, bmasdq, bpasdq, 1); break;
     case ORC_RP_SHALLOW: rpt_shallow(p, ixy, qL, qR, asdq, bmasdq, bpasdq); break;
     case ORC_RP_ADVECTION: rpt_advection(p, ixy, asdq, bmasdq, bpasdq); break;
     default: break;

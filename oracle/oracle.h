@@ -30,7 +30,9 @@ enum {
                                * meqn 1, and its transverse solver rpt2 */
   ORC_RP_VC_ACOUSTICS = 8,     /* clawpack.riemann vc_acoustics_2D: meqn 3, mwaves 2,
                                * aux = (density, sound speed) per cell */
-  ORC_RP_SHALLOW_HLLE = 9      /* clawpack.riemann shallow_hlle_2D: meqn 3, mwaves 2 */
+  ORC_RP_SHALLOW_HLLE = 9,     /* clawpack.riemann shallow_hlle_2D: meqn 3, mwaves 2 */
+  ORC_RP_EULER5 = 10           /* euler_5wave_2D (development/rp_approaches/
+                               * rpn2_euler_5wave.f, rpt2_euler_5wave.f): meqn 5, mwaves 5 */
 };
 
 /* Problem parameters: the reference passes these through the Fortran /cparam/

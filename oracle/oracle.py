@@ -21,12 +21,13 @@ REF_SHARP_PATH = os.path.join(HERE, "_ref", "libref_sharp.so")
 ACOUSTICS, EULER, SHALLOW, ADVECTION, EULER3 = 1, 2, 3, 4, 5
 CUSTOM, EXTRAP, PERIODIC, WALL = 0, 1, 2, 3
 
-MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5, 8: 3, 9: 3}
-MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1, 7: 1, 8: 2, 9: 2}
+MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5, 8: 3, 9: 3, 10: 5}
+MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1, 7: 1, 8: 2, 9: 2, 10: 5}
 VC_ADVECTION = 6
 VC_ADVECTION_2D = 7
 VC_ACOUSTICS = 8
 SHALLOW_HLLE = 9
+EULER5 = 10
 
 
 class RP(ctypes.Structure):

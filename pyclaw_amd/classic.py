@@ -72,7 +72,9 @@ class ClawSolver(Solver):
         self._method[3] = self.verbosity
         self._method[4] = 0
         self._method[5] = state.index_capa + 1
-        self._method[6] = state.num_aux
+        # aux fields the Riemann solver does not read (kept for the user's
+        # step_source / BC functions) are not handed to the kernels
+        self._method[6] = state.num_aux if getattr(self.rp, "num_aux", 0) else 0
 
     def setup(self, solution):
         self._check_cfl_settings()
@@ -83,9 +85,6 @@ class ClawSolver(Solver):
         if self.rp is None or not hasattr(self.rp, "kind"):
             raise Exception("the solver needs one of the pyclaw_amd.riemann modules")
         rp_aux = getattr(self.rp, "num_aux", 0)
-        if rp_aux == 0 and state.num_aux != 0:
-            raise NotImplementedError("aux fields are supported with the variable-coefficient "
-                                      "solvers (vc_advection, vc_acoustics)")
         if rp_aux and not rp_aux <= state.num_aux <= 3:
             raise NotImplementedError("the Riemann solver reads %d aux fields; the device steps "
                                       "take %d to 3, the state has %d"
@@ -155,7 +154,7 @@ class ClawSolver2D(ClawSolver):
         gin = None
         src = qin
         aux = alo = ahi = None
-        if state.num_aux > 0:
+        if state.num_aux > 0 and getattr(self.rp, "num_aux", 0):
             aux, alo, ahi = self._aux_input(state)
         if BC.custom in lo or BC.custom in hi:
             src, gin = self._custom_bc_input(state, lo, hi)
@@ -305,6 +304,9 @@ class ClawSolver2D(ClawSolver):
         q = state.q
         qbc = np.zeros((q.shape[0],) + tuple(n + 2 * ng for n in q.shape[1:]), order="F")
         qbc[(slice(None),) + (slice(ng, -ng),) * self.num_dim] = q
+        if state.num_aux > 0:
+            # the user's functions get auxbc too (solver.py:371-398)
+            self._host_auxbc(state)
         _fill_bcs_host(self, state, qbc, lo, hi)
         t = torch.from_numpy(to_planar(qbc)).to("cuda", torch_dtype(self._dtype_code))
         comp = state.device_q(self._dtype_code)[1]

@@ -109,7 +109,7 @@ int run_unsplit(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q
   a.gin = gin; a.gout = gout;
   a.bc = to_bc(lo, hi, refl, 2);
   a.ng = ng; a.order = method[1]; a.trans = method[2];
-  for (int w = 0; w < 4; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
+  for (int w = 0; w < kMaxWaves; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
   a.dtdx = T(dt / dx); a.dtdy = T(dt / dy);
   a.nstrips = (gout.mx + 59) / 60;
   a.H = choose_rows(k_unsplit<RS, T, KAHAN, AUX, CAPA>, a.nstrips, gout.my);
@@ -133,7 +133,7 @@ int run_unsplit_ctl(const clawhip_rp& rp, const Geom& g, const int* lo, const in
   a.gin = g; a.gout = g;
   a.bc = to_bc(lo, hi, refl, 2);
   a.ng = ng; a.order = method[1]; a.trans = method[2];
-  for (int w = 0; w < 4; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
+  for (int w = 0; w < kMaxWaves; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
   a.dtdx = T(0); a.dtdy = T(0);
   a.nstrips = (g.mx + 59) / 60;
   a.H = choose_rows(k_unsplit_ctl<RS, T, KAHAN>, a.nstrips, g.my);
@@ -260,6 +260,9 @@ int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom
       return dispatch_unsplit<ShallowHLLE>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                            lo, hi, refl, ng, method, mthlim, dx, dy, dt,
                                            cfl_bits, stream);
+    case CLAWHIP_RP_EULER5:
+      return dispatch_unsplit<Euler5>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi,
+                                      refl, ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
   }
   return CLAWHIP_E_KIND;
 }
@@ -284,6 +287,9 @@ int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int
     case CLAWHIP_RP_SHALLOW_HLLE:
       return dispatch_unsplit_ctl<ShallowHLLE>(dtype, rp, g, lo, hi, refl, ng, method, mthlim,
                                                ctl, bufs, nsteps, stream);
+    case CLAWHIP_RP_EULER5:
+      return dispatch_unsplit_ctl<Euler5>(dtype, rp, g, lo, hi, refl, ng, method, mthlim, ctl,
+                                          bufs, nsteps, stream);
   }
   return CLAWHIP_E_KIND;
 }

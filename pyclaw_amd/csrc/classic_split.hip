@@ -54,7 +54,7 @@ struct SweepArgs {
   Geom gin, gout;       // input / output geometry
   BCSpec bc;            // used when reading a compact input (x-sweeps)
   int ng, order;
-  int mthlim[4];
+  int mthlim[kMaxWaves];
   T dtdx;               // dt / delta of the sweep direction
   // k_xsweep
   int nstrips;
@@ -333,7 +333,7 @@ SweepArgs<T> base_args(const clawhip_rp& rp, const Geom& gin, const Geom& gout, 
   a.gin = gin; a.gout = gout;
   a.bc = to_bc(lo, hi, refl, ndim);
   a.ng = ng; a.order = method[1];
-  for (int w = 0; w < 4; w++) a.mthlim[w] = w < mwaves ? mthlim[w] : 0;
+  for (int w = 0; w < kMaxWaves; w++) a.mthlim[w] = w < mwaves ? mthlim[w] : 0;
   a.dtdx = T(dtdx);
   a.qin = (const T*)qin; a.qout = (T*)qout;
   a.comp_in = (const T*)cin; a.comp_out = (T*)cout;
@@ -542,6 +542,10 @@ int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
       return dispatch_split2<ShallowHLLE>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                           work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
                                           dt, aux, cfl, s);
+    case CLAWHIP_RP_EULER5:
+      return dispatch_split2<Euler5>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out, work,
+                                     cwork, lo, hi, refl, ng, method, mthlim, dx, dy, dt, aux,
+                                     cfl, s);
   }
   return CLAWHIP_E_KIND;
 }
@@ -599,6 +603,9 @@ int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& g
     case CLAWHIP_RP_SHALLOW_HLLE:
       return run_sweep<ShallowHLLE<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method,
                                             mthlim, dtdx, aux, cfl, s);
+    case CLAWHIP_RP_EULER5:
+      return run_sweep<Euler5<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
+                                       dtdx, aux, cfl, s);
   }
   return CLAWHIP_E_KIND;
 }

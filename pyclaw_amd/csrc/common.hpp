@@ -8,6 +8,10 @@
 
 namespace claw {
 
+// Most waves of any solver here (euler_5wave_2D): the size of the per-wave
+// limiter table the classic kernels take.
+constexpr int kMaxWaves = 5;
+
 // Boundary-condition codes: pyclaw.BC (solver.py:13-19) plus GIVEN, which means
 // the ghost cells are stored in the array (a user-filled qbc) and read as they are.
 enum BCCode { BC_CUSTOM = 0, BC_EXTRAP = 1, BC_PERIODIC = 2, BC_WALL = 3, BC_GIVEN = 4 };
@@ -156,9 +160,10 @@ __device__ __forceinline__ T philim(T a, T b, int meth) {
 
 // One interface of limiter.f90:30-54: wnorm2 and the two dot products are of the
 // unlimited waves; the upwind one (by the sign of s) enters phi.
-template <int ME, int MW, typename T>
+template <int ME, int MW, int NL, typename T>
 __device__ __forceinline__ void limit_waves(T (&W)[MW][ME], const T (&s)[MW], const T (&dotl)[MW],
-                                            const T (&dotr)[MW], const int (&mthlim)[4]) {
+                                            const T (&dotr)[MW], const int (&mthlim)[NL]) {
+  static_assert(MW <= NL, "one limiter per wave");
 #pragma unroll
   for (int w = 0; w < MW; w++) {
     if (mthlim[w] == 0) continue;

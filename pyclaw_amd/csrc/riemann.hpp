@@ -87,9 +87,14 @@ struct Acoustics {
 };
 
 // ---------------------------------------------------------------- 2D Euler
-template <typename T>
-struct Euler {
-  static constexpr int meqn = 4, mwaves = 4, maux = 0;
+// TR = 0: euler_4wave_2D.  TR = 1: euler_5wave_2D, the same with a passive tracer
+// q[4] carried by a fifth wave (its jump, speed u) in the normal solve and a
+// fourth transverse wave (asdq[4], speed v); its components of the other waves
+// are zero (development/rp_approaches/rpn2_euler_5wave.f:96-103,
+// rpt2_euler_5wave.f:62-83).
+template <typename T, int TR>
+struct EulerT {
+  static constexpr int meqn = 4 + TR, mwaves = 4 + TR, maux = 0;
   struct Roe { T u, v, enth, u2v2, a, g1a2, euv; };
   // normal flux; the Roe linearisation gives amdq + apdq = flux(qR) - flux(qL)
   template <int IXY>
@@ -101,6 +106,7 @@ struct Euler {
     f[mu] = q[mu] * un + pr;
     f[mv] = q[mv] * un;
     f[3] = (q[3] + pr) * un;
+    if constexpr (TR) f[4] = q[4] * un;
   }
 
   template <int IXY>
@@ -142,6 +148,11 @@ struct Euler {
     wave[2][3] = a3 * T(0.5) * r.u2v2; s[2] = r.u;
     wave[3][0] = a4; wave[3][mu] = a4 * (r.u + r.a); wave[3][mv] = a4 * r.v;
     wave[3][3] = a4 * (r.enth + r.u * r.a); s[3] = r.u + r.a;
+    if constexpr (TR) {
+      wave[0][4] = T(0); wave[1][4] = T(0); wave[2][4] = T(0); wave[3][4] = T(0);
+      wave[4][0] = T(0); wave[4][1] = T(0); wave[4][2] = T(0); wave[4][3] = T(0);
+      wave[4][4] = qR[4] - qL[4]; s[4] = r.u;
+    }
 
     if (!p.efix) {
 #pragma unroll
@@ -183,6 +194,7 @@ struct Euler {
       for (int m = 0; m < meqn; m++) {
         amdq[m] = amdq[m] + s[1] * wave[1][m];
         amdq[m] = amdq[m] + s[2] * wave[2][m];
+        if constexpr (TR) amdq[m] = amdq[m] + s[4] * wave[4][m];
       }
       T rhoi = qR[0];
       T pi = gamma1 * (qR[3] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
@@ -218,24 +230,32 @@ struct Euler {
     T a2 = asdq[mu] - r.u * asdq[0];
     T a4 = (asdq[mv] + (r.a - r.v) * asdq[0] - r.a * a3) / (T(2) * r.a);
     T a1 = asdq[0] - a3 - a4;
-    T wb[3][meqn], sb[3];
+    T wb[3 + TR][meqn], sb[3 + TR];
     wb[0][0] = a1; wb[0][mu] = a1 * r.u; wb[0][mv] = a1 * (r.v - r.a);
     wb[0][3] = a1 * (r.enth - r.v * r.a); sb[0] = r.v - r.a;
     wb[1][0] = a3; wb[1][mu] = a3 * r.u + a2; wb[1][mv] = a3 * r.v;
     wb[1][3] = a3 * T(0.5) * r.u2v2 + a2 * r.u; sb[1] = r.v;
     wb[2][0] = a4; wb[2][mu] = a4 * r.u; wb[2][mv] = a4 * (r.v + r.a);
     wb[2][3] = a4 * (r.enth + r.v * r.a); sb[2] = r.v + r.a;
+    if constexpr (TR) {
+      wb[0][4] = T(0); wb[1][4] = T(0); wb[2][4] = T(0);
+      wb[3][0] = T(0); wb[3][1] = T(0); wb[3][2] = T(0); wb[3][3] = T(0);
+      wb[3][4] = asdq[4]; sb[3] = r.v;
+    }
 #pragma unroll
     for (int m = 0; m < meqn; m++) {
       bm[m] = T(0); bp[m] = T(0);
 #pragma unroll
-      for (int w = 0; w < 3; w++) {
+      for (int w = 0; w < 3 + TR; w++) {
         bm[m] = bm[m] + dmin(sb[w], T(0)) * wb[w][m];
         bp[m] = bp[m] + dmax(sb[w], T(0)) * wb[w][m];
       }
     }
   }
 };
+
+template <typename T> using Euler = EulerT<T, 0>;
+template <typename T> using Euler5 = EulerT<T, 1>;
 
 // ---------------------------------------------------------------- shallow water
 template <typename T>

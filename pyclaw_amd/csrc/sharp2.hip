@@ -848,6 +848,7 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
     case CLAWHIP_RP_SHALLOW: return dispatch_stage<Shallow, false>(ARGS);
     case CLAWHIP_RP_SHALLOW_HLLE: return dispatch_stage<ShallowHLLE, false>(ARGS);
     case CLAWHIP_RP_EULER: return dispatch_stage<Euler, false>(ARGS);
+    case CLAWHIP_RP_EULER5: return dispatch_stage<Euler5, false>(ARGS);
     case CLAWHIP_RP_ACOUSTICS: return dispatch_stage<Acoustics, false>(ARGS);
     case CLAWHIP_RP_ADVECTION: return dispatch_stage<Advection, false>(ARGS);
   }

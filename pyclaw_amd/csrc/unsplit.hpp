@@ -35,7 +35,7 @@ struct UnsplitArgs {
   Geom gin, gout;        // input (any ghosts) / output (compact) geometry
   BCSpec bc;
   int ng, order, trans;
-  int mthlim[4];
+  int mthlim[kMaxWaves];
   T dtdx, dtdy;
   int H, nstrips, nrb;
   const T* qin;

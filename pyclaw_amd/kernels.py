@@ -43,7 +43,7 @@ def cfl_reset(cfl):
     L.check(L.lib().clawhip_cfl_reset(_ptr(cfl), _stream()), "clawhip_cfl_reset")
 
 
-_RP_MEQN = {1: 3, 2: 4, 3: 3, 4: 1, 5: 5, 7: 1, 8: 3, 9: 3}
+_RP_MEQN = {1: 3, 2: 4, 3: 3, 4: 1, 5: 5, 7: 1, 8: 3, 9: 3, 10: 5}
 
 
 def _check_fields(rp, meqn, aux, method):

@@ -109,9 +109,6 @@ class SharpClawSolver(Solver):
         rp_aux = getattr(self.rp, "num_aux", 0)
         if state.index_capa >= 0:
             raise NotImplementedError("capacity functions are not supported by SharpClaw on the device")
-        if rp_aux == 0 and state.num_aux != 0:
-            raise NotImplementedError("aux fields are supported with the variable-coefficient "
-                                      "solvers (vc_advection, vc_acoustics)")
         if rp_aux and not rp_aux <= state.num_aux <= 3:
             raise NotImplementedError("the Riemann solver reads %d aux fields; the device "
                                       "operator takes %d to 3, the state has %d"
@@ -121,6 +118,10 @@ class SharpClawSolver(Solver):
                                       "no indicator")
         self._rp_struct = K.make_rp(self.rp.kind, state.problem_data, meqn=state.num_eqn)
         self._is_set_up = True
+
+    def _reads_aux(self, state):
+        # aux fields only the user's callbacks read stay on the host
+        return state.num_aux > 0 and getattr(self.rp, "num_aux", 0) > 0
 
     def _set_mthlim(self):
         self._mthlim = self.limiters
@@ -214,7 +215,7 @@ class SharpClawSolver(Solver):
         q, comp = state.device_q(self._dtype_code)
         # aux and its ghosts once per step (dq_hyperbolic applies the aux BCs of
         # the step's state at every stage, sharpclaw/solver.py:848-898)
-        self._qaux = self._aux_input(state) if state.num_aux > 0 else None
+        self._qaux = self._aux_input(state) if self._reads_aux(state) else None
         self._flags = None
         if self.indicator is not None:
             self._flags = self.indicator.step_flags(q, self._bc_codes(), self.reflect_index,
@@ -419,7 +420,7 @@ class SharpClawSolver(Solver):
         from .device import q_from_device
         self._grid = state.grid
         q, _ = state.device_q(self._dtype_code)
-        self._qaux = self._aux_input(state) if state.num_aux > 0 else None
+        self._qaux = self._aux_input(state) if self._reads_aux(state) else None
         out = torch.empty_like(q)
         self._stage(q, out, q, out_mode=K.OUT_D)
         return q_from_device(out)

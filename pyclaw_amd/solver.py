@@ -191,6 +191,13 @@ class Solver(object):
         alo, ahi = self._aux_bc_codes()
         if BC.custom not in alo and BC.custom not in ahi:
             return state.device_aux(self._dtype_code), alo, ahi
+        auxbc = self._host_auxbc(state)
+        return q_to_device(auxbc, self._dtype_code), [4] * self.num_dim, [4] * self.num_dim
+
+    def _host_auxbc(self, state):
+        """solver.auxbc on the host: aux with its ghosts filled by the aux BCs in
+        the reference's order (walls mirror without negation)."""
+        alo, ahi = self._aux_bc_codes()
         ng = self.num_ghost
         aux = np.asarray(state.aux)
         auxbc = np.zeros((aux.shape[0],) + tuple(n + 2 * ng for n in aux.shape[1:]), order="F")
@@ -198,7 +205,7 @@ class Solver(object):
         self.auxbc = auxbc
         from .classic import _fill_bcs_host
         _fill_bcs_host(self, state, auxbc, alo, ahi, aux=True)
-        return q_to_device(auxbc, self._dtype_code), [4] * self.num_dim, [4] * self.num_dim
+        return auxbc
 
     # ---- time-step control (solver.py:524-552)
     def accept_reject_step(self, state):

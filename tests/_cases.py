@@ -167,6 +167,76 @@ def euler_quadrants():
     return q, d
 
 
+def shock_bubble(mx=160, my=40, gamma=1.4):
+    """examples/euler_2d/shock_bubble_interaction.py: a planar shock (p = 5 behind
+    it) entering at x = 0 and hitting a low-density bubble (rho 0.1, radius 0.2,
+    centre (0.5, 0)) in r-z coordinates on [0,2]x[0,0.5].  qinit 41-87 (bubble
+    edge cells averaged with scipy's quad), auxinit 90-96 (aux[0] = r), the
+    incoming-shock ghost state 99-115 and the two-stage geometric source
+    step_Euler_radial 118-153.  Returns (q, aux, delta, shock_state,
+    step_source(q, aux, dt))."""
+    from scipy import integrate
+    g1 = gamma - 1.
+    x0, y0, r0 = 0.5, 0., 0.2
+    rhoin, pinf, rhoout, pout, pin, xshock = 0.1, 5., 1., 1., 1., 0.2
+    (X, Y), d = grid([0., 0.], [2., 0.5], [mx, my])
+    dx, dy = d
+    rinf = (g1 + pinf * (gamma + 1.)) / ((gamma + 1.) + g1 * pinf)
+    vinf = 1. / np.sqrt(gamma) * (pinf - 1.) / np.sqrt(0.5 * ((gamma + 1.) / gamma) * pinf
+                                                       + 0.5 * g1 / gamma)
+    einf = 0.5 * rinf * vinf ** 2 + pinf / g1
+    r = np.sqrt((X - x0) ** 2 + (Y - y0) ** 2)
+    q = np.zeros((5, mx, my), order="F")
+    q[0] = rinf * (X < xshock) + rhoin * (r <= r0) + rhoout * (r > r0) * (X >= xshock)
+    q[1] = rinf * vinf * (X < xshock)
+    q[3] = einf * (X < xshock) + (pin * (r <= r0) + pout * (r > r0) * (X >= xshock)) / g1
+    q[4] = 1. * (r <= r0)
+
+    def height_inside(x, ylo, yhi):
+        # length of [ylo, yhi] under the bubble's upper boundary at x
+        if (x - x0) ** 2 < r0 ** 2:
+            return max(min(y0 + np.sqrt(r0 ** 2 - (x - x0) ** 2), yhi) - ylo, 0.)
+        return 0
+
+    half_diag = np.linalg.norm([dx, dy]) / 2.
+    for i in range(mx):
+        for j in range(my):
+            if abs(r[i, j] - r0) < half_diag:
+                frac, _ = integrate.quad(height_inside, X[i, j] - dx / 2., X[i, j] + dx / 2.,
+                                         args=(Y[i, j] - dy / 2., Y[i, j] + dy / 2.),
+                                         epsabs=1.e-8, epsrel=1.e-5)
+                frac = frac / (dx * dy)
+                q[0, i, j] = rhoin * frac + rhoout * (1. - frac)
+                q[3, i, j] = (pin * frac + pout * (1. - frac)) / g1
+                q[4, i, j] = 1. * frac
+    aux = np.asfortranarray(Y[None].copy())
+    shock_state = np.array([rinf, rinf * vinf, 0., einf, 0.])
+
+    def step_source(q, aux, dt):
+        rad = aux[0]
+
+        def rates(qq):
+            rho = qq[0]
+            u, v = qq[1] / rho, qq[2] / rho
+            press = g1 * (qq[3] - 0.5 * rho * (u ** 2 + v ** 2))
+            return rho, u, v, press
+
+        rho, u, v, press = rates(q)
+        h = dt / 2.
+        qs = np.empty(q.shape)
+        qs[0] = q[0] - h / rad * q[2]
+        qs[1] = q[1] - h / rad * rho * u * v
+        qs[2] = q[2] - h / rad * rho * v * v
+        qs[3] = q[3] - h / rad * v * (q[3] + press)
+        rho, u, v, press = rates(qs)
+        q[0] = q[0] - dt / rad * qs[2]
+        q[1] = q[1] - dt / rad * rho * u * v
+        q[2] = q[2] - dt / rad * rho * v * v
+        q[3] = q[3] - dt / rad * v * (qs[3] + press)
+
+    return q, aux, d, shock_state, step_source
+
+
 def radial_dam_break():
     """examples/shallow_2d/radial_dam_break.py: qinit 22-30, setup 33-80."""
     (X, Y), d = grid([-2.5, -2.5], [2.5, 2.5], [150, 150])
@@ -198,8 +268,8 @@ def seeded_state(rng, kind, shape):
     q = np.zeros((meqn,) + shape)
     if kind == O.ACOUSTICS:
         q[:] = rng.uniform(-1, 1, q.shape)
-    elif kind in (O.EULER, O.EULER3):
-        nd = 2 if kind == O.EULER else 3
+    elif kind in (O.EULER, O.EULER3, O.EULER5):
+        nd = 3 if kind == O.EULER3 else 2
         rho = rng.uniform(0.5, 1.5, shape)
         vel = [rng.uniform(-0.5, 0.5, shape) for _ in range(nd)]
         p = rng.uniform(0.1, 1.0, shape)
@@ -207,6 +277,8 @@ def seeded_state(rng, kind, shape):
         for d in range(nd):
             q[1 + d] = rho * vel[d]
         q[nd + 1] = p / 0.4 + 0.5 * rho * sum(v * v for v in vel)
+        if kind == O.EULER5:
+            q[4] = rng.uniform(0, 1, shape)
     elif kind in (O.SHALLOW, O.SHALLOW_HLLE):
         h = rng.uniform(0.5, 1.5, shape)
         q[0] = h

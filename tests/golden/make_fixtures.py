@@ -39,7 +39,7 @@ WANT = {
     "advection_2d": ["classic_dim_split", "classic_unsplit_no_trans",
                      "classic_unsplit_trans_inc", "classic_unsplit_trans_cor",
                      "sharpclaw"],
-    "euler_2d": ["quadrants_roe"],
+    "euler_2d": ["quadrants_roe", "shock_bubble_interaction"],
     "shallow_2d": ["radialdambreak_classic_roe", "radialdambreak_sharpclaw_roe",
                    "radialdambreak_classic_hlle", "radialdambreak_sharpclaw_hlle"],
     "euler_3d": ["shocktube", "shock_bubble"],
