@@ -8,7 +8,9 @@
  * reference does not vendor; they are restated here from the published Roe /
  * Harten-Hyman formulas in the form of the reference's own in-tree Euler solver
  * (development/rp_approaches/rpn2_euler_5wave.f, rpt2_euler_5wave.f) and are
- * pinned end to end by the reference's example fixtures.
+ * pinned end to end by the reference's example fixtures.  The two Euler solvers
+ * (4 waves, and 5 with the tracer) are also pinned step by step: they equal those
+ * in-tree Fortran files, built from source, inside the reference's steps.
  */
 #include "oracle.h"
 

@@ -257,6 +257,34 @@ def ref_step2ds_aux(rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, ids, 
              ctypes.c_double(dt), _ip(method), _ip(mthlim))
 
 
+_ref_euler5 = None
+REF_EULER5_PATH = os.path.join(os.path.dirname(REF_CLASSIC_PATH), "libref_euler5.so")
+
+
+def ref_euler5(gamma):
+    """Addresses of the reference tree's own euler_5wave_2D rpn2/rpt2 (built from
+    development/rp_approaches, behind an adapter to step2's argument lists),
+    with their /cparam/ gamma set."""
+    global _ref_euler5
+    if _ref_euler5 is None:
+        _ref_euler5 = ctypes.CDLL(REF_EULER5_PATH)
+        _ref_euler5.euler5_set_gamma.argtypes = [ctypes.c_double]
+    _ref_euler5.euler5_set_gamma(float(gamma))
+    return (ctypes.cast(_ref_euler5.euler5_rpn2_, ctypes.c_void_p),
+            ctypes.cast(_ref_euler5.euler5_rpt2_, ctypes.c_void_p))
+
+
+def ref_step2_fns(rp, rpn2, rpt2, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0):
+    """The reference's step2 (ids 0) or step2ds sweep (1, 2) with the given
+    Riemann solver addresses (None: the oracle's for rp); no aux fields."""
+    meqn, nx, ny = qold.shape
+    f = ref_classic().ref_step2_fns
+    f.restype = ctypes.c_double
+    return f(ctypes.byref(rp), rpn2, rpt2, int(ids), meqn, MWAVES[rp.kind], ng, nx - 2 * ng,
+             ny - 2 * ng, _dp(qold), _dp(qnew), ctypes.c_double(dx), ctypes.c_double(dy),
+             ctypes.c_double(dt), _ip(method), _ip(mthlim))
+
+
 def ref_step2_aux(rp, rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, mwaves=1):
     """The reference's unsplit step2 with aux fields (and a capacity field when
     method[5] > 0), the given rpn2 address and the oracle's rpt2 for rp."""
@@ -277,17 +305,23 @@ def ref_step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir):
                                      _ip(mthlim))
 
 
-def ref_sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp, aux=None):
+def ref_sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp, aux=None, rpn2=None):
     """The reference's SharpClaw flux2 (aux: ghost-filled aux fields or None; the
-    Riemann solver with aux is the oracle's)."""
+    Riemann solver with aux is the oracle's).  rpn2: the address of a solver with
+    the reference's argument list to use instead of the oracle's (no aux)."""
     meqn, nx, ny = q.shape
     dq = np.zeros_like(q, order="F")
     maux = 0 if aux is None else aux.shape[0]
-    cfl = ref_sharp().ref_sharp_flux2(ctypes.byref(rp), int(char_decomp), meqn,
-                                      MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, _dp(q),
-                                      _dp(dq), ctypes.c_double(dx), ctypes.c_double(dy),
-                                      ctypes.c_double(dt), int(maux),
-                                      _dp(aux) if aux is not None else None)
+    L = ref_sharp()
+    L.shim_set_sharp_rpn2(rpn2)
+    try:
+        cfl = L.ref_sharp_flux2(ctypes.byref(rp), int(char_decomp), meqn,
+                                MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, _dp(q),
+                                _dp(dq), ctypes.c_double(dx), ctypes.c_double(dy),
+                                ctypes.c_double(dt), int(maux),
+                                _dp(aux) if aux is not None else None)
+    finally:
+        L.shim_set_sharp_rpn2(None)
     return dq, cfl
 
 

@@ -20,8 +20,19 @@ SHARP = $(REF)/sharpclaw/ClawParams.f90 $(REF)/sharpclaw/workspace.f90 \
 
 EX ?= /root/reference/examples
 VCADV = $(EX)/advection_reaction_2d/rpn2_vc_advection_vector.f90
+DEV ?= /root/reference/development/rp_approaches
+EULER5 = $(DEV)/rpn2_euler_5wave.f $(DEV)/rpt2_euler_5wave.f
 
-all: $(OUT)/libref_classic.so $(OUT)/libref_sharp.so $(OUT)/libref_vcadv.so
+all: $(OUT)/libref_classic.so $(OUT)/libref_sharp.so $(OUT)/libref_vcadv.so \
+     $(OUT)/libref_euler5.so
+
+# the reference tree's euler_5wave_2D solvers (rpn2 and rpt2, older argument
+# lists) with a C adapter to the lists step2/step2ds pass, in a library of their own
+$(OUT)/libref_euler5.so: $(EULER5) $(HERE)ref_euler5.c
+	mkdir -p $(OUT)/euler5
+	cd $(OUT)/euler5 && $(FC) $(FFLAGS) -c $(EULER5)
+	$(CC) $(CFLAGS) -c $(HERE)ref_euler5.c -o $(OUT)/euler5/ref_euler5.o
+	$(FC) -shared -fPIC -Wl,-Bsymbolic -o $@ $(OUT)/euler5/*.o
 
 # the advection_reaction_2d example's own Riemann solver (rpn2 only), in a
 # library of its own: its symbol is rpn2, and the step takes it as an argument

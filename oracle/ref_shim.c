@@ -132,12 +132,15 @@ extern void step3ds_(const int *, const int *, const int *, const int *, const i
                      double *, const int *, double *, const int *, const int *, const int *,
                      rp_fn, rp_fn, rp_fn);
 
-/* ids = 0: unsplit step2; 1,2: step2ds sweep.  Work arrays sized as
- * ClawSolver2D._allocate_workspace does (classic/solver.py:466-494). */
-double ref_step2(const orc_rp *p, int ids, int meqn, int mwaves, int ng, int mx, int my,
-                 const double *qold, double *qnew, double dx, double dy, double dt,
-                 const int *method, const int *mthlim) {
+/* ids = 0: unsplit step2; 1,2: step2ds sweep, with the given rpn2/rpt2 (the
+ * reference argument lists) -- NULL: the oracle's solver p.  Work arrays sized
+ * as ClawSolver2D._allocate_workspace does (classic/solver.py:466-494). */
+double ref_step2_fns(const orc_rp *p, rp_fn rpn2, rp_fn rpt2, int ids, int meqn, int mwaves,
+                     int ng, int mx, int my, const double *qold, double *qnew, double dx,
+                     double dy, double dt, const int *method, const int *mthlim) {
   shim_set_rp(p);
+  if (!rpn2) rpn2 = (rp_fn)shim_rpn2_;
+  if (!rpt2) rpt2 = (rp_fn)shim_rpt2_;
   int maxm = mx > my ? mx : my, maux = 0, use_fwave = 0;
   long L = maxm + 2 * ng;
   int mwork = (int)(L * (5 * meqn + mwaves + meqn * mwaves));
@@ -150,13 +153,20 @@ double ref_step2(const orc_rp *p, int ids, int meqn, int mwaves, int ng, int mx,
   if (ids == 0)
     step2_(&maxm, &meqn, &mwaves, &maux, &ng, &mx, &my, qold, qnew, aux, &dx, &dy, &dt, method,
            mthlim, &cfl, qadd, fadd, gadd, q1d, dtdx1d, dtdy1d, aux1, aux2, aux3, work, &mwork,
-           &use_fwave, (rp_fn)shim_rpn2_, (rp_fn)shim_rpt2_);
+           &use_fwave, rpn2, rpt2);
   else
     step2ds_(&maxm, &meqn, &mwaves, &maux, &ng, &mx, &my, qold, qnew, aux, &dx, &dy, &dt,
              method, mthlim, &cfl, qadd, fadd, gadd, q1d, dtdx1d, dtdy1d, aux1, aux2, aux3,
-             work, &mwork, &ids, &use_fwave, (rp_fn)shim_rpn2_, (rp_fn)shim_rpt2_);
+             work, &mwork, &ids, &use_fwave, rpn2, rpt2);
   free(buf);
   return cfl;
+}
+
+double ref_step2(const orc_rp *p, int ids, int meqn, int mwaves, int ng, int mx, int my,
+                 const double *qold, double *qnew, double dx, double dy, double dt,
+                 const int *method, const int *mthlim) {
+  return ref_step2_fns(p, 0, 0, ids, meqn, mwaves, ng, mx, my, qold, qnew, dx, dy, dt, method,
+                       mthlim);
 }
 
 /* The unsplit step2 with aux fields (method[6] = maux; method[5] = index_capa
@@ -246,6 +256,11 @@ extern void flux1_(const double *q1d, double *dq1d, const double *aux, const dou
                    const int *num_eqn, const int *mx, const int *num_ghost, const int *maxnx,
                    rp_fn rp, rp_fn tfluct);
 
+/* An rpn2 (the reference argument list) to use instead of the oracle's solver
+ * in the next ref_sharp_flux2 calls without aux; NULL restores the oracle's. */
+static rp_fn g_sharp_rpn2 = 0;
+void shim_set_sharp_rpn2(rp_fn f) { g_sharp_rpn2 = f; }
+
 /* SharpClawSolver2D.dq_hyperbolic's call (sharpclaw/solver.py:893) with the
  * module set-up of _set_fortran_parameters (sharpclaw/solver.py:563-588). */
 double ref_sharp_flux2(const orc_rp *p, int char_decomp, int meqn, int mwaves, int ng, int mx,
@@ -261,7 +276,8 @@ double ref_sharp_flux2(const orc_rp *p, int char_decomp, int meqn, int mwaves, i
   memset(dq, 0, sizeof(double) * nq);
   double aux0[1] = {0.0}, t = 0.0, cfl = 0.0;
   flux2_(q, dq, num_aux > 0 ? auxq : aux0, &dt, &cfl, &t, &num_aux, &meqn, &ng, &maxm, &mx, &my,
-         num_aux > 0 ? (rp_fn)shim_rpn2_aux_ : (rp_fn)shim_rpn2_, (rp_fn)shim_unused_);
+         num_aux > 0 ? (rp_fn)shim_rpn2_aux_ : g_sharp_rpn2 ? g_sharp_rpn2 : (rp_fn)shim_rpn2_,
+         (rp_fn)shim_unused_);
   return cfl;
 }
 

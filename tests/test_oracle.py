@@ -90,6 +90,64 @@ def test_live_reference_bit_exact():
         assert c1 == c2
 
 
+def _mild_euler5(rng, shape):
+    """Euler states with a tracer, varied enough to exercise every wave and the
+    entropy fix's branches but smooth enough that WENO5 keeps the pressure
+    positive."""
+    rho = rng.uniform(0.7, 1.3, shape)
+    u, v = rng.uniform(-1.2, 1.2, shape), rng.uniform(-1.2, 1.2, shape)
+    p = rng.uniform(0.6, 1.0, shape)
+    return np.asfortranarray(np.array([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v),
+                                       rng.uniform(0, 1, shape)]))
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_EULER5_PATH),
+                    reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0])
+def test_live_reference_euler_solvers_bit_exact(gamma):
+    """The oracle's Euler solvers against the reference tree's own Fortran ones
+    (development/rp_approaches/rpn2_euler_5wave.f, rpt2_euler_5wave.f, entropy
+    fix on), both inside the reference's step2 (0/1/2 transverse waves), step2ds
+    sweeps and SharpClaw flux2.  euler_4wave_2D is the same solver without the
+    tracer: it is compared on a state whose tracer is zero, on the first four
+    fields.  This pins the Euler rpt2 per step, which no fixture does."""
+    rng = np.random.default_rng(41)
+    rn, rt = O.ref_euler5(gamma)
+    rp5 = O.make_rp(O.EULER5, efix=1, gamma=gamma)
+    rp4 = O.make_rp(O.EULER, efix=1, gamma=gamma)
+    ng = 2
+    s = (slice(None), slice(ng, -ng), slice(ng, -ng))
+    for ids, trans in ((0, 0), (0, 1), (0, 2), (1, 0), (2, 0)):
+        meth = (O.method_array(transverse=trans) if ids == 0
+                else O.method_array(dimensional_split=True))
+        q = _mild_euler5(rng, (17, 11))
+        for four in (False, True):
+            if four:
+                q[4] = 0.0
+            qbc = O.with_ghosts(q, ng)
+            O.bc2(qbc, ng, [3, 1], [2, 3])
+            a, b = qbc.copy(order="F"), qbc.copy(order="F")
+            c2 = O.ref_step2_fns(rp5, rn, rt, qbc, b, ng, 0.07, 0.05, 0.011, meth, [4, 3, 2, 1, 4],
+                                 ids=ids)
+            if four:
+                a = np.asfortranarray(a[:4])
+                c1 = O.step2(rp4, np.asfortranarray(qbc[:4]), a, ng, 0.07, 0.05, 0.011, meth,
+                             [4, 3, 2, 1], ids=ids)
+                b = b[:4]
+            else:
+                c1 = O.step2(rp5, qbc, a, ng, 0.07, 0.05, 0.011, meth, [4, 3, 2, 1, 4], ids=ids)
+            assert np.array_equal(a[s], b[s]), (ids, trans, four, np.abs(a[s] - b[s]).max())
+            assert c1 == c2
+    ng = 3
+    s = (slice(None), slice(ng, -ng), slice(ng, -ng))
+    qbc = O.with_ghosts(_mild_euler5(rng, (13, 9)), ng)
+    O.bc2(qbc, ng, [3, 1], [2, 3])
+    d1, c1 = O.sharp_flux2(rp5, qbc, ng, 0.1, 0.08, 0.01, 0)
+    d2, c2 = O.ref_sharp_flux2(rp5, qbc, ng, 0.1, 0.08, 0.01, 0, rpn2=rn)
+    assert np.isfinite(d1[s]).all()
+    assert np.array_equal(d1[s], d2[s]) and c1 == c2
+
+
 def _aux_case(vec, key):
     meta = vec[key + "_meta"]
     kind, ids, trans, lim, ng, mx, my, meqn = [int(x) for x in meta]
