@@ -221,12 +221,16 @@ int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
 int clawhip_ssp104_mid(int dtype, void *q, void *s1, long long n, void *stream);
 
 /* The time-integrator updates that take no operator evaluation, over n
- * elements (SharpClawSolver.step, sharpclaw/solver.py:306-307 Euler and
- * 351-374 SSPLMMk2/k3), with the reference's grouping; c[5] are the FP64
- * scalars the solver computed:
- *   mode 0: out = q + c1*u                                 (Euler: c1 = dt, u = dq_dt)
+ * elements (SharpClawSolver.step, sharpclaw/solver.py:306-307 Euler, 328-348
+ * RK, 351-374 SSPLMMk2/k3 and 376-384 LMM), with the reference's grouping;
+ * c[5] are the FP64 scalars the solver computed:
+ *   mode 0: out = q + c1*u                                 (Euler: c1 = dt, u = dq_dt;
+ *                                                           RK: one a[i,j] or b[j] term)
  *   mode 1: out = c0*(c1*q + c2*u) + c3*R                  (SSPLMMk2)
  *   mode 2: out = (c0*(c1*q + c2*u) + c3*R) + c4*v         (SSPLMMk3)
+ *   mode 3: out = c1*u                                     (RK: dt*dq_dt)
+ *   mode 4: out = c1*u + c2*v                              (LMM: the newest term)
+ *   mode 5: out = q + (c1*u + c2*v)                        (LMM: each older term)
  * R and v may be NULL where the mode does not read them; out may alias q. */
 int clawhip_lmm_update(int dtype, int mode, long long n, const void *q, const void *u,
                        const void *R, const void *v, const double *c, void *out, void *stream);

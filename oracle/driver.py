@@ -28,7 +28,8 @@ class OracleRun:
                  max_steps=10000, dt_variable=True, time_integrator="SSP104",
                  char_decomp=0, num_ghost=None, aux=None, aux_bc_lower=None,
                  aux_bc_upper=None, before_step=None, step_source=None, source_split=1,
-                 index_capa=-1, auxbc_fn=None, lmm_steps=4, user_bc_lower=None):
+                 index_capa=-1, auxbc_fn=None, lmm_steps=4, user_bc_lower=None,
+                 rk_a=None, rk_b=None, lmm_alpha=None, lmm_beta=None):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -77,7 +78,12 @@ class OracleRun:
         elif time_integrator == "SSPLMMk3":
             self.sspcoeff = (lmm_steps - 3.) / (lmm_steps - 1.)
         self.sspcoeff0 = None
-        self.registers = [None] * lmm_steps
+        # the general methods: RK (Butcher a, b) and LMM (alpha, beta)
+        self.rk_a, self.rk_b = rk_a, rk_b
+        self.lmm_alpha, self.lmm_beta = lmm_alpha, lmm_beta
+        # number of saved states (sharpclaw/solver.py:596-610)
+        self.nreg = len(lmm_alpha) if time_integrator == "LMM" else lmm_steps
+        self.registers = [None] * self.nreg
         self.prev_dq_dt, self.prev_dt, self.prev_dtFE = [], [], []
 
     # ---- boundary conditions: Solver._apply_bcs (solver.py:351-437)
@@ -143,8 +149,9 @@ class OracleRun:
     # ---- SharpClawSolver.accept_reject_step (sharpclaw/solver.py:622-650)
     def _accept(self):
         if self.solver == "sharp" and "LMM" in self.time_integrator:
-            if self.numsteps + 1 < self.lmm_steps:
-                return not (self.cfl > self.sspcoeff0 / self.sspcoeff * self.cfl_max)
+            if self.numsteps + 1 < self.nreg:
+                ratio = 1. if self.time_integrator == "LMM" else self.sspcoeff0 / self.sspcoeff
+                return not (self.cfl > ratio * self.cfl_max)
             return True
         return not (self.cfl > self.cfl_max)
 
@@ -154,9 +161,18 @@ class OracleRun:
             self.registers[-1] = self.q.copy(order="F")
             self.prev_dq_dt.append(self.dq_dt)
             self.prev_dt.append(0.)
-            dtFE = self.dt / self.cfl * self.cfl_max / self.sspcoeff
-            self.prev_dtFE.append(dtFE)
-            self.sspcoeff0 = 1.0   # _sspcoeff['SSP22']
+            if "SSPLMM" in self.time_integrator:
+                dtFE = self.dt / self.cfl * self.cfl_max / self.sspcoeff
+                self.prev_dtFE.append(dtFE)
+                self.sspcoeff0 = 1.0   # _sspcoeff['SSP22']
+        elif self.accept_step and self.time_integrator == "LMM":
+            if step_index <= self.nreg:
+                self.prev_dq_dt.append(self.dq_dt)
+            else:
+                self.prev_dq_dt = self.prev_dq_dt[1:] + self.prev_dq_dt[:1]
+                self.prev_dq_dt[-1] = self.dq_dt
+            self.registers = self.registers[1:] + self.registers[:1]
+            self.registers[-1] = self.q.copy(order="F")
         elif self.accept_step:
             dtFE = self.dt / self.cfl * self.cfl_max / self.sspcoeff
             if step_index <= self.lmm_steps:
@@ -278,7 +294,24 @@ class OracleRun:
             r = q + dt * self.dq_dt
             r = 0.75 * q + 0.25 * (r + self._dq(r))
             self.q = 1. / 3. * q + 2. / 3. * (r + self._dq(r))
-        elif self.time_integrator == "SSP104":
+        elif self.time_integrator == "RK":
+            a, b = self.rk_a, self.rk_b
+            regs = [dt * self.dq_dt]
+            for i in range(1, len(b)):
+                r = q.copy(order="F")
+                for j in range(i):
+                    r = r + a[i][j] * regs[j]
+                regs.append(self._dq(r))
+            for j in range(len(b)):
+                q = q + b[j] * regs[j]
+            self.q = q
+        elif self.time_integrator == "LMM" and step_index >= self.nreg:
+            al, be, R, D = self.lmm_alpha, self.lmm_beta, self.registers, self.prev_dq_dt
+            qn = al[-1] * R[-1] + be[-1] * dt * D[-1]
+            for i in range(self.lmm_steps - 1):
+                qn = qn + (al[i] * R[i] + be[i] * dt * D[i])
+            self.q = qn
+        elif self.time_integrator in ("SSP104", "LMM"):
             s1 = q + dt * self.dq_dt / 6.
             for _ in range(4):
                 s1 = s1 + self._dq(s1) / 6.

@@ -231,9 +231,9 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
 int clawhip_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
                        const void* R, const void* v, const double* c, void* out,
                        void* stream) {
-  if (!q || !u || !c || !out || n < 0 || mode < 0 || mode > 2) return CLAWHIP_E_ARG;
-  if (mode >= 1 && !R) return CLAWHIP_E_ARG;
-  if (mode == 2 && !v) return CLAWHIP_E_ARG;
+  if (!q || !u || !c || !out || n < 0 || mode < 0 || mode > 5) return CLAWHIP_E_ARG;
+  if ((mode == 1 || mode == 2) && !R) return CLAWHIP_E_ARG;
+  if ((mode == 2 || mode == 4 || mode == 5) && !v) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   return launch_lmm_update(dtype, mode, n, q, u, R, v, c, out, (hipStream_t)stream);
 }

@@ -695,11 +695,14 @@ __global__ void k_lmm_update(int mode, long long n, const T* q, const T* u, cons
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   T o;
-  if (mode == 0) {
-    o = q[i] + c1 * u[i];
-  } else {
-    o = c0 * (c1 * q[i] + c2 * u[i]) + c3 * R[i];
-    if (mode == 2) o = o + c4 * v[i];
+  switch (mode) {
+    case 0: o = q[i] + c1 * u[i]; break;
+    case 3: o = c1 * u[i]; break;
+    case 4: o = c1 * u[i] + c2 * v[i]; break;
+    case 5: o = q[i] + (c1 * u[i] + c2 * v[i]); break;
+    default:
+      o = c0 * (c1 * q[i] + c2 * u[i]) + c3 * R[i];
+      if (mode == 2) o = o + c4 * v[i];
   }
   out[i] = o;
 }

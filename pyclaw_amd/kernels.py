@@ -192,7 +192,8 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
 
 
 def lmm_update(mode, q, u, coeffs, out, R=None, v=None):
-    """Euler / SSPLMMk2 / SSPLMMk3 update (include/clawhip.h, clawhip_lmm_update)."""
+    """Euler / RK / SSPLMMk2 / SSPLMMk3 / LMM updates (include/clawhip.h,
+    clawhip_lmm_update; q is read by modes 0, 1, 2 and 5)."""
     c = (ctypes.c_double * 5)(*[float(x) for x in (list(coeffs) + [0.0] * 5)[:5]])
     for t in (u, R, v, out):
         assert t is None or (t.shape == q.shape and t.dtype == q.dtype and t.is_contiguous())

@@ -2,10 +2,11 @@
 
 The semi-discrete operator dq = dt*L(q) (ghost fill + WENO5 reconstruction +
 Riemann solves) runs in clawhip_sharp2_stage, fused with the Runge-Kutta stage
-combination; Euler, SSP22, SSP33, SSP104 and the multistep SSPLMMk2/k3 keep the
-reference's stage sequence, step-size control and arithmetic
-(sharpclaw/solver.py:283-490, 595-690), so the FP64 path equals the reference
-step bit for bit.  precision = "fp32k" runs SSP33 with FP32 stages and a
+combination; Euler, SSP22, SSP33, SSP104, a general explicit Runge-Kutta method
+(RK: solver.a, b, c), the multistep SSPLMMk2/k3 and a general linear multistep
+method (LMM: solver.alpha, beta; fixed dt) keep the reference's stage sequence,
+step-size control and arithmetic (sharpclaw/solver.py:283-490, 595-690), so the
+FP64 path equals the reference step bit for bit.  precision = "fp32k" runs SSP33 with FP32 stages and a
 Kahan-compensated final update.
 """
 import torch
@@ -16,10 +17,11 @@ from .solver import BC, Solver
 
 class SharpClawSolver(Solver):
     _sspcoeff = {"Euler": 1.0, "SSP22": 1.0, "SSP33": 1.0, "SSP104": 6.0,
-                 "SSPLMM32": 0.5, "SSPLMM43": 1. / 3., "SSPLMM53": 0.5}
+                 "SSPLMM32": 0.5, "SSPLMM43": 1. / 3., "SSPLMM53": 0.5, "RK": None,
+                 "LMM": None}
     _cfl_default = {"SSP104": [2.45, 2.5], "SSPLMM32": [0.24, 0.25],
                     "SSPLMM43": [0.15, 1. / 6.], "SSPLMM53": [0.24, 0.25]}
-    _integrators = ("Euler", "SSP22", "SSP33", "SSP104", "SSPLMMk2", "SSPLMMk3")
+    _integrators = ("Euler", "SSP22", "SSP33", "SSP104", "SSPLMMk2", "SSPLMMk3", "RK", "LMM")
 
     def __init__(self, riemann_solver=None, claw_package=None):
         self.limiters = [1]
@@ -47,7 +49,13 @@ class SharpClawSolver(Solver):
         self.indicator = None
         self._flags = None
         self._qaux = None     # (device aux, lower codes, upper codes) for the current step
+        # general Runge-Kutta coefficients (time_integrator 'RK')
+        self.a = None
+        self.b = None
+        self.c = None
         # multistep methods (sharpclaw/solver.py:186-195)
+        self.alpha = None
+        self.beta = None
         self.sspcoeff0 = None
         self.lmm_steps = 4
         self.sspcoeff = None
@@ -74,6 +82,13 @@ class SharpClawSolver(Solver):
             raise NotImplementedError("the FP32 path implements SSP33")
         if self.indicator is not None and self.time_integrator not in ("SSP33", "SSP104"):
             raise NotImplementedError("the indicator runs with SSP33 or SSP104")
+        if self.time_integrator == "LMM":
+            assert self.dt_variable == False, \
+                "Must set solver.dt_variable=False for LMM integrator."
+            if self.alpha is None or self.beta is None:
+                raise ValueError("the LMM integrator needs solver.alpha and solver.beta")
+        if self.time_integrator == "RK" and (self.a is None or self.b is None):
+            raise ValueError("the RK integrator needs solver.a and solver.b")
         if self.check_lmm_cond:
             raise NotImplementedError("check_lmm_cond (the third-order SSPLMM a-posteriori "
                                       "conditions) is not supported")
@@ -153,13 +168,22 @@ class SharpClawSolver(Solver):
         else:
             self.dt = self.dt * self.cfl_desired / cfl
 
+    def _num_registers(self):
+        """The number of saved states (sharpclaw/solver.py:596-610)."""
+        if self.time_integrator == "RK":
+            return len(self.b) + 1
+        if self.time_integrator == "LMM":
+            return len(self.alpha)
+        return self.lmm_steps
+
     # ---- sharpclaw/solver.py:622-650
     def accept_reject_step(self, state):
         cfl = self.cfl.get_cached_max()
         if "LMM" in self.time_integrator:
             step_index = self.status["numsteps"] + 1
-            if step_index < self.lmm_steps:   # only the starting steps are checked
-                return not (cfl > self.sspcoeff0 / self.sspcoeff * self.cfl_max)
+            if step_index < self._num_registers():   # only the starting steps are checked
+                ratio = 1. if self.time_integrator == "LMM" else self.sspcoeff0 / self.sspcoeff
+                return not (cfl > ratio * self.cfl_max)
             return True
         return not (cfl > self.cfl_max)
 
@@ -242,7 +266,7 @@ class SharpClawSolver(Solver):
         # read at the end of the step (max is order-free, so self.cfl ends as the
         # stage-by-stage updates of sharpclaw/solver.py:283-430 leave it)
         if "LMM" in self.time_integrator:
-            self._lmm_step(q, out_q, step_index, cs)
+            self._lmm_step(q, comp, out_q, step_index, cs)
         else:
             self._rk_stages(q, comp, self.dq_dt, out_q, self.dt, cs)
         if self._flags is not None:
@@ -258,17 +282,27 @@ class SharpClawSolver(Solver):
         """sharpclaw/solver.py:432-490: the saved states, step sizes and
         forward-Euler step sizes of the multistep method; the oldest saved state
         is rolled to the end and overwritten with the current one."""
+        nreg = self._num_registers()
         if self._registers is None:
-            self._registers = [torch.empty_like(q) for _ in range(self.lmm_steps)]
+            self._registers = [torch.empty_like(q) for _ in range(nreg)]
         if self.prev_dt_values == []:
             # the very first step
             self._registers[-1].copy_(q)
             self.prev_dq_dt_values.append(self.dq_dt.clone())
             self.prev_dt_values.append(0.)   # not used
-            cfl = self.cfl.get_cached_max()
-            dtFE = self.dt / cfl * self.cfl_max / self.sspcoeff
-            self.prev_dtFE_values.append(dtFE)
-            self.sspcoeff0 = self._sspcoeff["SSP22"]
+            if "SSPLMM" in self.time_integrator:
+                cfl = self.cfl.get_cached_max()
+                dtFE = self.dt / cfl * self.cfl_max / self.sspcoeff
+                self.prev_dtFE_values.append(dtFE)
+                self.sspcoeff0 = self._sspcoeff["SSP22"]
+        elif self.accept_step and self.time_integrator == "LMM":
+            if step_index <= nreg:
+                self.prev_dq_dt_values.append(self.dq_dt.clone())
+            else:
+                self.prev_dq_dt_values = self.prev_dq_dt_values[1:] + self.prev_dq_dt_values[:1]
+                self.prev_dq_dt_values[-1].copy_(self.dq_dt)
+            self._registers = self._registers[1:] + self._registers[:1]
+            self._registers[-1].copy_(q)
         elif self.accept_step:
             cfl = self.cfl.get_cached_max()
             dtFE = self.dt / cfl * self.cfl_max / self.sspcoeff
@@ -289,10 +323,21 @@ class SharpClawSolver(Solver):
             self._registers[-1].copy_(q)
         return step_index
 
-    def _lmm_step(self, q, out_q, step_index, cs):
-        """sharpclaw/solver.py:351-374: SSP22 for the starting values, then the
-        SSPLMMk2 / k3 combination (no operator evaluation beyond dq_dt)."""
+    def _lmm_step(self, q, comp, out_q, step_index, cs):
+        """sharpclaw/solver.py:351-384: SSP22 (SSPLMM) or SSP104 (LMM) for the
+        starting values, then the multistep combination (no operator evaluation
+        beyond dq_dt)."""
         dt = self.dt
+        if self.time_integrator == "LMM":
+            regs, prev = self._registers, self.prev_dq_dt_values
+            if step_index < len(regs):
+                self._rk_stages(q, comp, self.dq_dt, out_q, dt, cs, method="SSP104")
+                return
+            al, be = self.alpha, self.beta
+            K.lmm_update(4, regs[-1], regs[-1], (0.0, al[-1], be[-1] * dt), out_q[0], v=prev[-1])
+            for i in range(self.lmm_steps - 1):
+                K.lmm_update(5, out_q[0], regs[i], (0.0, al[i], be[i] * dt), out_q[0], v=prev[i])
+            return
         num_steps = self.lmm_steps
         if step_index < num_steps:
             self._stage(q, out_q[0], q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
@@ -315,15 +360,31 @@ class SharpClawSolver(Solver):
                          (beta_k_minus_1, r, dt, r * beta0 + delta0, beta0 * dt), out_q[0],
                          R=self._registers[-num_steps], v=self.prev_dq_dt_values[-num_steps])
 
-    def _rk_stages(self, q, comp, dq_dt, out_q, dt, cs):
+    def _rk_stages(self, q, comp, dq_dt, out_q, dt, cs, method=None):
         """The Runge-Kutta stages after the first operator evaluation
-        (sharpclaw/solver.py:306-323, 388-430), into out_q."""
-        if self.time_integrator == "Euler":
+        (sharpclaw/solver.py:306-348, 388-430), into out_q."""
+        ti = self.time_integrator if method is None else method
+        if ti == "Euler":
             K.lmm_update(0, q, dq_dt, (0.0, dt), out_q[0])
-        elif self.time_integrator == "SSP22":
+        elif ti == "SSP22":
             self._stage(q, out_q[0], q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
                         out_mode=K.OUT_SSP22, A=q)
-        elif self.time_integrator == "SSP33":
+        elif ti == "RK":
+            # registers[0] = dt*dq_dt; stage i: q + sum_j a[i,j]*registers[j], term
+            # by term, then registers[i] = dq(stage i); q += b[j]*registers[j]
+            a, b = self.a, self.b
+            regs = [self._buf("rk%d" % j, q) for j in range(len(b))]
+            K.lmm_update(3, dq_dt, dq_dt, (0.0, dt), regs[0])
+            y = self._buf("rk_y", q)
+            for i in range(1, len(b)):
+                y.copy_(q)
+                for j in range(i):
+                    K.lmm_update(0, y, regs[j], (0.0, float(a[i][j])), y)
+                self._stage(q, regs[i], y, cfl=cs, out_mode=K.OUT_D)
+            out_q[0].copy_(q)
+            for j in range(len(b)):
+                K.lmm_update(0, out_q[0], regs[j], (0.0, float(b[j])), out_q[0])
+        elif ti == "SSP33":
             r2 = self._buf("r2", q)
             if self.precision == "fp64":
                 self._stage(q, r2, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,

@@ -219,6 +219,40 @@ def test_more_integrators_match_oracle(ti, steps):
     assert np.array_equal(got, run.q), np.abs(got - run.q).max()
 
 
+RK3_A = [[0., 0., 0.], [0.5, 0., 0.], [-1., 2., 0.]]   # Kutta's third-order method
+RK3_B = [1. / 6., 2. / 3., 1. / 6.]
+
+
+@pytest.mark.parametrize("ti", ["RK", "LMM"])
+def test_general_integrators_match_oracle(ti):
+    """The general explicit Runge-Kutta method (solver.a, b: Kutta's third-order
+    tableau) with variable dt, and the general linear multistep method
+    (solver.alpha, beta: the 3-step Adams-Bashforth weights, SSP104 starting
+    steps, fixed dt): whole runs on shallow water equal the oracle's bit for bit."""
+    from pyclaw_amd import riemann
+    rng = np.random.default_rng(56)
+    q = seeded_state(rng, O.SHALLOW, (40, 30))
+    if ti == "RK":
+        opts = dict(time_integrator="RK", a=RK3_A, b=RK3_B, c=[0., 0.5, 1.], cfl_desired=0.45,
+                    cfl_max=0.5, dt_initial=0.0005)
+        kw = dict(time_integrator="RK", rk_a=RK3_A, rk_b=RK3_B, cfl_desired=0.45, cfl_max=0.5,
+                  dt_initial=0.0005)
+    else:
+        al, be = [0., 0., 1.], [5. / 12., -16. / 12., 23. / 12.]
+        opts = dict(time_integrator="LMM", alpha=al, beta=be, lmm_steps=3, dt_variable=False,
+                    dt_initial=0.0005, cfl_max=0.5, cfl_desired=0.45)
+        kw = dict(time_integrator="LMM", lmm_alpha=al, lmm_beta=be, lmm_steps=3,
+                  dt_variable=False, dt_initial=0.0005, cfl_max=0.5, cfl_desired=0.45)
+    claw = _run_sharp(riemann.shallow_roe_with_efix_2D, q, ([0, 0], [1, 1]), [2, 1], [2, 3],
+                      0.01, 2, {"grav": 1.0}, **opts)
+    run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, [1.0 / 40, 1.0 / 30], [2, 1],
+                    [2, 3], solver="sharp", **kw)
+    run.run(0.01, 2)
+    got = claw.frames[-1].state.q
+    assert claw.solver.status["numsteps"] == run.numsteps
+    assert np.array_equal(got, run.q), np.abs(got - run.q).max()
+
+
 @pytest.mark.parametrize("kind,meqn,maux", [(O.VC_ACOUSTICS, 3, 2), (O.VC_ACOUSTICS, 3, 3),
                                             (O.VC_ADVECTION_2D, 1, 2), (O.VC_ADVECTION, 2, 2)])
 def test_operator_aux_bit_exact(kind, meqn, maux):

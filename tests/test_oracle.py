@@ -520,6 +520,67 @@ def test_euler5_tracer_test_catches_planted(plant, split, trans):
     assert fails > 0
 
 
+def _sharp_run(ti, **kw):
+    rng = np.random.default_rng(77)
+    q = seeded_state(rng, O.SHALLOW, (24, 18))
+    run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, [1.0 / 24, 1.0 / 18], [2, 1],
+                    [2, 3], solver="sharp", time_integrator=ti, dt_initial=0.001, **kw)
+    run.run(0.01, 1)
+    return run
+
+
+def test_general_rk_reduces_to_named_methods():
+    """sharpclaw/solver.py:328-348 (time_integrator 'RK'), pinned by the methods
+    it must reproduce: the one-stage tableau is forward Euler, bit for bit, and
+    SSP33's Shu-Osher tableau gives SSP33's step to round-off (the two group
+    the sums differently)."""
+    fixed = dict(dt_variable=False, cfl_max=0.5, cfl_desired=0.45)
+    e = _sharp_run("Euler", **fixed)
+    r = _sharp_run("RK", rk_a=[[0.]], rk_b=[1.], **fixed)
+    assert np.array_equal(e.q, r.q)
+    s = _sharp_run("SSP33", **fixed)
+    r3 = _sharp_run("RK", rk_a=[[0., 0., 0.], [1., 0., 0.], [0.25, 0.25, 0.]],
+                    rk_b=[1. / 6., 1. / 6., 2. / 3.], **fixed)
+    assert e.numsteps == r.numsteps == s.numsteps == r3.numsteps == 10
+    assert np.abs(s.q - r3.q).max() < 1e-13 * np.abs(s.q).max()
+    assert np.abs(s.q - e.q).max() > 1e-6   # the comparison can tell methods apart
+
+
+def test_general_lmm_reduces_to_named_methods():
+    """sharpclaw/solver.py:376-384 and the 'LMM' parts of update_saved_values
+    (476-482), accept_reject_step (633-642): with alpha = (0, 1), beta = (0, 1)
+    the method is forward Euler after one SSP104 starting step, bit for bit;
+    with the Adams-Bashforth weights on a linear problem (advection) the error
+    against the exact translate shrinks at second order."""
+    fixed = dict(dt_variable=False, cfl_max=2.5, cfl_desired=2.45)
+    lmm = _sharp_run("LMM", lmm_alpha=[0., 1.], lmm_beta=[0., 1.], lmm_steps=2, **fixed)
+    rng = np.random.default_rng(77)
+    q = seeded_state(rng, O.SHALLOW, (24, 18))
+    one = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, [1.0 / 24, 1.0 / 18], [2, 1],
+                    [2, 3], solver="sharp", time_integrator="SSP104", dt_initial=0.001, **fixed)
+    one.run(0.001, 1)
+    rest = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), one.q, [1.0 / 24, 1.0 / 18],
+                     [2, 1], [2, 3], solver="sharp", time_integrator="Euler", dt_initial=0.001,
+                     **fixed)
+    rest.run(0.009, 1)
+    assert lmm.numsteps == 10
+    assert np.array_equal(lmm.q, rest.q)
+    errs = []
+    for n in (32, 64):
+        x = (np.arange(n) + 0.5) / n
+        X, Y = np.meshgrid(x, x, indexing="ij")
+        q0 = np.sin(2 * np.pi * X)[None] + 0 * Y
+        run = OracleRun(O.make_rp(O.ADVECTION, u=1.0, v=0.0), np.asfortranarray(q0), [1. / n] * 2,
+                        [2, 2], [2, 2], solver="sharp", time_integrator="LMM",
+                        lmm_alpha=[0., 1.], lmm_beta=[-0.5, 1.5], lmm_steps=2,
+                        dt_variable=False, dt_initial=0.4 / n, cfl_max=2.5, cfl_desired=2.45)
+        run.run(0.2, 1)
+        errs.append(np.abs(run.q[0] - np.sin(2 * np.pi * (X - 0.2))).max())
+    rate = np.log2(errs[0] / errs[1])
+    print("AB2 errors", errs, "rate", rate)
+    assert 1.8 < rate < 3.2
+
+
 def test_fixture_radial_dam_break_classic(golden_dir):
     q, d = C.radial_dam_break()
     run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, d, [1, 1], [3, 3], limiters=4)
