@@ -193,8 +193,14 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
  *            6: FP32 SSP33 stage 3: Kahan update of A by aux_in + 2/3*D (comp_in->comp_out)
  *            7: out = D                         (fmod.flux2's dq on the interior)
  *            8: out = 0.5*((A + Y) + D)         (SSP22, solver.py:388-393)
- * All arrays compact (ghosts = 0) with geometry g; num_ghost is 3 (WENO5).
- * Boundary codes 1..3 only.  *cfl_dev = max(*cfl_dev, CFL of this evaluation).
+ * All arrays compact (ghosts = 0) with geometry g, except P when gin (may be
+ * NULL) gives it >= 3 stored ghosts: then the caller has filled P's ghosts
+ * (custom boundary functions run on the host) and boundary code 4 (GIVEN)
+ * reads them; that form takes y_mode 0 and no flags.  num_ghost is 3 (WENO5).
+ * Boundary codes 1..3, or 4 with a ghosted P.
+ * src (may be NULL, geometry g): a source increment added to D before the
+ * stage combination, dq = dq_hyperbolic + dq_src (sharpclaw/solver.py:537-540).
+ * *cfl_dev = max(*cfl_dev, CFL of this evaluation).
  * flags (may be NULL): per-interface troubled flags from clawhip_mlp_flags.
  * An interface whose flag and whose two neighbours' flags (along its normal) are
  * 0 uses WENO5's linear 5th-order weights; the others use the WENO5 selected by
@@ -209,9 +215,10 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
 int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
                          const clawhip_geom *g, const int *bc_lower, const int *bc_upper,
                          const int *reflect, int y_mode, int out_mode, double dt, double s,
-                         double dx, double dy, const void *P, const void *Q, const void *A,
-                         const void *B, const void *aux_in, const void *comp_in, void *out,
-                         void *aux_out, void *comp_out, const unsigned char *flags,
+                         double dx, double dy, const void *P, const clawhip_geom *gin,
+                         const void *Q, const void *A, const void *B, const void *aux_in,
+                         const void *comp_in, void *out, void *aux_out, void *comp_out,
+                         const void *src, const unsigned char *flags,
                          const int *troubled_cells, const int *troubled_count, const void *aux,
                          const clawhip_geom *gaux, const int *aux_bc_lower,
                          const int *aux_bc_upper, int num_aux, double *cfl_dev, void *stream);

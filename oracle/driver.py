@@ -29,7 +29,7 @@ class OracleRun:
                  char_decomp=0, num_ghost=None, aux=None, aux_bc_lower=None,
                  aux_bc_upper=None, before_step=None, step_source=None, source_split=1,
                  index_capa=-1, auxbc_fn=None, lmm_steps=4, user_bc_lower=None,
-                 rk_a=None, rk_b=None, lmm_alpha=None, lmm_beta=None):
+                 rk_a=None, rk_b=None, lmm_alpha=None, lmm_beta=None, dq_src=None):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -70,6 +70,8 @@ class OracleRun:
         # user_bc_lower(run, dim, qbc): Solver.user_bc_lower for BC.custom lower
         # sides, called in the reference's per-dimension order
         self.user_bc_lower = user_bc_lower
+        # dq_src(run, q, dt) -> the source increment SharpClaw adds to dq
+        self.dq_src = dq_src
         # multistep state (SSPLMMk2 / k3)
         self.lmm_steps = lmm_steps
         self.sspcoeff = None
@@ -250,10 +252,15 @@ class OracleRun:
     def _dq(self, q):
         qbc = self._apply_bcs(q)
         dx, dy = self.delta
+        reads_aux = self.aux is not None and self.rp.kind in (6, 7, 8)
         dq, cfl = O.sharp_flux2(self.rp, qbc, self.ng, dx, dy, self.dt, self.char_decomp,
-                                aux=None if self.aux is None else self._auxbc())
+                                aux=self._auxbc() if reads_aux else None)
         self.cfl = max(self.cfl, cfl)
-        return self._interior(dq)
+        dq = self._interior(dq)
+        if self.dq_src is not None:
+            # dq_hyperbolic + dq_src (sharpclaw/solver.py:537-540)
+            dq = dq + self.dq_src(self, q, self.dt)
+        return dq
 
     # ---- SharpClawSolver.step (sharpclaw/solver.py:283-430)
     def _step_sharp(self, t, tstart, tend, take_one_step):

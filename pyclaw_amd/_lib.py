@@ -90,8 +90,9 @@ _SIGS = {
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                             _D, _D, _D, _D, _P, _P],
     "clawhip_sharp2_stage": [ctypes.POINTER(RP), _I, _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
-                             _I, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                             _P, _P, ctypes.POINTER(Geom), _IP, _IP, _I, _P, _P],
+                             _I, _I, _D, _D, _D, _D, _P, ctypes.POINTER(Geom), _P, _P, _P, _P,
+                             _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(Geom), _IP, _IP,
+                             _I, _P, _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_lmm_update": [_I, _I, ctypes.c_longlong, _P, _P, _P, _P,
                            ctypes.POINTER(ctypes.c_double), _P, _P],

@@ -191,14 +191,21 @@ int clawhip_step3_split(const clawhip_rp* rp, int dtype, const void* q_in,
 int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
                          const clawhip_geom* g, const int* bc_lower, const int* bc_upper,
                          const int* reflect, int y_mode, int out_mode, double dt, double s,
-                         double dx, double dy, const void* P, const void* Q, const void* A,
-                         const void* B, const void* aux_in, const void* comp_in, void* out,
-                         void* aux_out, void* comp_out, const unsigned char* flags,
+                         double dx, double dy, const void* P, const clawhip_geom* gin,
+                         const void* Q, const void* A, const void* B, const void* aux_in,
+                         const void* comp_in, void* out, void* aux_out, void* comp_out,
+                         const void* src, const unsigned char* flags,
                          const int* troubled_cells, const int* troubled_count, const void* aux,
                          const clawhip_geom* gaux, const int* aux_bc_lower,
                          const int* aux_bc_upper, int num_aux, double* cfl_dev, void* stream) {
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !P || !out || !cfl_dev)
     return CLAWHIP_E_ARG;
+  // P with stored ghosts (gin, BC code 4 = given): a stage input whose ghosts
+  // the caller filled, read as it is (y_mode 0, no indicator flags)
+  if (gin && (!geom_ok(gin, 2) || gin->mx != g->mx || gin->my != g->my || gin->ghosts < 0))
+    return CLAWHIP_E_ARG;
+  const bool ghosted = gin && gin->ghosts > 0;
+  if (ghosted && (y_mode != 0 || flags)) return CLAWHIP_E_ARG;
   if (flags && (!troubled_cells || !troubled_count)) return CLAWHIP_E_ARG;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
@@ -212,9 +219,12 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
     return CLAWHIP_E_ARG;
   if (out == P || (Q && out == Q) || (A && out == A)) return CLAWHIP_E_ARG;
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt > 0.0)) return CLAWHIP_E_ARG;
-  for (int d = 0; d < 2; d++)
-    if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
-      return CLAWHIP_E_ARG;
+  for (int d = 0; d < 2; d++) {
+    for (int bc : {bc_lower[d], bc_upper[d]}) {
+      if (bc == CLAWHIP_BC_GIVEN ? !ghosted || gin->ghosts < 3 : bc < 1 || bc > 3)
+        return CLAWHIP_E_ARG;
+    }
+  }
   // aux: the solver's fields (at least those it reads, at most 3)
   const int na = rp_num_aux(rp);
   if (na > 0 && (num_aux < na || num_aux > 3)) return CLAWHIP_E_ARG;
@@ -222,10 +232,10 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
   int rc = make_aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, g, 3, ax);
   if (rc) return rc;
   if ((na > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
-  return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), to_geom(g), bc_lower, bc_upper, reflect,
-                            y_mode, out_mode, dt, s, dx, dy, P, Q, A, B, aux_in, comp_in, out,
-                            aux_out, comp_out, (unsigned long long*)cfl_dev, flags,
-                            troubled_cells, troubled_count, (hipStream_t)stream, ax);
+  return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), to_geom(gin ? gin : g), bc_lower,
+                            bc_upper, reflect, y_mode, out_mode, dt, s, dx, dy, P, Q, A, B,
+                            aux_in, comp_in, out, aux_out, comp_out, (unsigned long long*)cfl_dev,
+                            flags, troubled_cells, troubled_count, (hipStream_t)stream, ax, src);
 }
 
 int clawhip_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,

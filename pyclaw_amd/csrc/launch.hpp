@@ -59,7 +59,7 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
                        const unsigned char* flags, const int* cells, const int* ncells,
-                       hipStream_t stream, const AuxIn& aux);
+                       hipStream_t stream, const AuxIn& aux, const void* src = nullptr);
 int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream);
 int launch_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
                       const void* R, const void* v, const double* c, void* out,

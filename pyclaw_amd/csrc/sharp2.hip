@@ -75,6 +75,7 @@ struct SharpArgs {
   T* out;
   T* aux_out;
   T* comp_out;
+  const T* src;         // optional source increment added to D (SharpClaw dq_src)
   unsigned long long* cfl_bits;
   const unsigned char* flags;   // troubled-interface flags (learned indicator) or null
   const int* cells;             // with flags: cells next to a flagged interface
@@ -295,6 +296,8 @@ template <typename T>
 __device__ __forceinline__ void stage_store(const SharpArgs<T>& a, long long off, int m, T D,
                                             T yv) {
   const long long fs = a.g.fs;
+  // dq = dq_hyperbolic + dq_src (sharpclaw/solver.py:537-540)
+  if (a.src) D = D + a.src[off + m * fs];
   T o;
   switch (a.omode) {
     case OUT_DQDT: o = D / a.dt; break;
@@ -749,7 +752,7 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
               const void* P, const void* Q, const void* A, const void* B, const void* aux_in,
               const void* comp_in, void* out, void* aux_out, void* comp_out,
               unsigned long long* cfl, const unsigned char* flags, const int* cells,
-              const int* ncells, hipStream_t stream, const AuxIn& ax) {
+              const int* ncells, hipStream_t stream, const AuxIn& ax, const void* src) {
   SharpArgs<T> a;
   a.rp = to_params<T>(rp);
   a.g = g;
@@ -763,6 +766,7 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
   a.P = (const T*)P; a.Q = (const T*)Q; a.A = (const T*)A; a.B = (const T*)B;
   a.aux_in = (const T*)aux_in; a.comp_in = (const T*)comp_in;
   a.out = (T*)out; a.aux_out = (T*)aux_out; a.comp_out = (T*)comp_out;
+  a.src = (const T*)src;
   a.cfl_bits = cfl;
   a.flags = flags;
   a.cells = cells;
@@ -805,14 +809,16 @@ int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const Geom& g
                    double dy, const void* P, const void* Q, const void* A, const void* B,
                    const void* aux_in, const void* comp_in, void* out, void* aux_out,
                    void* comp_out, unsigned long long* cfl, const unsigned char* flags,
-                   const int* cells, const int* ncells, hipStream_t stream, const AuxIn& ax) {
+                   const int* cells, const int* ncells, hipStream_t stream, const AuxIn& ax,
+                   const void* src) {
   if (dtype == CLAWHIP_F64)
     return run_stage<RSF<double>, double, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx,
                                                 dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
-                                                comp_out, cfl, flags, cells, ncells, stream, ax);
+                                                comp_out, cfl, flags, cells, ncells, stream, ax,
+                                                src);
   return run_stage<RSF<float>, float, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
                                             Q, A, B, aux_in, comp_in, out, aux_out, comp_out,
-                                            cfl, flags, cells, ncells, stream, ax);
+                                            cfl, flags, cells, ncells, stream, ax, src);
 }
 }  // namespace
 
@@ -822,9 +828,9 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
                        const unsigned char* flags, const int* cells, const int* ncells,
-                       hipStream_t stream, const AuxIn& ax) {
+                       hipStream_t stream, const AuxIn& ax, const void* src) {
 #define ARGS dtype, rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P, Q, A, B, aux_in, \
-             comp_in, out, aux_out, comp_out, cfl, flags, cells, ncells, stream, ax
+             comp_in, out, aux_out, comp_out, cfl, flags, cells, ncells, stream, ax, src
   if (rp.kind == CLAWHIP_RP_VC_ACOUSTICS || rp.kind == CLAWHIP_RP_VC_ADVECTION ||
       rp.kind == CLAWHIP_RP_VC_ADVECTION_2D) {
     if (char_decomp != 0) return CLAWHIP_E_KIND;   // component WENO5 only

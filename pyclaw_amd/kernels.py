@@ -159,20 +159,25 @@ OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F3
 def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx, dy, P,
                  Q=None, s=0.0, y_mode=Y_ARRAY, out_mode=OUT_D, A=None, B=None, aux_in=None,
                  comp_in=None, aux_out=None, comp_out=None, flags=None, cells=None, qaux=None,
-                 qaux_bc_lower=None, qaux_bc_upper=None):
+                 qaux_bc_lower=None, qaux_bc_upper=None, src=None):
     """D = dt*L(Y) (SharpClaw flux2) fused with one RK stage; planar (meqn, my, mx).
     flags: uint8 troubled-interface flags (mlp_flags) selecting full WENO5 per
     interface, with cells = (list, count) from mlp_troubled_cells (built here when
     None); flags None = full WENO5 everywhere.  qaux: the solver's aux fields
     (planar, compact or with 3 stored ghosts and codes 4) for the solvers that
-    read them."""
+    read them.  P may carry 3 stored ghosts (planar (meqn, my+6, mx+6), boundary
+    codes 4), filled by the caller; src: planar interior dq_src increment."""
     if flags is not None and cells is None:
         cells = mlp_troubled_cells(flags, out.shape[2], out.shape[1])
     meqn, my, mx = out.shape
     _check_fields(rp, meqn, None, None)
-    for t in (P, Q, A, B):
+    for t in (P, Q, A, B, src):
         assert t is None or (t.dtype == out.dtype and t.numel() >= out.numel())
+    assert src is None or src.shape == out.shape
     g = L.geom2(mx, my)
+    gp = (P.shape[2] - mx) // 2
+    assert P.shape[1] == my + 2 * gp and P.shape[2] == mx + 2 * gp and P.shape[0] == meqn
+    gin = L.geom2(mx, my, gp) if gp else None
     code = _dtype_code(out)
     gaux = _aux_geom(qaux, mx, my)
     if qaux is not None:
@@ -180,8 +185,9 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
     rc = L.lib().clawhip_sharp2_stage(
         ctypes.byref(rp), code, int(char_decomp), ctypes.byref(g), L.ints(bc_lower),
         L.ints(bc_upper), L.ints(reflect), int(y_mode), int(out_mode), float(dt), float(s),
-        float(dx), float(dy), _ptr(P), _ptr(Q), _ptr(A), _ptr(B), _ptr(aux_in), _ptr(comp_in),
-        _ptr(out), _ptr(aux_out), _ptr(comp_out), _ptr(flags),
+        float(dx), float(dy), _ptr(P), ctypes.byref(gin) if gin is not None else None, _ptr(Q),
+        _ptr(A), _ptr(B), _ptr(aux_in), _ptr(comp_in), _ptr(out), _ptr(aux_out), _ptr(comp_out),
+        _ptr(src), _ptr(flags),
         _ptr(cells[0]) if cells is not None else None,
         _ptr(cells[1]) if cells is not None else None, _ptr(qaux),
         ctypes.byref(gaux) if qaux is not None else None,

@@ -9,10 +9,24 @@ step-size control and arithmetic (sharpclaw/solver.py:283-490, 595-690), so the
 FP64 path equals the reference step bit for bit.  precision = "fp32k" runs SSP33 with FP32 stages and a
 Kahan-compensated final update.
 """
+import numpy as np
 import torch
 
 from . import kernels as K
 from .solver import BC, Solver
+
+
+class _StageState(object):
+    """The state one operator evaluation sees (the reference's register States,
+    sharpclaw/solver.py:306-430): the solver's state with q replaced by the
+    stage's (a host array), everything else read from the state."""
+
+    def __init__(self, state, q):
+        self._state = state
+        self.q = q
+
+    def __getattr__(self, name):
+        return getattr(self._state, name)
 
 
 class SharpClawSolver(Solver):
@@ -49,6 +63,7 @@ class SharpClawSolver(Solver):
         self.indicator = None
         self._flags = None
         self._qaux = None     # (device aux, lower codes, upper codes) for the current step
+        self._state = None    # the state being stepped
         # general Runge-Kutta coefficients (time_integrator 'RK')
         self.a = None
         self.b = None
@@ -74,8 +89,12 @@ class SharpClawSolver(Solver):
             raise NotImplementedError("pyclaw_amd implements lim_type=2 with weno_order=5")
         if self.char_decomp not in (0, 2):
             raise NotImplementedError("char_decomp must be 0 (component) or 2 (characteristic)")
-        if self.tfluct_solver or self.fwave or self.dq_src is not None:
-            raise NotImplementedError("tfluct solvers, f-waves and source terms are not supported")
+        if self.tfluct_solver or self.fwave:
+            raise NotImplementedError("tfluct solvers and f-waves are not supported")
+        lo, hi = self._bc_codes()
+        if self.indicator is not None and (self.dq_src is not None or BC.custom in lo
+                                           or BC.custom in hi):
+            raise NotImplementedError("the indicator runs without dq_src and custom BCs")
         if self.time_integrator not in self._integrators:
             raise NotImplementedError("time_integrator must be one of %s" % (self._integrators,))
         if self.precision == "fp32k" and self.time_integrator != "SSP33":
@@ -199,8 +218,8 @@ class SharpClawSolver(Solver):
         Courant number goes into self.cfl at once; otherwise it is max-reduced
         into the given device buffer, which the caller reads later."""
         lo, hi = self._bc_codes()
-        if BC.custom in lo or BC.custom in hi:
-            raise NotImplementedError("custom boundary conditions are not supported by SharpClaw on the device")
+        if self.dq_src is not None or BC.custom in lo or BC.custom in hi:
+            P, lo, hi = self._host_stage_input(P, kw, lo, hi)
         dx, dy = self._grid.delta
         now = cfl is None
         if now:
@@ -213,6 +232,34 @@ class SharpClawSolver(Solver):
                        qaux=qaux, qaux_bc_lower=alo, qaux_bc_upper=ahi, **kw)
         if now:
             self.cfl.update_global_max(self._read_back(cfl)[0])
+
+    def _host_stage_input(self, P, kw, lo, hi):
+        """The stage input on the host, for what runs there: the user's custom BC
+        functions fill its ghosts (solver.py:351-437, on the stage's state) and
+        dq_src adds its increment (sharpclaw/solver.py:537-540).  Returns the
+        device input (with stored ghosts under custom BCs) and its BC codes; kw
+        gets the source and loses the in-kernel stage-input form."""
+        from .classic import _fill_bcs_host
+        from .device import q_from_device, q_to_device
+        y_mode, Q, s = kw.pop("y_mode", K.Y_ARRAY), kw.pop("Q", None), kw.pop("s", 0.0)
+        y = q_from_device(P)
+        if y_mode == K.Y_AXPY:
+            y = y + s * q_from_device(Q)
+        elif y_mode == K.Y_AXPY6:
+            y = y + (s * q_from_device(Q)) / 6.
+        st = _StageState(self._state, y)
+        if self.dq_src is not None:
+            S = np.asarray(self.dq_src(self, st, self.dt))
+            kw["src"] = q_to_device(S, self._dtype_code)
+        if BC.custom not in lo and BC.custom not in hi:
+            return q_to_device(y, self._dtype_code), lo, hi
+        ng = self.num_ghost
+        qbc = np.zeros((y.shape[0],) + tuple(n + 2 * ng for n in y.shape[1:]), order="F")
+        qbc[(slice(None),) + (slice(ng, -ng),) * 2] = y
+        if self._state.num_aux > 0:
+            self._host_auxbc(self._state)
+        _fill_bcs_host(self, st, qbc, lo, hi)
+        return q_to_device(qbc, self._dtype_code), [4, 4], [4, 4]
 
     def _step_buffer(self, meqn):
         """Device float64 [Courant number of the later stages, totals of the
@@ -236,6 +283,7 @@ class SharpClawSolver(Solver):
     def step(self, solution, take_one_step, tstart, tend):
         state = solution.states[0]
         self._grid = state.grid
+        self._state = state
         q, comp = state.device_q(self._dtype_code)
         # aux and its ghosts once per step (dq_hyperbolic applies the aux BCs of
         # the step's state at every stage, sharpclaw/solver.py:848-898)
@@ -480,6 +528,7 @@ class SharpClawSolver(Solver):
         """dt*L(q) on the interior as a host array (sharpclaw/solver.py:532-542)."""
         from .device import q_from_device
         self._grid = state.grid
+        self._state = state
         q, _ = state.device_q(self._dtype_code)
         self._qaux = self._aux_input(state) if self._reads_aux(state) else None
         out = torch.empty_like(q)

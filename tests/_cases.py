@@ -237,6 +237,28 @@ def shock_bubble(mx=160, my=40, gamma=1.4):
     return q, aux, d, shock_state, step_source
 
 
+def shock_bubble_dq_src(gamma=1.4):
+    """examples/euler_2d/shock_bubble_interaction.py dq_Euler_radial 156-178: the
+    SharpClaw form of the radial source, dt times its value.  Returns
+    dq_src(q, aux, dt)."""
+    g1 = gamma - 1.
+
+    def dq_src(q, aux, dt):
+        rad = aux[0]
+        rho = q[0]
+        u, v = q[1] / rho, q[2] / rho
+        press = g1 * (q[3] - 0.5 * rho * (u ** 2 + v ** 2))
+        dq = np.empty(q.shape)
+        dq[0] = -dt / rad * q[2]
+        dq[1] = -dt / rad * rho * u * v
+        dq[2] = -dt / rad * rho * v * v
+        dq[3] = -dt / rad * v * (q[3] + press)
+        dq[4] = 0
+        return dq
+
+    return dq_src
+
+
 def radial_dam_break():
     """examples/shallow_2d/radial_dam_break.py: qinit 22-30, setup 33-80."""
     (X, Y), d = grid([-2.5, -2.5], [2.5, 2.5], [150, 150])

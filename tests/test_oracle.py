@@ -448,6 +448,39 @@ def shock_bubble_run(mx=160, my=40):
                      step_source=step_source, user_bc_lower=user_bc), d
 
 
+def shock_bubble_sharp_run(mx=80, my=20, dt_initial=0.1):
+    """The example's solver_type='sharpclaw' set-up (setup 182-189): WENO5 with
+    SSP104 (the default), dq_src = dq_Euler_radial, the same custom BC."""
+    q, aux, d, shock, _ = C.shock_bubble(mx, my)
+    src = C.shock_bubble_dq_src()
+
+    def user_bc(run, dim, qbc):
+        for i in range(run.ng):
+            qbc[:, i, :] = shock[:, None]
+
+    return OracleRun(O.make_rp(O.EULER5, efix=1, gamma=1.4), q, d, [O.CUSTOM, O.WALL],
+                     [O.EXTRAP, O.EXTRAP], solver="sharp", cfl_desired=2.45, cfl_max=2.5,
+                     dt_initial=dt_initial, aux=aux, aux_bc_lower=[1, 1],
+                     aux_bc_upper=[1, 1], user_bc_lower=user_bc,
+                     dq_src=lambda run, qq, dt: src(qq, run.aux, dt)), d
+
+
+def test_sharpclaw_dq_src_constant_source():
+    """dq = dq_hyperbolic + dq_src (sharpclaw/solver.py:537-540), pinned where
+    the answer is known: on a uniform state the hyperbolic part vanishes, and a
+    constant source rate S added by dq_src must advance q by dt*S per step
+    under each integrator (SSP104, SSP33, Euler), to round-off."""
+    q = np.ones((3, 12, 10), order="F")
+    S = np.array([0.3, -0.2, 0.1])[:, None, None]
+    for ti in ("SSP104", "SSP33", "Euler"):
+        run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, [0.1, 0.1], [2, 2], [2, 2],
+                        solver="sharp", time_integrator=ti, dt_variable=False, dt_initial=0.01,
+                        cfl_max=2.5, cfl_desired=2.4, dq_src=lambda run, qq, dt: dt * S + 0 * qq)
+        run.run(0.05, 1)
+        assert run.numsteps == 5
+        np.testing.assert_allclose(run.q, q + 0.05 * S, rtol=0, atol=1e-14)
+
+
 def test_fixture_shock_bubble_interaction(golden_dir):
     """examples/euler_2d/test_euler2d.py:36-38 (160x40, tfinal 0.2, L1 of density
     < 1e-6): euler_5wave_2D restated, with the example's custom BC and source."""
