@@ -91,9 +91,11 @@ class SharpClawSolver(Solver):
             raise NotImplementedError("char_decomp must be 0 (component) or 2 (characteristic)")
         if self.tfluct_solver or self.fwave:
             raise NotImplementedError("tfluct solvers and f-waves are not supported")
-        lo, hi = self._bc_codes()
-        if self.indicator is not None and (self.dq_src is not None or BC.custom in lo
-                                           or BC.custom in hi):
+        if self.call_before_step_each_stage and self.before_step is not None:
+            raise NotImplementedError("call_before_step_each_stage is not supported: "
+                                      "before_step runs once per step")
+        custom = BC.custom in list(self.bc_lower) + list(self.bc_upper)
+        if self.indicator is not None and (self.dq_src is not None or custom):
             raise NotImplementedError("the indicator runs without dq_src and custom BCs")
         if self.time_integrator not in self._integrators:
             raise NotImplementedError("time_integrator must be one of %s" % (self._integrators,))

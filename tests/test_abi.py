@@ -27,3 +27,39 @@ def test_missing_library_raises(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "libclawhip.so"))
     with pytest.raises(L.ClawhipError):
         L.lib()
+
+
+def _small_solution(pyclaw, meqn, num_aux=0):
+    x = pyclaw.Dimension(0.0, 1.0, 8, name="x")
+    y = pyclaw.Dimension(0.0, 1.0, 8, name="y")
+    domain = pyclaw.Domain([x, y])
+    state = pyclaw.State(domain, meqn, num_aux)
+    state.q[...] = 1.0
+    return pyclaw.Solution(state, domain)
+
+
+def test_unsupported_options_raise_at_setup():
+    """Options the device solvers do not implement fail loudly at setup instead of
+    being ignored (no GPU needed: setup raises before any device work)."""
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    s = pyclaw.SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
+    s.call_before_step_each_stage = True
+    s.before_step = lambda solver, state: None
+    with pytest.raises(NotImplementedError):
+        s.setup(_small_solution(pyclaw, 3))
+    s = pyclaw.SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
+    s.time_integrator = "LMM"
+    s.alpha, s.beta = [0.0, 1.0], [0.0, 1.0]
+    s.cfl_max = 0.5
+    with pytest.raises(AssertionError):   # LMM needs a fixed dt (sharpclaw/solver.py:222-224)
+        s.setup(_small_solution(pyclaw, 3))
+    s = pyclaw.SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
+    s.time_integrator = "RK"
+    s.a, s.b = [[0.0]], [1.0]
+    with pytest.raises(KeyError):         # RK has no default CFL (sharpclaw/solver.py:246-247)
+        s.setup(_small_solution(pyclaw, 3))
+    c = pyclaw.ClawSolver2D(riemann.euler_4wave_2D)
+    c.fwave = True
+    with pytest.raises(NotImplementedError):
+        c.setup(_small_solution(pyclaw, 4))
