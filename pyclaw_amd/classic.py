@@ -283,10 +283,13 @@ class ClawSolver2D(ClawSolver):
         key = (bufs[0][0].data_ptr(), bufs[1][0].data_ptr())
         ent = self._graphs.get(key)
         if ent is None:
+            from .device import no_gc
             g = torch.cuda.CUDAGraph()
             s = torch.cuda.Stream()
             s.wait_stream(torch.cuda.current_stream())
-            with torch.cuda.stream(s):
+            # no garbage collection while capturing: collecting an old graph
+            # would destroy it mid-capture, which HIP refuses (an abort)
+            with no_gc(), torch.cuda.stream(s):
                 with torch.cuda.graph(g, stream=s):
                     K.step2_unsplit_run(*args, n)
             torch.cuda.current_stream().wait_stream(s)

@@ -40,3 +40,22 @@ def stream_handle():
 def ctypes_void_p(x):
     import ctypes
     return ctypes.c_void_p(int(x))
+
+
+class no_gc(object):
+    """Context: collect garbage now, then keep the collector off until exit.  Used
+    around graph capture, where the collector could otherwise free an earlier
+    graph (its destructor calls HIP, which a capturing stream refuses)."""
+
+    def __enter__(self):
+        import gc
+        gc.collect()
+        self._was = gc.isenabled()
+        gc.disable()
+        return self
+
+    def __exit__(self, *exc):
+        import gc
+        if self._was:
+            gc.enable()
+        return False

@@ -119,10 +119,11 @@ class Ensemble(object):
             if use_graph and n == chunk:
                 g = self._graphs.get(n)
                 if g is None:
+                    from .device import no_gc
                     g = torch.cuda.CUDAGraph()
                     side = torch.cuda.Stream()
                     side.wait_stream(torch.cuda.current_stream())
-                    with torch.cuda.stream(side):
+                    with no_gc(), torch.cuda.stream(side):
                         with torch.cuda.graph(g, stream=side):
                             self._launch(n)
                     torch.cuda.current_stream().wait_stream(side)
