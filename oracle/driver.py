@@ -29,7 +29,8 @@ class OracleRun:
                  char_decomp=0, num_ghost=None, aux=None, aux_bc_lower=None,
                  aux_bc_upper=None, before_step=None, step_source=None, source_split=1,
                  index_capa=-1, auxbc_fn=None, lmm_steps=4, user_bc_lower=None,
-                 rk_a=None, rk_b=None, lmm_alpha=None, lmm_beta=None, dq_src=None):
+                 rk_a=None, rk_b=None, lmm_alpha=None, lmm_beta=None, dq_src=None,
+                 rk_c=None):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -81,7 +82,9 @@ class OracleRun:
             self.sspcoeff = (lmm_steps - 3.) / (lmm_steps - 1.)
         self.sspcoeff0 = None
         # the general methods: RK (Butcher a, b) and LMM (alpha, beta)
-        self.rk_a, self.rk_b = rk_a, rk_b
+        self.rk_a, self.rk_b, self.rk_c = rk_a, rk_b, rk_c
+        self.stage_t = 0.0
+        self.plant_stage_time = False
         self.lmm_alpha, self.lmm_beta = lmm_alpha, lmm_beta
         # number of saved states (sharpclaw/solver.py:596-610)
         self.nreg = len(lmm_alpha) if time_integrator == "LMM" else lmm_steps
@@ -248,8 +251,12 @@ class OracleRun:
         if self.step_source is not None:
             self.step_source(self, self.dt / 2.0 if self.source_split == 2 else self.dt)
 
-    # ---- SharpClaw dq (sharpclaw/solver.py:848-898), returns dt*L(q) interior
-    def _dq(self, q):
+    # ---- SharpClaw dq (sharpclaw/solver.py:848-898), returns dt*L(q) interior.
+    # t: the time of the state dq acts on (the register's t,
+    # sharpclaw/solver.py:313-338, 409-427); callbacks read it as run.stage_t.
+    def _dq(self, q, t=None):
+        # plant_stage_time (tests only): every stage sees the step's start time
+        self.stage_t = self.t if (t is None or self.plant_stage_time) else t
         qbc = self._apply_bcs(q)
         dx, dy = self.delta
         reads_aux = self.aux is not None and self.rp.kind in (6, 7, 8)
@@ -275,12 +282,13 @@ class OracleRun:
         self.cfl = self.dt / self.dt_old * cfl
         self.dt_old = self.dt
         q, dt = self.q, self.dt
+        t0 = self.t
         if self.time_integrator == "Euler":
             self.q = q + dt * self.dq_dt
         elif self.time_integrator == "SSP22" or ("SSPLMM" in self.time_integrator
                                                    and step_index < self.lmm_steps):
             r = q + dt * self.dq_dt
-            self.q = 0.5 * (q + r + self._dq(r))
+            self.q = 0.5 * (q + r + self._dq(r, t0 + dt))
         elif self.time_integrator == "SSPLMMk2":
             om = sum(self.prev_dt[1:]) / dt
             r = (om - 1.) / om
@@ -299,8 +307,8 @@ class OracleRun:
                       beta0 * dt * self.prev_dq_dt[-self.lmm_steps])
         elif self.time_integrator == "SSP33":
             r = q + dt * self.dq_dt
-            r = 0.75 * q + 0.25 * (r + self._dq(r))
-            self.q = 1. / 3. * q + 2. / 3. * (r + self._dq(r))
+            r = 0.75 * q + 0.25 * (r + self._dq(r, t0 + dt))
+            self.q = 1. / 3. * q + 2. / 3. * (r + self._dq(r, t0 + 0.5 * dt))
         elif self.time_integrator == "RK":
             a, b = self.rk_a, self.rk_b
             regs = [dt * self.dq_dt]
@@ -308,7 +316,7 @@ class OracleRun:
                 r = q.copy(order="F")
                 for j in range(i):
                     r = r + a[i][j] * regs[j]
-                regs.append(self._dq(r))
+                regs.append(self._dq(r, t0 + dt * self.rk_c[i]))
             for j in range(len(b)):
                 q = q + b[j] * regs[j]
             self.q = q
@@ -320,13 +328,17 @@ class OracleRun:
             self.q = qn
         elif self.time_integrator in ("SSP104", "LMM"):
             s1 = q + dt * self.dq_dt / 6.
+            ts = t0 + dt / 6.
             for _ in range(4):
-                s1 = s1 + self._dq(s1) / 6.
+                s1 = s1 + self._dq(s1, ts) / 6.
+                ts = ts + dt / 6.
             q = q / 25. + 0.36 * s1
             s1 = 15. * q - 5. * s1
+            ts = t0 + dt / 3.
             for _ in range(4):
-                s1 = s1 + self._dq(s1) / 6.
-            self.q = q + (0.6 * s1 + 0.1 * self._dq(s1))
+                s1 = s1 + self._dq(s1, ts) / 6.
+                ts = ts + dt / 6.
+            self.q = q + (0.6 * s1 + 0.1 * self._dq(s1, ts))
         else:
             raise ValueError(self.time_integrator)
         self.q = np.asfortranarray(self.q)

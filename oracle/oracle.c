@@ -110,6 +110,18 @@ static roe2 roe_euler2(double gamma1, int mu, int mv, const double *qL, const do
 
 static int g_plant;   /* planted mistakes, listed at orc_set_plant */
 
+/* Running maximum of Courant-number terms (flux2.f90:111-119, flux3.f90:211-218,
+ * flux1.f90:136-143, and the slice maxima of step2/step2ds/step3ds/flux2).  A NaN
+ * term -- a wave speed of a state that overflowed -- is left out, whatever its
+ * position in the loop.  The reference leaves MAX with a NaN argument to the
+ * Fortran processor (amdflang's MAX restarts the running maximum there); the
+ * device kernels apply this same rule, so both reject such an attempt with the
+ * same Courant number. */
+static inline double cmax(double acc, double v) {
+  if (g_plant == 6) return acc > v ? acc : v;   /* planted: a NaN restarts the maximum */
+  return v > acc ? v : acc;
+}
+
 /* euler_4wave_2D rpn2: Roe solver, waves (u-a, shear u, entropy u, u+a), with the
  * Harten-Hyman entropy fix of rpn2_euler_5wave.f:137-234 (tracer wave dropped).
  * tr = 1: euler_5wave_2D, the same with the tracer q[4] and its wave (the jump
@@ -536,7 +548,10 @@ static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const d
  *      left-going fluctuation;
  *   5  euler_5wave_2D rpt2: the tracer's transverse wave at the normal velocity
  *      (4 and 5 are caught by the uniform-flow tracer test, where the tracer
- *      must move exactly as advection_2D moves its scalar).
+ *      must move exactly as advection_2D moves its scalar);
+ *   6  the Courant-number maximum with the Fortran build's NaN behaviour (a NaN
+ *      term restarts the running maximum) instead of leaving NaN terms out
+ *      (caught by the NaN-cell Courant-number test).
  * Which edge (aux2 or aux3) and which cell (i1) vc_advection_2D's rpt2 reads is
  * NOT pinned by any fixture: the annulus field has zero radial velocity and a
  * theta velocity that depends on r only, so those choices leave its solution
@@ -867,7 +882,7 @@ static double flux2(slice_ws *w, int ixy, int mx, const double *q1d, const int *
   double cfl1d = 0.0;
   for (int mw = 0; mw < mwaves; mw++)
     for (int i = 1; i <= mx + 1; i++)
-      cfl1d = dmax(dmax(cfl1d, DT1(i) * SV(mw, i)), -DT1(i - 1) * SV(mw, i));
+      cfl1d = cmax(cmax(cfl1d, DT1(i) * SV(mw, i)), -DT1(i - 1) * SV(mw, i));
   if (method[1] != 1) {
     if (limit) limiter(w, mx, mthlim);
     for (int i = 2 - ng; i <= mx + ng; i++) {
@@ -938,7 +953,7 @@ static double flux3_split(slice_ws *w, int ixyz, int mx, const double *q1d, doub
   double cfl1d = 0.0;
   for (int i = 1; i <= mx + 1; i++)
     for (int mw = 0; mw < mwaves; mw++)
-      cfl1d = dmax(dmax(cfl1d, dtdx * SV(mw, i)), -dtdx * SV(mw, i));
+      cfl1d = cmax(cmax(cfl1d, dtdx * SV(mw, i)), -dtdx * SV(mw, i));
   if (method[1] == 1) return cfl1d;
   if (limit) limiter(w, mx, mthlim);
   for (int i = 2 - ng; i <= mx + ng; i++) {
@@ -990,7 +1005,7 @@ double orc_step2(int meqn, int mwaves, int ng, int mx, int my, const double *qol
           w->aux3[ma + (long)maux * IX(i)] = AUX2(ma, i, j + 1);
         }
     double cfl1d = flux2(w, 1, mx, q1d, method, mthlim, use_fwave, p);
-    cfl = dmax(cfl, cfl1d);
+    cfl = cmax(cfl, cfl1d);
     for (int m = 0; m < meqn; m++)
       for (int i = 1; i <= mx; i++) {
         if (capa == 0) {
@@ -1021,7 +1036,7 @@ double orc_step2(int meqn, int mwaves, int ng, int mx, int my, const double *qol
           w->aux3[ma + (long)maux * IX(j)] = AUX2(ma, i + 1, j);
         }
     double cfl1d = flux2(w, 2, my, q1d, method, mthlim, use_fwave, p);
-    cfl = dmax(cfl, cfl1d);
+    cfl = cmax(cfl, cfl1d);
     for (int m = 0; m < meqn; m++)
       for (int j = 1; j <= my; j++) {
         if (capa == 0) {
@@ -1069,7 +1084,7 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
         for (int i = 1 - ng; i <= mx + ng; i++)
           for (int ma = 0; ma < maux; ma++) w->aux2[ma + (long)maux * IX(i)] = AUX2(ma, i, j);
       double cfl1d = flux2(w, 1, mx, q1d, method, mthlim, use_fwave, p);
-      cfl = dmax(cfl, cfl1d);
+      cfl = cmax(cfl, cfl1d);
       for (int i = 1; i <= mx; i++)
         for (int m = 0; m < meqn; m++)
           Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, i)
@@ -1083,7 +1098,7 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
         for (int j = 1 - ng; j <= my + ng; j++)
           for (int ma = 0; ma < maux; ma++) w->aux2[ma + (long)maux * IX(j)] = AUX2(ma, i, j);
       double cfl1d = flux2(w, 2, my, q1d, method, mthlim, use_fwave, p);
-      cfl = dmax(cfl, cfl1d);
+      cfl = cmax(cfl, cfl1d);
       for (int j = 1; j <= my; j++)
         for (int m = 0; m < meqn; m++)
           Q2(qnew, m, i, j) = Q2(qnew, m, i, j) + V1(w->qadd, m, j)
@@ -1119,7 +1134,7 @@ double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz, const d
         for (int i = 1 - ng; i <= mx + ng; i++)
           for (int m = 0; m < meqn; m++) V1(q1d, m, i) = Q3(qold, m, i, j, k);
         double cfl1d = flux3_split(w, 1, mx, q1d, dtdx, method, mthlim, use_fwave, p);
-        cfl = dmax(cfl, cfl1d);
+        cfl = cmax(cfl, cfl1d);
         for (int i = 1; i <= mx; i++)
           for (int m = 0; m < meqn; m++)
             Q3(qnew, m, i, j, k) = Q3(qnew, m, i, j, k) + V1(w->qadd, m, i)
@@ -1131,7 +1146,7 @@ double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz, const d
         for (int j = 1 - ng; j <= my + ng; j++)
           for (int m = 0; m < meqn; m++) V1(q1d, m, j) = Q3(qold, m, i, j, k);
         double cfl1d = flux3_split(w, 2, my, q1d, dtdy, method, mthlim, use_fwave, p);
-        cfl = dmax(cfl, cfl1d);
+        cfl = cmax(cfl, cfl1d);
         for (int j = 1; j <= my; j++)
           for (int m = 0; m < meqn; m++)
             Q3(qnew, m, i, j, k) = Q3(qnew, m, i, j, k) + V1(w->qadd, m, j)
@@ -1143,7 +1158,7 @@ double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz, const d
         for (int k = 1 - ng; k <= mz + ng; k++)
           for (int m = 0; m < meqn; m++) V1(q1d, m, k) = Q3(qold, m, i, j, k);
         double cfl1d = flux3_split(w, 3, mz, q1d, dtdz, method, mthlim, use_fwave, p);
-        cfl = dmax(cfl, cfl1d);
+        cfl = cmax(cfl, cfl1d);
         for (int k = 1; k <= mz; k++)
           for (int m = 0; m < meqn; m++)
             Q3(qnew, m, i, j, k) = Q3(qnew, m, i, j, k) + V1(w->qadd, m, k)
@@ -1377,7 +1392,7 @@ double orc_sharp_flux1(int meqn, int mwaves, int ng, int n, int ixy, const doubl
       apdq_i[m + (long)meqn * c] = apdq[m];
     }
     for (int mw = 0; mw < mwaves; mw++)
-      cfl = dmax(dmax(cfl, dtdx * s[mw]), -dtdx * s[mw]);
+      cfl = cmax(cmax(cfl, dtdx * s[mw]), -dtdx * s[mw]);
   }
   /* the CFL loop in the reference is mw-outer; max is exact so the order is free */
   for (int c = 1; c <= n; c++) {
@@ -1420,7 +1435,7 @@ double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my, const doubl
       for (int ma = 0; ma < maux; ma++) a1d[ma + (long)maux * (i + ng - 1)] = AUX2(ma, i, j);
     double c1 = orc_sharp_flux1(meqn, mwaves, ng, mx, 1, q1d, dq1d, dtdx, char_decomp, p, maux,
                                 a1d);
-    cfl = dmax(cfl, c1);
+    cfl = cmax(cfl, c1);
     for (int i = 1; i <= mx; i++)
       for (int m = 0; m < meqn; m++) Q2(dq, m, i, j) = Q2(dq, m, i, j) + B1(dq1d, m, i + ng);
   }
@@ -1431,7 +1446,7 @@ double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my, const doubl
       for (int ma = 0; ma < maux; ma++) a1d[ma + (long)maux * (j + ng - 1)] = AUX2(ma, i, j);
     double c1 = orc_sharp_flux1(meqn, mwaves, ng, my, 2, q1d, dq1d, dtdy, char_decomp, p, maux,
                                 a1d);
-    cfl = dmax(cfl, c1);
+    cfl = cmax(cfl, c1);
     for (int j = 1; j <= my; j++)
       for (int m = 0; m < meqn; m++) Q2(dq, m, i, j) = Q2(dq, m, i, j) + B1(dq1d, m, j + ng);
   }

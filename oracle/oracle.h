@@ -71,7 +71,8 @@ void orc_rpt_aux(const orc_rp *p, int ixy, int imp, const double *qL, const doub
                  const double *asdq, double *bmasdq, double *bpasdq);
 /* Test hook: plant mistake k in a restated transverse solver (1: vc_advection
  * normal velocity; 2: vc_acoustics cells below/above swapped; 3: vc_acoustics
- * cell i for both imp); 0 removes it. */
+ * cell i for both imp; 4, 5: euler_5wave_2D tracer; 6: a NaN Courant-number term
+ * restarts the running maximum); 0 removes it. */
 void orc_set_plant(int k);
 int orc_rp_meqn(int kind);
 int orc_rp_mwaves(int kind);

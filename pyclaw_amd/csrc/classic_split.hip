@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_xsweep(const SweepArgs<T> a) {
     T smax = T(0);
     if (lane >= 1 && c >= 1 && c <= mx + 1) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(s[w]));
+      for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(s[w]));
     }
     cfl = (double)a.dtdx * (double)smax;
     T F[ME];
@@ -238,11 +238,11 @@ __global__ void __launch_bounds__(256) k_march(const SweepArgs<T> a) {
     rpn_call<RS, IXY>(a.rp, Qm, Q0, Am, A0, Wp, sp, amp, app, roep);
     if (j0 - 1 >= 1) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(sp[w]));
+      for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(sp[w]));
     }
     rpn_call<RS, IXY>(a.rp, Q0, Q1, A0, A1, Wt, st, amt, apt, roet);
 #pragma unroll
-    for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(st[w]));
+    for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(st[w]));
     wave_dots<ME, MW>(Wp, Wt, dotl);
   }
   T qold_prev[ME], qadd_p[ME], F_p[ME];
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) k_march(const SweepArgs<T> a) {
     rpn_call<RS, IXY>(a.rp, Q0, Q1, A0, A1, Wn, sn, amn, apn, roen);
     if (t + 1 <= m_ax + 1) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax = dmax(smax, fabs(sn[w]));
+      for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(sn[w]));
     }
     T F[ME];
 #pragma unroll

@@ -130,9 +130,19 @@ __device__ __forceinline__ void from_right(const T (&v)[N], T (&o)[N]) {
   for (int m = 0; m < N; m++) o[m] = from_right(v[m]);
 }
 
-// Wave-wide max of a non-negative double, then one atomic per wave.  Ordering of
-// non-negative IEEE doubles equals the ordering of their bit patterns.
+// Running maximum of Courant-number terms (wave speeds, or dt/dx times them).  A
+// NaN term -- a wave speed of a state that overflowed -- is left out, whatever
+// the lane or the order: v > acc is false for a NaN v.  The reference leaves MAX
+// with a NaN argument to the Fortran processor; the oracle applies this rule.
+template <typename T>
+__device__ __forceinline__ T cfl_term(T acc, T v) { return v > acc ? v : acc; }
+
+// Wave-wide max of a lane's Courant number, then one atomic per wave.  Values
+// that are not >= 0 (a NaN) count as 0, so the result does not depend on which
+// lane held them.  Ordering of non-negative IEEE doubles equals the ordering of
+// their bit patterns.
 __device__ __forceinline__ void wave_atomic_max(double v, unsigned long long* dst) {
+  if (!(v >= 0.0)) v = 0.0;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(v, off, 64);

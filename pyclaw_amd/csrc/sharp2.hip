@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     rpn_call<RS, 1>(rp, qrL, qlR, AL, Ar, W, sp, am, ap, roe);
     if (xint_lane && counted) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
+      for (int w = 0; w < MW; w++) smax_x = cfl_term(smax_x, (T)fabs(sp[w]));
     }
   };
   // y-interface t (rows t-1 | t, aux Alo | Ahi) of this lane's column from the
@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     rpn_call<RS, 2>(rp, qrL, qlR, Alo, Ahi, W, sp, am, ap, roe);
     if (ycfl_lane && t <= my + 1 && counted) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
+      for (int w = 0; w < MW; w++) smax_y = cfl_term(smax_y, (T)fabs(sp[w]));
     }
   };
   // x contribution of this lane's cell in row r (a whole x-slice pass)
@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(256) k_sharp_patch(const SharpArgs<T> a, const
         RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
         if (weno) {
 #pragma unroll
-          for (int w = 0; w < MW; w++) smax_x = dmax(smax_x, (T)fabs(sp[w]));
+          for (int w = 0; w < MW; w++) smax_x = cfl_term(smax_x, (T)fabs(sp[w]));
         }
 #pragma unroll
         for (int m = 0; m < ME; m++) {
@@ -641,7 +641,7 @@ __global__ void __launch_bounds__(256) k_sharp_patch(const SharpArgs<T> a, const
         RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
         if (weno) {
 #pragma unroll
-          for (int w = 0; w < MW; w++) smax_y = dmax(smax_y, (T)fabs(sp[w]));
+          for (int w = 0; w < MW; w++) smax_y = cfl_term(smax_y, (T)fabs(sp[w]));
         }
 #pragma unroll
         for (int m = 0; m < ME; m++) {

@@ -112,8 +112,8 @@ __device__ __forceinline__ void xpass(const UnsplitArgs<T>& a, const T (&Qc)[RS:
   if (cfl_ok) {
 #pragma unroll
     for (int w = 0; w < MW; w++) {
-      if constexpr (CAPA) smax = dmax(dmax(smax, dtdx_i * s[w]), -(dtdx_l * s[w]));
-      else smax = dmax(smax, fabs(s[w]));
+      if constexpr (CAPA) smax = cfl_term(cfl_term(smax, dtdx_i * s[w]), -(dtdx_l * s[w]));
+      else smax = cfl_term(smax, fabs(s[w]));
     }
   }
   T F[ME], cq[ME];

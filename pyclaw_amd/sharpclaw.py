@@ -19,11 +19,14 @@ from .solver import BC, Solver
 class _StageState(object):
     """The state one operator evaluation sees (the reference's register States,
     sharpclaw/solver.py:306-430): the solver's state with q replaced by the
-    stage's (a host array), everything else read from the state."""
+    stage's (a host array) and t by the stage's time (None: the state's own),
+    everything else read from the state."""
 
-    def __init__(self, state, q):
+    def __init__(self, state, q, t=None):
         self._state = state
         self.q = q
+        if t is not None:
+            self.t = t
 
     def __getattr__(self, name):
         return getattr(self._state, name)
@@ -64,6 +67,7 @@ class SharpClawSolver(Solver):
         self._flags = None
         self._qaux = None     # (device aux, lower codes, upper codes) for the current step
         self._state = None    # the state being stepped
+        self._t0 = 0.0        # the start time of the step being taken
         # general Runge-Kutta coefficients (time_integrator 'RK')
         self.a = None
         self.b = None
@@ -136,6 +140,9 @@ class SharpClawSolver(Solver):
                 self.cfl_desired = 0.9 * self.cfl_max
         except KeyError:
             raise KeyError("Maximum CFL number is not provided.")
+        if self.time_integrator == "RK" and self.c is None:
+            # the stage times are t + dt*c[i] (sharpclaw/solver.py:338)
+            raise ValueError("the RK integrator needs solver.c (its stage times)")
         self._registers = None
         self.prev_dq_dt_values, self.prev_dt_values, self.prev_dtFE_values = [], [], []
         self._set_mthlim()
@@ -215,13 +222,15 @@ class SharpClawSolver(Solver):
             self._bufs[name] = t
         return t
 
-    def _stage(self, state_q, out, P, cfl=None, **kw):
+    def _stage(self, state_q, out, P, cfl=None, stage_t=None, **kw):
         """One operator evaluation (dq) fused with a stage.  cfl=None: the stage's
         Courant number goes into self.cfl at once; otherwise it is max-reduced
-        into the given device buffer, which the caller reads later."""
+        into the given device buffer, which the caller reads later.  stage_t:
+        the time of the stage's state, which host callbacks see (None: the
+        step's start time)."""
         lo, hi = self._bc_codes()
         if self.dq_src is not None or BC.custom in lo or BC.custom in hi:
-            P, lo, hi = self._host_stage_input(P, kw, lo, hi)
+            P, lo, hi = self._host_stage_input(P, kw, lo, hi, stage_t)
         dx, dy = self._grid.delta
         now = cfl is None
         if now:
@@ -235,21 +244,23 @@ class SharpClawSolver(Solver):
         if now:
             self.cfl.update_global_max(self._read_back(cfl)[0])
 
-    def _host_stage_input(self, P, kw, lo, hi):
+    def _host_stage_input(self, P, kw, lo, hi, stage_t=None):
         """The stage input on the host, for what runs there: the user's custom BC
-        functions fill its ghosts (solver.py:351-437, on the stage's state) and
-        dq_src adds its increment (sharpclaw/solver.py:537-540).  Returns the
-        device input (with stored ghosts under custom BCs) and its BC codes; kw
-        gets the source and loses the in-kernel stage-input form."""
+        functions fill its ghosts (solver.py:351-437, on the stage's state, whose
+        time is stage_t) and dq_src adds its increment
+        (sharpclaw/solver.py:537-540).  Returns the device input (with stored
+        ghosts under custom BCs) and its BC codes; kw gets the source and loses
+        the in-kernel stage-input form (y_mode, Q).  kw keeps s: the FP32
+        stage-2 combination (OUT_F32_S2) reads it as the step's dt."""
         from .classic import _fill_bcs_host
         from .device import q_from_device, q_to_device
-        y_mode, Q, s = kw.pop("y_mode", K.Y_ARRAY), kw.pop("Q", None), kw.pop("s", 0.0)
+        y_mode, Q, s = kw.pop("y_mode", K.Y_ARRAY), kw.pop("Q", None), kw.get("s", 0.0)
         y = q_from_device(P)
         if y_mode == K.Y_AXPY:
             y = y + s * q_from_device(Q)
         elif y_mode == K.Y_AXPY6:
             y = y + (s * q_from_device(Q)) / 6.
-        st = _StageState(self._state, y)
+        st = _StageState(self._state, y, stage_t)
         if self.dq_src is not None:
             S = np.asarray(self.dq_src(self, st, self.dt))
             kw["src"] = q_to_device(S, self._dtype_code)
@@ -315,6 +326,7 @@ class SharpClawSolver(Solver):
         # the later stages max-reduce their Courant numbers on the device; one
         # read at the end of the step (max is order-free, so self.cfl ends as the
         # stage-by-stage updates of sharpclaw/solver.py:283-430 leave it)
+        self._t0 = solution.t
         if "LMM" in self.time_integrator:
             self._lmm_step(q, comp, out_q, step_index, cs)
         else:
@@ -390,8 +402,8 @@ class SharpClawSolver(Solver):
             return
         num_steps = self.lmm_steps
         if step_index < num_steps:
-            self._stage(q, out_q[0], q, cfl=cs, Q=self.dq_dt, s=dt, y_mode=K.Y_AXPY,
-                        out_mode=K.OUT_SSP22, A=q)
+            self._stage(q, out_q[0], q, cfl=cs, stage_t=self._t0 + dt, Q=self.dq_dt, s=dt,
+                        y_mode=K.Y_AXPY, out_mode=K.OUT_SSP22, A=q)
             return
         omega_k_minus_1 = sum(self.prev_dt_values[1:]) / dt
         if self.time_integrator == "SSPLMMk2":
@@ -412,13 +424,16 @@ class SharpClawSolver(Solver):
 
     def _rk_stages(self, q, comp, dq_dt, out_q, dt, cs, method=None):
         """The Runge-Kutta stages after the first operator evaluation
-        (sharpclaw/solver.py:306-348, 388-430), into out_q."""
+        (sharpclaw/solver.py:306-348, 388-430), into out_q.  Each stage's state
+        carries the reference's stage time (the registers' t), computed from the
+        step's start time self._t0 with the reference's arithmetic."""
         ti = self.time_integrator if method is None else method
+        t0 = self._t0
         if ti == "Euler":
             K.lmm_update(0, q, dq_dt, (0.0, dt), out_q[0])
         elif ti == "SSP22":
-            self._stage(q, out_q[0], q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
-                        out_mode=K.OUT_SSP22, A=q)
+            self._stage(q, out_q[0], q, cfl=cs, stage_t=t0 + dt, Q=dq_dt, s=dt,
+                        y_mode=K.Y_AXPY, out_mode=K.OUT_SSP22, A=q)
         elif ti == "RK":
             # registers[0] = dt*dq_dt; stage i: q + sum_j a[i,j]*registers[j], term
             # by term, then registers[i] = dq(stage i); q += b[j]*registers[j]
@@ -430,35 +445,46 @@ class SharpClawSolver(Solver):
                 y.copy_(q)
                 for j in range(i):
                     K.lmm_update(0, y, regs[j], (0.0, float(a[i][j])), y)
-                self._stage(q, regs[i], y, cfl=cs, out_mode=K.OUT_D)
+                self._stage(q, regs[i], y, cfl=cs, stage_t=t0 + dt * self.c[i],
+                            out_mode=K.OUT_D)
             out_q[0].copy_(q)
             for j in range(len(b)):
                 K.lmm_update(0, out_q[0], regs[j], (0.0, float(b[j])), out_q[0])
         elif ti == "SSP33":
             r2 = self._buf("r2", q)
+            # register times t + dt, then t + 0.5*dt (sharpclaw/solver.py:313-319)
+            t2, t3 = t0 + dt, t0 + 0.5 * dt
             if self.precision == "fp64":
-                self._stage(q, r2, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
+                self._stage(q, r2, q, cfl=cs, stage_t=t2, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
                             out_mode=K.OUT_SSP33_2, A=q)
-                self._stage(q, out_q[0], r2, cfl=cs, out_mode=K.OUT_SSP33_3, A=q)
+                self._stage(q, out_q[0], r2, cfl=cs, stage_t=t3, out_mode=K.OUT_SSP33_3, A=q)
             else:
                 aux = self._buf("aux", q)
-                self._stage(q, r2, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
+                self._stage(q, r2, q, cfl=cs, stage_t=t2, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
                             out_mode=K.OUT_F32_S2, A=q, B=dq_dt, aux_out=aux)
-                self._stage(q, out_q[0], r2, cfl=cs, out_mode=K.OUT_F32_S3, A=q, aux_in=aux,
-                            comp_in=comp, comp_out=out_q[1])
+                self._stage(q, out_q[0], r2, cfl=cs, stage_t=t3, out_mode=K.OUT_F32_S3, A=q,
+                            aux_in=aux, comp_in=comp, comp_out=out_q[1])
         else:   # SSP104
+            # s1.t: t + dt/6., then += dt/6. per stage; t + dt/3. after the middle
+            # update (sharpclaw/solver.py:409-427)
             s1a, s1b, qm = self._buf("s1a", q), self._buf("s1b", q), self._buf("qm", q)
-            self._stage(q, s1a, q, cfl=cs, Q=dq_dt, s=dt, y_mode=K.Y_AXPY6, out_mode=K.OUT_ADD6)
+            ts = t0 + dt / 6.
+            self._stage(q, s1a, q, cfl=cs, stage_t=ts, Q=dq_dt, s=dt, y_mode=K.Y_AXPY6,
+                        out_mode=K.OUT_ADD6)
+            ts = ts + dt / 6.
             cur, nxt = s1a, s1b
             for _ in range(3):
-                self._stage(q, nxt, cur, cfl=cs, out_mode=K.OUT_ADD6)
+                self._stage(q, nxt, cur, cfl=cs, stage_t=ts, out_mode=K.OUT_ADD6)
+                ts = ts + dt / 6.
                 cur, nxt = nxt, cur
             qm.copy_(q)
             K.ssp104_mid(qm, cur)
+            ts = t0 + dt / 3.
             for _ in range(4):
-                self._stage(q, nxt, cur, cfl=cs, out_mode=K.OUT_ADD6)
+                self._stage(q, nxt, cur, cfl=cs, stage_t=ts, out_mode=K.OUT_ADD6)
+                ts = ts + dt / 6.
                 cur, nxt = nxt, cur
-            self._stage(q, out_q[0], cur, cfl=cs, out_mode=K.OUT_SSP104_FIN, A=qm)
+            self._stage(q, out_q[0], cur, cfl=cs, stage_t=ts, out_mode=K.OUT_SSP104_FIN, A=qm)
 
     def _full_weno_step(self, q, comp, out_q, cs):
         """The same step (same dt) with full WENO5 at every interface: the first

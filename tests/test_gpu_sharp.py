@@ -235,8 +235,8 @@ def test_general_integrators_match_oracle(ti):
     if ti == "RK":
         opts = dict(time_integrator="RK", a=RK3_A, b=RK3_B, c=[0., 0.5, 1.], cfl_desired=0.45,
                     cfl_max=0.5, dt_initial=0.0005)
-        kw = dict(time_integrator="RK", rk_a=RK3_A, rk_b=RK3_B, cfl_desired=0.45, cfl_max=0.5,
-                  dt_initial=0.0005)
+        kw = dict(time_integrator="RK", rk_a=RK3_A, rk_b=RK3_B, rk_c=[0., 0.5, 1.],
+                  cfl_desired=0.45, cfl_max=0.5, dt_initial=0.0005)
     else:
         al, be = [0., 0., 1.], [5. / 12., -16. / 12., 23. / 12.]
         opts = dict(time_integrator="LMM", alpha=al, beta=be, lmm_steps=3, dt_variable=False,
@@ -342,3 +342,70 @@ def test_acoustics_2d_variable_sharpclaw_fixture(golden_dir, ti, cd, cm):
     assert np.array_equal(final, run.q), np.abs(final - run.q).max()
     p = np.load(f"{golden_dir}/acoustics_2d_variable_pressure_sharpclaw.npz")["pressure"]
     assert np.abs(p.reshape(-1) - final[0].reshape(-1)).max() < 1e-1
+
+
+def _timed_callback_runs(ti, precision="fp64"):
+    """Shallow water with a time-dependent inflow at x lower (custom BC) and a
+    time-dependent source (dq_src): through the API and in the oracle, where
+    both callbacks read the time of the stage's state."""
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    rng = np.random.default_rng(91)
+    q = seeded_state(rng, O.SHALLOW, (30, 24))
+    S = np.array([0.2, -0.1, 0.05])[:, None, None]
+
+    def inflow(t):
+        return np.array([1.2 + 0.5 * t, 0.3 - t, 0.1 * t])
+
+    def api_bc(state, dim, t, qbc, auxbc, num_ghost):
+        for i in range(num_ghost):
+            qbc[:, i, :] = inflow(t)[:, None]
+
+    def api_src(solver, state, dt):
+        return dt * (state.t * S - 0.1 * state.q)
+
+    def orc_bc(run, dim, qbc):
+        for i in range(run.ng):
+            qbc[:, i, :] = inflow(run.stage_t)[:, None]
+
+    def orc_src(run, qq, dt):
+        return dt * (run.stage_t * S - 0.1 * qq)
+
+    kw = dict(time_integrator=ti, cfl_desired=0.45 if ti != "SSP104" else 2.4,
+              cfl_max=0.5 if ti != "SSP104" else 2.5)
+    claw = _run_sharp(riemann.shallow_roe_with_efix_2D, q, ([0, 0], [1, 1]),
+                      [pyclaw.BC.custom, 1], [1, 1], 0.3, 2, {"grav": 1.0},
+                      user_bc_lower=api_bc, dq_src=api_src, dt_initial=0.005,
+                      precision=precision, **kw)
+    run = OracleRun(_orp(O.SHALLOW), q, [1.0 / 30, 1.0 / 24], [O.CUSTOM, 1], [1, 1],
+                    solver="sharp", dt_initial=0.005, user_bc_lower=orc_bc, dq_src=orc_src,
+                    **kw)
+    run.run(0.3, 2)
+    return claw, run
+
+
+@pytest.mark.parametrize("ti", ["SSP33", "SSP104", "SSP22"])
+def test_time_dependent_bc_and_source_match_oracle(ti):
+    """Custom BC functions and dq_src see each stage's own time (the registers'
+    t, sharpclaw/solver.py:313-338, 391, 409-427): whole FP64 runs with a
+    time-dependent inflow and source equal the oracle's bit for bit (the
+    oracle's stage times are pinned by quadrature in tests/test_oracle.py)."""
+    claw, run = _timed_callback_runs(ti)
+    got = claw.frames[-1].state.q
+    assert claw.solver.status["numsteps"] == run.numsteps
+    assert np.array_equal(got, run.q), np.abs(got - run.q).max()
+
+
+def test_fp32_kahan_with_host_callbacks_close_to_fp64():
+    """FP32 + Kahan SSP33 whose stage inputs are formed on the host (custom BC and
+    dq_src): its second stage still takes the step's dt*dq_dt.  After ~27 steps
+    the FP32 run is within 5e-5 relative L1 and 1e-4 relative max of the FP64
+    run (which equals the oracle's); dropping dt*dq_dt from the second stage
+    moves the state by ~0.25*dt*|dq/dt|, above 1e-3 relative."""
+    a, run = _timed_callback_runs("SSP33")
+    b, _ = _timed_callback_runs("SSP33", precision="fp32k")
+    qa, qb = a.frames[-1].state.q, b.frames[-1].state.q
+    assert np.array_equal(qa, run.q)
+    assert abs(b.solution.t - a.solution.t) < 1e-6 * a.solution.t
+    assert np.abs(qa - qb).sum() / np.abs(qa).sum() < 5e-5
+    assert np.abs(qa - qb).max() / np.abs(qa).max() < 1e-4

@@ -481,6 +481,51 @@ def test_sharpclaw_dq_src_constant_source():
         np.testing.assert_allclose(run.q, q + 0.05 * S, rtol=0, atol=1e-14)
 
 
+# integrator -> (RK coefficients c for 'RK', source rate f(t), its integral F(t));
+# each rate is a polynomial the integrator's quadrature of y' = f(t) integrates
+# exactly when every stage sees its own time
+_STAGE_TIME_CASES = {
+    "SSP22": (None, lambda t: 2.0 * t, lambda t: t * t),                     # trapezoid
+    "SSP33": (None, lambda t: 3.0 * t * t, lambda t: t ** 3),                # Simpson
+    "SSP104": (None, lambda t: 4.0 * t ** 3, lambda t: t ** 4),              # order 4
+    "RK": ([0., 0.5, 1.], lambda t: 3.0 * t * t, lambda t: t ** 3),          # Kutta 3
+}
+
+
+def _stage_time_run(ti, plant=False):
+    c, f, F = _STAGE_TIME_CASES[ti]
+    q = np.ones((3, 12, 10), order="F")
+    S = np.array([0.3, -0.2, 0.1])[:, None, None]
+    kw = {}
+    if ti == "RK":
+        kw = dict(rk_a=[[0., 0., 0.], [0.5, 0., 0.], [-1., 2., 0.]],
+                  rk_b=[1. / 6., 2. / 3., 1. / 6.], rk_c=c)
+    run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, [0.1, 0.1], [2, 2], [2, 2],
+                    solver="sharp", time_integrator=ti, dt_variable=False, dt_initial=0.05,
+                    cfl_max=2.5, cfl_desired=2.4,
+                    dq_src=lambda run, qq, dt: dt * f(run.stage_t) * S + 0 * qq, **kw)
+    run.plant_stage_time = plant
+    run.run(0.5, 1)
+    assert run.numsteps == 10
+    return run.q, q + F(0.5) * S
+
+
+@pytest.mark.parametrize("ti", sorted(_STAGE_TIME_CASES))
+def test_sharpclaw_stage_times_pinned_by_quadrature(ti):
+    """The time each stage's state carries (the registers' t,
+    sharpclaw/solver.py:313-338, 391, 409-427), which dq_src and custom BC
+    functions see.  On a uniform state the hyperbolic part vanishes and the
+    step integrates y' = f(t) with the integrator's quadrature rule: trapezoid
+    (SSP22), Simpson (SSP33), a fourth-order rule (SSP104), Kutta's rule (RK
+    with c = 0, 1/2, 1).  Each is exact for the polynomial rate chosen, so
+    q(T) = q(0) + F(T)*S to round-off; with every stage planted at the step's
+    start time the result is a left Riemann sum and the check fails."""
+    got, exact = _stage_time_run(ti)
+    np.testing.assert_allclose(got, exact, rtol=0, atol=1e-14)
+    planted, _ = _stage_time_run(ti, plant=True)
+    assert np.abs(planted - exact).max() > 1e-4
+
+
 def test_fixture_shock_bubble_interaction(golden_dir):
     """examples/euler_2d/test_euler2d.py:36-38 (160x40, tfinal 0.2, L1 of density
     < 1e-6): euler_5wave_2D restated, with the example's custom BC and source."""
@@ -573,7 +618,7 @@ def test_general_rk_reduces_to_named_methods():
     assert np.array_equal(e.q, r.q)
     s = _sharp_run("SSP33", **fixed)
     r3 = _sharp_run("RK", rk_a=[[0., 0., 0.], [1., 0., 0.], [0.25, 0.25, 0.]],
-                    rk_b=[1. / 6., 1. / 6., 2. / 3.], **fixed)
+                    rk_b=[1. / 6., 1. / 6., 2. / 3.], rk_c=[0., 1., 0.5], **fixed)
     assert e.numsteps == r.numsteps == s.numsteps == r3.numsteps == 10
     assert np.abs(s.q - r3.q).max() < 1e-13 * np.abs(s.q).max()
     assert np.abs(s.q - e.q).max() > 1e-6   # the comparison can tell methods apart
@@ -889,3 +934,66 @@ def test_limiter_closed_forms_through_the_step():
         for c in range(3, 8):       # 0-based cells with full stencils
             expect = vals[c] - nu * W[c - 1] - nu * (F(c + 1) - F(c))
             assert abs(qnew[0, c, 2] - expect) < 1e-14, (lim, c)
+
+
+def nan_cell_state(mx=40, my=12, j=5, inan=30):
+    """2D Euler (gamma 1.4) at rest (rho 1, p 1) with a jet u = 4 along x in
+    columns 0..mx/2-1 of row j, and, unless inan is None, every field of cell
+    (inan, j) NaN (a state that overflowed).  The fastest waves are the jet's
+    x-waves, in the x-slice of row j before the NaN cell; the waves next to the
+    NaN cell are slow (the fluid at rest around it)."""
+    q = np.zeros((4, mx, my), order="F")
+    u = np.zeros((mx, my))
+    u[:mx // 2, j] = 4.0
+    q[0] = 1.0
+    q[1] = u
+    q[3] = 1.0 / 0.4 + 0.5 * u * u
+    if inan is not None:
+        q[:, inan, j] = np.nan
+    return q
+
+
+def nan_cell_cfls(q, ng):
+    """Courant numbers of the unsplit step, the step2ds x-sweep and the SharpClaw
+    operator (component and characteristic WENO5) on q, extrapolation BCs."""
+    rp = O.make_rp(O.EULER, efix=1, gamma=1.4)
+    mx, my = q.shape[1:]
+    dx, dy = 1.0 / mx, 1.0 / my
+    dt = 0.1 * dx
+    out = []
+    for kind in ("step2", "step2ds", "sharp0", "sharp2"):
+        g = 3 if kind.startswith("sharp") else 2
+        qbc = O.with_ghosts(q, g)
+        O.bc2(qbc, g, [1, 1], [1, 1])
+        if kind == "step2":
+            out.append(O.step2(rp, qbc, qbc.copy(order="F"), g, dx, dy, dt,
+                               O.method_array(transverse=2), [4] * 4))
+        elif kind == "step2ds":
+            out.append(O.step2(rp, qbc, qbc.copy(order="F"), g, dx, dy, dt,
+                               O.method_array(dimensional_split=True), [4] * 4, ids=1))
+        else:
+            out.append(O.sharp_flux2(rp, qbc, g, dx, dy, dt, int(kind[-1]))[1])
+    return out
+
+
+@pytest.mark.parametrize("plant", [0, 6])
+def test_nan_wave_speeds_left_out_of_the_courant_number(plant):
+    """A NaN wave speed (next to a cell that overflowed) is left out of the
+    Courant number (flux2.f90:111-119, flux1.f90:136-143, with the NaN rule
+    stated where the oracle's maximum is), whatever its place in the loop: with
+    a NaN cell in the slow part of the jet's row, the Courant number equals the
+    one without it (the fast jet waves come first in that row).  Planting the
+    Fortran build's behaviour, where a NaN restarts the running maximum
+    (plant 6), loses the jet's waves in that row and fails the check."""
+    O.lib().orc_set_plant(plant)
+    try:
+        with_nan = nan_cell_cfls(nan_cell_state(), 3)
+        clean = nan_cell_cfls(nan_cell_state(inan=None), 3)
+    finally:
+        O.lib().orc_set_plant(0)
+    assert all(np.isfinite(c) and c > 0 for c in with_nan)
+    same = [a == b for a, b in zip(with_nan, clean)]
+    if plant == 0:
+        assert all(same), (with_nan, clean)
+    else:
+        assert not any(same), (with_nan, clean)
