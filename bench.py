@@ -1,18 +1,22 @@
 """Benchmark: cell-updates/s of the PyClaw time-step hot path on one MI355X.
 
-Workload (the headline configuration): 2D Euler, gamma = 1.4, Roe solver with
-Harten entropy fix, classic unsplit wave propagation with transverse
-corrections (trans_cor) and the MC limiter, 1024 x 1024 cells on [0,1]^2,
-extrapolation boundaries, CFL 0.8 (cfl_max 1.0), 400 steps.  Initial data: a
-four-quadrant Riemann problem whose quadrant states are drawn from
-numpy.random.default_rng(2): rho ~ U[0.5,1.5], p ~ U[0.1,1.0], u, v ~ U[-0.5,0.5].
+Headline workload (the metric's configuration): 2D shallow water (g = 1) on
+the periodic unit square, 2048 x 2048 cells, SharpClaw characteristic WENO5 +
+SSP33, CFL 0.45 (cfl_max 0.5).  Initial data: h = 1 + 8 Gaussian humps
+(amplitude U[0.05, 0.3], width U[0.02, 0.1]) + 2 circular dam breaks (radius
+U[0.05, 0.15], jump U[0.1, 0.5]) drawn from numpy.random.default_rng(3),
+hu = hv = 0.  `value` is the FP32 + Kahan path with the learned troubled-
+interface indicator (shipped weights; full WENO5 on flagged interfaces, the
+linear weights elsewhere, full WENO5 judging every step's positivity); the
+plain FP64 full-WENO5 path (bitwise equal to the reference step) runs the same
+steps beside it, and their ratio and the error against it are reported.
 
-A step is one full ClawSolver2D time step (dt from the previous step's CFL,
-accept/reject) in the device-controlled loop (ClawSolver2D.run_steps: the
-step kernel applies the controller itself; 16 steps per captured graph),
-state resident in HBM.  `value` is the FP64
-path (bitwise equal to the reference step); the FP32 Kahan-compensated path runs
-the same steps beside it and its error against the FP64 run is reported.
+A step is one full SSP33 time step (dt from the previous step's CFL,
+accept/reject) on the state resident in HBM.  W untimed warm-up steps, then
+exactly K timed steps.  The `full_run` block runs both paths to t = 1.0 and
+reports the whole-run rate ratio, the error at t = 1.0 and the indicator's
+trajectory.  Legs beside it: the classic 2D Euler step (1024^2), acoustics
+(256^2), the 512-member ensemble and 3D Euler (384^3).
 
 Prints ONE JSON line.
 """
@@ -27,11 +31,20 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MX = MY = 1024
+MX = MY = 1024              # classic Euler leg
+SHARP_N = 2048              # the headline grid
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
-BYTES_PER_CELL_F64 = 64     # read q + write q, 4 fields x 8 B
+FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
+BYTES_PER_CELL_F64 = 64     # classic Euler: read q + write q, 4 fields x 8 B
 BYTES_PER_CELL_F32K = 64    # q + compensation, read + write, 4 fields x 4 B x 2
+# SSP33 shallow water, algorithmic HBM bytes per cell-step (3 fields): FP64 reads
+# q^n, q^1, q^n, q^2, q^n and writes q^1, q^2, q^n+1 (8 passes x 24 B); FP32 + Kahan
+# the same 8 passes x 12 B plus the compensation read and written (2 x 12 B).
+# One SSP33 step is 3 stage launches, so a launch carries a third of it.
+SHARP_BYTES_F64 = 192
+SHARP_BYTES_F32K = 120
+SHARP_STAGES = 3
 
 
 def quadrant_state(mx=MX, my=MY, seed=2, gamma=1.4):
@@ -105,12 +118,13 @@ class ChunkTimer(object):
 
 
 def run_gpu(precision, steps, warmup, use_graph=True):
-    """warmup steps (they include the graph capture), then `steps` timed
-    device-loop steps (graph replays of 16 steps)."""
+    """At least one full graph chunk of untimed warm-up steps (so the graph is
+    captured and instantiated before the timed region, whatever `warmup` is),
+    then `steps` timed device-loop steps (graph replays of 16 steps)."""
     import torch
     solver, solution = make_problem(precision)
     solver.use_graph = use_graph
-    solver.run_steps(solution, warmup)
+    solver.run_steps(solution, max(int(warmup), solver.graph_chunk))
     torch.cuda.synchronize()
     timer = ChunkTimer(solver)
     try:
@@ -127,13 +141,13 @@ def run_gpu(precision, steps, warmup, use_graph=True):
     return wall, kms, nk, host, solution
 
 
-def measured_counts():
-    """Per-launch counter figures of k_unsplit_ctl<Euler<double>> on this
-    workload, from the rocprofv3 runs summarised in profiles/: HBM bytes
+def measured_counts(name="k_unsplit_ctl_euler2d_1024"):
+    """Per-launch counter figures of a kernel on its workload, from the
+    rocprofv3 runs summarised in profiles/<name>.traffic.json: HBM bytes
     (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's reading for a
-    256 MiB copy) and executed FP64 operations (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64
-    x 64 lanes, FMA counted twice).  {} if absent."""
-    path = os.path.join(ROOT, "profiles", "k_unsplit_ctl_euler2d_1024.traffic.json")
+    256 MiB copy) and executed vector operations (SQ_INSTS_VALU_* x 64 lanes,
+    FMA counted twice).  {} if absent."""
+    path = os.path.join(ROOT, "profiles", name + ".traffic.json")
     try:
         with open(path) as f:
             return json.load(f)
@@ -141,7 +155,7 @@ def measured_counts():
         return {}
 
 
-def cpu_baseline(sample_steps=3):
+def classic_cpu_baseline(sample_steps=3):
     """The reference's own Fortran step2 (classic/step2.f90 + flux2.f90, built from
     the reference sources by oracle/ref.mk), single thread, on the same 1024^2
     state: `sample_steps` unsplit steps with the reference BC fill before each."""
@@ -328,60 +342,209 @@ def euler3d_leg(steps=10, n=384, seed=5):
     return out
 
 
-def sharpclaw_leg(steps=20, n=2048, seed=3):
-    """2D shallow water on the periodic unit square, 2048^2: h = 1 + 8 seeded
-    Gaussian humps + 2 seeded circular dam breaks, hu = hv = 0; SharpClaw
-    characteristic WENO5 + SSP33, CFL 0.45 (max 0.5).  FP64 full WENO5 against
-    FP32 + Kahan, with and without the learned troubled-interface indicator
-    (shipped weights; flags refreshed every second step).  Each leg: one
-    untimed step, then `steps` timed evolve_to_time steps from the host."""
+class StageTimer(object):
+    """HIP events around every SharpClaw stage launch (clawhip_sharp2_stage:
+    k_sharp_march, and k_sharp_patch on the indicator path) on the stream the
+    stage runs on; ms() = mean event time per stage launch."""
+
+    def __init__(self):
+        import torch
+        from pyclaw_amd import kernels
+        self.torch, self.kernels = torch, kernels
+        self.orig = kernels.sharp2_stage
+        self.pairs = []
+
+        def timed(*a, **kw):
+            st = torch.cuda.current_stream()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            self.orig(*a, **kw)
+            e1.record(st)
+            self.pairs.append((e0, e1))
+        kernels.sharp2_stage = timed
+
+    def close(self):
+        self.kernels.sharp2_stage = self.orig
+
+    def ms(self):
+        self.torch.cuda.synchronize()
+        n = len(self.pairs)
+        return sum(a.elapsed_time(b) for a, b in self.pairs) / max(n, 1), n
+
+
+def sharp_problem(prec, indicator, n=SHARP_N, seed=3):
+    from pyclaw_amd import indicator as I
+    solver = I.swe_solver(prec, indicator)
+    sol = I.swe_solution(seed, n, n)
+    solver.dt = 0.45 / n / 1.6
+    solver.setup(sol)
+    return solver, sol
+
+
+def sharp_steps(prec, use_indicator, steps, warmup, n=SHARP_N):
+    """`warmup` untimed steps, then exactly `steps` timed SSP33 steps (attempts)
+    of the 2048^2 shallow-water workload; HIP events around every stage launch
+    of the timed steps."""
     import torch
     from pyclaw_amd import indicator as I
-    res = {}
-    qs = {}
-    for name, prec, ind in (("fp64_weno", "fp64", None), ("fp32k_weno", "fp32k", None),
-                            ("fp32k_mlp", "fp32k", "shipped")):
-        indicator = I.TroubledInterfaceIndicator() if ind else None
-        solver = I.swe_solver(prec, indicator)
-        sol = I.swe_solution(seed, n, n)
-        solver.dt = 0.45 / n / 1.6
-        solver.setup(sol)
+    ind = I.TroubledInterfaceIndicator() if use_indicator else None
+    solver, sol = sharp_problem(prec, ind, n)
+    for _ in range(warmup):
         solver.evolve_to_time(sol)
-        torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    acc0 = solver.status["numsteps"]
+    timer = StageTimer()
+    try:
         t0 = time.perf_counter()
         for _ in range(steps):
             solver.evolve_to_time(sol)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        res[name] = {"value": n * n * steps / wall, "ms_per_step": 1e3 * wall / steps,
-                     "t_end": sol.t}
-        if indicator is not None:
-            res[name]["troubled_fraction"] = indicator.troubled_fraction()
-            res[name]["flag_refresh_every"] = indicator.refresh_interval(solver.cfl_max)
-            res[name]["judge"] = dict(indicator.judge_stats)
-        qs[name] = sol.state.q
-    ref = qs["fp64_weno"]
-    for name in ("fp32k_weno", "fp32k_mlp"):
-        res[name]["rel_l1_vs_fp64"] = float(np.abs(qs[name] - ref).sum() / np.abs(ref).sum())
-        res[name]["rel_max_vs_fp64"] = float(np.abs(qs[name] - ref).max() / np.abs(ref).max())
+    finally:
+        timer.close()
+    stage_ms, nstage = timer.ms()
+    per_cell = SHARP_BYTES_F64 if prec == "fp64" else SHARP_BYTES_F32K
+    achieved = per_cell / SHARP_STAGES * n * n / (stage_ms * 1e-3) / 1e9
+    out = {"value": n * n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+           "stage_ms": stage_ms, "stage_launches_timed": nstage, "t_end": sol.t,
+           "accepted": solver.status["numsteps"] - acc0,
+           "hbm": {"achieved": achieved, "frac": achieved / HBM_PEAK_GBS,
+                   "bytes_per_cell_step": per_cell}}
+    if ind is not None:
+        out["troubled_fraction"] = ind.troubled_fraction()
+        out["flag_refresh_every"] = ind.refresh_interval(solver.cfl_max)
+        out["judge"] = dict(ind.judge_stats)
+    return out, sol
+
+
+def rel_err(q, ref):
+    return (float(np.abs(q - ref).sum() / np.abs(ref).sum()),
+            float(np.abs(q - ref).max() / np.abs(ref).max()))
+
+
+def full_run(n=SHARP_N, t_final=1.0, nout=10):
+    """Both paths from t = 0 to t_final with output times every t_final/nout
+    (the same clipped dt sequence for both): wall time, steps, the error of the
+    FP32 + indicator state against the FP64 full-WENO5 state at every output
+    time, and the indicator's troubled fraction and judge statistics."""
+    import torch
+    from pyclaw_amd import indicator as I
+    res, frames = {}, {}
+    for name, prec, use in (("fp64_weno", "fp64", False), ("fp32k_mlp", "fp32k", True)):
+        ind = I.TroubledInterfaceIndicator() if use else None
+        solver, sol = sharp_problem(prec, ind, n)
+        traj, qs, wall = [], [], 0.0
+        for t in np.linspace(0.0, t_final, nout + 1)[1:]:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            solver.evolve_to_time(sol, float(t))
+            torch.cuda.synchronize()
+            wall += time.perf_counter() - t0
+            qs.append(sol.state.q.copy())
+            if ind is not None:
+                traj.append(ind.troubled_fraction())
+        steps = solver.status["numsteps"]
+        res[name] = {"wall_s": wall, "steps": steps, "t_end": sol.t,
+                     "value": n * n * steps / wall}
+        if ind is not None:
+            res[name]["troubled_fraction_at_outputs"] = traj
+            res[name]["judge"] = dict(ind.judge_stats)
+        frames[name] = qs
+        print(f"[bench] full run {name}: {steps} steps in {wall:.1f} s", file=sys.stderr,
+              flush=True)
+    errs = [rel_err(a, b) for a, b in zip(frames["fp32k_mlp"], frames["fp64_weno"])]
+    res["rel_l1_vs_fp64_at_outputs"] = [e[0] for e in errs]
+    res["rel_max_vs_fp64_at_outputs"] = [e[1] for e in errs]
+    res["rel_l1_vs_fp64_at_t_final"], res["rel_max_vs_fp64_at_t_final"] = errs[-1]
     res["speedup_fp32k_mlp_vs_fp64_weno"] = res["fp32k_mlp"]["value"] / res["fp64_weno"]["value"]
-    res["speedup_fp32k_weno_vs_fp64_weno"] = (res["fp32k_weno"]["value"] /
-                                              res["fp64_weno"]["value"])
-    res.update({"workload": "swe2d_periodic_humps_dambreaks_sharpclaw_weno5char_ssp33_2048",
-                "unit": "cell-updates/s", "steps": steps})
+    res.update({"t_final": t_final, "output_times": nout, "grid": [n, n]})
     return res
+
+
+def sharp_cpu_baseline(sample_steps=2, n=SHARP_N, seed=3):
+    """The reference's own SharpClaw operator (sharpclaw/flux2.f90 + flux1.f90 +
+    reconstruct.f90 weno5_char, built from the reference sources by oracle/ref.mk)
+    in the reference's SSP33 step (sharpclaw/solver.py:289-323, periodic ghost
+    fill before each of the 3 operator evaluations, numpy stage combinations,
+    dt from the CFL), single thread, on the same 2048^2 initial state."""
+    from oracle import oracle as O
+    from pyclaw_amd import indicator as I
+    kind = "reference" if O.ref_available() else "port"
+    flux = O.ref_sharp_flux2 if kind == "reference" else O.sharp_flux2
+    rp = O.make_rp(O.SHALLOW, efix=1, grav=1.0)
+    q = I.swe_problem(seed, n, n)
+    dx = 1.0 / n
+    dt = 0.45 / n / 1.6
+    ng = 3
+
+    def dq(y):
+        ybc = O.with_ghosts(y, ng)
+        O.bc2(ybc, ng, [2, 2], [2, 2])
+        d, c = flux(rp, ybc, ng, dx, dx, dt, 2)
+        return O.interior(d, ng), c
+
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        d1, c1 = dq(q)
+        dt_new = dt * 0.45 / c1
+        d1 = d1 * (dt_new / dt)    # dq_dt*dt at the new dt
+        dt = dt_new
+        r = q + d1
+        d2, _ = dq(r)
+        r = 0.75 * q + 0.25 * (r + d2)
+        d3, _ = dq(r)
+        q = 1. / 3. * q + 2. / 3. * (r + d3)
+    wall = time.perf_counter() - t0
+    return {"value": n * n * sample_steps / wall, "unit": "cell-updates/s", "cores": 1,
+            "kind": kind,
+            "sample": f"{sample_steps} SSP33 steps of the {n}x{n} shallow-water workload "
+                      f"(the reference's sharpclaw/flux2.f90 characteristic WENO5 compiled "
+                      f"with amdflang -O2, periodic ghost fill and numpy stage updates), "
+                      f"{wall:.1f} s"}
+
+
+def classic_leg(steps=400, warmup=16):
+    """The classic unsplit 2D Euler step on 1024^2 (FP64 and FP32 + Kahan),
+    device-controlled, graph captured before the timed region."""
+    cells = MX * MY
+    wall64, k64, n64, host64, sol64 = run_gpu("fp64", steps, warmup)
+    wall32, k32, n32, host32, sol32 = run_gpu("fp32k", steps, warmup)
+    err_l1, err_max = rel_err(sol32.state.q, sol64.state.q)
+    achieved = BYTES_PER_CELL_F64 * cells / (k64 * 1e-3) / 1e9
+    counts = measured_counts()
+    flops = counts.get("fp64_flops_per_launch")
+    return {
+        "workload": "euler2d_quadrants_roe_efix_unsplit_transcor_mc_1024x1024",
+        "value": cells * steps / wall64, "unit": "cell-updates/s", "steps": steps,
+        "ms_per_step": 1e3 * wall64 / steps, "t_end": sol64.t, "accepted": host64.accepted,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": counts.get("bytes_per_launch"),
+                     "kernel": "k_unsplit_ctl<Euler<double>>", "kernel_ms": k64,
+                     "kernels_timed": n64, "bytes_per_cell": BYTES_PER_CELL_F64},
+        "fp64_valu": (None if flops is None else
+                      {"achieved": flops / (k64 * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s",
+                       "frac": flops / (k64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                       "flops_per_launch": flops}),
+        "fp32_kahan": {"value": cells * steps / wall32, "ms_per_step": 1e3 * wall32 / steps,
+                       "kernel_ms": k32,
+                       "hbm_frac": BYTES_PER_CELL_F32K * cells / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "rel_l1_vs_fp64": err_l1, "rel_max_vs_fp64": err_max, "t_end": sol32.t},
+        "cpu_baseline": classic_cpu_baseline(),
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="acoustics,ensemble,sharpclaw,euler3d",
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the runs to t = 1.0 of both SharpClaw paths")
+    ap.add_argument("--extras", default="classic,acoustics,ensemble,euler3d",
                     help="comma list of extra workloads to time beside the headline")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="launch the steps one by one (for per-dispatch counter runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a plain 256 MiB device copy (known bytes for counter runs)")
     args = ap.parse_args()
@@ -389,54 +552,53 @@ def main():
         print("bench.py measures one GPU: the hot path runs on a single MI355X in this build "
               "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
         sys.exit(2)
-    cells = MX * MY
-    wall64, k64, n64, host64, sol64 = run_gpu("fp64", args.steps, args.warmup, not args.no_graph)
-    wall32, k32, n32, host32, sol32 = run_gpu("fp32k", args.steps, args.warmup,
-                                              not args.no_graph)
-    q64 = sol64.state.q
-    q32 = sol32.state.q
-    err_l1 = float(np.abs(q32 - q64).sum() / np.abs(q64).sum())
-    err_max = float(np.abs(q32 - q64).max() / np.abs(q64).max())
-    value = cells * args.steps / wall64
-    achieved = BYTES_PER_CELL_F64 * cells / (k64 * 1e-3) / 1e9
-    counts = measured_counts()
-    flops = counts.get("fp64_flops_per_launch")
+    n = SHARP_N
+    head, sol_mlp = sharp_steps("fp32k", True, args.steps, args.warmup)
+    print("[bench] fp32k + indicator done", file=sys.stderr, flush=True)
+    f64, sol64 = sharp_steps("fp64", False, args.steps, args.warmup)
+    print("[bench] fp64 full WENO5 done", file=sys.stderr, flush=True)
+    f32, sol32 = sharp_steps("fp32k", False, args.steps, args.warmup)
+    print("[bench] fp32k full WENO5 done", file=sys.stderr, flush=True)
+    l1, mx_err = rel_err(sol_mlp.state.q, sol64.state.q)
+    f32["rel_l1_vs_fp64"], f32["rel_max_vs_fp64"] = rel_err(sol32.state.q, sol64.state.q)
+    counts = measured_counts("sharp_swe2d_2048_fp32k_mlp")
     rec = {
         "metric": "cell-updates/s, 2D Euler/SWE 2048\u00b2: FP64 plain vs FP32+learned-WENO; "
                   "% HBM peak",
-        "value": value,
+        "value": head["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall64 / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "euler2d_quadrants_roe_efix_unsplit_transcor_mc_1024x1024",
-                   "grid": [MX, MY], "steps": args.steps, "cfl": 0.8, "bc": "extrap",
-                   "seed": 2, "t_end": sol64.t, "accepted": host64.accepted,
-                   "rejected": host64.nrejected},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": counts.get("bytes_per_launch"),
-                     "kernel": "k_unsplit_ctl<Euler<double>>", "kernel_ms": k64,
-                     "kernels_timed": n64, "bytes_per_cell": BYTES_PER_CELL_F64},
-        # the same kernel against the FP64 vector peak: executed FP64 operations
-        # per launch (counter run) over the live kernel time
-        "fp64_valu": (None if flops is None else
-                      {"achieved": flops / (k64 * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
-                       "unit": "TFLOP/s",
-                       "frac": flops / (k64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                       "flops_per_launch": flops}),
-        "fp32_kahan": {"value": cells * args.steps / wall32,
-                       "ms_per_step": 1e3 * wall32 / args.steps, "kernel_ms": k32,
-                       "hbm_frac": BYTES_PER_CELL_F32K * cells / (k32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                       "rel_l1_vs_fp64": err_l1, "rel_max_vs_fp64": err_max,
-                       "t_end": sol32.t},
+        "config": {"workload": "swe2d_periodic_humps_dambreaks_sharpclaw_weno5char_ssp33_2048",
+                   "grid": [n, n], "cfl": 0.45, "cfl_max": 0.5, "bc": "periodic", "seed": 3,
+                   "time_integrator": "SSP33", "char_decomp": 2,
+                   "precision": "fp32 state and fluxes, Kahan-compensated update, fp64 CFL",
+                   "indicator": "shipped weights, bf16 MFMA, flags every 2nd step",
+                   "t_end": head["t_end"], "accepted": head["accepted"]},
+        "roofline": {"bound": "hbm", "achieved": head["hbm"]["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["hbm"]["frac"],
+                     "traffic": counts.get("bytes_per_launch"),
+                     "kernel": "k_sharp_march<Shallow<float>,char,flags> + k_sharp_patch",
+                     "kernel_ms": head["stage_ms"], "kernels_timed": head["stage_launches_timed"],
+                     "bytes_per_cell_per_launch": SHARP_BYTES_F32K / SHARP_STAGES},
+        "indicator": {k: head[k] for k in ("troubled_fraction", "flag_refresh_every", "judge")},
+        "fp64_weno": f64,
+        "fp32k_weno": f32,
+        "speedup_vs_fp64_weno": head["value"] / f64["value"],
+        "rel_l1_vs_fp64": l1,
+        "rel_max_vs_fp64": mx_err,
     }
-    legs = {"acoustics": acoustics_leg, "ensemble": ensemble_leg, "sharpclaw": sharpclaw_leg,
+    if not args.no_full_run:
+        rec["full_run"] = full_run()
+        print("[bench] full run done", file=sys.stderr, flush=True)
+    legs = {"classic": classic_leg, "acoustics": acoustics_leg, "ensemble": ensemble_leg,
             "euler3d": euler3d_leg}
     for name in [x for x in args.extras.split(",") if x]:
         try:
@@ -445,7 +607,7 @@ def main():
             rec[name] = {"error": repr(e)}
         print(f"[bench] {name} done", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline()
+        rec["cpu_baseline"] = sharp_cpu_baseline()
     if args.calibrate_copy:
         import torch
         a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
