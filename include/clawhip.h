@@ -368,6 +368,19 @@ long long clawhip_sums_work_doubles(int nfields);
 int clawhip_state_sums(int dtype, const void *q, const clawhip_geom *g, int meqn,
                        unsigned positive_mask, double *out, double *work, void *stream);
 
+/* clawhip_state_sums without the work argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It has no counterpart in the reference.
+ * The per-block partials go to a fixed array in the library's device memory
+ * (2048 x 8 doubles = 128 KiB per device, held until the process ends); the
+ * call itself allocates nothing.  meqn above CLAWHIP_SUMS_NOALLOC_MAX_MEQN is
+ * rejected with CLAWHIP_E_ARG.  Results equal clawhip_state_sums bit for bit.
+ * Every call on one device shares that array: calls that overlap on one device
+ * (two streams, or two host threads) race and must be serialised by the caller. */
+#define CLAWHIP_SUMS_NOALLOC_MAX_MEQN 7
+int clawhip_state_sums_noalloc(int dtype, const void *q, const clawhip_geom *g, int meqn,
+                               unsigned positive_mask, double *out, void *stream);
+
 /* out[0] = sum |a[i] - b[i]|, out[1] = sum |b[i]| over n elements, in FP64. */
 int clawhip_l1_diff(int dtype, const void *a, const void *b, long long n, double *out,
                     double *work, void *stream);

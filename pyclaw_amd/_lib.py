@@ -121,6 +121,7 @@ _SIGS = {
     "clawhip_mlp_flags": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _P, _F, _P, _P],
     "clawhip_sums_work_doubles": [_I],
     "clawhip_state_sums": [_I, _P, ctypes.POINTER(Geom), _I, ctypes.c_uint, _P, _P, _P],
+    "clawhip_state_sums_noalloc": [_I, _P, ctypes.POINTER(Geom), _I, ctypes.c_uint, _P, _P],
     "clawhip_l1_diff": [_I, _P, _P, ctypes.c_longlong, _P, _P, _P],
 }
 

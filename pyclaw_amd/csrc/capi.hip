@@ -474,6 +474,16 @@ int clawhip_state_sums(int dtype, const void* q, const clawhip_geom* g, int meqn
                            (hipStream_t)stream);
 }
 
+int clawhip_state_sums_noalloc(int dtype, const void* q, const clawhip_geom* g, int meqn,
+                               unsigned positive_mask, double* out, void* stream) {
+  if (!q || !g || !out) return CLAWHIP_E_ARG;
+  if (meqn < 1 || meqn > CLAWHIP_SUMS_NOALLOC_MAX_MEQN) return CLAWHIP_E_ARG;
+  if (!geom_ok(g, g->mz > 1 ? 3 : 2) || g->mz < 1) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  return launch_state_sums_noalloc(dtype, q, to_geom(g), meqn, positive_mask, out,
+                                   (hipStream_t)stream);
+}
+
 int clawhip_l1_diff(int dtype, const void* a, const void* b, long long n, double* out,
                     double* work, void* stream) {
   if (!a || !b || !out || !work || n < 0) return CLAWHIP_E_ARG;

@@ -99,6 +99,8 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
                      unsigned char* flags, hipStream_t s);
 int launch_state_sums(int dtype, const void* q, const Geom& g, int meqn, unsigned positive_mask,
                       double* out, double* work, hipStream_t s);
+int launch_state_sums_noalloc(int dtype, const void* q, const Geom& g, int meqn,
+                              unsigned positive_mask, double* out, hipStream_t s);
 int launch_l1_diff(int dtype, const void* a, const void* b, long long n, double* out,
                    double* work, hipStream_t s);
 

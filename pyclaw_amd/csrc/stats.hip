@@ -6,6 +6,8 @@
 // walking a fixed set of cells in order, a fixed-order tree in LDS per block,
 // then one block summing the per-block partials in block order.  Sums are FP64
 // for FP32 states too.
+#include <mutex>
+
 #include "launch.hpp"
 
 namespace claw {
@@ -112,6 +114,53 @@ int launch_state_sums(int dtype, const void* q, const Geom& g, int meqn, unsigne
   hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(SUMS_THREADS), 0, s, work, meqn + 1, out);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
+}
+
+// ---------------------------------------------------------------- no-allocation form
+// The per-block partials of launch_state_sums_noalloc live in this file-scope
+// device array instead of a caller's work buffer: SUMS_BLOCKS * SUMS_MAXF doubles
+// (128 KiB), held by every device the library runs on until the process ends.
+// Every block of k_state_sums writes all meqn + 1 of its partials, and
+// k_finish_sums reads only those, so nothing in it has to start at zero.  Calls
+// on one device share the array: calls that overlap on one device (two streams,
+// or two host threads) race.
+static __device__ __align__(256) double scratch[SUMS_BLOCKS * SUMS_MAXF];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy at its own address).  Host code uses
+// only this address: the array's name is a host-side stand-in.
+hipError_t scratch_address(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+}  // namespace
+
+int launch_state_sums_noalloc(int dtype, const void* q, const Geom& g, int meqn,
+                              unsigned positive_mask, double* out, hipStream_t s) {
+  if (meqn < 1 || meqn + 1 > SUMS_MAXF) return CLAWHIP_E_ARG;
+  static_assert(SUMS_BLOCKS * SUMS_MAXF * sizeof(double) == sizeof(scratch),
+                "scratch holds the partials of SUMS_MAXF fields");
+  double* work = nullptr;
+  hipError_t e = scratch_address(&work);
+  if (e != hipSuccess) return (int)e;
+  return launch_state_sums(dtype, q, g, meqn, positive_mask, out, work, s);
 }
 
 int launch_l1_diff(int dtype, const void* a, const void* b, long long n, double* out,

@@ -17,6 +17,17 @@ def test_library_exports_match_header():
     assert set(names) <= set(L._SIGS), set(names) - set(L._SIGS)
 
 
+def test_noalloc_sums_reject_sizes_before_device_work():
+    """The allocation-free reduction refuses more fields than its fixed scratch
+    holds (and null pointers) before it touches a device."""
+    lib = L.lib()
+    g = L.geom2(4, 4)
+    dummy = ctypes.c_void_p(16)
+    assert lib.clawhip_state_sums_noalloc(L.F64, dummy, ctypes.byref(g), 8, 0, dummy, None) == -1
+    assert lib.clawhip_state_sums_noalloc(L.F64, dummy, ctypes.byref(g), 0, 0, dummy, None) == -1
+    assert lib.clawhip_state_sums_noalloc(L.F64, None, ctypes.byref(g), 3, 0, dummy, None) == -1
+
+
 def test_mlp_param_count():
     # W1[16][32] (row 6 = bias), W2[32][32], b2[32], w3[32], b3
     assert L.lib().clawhip_mlp_param_count() == 16 * 32 + 32 * 32 + 32 + 32 + 1
