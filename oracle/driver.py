@@ -85,6 +85,7 @@ class OracleRun:
         self.rk_a, self.rk_b, self.rk_c = rk_a, rk_b, rk_c
         self.stage_t = 0.0
         self.plant_stage_time = False
+        self.plant_lmm3 = False
         self.lmm_alpha, self.lmm_beta = lmm_alpha, lmm_beta
         # number of saved states (sharpclaw/solver.py:596-610)
         self.nreg = len(lmm_alpha) if time_integrator == "LMM" else lmm_steps
@@ -302,9 +303,11 @@ class OracleRun:
             delta0 = (4 * omk - om ** 2) / om ** 3
             beta0 = omk / om ** 2
             bk = omk ** 2 / om ** 2
+            # plant_lmm3 (tests only): the oldest rate replaced by the newest one
+            old = self.prev_dq_dt[-1 if self.plant_lmm3 else -self.lmm_steps]
             self.q = (bk * (r * q + dt * self.prev_dq_dt[-1]) +
                       (r * beta0 + delta0) * self.registers[-self.lmm_steps] +
-                      beta0 * dt * self.prev_dq_dt[-self.lmm_steps])
+                      beta0 * dt * old)
         elif self.time_integrator == "SSP33":
             r = q + dt * self.dq_dt
             r = 0.75 * q + 0.25 * (r + self._dq(r, t0 + dt))

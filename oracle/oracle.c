@@ -336,9 +336,10 @@ static void rpt_shallow(const orc_rp *p, int ixy, const double *qL, const double
   double a2 = asdq[mu] - u * asdq[0];
   double a3 = (0.50 / a) * (-(v - a) * asdq[0] + asdq[mv]);
   double wb[3][3], sb[3];
-  wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
-  wb[1][0] = 0.0; wb[1][mu] = a2; wb[1][mv] = 0.0; sb[1] = v;
-  wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
+  const double vs = g_plant == 8 ? u : v;   /* planted: the normal velocity as the speed */
+  wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = vs - a;
+  wb[1][0] = 0.0; wb[1][mu] = a2; wb[1][mv] = 0.0; sb[1] = vs;
+  wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = vs + a;
   for (int m = 0; m < 3; m++) {
     bm[m] = 0.0; bp[m] = 0.0;
     for (int mw = 0; mw < 3; mw++) {
@@ -551,11 +552,18 @@ static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const d
  *      must move exactly as advection_2D moves its scalar);
  *   6  the Courant-number maximum with the Fortran build's NaN behaviour (a NaN
  *      term restarts the running maximum) instead of leaving NaN terms out
- *      (caught by the NaN-cell Courant-number test).
- * Which edge (aux2 or aux3) and which cell (i1) vc_advection_2D's rpt2 reads is
- * NOT pinned by any fixture: the annulus field has zero radial velocity and a
- * theta velocity that depends on r only, so those choices leave its solution
- * unchanged (parity unpinned). */
+ *      (caught by the NaN-cell Courant-number test);
+ *   7  evec: the x-sweep's eigenvectors in both sweeps (caught by the eigen
+ *      relation evl * A(q_roe) * evr = diag(lambda) in the sweep direction);
+ *   8  shallow-water rpt2: the normal velocity as the transverse speed (caught
+ *      by the transverse Roe-split test);
+ *   9  vc_advection_2D rpt2: the cell's edges swapped (upper edge for the
+ *      down-going part, lower edge for the up-going part);
+ *  10  vc_advection_2D rpt2: cell i for both imp (9 and 10 are caught by the
+ *      discrete maximum principle of the first-order corner-transport step on
+ *      random edge-velocity fields; no fixture tells them apart: the annulus
+ *      field has zero radial velocity and a theta velocity that depends on r
+ *      only, so those choices leave its solution unchanged). */
 static int g_plant = 0;
 void orc_set_plant(int k) { g_plant = k; }
 
@@ -609,8 +617,11 @@ void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
 static void rpt_vc_advection(const orc_rp *p, int ixy, int imp, int maux, const double *a2,
                              const double *a3, const double *asdq, double *bm, double *bp) {
   const int kv = g_plant == 1 ? ixy : 3 - ixy;   /* 1-based aux index */
-  const long i1 = imp == 1 ? 0 : maux;           /* a2, a3 point at cell i-1 */
-  const double vm = dmin(a2[i1 + kv - 1], 0.0), vp = dmax(a3[i1 + kv - 1], 0.0);
+  /* a2, a3 point at cell i-1; planted 10: cell i for both imp */
+  const long i1 = (imp == 1 && g_plant != 10) ? 0 : maux;
+  /* planted 9: the edges swapped (upper edge for the down-going part) */
+  const double *ed = g_plant == 9 ? a3 : a2, *eu = g_plant == 9 ? a2 : a3;
+  const double vm = dmin(ed[i1 + kv - 1], 0.0), vp = dmax(eu[i1 + kv - 1], 0.0);
   for (int m = 0; m < vc_meqn(p); m++) {
     bm[m] = vm * asdq[m];
     bp[m] = vp * asdq[m];
@@ -1309,7 +1320,8 @@ void orc_evec(const orc_rp *p, int ixy, int meqn, int n2, const double *q, doubl
               double *evr) {
 #define EVL(ip, m, i) evl[(ip) + (long)meqn * ((m) + (long)meqn * ((i) - 1))]
 #define EVR(m, ip, i) evr[(m) + (long)meqn * ((ip) + (long)meqn * ((i) - 1))]
-  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const int xy = g_plant == 7 ? 1 : ixy;   /* planted: the x-sweep's vectors for both */
+  const int mu = xy == 1 ? 1 : 2, mv = xy == 1 ? 2 : 1;
   for (int i = 2; i <= n2; i++) {
     const double *qL = &B1(q, 0, i - 1), *qR = &B1(q, 0, i);
     for (int a = 0; a < meqn; a++)

@@ -997,3 +997,200 @@ def test_nan_wave_speeds_left_out_of_the_courant_number(plant):
         assert all(same), (with_nan, clean)
     else:
         assert not any(same), (with_nan, clean)
+
+
+def _swe_roe(qL, qR, d, grav=1.0):
+    """Roe state of shallow water for the sweep direction d (0: x, 1: y): the
+    normal and transverse velocities (sqrt(h)-weighted) and c = sqrt(g*h_mean)."""
+    mu, mv = (1, 2) if d == 0 else (2, 1)
+    sl, sr = np.sqrt(qL[0]), np.sqrt(qR[0])
+    un = (qL[mu] / sl + qR[mu] / sr) / (sl + sr)
+    ut = (qL[mv] / sl + qR[mv] / sr) / (sl + sr)
+    return un, ut, np.sqrt(grav * 0.5 * (qL[0] + qR[0]))
+
+
+def _swe_jacobian(un, ut, c, d):
+    """Flux Jacobian of shallow water in direction d at (h, u_n, u_t), c^2 = g*h:
+    rows/columns (h, hu, hv)."""
+    mu, mv = (1, 2) if d == 0 else (2, 1)
+    J = np.zeros((3, 3))
+    J[0, mu] = 1.0
+    J[mu, 0], J[mu, mu] = c * c - un * un, 2 * un
+    J[mv, 0], J[mv, mu], J[mv, mv] = -un * ut, ut, un
+    return J
+
+
+def _evec_eigen_residual(kind, plant=0):
+    """max over interfaces and sweeps of |evl * A(q_roe) * evr - diag(lambda)|,
+    relative to the largest speed: A the flux Jacobian in the sweep direction at
+    the Roe state of the interface, lambda = (u_n - c, u_n, [u_n,] u_n + c)."""
+    rng = np.random.default_rng(17 + kind)
+    rp = O.make_rp(kind)
+    q = seeded_state(rng, kind, (24,))
+    worst = 0.0
+    O.lib().orc_set_plant(plant)
+    try:
+        for ixy in (1, 2):
+            d = ixy - 1
+            evl, evr = O.evec(rp, ixy, q)
+            for i in range(1, q.shape[1]):
+                qL, qR = q[:, i - 1], q[:, i]
+                if kind == O.SHALLOW:
+                    un, ut, c = _swe_roe(qL, qR, d)
+                    A = _swe_jacobian(un, ut, c, d)
+                    lam = [un - c, un, un + c]
+                else:
+                    sl, sr = np.sqrt(qL[0]), np.sqrt(qR[0])
+                    u = (qL[1] / sl + qR[1] / sr) / (sl + sr)
+                    v = (qL[2] / sl + qR[2] / sr) / (sl + sr)
+                    p = [0.4 * (s[3] - 0.5 * (s[1] ** 2 + s[2] ** 2) / s[0]) for s in (qL, qR)]
+                    H = ((qL[3] + p[0]) / sl + (qR[3] + p[1]) / sr) / (sl + sr)
+                    c = np.sqrt(0.4 * (H - 0.5 * (u * u + v * v)))
+                    A = _euler_jacobian(u, v, H, d)
+                    un = u if d == 0 else v
+                    lam = [un - c, un, un, un + c]
+                R = evl[:, :, i] @ A @ evr[:, :, i] - np.diag(lam)
+                worst = max(worst, np.abs(R).max() / np.abs(lam).max())
+    finally:
+        O.lib().orc_set_plant(0)
+    return worst
+
+
+@pytest.mark.parametrize("kind", [O.SHALLOW, O.EULER])
+def test_evec_diagonalises_the_sweep_jacobian(kind):
+    """The eigenvectors the characteristic WENO5 projects with
+    (reconstruct.f90:201-299, the user evec at the Roe average of cells i-1, i):
+    evl * A(q_roe) * evr = diag(u_n - c, u_n, [u_n,] u_n + c) with A the flux
+    Jacobian in the sweep's direction, in both sweeps.  A basis of the other
+    direction (plant 7: the x-sweep's vectors used for y) fails it, as would
+    any invertible basis that is not an eigenbasis or is ordered otherwise."""
+    assert _evec_eigen_residual(kind) < 1e-13
+    assert _evec_eigen_residual(kind, plant=7) > 1e-2
+
+
+def _swe_rpt_residuals(plant=0):
+    """shallow_roe_with_efix_2D rpt2 against the Roe split of the transverse
+    Jacobian B at the interface's Roe state: |bm + bp - B asdq| and
+    |bm - R min(Lambda, 0) R^-1 asdq| (numpy eigen-decomposition)."""
+    import ctypes
+    rng = np.random.default_rng(29)
+    rp = O.make_rp(O.SHALLOW, grav=1.0)
+    e_sum = e_split = 0.0
+    O.lib().orc_set_plant(plant)
+    try:
+        for _ in range(40):
+            for ixy in (1, 2):
+                qL = seeded_state(rng, O.SHALLOW, (1,))[:, 0].copy()
+                qR = seeded_state(rng, O.SHALLOW, (1,))[:, 0].copy()
+                asdq = rng.normal(size=3)
+                bm, bp = np.zeros(3), np.zeros(3)
+                O.lib().orc_rpt(ctypes.byref(rp), ixy, 1, O._dp(qL), O._dp(qR), O._dp(asdq),
+                                O._dp(bm), O._dp(bp))
+                # Roe state of the normal solve, Jacobian in the transverse direction
+                un, ut, c = _swe_roe(qL, qR, ixy - 1)
+                B = _swe_jacobian(ut, un, c, 2 - ixy)
+                lam, R = np.linalg.eig(B)
+                Rinv = np.linalg.inv(R)
+                bm_ref = (R @ np.diag(np.minimum(lam, 0.0)) @ Rinv @ asdq).real
+                scale = np.abs(B @ asdq).max() + 1e-300
+                e_sum = max(e_sum, np.abs(bm + bp - B @ asdq).max() / scale)
+                e_split = max(e_split, np.abs(bm - bm_ref).max() / scale)
+    finally:
+        O.lib().orc_set_plant(0)
+    return e_sum, e_split
+
+
+def test_swe_transverse_split_is_roe_split():
+    """shallow_roe_with_efix_2D's rpt2 (used at flux2.f90:181-203): asdq split by
+    the eigenvectors of the transverse Jacobian at the Roe state, bmasdq from the
+    negative transverse speeds only and bmasdq + bpasdq = B(q_roe) asdq.  The
+    normal velocity planted as the transverse speed (plant 8) fails it."""
+    e_sum, e_split = _swe_rpt_residuals()
+    assert e_sum < 1e-12 and e_split < 1e-12, (e_sum, e_split)
+    assert max(_swe_rpt_residuals(plant=8)) > 1e-3
+
+
+def _vc_ctu_max_principle_violation(plant=0, trials=100):
+    """First-order unsplit steps (no limiter; transverse_waves 1 and 2) of
+    vc_advection_2D on random data in [0, 1] with random edge velocities (one
+    sign per component and trial, magnitudes U[0, 1]) at Courant number 0.45 per
+    direction, periodic: the largest excursion of the new state beyond the old
+    state's range."""
+    rng = np.random.default_rng(3)
+    mx, my, ng = 16, 14, 2
+    rp = O.make_rp(O.VC_ADVECTION_2D)
+    dx, dy = 1.0 / mx, 1.0 / my
+    dt = 0.45 * min(dx, dy)
+    worst = 0.0
+    O.lib().orc_set_plant(plant)
+    try:
+        for _ in range(trials):
+            q = np.asfortranarray(rng.uniform(0.0, 1.0, (1, mx, my)))
+            sgn = rng.choice([-1.0, 1.0], 2)
+            aux = np.zeros((2, mx + 2 * ng, my + 2 * ng), order="F")
+            aux[0] = sgn[0] * rng.uniform(0.0, 1.0, aux[0].shape)
+            aux[1] = sgn[1] * rng.uniform(0.0, 1.0, aux[1].shape)
+            qbc = O.with_ghosts(q, ng)
+            O.bc2(qbc, ng, [2, 2], [2, 2])
+            for trans in (1, 2):
+                qnew = qbc.copy(order="F")
+                O.step2(rp, qbc, qnew, ng, dx, dy, dt,
+                        O.method_array(order=1, transverse=trans, num_aux=2), [0], aux=aux)
+                qi = O.interior(qnew, ng)
+                worst = max(worst, qbc.min() - qi.min(), qi.max() - qbc.max())
+    finally:
+        O.lib().orc_set_plant(0)
+    return worst
+
+
+def test_vc_advection_transverse_edges_keep_the_maximum_principle():
+    """vc_advection_2D's rpt2 (the published rpt2_vc_advection: the fluctuation
+    that entered cell i1 = i-2+imp leaves it downward with that cell's lower-edge
+    velocity and upward with its upper-edge velocity).  With those choices the
+    first-order corner-transport update is a convex combination of neighbouring
+    values (LeVeque, Finite Volume Methods for Hyperbolic Problems, 20.5), so it
+    keeps the old range on any edge-velocity field within the CFL bound.  The
+    edges swapped (plant 9) or cell i used for both parts (plant 10) produce
+    new extrema."""
+    assert _vc_ctu_max_principle_violation() == 0.0
+    assert _vc_ctu_max_principle_violation(plant=9) > 1e-3
+    assert _vc_ctu_max_principle_violation(plant=10) > 1e-3
+
+
+def _lmm_convergence(ti, k, plant=False, n=24, T=0.2):
+    """Errors of fixed-step runs (48, 96, 192 steps) of scalar advection (u = 1,
+    v = 0.5, periodic, component WENO5) of a smooth state, against a 1536-step
+    run on the same grid, so that only the time error remains."""
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    q = np.asfortranarray((1.0 + 0.5 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y))[None])
+    rp = O.make_rp(O.ADVECTION, u=1.0, v=0.5)
+
+    def run(nsteps, planted):
+        r = OracleRun(rp, q, [1.0 / n, 1.0 / n], [2, 2], [2, 2], solver="sharp",
+                      time_integrator=ti, lmm_steps=k, dt_variable=False, dt_initial=T / nsteps,
+                      cfl_max=1.0, cfl_desired=0.9)
+        r.plant_lmm3 = planted
+        r.run(T, 1)
+        assert r.numsteps == nsteps
+        return r.q
+
+    ref = run(1536, False)
+    return [np.abs(run(m, plant) - ref).max() for m in (48, 96, 192)]
+
+
+@pytest.mark.parametrize("ti,k,order", [("SSPLMMk3", 4, 3), ("SSPLMMk3", 5, 3),
+                                        ("SSPLMMk2", 3, 2)])
+def test_ssplmm_convergence_order(ti, k, order):
+    """The multistep updates (sharpclaw/solver.py:351-374) with their SSP22
+    starting steps converge at their design order in time: errors fall by
+    2^order per halving of dt (observed 2.97-3.00 for SSPLMMk3 with 4 and 5 steps,
+    1.99-2.01 for SSPLMMk2).  Using the newest rate in place of the oldest in
+    SSPLMMk3's update (plant) drops it to first order (rates 0.95-0.98)."""
+    e = _lmm_convergence(ti, k)
+    rates = [np.log2(e[i] / e[i + 1]) for i in range(2)]
+    assert all(abs(r - order) < 0.15 for r in rates), rates
+    if ti == "SSPLMMk3":
+        ep = _lmm_convergence(ti, k, plant=True)
+        rp_ = [np.log2(ep[i] / ep[i + 1]) for i in range(2)]
+        assert not all(abs(r - order) < 0.15 for r in rp_), rp_
