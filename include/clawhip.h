@@ -100,6 +100,32 @@ typedef struct {
   unsigned long long cfl_bits;
 } clawhip_stepctl;
 
+/* Device-resident control of SharpClaw's SSP33 step (clawhip_sharp2_ssp33_run):
+ * what SharpClawSolver.step and Solver.evolve_to_time keep between attempts
+ * (pyclaw/sharpclaw/solver.py:283-323, 622-685; pyclaw/solver.py:535-666).
+ *   t, dt, dt_old, tstart, tend; cfl = the CFL object's cached maximum;
+ *   cfl_desired, cfl_max, dt_max; cflmax, dtmin, dtmax = solver.status;
+ *   have_dt_old: dt_old is set; accept_prev = solver.accept_step;
+ *   done: 0 running, 1 reached tend, 3 max_steps attempts made; cur: the buffer
+ *   (q0/q1, comp0/comp1) holding the state; attempts, max_steps, accepted,
+ *   nrejected; judge: the learned indicator's path (flags + positivity judge);
+ *   age, every: its flag-refresh cadence; judge_steps, fallbacks: its judge's
+ *   counts; run_flags, run1, run23, run_fb: the next attempt's gates (flags,
+ *   stage 1, stages 2-3, the full-WENO5 recomputation); the rest is the
+ *   launches' scratch (zero on entry). */
+typedef struct clawhip_sharpctl {
+  double t, dt, dt_old, tstart, tend;
+  double cfl, cfl_desired, cfl_max, dt_max;
+  double cflmax, dtmin, dtmax;
+  int have_dt_old, accept_prev, done, cur;
+  int attempts, max_steps, accepted, nrejected;
+  int judge, age, every, judge_steps;
+  int fallbacks, run_flags, run1, run23;
+  int run_fb, pad0;
+  unsigned int pad1, nbad;
+  unsigned long long cfl1_bits, cfl23_bits, cflfb_bits;
+} clawhip_sharpctl;
+
 /* Library / device information: writes the HIP device count; 0 on success. */
 int clawhip_device_count(int *count);
 
@@ -204,9 +230,11 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
  * flags (may be NULL): per-interface troubled flags from clawhip_mlp_flags.
  * An interface whose flag and whose two neighbours' flags (along its normal) are
  * 0 uses WENO5's linear 5th-order weights; the others use the WENO5 selected by
- * char_decomp.  troubled_cells / troubled_count: the cells next to a flagged
- * interface, from clawhip_mlp_troubled_cells (required with flags).  NULL flags
- * = WENO5 everywhere (the reference's operator).
+ * char_decomp.  NULL flags = WENO5 everywhere (the reference's operator).
+ * With flags: flag_list / flag_count from clawhip_mlp_flag_list of the same
+ * flags, and recon, a device buffer of 2*num_eqn*((mx+1)*my + mx*(my+1))
+ * elements of the state's type that takes the full-WENO5 edge values of the
+ * listed interfaces (work space; its contents on entry do not matter).
  * aux (geometry gaux, num_aux fields, codes aux_bc_lower/upper 1..4 as for
  * clawhip_step2_split; 4 needs >= 3 stored ghosts): the solver's aux fields for
  * the solvers that read them (kinds 6, 7, 8; char_decomp 0, no flags), else
@@ -218,10 +246,36 @@ int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
                          double dx, double dy, const void *P, const clawhip_geom *gin,
                          const void *Q, const void *A, const void *B, const void *aux_in,
                          const void *comp_in, void *out, void *aux_out, void *comp_out,
-                         const void *src, const unsigned char *flags,
-                         const int *troubled_cells, const int *troubled_count, const void *aux,
+                         const void *src, const unsigned char *flags, const int *flag_list,
+                         const int *flag_count, void *recon, const void *aux,
                          const clawhip_geom *gaux, const int *aux_bc_lower,
                          const int *aux_bc_upper, int num_aux, double *cfl_dev, void *stream);
+
+/* nsteps attempts of SharpClawSolver's SSP33 step (sharpclaw/solver.py:283-323)
+ * within Solver.evolve_to_time (solver.py:554-666), controlled on the device:
+ * each stage kernel reads dt and the current buffer from ctl_dev and runs only
+ * while its gate is set; a one-wavefront kernel after stage 1 applies get_dt
+ * and the CFL rescale, one after stage 3 accept_reject_step, t and the next
+ * attempt's gates.  Launches after ctl->done != 0 do nothing.  No host
+ * synchronisation: the launches may be captured in a HIP graph.
+ *   q0/q1 (comp0/comp1 for CLAWHIP_F32K): the state, compact planar, current
+ *   one by ctl->cur; dq_dt, r2: work of the state's size; aux23: FP32 work of
+ *   the state's size (CLAWHIP_F32K; may be NULL for F64).
+ *   flags (NULL: full WENO5, no judge): the learned indicator's path -- flags
+ *   of the state refreshed by ctl's cadence (mlp_params, flat, logit_threshold
+ *   as clawhip_mlp_flags), their list (flag_list, flag_count as
+ *   clawhip_mlp_flag_list), recon (as clawhip_sharp2_stage), and the judge:
+ *   a step that leaves a field of positive_mask not > 0 (or not finite) is
+ *   recomputed with full WENO5 into the same buffers (dq_dt_full: work of the
+ *   state's size).  Boundary codes 1..3. */
+int clawhip_sharp2_ssp33_run(const clawhip_rp *rp, int dtype, int char_decomp,
+                             const clawhip_geom *g, const int *bc_lower, const int *bc_upper,
+                             const int *reflect, double dx, double dy, void *q0, void *q1,
+                             void *comp0, void *comp1, void *dq_dt, void *r2, void *aux23,
+                             unsigned char *flags, int *flag_list, int *flag_count, void *recon,
+                             void *dq_dt_full, const float *mlp_params, float flat,
+                             float logit_threshold, unsigned positive_mask,
+                             clawhip_sharpctl *ctl_dev, int nsteps, void *stream);
 
 /* The elementwise middle of SSP104 (sharpclaw/solver.py:418-419):
  *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */
@@ -341,11 +395,12 @@ int clawhip_mlp_train_step(const float *feat, long long nsamples, const unsigned
 int clawhip_mlp_logits(const float *feat, long long n, const float *params, float *logits,
                        void *stream);
 
-/* The cells (index (r-1)*mx + (c-1)) with at least one flagged interface among
- * their four, flags dilated as clawhip_sharp2_stage applies them; *count = their
- * number (cells holds up to mx*my entries, in no particular order). */
-int clawhip_mlp_troubled_cells(const unsigned char *flags, int mx, int my, int *cells,
-                               int *count, void *stream);
+/* The interfaces whose flag, or a neighbour's along the normal, is set -- the
+ * ones clawhip_sharp2_stage reconstructs with the full WENO5 -- as interface ids
+ * in the flags' layout, in no particular order; *count = their number.  list
+ * holds up to (mx+1)*my + mx*(my+1) entries. */
+int clawhip_mlp_flag_list(const unsigned char *flags, int mx, int my, int *list, int *count,
+                          void *stream);
 
 /* Per-interface flags of a state: 1 when any component whose stencil range exceeds
  * flat has a logit > logit_threshold. */

@@ -44,6 +44,20 @@ class StepCtl(ctypes.Structure):
                 ("cfl_bits", ctypes.c_ulonglong)]
 
 
+class SharpCtl(ctypes.Structure):
+    """clawhip_sharpctl (include/clawhip.h)."""
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("t", "dt", "dt_old", "tstart", "tend", "cfl", "cfl_desired", "cfl_max",
+                 "dt_max", "cflmax", "dtmin", "dtmax")] + \
+               [(n, ctypes.c_int) for n in
+                ("have_dt_old", "accept_prev", "done", "cur", "attempts", "max_steps",
+                 "accepted", "nrejected", "judge", "age", "every", "judge_steps", "fallbacks",
+                 "run_flags", "run1", "run23", "run_fb", "pad0")] + \
+               [("pad1", ctypes.c_uint), ("nbad", ctypes.c_uint),
+                ("cfl1_bits", ctypes.c_ulonglong), ("cfl23_bits", ctypes.c_ulonglong),
+                ("cflfb_bits", ctypes.c_ulonglong)]
+
+
 class EnsMember(ctypes.Structure):
     """clawhip_ens_member"""
     _fields_ = [("mx", ctypes.c_int), ("my", ctypes.c_int),
@@ -91,8 +105,11 @@ _SIGS = {
                             _D, _D, _D, _D, _P, _P],
     "clawhip_sharp2_stage": [ctypes.POINTER(RP), _I, _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                              _I, _I, _D, _D, _D, _D, _P, ctypes.POINTER(Geom), _P, _P, _P, _P,
-                             _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(Geom), _IP, _IP,
-                             _I, _P, _P],
+                             _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(Geom), _IP,
+                             _IP, _I, _P, _P],
+    "clawhip_sharp2_ssp33_run": [ctypes.POINTER(RP), _I, _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
+                                 _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                 _F, _F, ctypes.c_uint, _P, _I, _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_lmm_update": [_I, _I, ctypes.c_longlong, _P, _P, _P, _P,
                            ctypes.POINTER(ctypes.c_double), _P, _P],
@@ -117,7 +134,7 @@ _SIGS = {
     "clawhip_mlp_train_step": [_P, ctypes.c_longlong, _P, _P, _I, _P, _P, _P, _P, _I, _F, _F,
                                _P, _P],
     "clawhip_mlp_logits": [_P, ctypes.c_longlong, _P, _P, _P],
-    "clawhip_mlp_troubled_cells": [_P, _I, _I, _P, _P, _P],
+    "clawhip_mlp_flag_list": [_P, _I, _I, _P, _P, _P],
     "clawhip_mlp_flags": [_I, _P, ctypes.POINTER(Geom), _IP, _IP, _IP, _I, _F, _P, _F, _P, _P],
     "clawhip_sums_work_doubles": [_I],
     "clawhip_state_sums": [_I, _P, ctypes.POINTER(Geom), _I, ctypes.c_uint, _P, _P, _P],

@@ -194,8 +194,8 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
                          double dx, double dy, const void* P, const clawhip_geom* gin,
                          const void* Q, const void* A, const void* B, const void* aux_in,
                          const void* comp_in, void* out, void* aux_out, void* comp_out,
-                         const void* src, const unsigned char* flags,
-                         const int* troubled_cells, const int* troubled_count, const void* aux,
+                         const void* src, const unsigned char* flags, const int* flag_list,
+                         const int* flag_count, void* recon, const void* aux,
                          const clawhip_geom* gaux, const int* aux_bc_lower,
                          const int* aux_bc_upper, int num_aux, double* cfl_dev, void* stream) {
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !P || !out || !cfl_dev)
@@ -206,7 +206,7 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
     return CLAWHIP_E_ARG;
   const bool ghosted = gin && gin->ghosts > 0;
   if (ghosted && (y_mode != 0 || flags)) return CLAWHIP_E_ARG;
-  if (flags && (!troubled_cells || !troubled_count)) return CLAWHIP_E_ARG;
+  if (flags && (!flag_list || !flag_count || !recon)) return CLAWHIP_E_ARG;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
@@ -232,10 +232,50 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
   int rc = make_aux_in(aux, gaux, aux_bc_lower, aux_bc_upper, g, 3, ax);
   if (rc) return rc;
   if ((na > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
+  FlagSet fs;
+  fs.flags = flags;
+  fs.list = flag_list;
+  fs.count = flag_count;
+  fs.recon = recon;
   return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(g), to_geom(gin ? gin : g), bc_lower,
                             bc_upper, reflect, y_mode, out_mode, dt, s, dx, dy, P, Q, A, B,
                             aux_in, comp_in, out, aux_out, comp_out, (unsigned long long*)cfl_dev,
-                            flags, troubled_cells, troubled_count, (hipStream_t)stream, ax, src);
+                            fs, (hipStream_t)stream, ax, src);
+}
+
+int clawhip_sharp2_ssp33_run(const clawhip_rp* rp, int dtype, int char_decomp,
+                             const clawhip_geom* g, const int* bc_lower, const int* bc_upper,
+                             const int* reflect, double dx, double dy, void* q0, void* q1,
+                             void* comp0, void* comp1, void* dq_dt, void* r2, void* aux23,
+                             unsigned char* flags, int* flag_list, int* flag_count, void* recon,
+                             void* dq_dt_full, const float* mlp_params, float flat,
+                             float logit_threshold, unsigned positive_mask,
+                             clawhip_sharpctl* ctl_dev, int nsteps, void* stream) {
+  static_assert(sizeof(clawhip_sharpctl) == sizeof(SharpCtl), "clawhip_sharpctl layout");
+  if (!rp || !g || !bc_lower || !bc_upper || !reflect || !q0 || !q1 || !dq_dt || !r2 ||
+      !ctl_dev || nsteps < 0)
+    return CLAWHIP_E_ARG;
+  if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
+  if (q0 == q1) return CLAWHIP_E_ARG;
+  if (dtype == CLAWHIP_F32K && (!comp0 || !comp1 || comp0 == comp1 || !aux23))
+    return CLAWHIP_E_ARG;
+  if (flags && (!flag_list || !flag_count || !recon || !dq_dt_full || !mlp_params))
+    return CLAWHIP_E_ARG;
+  if (rp_num_aux(rp) > 0) return CLAWHIP_E_ARG;
+  if (!(dx > 0.0) || !(dy > 0.0)) return CLAWHIP_E_ARG;
+  for (int d = 0; d < 2; d++)
+    if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
+      return CLAWHIP_E_ARG;
+  SharpRunBufs b;
+  b.q[0] = q0; b.q[1] = q1; b.comp[0] = comp0; b.comp[1] = comp1;
+  b.dqdt = dq_dt; b.dqdt_fb = dq_dt_full; b.r2 = r2; b.aux23 = aux23;
+  b.flags = flags; b.flag_list = flag_list; b.flag_count = flag_count; b.recon = recon;
+  b.params = mlp_params; b.flat = flat; b.threshold = logit_threshold;
+  b.pos_mask = positive_mask;
+  return launch_sharp_ssp33_run(*rp, dtype, char_decomp, to_geom(g), bc_lower, bc_upper, reflect,
+                               dx, dy, b, (SharpCtl*)ctl_dev, nsteps, (hipStream_t)stream);
 }
 
 int clawhip_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
@@ -404,8 +444,8 @@ int clawhip_sharp2_flux(const clawhip_rp* rp, int dtype, int char_decomp, const 
   if ((rp_num_aux(rp) > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
   return launch_sharp_stage(*rp, dtype, char_decomp, to_geom(gout), to_geom(gin), given, given,
                             refl, 0, 7, dt, 0.0, dx, dy, q, nullptr, nullptr, nullptr, nullptr,
-                            nullptr, dq, nullptr, nullptr, (unsigned long long*)cfl_dev, nullptr,
-                            nullptr, nullptr, (hipStream_t)stream, ax);
+                            nullptr, dq, nullptr, nullptr, (unsigned long long*)cfl_dev, FlagSet(),
+                            (hipStream_t)stream, ax);
 }
 
 int clawhip_mlp_param_count(void) { return 16 * 32 + 32 * 32 + 32 + 32 + 1; }
@@ -441,10 +481,10 @@ int clawhip_mlp_logits(const float* feat, long long n, const float* params, floa
   return launch_mlp_logits(feat, n, params, logits, (hipStream_t)stream);
 }
 
-int clawhip_mlp_troubled_cells(const unsigned char* flags, int mx, int my, int* cells,
-                               int* count, void* stream) {
-  if (!flags || !cells || !count || mx < 3 || my < 3) return CLAWHIP_E_ARG;
-  return launch_troubled_cells(flags, mx, my, cells, count, (hipStream_t)stream);
+int clawhip_mlp_flag_list(const unsigned char* flags, int mx, int my, int* list, int* count,
+                          void* stream) {
+  if (!flags || !list || !count || mx < 3 || my < 3) return CLAWHIP_E_ARG;
+  return launch_flag_list(flags, mx, my, list, count, (hipStream_t)stream);
 }
 
 int clawhip_mlp_flags(int dtype, const void* q, const clawhip_geom* g, const int* bc_lower,

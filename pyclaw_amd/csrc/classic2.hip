@@ -58,18 +58,10 @@ namespace {
 // overrides the choice (tuning sweeps).
 template <class Kernel>
 int choose_rows(Kernel kernel, int nstrips, int my) {
-  static long long resident = 0;   // per kernel instantiation
-  if (resident == 0) {
-    int dev = 0, ncu = 0, nb = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess ||
-        ncu < 1 || nb < 1)
-      return 16;
-    // beyond two wavefronts per SIMD the measured gain of a third is smaller
-    // than the halo rows the shorter marches add, so count at most two
-    resident = (long long)ncu * (nb < 2 ? nb : 2) * 4;
-  }
+  // beyond two wavefronts per SIMD the measured gain of a third is smaller than
+  // the halo rows the shorter marches add, so count at most two
+  const long long resident = resident_waves((const void*)kernel, 2);
+  if (resident == 0) return 16;
   if (const char* env = getenv("CLAWHIP_UNSPLIT_ROWS")) {
     int h = atoi(env);
     if (h >= 1 && h <= 4096) return h;

@@ -95,19 +95,12 @@ k_ens_step(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem)
 
 namespace {
 
-int ens_resident_waves() {
-  static int resident = 0;
-  if (resident == 0) {
-    int dev = 0, ncu = 0, nb = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, k_ens_step<Euler<float>, float, true>, 256, 0) != hipSuccess ||
-        ncu < 1 || nb < 1)
-      return 1024;
-    resident = ncu * (nb < 2 ? nb : 2) * 4;   // as the single-grid row chooser
-  }
-  return resident;
+// The table does not know the state's type, so the default rows follow the
+// occupancy of the FP32 + Kahan Euler step (the ensemble workload's); the
+// autotuner (Ensemble.autotune) replaces them by measured rows.
+long long ens_resident_waves() {
+  const long long r = resident_waves((const void*)k_ens_step<Euler<float>, float, true>, 2);
+  return r > 0 ? r : 1024;
 }
 
 long long ens_waves(const std::vector<EnsMember>& v, int H) {

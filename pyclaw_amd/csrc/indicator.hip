@@ -572,7 +572,11 @@ __device__ __forceinline__ void flag_stencils(const T* q, const Geom& g, const B
 template <typename T>
 __global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc, int meqn,
                                                    float flat, const float* master, float zthr,
-                                                   unsigned char* flags) {
+                                                   unsigned char* flags, const BufSel sel) {
+  if (sel.gate) {
+    if (!*sel.gate) return;
+    q = (const T*)sel.q[*sel.cur];
+  }
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int stride = gridDim.x * 4;
@@ -637,7 +641,11 @@ __global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc
 template <typename T, int ME>
 __global__ void __launch_bounds__(256) k_mlp_flags_row(const T* q, Geom g, BCSpec bc, float flat,
                                                        const float* master, float zthr,
-                                                       unsigned char* flags) {
+                                                       unsigned char* flags, const BufSel sel) {
+  if (sel.gate) {
+    if (!*sel.gate) return;
+    q = (const T*)sel.q[*sel.cur];
+  }
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int stride = gridDim.x * 4;
@@ -700,53 +708,6 @@ __global__ void __launch_bounds__(256) k_mlp_flags_row(const T* q, Geom g, BCSpe
   }
 }
 
-// ---------------------------------------------------------------- troubled cells
-__global__ void k_zero_count(int* count) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0;
-}
-
-// Cells with a flagged interface among their four, with the flags dilated by one
-// interface along the normal as the stage applies them (x-interfaces c-1..c+2 of
-// row r, y-interfaces r-1..r+2 of column c); appended to `cells` with one atomic
-// per wavefront.
-__global__ void __launch_bounds__(256) k_troubled_cells(const unsigned char* f, int mx, int my,
-                                                        int* cells, int* count) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  bool t = false;
-  if (i < (long long)mx * my) {
-    const int c = 1 + (int)(i % mx), r = 1 + (int)(i / mx);
-    const unsigned char* fx = f + (long long)(r - 1) * (mx + 1);
-#pragma unroll
-    for (int k = -1; k <= 2; k++) {
-      const int kk = min(max(c + k, 1), mx + 1);
-      t = t || fx[kk - 1];
-    }
-    const unsigned char* fy = f + (long long)(mx + 1) * my + (c - 1);
-#pragma unroll
-    for (int k = -1; k <= 2; k++) {
-      const int kk = min(max(r + k, 1), my + 1);
-      t = t || fy[(long long)(kk - 1) * mx];
-    }
-  }
-  const unsigned long long mask = __ballot(t);
-  const int n = __popcll(mask);
-  int base = 0;
-  if (lane == 0 && n > 0) base = atomicAdd(count, n);
-  base = __shfl(base, 0, 64);
-  if (t) cells[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
-}
-
-int launch_troubled_cells(const unsigned char* flags, int mx, int my, int* cells, int* count,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, s, count);
-  const long long n = (long long)mx * my;
-  hipLaunchKernelGGL(k_troubled_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags,
-                     mx, my, cells, count);
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : (int)e;
-}
-
 // ---------------------------------------------------------------- launchers
 int launch_features_labels(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
                            const int* refl, int meqn, float flat, float wtol, float* feat,
@@ -793,7 +754,7 @@ int launch_mlp_logits(const float* feat, long long n, const float* master, float
 
 int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
                      const int* refl, int meqn, float flat, const float* master, float zthr,
-                     unsigned char* flags, hipStream_t s) {
+                     unsigned char* flags, hipStream_t s, const BufSel& sel) {
   BCSpec bc = to_bc(lo, hi, refl, 2);
   if (meqn < 1 || meqn > 64) return CLAWHIP_E_ARG;
   const int ni = 64 / meqn;
@@ -817,26 +778,26 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
     if (dtype == CLAWHIP_F64) {
       const double* qd = (const double*)q;
       switch (meqn) {
-        case 1: hipLaunchKernelGGL((k_mlp_flags_row<double, 1>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
-        case 2: hipLaunchKernelGGL((k_mlp_flags_row<double, 2>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
-        case 3: hipLaunchKernelGGL((k_mlp_flags_row<double, 3>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
-        default: hipLaunchKernelGGL((k_mlp_flags_row<double, 4>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags); break;
+        case 1: hipLaunchKernelGGL((k_mlp_flags_row<double, 1>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags, sel); break;
+        case 2: hipLaunchKernelGGL((k_mlp_flags_row<double, 2>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags, sel); break;
+        case 3: hipLaunchKernelGGL((k_mlp_flags_row<double, 3>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags, sel); break;
+        default: hipLaunchKernelGGL((k_mlp_flags_row<double, 4>), grid, dim3(256), 0, s, qd, g, bc, flat, master, zthr, flags, sel); break;
       }
     } else {
       const float* qf = (const float*)q;
       switch (meqn) {
-        case 1: hipLaunchKernelGGL((k_mlp_flags_row<float, 1>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
-        case 2: hipLaunchKernelGGL((k_mlp_flags_row<float, 2>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
-        case 3: hipLaunchKernelGGL((k_mlp_flags_row<float, 3>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
-        default: hipLaunchKernelGGL((k_mlp_flags_row<float, 4>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags); break;
+        case 1: hipLaunchKernelGGL((k_mlp_flags_row<float, 1>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags, sel); break;
+        case 2: hipLaunchKernelGGL((k_mlp_flags_row<float, 2>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags, sel); break;
+        case 3: hipLaunchKernelGGL((k_mlp_flags_row<float, 3>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags, sel); break;
+        default: hipLaunchKernelGGL((k_mlp_flags_row<float, 4>), grid, dim3(256), 0, s, qf, g, bc, flat, master, zthr, flags, sel); break;
       }
     }
   } else if (dtype == CLAWHIP_F64) {
     hipLaunchKernelGGL(k_mlp_flags<double>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       (const double*)q, g, bc, meqn, flat, master, zthr, flags);
+                       (const double*)q, g, bc, meqn, flat, master, zthr, flags, sel);
   } else {
     hipLaunchKernelGGL(k_mlp_flags<float>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       (const float*)q, g, bc, meqn, flat, master, zthr, flags);
+                       (const float*)q, g, bc, meqn, flat, master, zthr, flags, sel);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;

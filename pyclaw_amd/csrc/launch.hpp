@@ -5,6 +5,38 @@
 #include "../../include/clawhip.h"
 #include "common.hpp"
 #include "stepctl.hpp"
+#include "sharpctl.hpp"
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+namespace claw {
+// Wavefronts of `kernel` (launched in 256-thread blocks) resident on the current
+// device at once, from its occupancy, counting at most per_simd_cap per SIMD
+// (0: no cap); cached per (device, kernel address), so every instantiation gets
+// its own figure.  0 when the runtime cannot tell.
+inline long long resident_waves(const void* kernel, int per_simd_cap) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, long long> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(dev, kernel);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int ncu = 0, nb = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess ||
+      ncu < 1 || nb < 1)
+    return 0;
+  // nb blocks of 4 wavefronts per CU = nb wavefronts per SIMD
+  const int per_simd = per_simd_cap > 0 && nb > per_simd_cap ? per_simd_cap : nb;
+  const long long r = (long long)ncu * per_simd * 4;
+  cache[key] = r;
+  return r;
+}
+}  // namespace claw
 
 namespace claw {
 
@@ -53,13 +85,58 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
                   int ng, const int* method, const int* mthlim, double dx, double dy,
                   double dz, double dt, unsigned long long* cfl, hipStream_t s);
 
+// A kernel of a device-controlled run: it runs only while *gate != 0 and reads
+// the state buffer q[*cur] (gate NULL: an eager launch on the given pointer).
+struct BufSel {
+  const int* gate = nullptr;
+  const int* cur = nullptr;
+  const void* q[2] = {nullptr, nullptr};
+};
+
+// The learned indicator's selection for a SharpClaw stage: per-interface flags
+// (NULL: full WENO5 everywhere), the ids of the interfaces whose dilated flag is
+// set and their number (device), and the buffer their full-WENO5 edge values go
+// to (2*meqn values of the state's type per interface id).
+struct FlagSet {
+  const unsigned char* flags = nullptr;
+  const int* list = nullptr;
+  const int* count = nullptr;
+  void* recon = nullptr;
+};
+
+// The buffers of a device-controlled SSP33 run: the state and its Kahan
+// compensation (ping-pong by SharpCtl::cur), dq_dt, the second stage's state
+// and FP32 increment, and -- on the indicator path -- the flags, their list, the
+// full-WENO5 edge values, the network and the fields the judge keeps positive.
+struct SharpRunBufs {
+  void* q[2] = {nullptr, nullptr};
+  void* comp[2] = {nullptr, nullptr};
+  void* dqdt = nullptr;
+  void* dqdt_fb = nullptr;
+  void* r2 = nullptr;
+  void* aux23 = nullptr;
+  unsigned char* flags = nullptr;
+  int* flag_list = nullptr;
+  int* flag_count = nullptr;
+  void* recon = nullptr;
+  const float* params = nullptr;
+  float flat = 0.0f, threshold = 0.0f;
+  unsigned pos_mask = 0u;
+};
+
+int launch_sharp_ssp33_run(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
+                          const int* lo, const int* hi, const int* refl, double dx, double dy,
+                          const SharpRunBufs& b, SharpCtl* ctl, int nsteps,
+                          hipStream_t stream);
+int launch_flag_list(const unsigned char* flags, int mx, int my, int* list, int* count,
+                     hipStream_t s, const int* gate = nullptr);
 int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
                        const Geom& gin, const int* lo, const int* hi, const int* refl, int ymode, int omode,
                        double dt, double s, double dx, double dy, const void* P, const void* Q,
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
-                       const unsigned char* flags, const int* cells, const int* ncells,
-                       hipStream_t stream, const AuxIn& aux, const void* src = nullptr);
+                       const FlagSet& flags, hipStream_t stream, const AuxIn& aux,
+                       const void* src = nullptr);
 int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream);
 int launch_lmm_update(int dtype, int mode, long long n, const void* q, const void* u,
                       const void* R, const void* v, const double* c, void* out,
@@ -92,11 +169,9 @@ int launch_mlp_train(const float* feat, long long nsamples, const unsigned char*
                      float* m2, int step, float lr, float pos_weight, float* loss, hipStream_t s);
 int launch_mlp_logits(const float* feat, long long n, const float* master, float* logits,
                       hipStream_t s);
-int launch_troubled_cells(const unsigned char* flags, int mx, int my, int* cells, int* count,
-                          hipStream_t s);
 int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, const int* hi,
                      const int* refl, int meqn, float flat, const float* master, float zthr,
-                     unsigned char* flags, hipStream_t s);
+                     unsigned char* flags, hipStream_t s, const BufSel& sel = BufSel());
 int launch_state_sums(int dtype, const void* q, const Geom& g, int meqn, unsigned positive_mask,
                       double* out, double* work, hipStream_t s);
 int launch_state_sums_noalloc(int dtype, const void* q, const Geom& g, int meqn,

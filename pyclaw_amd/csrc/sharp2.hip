@@ -14,12 +14,13 @@
 // k_sharp_march computes a stage in one pass over HBM: each wavefront marches a
 // 64-column strip down its rows (details below), and the stage combination of
 // SharpClawSolver.step (sharpclaw/solver.py:312-323, 400-430) is applied as each
-// cell's dq is complete.  With the learned indicator's flags, k_sharp_march
-// takes the linear weights everywhere and k_sharp_patch recomputes the cells
-// next to a flagged interface.
+// cell's dq is complete.  With the learned indicator's flags, each interface
+// takes WENO5's linear weights unless its (dilated) flag is set; a flagged
+// interface takes the full WENO5, in the lanes that hold one.
 #include <stdlib.h>
 
 #include "launch.hpp"
+#include "sharpctl.hpp"
 
 namespace claw {
 
@@ -78,8 +79,20 @@ struct SharpArgs {
   const T* src;         // optional source increment added to D (SharpClaw dq_src)
   unsigned long long* cfl_bits;
   const unsigned char* flags;   // troubled-interface flags (learned indicator) or null
-  const int* cells;             // with flags: cells next to a flagged interface
-  const int* ncells;            //   and their number (device)
+  const int* flag_list;         // with flags: ids of the interfaces whose dilated flag is set
+  const int* flag_count;        //   and their number (device)
+  T* recon;                     //   their full-WENO5 edge values, 2*meqn per interface id
+  // device-controlled stage (ctl non-null): it runs while *gate != 0, takes dt
+  // from ctl and the pointers named in `res` from the state's two buffers by
+  // ctl->cur; its last wavefront applies phase ctl_phase; with count_bad it
+  // counts the stored cells whose fields in pos_mask are not > 0 (or not finite)
+  SharpCtl* ctl;
+  const int* gate;
+  int ctl_phase, count_bad;
+  unsigned res, pos_mask;
+  void* qb[2];
+  void* cb[2];
+  double dx, dy;
   const T* qaux;                // the solver's aux fields (solvers with maux > 0), geometry
   Geom gaux;                    //   gaux, boundary codes bcaux (never negated)
   BCSpec bcaux;
@@ -290,41 +303,136 @@ __device__ __forceinline__ void recon_interface(const RPParams<T>& p,
   }
 }
 
+// Pointers a controlled stage takes from the state's buffers (SharpArgs::res):
+// q[cur] as P, q[cur] as A, q[1-cur] as out, comp[cur] as comp_in,
+// comp[1-cur] as comp_out.
+enum { RES_P = 1, RES_A = 2, RES_OUT = 4, RES_CIN = 8, RES_COUT = 16 };
+
+// What a stage takes from its SharpCtl when it is device-controlled: dt (and s,
+// dtdx, dtdy as the eager launch forms them on the host: T(dt / dx)) and the
+// pointers named in SharpArgs::res.  A few uniform values, kept apart from the
+// (large, constant) argument block.
+template <typename T>
+struct StageView {
+  const T* P;
+  const T* A;
+  T* out;
+  const T* comp_in;
+  T* comp_out;
+  T dt, s, dtdx, dtdy;
+};
+
+template <typename T>
+__device__ __forceinline__ StageView<T> stage_view(const SharpArgs<T>& a) {
+  StageView<T> v{a.P, a.A, a.out, a.comp_in, a.comp_out, a.dt, a.s, a.dtdx, a.dtdy};
+  if (a.ctl) {
+    const int cur = a.ctl->cur;
+    const double dt = a.ctl->dt;
+    v.dt = T(dt);
+    v.s = T(dt);
+    v.dtdx = T(dt / a.dx);
+    v.dtdy = T(dt / a.dy);
+    if (a.res & RES_P) v.P = (const T*)a.qb[cur];
+    if (a.res & RES_A) v.A = (const T*)a.qb[cur];
+    if (a.res & RES_OUT) v.out = (T*)a.qb[1 - cur];
+    if (a.res & RES_CIN) v.comp_in = (const T*)a.cb[cur];
+    if (a.res & RES_COUT) v.comp_out = (T*)a.cb[1 - cur];
+  }
+  return v;
+}
+
+// The inputs a cell's stage combination reads besides D and Y, loaded at the top
+// of the march iteration that stores the cell (so that their latency overlaps
+// the row's reconstruction and Riemann solves): pa = A; pb = B (OUT_F32_S2) or
+// aux_in (OUT_F32_S3); pc = comp_in (OUT_F32_S3).
+template <int ME, typename T>
+struct StoreIn {
+  T pa[ME], pb[ME], pc[ME];
+};
+
+template <int ME, typename T>
+__device__ __forceinline__ void load_store_in(const SharpArgs<T>& a, const StageView<T>& v,
+                                              long long off, StoreIn<ME, T>& si) {
+  const long long fs = a.g.fs;
+  const int om = a.omode;
+  // uniform choices; each array is read whole or not at all
+  const T* pa = (om == OUT_DQDT || om == OUT_ADD6 || om == OUT_D) ? nullptr : v.A;
+  const T* pb = om == OUT_F32_S2 ? a.B : om == OUT_F32_S3 ? a.aux_in : nullptr;
+  const T* pc = om == OUT_F32_S3 ? v.comp_in : nullptr;
+#pragma unroll
+  for (int m = 0; m < ME; m++) {
+    si.pa[m] = pa ? pa[off + m * fs] : T(0);
+    si.pb[m] = pb ? pb[off + m * fs] : T(0);
+    si.pc[m] = pc ? pc[off + m * fs] : T(0);
+  }
+}
+
 // Stage combination of cell offset `off`, field m (SharpClawSolver.step,
 // sharpclaw/solver.py:312-323, 400-430; the FP32 stages keep a compensated sum).
-template <typename T>
-__device__ __forceinline__ void stage_store(const SharpArgs<T>& a, long long off, int m, T D,
-                                            T yv) {
+template <int ME, typename T>
+__device__ __forceinline__ T stage_store(const SharpArgs<T>& a, const StageView<T>& v,
+                                         const StoreIn<ME, T>& si, long long off, int m, T D,
+                                         T yv) {
   const long long fs = a.g.fs;
   // dq = dq_hyperbolic + dq_src (sharpclaw/solver.py:537-540)
   if (a.src) D = D + a.src[off + m * fs];
+  const int om = a.omode;
   T o;
-  switch (a.omode) {
-    case OUT_DQDT: o = D / a.dt; break;
-    case OUT_SSP33_2: o = T(0.75) * a.A[off + m * fs] + T(0.25) * (yv + D); break;
-    case OUT_SSP33_3: o = T(1. / 3.) * a.A[off + m * fs] + T(2. / 3.) * (yv + D); break;
-    case OUT_ADD6: o = yv + D / T(6); break;
-    case OUT_SSP104_FIN: o = a.A[off + m * fs] + (T(0.6) * yv + T(0.1) * D); break;
-    case OUT_D: o = D; break;
-    case OUT_SSP22: o = T(0.5) * ((a.A[off + m * fs] + yv) + D); break;
-    case OUT_F32_S2: {
-      T e1 = a.s * a.B[off + m * fs];
-      T inc = e1 + D;
-      o = a.A[off + m * fs] + T(0.25) * inc;
-      a.aux_out[off + m * fs] = inc * T(1. / 6.);
-      break;
-    }
-    default: {   // OUT_F32_S3
-      T delta = a.aux_in[off + m * fs] + T(2. / 3.) * D;
-      T qa = a.A[off + m * fs];
-      T yk = delta - a.comp_in[off + m * fs];
-      T tq = qa + yk;
-      a.comp_out[off + m * fs] = (tq - qa) - yk;
-      o = tq;
-      break;
-    }
+  if (om == OUT_DQDT) {
+    o = D / v.dt;
+  } else if (om == OUT_SSP33_2) {
+    o = T(0.75) * si.pa[m] + T(0.25) * (yv + D);
+  } else if (om == OUT_SSP33_3) {
+    o = T(1. / 3.) * si.pa[m] + T(2. / 3.) * (yv + D);
+  } else if (om == OUT_ADD6) {
+    o = yv + D / T(6);
+  } else if (om == OUT_SSP104_FIN) {
+    o = si.pa[m] + (T(0.6) * yv + T(0.1) * D);
+  } else if (om == OUT_D) {
+    o = D;
+  } else if (om == OUT_SSP22) {
+    o = T(0.5) * ((si.pa[m] + yv) + D);
+  } else if (om == OUT_F32_S2) {
+    T e1 = v.s * si.pb[m];
+    T inc = e1 + D;
+    o = si.pa[m] + T(0.25) * inc;
+    a.aux_out[off + m * fs] = inc * T(1. / 6.);
+  } else {   // OUT_F32_S3
+    T delta = si.pb[m] + T(2. / 3.) * D;
+    T qa = si.pa[m];
+    T yk = delta - si.pc[m];
+    T tq = qa + yk;
+    v.comp_out[off + m * fs] = (tq - qa) - yk;
+    o = tq;
   }
-  a.out[off + m * fs] = o;
+  v.out[off + m * fs] = o;
+  return o;
+}
+
+// The stage input Y of cell (c, r): P, P + s*Q or P + (s*Q)/6 (y_mode), with the
+// boundary map applied to both.
+template <int ME, typename T>
+__device__ __forceinline__ void stage_input(const SharpArgs<T>& a, const StageView<T>& v, int c,
+                                            int r, T (&y)[ME]) {
+  load_cell2<ME>(v.P, a.gin, a.bc, a.ng, c, r, y);
+  if (a.ymode != Y_ARRAY) {
+    T qv[ME];
+    load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, qv);
+#pragma unroll
+    for (int m = 0; m < ME; m++)
+      y[m] = a.ymode == Y_AXPY ? y[m] + v.s * qv[m] : y[m] + (v.s * qv[m]) / T(6);
+  }
+}
+
+// Interface ids (the layout of the indicator's flags): x-interface c (left face
+// of cell c, c = 1..mx+1) of row r (1..my) is (r-1)*(mx+1) + c-1; y-interface t
+// (lower face of row t, t = 1..my+1) of column c (1..mx) follows them at
+// (mx+1)*my + (t-1)*mx + c-1.
+__device__ __forceinline__ long long xface_id(int mx, int c, int r) {
+  return (long long)(r - 1) * (mx + 1) + (c - 1);
+}
+__device__ __forceinline__ long long yface_id(int mx, int my, int c, int t) {
+  return (long long)(mx + 1) * my + (long long)(t - 1) * mx + (c - 1);
 }
 
 // Flags of the interfaces (dilated by one interface along the normal).
@@ -336,6 +444,107 @@ __device__ __forceinline__ bool yflag(const unsigned char* f, int mx, int my, in
   const unsigned char* fc = f + (long long)(mx + 1) * my + (c - 1);
   return (fc[(long long)(t - 1) * mx] | fc[(long long)max(t - 2, 0) * mx] |
           fc[(long long)min(t, my) * mx]) != 0;
+}
+
+template <int ME, typename T>
+__device__ __forceinline__ void load_recon(const T* p, T (&qrL)[ME], T (&qlR)[ME]) {
+#pragma unroll
+  for (int m = 0; m < ME; m++) {
+    qrL[m] = p[m];
+    qlR[m] = p[ME + m];
+  }
+}
+
+// ---------------------------------------------------------------- flagged interfaces
+// The ids of the interfaces whose dilated flag is set (xflag / yflag), appended
+// to `list` with one atomic per 1024-thread block (a single counter taking one
+// atomic per wavefront serialised the pass); the order is free, each entry is
+// computed on its own.  *count must be 0 on entry (k_zero_flag_count).
+__global__ void k_zero_flag_count(int* count, const int* gate) {
+  if (gate && !*gate) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0;
+}
+
+constexpr int FLAG_LIST_THREADS = 1024;
+
+__global__ void __launch_bounds__(FLAG_LIST_THREADS) k_flag_list(const unsigned char* f, int mx,
+                                                                 int my, int* list, int* count,
+                                                                 const int* gate) {
+  if (gate && !*gate) return;
+  __shared__ int wave_n[FLAG_LIST_THREADS / 64];
+  __shared__ int block_base;
+  const long long nx = (long long)(mx + 1) * my, n = nx + (long long)mx * (my + 1);
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  bool t = false;
+  if (i < nx) {
+    t = xflag(f, mx, 1 + (int)(i % (mx + 1)), 1 + (int)(i / (mx + 1)));
+  } else if (i < n) {
+    const long long j = i - nx;
+    t = yflag(f, mx, my, 1 + (int)(j % mx), 1 + (int)(j / mx));
+  }
+  const unsigned long long mask = __ballot(t);
+  if (lane == 0) wave_n[wave] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < FLAG_LIST_THREADS / 64; w++) {
+      const int k = wave_n[w];
+      wave_n[w] = tot;   // exclusive prefix
+      tot += k;
+    }
+    block_base = tot > 0 ? atomicAdd(count, tot) : 0;
+  }
+  __syncthreads();
+  if (t) list[block_base + wave_n[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+}
+
+// The full WENO5 (characteristic or component, as configured) of every listed
+// interface of the stage input, stored at its id: the right edge of the cell
+// before it (qrL), then the left edge of the cell after it (qlR) -- the values
+// recon_interface gives k_sharp_march for the same stencil.  One thread per
+// listed interface, so the cost follows the flagged fraction; the list length
+// is read on the device.
+template <class RS, typename T, bool CHAR>
+__global__ void __launch_bounds__(256) k_weno_flagged(const SharpArgs<T> a) {
+  constexpr int ME = RS::meqn;
+  if (a.ctl && !*a.gate) return;
+  const StageView<T> v = stage_view(a);
+  const int n = *a.flag_count;
+  const int mx = a.g.mx, my = a.g.my;
+  const long long nx = (long long)(mx + 1) * my;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const long long id = a.flag_list[i];
+    T st[6][ME], qrL[ME], qlR[ME];
+    if (id < nx) {
+      const int c = 1 + (int)(id % (mx + 1)), r = 1 + (int)(id / (mx + 1));
+#pragma unroll
+      for (int k = 0; k < 6; k++) stage_input<ME>(a, v, c - 3 + k, r, st[k]);
+      recon_interface<RS, 1, T, CHAR>(a.rp, st, qrL, qlR);
+    } else {
+      const long long j = id - nx;
+      const int c = 1 + (int)(j % mx), t = 1 + (int)(j / mx);
+#pragma unroll
+      for (int k = 0; k < 6; k++) stage_input<ME>(a, v, c, t - 3 + k, st[k]);
+      recon_interface<RS, 2, T, CHAR>(a.rp, st, qrL, qlR);
+    }
+    T* out = a.recon + id * (2 * ME);
+#pragma unroll
+    for (int m = 0; m < ME; m++) {
+      out[m] = qrL[m];
+      out[ME + m] = qlR[m];
+    }
+  }
+}
+
+int launch_flag_list(const unsigned char* flags, int mx, int my, int* list, int* count,
+                     hipStream_t s, const int* gate) {
+  hipLaunchKernelGGL(k_zero_flag_count, dim3(1), dim3(64), 0, s, count, gate);
+  const long long n = (long long)(mx + 1) * my + (long long)mx * (my + 1);
+  hipLaunchKernelGGL(k_flag_list, dim3((unsigned)((n + FLAG_LIST_THREADS - 1) / FLAG_LIST_THREADS)),
+                     dim3(FLAG_LIST_THREADS), 0, s, flags, mx, my, list, count, gate);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
 }
 
 // ---------------------------------------------------------------- march form
@@ -373,6 +582,8 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   constexpr int MA = AUX ? RS::maux : 1;
   constexpr bool TFLUCT = sizeof(T) == 4 && !AUX;
   const SharpArgs<T>& a = ma.s;
+  if (a.ctl && !*a.gate) return;   // uniform over the launch
+  const StageView<T> v = stage_view(a);
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wid >= ma.nstrips * ma.nrb) return;   // uniform per wavefront
@@ -387,15 +598,21 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   const bool ycfl_lane = c >= 0 && c <= mx + 1;
   const RPParams<T>& rp = a.rp;
   T smax_x = T(0), smax_y = T(0);
+  int nbad = 0;   // stored cells that lost positivity (count_bad)
 
-  auto load_y = [&](int r, T (&y)[ME]) {
-    load_cell2<ME>(a.P, a.gin, a.bc, a.ng, c, r, y);
-    if (a.ymode != Y_ARRAY) {
-      T qv[ME];
-      load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, qv);
+  auto load_y = [&](int r, T (&y)[ME]) { stage_input<ME>(a, v, c, r, y); };
+  // a window row in two parts: its loads (issued one iteration ahead) and the
+  // stage-input combination stage_input applies (the same expression)
+  auto load_raw = [&](int r, T (&p)[ME], T (&q)[ME]) {
+    load_cell2<ME>(v.P, a.gin, a.bc, a.ng, c, r, p);
+    if (a.ymode != Y_ARRAY) load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, q);
+  };
+  auto combine = [&](const T (&p)[ME], const T (&q)[ME], T (&y)[ME]) {
 #pragma unroll
-      for (int m = 0; m < ME; m++)
-        y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
+    for (int m = 0; m < ME; m++) {
+      y[m] = p[m];
+      if (a.ymode != Y_ARRAY)
+        y[m] = a.ymode == Y_AXPY ? y[m] + v.s * q[m] : y[m] + (v.s * q[m]) / T(6);
     }
   };
   auto load_a = [&](int r, T (&A)[MA]) {
@@ -415,12 +632,13 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     from_left<ME>(st[1], st[0]);
     from_right<ME>(st[3], st[4]);
     from_right<ME>(st[4], st[5]);
-    // flagged path: every interface takes the linear weights here, and the
-    // cells next to a flagged interface are recomputed by k_sharp_patch
-    bool counted = true;
+    // flagged path: the linear weights, or, where the interface's dilated flag
+    // is set, the full WENO5 edge values k_weno_flagged stored (interfaces of
+    // ghost rows enter only the CFL and take the linear weights)
     if (FLAGS) {
-      if (c >= 1 && c <= mx + 1 && r >= 1 && r <= my) counted = !xflag(a.flags, mx, c, r);
       recon_linear<ME>(st, qrL, qlR);
+      if (c >= 1 && c <= mx + 1 && r >= 1 && r <= my && xflag(a.flags, mx, c, r))
+        load_recon<ME>(a.recon + xface_id(mx, c, r) * (2 * ME), qrL, qlR);
     } else {
       recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
     }
@@ -429,7 +647,7 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     T AL[MA];
     if constexpr (AUX) from_left<MA>(Ar, AL);
     rpn_call<RS, 1>(rp, qrL, qlR, AL, Ar, W, sp, am, ap, roe);
-    if (xint_lane && counted) {
+    if (xint_lane) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax_x = cfl_term(smax_x, (T)fabs(sp[w]));
     }
@@ -438,17 +656,17 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   // window win[0..5] = rows t-3..t+2
   auto y_interface = [&](const T (&win)[6][ME], const T (&Alo)[MA], const T (&Ahi)[MA], int t,
                          T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
-    bool counted = true;
     if (FLAGS) {
-      if (c >= 1 && c <= mx && t <= my + 1) counted = !yflag(a.flags, mx, my, c, t);
       recon_linear<ME>(win, qrL, qlR);
+      if (c >= 1 && c <= mx && t <= my + 1 && yflag(a.flags, mx, my, c, t))
+        load_recon<ME>(a.recon + yface_id(mx, my, c, t) * (2 * ME), qrL, qlR);
     } else {
       recon_interface<RS, 2, T, CHAR>(rp, win, qrL, qlR);
     }
     T W[MW][ME], sp[MW];
     typename RS::Roe roe;
     rpn_call<RS, 2>(rp, qrL, qlR, Alo, Ahi, W, sp, am, ap, roe);
-    if (ycfl_lane && t <= my + 1 && counted) {
+    if (ycfl_lane && t <= my + 1) {
 #pragma unroll
       for (int w = 0; w < MW; w++) smax_y = cfl_term(smax_y, (T)fabs(sp[w]));
     }
@@ -465,14 +683,14 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
       RS::template flux<1>(rp, qlR, fl);
       RS::template flux<1>(rp, qr, fr);
 #pragma unroll
-      for (int m = 0; m < ME; m++) dqx[m] = T(0) - a.dtdx * ((amn[m] + ap[m]) + (fr[m] - fl[m]));
+      for (int m = 0; m < ME; m++) dqx[m] = T(0) - v.dtdx * ((amn[m] + ap[m]) + (fr[m] - fl[m]));
     } else {
       T W[MW][ME], sp[MW], am2[ME], ap2[ME];
       typename RS::Roe roe;
       rpn_call<RS, 1>(rp, qlR, qr, Ar, Ar, W, sp, am2, ap2, roe);
 #pragma unroll
       for (int m = 0; m < ME; m++)
-        dqx[m] = T(0) - a.dtdx * (((amn[m] + ap[m]) + am2[m]) + ap2[m]);
+        dqx[m] = T(0) - v.dtdx * (((amn[m] + ap[m]) + am2[m]) + ap2[m]);
     }
   };
 
@@ -492,13 +710,22 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     T qrL[ME], am[ME];
     y_interface(win, Alo, Ahi, r0, qrL, qlY, am, apY);
   }
+  T pf_p[ME], pf_q[ME];   // the loads of the next iteration's window row
+#pragma unroll
+  for (int m = 0; m < ME; m++) pf_q[m] = T(0);
+  load_raw(r0 + 3, pf_p, pf_q);
   for (int t = r0 + 1; t <= rend + 1; t++) {
     // row t-1 = win[3]; shift the window to rows t-3..t+2
 #pragma unroll
     for (int k = 0; k < 5; k++)
 #pragma unroll
       for (int m = 0; m < ME; m++) win[k][m] = win[k + 1][m];
-    load_y(t + 2, win[5]);
+    combine(pf_p, pf_q, win[5]);
+    if (t <= rend) load_raw(t + 3, pf_p, pf_q);
+    // the store-time inputs of row t-1
+    StoreIn<ME, T> si;
+    const long long off = (long long)(c - 1) + (long long)(t - 2) * a.g.rs;
+    if (out_lane) load_store_in<ME>(a, v, off, si);
 #pragma unroll
     for (int k = 0; k < MA; k++) Alo[k] = Ahi[k];
     load_a(t, Ahi);
@@ -511,23 +738,26 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
       RS::template flux<2>(rp, qlY, fl);
       RS::template flux<2>(rp, qrL, fr);
 #pragma unroll
-      for (int m = 0; m < ME; m++) dqy[m] = T(0) - a.dtdy * ((am[m] + apY[m]) + (fr[m] - fl[m]));
+      for (int m = 0; m < ME; m++) dqy[m] = T(0) - v.dtdy * ((am[m] + apY[m]) + (fr[m] - fl[m]));
     } else {
       T W[MW][ME], sp[MW], am2[ME], ap2[ME];
       typename RS::Roe roe;
       rpn_call<RS, 2>(rp, qlY, qrL, Alo, Alo, W, sp, am2, ap2, roe);
 #pragma unroll
       for (int m = 0; m < ME; m++)
-        dqy[m] = T(0) - a.dtdy * (((am[m] + apY[m]) + am2[m]) + ap2[m]);
+        dqy[m] = T(0) - v.dtdy * (((am[m] + apY[m]) + am2[m]) + ap2[m]);
     }
     // x-slice of row t-1
     T dqx[ME];
     x_cell(win[2], Alo, t - 1, dqx);
-    const int r = t - 1;
     if (out_lane) {
-      const long long off = (long long)(c - 1) + (long long)(r - 1) * a.g.rs;
+      bool bad = false;
 #pragma unroll
-      for (int m = 0; m < ME; m++) stage_store(a, off, m, (T(0) + dqx[m]) + dqy[m], win[2][m]);
+      for (int m = 0; m < ME; m++) {
+        const T o = stage_store<ME>(a, v, si, off, m, (T(0) + dqx[m]) + dqy[m], win[2][m]);
+        bad = bad || (((a.pos_mask >> m) & 1u) && (!(o > T(0)) || !isfinite(o)));
+      }
+      nbad += bad ? 1 : 0;
     }
 #pragma unroll
     for (int m = 0; m < ME; m++) { qlY[m] = qlR[m]; apY[m] = ap[m]; }
@@ -536,142 +766,20 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     T d[ME];
     x_cell(win[3], Ahi, my + 1, d);
   }
-  double cx = (double)a.dtdx * (double)smax_x;
-  double cy = (double)a.dtdy * (double)smax_y;
+  double cx = (double)v.dtdx * (double)smax_x;
+  double cy = (double)v.dtdy * (double)smax_y;
   wave_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
+  if (a.ctl && a.count_bad) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nbad += __shfl_xor(nbad, off, 64);
+    if (lane == 0 && nbad > 0) atomicAdd(&a.ctl->nbad, (unsigned)nbad);
+  }
 }
 
-// Flagged path, second pass: every cell next to a flagged interface (listed by
-// clawhip_mlp_troubled_cells) is recomputed with the characteristic (or
-// component) WENO5 on its flagged interfaces and the linear weights on the
-// others -- per interface and per cell the arithmetic of k_sharp_march -- and
-// its stage output is rewritten.  The speeds of the flagged interfaces enter
-// the CFL here.  One thread per listed cell; the list length is read on the
-// device, so the launch needs no host synchronisation.
-template <class RS, typename T, bool CHAR>
-__global__ void __launch_bounds__(256) k_sharp_patch(const SharpArgs<T> a, const int* cells,
-                                                     const int* count) {
-  constexpr int ME = RS::meqn, MW = RS::mwaves;
-  constexpr bool TFLUCT = sizeof(T) == 4;
-  const int n = *count;
-  const int mx = a.g.mx, my = a.g.my;
-  const RPParams<T>& rp = a.rp;
-  T smax_x = T(0), smax_y = T(0);
-  auto load_y = [&](int c, int r, T (&y)[ME]) {
-    load_cell2<ME>(a.P, a.gin, a.bc, a.ng, c, r, y);
-    if (a.ymode != Y_ARRAY) {
-      T qv[ME];
-      load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, qv);
-#pragma unroll
-      for (int m = 0; m < ME; m++)
-        y[m] = a.ymode == Y_AXPY ? y[m] + a.s * qv[m] : y[m] + (a.s * qv[m]) / T(6);
-    }
-  };
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int cell = cells[i];
-    const int c = 1 + cell % mx, r = 1 + cell / mx;
-    T X[7][ME];   // cells c-3..c+3 of row r
-    T Y[7][ME];   // rows r-3..r+3 of column c
-#pragma unroll
-    for (int k = 0; k < 7; k++) load_y(c - 3 + k, r, X[k]);
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-      if (k == 3) {
-#pragma unroll
-        for (int m = 0; m < ME; m++) Y[3][m] = X[3][m];
-      } else {
-        load_y(c, r - 3 + k, Y[k]);
-      }
-    }
-    T dqx[ME], dqy[ME];
-    {
-      // x-interfaces c (stencil X[0..5]) and c+1 (X[1..6])
-      T qrL0[ME], qlR0[ME], am0[ME], ap0[ME], qrL1[ME], qlR1[ME], am1[ME], ap1[ME];
-      for (int side = 0; side < 2; side++) {
-        T st[6][ME];
-#pragma unroll
-        for (int k = 0; k < 6; k++)
-#pragma unroll
-          for (int m = 0; m < ME; m++) st[k][m] = X[k + side][m];
-        const bool weno = xflag(a.flags, mx, c + side, r);
-        T qrL[ME], qlR[ME], W[MW][ME], sp[MW], am[ME], ap[ME];
-        if (weno) recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
-        else recon_linear<ME>(st, qrL, qlR);
-        typename RS::Roe roe;
-        RS::template rpn<1>(rp, qrL, qlR, W, sp, am, ap, roe);
-        if (weno) {
-#pragma unroll
-          for (int w = 0; w < MW; w++) smax_x = cfl_term(smax_x, (T)fabs(sp[w]));
-        }
-#pragma unroll
-        for (int m = 0; m < ME; m++) {
-          if (side == 0) { qrL0[m] = qrL[m]; qlR0[m] = qlR[m]; am0[m] = am[m]; ap0[m] = ap[m]; }
-          else { qrL1[m] = qrL[m]; qlR1[m] = qlR[m]; am1[m] = am[m]; ap1[m] = ap[m]; }
-        }
-      }
-      if (TFLUCT) {
-        T fl[ME], fr[ME];
-        RS::template flux<1>(rp, qlR0, fl);
-        RS::template flux<1>(rp, qrL1, fr);
-#pragma unroll
-        for (int m = 0; m < ME; m++)
-          dqx[m] = T(0) - a.dtdx * ((am1[m] + ap0[m]) + (fr[m] - fl[m]));
-      } else {
-        T W[MW][ME], sp[MW], am2[ME], ap2[ME];
-        typename RS::Roe roe;
-        RS::template rpn<1>(rp, qlR0, qrL1, W, sp, am2, ap2, roe);
-#pragma unroll
-        for (int m = 0; m < ME; m++)
-          dqx[m] = T(0) - a.dtdx * (((am1[m] + ap0[m]) + am2[m]) + ap2[m]);
-      }
-    }
-    {
-      T qrL0[ME], qlR0[ME], am0[ME], ap0[ME], qrL1[ME], qlR1[ME], am1[ME], ap1[ME];
-      for (int side = 0; side < 2; side++) {
-        T st[6][ME];
-#pragma unroll
-        for (int k = 0; k < 6; k++)
-#pragma unroll
-          for (int m = 0; m < ME; m++) st[k][m] = Y[k + side][m];
-        const bool weno = yflag(a.flags, mx, my, c, r + side);
-        T qrL[ME], qlR[ME], W[MW][ME], sp[MW], am[ME], ap[ME];
-        if (weno) recon_interface<RS, 2, T, CHAR>(rp, st, qrL, qlR);
-        else recon_linear<ME>(st, qrL, qlR);
-        typename RS::Roe roe;
-        RS::template rpn<2>(rp, qrL, qlR, W, sp, am, ap, roe);
-        if (weno) {
-#pragma unroll
-          for (int w = 0; w < MW; w++) smax_y = cfl_term(smax_y, (T)fabs(sp[w]));
-        }
-#pragma unroll
-        for (int m = 0; m < ME; m++) {
-          if (side == 0) { qrL0[m] = qrL[m]; qlR0[m] = qlR[m]; am0[m] = am[m]; ap0[m] = ap[m]; }
-          else { qrL1[m] = qrL[m]; qlR1[m] = qlR[m]; am1[m] = am[m]; ap1[m] = ap[m]; }
-        }
-      }
-      if (TFLUCT) {
-        T fl[ME], fr[ME];
-        RS::template flux<2>(rp, qlR0, fl);
-        RS::template flux<2>(rp, qrL1, fr);
-#pragma unroll
-        for (int m = 0; m < ME; m++)
-          dqy[m] = T(0) - a.dtdy * ((am1[m] + ap0[m]) + (fr[m] - fl[m]));
-      } else {
-        T W[MW][ME], sp[MW], am2[ME], ap2[ME];
-        typename RS::Roe roe;
-        RS::template rpn<2>(rp, qlR0, qrL1, W, sp, am2, ap2, roe);
-#pragma unroll
-        for (int m = 0; m < ME; m++)
-          dqy[m] = T(0) - a.dtdy * (((am1[m] + ap0[m]) + am2[m]) + ap2[m]);
-      }
-    }
-    const long long off = (long long)(c - 1) + (long long)(r - 1) * a.g.rs;
-#pragma unroll
-    for (int m = 0; m < ME; m++) stage_store(a, off, m, (T(0) + dqx[m]) + dqy[m], X[3][m]);
-  }
-  double cx = (double)a.dtdx * (double)smax_x;
-  double cy = (double)a.dtdy * (double)smax_y;
-  wave_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
+// A controlled stage's control phase (sharp_ctl_phase), when its gate is open.
+__global__ void __launch_bounds__(64) k_sharp_ctl(SharpCtl* ctl, const int* gate, int phase) {
+  if (threadIdx.x != 0 || !*gate) return;
+  sharp_ctl_phase(ctl, phase);
 }
 
 // elementwise SSP104 middle update (solver.py:418-419):
@@ -718,16 +826,8 @@ namespace {
 // overrides it.
 template <class Kernel>
 int march_rows(Kernel kernel, int nstrips, int my) {
-  static long long resident = 0;   // per kernel instantiation
-  if (resident == 0) {
-    int dev = 0, ncu = 0, nb = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess ||
-        ncu < 1 || nb < 1)
-      return 16;
-    resident = (long long)ncu * nb * 4;
-  }
+  const long long resident = resident_waves((const void*)kernel, 0);
+  if (resident == 0) return 16;
   if (const char* env = getenv("CLAWHIP_SHARP_ROWS")) {
     int h = atoi(env);
     if (h >= 1 && h <= 65536) return h;
@@ -746,13 +846,45 @@ int march_rows(Kernel kernel, int nstrips, int my) {
   return best;
 }
 
+// The march launch of a stage (and, with flags, the full-WENO5 pass over the
+// flagged interfaces before it).
 template <class RS, typename T, bool CHAR>
-int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* lo, const int* hi,
-              const int* refl, int ymode, int omode, double dt, double s, double dx, double dy,
-              const void* P, const void* Q, const void* A, const void* B, const void* aux_in,
-              const void* comp_in, void* out, void* aux_out, void* comp_out,
-              unsigned long long* cfl, const unsigned char* flags, const int* cells,
-              const int* ncells, hipStream_t stream, const AuxIn& ax, const void* src) {
+int launch_args(const SharpArgs<T>& a, hipStream_t stream) {
+  const bool flags = a.flags != nullptr;
+  if constexpr (RS::maux > 0) {
+    if (flags) return CLAWHIP_E_ARG;   // the indicator path takes solvers without aux
+  }
+  MarchArgs<T> ma;
+  ma.s = a;
+  ma.nstrips = (a.g.mx + 57) / 58;
+  ma.H = (flags && RS::maux == 0)
+             ? march_rows(k_sharp_march<RS, T, CHAR, RS::maux == 0>, ma.nstrips, a.g.my)
+             : march_rows(k_sharp_march<RS, T, CHAR, false>, ma.nstrips, a.g.my);
+  ma.nrb = (a.g.my + ma.H - 1) / ma.H;
+  const unsigned blocks = (unsigned)(((long long)ma.nstrips * ma.nrb + 3) / 4);
+  if (flags) {
+    if constexpr (RS::maux == 0) {
+      hipLaunchKernelGGL((k_weno_flagged<RS, T, CHAR>), dim3(2048), dim3(256), 0, stream, a);
+      hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
+                         ma);
+    }
+  } else {
+    hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, false>), dim3(blocks), dim3(256), 0, stream,
+                       ma);
+  }
+  if (a.ctl && a.ctl_phase != PH_NONE)
+    hipLaunchKernelGGL(k_sharp_ctl, dim3(1), dim3(64), 0, stream, a.ctl, a.gate, a.ctl_phase);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+template <class RS, typename T, bool CHAR>
+SharpArgs<T> make_args(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* lo,
+                       const int* hi, const int* refl, int ymode, int omode, double dt, double s,
+                       double dx, double dy, const void* P, const void* Q, const void* A,
+                       const void* B, const void* aux_in, const void* comp_in, void* out,
+                       void* aux_out, void* comp_out, unsigned long long* cfl, const FlagSet& fs,
+                       const AuxIn& ax, const void* src) {
   SharpArgs<T> a;
   a.rp = to_params<T>(rp);
   a.g = g;
@@ -768,38 +900,128 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
   a.out = (T*)out; a.aux_out = (T*)aux_out; a.comp_out = (T*)comp_out;
   a.src = (const T*)src;
   a.cfl_bits = cfl;
-  a.flags = flags;
-  a.cells = cells;
-  a.ncells = ncells;
+  a.flags = fs.flags;
+  a.flag_list = fs.list;
+  a.flag_count = fs.count;
+  a.recon = (T*)fs.recon;
   a.qaux = (const T*)ax.p;
   a.gaux = ax.g;
   const int never[3] = {-1, -1, -1};
   a.bcaux = to_bc(ax.lo, ax.hi, never, 2);
+  a.ctl = nullptr;
+  a.gate = nullptr;
+  a.ctl_phase = PH_NONE;
+  a.count_bad = 0;
+  a.res = 0u;
+  a.pos_mask = 0u;
+  a.qb[0] = a.qb[1] = nullptr;
+  a.cb[0] = a.cb[1] = nullptr;
+  a.dx = dx;
+  a.dy = dy;
+  return a;
+}
+
+template <class RS, typename T, bool CHAR>
+int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* lo, const int* hi,
+              const int* refl, int ymode, int omode, double dt, double s, double dx, double dy,
+              const void* P, const void* Q, const void* A, const void* B, const void* aux_in,
+              const void* comp_in, void* out, void* aux_out, void* comp_out,
+              unsigned long long* cfl, const FlagSet& fs, hipStream_t stream,
+              const AuxIn& ax, const void* src) {
   if ((RS::maux > 0) != (ax.p != nullptr)) return CLAWHIP_E_ARG;
-  if constexpr (RS::maux > 0) {
-    if (flags) return CLAWHIP_E_ARG;   // the indicator path takes solvers without aux
+  const SharpArgs<T> a = make_args<RS, T, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s,
+                                                dx, dy, P, Q, A, B, aux_in, comp_in, out,
+                                                aux_out, comp_out, cfl, fs, ax, src);
+  return launch_args<RS, T, CHAR>(a, stream);
+}
+
+// ---------------------------------------------------------------- controlled SSP33
+// nsteps attempts of SharpClawSolver's SSP33 step under the device controller:
+//   [indicator: flags of q[cur] and their list]   gate run_flags
+//   stage 1: dq_dt = D(q)/dt                         gate run1, PH_MID
+//   stage 2: r2 (SSP33 or FP32 stage 2)              gate run23
+//   stage 3: q[1-cur] (SSP33 or Kahan stage 3)       gate run23, PH_END / PH_END_JUDGE
+//   [indicator: the same step with full WENO5]       gate run_fb, PH_END_FB
+// No host synchronisation: the launches may be captured in a HIP graph.
+template <class RS, typename T, bool CHAR>
+int run_ssp33(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi, const int* refl,
+              double dx, double dy, const SharpRunBufs& b, SharpCtl* ctl, int nsteps,
+              hipStream_t stream) {
+  constexpr bool F32 = sizeof(T) == 4;
+  const AuxIn noaux;
+  const bool judge = b.flags != nullptr;
+  FlagSet fs;
+  if (judge) {
+    fs.flags = b.flags;
+    fs.list = b.flag_list;
+    fs.count = b.flag_count;
+    fs.recon = b.recon;
   }
-  MarchArgs<T> ma;
-  ma.s = a;
-  ma.nstrips = (g.mx + 57) / 58;
-  ma.H = (flags && RS::maux == 0)
-             ? march_rows(k_sharp_march<RS, T, CHAR, RS::maux == 0>, ma.nstrips, g.my)
-             : march_rows(k_sharp_march<RS, T, CHAR, false>, ma.nstrips, g.my);
-  ma.nrb = (g.my + ma.H - 1) / ma.H;
-  const unsigned blocks = (unsigned)(((long long)ma.nstrips * ma.nrb + 3) / 4);
-  if (flags) {
-    if constexpr (RS::maux == 0) {
-      hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
-                         ma);
-      hipLaunchKernelGGL((k_sharp_patch<RS, T, CHAR>), dim3(1024), dim3(256), 0, stream, a,
-                         cells, ncells);
+  auto base = [&](int ymode, int omode, const void* Q, const void* B, void* out, void* aux_out,
+                  unsigned long long* cfl, const FlagSet& f, const int* gate, int phase,
+                  unsigned res) {
+    // dt, s and the buffers named in res come from ctl when the kernel runs
+    SharpArgs<T> a = make_args<RS, T, CHAR>(rp, g, g, lo, hi, refl, ymode, omode, 1.0, 1.0, dx,
+                                            dy, nullptr, Q, nullptr, B, nullptr, nullptr, out,
+                                            aux_out, nullptr, cfl, f, noaux, nullptr);
+    a.ctl = ctl;
+    a.gate = gate;
+    a.ctl_phase = phase;
+    a.res = res;
+    a.qb[0] = b.q[0]; a.qb[1] = b.q[1];
+    a.cb[0] = b.comp[0]; a.cb[1] = b.comp[1];
+    return a;
+  };
+  const int end_phase = judge ? PH_END_JUDGE : PH_END;
+  // stage 1 -> dq_dt
+  SharpArgs<T> s1 = base(Y_ARRAY, OUT_DQDT, nullptr, nullptr, b.dqdt, nullptr, &ctl->cfl1_bits,
+                         fs, &ctl->run1, PH_MID, RES_P);
+  // stage 2 -> r2 (Y = q + dt*dq_dt)
+  SharpArgs<T> s2 = base(Y_AXPY, F32 ? OUT_F32_S2 : OUT_SSP33_2, b.dqdt, b.dqdt, b.r2, b.aux23,
+                         &ctl->cfl23_bits, fs, &ctl->run23, PH_NONE, RES_P | RES_A);
+  // stage 3 -> q[1-cur]
+  SharpArgs<T> s3 = base(Y_ARRAY, F32 ? OUT_F32_S3 : OUT_SSP33_3, nullptr, nullptr, nullptr,
+                         nullptr, &ctl->cfl23_bits, fs, &ctl->run23, end_phase,
+                         RES_A | RES_OUT | (F32 ? RES_CIN | RES_COUT : 0u));
+  s3.P = (const T*)b.r2;
+  s3.aux_in = (const T*)b.aux23;
+  s3.count_bad = judge ? 1 : 0;
+  s3.pos_mask = b.pos_mask;
+  // the judge's recomputation with full WENO5 (no flags)
+  const FlagSet none;
+  SharpArgs<T> f1 = base(Y_ARRAY, OUT_DQDT, nullptr, nullptr, b.dqdt_fb, nullptr,
+                         &ctl->cflfb_bits, none, &ctl->run_fb, PH_NONE, RES_P);
+  SharpArgs<T> f2 = base(Y_AXPY, F32 ? OUT_F32_S2 : OUT_SSP33_2, b.dqdt_fb, b.dqdt_fb, b.r2,
+                         b.aux23, &ctl->cflfb_bits, none, &ctl->run_fb, PH_NONE, RES_P | RES_A);
+  SharpArgs<T> f3 = base(Y_ARRAY, F32 ? OUT_F32_S3 : OUT_SSP33_3, nullptr, nullptr, nullptr,
+                         nullptr, &ctl->cflfb_bits, none, &ctl->run_fb, PH_END_FB,
+                         RES_A | RES_OUT | (F32 ? RES_CIN | RES_COUT : 0u));
+  f3.P = (const T*)b.r2;
+  f3.aux_in = (const T*)b.aux23;
+  BufSel sel;
+  sel.gate = &ctl->run_flags;
+  sel.cur = &ctl->cur;
+  sel.q[0] = b.q[0];
+  sel.q[1] = b.q[1];
+  for (int n = 0; n < nsteps; n++) {
+    int rc = 0;
+    if (judge) {
+      rc = launch_mlp_flags(F32 ? CLAWHIP_F32K : CLAWHIP_F64, b.q[0], g, lo, hi, refl, RS::meqn,
+                            b.flat, b.params, b.threshold, b.flags, stream, sel);
+      if (!rc) rc = launch_flag_list(b.flags, g.mx, g.my, b.flag_list, b.flag_count, stream,
+                                     &ctl->run_flags);
     }
-  } else {
-    hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, false>), dim3(blocks), dim3(256), 0, stream,
-                       ma);
+    if (!rc) rc = launch_args<RS, T, CHAR>(s1, stream);
+    if (!rc) rc = launch_args<RS, T, CHAR>(s2, stream);
+    if (!rc) rc = launch_args<RS, T, CHAR>(s3, stream);
+    if (judge) {
+      if (!rc) rc = launch_args<RS, T, CHAR>(f1, stream);
+      if (!rc) rc = launch_args<RS, T, CHAR>(f2, stream);
+      if (!rc) rc = launch_args<RS, T, CHAR>(f3, stream);
+    }
+    if (rc) return rc;
   }
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : (int)e;
+  return 0;
 }
 
 template <template <typename> class RSF, bool CHAR>
@@ -808,17 +1030,15 @@ int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const Geom& g
                    const int* refl, int ymode, int omode, double dt, double s, double dx,
                    double dy, const void* P, const void* Q, const void* A, const void* B,
                    const void* aux_in, const void* comp_in, void* out, void* aux_out,
-                   void* comp_out, unsigned long long* cfl, const unsigned char* flags,
-                   const int* cells, const int* ncells, hipStream_t stream, const AuxIn& ax,
-                   const void* src) {
+                   void* comp_out, unsigned long long* cfl, const FlagSet& flags,
+                   hipStream_t stream, const AuxIn& ax, const void* src) {
   if (dtype == CLAWHIP_F64)
     return run_stage<RSF<double>, double, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx,
                                                 dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
-                                                comp_out, cfl, flags, cells, ncells, stream, ax,
-                                                src);
+                                                comp_out, cfl, flags, stream, ax, src);
   return run_stage<RSF<float>, float, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
                                             Q, A, B, aux_in, comp_in, out, aux_out, comp_out,
-                                            cfl, flags, cells, ncells, stream, ax, src);
+                                            cfl, flags, stream, ax, src);
 }
 }  // namespace
 
@@ -827,10 +1047,10 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
                        double dt, double s, double dx, double dy, const void* P, const void* Q,
                        const void* A, const void* B, const void* aux_in, const void* comp_in,
                        void* out, void* aux_out, void* comp_out, unsigned long long* cfl,
-                       const unsigned char* flags, const int* cells, const int* ncells,
-                       hipStream_t stream, const AuxIn& ax, const void* src) {
+                       const FlagSet& flags, hipStream_t stream, const AuxIn& ax,
+                       const void* src) {
 #define ARGS dtype, rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P, Q, A, B, aux_in, \
-             comp_in, out, aux_out, comp_out, cfl, flags, cells, ncells, stream, ax, src
+             comp_in, out, aux_out, comp_out, cfl, flags, stream, ax, src
   if (rp.kind == CLAWHIP_RP_VC_ACOUSTICS || rp.kind == CLAWHIP_RP_VC_ADVECTION ||
       rp.kind == CLAWHIP_RP_VC_ADVECTION_2D) {
     if (char_decomp != 0) return CLAWHIP_E_KIND;   // component WENO5 only
@@ -862,6 +1082,33 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
     case CLAWHIP_RP_ADVECTION: return dispatch_stage<Advection, false>(ARGS);
   }
 #undef ARGS
+  return CLAWHIP_E_KIND;
+}
+
+int launch_sharp_ssp33_run(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
+                          const int* lo, const int* hi, const int* refl, double dx, double dy,
+                          const SharpRunBufs& b, SharpCtl* ctl, int nsteps, hipStream_t stream) {
+  const bool f64 = dtype == CLAWHIP_F64;
+#define RUN(RSF, CH)                                                                          \
+  (f64 ? run_ssp33<RSF<double>, double, CH>(rp, g, lo, hi, refl, dx, dy, b, ctl, nsteps, stream) \
+       : run_ssp33<RSF<float>, float, CH>(rp, g, lo, hi, refl, dx, dy, b, ctl, nsteps, stream))
+  if (char_decomp == 2) {
+    switch (rp.kind) {
+      case CLAWHIP_RP_SHALLOW: return RUN(Shallow, true);
+      case CLAWHIP_RP_SHALLOW_HLLE: return RUN(ShallowHLLE, true);
+      case CLAWHIP_RP_EULER: return RUN(Euler, true);
+    }
+    return CLAWHIP_E_KIND;
+  }
+  switch (rp.kind) {
+    case CLAWHIP_RP_SHALLOW: return RUN(Shallow, false);
+    case CLAWHIP_RP_SHALLOW_HLLE: return RUN(ShallowHLLE, false);
+    case CLAWHIP_RP_EULER: return RUN(Euler, false);
+    case CLAWHIP_RP_EULER5: return RUN(Euler5, false);
+    case CLAWHIP_RP_ACOUSTICS: return RUN(Acoustics, false);
+    case CLAWHIP_RP_ADVECTION: return RUN(Advection, false);
+  }
+#undef RUN
   return CLAWHIP_E_KIND;
 }
 

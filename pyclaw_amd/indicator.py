@@ -55,7 +55,8 @@ class TroubledInterfaceIndicator(object):
         self.threshold = float(meta.get("threshold", 0.0) if threshold is None else threshold)
         self.flat = float(meta.get("flat", 1e-6) if flat is None else flat)
         self._flags = None
-        self._cells = None
+        self._list = None       # (ids, count) of the interfaces the stages reconstruct in full
+        self._recon = {}        # dtype -> work buffer of their full-WENO5 edge values
         self.every = None       # steps between flag refreshes; None: from the CFL bound
         self._age = None
         # full WENO5 judges the indicator's steps (SharpClawSolver._judge): a step
@@ -75,13 +76,26 @@ class TroubledInterfaceIndicator(object):
             self._flags = torch.empty(n, dtype=torch.uint8, device=q.device)
         K.mlp_flags(q, bc_lower, bc_upper, reflect, self.flat, self.params, self.threshold,
                     self._flags)
-        self._cells = K.mlp_troubled_cells(self._flags, mx, my, self._cells)
+        self._list = K.mlp_flag_list(self._flags, mx, my, self._list)
         return self._flags
 
     @property
-    def cells(self):
-        """(cells, count) of the last flags() call: the cells the stage recomputes."""
-        return self._cells
+    def flag_list(self):
+        """(ids, count) of the last flags() call: the interfaces reconstructed with
+        the full WENO5."""
+        return self._list
+
+    def recon_buffer(self, like):
+        """The work buffer of the full-WENO5 edge values for states like `like`
+        (planar (meqn, my, mx))."""
+        meqn, my, mx = like.shape
+        n = K.recon_elems(mx, my, meqn)
+        key = (like.dtype, str(like.device))
+        b = self._recon.get(key)
+        if b is None or b.numel() < n:
+            b = torch.empty(n, dtype=like.dtype, device=like.device)
+            self._recon[key] = b
+        return b
 
     def refresh_interval(self, cfl_max=None):
         """Steps between flag refreshes: `every` when set, else 2 when the

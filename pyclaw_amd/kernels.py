@@ -158,18 +158,24 @@ OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F3
 
 def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx, dy, P,
                  Q=None, s=0.0, y_mode=Y_ARRAY, out_mode=OUT_D, A=None, B=None, aux_in=None,
-                 comp_in=None, aux_out=None, comp_out=None, flags=None, cells=None, qaux=None,
-                 qaux_bc_lower=None, qaux_bc_upper=None, src=None):
+                 comp_in=None, aux_out=None, comp_out=None, flags=None, flag_list=None,
+                 recon=None, qaux=None, qaux_bc_lower=None, qaux_bc_upper=None, src=None):
     """D = dt*L(Y) (SharpClaw flux2) fused with one RK stage; planar (meqn, my, mx).
     flags: uint8 troubled-interface flags (mlp_flags) selecting full WENO5 per
-    interface, with cells = (list, count) from mlp_troubled_cells (built here when
-    None); flags None = full WENO5 everywhere.  qaux: the solver's aux fields
-    (planar, compact or with 3 stored ghosts and codes 4) for the solvers that
-    read them.  P may carry 3 stored ghosts (planar (meqn, my+6, mx+6), boundary
-    codes 4), filled by the caller; src: planar interior dq_src increment."""
-    if flags is not None and cells is None:
-        cells = mlp_troubled_cells(flags, out.shape[2], out.shape[1])
+    interface (the linear weights elsewhere), with flag_list = (ids, count) from
+    mlp_flag_list and recon a work tensor of recon_elems(...) elements of the
+    state's dtype (both made here when None); flags None = full WENO5
+    everywhere.  qaux: the solver's aux fields (planar, compact or with 3 stored
+    ghosts and codes 4) for the solvers that read them.  P may carry 3 stored
+    ghosts (planar (meqn, my+6, mx+6), boundary codes 4), filled by the caller;
+    src: planar interior dq_src increment."""
     meqn, my, mx = out.shape
+    if flags is not None:
+        if flag_list is None:
+            flag_list = mlp_flag_list(flags, mx, my)
+        if recon is None:
+            recon = torch.empty(recon_elems(mx, my, meqn), dtype=out.dtype, device=out.device)
+        assert recon.dtype == out.dtype and recon.numel() >= recon_elems(mx, my, meqn)
     _check_fields(rp, meqn, None, None)
     for t in (P, Q, A, B, src):
         assert t is None or (t.dtype == out.dtype and t.numel() >= out.numel())
@@ -187,14 +193,40 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
         L.ints(bc_upper), L.ints(reflect), int(y_mode), int(out_mode), float(dt), float(s),
         float(dx), float(dy), _ptr(P), ctypes.byref(gin) if gin is not None else None, _ptr(Q),
         _ptr(A), _ptr(B), _ptr(aux_in), _ptr(comp_in), _ptr(out), _ptr(aux_out), _ptr(comp_out),
-        _ptr(src), _ptr(flags),
-        _ptr(cells[0]) if cells is not None else None,
-        _ptr(cells[1]) if cells is not None else None, _ptr(qaux),
+        _ptr(src), _ptr(flags), _ptr(flag_list[0]) if flags is not None else None,
+        _ptr(flag_list[1]) if flags is not None else None, _ptr(recon), _ptr(qaux),
         ctypes.byref(gaux) if qaux is not None else None,
         L.ints(qaux_bc_lower) if qaux is not None else None,
         L.ints(qaux_bc_upper) if qaux is not None else None,
         int(qaux.shape[0]) if qaux is not None else 0, _ptr(cfl), _stream())
     L.check(rc, "clawhip_sharp2_stage")
+
+
+def sharp2_ssp33_run(rp, char_decomp, bufs, comps, ctl_dev, bc_lower, bc_upper, reflect, dx, dy,
+                     dq_dt, r2, aux23, nsteps, indicator=None, dq_dt_full=None, recon=None,
+                     positive=()):
+    """nsteps device-controlled SSP33 attempts (clawhip_sharp2_ssp33_run) on the
+    planar ping-pong buffers bufs = (q0, q1) (comps: their Kahan compensations
+    or (None, None)); indicator: the learned indicator's path (its flags, list,
+    network) with the positivity judge over the fields listed in `positive`."""
+    meqn, my, mx = bufs[0].shape
+    g = L.geom2(mx, my)
+    code = _dtype_code(bufs[0])
+    fl = lst = cnt = params = None
+    flat = thr = 0.0
+    mask = 0
+    if indicator is not None:
+        fl, (lst, cnt) = indicator._flags, indicator.flag_list
+        params, flat, thr = indicator.params, indicator.flat, indicator.threshold
+        for f in positive:
+            mask |= 1 << int(f)
+    rc = L.lib().clawhip_sharp2_ssp33_run(
+        ctypes.byref(rp), code, int(char_decomp), ctypes.byref(g), L.ints(bc_lower),
+        L.ints(bc_upper), L.ints(reflect), float(dx), float(dy), _ptr(bufs[0]), _ptr(bufs[1]),
+        _ptr(comps[0]), _ptr(comps[1]), _ptr(dq_dt), _ptr(r2), _ptr(aux23), _ptr(fl), _ptr(lst),
+        _ptr(cnt), _ptr(recon), _ptr(dq_dt_full), _ptr(params), float(flat), float(thr), mask,
+        _ptr(ctl_dev), int(nsteps), _stream())
+    L.check(rc, "clawhip_sharp2_ssp33_run")
 
 
 def lmm_update(mode, q, u, coeffs, out, R=None, v=None):
@@ -269,6 +301,25 @@ def mlp_logits(feat, n, params, logits):
     L.check(rc, "clawhip_mlp_logits")
 
 
+def recon_elems(mx, my, meqn):
+    """Elements of the work buffer a flagged stage stores its full-WENO5 edge
+    values in: 2*meqn per interface."""
+    return 2 * meqn * mlp_num_interfaces(mx, my)
+
+
+def mlp_flag_list(flags, mx, my, out=None):
+    """(ids int32, count int32 (1,)) of the interfaces a flagged stage
+    reconstructs with the full WENO5 (clawhip_mlp_flag_list)."""
+    n = mlp_num_interfaces(mx, my)
+    if out is None or out[0].numel() < n:
+        out = (torch.empty(n, dtype=torch.int32, device=flags.device),
+               torch.zeros(1, dtype=torch.int32, device=flags.device))
+    rc = L.lib().clawhip_mlp_flag_list(_ptr(flags), int(mx), int(my), _ptr(out[0]),
+                                       _ptr(out[1]), _stream())
+    L.check(rc, "clawhip_mlp_flag_list")
+    return out
+
+
 def mlp_flags(q, bc_lower, bc_upper, reflect, flat, params, threshold, flags):
     meqn, my, mx = q.shape
     if flags.numel() < mlp_num_interfaces(mx, my) or flags.dtype != torch.uint8:
@@ -278,18 +329,6 @@ def mlp_flags(q, bc_lower, bc_upper, reflect, flat, params, threshold, flags):
                                    L.ints(bc_upper), L.ints(reflect), meqn, float(flat),
                                    _ptr(params), float(threshold), _ptr(flags), _stream())
     L.check(rc, "clawhip_mlp_flags")
-
-
-def mlp_troubled_cells(flags, mx, my, out=None):
-    """(cells int32 (mx*my,), count int32 (1,)) of the cells next to a flagged
-    interface (clawhip_mlp_troubled_cells)."""
-    if out is None or out[0].numel() < mx * my:
-        out = (torch.empty(mx * my, dtype=torch.int32, device=flags.device),
-               torch.zeros(1, dtype=torch.int32, device=flags.device))
-    rc = L.lib().clawhip_mlp_troubled_cells(_ptr(flags), int(mx), int(my), _ptr(out[0]),
-                                            _ptr(out[1]), _stream())
-    L.check(rc, "clawhip_mlp_troubled_cells")
-    return out
 
 
 # ---- FP64 reductions over a state

@@ -16,6 +16,12 @@ from . import kernels as K
 from .solver import BC, Solver
 
 
+def ctypes_sizeof_sharpctl():
+    import ctypes
+    from . import _lib as L
+    return ctypes.sizeof(L.SharpCtl)
+
+
 class _StageState(object):
     """The state one operator evaluation sees (the reference's register States,
     sharpclaw/solver.py:306-430): the solver's state with q replaced by the
@@ -83,6 +89,14 @@ class SharpClawSolver(Solver):
         self.prev_dtFE_values = []
         self.check_lmm_cond = False
         self.lmm_cond = True
+        # evolve_to_time(solution, tend) and run_steps run SSP33 as device-
+        # controlled launches (clawhip_sharp2_ssp33_run, graph-captured chunks of
+        # graph_chunk attempts) when no host callback takes part
+        self.device_loop = True
+        self.use_graph = True
+        self.graph_chunk = 16
+        self._graphs = {}
+        self._ctl = None
         super(SharpClawSolver, self).__init__(riemann_solver, claw_package)
 
     # ---- sharpclaw/solver.py:202-265
@@ -239,8 +253,9 @@ class SharpClawSolver(Solver):
         qaux, alo, ahi = self._qaux if self._qaux is not None else (None, None, None)
         K.sharp2_stage(self._rp_struct, self.char_decomp, out, cfl, lo, hi, self.reflect_index,
                        self.dt, dx, dy, P, flags=self._flags,
-                       cells=self.indicator.cells if self._flags is not None else None,
-                       qaux=qaux, qaux_bc_lower=alo, qaux_bc_upper=ahi, **kw)
+                       flag_list=self.indicator.flag_list if self._flags is not None else None,
+                       recon=self.indicator.recon_buffer(out) if self._flags is not None
+                       else None, qaux=qaux, qaux_bc_lower=alo, qaux_bc_upper=ahi, **kw)
         if now:
             self.cfl.update_global_max(self._read_back(cfl)[0])
 
@@ -551,6 +566,272 @@ class SharpClawSolver(Solver):
                     self._bufs["audit_comp"] = out_q[1]
                 return alt
         return out_q
+
+    # ---- device-resident SSP33 loop
+    def _device_loop_ok(self, solution):
+        lo, hi = self._bc_codes()
+        return (self.device_loop and self.time_integrator == "SSP33" and self.dt_variable
+                and self.before_step is None and self.dq_src is None
+                and not self.call_before_step_each_stage and BC.custom not in lo
+                and BC.custom not in hi and getattr(self.rp, "num_aux", 0) == 0)
+
+    def evolve_to_time(self, solution, tend=None):
+        """Solver.evolve_to_time (solver.py:554-666); towards a given tend, SSP33
+        without host callbacks runs on the device (same arithmetic, decisions and
+        dt sequence)."""
+        if not self._is_set_up:
+            self.setup(solution)
+        if tend is None or not self._device_loop_ok(solution):
+            return super(SharpClawSolver, self).evolve_to_time(solution, tend)
+        tstart = solution.t
+        if self.cfl_desired > self.cfl_max:
+            raise Exception("Variable time-stepping and desired CFL > maximum CFL")
+        if tend <= tstart:
+            self.logger.info("Already at or beyond end time: no evolution required.")
+            return self.status
+        _, accepted = self._run_device(solution, tstart, tend, self.max_steps)
+        if solution.t < tend and accepted == self.max_steps:
+            raise Exception("Maximum number of timesteps have been taken")
+        return self.status
+
+    def run_steps(self, solution, nsteps):
+        """Attempt exactly nsteps SSP33 steps (no end time) in the device loop."""
+        if not self._is_set_up:
+            self.setup(solution)
+        if not self._device_loop_ok(solution):
+            raise ValueError("run_steps needs the device loop: SSP33, variable dt, no host "
+                             "callbacks, built-in boundary conditions, no aux fields")
+        return self._run_device(solution, solution.t, float("inf"), int(nsteps))
+
+    def _run_device(self, solution, tstart, tend, max_attempts):
+        """Up to max_attempts attempts towards tend; the indicator's audit
+        attempts (every audit_every judged steps) run on the host path.  Returns
+        (attempts, accepted)."""
+        ind = self.indicator
+        attempts = accepted = 0
+        done = False
+        while attempts < max_attempts and not done:
+            n = max_attempts - attempts
+            if ind is not None and ind.audit_every:
+                every = int(ind.audit_every)
+                k = every - ind.judge_stats["steps"] % every   # the k-th next one audits
+                if k == 1:
+                    acc, done = self._host_attempt(solution, tstart, tend)
+                    attempts += 1
+                    accepted += acc
+                    continue
+                n = min(n, k - 1)
+            a, acc, done = self._device_attempts(solution, tstart, tend, n)
+            attempts += a
+            accepted += acc
+        return attempts, accepted
+
+    def _host_attempt(self, solution, tstart, tend):
+        """One attempt of Solver.evolve_to_time's loop (solver.py:607-651) on the
+        host path; returns (accepted 0/1, reached tend)."""
+        state = solution.state
+        backup = self._backup(state)
+        told = solution.t
+        self.step(solution, False, tstart, tend)
+        cfl = self.cfl.get_cached_max()
+        self.accept_step = self.accept_reject_step(state)
+        if self.accept_step:
+            self.status["cflmax"] = max(cfl, self.status["cflmax"])
+            solution.t += self.dt
+            self.status["numsteps"] += 1
+            return 1, solution.t >= tend
+        self._restore(state, backup)
+        solution.t = told
+        return 0, False
+
+    def _device_setup(self, solution):
+        """The device loop's buffers and launch arguments for the state's current
+        buffers: a ping-pong pair whose identities captured graphs keep, the
+        work buffers, and on the indicator path its flags, their list and the
+        full-WENO5 edge buffer (allocated when missing).  Returns
+        (bufs, cur, args, kw, dq_dt buffer)."""
+        state = solution.states[0]
+        self._grid = state.grid
+        self._state = state
+        ind = self.indicator
+        qin = state.device_q(self._dtype_code)
+        other = self._take_scratch(qin)
+        bufs, cur = (qin, other), 0
+        for ent in self._graphs.values():
+            b = ent["bufs"]
+            if b[0][0] is qin[0] or b[1][0] is qin[0]:
+                bufs = b
+                cur = 0 if b[0][0] is qin[0] else 1
+                break
+        q = bufs[0][0]
+        fp32 = self.precision == "fp32k"
+        dqdt, r2 = self._buf("ctl_dqdt", q), self._buf("ctl_r2", q)
+        aux23 = self._buf("ctl_aux23", q) if fp32 else None
+        kw = {}
+        if ind is not None:
+            meqn, my, mx = q.shape
+            nint = K.mlp_num_interfaces(mx, my)
+            if ind._flags is None or ind._flags.numel() != nint or ind._list is None:
+                ind._flags = torch.empty(nint, dtype=torch.uint8, device=q.device)
+                ind._list = (torch.empty(nint, dtype=torch.int32, device=q.device),
+                             torch.zeros(1, dtype=torch.int32, device=q.device))
+                ind._age = None   # nothing computed yet: the next attempt refreshes
+            kw = dict(indicator=ind, dq_dt_full=self._buf("ctl_dqdt_fb", q),
+                      recon=ind.recon_buffer(q), positive=self._positive_fields())
+        if self._ctl is None:
+            self._ctl = torch.zeros(ctypes_sizeof_sharpctl(), dtype=torch.uint8, device="cuda")
+        lo, hi = self._bc_codes()
+        dx, dy = self._grid.delta
+        comps = (bufs[0][1], bufs[1][1])
+        args = (self._rp_struct, self.char_decomp, (bufs[0][0], bufs[1][0]), comps, self._ctl,
+                lo, hi, self.reflect_index, dx, dy, dqdt, r2, aux23)
+        return bufs, cur, args, kw, dqdt
+
+    def capture_graphs(self, solution, nsteps):
+        """Capture (without running) the graphs run_steps(solution, nsteps) will
+        replay, so that a timed run holds no capture."""
+        if not self._is_set_up:
+            self.setup(solution)
+        if not (self.use_graph and self._device_loop_ok(solution)):
+            return
+        bufs, cur, args, kw, _ = self._device_setup(solution)
+        for m in sorted(self._chunk_sizes(int(nsteps))):
+            self._graph(bufs, args, kw, m)
+        self._scratch = bufs[1 - cur]
+
+    def _chunk_sizes(self, nsteps):
+        """The launch sizes run_steps(solution, nsteps) uses: device runs up to
+        each indicator audit (an audit attempt runs on the host path), each cut
+        into chunks of at most graph_chunk attempts."""
+        ind = self.indicator
+        every = int(ind.audit_every) if ind is not None and ind.audit_every else 0
+        steps = ind.judge_stats["steps"] if ind is not None else 0
+        sizes, attempts = set(), 0
+        while attempts < nsteps:
+            n = nsteps - attempts
+            if every:
+                k = every - steps % every
+                if k == 1:
+                    attempts += 1
+                    steps += 1
+                    continue
+                n = min(n, k - 1)
+            left = n
+            while left > 0:
+                m = min(self.graph_chunk, left)
+                sizes.add(m)
+                left -= m
+            attempts += n
+            steps += n
+        return sizes
+
+    def _device_attempts(self, solution, tstart, tend, n):
+        """n attempts in the device loop; the solver, the state and the
+        indicator take the controller's state afterwards."""
+        import ctypes
+        from . import _lib as L
+        state = solution.states[0]
+        ind = self.indicator
+        bufs, cur, args, kw, dqdt = self._device_setup(solution)
+        if not self.accept_step:
+            dqdt.copy_(self.dq_dt)   # the rate a rejected attempt's successor reuses
+            # that successor skips stage 1: its get_dt and the CFL rescale
+            # (sharpclaw/solver.py:296-303) happen here
+            self.get_dt(solution.t, tstart, tend, False)
+            cfl = self.cfl.get_cached_max()
+            self.cfl.set_global_max(self.dt / self.dt_old * cfl)
+            self.dt_old = self.dt
+        c = L.SharpCtl()
+        c.t, c.dt, c.tstart, c.tend = solution.t, self.dt, tstart, tend
+        c.have_dt_old = int(self.dt_old is not None)
+        c.dt_old = self.dt_old if self.dt_old is not None else 0.0
+        cached = self.cfl.get_cached_max()
+        c.cfl = 0.0 if cached is None else cached   # None before the first step: stage 1 sets it
+        c.cfl_desired, c.cfl_max, c.dt_max = self.cfl_desired, self.cfl_max, self.dt_max
+        c.cflmax, c.dtmin, c.dtmax = (self.status["cflmax"], self.status["dtmin"],
+                                      self.status["dtmax"])
+        c.accept_prev = int(bool(self.accept_step))
+        c.cur = cur
+        c.max_steps = int(n)
+        c.run1, c.run23 = c.accept_prev, 1
+        if ind is not None:
+            # the first attempt's flag refresh (step_flags); later ones on the device
+            every = ind.refresh_interval(self.cfl_max)
+            age = ind._age
+            refresh = age is None or age >= every
+            if refresh:
+                age = 0
+            c.age, c.every, c.run_flags = age + 1, every, int(refresh)
+            c.judge = 1
+            c.judge_steps = ind.judge_stats["steps"]
+            c.fallbacks = ind.judge_stats["positivity_fallbacks"]
+        self._ctl.copy_(torch.frombuffer(bytearray(bytes(c)), dtype=torch.uint8))
+        launched = 0
+        host = L.SharpCtl()
+        while launched < n:
+            m = min(self.graph_chunk, n - launched)
+            if self.use_graph:
+                self._graph(bufs, args, kw, m).replay()
+            else:
+                K.sharp2_ssp33_run(*args, m, **kw)
+            launched += m
+            ctypes.memmove(ctypes.addressof(host), bytes(self._ctl.cpu().numpy().tobytes()),
+                           ctypes.sizeof(host))
+            if host.done:
+                break
+        solution.t = host.t
+        self.dt = host.dt
+        self.dt_old = host.dt_old if host.have_dt_old else None
+        self.cfl.set_global_max(host.cfl)
+        self.status["cflmax"], self.status["dtmin"], self.status["dtmax"] = (host.cflmax,
+                                                                              host.dtmin, host.dtmax)
+        self.status["numsteps"] += host.accepted
+        self.accept_step = bool(host.accept_prev)
+        self.dq_dt = dqdt
+        state.set_device_q(*bufs[host.cur])
+        self._scratch = bufs[1 - host.cur]
+        if ind is not None:
+            ind._age = host.age
+            ind.judge_stats["steps"] = host.judge_steps
+            ind.judge_stats["positivity_fallbacks"] = host.fallbacks
+        return host.attempts, host.accepted, host.done == 1
+
+    @staticmethod
+    def _graph_key(args, kw, n):
+        """What a captured run depends on: the address of every buffer it
+        touches, the indicator's scalars and the attempt count."""
+        key = [n]
+        for x in list(args) + [kw.get(k) for k in sorted(kw)]:
+            if isinstance(x, torch.Tensor):
+                key.append(x.data_ptr())
+            elif isinstance(x, (tuple, list)):
+                key.extend(t.data_ptr() if isinstance(t, torch.Tensor) else t for t in x)
+            elif x is not None and hasattr(x, "params"):   # the indicator
+                key.extend([x.params.data_ptr(), x._flags.data_ptr(), x._list[0].data_ptr(),
+                            x._list[1].data_ptr(), x.flat, x.threshold])
+            elif isinstance(x, (int, float, str)) or x is None:
+                key.append(x)
+        return tuple(key)
+
+    def _graph(self, bufs, args, kw, n):
+        """The captured graph of n attempts with these arguments (captured on
+        first use)."""
+        key = self._graph_key(args, kw, n)
+        ent = self._graphs.get(key)
+        if ent is None:
+            from .device import no_gc
+            g = torch.cuda.CUDAGraph()
+            st = torch.cuda.Stream()
+            st.wait_stream(torch.cuda.current_stream())
+            # no garbage collection while capturing: collecting an old graph
+            # would destroy it mid-capture, which HIP refuses
+            with no_gc(), torch.cuda.stream(st):
+                with torch.cuda.graph(g, stream=st):
+                    K.sharp2_ssp33_run(*args, n, **kw)
+            torch.cuda.current_stream().wait_stream(st)
+            ent = {"graph": g, "bufs": bufs, "args": args, "kw": kw}
+            self._graphs[key] = ent
+        return ent["graph"]
 
     def dq(self, state):
         """dt*L(q) on the interior as a host array (sharpclaw/solver.py:532-542)."""

@@ -17,6 +17,18 @@ def test_library_exports_match_header():
     assert set(names) <= set(L._SIGS), set(names) - set(L._SIGS)
 
 
+def test_ctypes_signatures_match_header_arity():
+    """Every ctypes signature has as many arguments as the header declares."""
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(L.HERE), "include", "clawhip.h")).read()
+    for name, sig in L._SIGS.items():
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", hdr)
+        assert m, name
+        args = [a for a in m.group(1).split(",") if a.strip() and a.strip() != "void"]
+        assert len(args) == len(sig), (name, len(args), len(sig))
+
+
 def test_noalloc_sums_reject_sizes_before_device_work():
     """The allocation-free reduction refuses more fields than its fixed scratch
     holds (and null pointers) before it touches a device."""

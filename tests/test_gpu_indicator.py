@@ -250,13 +250,14 @@ def test_stage_all_flags_equals_char_weno():
 
 
 
-def _stage_ms(K, rp, q, flags=None, cells=None, reps=5, n=10):
+def _stage_ms(K, rp, q, flags=None, flag_list=None, recon=None, reps=5, n=10):
     out = torch.empty_like(q)
     cfl = torch.zeros(1, dtype=torch.float64, device=q.device)
     my, mx = q.shape[1], q.shape[2]
     args = (rp, 2, out, cfl, [2, 2], [2, 2], [1, 2], 1e-4, 1.0 / mx, 1.0 / my, q)
-    K.sharp2_stage(*args, flags=flags, cells=cells)
-    return _best_ms(lambda: K.sharp2_stage(*args, flags=flags, cells=cells), reps, n)
+    kw = dict(flags=flags, flag_list=flag_list, recon=recon)
+    K.sharp2_stage(*args, **kw)
+    return _best_ms(lambda: K.sharp2_stage(*args, **kw), reps, n)
 
 
 def _best_ms(fn, reps=5, n=10):
@@ -280,7 +281,7 @@ def test_indicator_gain_accuracy_and_speed():
     the learned selection (flags refreshed every second step) stays within 1e-4
     relative L1 of FP64 full WENO5 over 100 steps at 256^2 while most interfaces
     take the linear weights; at 2048^2 its step (three flagged FP32 stages plus
-    half a flags + troubled-cells pass) is at least 1.8x faster than the FP64
+    half a flags pass) is at least 1.8x faster than the FP64
     full-WENO step (three stages), and faster than the FP32 full-WENO step."""
     from pyclaw_amd import indicator as I
     from pyclaw_amd import kernels as K
@@ -323,13 +324,13 @@ def test_indicator_gain_accuracy_and_speed():
     torch.cuda.synchronize()
     t64 = _stage_ms(K, rp, q64)
     t32 = _stage_ms(K, rp, q32)
-    t32f = _stage_ms(K, rp, q32, fl, ind.cells)
+    t32f = _stage_ms(K, rp, q32, fl, ind.flag_list, ind.recon_buffer(q32))
     tflags = _best_ms(lambda: ind.flags(q32, [2, 2], [2, 2]))
     step64 = 3 * t64
     step32 = 3 * t32
     step_mlp = 3 * t32f + tflags / 2
     print(f"2048^2 stage ms: fp64 {t64:.3f} fp32 {t32:.3f} fp32 flagged {t32f:.3f} "
-          f"flags+cells {tflags:.3f} troubled {ind.troubled_fraction():.4f}; step ratio "
+          f"flags {tflags:.3f} troubled {ind.troubled_fraction():.4f}; step ratio "
           f"fp64/mlp {step64 / step_mlp:.2f}")
     assert step_mlp < step32
     assert step64 / step_mlp >= 1.8
