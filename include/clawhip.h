@@ -96,7 +96,7 @@ typedef struct {
   double cflmax, dtmin, dtmax;
   int dt_variable, done, cur, attempts;
   int max_steps, accepted, nrejected, pad0;
-  unsigned int waves_done, pad1;
+  unsigned int pad2, pad1;
   unsigned long long cfl_bits;
 } clawhip_stepctl;
 
@@ -315,9 +315,10 @@ long long clawhip_ensemble_table_bytes(int n);
 /* Validates the members and writes the launch table (member order, wavefront
  * offsets, method) into table_dev (device memory of clawhip_ensemble_table_bytes
  * bytes; a synchronous host-to-device copy).  rows: rows per wavefront (0 =
- * chosen from the occupancy and the total work).  waves[2] receives the grid
- * size of the step launch (waves[1] is 0), to pass to clawhip_ensemble_run.  Boundary codes
- * 1..3, num_ghost 2, method/mthlim as clawhip_step2_unsplit. */
+ * chosen from the occupancy and the total work).  waves[2] receives the
+ * wavefronts of the step launch and the member count, to pass to
+ * clawhip_ensemble_run.  Boundary codes 1..3, num_ghost 2, method/mthlim as
+ * clawhip_step2_unsplit. */
 int clawhip_ensemble_prepare(const clawhip_rp *rp, int dtype, const clawhip_ens_member *members,
                              int n, const int *bc_lower, const int *bc_upper, const int *reflect,
                              int num_ghost, const int *method, const int *mthlim, int rows,

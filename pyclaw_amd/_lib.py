@@ -40,7 +40,7 @@ class StepCtl(ctypes.Structure):
                 ("dt_variable", ctypes.c_int), ("done", ctypes.c_int), ("cur", ctypes.c_int),
                 ("attempts", ctypes.c_int), ("max_steps", ctypes.c_int),
                 ("accepted", ctypes.c_int), ("nrejected", ctypes.c_int), ("pad0", ctypes.c_int),
-                ("waves_done", ctypes.c_uint), ("pad1", ctypes.c_uint),
+                ("pad2", ctypes.c_uint), ("pad1", ctypes.c_uint),
                 ("cfl_bits", ctypes.c_ulonglong)]
 
 

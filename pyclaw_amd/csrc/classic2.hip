@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
 // Device-resident stepping: one launch = one attempted step of
 // Solver.evolve_to_time.  The state ping-pongs between two buffers; `ctl->cur`
 // names the current one.  The last workgroup to finish applies the CFL check,
-// accepts or rejects the step and computes the next dt (step_ctl_finish).
+// accepts or rejects the step and computes the next dt (k_step_ctl after it).
 template <class RS, typename T, bool KAHAN>
 __global__ void __launch_bounds__(256) k_unsplit_ctl(UnsplitArgs<T> a, StepCtl* ctl,
                                                      CtlBuffers bufs) {
@@ -40,8 +40,11 @@ __global__ void __launch_bounds__(256) k_unsplit_ctl(UnsplitArgs<T> a, StepCtl* 
   a.cfl_bits = &ctl->cfl_bits;
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   unsplit_body<RS, T, KAHAN>(a, wid);
-  const int total = a.nstrips * a.nrb;
-  if (wid < total) step_ctl_finish(ctl, (unsigned)total);
+}
+
+// The step's control after it (step_ctl_apply).
+__global__ void __launch_bounds__(64) k_step_ctl(StepCtl* ctl) {
+  if (threadIdx.x == 0) step_ctl_apply(ctl);
 }
 
 }  // namespace claw
@@ -134,9 +137,11 @@ int run_unsplit_ctl(const clawhip_rp& rp, const Geom& g, const int* lo, const in
   a.cfl_bits = nullptr;
   long long waves = (long long)a.nstrips * a.nrb;
   int blocks = (int)((waves + 3) / 4);
-  for (int n = 0; n < nsteps; n++)
+  for (int n = 0; n < nsteps; n++) {
     hipLaunchKernelGGL((k_unsplit_ctl<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, stream, a,
                        ctl, bufs);
+    hipLaunchKernelGGL(k_step_ctl, dim3(1), dim3(64), 0, stream, ctl);
+  }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
 }

@@ -90,7 +90,14 @@ k_ens_step(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem)
   a.cfl_bits = &ctl->cfl_bits;
   const int local = (int)(gw - m.base);
   unsplit_body<RS, T, KAHAN>(a, local);
-  step_ctl_finish(ctl, (unsigned)(m.nstrips * m.nrb));
+}
+
+// Every member's step control after the step launch (step_ctl_apply), one
+// thread per member.
+__global__ void __launch_bounds__(256) k_ens_ctl(const EnsHeader* __restrict__ hdr,
+                                                 const EnsMember* __restrict__ mem) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < hdr->n) step_ctl_apply(mem[k].ctl);
 }
 
 namespace {
@@ -129,8 +136,11 @@ template <class RS, typename T, bool KAHAN>
 void ens_launch(const EnsHeader* h, const EnsMember* m, const long long* waves, int nsteps,
                 hipStream_t s) {
   const unsigned blocks = (unsigned)((waves[0] + 3) / 4);
-  for (int n = 0; n < nsteps; n++)
+  const unsigned ctl_blocks = (unsigned)((waves[1] + 255) / 256);
+  for (int n = 0; n < nsteps; n++) {
     hipLaunchKernelGGL((k_ens_step<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, s, h, m);
+    hipLaunchKernelGGL(k_ens_ctl, dim3(ctl_blocks), dim3(256), 0, s, h, m);
+  }
 }
 
 template <template <typename> class RSF>
@@ -183,7 +193,7 @@ int ensemble_prepare(const clawhip_rp& rp, const clawhip_ens_member* members, in
   ::memcpy(buf.data() + ens_members_offset(), v.data(), (size_t)n * sizeof(EnsMember));
   hipError_t e = hipMemcpy(table_dev, buf.data(), buf.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) return (int)e;
-  waves[0] = base; waves[1] = 0;
+  waves[0] = base; waves[1] = n;   // wavefronts of a step launch; members
   return 0;
 }
 

@@ -2,8 +2,8 @@
 //
 // StepCtl holds what Solver.evolve_to_time keeps between steps (pyclaw/solver.py:
 // 554-666): t, dt, the cached CFL, the method's CFL targets and the status
-// counters.  A step kernel reads dt and the current buffer from it; the last
-// wavefront to finish the step calls step_ctl_finish, which performs -- in FP64,
+// counters.  A step kernel reads dt and the current buffer from it; a
+// one-wavefront kernel after it calls step_ctl_apply, which performs -- in FP64,
 // with the reference's arithmetic -- the accept/reject decision
 // (solver.py:621-651, accept_reject_step 524-529) and the next get_dt
 // (solver.py:535-552, ClawSolver uses get_dt_new 531-533).  The layout equals
@@ -19,7 +19,7 @@ struct StepCtl {
   double cflmax, dtmin, dtmax;
   int dt_variable, done, cur, attempts;
   int max_steps, accepted, nrejected, pad0;
-  unsigned int waves_done, pad1;
+  unsigned int pad2, pad1;
   unsigned long long cfl_bits;
 };
 
@@ -32,15 +32,13 @@ struct CtlBuffers {
 // done codes
 enum { CTL_RUNNING = 0, CTL_REACHED_TEND = 1, CTL_CFL_FIXED_DT = 2, CTL_MAX_STEPS = 3 };
 
-// Called by every active wavefront after its CFL atomic; `total` = number of
-// active wavefronts of the launch.  The last one applies the controller.
-__device__ __forceinline__ void step_ctl_finish(StepCtl* ctl, unsigned total) {
-  if ((threadIdx.x & 63) != 0) return;
-  __threadfence();
-  unsigned prev = atomicAdd(&ctl->waves_done, 1u);
-  if (prev != total - 1) return;
-  __threadfence();
-  const unsigned long long bits = atomicAdd(&ctl->cfl_bits, 0ull);
+// The end of a step: applied by one thread of a kernel launched after the step
+// kernel on the same stream (the kernel boundary orders the step's CFL atomics
+// before this read, so the step kernel needs no fences).  Nothing happens once
+// ctl->done is set (the step kernel did not run either).
+__device__ __forceinline__ void step_ctl_apply(StepCtl* ctl) {
+  if (ctl->done) return;
+  const unsigned long long bits = ctl->cfl_bits;
   const double cfl = __longlong_as_double((long long)bits);
   ctl->cfl = cfl;
   const double dt = ctl->dt;
@@ -83,7 +81,6 @@ __device__ __forceinline__ void step_ctl_finish(StepCtl* ctl, unsigned total) {
   }
   ctl->done = done;
   ctl->cfl_bits = 0ull;
-  ctl->waves_done = 0u;
 }
 
 }  // namespace claw
