@@ -237,6 +237,53 @@ def acoustics_leg(n=256, seed=1, t_final=0.6):
     return out
 
 
+def classic_roofline_leg(n=4096, steps=64, seed=1):
+    """The plain FP64 classic step of the acoustics configuration (unsplit,
+    transverse order 2, MC, solid walls, CFL 0.9) at 4096^2, where a step moves
+    805 MB: the size at which the HBM roofline, not launch latency, bounds it.
+    Device-controlled steps in captured graphs; HIP events around every replay
+    (the step kernel and its one-wavefront control kernel)."""
+    import torch
+    import pyclaw_amd as pyclaw
+    from pyclaw_amd import riemann
+    rng = np.random.default_rng(seed)
+    xc = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    q0 = np.zeros((3, n, n), order="F")
+    q0[0] = np.exp(-100.0 * ((X - 0.5) ** 2 + (Y - 0.5) ** 2)) + 1e-3 * rng.uniform(-1, 1, X.shape)
+    del X, Y
+    solver = pyclaw.ClawSolver2D(riemann.acoustics_2D)
+    solver.dimensional_split = False
+    solver.transverse_waves = 2
+    solver.limiters = pyclaw.limiters.tvd.MC
+    solver.cfl_desired, solver.cfl_max = 0.9, 1.0
+    solver.all_bcs = pyclaw.BC.wall
+    sol = pyclaw.Solution(3, pyclaw.Domain([0., 0.], [1., 1.], [n, n]))
+    sol.problem_data.update({"rho": 1.0, "bulk": 4.0, "cc": 2.0, "zz": 2.0})
+    sol.q = q0
+    solver.dt_initial = 0.9 * (1.0 / n) / 2.0
+    solver.dt = solver.dt_initial
+    solver.run_steps(sol, solver.graph_chunk)
+    torch.cuda.synchronize()
+    timer = ChunkTimer(solver)
+    try:
+        t0 = time.perf_counter()
+        solver.run_steps(sol, steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        timer.close()
+    kms, nk = timer.kernel_ms()
+    achieved = 48.0 * n * n / (kms * 1e-3) / 1e9
+    return {"workload": "acoustics2d_gaussian_walls_unsplit_mc_4096", "grid": [n, n],
+            "value": n * n * steps / wall, "unit": "cell-updates/s", "steps": steps,
+            "ms_per_step": 1e3 * wall / steps, "kernel_ms": kms, "kernels_timed": nk,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "kernel": "k_unsplit_ctl<Acoustics<double>> + k_step_ctl",
+                         "bytes_per_cell": 48}}
+
+
 def ensemble_leg(steps=32, members=512, seed=4):
     """512 independent 2D Euler Roe/MC unsplit problems (sides log-uniform in
     [64, 512], independent in x and y; seeded quadrant states as the headline
@@ -587,7 +634,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the runs to t = 1.0 of both SharpClaw paths")
-    ap.add_argument("--extras", default="classic,acoustics,ensemble,euler3d",
+    ap.add_argument("--extras", default="classic,acoustics,classic_roofline,ensemble,euler3d",
                     help="comma list of extra workloads to time beside the headline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a plain 256 MiB device copy (known bytes for counter runs)")
@@ -644,7 +691,7 @@ def main():
         rec["full_run"] = full_run()
         print("[bench] full run done", file=sys.stderr, flush=True)
     legs = {"classic": classic_leg, "acoustics": acoustics_leg, "ensemble": ensemble_leg,
-            "euler3d": euler3d_leg}
+            "euler3d": euler3d_leg, "classic_roofline": classic_roofline_leg}
     for name in [x for x in args.extras.split(",") if x]:
         try:
             rec[name] = legs[name]()

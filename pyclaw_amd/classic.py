@@ -249,19 +249,26 @@ class ClawSolver2D(ClawSolver):
                 lo, hi, self.reflect_index, self._method, self._mthlim, dx, dy)
         launched = 0
         host = L.StepCtl()
-        while True:
-            n = min(self.graph_chunk, max_steps - launched)
-            if n <= 0:
-                break
-            if self.use_graph and n == self.graph_chunk:
-                self._replay_graph(bufs, args, n)
-            else:
-                K.step2_unsplit_run(*args, n)
-            launched += n
+        t_now, dt_now = solution.t, self.dt
+        while launched < max_steps:
+            # launch as many attempts as the remaining time needs (estimated
+            # from the current dt) before reading the controller back: the
+            # launches after it is done exit at once
+            todo = _attempts_to_launch(max_steps - launched, tend, t_now, dt_now,
+                                       self.graph_chunk)
+            while todo > 0:
+                n = min(self.graph_chunk, todo)
+                if self.use_graph and n == self.graph_chunk:
+                    self._replay_graph(bufs, args, n)
+                else:
+                    K.step2_unsplit_run(*args, n)
+                launched += n
+                todo -= n
             raw = self._ctl.cpu().numpy().tobytes()
             ctypes_memmove(host, raw)
             if host.done:
                 break
+            t_now, dt_now = host.t, host.dt
         solution.t = host.t
         self.dt = host.dt
         self.cfl.set_global_max(host.cfl)
@@ -365,6 +372,19 @@ class ClawSolver3D(ClawSolver):
 def ctypes_memmove(struct, raw):
     import ctypes
     ctypes.memmove(ctypes.addressof(struct), raw, ctypes.sizeof(struct))
+
+
+def _attempts_to_launch(left, tend, t, dt, chunk):
+    """Device-loop attempts to launch before the next read of the controller:
+    all that are left for a fixed count (tend infinite), else the estimate of
+    the attempts to tend at the current dt plus two, at least one chunk."""
+    import math
+    if not math.isfinite(tend) or not dt > 0:
+        return left
+    est = (tend - t) / dt
+    if not math.isfinite(est):
+        return min(left, chunk)
+    return min(left, max(chunk, int(est) + 2))
 
 
 def _fill_bcs_host(solver, state, qbc, lo, hi, aux=False):

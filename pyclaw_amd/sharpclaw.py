@@ -13,6 +13,7 @@ import numpy as np
 import torch
 
 from . import kernels as K
+from .classic import _attempts_to_launch
 from .solver import BC, Solver
 
 
@@ -768,17 +769,24 @@ class SharpClawSolver(Solver):
         self._ctl.copy_(torch.frombuffer(bytearray(bytes(c)), dtype=torch.uint8))
         launched = 0
         host = L.SharpCtl()
+        t_now, dt_now = solution.t, self.dt
         while launched < n:
-            m = min(self.graph_chunk, n - launched)
-            if self.use_graph:
-                self._graph(bufs, args, kw, m).replay()
-            else:
-                K.sharp2_ssp33_run(*args, m, **kw)
-            launched += m
+            # as many attempts as the remaining time needs (estimated from the
+            # current dt) before reading the controller back
+            todo = _attempts_to_launch(n - launched, tend, t_now, dt_now, self.graph_chunk)
+            while todo > 0:
+                m = min(self.graph_chunk, todo)
+                if self.use_graph:
+                    self._graph(bufs, args, kw, m).replay()
+                else:
+                    K.sharp2_ssp33_run(*args, m, **kw)
+                launched += m
+                todo -= m
             ctypes.memmove(ctypes.addressof(host), bytes(self._ctl.cpu().numpy().tobytes()),
                            ctypes.sizeof(host))
             if host.done:
                 break
+            t_now, dt_now = host.t, host.dt
         solution.t = host.t
         self.dt = host.dt
         self.dt_old = host.dt_old if host.have_dt_old else None

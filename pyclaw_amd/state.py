@@ -3,8 +3,9 @@
 `state.q` is the reference's host array (meqn, mx, my[, mz]) in Fortran order;
 users read and write it as before (e.g. `state.q[0,:,:] = ...` in qinit).  The
 solvers keep the authoritative copy on the GPU between steps: reading `state.q`
-downloads it, and a read or write marks the device copy stale so the next step
-uploads it again.  Nothing is copied while a solver steps.
+downloads it, and the next step uploads it again only if the host array was
+changed meanwhile (into the same device buffers).  Nothing is copied while a
+solver steps.
 """
 import copy
 
@@ -30,8 +31,10 @@ class State(object):
         self.F = None
         self._q = self.new_array(num_eqn)
         self.aux = self.new_array(num_aux) if num_aux > 0 else None
-        self._dev = None          # (q_tensor, comp_tensor_or_None) when the device is newer
-        self._dev_stale = True    # device copy must be re-uploaded before use
+        self._dev = None          # (q_tensor, comp_tensor_or_None) on the device
+        self._dev_stale = True    # the host array was handed out since the device copy
+        self._host_current = True # the host array holds the newest state
+        self._snap = None         # what the last download gave (see device_q)
         self._aux_dev = None      # device copy of aux (uploaded at every step that uses it)
 
     # ---- geometry passthrough
@@ -68,27 +71,54 @@ class State(object):
         self._dev_stale = True
 
     def _pull(self):
-        if self._dev is not None:
+        """Download the device state into the host array once per step run; a
+        copy of what was downloaded tells device_q later whether the host array
+        was changed meanwhile."""
+        if self._dev is not None and not self._host_current:
             from .device import q_from_device
             self._q = q_from_device(self._dev[0])
-            self._dev = None
+            self._snap = self._q.copy(order="K")
+            self._host_current = True
 
     def _pull_discard(self):
         self._dev = None
+        self._snap = None
+        self._host_current = True
 
     def device_q(self, dtype_code, ensure=True):
-        """(q, comp) device tensors for the solver; uploads when stale."""
+        """(q, comp) device tensors for the solver.  After the host array was
+        handed out (state.q), it is compared with what was downloaded: unchanged,
+        the device tensors (and the Kahan compensation) stay as they are;
+        changed, the new values go into the same device buffers (captured graphs
+        keep their addresses) and the compensation restarts at zero."""
         import torch
         from .device import q_to_device, torch_dtype
         want = torch_dtype(dtype_code)
-        if self._dev is not None and self._dev[0].dtype == want and not self._dev_stale:
-            return self._dev
-        if self._dev is not None:
+        dev = self._dev
+        if dev is not None and dev[0].dtype == want:
+            if not self._dev_stale:
+                return dev
+            if (self._snap is not None and self._q is not None
+                    and self._q.shape == self._snap.shape
+                    and np.array_equal(self._q, self._snap)):
+                self._dev_stale = False
+                return dev
+        if dev is not None and not self._host_current:
             self._pull()
         q = q_to_device(np.asarray(self._q), dtype_code)
-        comp = torch.zeros_like(q) if dtype_code == 1 else None
-        self._dev = (q, comp)
+        if dev is not None and dev[0].dtype == want and dev[0].shape == q.shape:
+            dev[0].copy_(q)
+            comp = dev[1]
+            if comp is not None:
+                comp.zero_()
+            elif dtype_code == 1:
+                comp = torch.zeros_like(q)
+            self._dev = (dev[0], comp)
+        else:
+            self._dev = (q, torch.zeros_like(q) if dtype_code == 1 else None)
         self._dev_stale = False
+        self._snap = None
+        self._host_current = True
         return self._dev
 
     def device_aux(self, dtype_code):
@@ -103,6 +133,8 @@ class State(object):
         """The solver hands back the newest state (device tensors)."""
         self._dev = (q, comp)
         self._dev_stale = False
+        self._host_current = False
+        self._snap = None
 
     def new_array(self, dof):
         if dof == 0:
