@@ -267,7 +267,10 @@ int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
  *   clawhip_mlp_flag_list), recon (as clawhip_sharp2_stage), and the judge:
  *   a step that leaves a field of positive_mask not > 0 (or not finite) is
  *   recomputed with full WENO5 into the same buffers (dq_dt_full: work of the
- *   state's size).  Boundary codes 1..3. */
+ *   state's size).  Boundary codes 1..3.
+ *   stage_events (may be NULL; not while the stream is captured): 6*nsteps
+ *   hipEvent_t handles, recorded before and after the march launch of stages
+ *   1, 2, 3 of each attempt (timing). */
 int clawhip_sharp2_ssp33_run(const clawhip_rp *rp, int dtype, int char_decomp,
                              const clawhip_geom *g, const int *bc_lower, const int *bc_upper,
                              const int *reflect, double dx, double dy, void *q0, void *q1,
@@ -275,7 +278,8 @@ int clawhip_sharp2_ssp33_run(const clawhip_rp *rp, int dtype, int char_decomp,
                              unsigned char *flags, int *flag_list, int *flag_count, void *recon,
                              void *dq_dt_full, const float *mlp_params, float flat,
                              float logit_threshold, unsigned positive_mask,
-                             clawhip_sharpctl *ctl_dev, int nsteps, void *stream);
+                             clawhip_sharpctl *ctl_dev, int nsteps, void *const *stage_events,
+                             void *stream);
 
 /* The elementwise middle of SSP104 (sharpclaw/solver.py:418-419):
  *   q = q/25. + 0.36*s1;  s1 = 15.*q - 5.*s1   over n elements. */

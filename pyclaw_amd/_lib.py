@@ -109,7 +109,7 @@ _SIGS = {
                              _IP, _I, _P, _P],
     "clawhip_sharp2_ssp33_run": [ctypes.POINTER(RP), _I, _I, ctypes.POINTER(Geom), _IP, _IP, _IP,
                                  _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                 _F, _F, ctypes.c_uint, _P, _I, _P],
+                                 _F, _F, ctypes.c_uint, _P, _I, _P, _P],
     "clawhip_ssp104_mid": [_I, _P, _P, ctypes.c_longlong, _P],
     "clawhip_lmm_update": [_I, _I, ctypes.c_longlong, _P, _P, _P, _P,
                            ctypes.POINTER(ctypes.c_double), _P, _P],

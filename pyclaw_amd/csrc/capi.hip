@@ -250,7 +250,8 @@ int clawhip_sharp2_ssp33_run(const clawhip_rp* rp, int dtype, int char_decomp,
                              unsigned char* flags, int* flag_list, int* flag_count, void* recon,
                              void* dq_dt_full, const float* mlp_params, float flat,
                              float logit_threshold, unsigned positive_mask,
-                             clawhip_sharpctl* ctl_dev, int nsteps, void* stream) {
+                             clawhip_sharpctl* ctl_dev, int nsteps, void* const* stage_events,
+                             void* stream) {
   static_assert(sizeof(clawhip_sharpctl) == sizeof(SharpCtl), "clawhip_sharpctl layout");
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !q0 || !q1 || !dq_dt || !r2 ||
       !ctl_dev || nsteps < 0)
@@ -274,6 +275,7 @@ int clawhip_sharp2_ssp33_run(const clawhip_rp* rp, int dtype, int char_decomp,
   b.flags = flags; b.flag_list = flag_list; b.flag_count = flag_count; b.recon = recon;
   b.params = mlp_params; b.flat = flat; b.threshold = logit_threshold;
   b.pos_mask = positive_mask;
+  b.stage_events = stage_events;
   return launch_sharp_ssp33_run(*rp, dtype, char_decomp, to_geom(g), bc_lower, bc_upper, reflect,
                                dx, dy, b, (SharpCtl*)ctl_dev, nsteps, (hipStream_t)stream);
 }

@@ -122,6 +122,9 @@ struct SharpRunBufs {
   const float* params = nullptr;
   float flat = 0.0f, threshold = 0.0f;
   unsigned pos_mask = 0u;
+  // optional timing: hipEvent_t handles, two per main stage per attempt, recorded
+  // around that stage's march launch (eager runs only)
+  void* const* stage_events = nullptr;
 };
 
 int launch_sharp_ssp33_run(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,

@@ -849,7 +849,7 @@ int march_rows(Kernel kernel, int nstrips, int my) {
 // The march launch of a stage (and, with flags, the full-WENO5 pass over the
 // flagged interfaces before it).
 template <class RS, typename T, bool CHAR>
-int launch_args(const SharpArgs<T>& a, hipStream_t stream) {
+int launch_args(const SharpArgs<T>& a, hipStream_t stream, void* const* ev = nullptr) {
   const bool flags = a.flags != nullptr;
   if constexpr (RS::maux > 0) {
     if (flags) return CLAWHIP_E_ARG;   // the indicator path takes solvers without aux
@@ -865,12 +865,16 @@ int launch_args(const SharpArgs<T>& a, hipStream_t stream) {
   if (flags) {
     if constexpr (RS::maux == 0) {
       hipLaunchKernelGGL((k_weno_flagged<RS, T, CHAR>), dim3(2048), dim3(256), 0, stream, a);
+      if (ev) hipEventRecord((hipEvent_t)ev[0], stream);
       hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, true>), dim3(blocks), dim3(256), 0, stream,
                          ma);
+      if (ev) hipEventRecord((hipEvent_t)ev[1], stream);
     }
   } else {
+    if (ev) hipEventRecord((hipEvent_t)ev[0], stream);
     hipLaunchKernelGGL((k_sharp_march<RS, T, CHAR, false>), dim3(blocks), dim3(256), 0, stream,
                        ma);
+    if (ev) hipEventRecord((hipEvent_t)ev[1], stream);
   }
   if (a.ctl && a.ctl_phase != PH_NONE)
     hipLaunchKernelGGL(k_sharp_ctl, dim3(1), dim3(64), 0, stream, a.ctl, a.gate, a.ctl_phase);
@@ -1011,9 +1015,10 @@ int run_ssp33(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
       if (!rc) rc = launch_flag_list(b.flags, g.mx, g.my, b.flag_list, b.flag_count, stream,
                                      &ctl->run_flags);
     }
-    if (!rc) rc = launch_args<RS, T, CHAR>(s1, stream);
-    if (!rc) rc = launch_args<RS, T, CHAR>(s2, stream);
-    if (!rc) rc = launch_args<RS, T, CHAR>(s3, stream);
+    void* const* ev = b.stage_events ? b.stage_events + 6LL * n : nullptr;
+    if (!rc) rc = launch_args<RS, T, CHAR>(s1, stream, ev);
+    if (!rc) rc = launch_args<RS, T, CHAR>(s2, stream, ev ? ev + 2 : nullptr);
+    if (!rc) rc = launch_args<RS, T, CHAR>(s3, stream, ev ? ev + 4 : nullptr);
     if (judge) {
       if (!rc) rc = launch_args<RS, T, CHAR>(f1, stream);
       if (!rc) rc = launch_args<RS, T, CHAR>(f2, stream);

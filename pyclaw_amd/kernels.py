@@ -204,11 +204,13 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
 
 def sharp2_ssp33_run(rp, char_decomp, bufs, comps, ctl_dev, bc_lower, bc_upper, reflect, dx, dy,
                      dq_dt, r2, aux23, nsteps, indicator=None, dq_dt_full=None, recon=None,
-                     positive=()):
+                     positive=(), stage_events=None):
     """nsteps device-controlled SSP33 attempts (clawhip_sharp2_ssp33_run) on the
     planar ping-pong buffers bufs = (q0, q1) (comps: their Kahan compensations
     or (None, None)); indicator: the learned indicator's path (its flags, list,
-    network) with the positivity judge over the fields listed in `positive`."""
+    network) with the positivity judge over the fields listed in `positive`;
+    stage_events: 6*nsteps recorded torch.cuda.Events, recorded again around the
+    march of stages 1-3 of every attempt (eager runs only)."""
     meqn, my, mx = bufs[0].shape
     g = L.geom2(mx, my)
     code = _dtype_code(bufs[0])
@@ -225,8 +227,23 @@ def sharp2_ssp33_run(rp, char_decomp, bufs, comps, ctl_dev, bc_lower, bc_upper, 
         L.ints(bc_upper), L.ints(reflect), float(dx), float(dy), _ptr(bufs[0]), _ptr(bufs[1]),
         _ptr(comps[0]), _ptr(comps[1]), _ptr(dq_dt), _ptr(r2), _ptr(aux23), _ptr(fl), _ptr(lst),
         _ptr(cnt), _ptr(recon), _ptr(dq_dt_full), _ptr(params), float(flat), float(thr), mask,
-        _ptr(ctl_dev), int(nsteps), _stream())
+        _ptr(ctl_dev), int(nsteps), _event_handles(stage_events, 6 * int(nsteps)), _stream())
     L.check(rc, "clawhip_sharp2_ssp33_run")
+
+
+def _event_handles(events, n):
+    """A C array of the hipEvent_t handles of n recorded torch.cuda.Events, or None."""
+    if events is None:
+        return None
+    if len(events) < n:
+        raise ValueError("stage_events needs %d events" % n)
+    arr = (ctypes.c_void_p * n)()
+    for i in range(n):
+        h = int(events[i].cuda_event)
+        if h == 0:
+            raise ValueError("stage events must have been recorded once (their handle exists)")
+        arr[i] = h
+    return arr
 
 
 def lmm_update(mode, q, u, coeffs, out, R=None, v=None):

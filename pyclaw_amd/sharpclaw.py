@@ -98,6 +98,8 @@ class SharpClawSolver(Solver):
         self.graph_chunk = 16
         self._graphs = {}
         self._ctl = None
+        self._stage_events = None   # time_stages: events of the eager launches
+        self._stage_ev_pos = 0
         super(SharpClawSolver, self).__init__(riemann_solver, claw_package)
 
     # ---- sharpclaw/solver.py:202-265
@@ -627,6 +629,28 @@ class SharpClawSolver(Solver):
             accepted += acc
         return attempts, accepted
 
+    def time_stages(self, solution, nsteps):
+        """nsteps attempts of the device loop launched without graphs, with HIP
+        events around the march launch of every main stage (on the stream it
+        runs on).  Returns the march durations in ms, three per attempt run on
+        the device (indicator audit attempts run on the host path and are not
+        timed)."""
+        n = int(nsteps)
+        events = [torch.cuda.Event(enable_timing=True) for _ in range(6 * n)]
+        st = torch.cuda.current_stream()
+        for e in events:
+            e.record(st)   # creates the event handles
+        use, self.use_graph = self.use_graph, False
+        self._stage_events, self._stage_ev_pos = events, 0
+        try:
+            self.run_steps(solution, n)
+            used = self._stage_ev_pos
+        finally:
+            self.use_graph = use
+            self._stage_events, self._stage_ev_pos = None, 0
+        torch.cuda.synchronize()
+        return [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(used // 2)]
+
     def _host_attempt(self, solution, tstart, tend):
         """One attempt of Solver.evolve_to_time's loop (solver.py:607-651) on the
         host path; returns (accepted 0/1, reached tend)."""
@@ -778,6 +802,11 @@ class SharpClawSolver(Solver):
                 m = min(self.graph_chunk, todo)
                 if self.use_graph:
                     self._graph(bufs, args, kw, m).replay()
+                elif self._stage_events is not None:
+                    p0 = self._stage_ev_pos
+                    K.sharp2_ssp33_run(*args, m, stage_events=self._stage_events[p0:p0 + 6 * m],
+                                       **kw)
+                    self._stage_ev_pos = p0 + 6 * m
                 else:
                     K.sharp2_ssp33_run(*args, m, **kw)
                 launched += m
