@@ -94,7 +94,7 @@ class Solution(object):
     def read(self, frame, path="./_output", file_format=None, file_prefix="fort",
              read_aux=True, options={}, **kargs):
         """solution.py:305-363: the format comes from the fort.t file when it
-        names one, else from file_format."""
+        names one, else from file_format; ascii or binary (binary64, binary32)."""
         import os
         from .fileio import ascii as A
         path = os.path.expandvars(os.path.expanduser(path))
@@ -107,12 +107,18 @@ class Solution(object):
             file_format = named
         if file_format is None:
             file_format = "ascii"
-        _fileio(file_format).read(self, frame, path, file_prefix=prefix, read_aux=read_aux,
-                                  options=options)
+        options = dict(options)
+        options["format"] = file_format       # binary / binary64 / binary32 select the type
+        _fileio(file_format, reading=True).read(self, frame, path, file_prefix=prefix,
+                                                read_aux=read_aux, options=options)
 
 
-def _fileio(file_format):
+def _fileio(file_format, reading=False):
     if file_format == "ascii":
         from .fileio import ascii
         return ascii
-    raise NotImplementedError("file format %r is not supported (ascii is)" % (file_format,))
+    if reading and str(file_format)[:6] == "binary":
+        from .fileio import binary
+        return binary
+    raise NotImplementedError("file format %r is not supported (ascii is; binary frames "
+                              "are read)" % (file_format,))
