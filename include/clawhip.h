@@ -57,11 +57,15 @@ enum {
   CLAWHIP_RP_SHALLOW_HLLE = 9,     /* shallow_hlle_2D (clawpack/riemann): HLLE shallow
                                    * water; no transverse solver (split or unsplit with
                                    * transverse_waves = 0; SharpClaw) */
-  CLAWHIP_RP_EULER5 = 10           /* euler_5wave_2D (clawpack/riemann; in-tree copy
+  CLAWHIP_RP_EULER5 = 10,          /* euler_5wave_2D (clawpack/riemann; in-tree copy
                                    * development/rp_approaches/rpn2_euler_5wave.f,
                                    * rpt2_euler_5wave.f): euler_4wave_2D plus a passive
                                    * tracer q[4]; classic 2D steps, SharpClaw with
                                    * char_decomp 0 */
+  CLAWHIP_RP_SHALLOW_FWAVE = 11    /* shallow_bathymetry_fwave_2D (clawpack/riemann): shallow
+                                   * water over bathymetry b = aux[0], f-waves (set
+                                   * rp.fwave = 1); no transverse solver (split, or unsplit
+                                   * with transverse_waves = 0); classic 2D steps */
 };
 
 /* Boundary codes: pyclaw.BC (pyclaw/solver.py:13-19) plus GIVEN. */
@@ -77,6 +81,11 @@ typedef struct {
   double grav;              /* shallow water */
   double u, v;              /* advection */
   int meqn;                 /* CLAWHIP_RP_VC_ADVECTION: number of fields */
+  int fwave;                /* the solver returns f-waves: step2 / step2ds's use_fwave
+                             * (classic/flux2.f90:134-160: dsign(1, s) in place of |s| in
+                             * the second-order correction); classic 2D steps */
+  double dry_tolerance;     /* shallow water over bathymetry */
+  double sea_level;
 } clawhip_rp;
 
 typedef struct {
@@ -182,7 +191,14 @@ int clawhip_step2_unsplit_run(const clawhip_rp *rp, int dtype, const clawhip_geo
  *   aux:       the solver's aux fields (state.aux; required exactly when the
  *              solver has them, method[6] = their number), planar like q in the
  *              state's type, geometry gaux; aux_bc_lower/upper: the
- *              solver.aux_bc codes (a wall mirrors without negating). */
+ *              solver.aux_bc codes (a wall mirrors without negating).
+ * q_out either compact (gout->ghosts = 0: the new interior) or the whole qbc
+ * (gout->ghosts = num_ghost, with q_in storing its ghosts too): then every cell
+ * the reference's two sweeps write is written as they write it -- the x-sweep's
+ * ghost rows, the y-sweep's ghost columns -- and the corners keep q_in's values,
+ * so a caller that keeps qbc between steps (pyclaw/solver.py:341-343; custom BC
+ * functions that set only some ghost cells) sees the reference's ghost values;
+ * work / comp_work are then unused (the sweeps run in q_out / comp_out). */
 int clawhip_step2_split(const clawhip_rp *rp, int dtype, const void *q_in,
                         const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
                         const void *comp_in, void *comp_out, void *work, void *comp_work,
@@ -191,6 +207,24 @@ int clawhip_step2_split(const clawhip_rp *rp, int dtype, const void *q_in,
                         double dy, double dt, const void *aux, const clawhip_geom *gaux,
                         const int *aux_bc_lower, const int *aux_bc_upper, double *cfl_dev,
                         void *stream);
+
+/* One unsplit 3D classic step: replaces fmod.step3(...) as called by
+ * ClawSolver3D.step_hyperbolic with dimensional_split=False
+ * (pyclaw/classic/solver.py:694-699; Fortran classic/step3.f90:2-597 with
+ * flux3.f90:2-594: transverse and double-transverse propagation), FP64
+ * (dtype CLAWHIP_F64), rp->kind = CLAWHIP_RP_EULER3 (euler_3D's rpn3, rpt3,
+ * rptt3).  method[2] = 0, 10, 11, 20, 21 or 22 (ClawSolver3D.trans_inc = 11,
+ * trans_cor = 22; 20-22 need method[1] = 2); no aux, no capacity.  q_in with
+ * geometry gin (compact, or with num_ghost stored ghosts for BC code 4), q_out
+ * compact; work: clawhip_step3_unsplit_work_elems(meqn, mx, my, mz) doubles
+ * (the slices' flux3 results).  Bitwise equal to the reference's step3. */
+long long clawhip_step3_unsplit_work_elems(int meqn, int mx, int my, int mz);
+int clawhip_step3_unsplit(const clawhip_rp *rp, int dtype, const void *q_in,
+                          const clawhip_geom *gin, void *q_out, const clawhip_geom *gout,
+                          void *work, const int *bc_lower, const int *bc_upper,
+                          const int *reflect, int num_ghost, const int *method,
+                          const int *mthlim, double dx, double dy, double dz, double dt,
+                          double *cfl_dev, void *stream);
 
 /* One dimension-split 3D step: ClawSolver3D.step_hyperbolic with
  * dimensional_split=True (pyclaw/classic/solver.py:676-692), i.e. fmod.step3ds

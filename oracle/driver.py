@@ -30,7 +30,7 @@ class OracleRun:
                  aux_bc_upper=None, before_step=None, step_source=None, source_split=1,
                  index_capa=-1, auxbc_fn=None, lmm_steps=4, user_bc_lower=None,
                  rk_a=None, rk_b=None, lmm_alpha=None, lmm_beta=None, dq_src=None,
-                 rk_c=None):
+                 rk_c=None, fwave=False):
         self.rp = rp
         self.ndim = q.ndim - 1
         self.delta = [float(d) for d in delta]
@@ -73,6 +73,8 @@ class OracleRun:
         self.user_bc_lower = user_bc_lower
         # dq_src(run, q, dt) -> the source increment SharpClaw adds to dq
         self.dq_src = dq_src
+        # the Riemann solver returns f-waves (ClawSolver.fwave -> step2's use_fwave)
+        self.fwave = bool(fwave)
         # multistep state (SSPLMMk2 / k3)
         self.lmm_steps = lmm_steps
         self.sspcoeff = None
@@ -97,14 +99,15 @@ class OracleRun:
         ng = self.ng
         sl = (slice(None),) + tuple(slice(ng, -ng) for _ in range(self.ndim))
         self.qbc[sl] = q
-        if self.ndim == 2 and self.user_bc_lower is not None:
+        if self.user_bc_lower is not None:
             # dimension by dimension, lower side then upper (solver.py:351-367)
-            for d in range(2):
-                lo, hi = [O.CUSTOM] * 2, [O.CUSTOM] * 2
+            nd = self.ndim
+            for d in range(nd):
+                lo, hi = [O.CUSTOM] * nd, [O.CUSTOM] * nd
                 lo[d], hi[d] = self.bc_lower[d], self.bc_upper[d]
                 if lo[d] == O.CUSTOM:
                     self.user_bc_lower(self, d, self.qbc)
-                O.bc2(self.qbc, ng, lo, hi)
+                (O.bc2 if nd == 2 else O.bc3)(self.qbc, ng, lo, hi)
         elif self.ndim == 2:
             O.bc2(self.qbc, ng, self.bc_lower, self.bc_upper)
         else:
@@ -225,9 +228,9 @@ class OracleRun:
                 meth = O.method_array(order=self.order, dimensional_split=True,
                                       dt_variable=self.dt_variable, num_aux=maux)
                 c1 = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=1,
-                             aux=abc)
+                             use_fwave=self.fwave, aux=abc)
                 c2 = O.step2(self.rp, qbc, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=2,
-                             aux=abc)
+                             use_fwave=self.fwave, aux=abc)
                 cfl = max(c1, c2)
             else:
                 maux = 0 if self.aux is None else self.aux.shape[0]
@@ -236,15 +239,20 @@ class OracleRun:
                                       dt_variable=self.dt_variable, num_aux=maux,
                                       index_capa=self.index_capa)
                 cfl = O.step2(self.rp, qold, qbc, ng, dx, dy, dt, meth, self.mthlim, ids=0,
-                              aux=abc)
+                              use_fwave=self.fwave, aux=abc)
         else:
             dx, dy, dz = self.delta
-            meth = O.method_array(order=self.order, dimensional_split=True,
-                                  dt_variable=self.dt_variable)
-            c1 = O.step3ds(self.rp, qold, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 1)
-            c2 = O.step3ds(self.rp, qbc, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 2)
-            c3 = O.step3ds(self.rp, qbc, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 3)
-            cfl = max(c1, c2, c3)
+            if self.split:
+                meth = O.method_array(order=self.order, dimensional_split=True,
+                                      dt_variable=self.dt_variable)
+                c1 = O.step3ds(self.rp, qold, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 1)
+                c2 = O.step3ds(self.rp, qbc, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 2)
+                c3 = O.step3ds(self.rp, qbc, qbc, ng, dx, dy, dz, dt, meth, self.mthlim, 3)
+                cfl = max(c1, c2, c3)
+            else:   # step3 (classic/solver.py:694-699); transverse = method(3)
+                meth = O.method_array(order=self.order, transverse=self.transverse,
+                                      dt_variable=self.dt_variable)
+                cfl = O.step3(self.rp, qold, qbc, ng, dx, dy, dz, dt, meth, self.mthlim)
         self.cfl = max(self.cfl, cfl)
         self.q = self._interior(qbc)
         if self.cfl >= self.cfl_max:

@@ -36,6 +36,7 @@ int orc_rp_meqn(int kind) {
     case ORC_RP_VC_ACOUSTICS: return 3;
     case ORC_RP_SHALLOW_HLLE: return 3;
     case ORC_RP_EULER5: return 5;
+    case ORC_RP_SHALLOW_FWAVE: return 3;
   }
   return 0;   /* ORC_RP_VC_ADVECTION: any (orc_rp.meqn) */
 }
@@ -52,6 +53,7 @@ int orc_rp_mwaves(int kind) {
     case ORC_RP_VC_ACOUSTICS: return 2;
     case ORC_RP_SHALLOW_HLLE: return 2;
     case ORC_RP_EULER5: return 5;
+    case ORC_RP_SHALLOW_FWAVE: return 3;
   }
   return 0;
 }
@@ -563,7 +565,11 @@ static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const d
  *      discrete maximum principle of the first-order corner-transport step on
  *      random edge-velocity fields; no fixture tells them apart: the annulus
  *      field has zero radial velocity and a theta velocity that depends on r
- *      only, so those choices leave its solution unchanged). */
+ *      only, so those choices leave its solution unchanged);
+ *  11  shallow_bathymetry_fwave_2D: the Roe speeds instead of Einfeldt's
+ *      (caught by the sill fixture);
+ *  12  euler_3D rpt3: the normal velocity as the transverse speed (caught by
+ *      the transverse Roe property and by the 3D shock-bubble fixture). */
 static int g_plant = 0;
 void orc_set_plant(int k) { g_plant = k; }
 
@@ -594,9 +600,83 @@ static void rpn_vc_acoustics(int ixy, const double *qL, const double *qR, const 
   }
 }
 
+/* shallow_bathymetry_fwave_2D rpn2 (clawpack.riemann
+ * rpn2_shallow_bathymetry_fwave.f90, published form; not in the reference
+ * tree): the Roe f-wave solver of Bale, LeVeque, Mitran & Rossmanith (2002)
+ * for the shallow-water equations with the bathymetry source -g h b_x folded
+ * into the flux difference.  Interface between cells L (qL, auxL) and R.
+ *   velocities: u = hu/h where h > dry_tolerance, else 0 (both sides dry:
+ *     no waves, zero speeds);
+ *   Roe averages: u^ = (sqrt(hL) uL + sqrt(hR) uR) / (sqrt(hL) + sqrt(hR)),
+ *     hbar = (hL + hR)/2, c^ = sqrt(g hbar);
+ *   speeds (Einfeldt): s1 = min(uL - sqrt(g hL), u^ - c^),
+ *     s3 = max(uR + sqrt(g hR), u^ + c^), s2 = (s1 + s3)/2;
+ *   flux difference with the source: d1 = (hu)R - (hu)L,
+ *     d2 = (hu^2 + g h^2/2)R - (...)L + g hbar (bR - bL), d3 = (huv)R - (huv)L;
+ *   f-wave strengths on the eigenvectors (1, s1, vL), (0, 0, 1), (1, s3, vR):
+ *     b1 = (s3 d1 - d2)/(s3 - s1), b3 = (d2 - s1 d1)/(s3 - s1),
+ *     b2 = d3 - b1 vL - b3 vR;
+ *   fluctuations: each f-wave goes to amdq (s < 0) or apdq (s > 0), half to
+ *   each when s = 0.
+ * Pinned by the shallow_2d sill fixture (examples/shallow_2d/test_shallow2d.py:
+ * 38-39; L1 difference 8e-16 against the bar 1e-6), which never dries: the
+ * dry-state branch is unpinned. */
+static void rpn_shallow_fwave(const orc_rp *p, int ixy, const double *qL, const double *qR,
+                              const double *auxL, const double *auxR, double *fw, double *s,
+                              double *amdq, double *apdq) {
+  const int mu = ixy == 1 ? 1 : 2, mv = ixy == 1 ? 2 : 1;
+  const double g = p->grav, dry = p->dry_tolerance;
+  for (int k = 0; k < 9; k++) fw[k] = 0.0;
+  for (int k = 0; k < 3; k++) s[k] = amdq[k] = apdq[k] = 0.0;
+  const double hl = qL[0], hr = qR[0];
+  if (hl <= dry && hr <= dry) return;
+  double ul = 0.0, vl = 0.0, ur = 0.0, vr = 0.0;
+  if (hl > dry) { ul = qL[mu] / hl; vl = qL[mv] / hl; }
+  if (hr > dry) { ur = qR[mu] / hr; vr = qR[mv] / hr; }
+  const double sl = sqrt(hl), sr = sqrt(hr);
+  const double hbar = 0.5 * (hl + hr);
+  const double uhat = (sl * ul + sr * ur) / (sl + sr);
+  const double chat = sqrt(g * hbar);
+  s[0] = uhat - chat;
+  s[1] = uhat;
+  s[2] = uhat + chat;
+  /* Einfeldt's bounds on the outer speeds; the middle speed their mean
+   * (planted 11: the Roe speeds u^ - c^, u^, u^ + c^) */
+  if (g_plant != 11) {
+    s[0] = dmin(ul - sqrt(g * hl), s[0]);
+    s[2] = dmax(ur + sqrt(g * hr), s[2]);
+    s[1] = 0.5 * (s[0] + s[2]);
+  }
+  const double d1 = hr * ur - hl * ul;
+  const double d2 = (hr * ur * ur + 0.5 * g * hr * hr) - (hl * ul * ul + 0.5 * g * hl * hl) +
+                    g * hbar * (auxR[0] - auxL[0]);
+  const double d3 = hr * ur * vr - hl * ul * vl;
+  const double b1 = (s[2] * d1 - d2) / (s[2] - s[0]);
+  const double b3 = (d2 - s[0] * d1) / (s[2] - s[0]);
+  const double b2 = d3 - b1 * vl - b3 * vr;
+  fw[0] = b1;
+  fw[mu] = b1 * s[0];
+  fw[mv] = b1 * vl;
+  fw[3 + mv] = b2;
+  fw[6] = b3;
+  fw[6 + mu] = b3 * s[2];
+  fw[6 + mv] = b3 * vr;
+  for (int mw = 0; mw < 3; mw++)
+    for (int m = 0; m < 3; m++) {
+      const double f = fw[3 * mw + m];
+      if (s[mw] < 0.0) amdq[m] += f;
+      else if (s[mw] > 0.0) apdq[m] += f;
+      else { amdq[m] += 0.5 * f; apdq[m] += 0.5 * f; }
+    }
+}
+
 void orc_rpn_aux(const orc_rp *p, int ixy, const double *qL, const double *qR,
                  const double *auxL, const double *auxR, double *wave, double *s,
                  double *amdq, double *apdq) {
+  if (p->kind == ORC_RP_SHALLOW_FWAVE) {
+    rpn_shallow_fwave(p, ixy, qL, qR, auxL, auxR, wave, s, amdq, apdq);
+    return;
+  }
   if (p->kind == ORC_RP_VC_ACOUSTICS) {
     rpn_vc_acoustics(ixy, qL, qR, auxL, auxR, wave, s, amdq, apdq);
     return;
@@ -674,6 +754,72 @@ void orc_rpt(const orc_rp *p, int ixy, int imp, const double *qL, const double *
     case ORC_RP_ADVECTION: rpt_advection(p, ixy, asdq, bmasdq, bpasdq); break;
     default: break;
   }
+}
+
+/* euler_3D transverse solvers (clawpack.riemann rpt3_euler.f90 and
+ * rptt3_euler.f90, published form; the package is not in the reference tree).
+ * asdq at the interface between cells L | R of a slice in direction ixyz is
+ * split in the transverse direction icoor (2: the direction after ixyz in the
+ * cycle x -> y -> z, whose momentum is mv; 3: the one after that, mw) with the
+ * interface's Roe averages (those of rpn_euler3): three waves at v - a, v, v + a
+ * (icoor 2; w for icoor 3), the two shear parts lumped with the entropy wave as
+ * in the normal solver,
+ *   a4 = g1a2 (euv d1 + u du + v dv + w dw - dE),
+ *   a5 = (d_t + (a - v_t) d1 - a a4) / (2a), a1 = d1 - a4 - a5,
+ * and bm = sum min(s, 0) wave, bp = sum max(s, 0) wave.  rptt3 (the
+ * double-transverse split of flux3.f90:353-360, 479-486) is the same split of
+ * its input.  Pinned by the euler_3d shock_bubble fixture and by
+ * bm + bp = B(q_roe) asdq with B the transverse flux Jacobian. */
+static void rpt_euler3(const orc_rp *p, int ixyz, int icoor, const double *qL, const double *qR,
+                       const double *asdq, double *bm, double *bp) {
+  int mu, mv, mw;
+  if (ixyz == 1) { mu = 1; mv = 2; mw = 3; }
+  else if (ixyz == 2) { mu = 2; mv = 3; mw = 1; }
+  else { mu = 3; mv = 1; mw = 2; }
+  const double gamma1 = p->gamma - 1.0;
+  double rhsqrtl = sqrt(qL[0]);
+  double rhsqrtr = sqrt(qR[0]);
+  double pl = gamma1 * (qL[4] - 0.5 * (qL[1] * qL[1] + qL[2] * qL[2] + qL[3] * qL[3]) / qL[0]);
+  double pr = gamma1 * (qR[4] - 0.5 * (qR[1] * qR[1] + qR[2] * qR[2] + qR[3] * qR[3]) / qR[0]);
+  double rhsq2 = rhsqrtl + rhsqrtr;
+  double u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
+  double v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
+  double w = (qL[mw] / rhsqrtl + qR[mw] / rhsqrtr) / rhsq2;
+  double enth = (((qL[4] + pl) / rhsqrtl + (qR[4] + pr) / rhsqrtr)) / rhsq2;
+  double u2v2w2 = u * u + v * v + w * w;
+  double a2 = gamma1 * (enth - .5 * u2v2w2);
+  double a = sqrt(a2);
+  double g1a2 = gamma1 / a2;
+  double euv = enth - u2v2w2;
+  /* the transverse velocity vt and momentum index mt, the other two (o1, o2) */
+  const int mt = icoor == 2 ? mv : mw, m1 = mu, m2 = icoor == 2 ? mw : mv;
+  /* planted 12: the normal velocity u as the transverse speed */
+  const double vt = g_plant == 12 ? u : (icoor == 2 ? v : w), c1 = u, c2 = icoor == 2 ? w : v;
+  double a4 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] + w * asdq[mw] - asdq[4]);
+  double b1 = asdq[m1] - c1 * asdq[0];
+  double b2 = asdq[m2] - c2 * asdq[0];
+  double a5 = (asdq[mt] + (a - vt) * asdq[0] - a * a4) / (2.0 * a);
+  double a1 = asdq[0] - a4 - a5;
+  double wv[3][5], sb[3];
+  wv[0][0] = a1; wv[0][m1] = a1 * c1; wv[0][mt] = a1 * (vt - a); wv[0][m2] = a1 * c2;
+  wv[0][4] = a1 * (enth - vt * a); sb[0] = vt - a;
+  wv[1][0] = a4; wv[1][m1] = a4 * c1 + b1; wv[1][mt] = a4 * vt; wv[1][m2] = a4 * c2 + b2;
+  wv[1][4] = a4 * 0.5 * u2v2w2 + b1 * c1 + b2 * c2; sb[1] = vt;
+  wv[2][0] = a5; wv[2][m1] = a5 * c1; wv[2][mt] = a5 * (vt + a); wv[2][m2] = a5 * c2;
+  wv[2][4] = a5 * (enth + vt * a); sb[2] = vt + a;
+  for (int m = 0; m < 5; m++) {
+    bm[m] = 0.0;
+    bp[m] = 0.0;
+    for (int k = 0; k < 3; k++) {
+      bm[m] = bm[m] + dmin(sb[k], 0.0) * wv[k][m];
+      bp[m] = bp[m] + dmax(sb[k], 0.0) * wv[k][m];
+    }
+  }
+}
+
+void orc_rpt3(const orc_rp *p, int ixyz, int icoor, const double *qL, const double *qR,
+              const double *asdq, double *bmasdq, double *bpasdq) {
+  if (p->kind == ORC_RP_EULER3) rpt_euler3(p, ixyz, icoor, qL, qR, asdq, bmasdq, bpasdq);
 }
 
 /* ======================================================================
@@ -902,7 +1048,7 @@ static double flux2(slice_ws *w, int ixy, int mx, const double *q1d, const int *
         V1(w->cqxx, m, i) = 0.0;
         for (int mw = 0; mw < mwaves; mw++) {
           double sa = fabs(SV(mw, i));
-          double f = use_fwave ? (SV(mw, i) >= 0.0 ? 1.0 : -1.0) : sa;
+          double f = use_fwave ? copysign(1.0, SV(mw, i)) : sa;
           V1(w->cqxx, m, i) = V1(w->cqxx, m, i) + f * (1.0 - sa * dtdxave) * WV(m, mw, i);
         }
         V1(w->fadd, m, i) = V1(w->fadd, m, i) + 0.5 * V1(w->cqxx, m, i);
@@ -973,7 +1119,7 @@ static double flux3_split(slice_ws *w, int ixyz, int mx, const double *q1d, doub
     for (int mw = 0; mw < mwaves; mw++)
       for (int m = 0; m < meqn; m++) {
         double sa = fabs(SV(mw, i));
-        double f = use_fwave ? (SV(mw, i) >= 0.0 ? 1.0 : -1.0) : sa;
+        double f = use_fwave ? copysign(1.0, SV(mw, i)) : sa;
         V1(w->cqxx, m, i) = V1(w->cqxx, m, i) + 0.5 * f * (1.0 - sa * dtdxave) * WV(m, mw, i);
       }
     for (int m = 0; m < meqn; m++) V1(w->fadd, m, i) = V1(w->fadd, m, i) + V1(w->cqxx, m, i);
@@ -1126,6 +1272,310 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my, const double *q
 #define Q3(a, m, i, j, k)                                                              \
   (a)[(m) + (long)meqn * (((i) + ng - 1) +                                          \
                           (long)(mx + 2 * ng) * (((j) + ng - 1) + (long)(my + 2 * ng) * ((k) + ng - 1)))]
+
+/* The transverse work of the unsplit flux3: the splits of flux3.f90:265-371
+ * and the two transverse flux modifications gadd(m, 2, -1:1, i) and hadd. */
+typedef struct {
+  double *bmamdq, *bmapdq, *bpamdq, *bpapdq, *cmamdq, *cmapdq, *cpamdq, *cpapdq;
+  double *cmamdq2, *cmapdq2, *cpamdq2, *cpapdq2;
+  double *bmcqxxp, *bpcqxxp, *bmcqxxm, *bpcqxxm, *cmcqxxp, *cpcqxxp, *cmcqxxm, *cpcqxxm;
+  double *bmcmamdq, *bmcmapdq, *bpcmamdq, *bpcmapdq, *bmcpamdq, *bmcpapdq, *bpcpamdq, *bpcpapdq;
+  double *g3, *h3;
+} t3_ws;
+
+#define T3_N 28
+static void t3_alloc(t3_ws *t, int meqn, long len) {
+  double **f = (double **)t;
+  for (int k = 0; k < T3_N; k++) f[k] = (double *)calloc(len * meqn, sizeof(double));
+  t->g3 = (double *)calloc(len * meqn * 6, sizeof(double));
+  t->h3 = (double *)calloc(len * meqn * 6, sizeof(double));
+}
+
+static void t3_free(t3_ws *t) {
+  double **f = (double **)t;
+  for (int k = 0; k < T3_N + 2; k++) free(f[k]);
+}
+
+/* gadd(m, k, j, i), k = 1, 2 (down, up), j = -1..1 */
+#define G3(a, m, k, j, i) (a)[(m) + (long)meqn * (((k) - 1) + 2 * ((j) + 1)) + 6L * meqn * IX(i)]
+
+/* One transverse split of a whole slice array (rpt3 / rptt3 over i = 2-ng..mx+ng). */
+static void slice_rpt3(slice_ws *w, int ixyz, int icoor, int mx, const double *q1d,
+                       const double *asdq, double *bm, double *bp, const orc_rp *p) {
+  const int meqn = w->meqn, ng = w->ng;
+  for (int i = 2 - ng; i <= mx + ng; i++)
+    orc_rpt3(p, ixyz, icoor, &V1(q1d, 0, i - 1), &V1(q1d, 0, i), &V1(asdq, 0, i),
+             &V1(bm, 0, i), &V1(bp, 0, i));
+}
+
+/* classic/flux3.f90:2-594 (no capacity, no aux): qadd, fadd, gadd and hadd of
+ * one slice; dtdy and dtdz are flux3's own arguments (the dt/d of the slice's
+ * first and second transverse directions). */
+static double flux3_full(slice_ws *w, t3_ws *t, int ixyz, int mx, const double *q1d,
+                         double dtdx, double dtdy, double dtdz, const int *method,
+                         const int *mthlim, int use_fwave, const orc_rp *p) {
+  const int meqn = w->meqn, mwaves = w->mwaves, ng = w->ng;
+  int limit = 0;
+  for (int mw = 0; mw < mwaves; mw++)
+    if (mthlim[mw] > 0) limit = 1;
+  for (int i = 1 - ng; i <= mx + ng; i++)
+    for (int m = 0; m < meqn; m++) {
+      V1(w->qadd, m, i) = 0.0;
+      V1(w->fadd, m, i) = 0.0;
+      for (int k = 1; k <= 2; k++)
+        for (int j = -1; j <= 1; j++) {
+          G3(t->g3, m, k, j, i) = 0.0;
+          G3(t->h3, m, k, j, i) = 0.0;
+        }
+    }
+  int m3, m4;
+  if (method[2] < 0) { m3 = -1; m4 = 0; }
+  else { m3 = method[2] / 10; m4 = method[2] - 10 * m3; }
+  slice_rpn(w, ixyz, mx, q1d, p);
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++) V1(w->qadd, m, i) = V1(w->qadd, m, i) - dtdx * V1(w->apdq, m, i);
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++)
+      V1(w->qadd, m, i - 1) = V1(w->qadd, m, i - 1) - dtdx * V1(w->amdq, m, i);
+  double cfl1d = 0.0;
+  for (int i = 1; i <= mx + 1; i++)
+    for (int mw = 0; mw < mwaves; mw++)
+      cfl1d = cmax(cmax(cfl1d, dtdx * SV(mw, i)), -dtdx * SV(mw, i));
+  if (method[1] != 1) {
+    if (limit) limiter(w, mx, mthlim);
+    for (int i = 2 - ng; i <= mx + ng; i++) {
+      double dtdxave = 0.5 * (dtdx + dtdx);
+      for (int m = 0; m < meqn; m++) V1(w->cqxx, m, i) = 0.0;
+      for (int mw = 0; mw < mwaves; mw++)
+        for (int m = 0; m < meqn; m++) {
+          double sa = fabs(SV(mw, i));
+          double f = use_fwave ? copysign(1.0, SV(mw, i)) : sa;
+          V1(w->cqxx, m, i) = V1(w->cqxx, m, i) + 0.5 * f * (1.0 - sa * dtdxave) * WV(m, mw, i);
+        }
+      for (int m = 0; m < meqn; m++) V1(w->fadd, m, i) = V1(w->fadd, m, i) + V1(w->cqxx, m, i);
+    }
+  }
+  if (m3 <= 0) return cfl1d;
+  /* flux3.f90:265-289 */
+  slice_rpt3(w, ixyz, 2, mx, q1d, w->amdq, t->bmamdq, t->bpamdq, p);
+  slice_rpt3(w, ixyz, 2, mx, q1d, w->apdq, t->bmapdq, t->bpapdq, p);
+  slice_rpt3(w, ixyz, 3, mx, q1d, w->amdq, t->cmamdq, t->cpamdq, p);
+  slice_rpt3(w, ixyz, 3, mx, q1d, w->apdq, t->cmapdq, t->cpapdq, p);
+  if (m3 == 2) {   /* flux3.f90:322-338: no aux, one split used on both sides */
+    slice_rpt3(w, ixyz, 2, mx, q1d, w->cqxx, t->bmcqxxm, t->bpcqxxm, p);
+    slice_rpt3(w, ixyz, 3, mx, q1d, w->cqxx, t->cmcqxxm, t->cpcqxxm, p);
+    for (int i = 0; i <= mx + 2; i++)
+      for (int m = 0; m < meqn; m++) {
+        V1(t->bmcqxxp, m, i) = V1(t->bmcqxxm, m, i);
+        V1(t->bpcqxxp, m, i) = V1(t->bpcqxxm, m, i);
+        V1(t->cmcqxxp, m, i) = V1(t->cmcqxxm, m, i);
+        V1(t->cpcqxxp, m, i) = V1(t->cpcqxxm, m, i);
+      }
+  }
+  /* flux3.f90:347-360 */
+  if (m4 == 1) {
+    for (int i = 0; i <= mx + 2; i++)
+      for (int m = 0; m < meqn; m++) {
+        V1(t->cpapdq2, m, i) = V1(t->cpapdq, m, i);
+        V1(t->cpamdq2, m, i) = V1(t->cpamdq, m, i);
+        V1(t->cmapdq2, m, i) = V1(t->cmapdq, m, i);
+        V1(t->cmamdq2, m, i) = V1(t->cmamdq, m, i);
+      }
+  } else if (m4 == 2) {
+    for (int i = 0; i <= mx + 2; i++)
+      for (int m = 0; m < meqn; m++) {
+        V1(t->cpapdq2, m, i) = V1(t->cpapdq, m, i) - 3.0 * V1(t->cpcqxxp, m, i);
+        V1(t->cpamdq2, m, i) = V1(t->cpamdq, m, i) + 3.0 * V1(t->cpcqxxm, m, i);
+        V1(t->cmapdq2, m, i) = V1(t->cmapdq, m, i) - 3.0 * V1(t->cmcqxxp, m, i);
+        V1(t->cmamdq2, m, i) = V1(t->cmamdq, m, i) + 3.0 * V1(t->cmcqxxm, m, i);
+      }
+  }
+  if (m4 > 0) {
+    slice_rpt3(w, ixyz, 2, mx, q1d, t->cpapdq2, t->bmcpapdq, t->bpcpapdq, p);
+    slice_rpt3(w, ixyz, 2, mx, q1d, t->cpamdq2, t->bmcpamdq, t->bpcpamdq, p);
+    slice_rpt3(w, ixyz, 2, mx, q1d, t->cmapdq2, t->bmcmapdq, t->bpcmapdq, p);
+    slice_rpt3(w, ixyz, 2, mx, q1d, t->cmamdq2, t->bmcmamdq, t->bpcmamdq, p);
+  }
+  const double sixth = 1.0 / 6.0;
+  /* flux3.f90:374-454: G */
+  double *g = t->g3;
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++) {
+      const double di = w->dtdx1d[IX(i)], dm = w->dtdx1d[IX(i - 1)];
+      G3(g, m, 1, 0, i - 1) = G3(g, m, 1, 0, i - 1) - 0.5 * dm * V1(t->bmamdq, m, i);
+      G3(g, m, 2, 0, i - 1) = G3(g, m, 2, 0, i - 1) - 0.5 * dm * V1(t->bpamdq, m, i);
+      G3(g, m, 1, 0, i) = G3(g, m, 1, 0, i) - 0.5 * di * V1(t->bmapdq, m, i);
+      G3(g, m, 2, 0, i) = G3(g, m, 2, 0, i) - 0.5 * di * V1(t->bpapdq, m, i);
+      if (m4 > 0) {
+        G3(g, m, 2, 0, i) = G3(g, m, 2, 0, i) + sixth * di * dtdz *
+                            (V1(t->bpcpapdq, m, i) - V1(t->bpcmapdq, m, i));
+        G3(g, m, 1, 0, i) = G3(g, m, 1, 0, i) + sixth * di * dtdz *
+                            (V1(t->bmcpapdq, m, i) - V1(t->bmcmapdq, m, i));
+        G3(g, m, 2, 1, i) = G3(g, m, 2, 1, i) - sixth * di * dtdz * V1(t->bpcpapdq, m, i);
+        G3(g, m, 1, 1, i) = G3(g, m, 1, 1, i) - sixth * di * dtdz * V1(t->bmcpapdq, m, i);
+        G3(g, m, 2, -1, i) = G3(g, m, 2, -1, i) + sixth * di * dtdz * V1(t->bpcmapdq, m, i);
+        G3(g, m, 1, -1, i) = G3(g, m, 1, -1, i) + sixth * di * dtdz * V1(t->bmcmapdq, m, i);
+        G3(g, m, 2, 0, i - 1) = G3(g, m, 2, 0, i - 1) + sixth * dm * dtdz *
+                                (V1(t->bpcpamdq, m, i) - V1(t->bpcmamdq, m, i));
+        G3(g, m, 1, 0, i - 1) = G3(g, m, 1, 0, i - 1) + sixth * dm * dtdz *
+                                (V1(t->bmcpamdq, m, i) - V1(t->bmcmamdq, m, i));
+        G3(g, m, 2, 1, i - 1) = G3(g, m, 2, 1, i - 1) - sixth * dm * dtdz * V1(t->bpcpamdq, m, i);
+        G3(g, m, 1, 1, i - 1) = G3(g, m, 1, 1, i - 1) - sixth * dm * dtdz * V1(t->bmcpamdq, m, i);
+        G3(g, m, 2, -1, i - 1) = G3(g, m, 2, -1, i - 1) + sixth * dm * dtdz * V1(t->bpcmamdq, m, i);
+        G3(g, m, 1, -1, i - 1) = G3(g, m, 1, -1, i - 1) + sixth * dm * dtdz * V1(t->bmcmamdq, m, i);
+      }
+      if (m3 >= 2) {
+        G3(g, m, 2, 0, i) = G3(g, m, 2, 0, i) + di * V1(t->bpcqxxp, m, i);
+        G3(g, m, 1, 0, i) = G3(g, m, 1, 0, i) + di * V1(t->bmcqxxp, m, i);
+        G3(g, m, 2, 0, i - 1) = G3(g, m, 2, 0, i - 1) - dm * V1(t->bpcqxxm, m, i);
+        G3(g, m, 1, 0, i - 1) = G3(g, m, 1, 0, i - 1) - dm * V1(t->bmcqxxm, m, i);
+      }
+    }
+  /* flux3.f90:462-486: the correction folded into the y-like splits, which are
+   * then split in the z-like direction */
+  if (m4 == 2) {
+    for (int i = 0; i <= mx + 2; i++)
+      for (int m = 0; m < meqn; m++) {
+        V1(t->bpapdq, m, i) = V1(t->bpapdq, m, i) - 3.0 * V1(t->bpcqxxp, m, i);
+        V1(t->bpamdq, m, i) = V1(t->bpamdq, m, i) + 3.0 * V1(t->bpcqxxm, m, i);
+        V1(t->bmapdq, m, i) = V1(t->bmapdq, m, i) - 3.0 * V1(t->bmcqxxp, m, i);
+        V1(t->bmamdq, m, i) = V1(t->bmamdq, m, i) + 3.0 * V1(t->bmcqxxm, m, i);
+      }
+  }
+  if (m4 > 0) {
+    slice_rpt3(w, ixyz, 3, mx, q1d, t->bpapdq, t->bmcpapdq, t->bpcpapdq, p);
+    slice_rpt3(w, ixyz, 3, mx, q1d, t->bpamdq, t->bmcpamdq, t->bpcpamdq, p);
+    slice_rpt3(w, ixyz, 3, mx, q1d, t->bmapdq, t->bmcmapdq, t->bpcmapdq, p);
+    slice_rpt3(w, ixyz, 3, mx, q1d, t->bmamdq, t->bmcmamdq, t->bpcmamdq, p);
+  }
+  /* flux3.f90:492-580: H */
+  double *h = t->h3;
+  for (int i = 1; i <= mx + 1; i++)
+    for (int m = 0; m < meqn; m++) {
+      const double di = w->dtdx1d[IX(i)], dm = w->dtdx1d[IX(i - 1)];
+      G3(h, m, 1, 0, i - 1) = G3(h, m, 1, 0, i - 1) - 0.5 * dm * V1(t->cmamdq, m, i);
+      G3(h, m, 2, 0, i - 1) = G3(h, m, 2, 0, i - 1) - 0.5 * dm * V1(t->cpamdq, m, i);
+      G3(h, m, 1, 0, i) = G3(h, m, 1, 0, i) - 0.5 * di * V1(t->cmapdq, m, i);
+      G3(h, m, 2, 0, i) = G3(h, m, 2, 0, i) - 0.5 * di * V1(t->cpapdq, m, i);
+      if (m4 > 0) {
+        G3(h, m, 2, 0, i) = G3(h, m, 2, 0, i) + sixth * di * dtdy *
+                            (V1(t->bpcpapdq, m, i) - V1(t->bpcmapdq, m, i));
+        G3(h, m, 1, 0, i) = G3(h, m, 1, 0, i) + sixth * di * dtdy *
+                            (V1(t->bmcpapdq, m, i) - V1(t->bmcmapdq, m, i));
+        G3(h, m, 2, 1, i) = G3(h, m, 2, 1, i) - sixth * di * dtdy * V1(t->bpcpapdq, m, i);
+        G3(h, m, 1, 1, i) = G3(h, m, 1, 1, i) - sixth * di * dtdy * V1(t->bmcpapdq, m, i);
+        G3(h, m, 2, -1, i) = G3(h, m, 2, -1, i) + sixth * di * dtdy * V1(t->bpcmapdq, m, i);
+        G3(h, m, 1, -1, i) = G3(h, m, 1, -1, i) + sixth * di * dtdy * V1(t->bmcmapdq, m, i);
+        G3(h, m, 2, 0, i - 1) = G3(h, m, 2, 0, i - 1) + sixth * dm * dtdy *
+                                (V1(t->bpcpamdq, m, i) - V1(t->bpcmamdq, m, i));
+        G3(h, m, 1, 0, i - 1) = G3(h, m, 1, 0, i - 1) + sixth * dm * dtdy *
+                                (V1(t->bmcpamdq, m, i) - V1(t->bmcmamdq, m, i));
+        G3(h, m, 2, 1, i - 1) = G3(h, m, 2, 1, i - 1) - sixth * dm * dtdy * V1(t->bpcpamdq, m, i);
+        G3(h, m, 1, 1, i - 1) = G3(h, m, 1, 1, i - 1) - sixth * dm * dtdy * V1(t->bmcpamdq, m, i);
+        G3(h, m, 2, -1, i - 1) = G3(h, m, 2, -1, i - 1) + sixth * dm * dtdy * V1(t->bpcmamdq, m, i);
+        G3(h, m, 1, -1, i - 1) = G3(h, m, 1, -1, i - 1) + sixth * dm * dtdy * V1(t->bmcmamdq, m, i);
+      }
+      if (m3 >= 2) {
+        G3(h, m, 2, 0, i) = G3(h, m, 2, 0, i) + di * V1(t->cpcqxxp, m, i);
+        G3(h, m, 1, 0, i) = G3(h, m, 1, 0, i) + di * V1(t->cmcqxxp, m, i);
+        G3(h, m, 2, 0, i - 1) = G3(h, m, 2, 0, i - 1) - dm * V1(t->cpcqxxm, m, i);
+        G3(h, m, 1, 0, i - 1) = G3(h, m, 1, 0, i - 1) - dm * V1(t->cmcqxxm, m, i);
+      }
+    }
+  return cfl1d;
+}
+
+/* One slice's nine updates of step3.f90:189-225 (x), 330-366 (y), 494-530 (z):
+ * the cell at transverse offsets (da, db) from the slice (da along the first
+ * transverse direction, whose fluxes gadd modifies, db along the second,
+ * hadd's), dt1 / dt2 their dt/d.  The (0, 0) cell also takes qadd and fadd. */
+static double step3_term(double qv, int meqn, const slice_ws *w, const t3_ws *t, int m, int i,
+                         int da, int db, double dtn, double dt1, double dt2) {
+  const int ng = w->ng;
+  const double *g = t->g3, *h = t->h3;
+#define GG(k, j) G3(g, m, k, j, i)
+#define HH(k, j) G3(h, m, k, j, i)
+  if (da == 0 && db == 0)
+    return qv + V1(w->qadd, m, i) - dtn * (V1(w->fadd, m, i + 1) - V1(w->fadd, m, i))
+           - dt1 * (GG(2, 0) - GG(1, 0)) - dt2 * (HH(2, 0) - HH(1, 0));
+  if (da == -1 && db == 0) return qv - dt1 * GG(1, 0) - dt2 * (HH(2, -1) - HH(1, -1));
+  if (da == -1 && db == -1) return qv - dt1 * GG(1, -1) - dt2 * HH(1, -1);
+  if (da == 0 && db == -1) return qv - dt1 * (GG(2, -1) - GG(1, -1)) - dt2 * HH(1, 0);
+  if (da == 1 && db == -1) return qv + dt1 * GG(2, -1) - dt2 * HH(1, 1);
+  if (da == 1 && db == 0) return qv + dt1 * GG(2, 0) - dt2 * (HH(2, 1) - HH(1, 1));
+  if (da == 1 && db == 1) return qv + dt1 * GG(2, 1) + dt2 * HH(2, 1);
+  if (da == 0 && db == 1) return qv - dt1 * (GG(2, 1) - GG(1, 1)) + dt2 * HH(2, 0);
+  return qv - dt1 * GG(1, 1) + dt2 * HH(2, -1);   /* (-1, 1) */
+#undef GG
+#undef HH
+}
+
+/* classic/step3.f90:2-597 (index_capa = 0, no aux): the unsplit 3D step.
+ * Slices in the reference's loop order (x: k then j; y: k then i; z: j then
+ * i), each flux3_full, then its nine updates of qnew (qnew starts as a copy of
+ * qold, ghosts included; the updates reach one ghost layer). */
+double orc_step3(int meqn, int mwaves, int ng, int mx, int my, int mz, const double *qold,
+                 double *qnew, double dx, double dy, double dz, double dt,
+                 const int *method, const int *mthlim, int use_fwave, const orc_rp *p) {
+  int maxm = mx > my ? mx : my;
+  if (mz > maxm) maxm = mz;
+  slice_ws W, *w = &W;
+  ws_alloc(w, meqn, mwaves, ng, maxm);
+  t3_ws T, *t = &T;
+  t3_alloc(t, meqn, w->len);
+  double *q1d = (double *)calloc((long)(maxm + 2 * ng) * meqn, sizeof(double));
+  double cfl = 0.0;
+  const double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
+  for (int i = 0; i < w->len; i++) w->dtdx1d[i] = dtdx;
+  for (int k = 0; k <= mz + 1; k++)
+    for (int j = 0; j <= my + 1; j++) {
+      for (int i = 1 - ng; i <= mx + ng; i++)
+        for (int m = 0; m < meqn; m++) V1(q1d, m, i) = Q3(qold, m, i, j, k);
+      cfl = cmax(cfl, flux3_full(w, t, 1, mx, q1d, dtdx, dtdy, dtdz, method, mthlim,
+                                 use_fwave, p));
+      for (int i = 1; i <= mx; i++)
+        for (int m = 0; m < meqn; m++)
+          for (int db = -1; db <= 1; db++)
+            for (int da = -1; da <= 1; da++)
+              Q3(qnew, m, i, j + da, k + db) =
+                  step3_term(Q3(qnew, m, i, j + da, k + db), meqn, w, t, m, i, da, db, dtdx,
+                             dtdy, dtdz);
+    }
+  for (int i = 0; i < w->len; i++) w->dtdx1d[i] = dtdy;
+  for (int k = 0; k <= mz + 1; k++)
+    for (int i = 0; i <= mx + 1; i++) {
+      for (int j = 1 - ng; j <= my + ng; j++)
+        for (int m = 0; m < meqn; m++) V1(q1d, m, j) = Q3(qold, m, i, j, k);
+      cfl = cmax(cfl, flux3_full(w, t, 2, my, q1d, dtdy, dtdz, dtdx, method, mthlim,
+                                 use_fwave, p));
+      for (int j = 1; j <= my; j++)
+        for (int m = 0; m < meqn; m++)
+          for (int db = -1; db <= 1; db++)
+            for (int da = -1; da <= 1; da++)
+              Q3(qnew, m, i + db, j, k + da) =
+                  step3_term(Q3(qnew, m, i + db, j, k + da), meqn, w, t, m, j, da, db, dtdy,
+                             dtdz, dtdx);
+    }
+  for (int i = 0; i < w->len; i++) w->dtdx1d[i] = dtdz;
+  for (int j = 0; j <= my + 1; j++)
+    for (int i = 0; i <= mx + 1; i++) {
+      for (int k = 1 - ng; k <= mz + ng; k++)
+        for (int m = 0; m < meqn; m++) V1(q1d, m, k) = Q3(qold, m, i, j, k);
+      cfl = cmax(cfl, flux3_full(w, t, 3, mz, q1d, dtdz, dtdx, dtdy, method, mthlim,
+                                 use_fwave, p));
+      for (int k = 1; k <= mz; k++)
+        for (int m = 0; m < meqn; m++)
+          for (int db = -1; db <= 1; db++)
+            for (int da = -1; da <= 1; da++)
+              Q3(qnew, m, i + da, j + db, k) =
+                  step3_term(Q3(qnew, m, i + da, j + db, k), meqn, w, t, m, k, da, db, dtdz,
+                             dtdx, dtdy);
+    }
+  free(q1d);
+  t3_free(t);
+  ws_free(w);
+  return cfl;
+}
 
 /* classic/step3ds.f90:112-378 (index_capa = 0). */
 double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz, const double *qold,

@@ -31,8 +31,10 @@ enum {
   ORC_RP_VC_ACOUSTICS = 8,     /* clawpack.riemann vc_acoustics_2D: meqn 3, mwaves 2,
                                * aux = (density, sound speed) per cell */
   ORC_RP_SHALLOW_HLLE = 9,     /* clawpack.riemann shallow_hlle_2D: meqn 3, mwaves 2 */
-  ORC_RP_EULER5 = 10           /* euler_5wave_2D (development/rp_approaches/
+  ORC_RP_EULER5 = 10,          /* euler_5wave_2D (development/rp_approaches/
                                * rpn2_euler_5wave.f, rpt2_euler_5wave.f): meqn 5, mwaves 5 */
+  ORC_RP_SHALLOW_FWAVE = 11    /* clawpack.riemann shallow_bathymetry_fwave_2D: meqn 3,
+                               * mwaves 3, f-waves, aux = bathymetry b per cell */
 };
 
 /* Problem parameters: the reference passes these through the Fortran /cparam/
@@ -45,6 +47,8 @@ typedef struct {
   double grav;                  /* shallow water */
   double u, v;                  /* advection velocities */
   int meqn;                     /* ORC_RP_VC_ADVECTION: number of fields */
+  double dry_tolerance;         /* shallow water with bathymetry */
+  double sea_level;
 } orc_rp;
 
 /* Boundary condition codes: pyclaw.BC (solver.py:13-19). */
@@ -101,6 +105,13 @@ double orc_step2ds(int meqn, int mwaves, int ng, int mx, int my,
                    const double *qold, double *qnew, double dx, double dy, double dt,
                    const int *method, const int *mthlim, int ids, int use_fwave,
                    const orc_rp *p, int maux, const double *aux);
+/* classic/step3.f90: the unsplit 3D step (transverse method(3) = 10, 11, 20,
+ * 21, 22), qnew starting as a copy of qold; rpt3 / rptt3 of euler_3D. */
+double orc_step3(int meqn, int mwaves, int ng, int mx, int my, int mz, const double *qold,
+                 double *qnew, double dx, double dy, double dz, double dt,
+                 const int *method, const int *mthlim, int use_fwave, const orc_rp *p);
+void orc_rpt3(const orc_rp *p, int ixyz, int icoor, const double *qL, const double *qR,
+              const double *asdq, double *bmasdq, double *bpasdq);
 double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz,
                    const double *qold, double *qnew, double dx, double dy, double dz,
                    double dt, const int *method, const int *mthlim, int idir,

@@ -21,13 +21,15 @@ REF_SHARP_PATH = os.path.join(HERE, "_ref", "libref_sharp.so")
 ACOUSTICS, EULER, SHALLOW, ADVECTION, EULER3 = 1, 2, 3, 4, 5
 CUSTOM, EXTRAP, PERIODIC, WALL = 0, 1, 2, 3
 
-MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5, 8: 3, 9: 3, 10: 5}
-MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1, 7: 1, 8: 2, 9: 2, 10: 5}
+MEQN = {ACOUSTICS: 3, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 5, 8: 3, 9: 3, 10: 5, 11: 3}
+MWAVES = {ACOUSTICS: 2, EULER: 4, SHALLOW: 3, ADVECTION: 1, EULER3: 3, 6: 1, 7: 1, 8: 2, 9: 2, 10: 5,
+          11: 3}
 VC_ADVECTION = 6
 VC_ADVECTION_2D = 7
 VC_ACOUSTICS = 8
 SHALLOW_HLLE = 9
 EULER5 = 10
+SHALLOW_FWAVE = 11
 
 
 class RP(ctypes.Structure):
@@ -36,12 +38,15 @@ class RP(ctypes.Structure):
                 ("rho", ctypes.c_double), ("bulk", ctypes.c_double),
                 ("cc", ctypes.c_double), ("zz", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("grav", ctypes.c_double),
-                ("u", ctypes.c_double), ("v", ctypes.c_double), ("meqn", ctypes.c_int)]
+                ("u", ctypes.c_double), ("v", ctypes.c_double), ("meqn", ctypes.c_int),
+                ("dry_tolerance", ctypes.c_double), ("sea_level", ctypes.c_double)]
 
 
-def make_rp(kind, efix=1, rho=1.0, bulk=4.0, gamma=1.4, grav=1.0, u=0.5, v=1.0, meqn=0):
+def make_rp(kind, efix=1, rho=1.0, bulk=4.0, gamma=1.4, grav=1.0, u=0.5, v=1.0, meqn=0,
+            dry_tolerance=1e-3, sea_level=0.0):
     cc = float(np.sqrt(bulk / rho))
-    return RP(kind, efix, rho, bulk, cc, rho * cc, gamma, grav, u, v, meqn)
+    return RP(kind, efix, rho, bulk, cc, rho * cc, gamma, grav, u, v, meqn, dry_tolerance,
+              sea_level)
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -160,6 +165,25 @@ def step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0, use_fwave=False
                              _dp(aux) if aux is not None else None)
 
 
+def step3(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, use_fwave=False):
+    """The unsplit 3D step (classic/step3.f90); qnew updated in place."""
+    meqn, nx, ny, nz = qold.shape
+    f = lib().orc_step3
+    f.restype = ctypes.c_double
+    return f(meqn, MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, nz - 2 * ng, _dp(qold),
+             _dp(qnew), ctypes.c_double(dx), ctypes.c_double(dy), ctypes.c_double(dz),
+             ctypes.c_double(dt), _ip(method), _ip(mthlim), int(use_fwave), ctypes.byref(rp))
+
+
+def rpt3(rp, ixyz, icoor, qL, qR, asdq):
+    """The oracle's euler_3D transverse split of asdq at the interface qL | qR."""
+    bm, bp = np.zeros(5), np.zeros(5)
+    lib().orc_rpt3(ctypes.byref(rp), int(ixyz), int(icoor), _dp(np.ascontiguousarray(qL, float)),
+                   _dp(np.ascontiguousarray(qR, float)), _dp(np.ascontiguousarray(asdq, float)),
+                   _dp(bm), _dp(bp))
+    return bm, bp
+
+
 def step3ds(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim, idir, use_fwave=False):
     meqn, nx, ny, nz = qold.shape
     return lib().orc_step3ds(meqn, MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, nz - 2 * ng,
@@ -223,12 +247,31 @@ def rpn(rp, ixy, qL, qR):
 
 
 # ---------------------------------------------------------------- reference build
-def ref_step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0):
+def ref_fwave(on):
+    """use_fwave for the reference step routines called through the shim."""
+    ref_classic().shim_set_fwave(int(bool(on)))
+
+
+def ref_step3(rp, qold, qnew, ng, dx, dy, dz, dt, method, mthlim):
+    """The reference's own unsplit step3 with the oracle's euler_3D solvers."""
+    meqn, nx, ny, nz = qold.shape
+    f = ref_classic().ref_step3
+    f.restype = ctypes.c_double
+    return f(ctypes.byref(rp), meqn, MWAVES[rp.kind], ng, nx - 2 * ng, ny - 2 * ng, nz - 2 * ng,
+             _dp(qold), _dp(qnew), ctypes.c_double(dx), ctypes.c_double(dy), ctypes.c_double(dz),
+             ctypes.c_double(dt), _ip(method), _ip(mthlim))
+
+
+def ref_step2(rp, qold, qnew, ng, dx, dy, dt, method, mthlim, ids=0, use_fwave=False):
     meqn, nx, ny = qold.shape
-    return ref_classic().ref_step2(ctypes.byref(rp), int(ids), meqn, MWAVES[rp.kind], ng,
-                                   nx - 2 * ng, ny - 2 * ng, _dp(qold), _dp(qnew),
-                                   ctypes.c_double(dx), ctypes.c_double(dy),
-                                   ctypes.c_double(dt), _ip(method), _ip(mthlim))
+    ref_fwave(use_fwave)
+    try:
+        return ref_classic().ref_step2(ctypes.byref(rp), int(ids), meqn, MWAVES[rp.kind], ng,
+                                       nx - 2 * ng, ny - 2 * ng, _dp(qold), _dp(qnew),
+                                       ctypes.c_double(dx), ctypes.c_double(dy),
+                                       ctypes.c_double(dt), _ip(method), _ip(mthlim))
+    finally:
+        ref_fwave(False)
 
 
 _ref_vcadv = None
@@ -244,7 +287,7 @@ def ref_vcadv_rpn2():
 
 
 def ref_step2ds_aux(rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, ids, mwaves=1,
-                    rp=None):
+                    rp=None, use_fwave=False):
     """The reference's step2ds sweep with aux fields and the given rpn2 address
     (None: the oracle's solver rp)."""
     meqn, nx, ny = qold.shape
@@ -252,9 +295,13 @@ def ref_step2ds_aux(rpn2, qold, qnew, aux, ng, dx, dy, dt, method, mthlim, ids, 
         ref_classic().shim_set_rp(ctypes.byref(rp))
     f = ref_classic().ref_step2ds_aux
     f.restype = ctypes.c_double
-    return f(rpn2, int(ids), meqn, int(mwaves), aux.shape[0], ng, nx - 2 * ng, ny - 2 * ng,
-             _dp(qold), _dp(qnew), _dp(aux), ctypes.c_double(dx), ctypes.c_double(dy),
-             ctypes.c_double(dt), _ip(method), _ip(mthlim))
+    ref_fwave(use_fwave)
+    try:
+        return f(rpn2, int(ids), meqn, int(mwaves), aux.shape[0], ng, nx - 2 * ng,
+                 ny - 2 * ng, _dp(qold), _dp(qnew), _dp(aux), ctypes.c_double(dx),
+                 ctypes.c_double(dy), ctypes.c_double(dt), _ip(method), _ip(mthlim))
+    finally:
+        ref_fwave(False)
 
 
 _ref_euler5 = None

@@ -13,7 +13,7 @@ CFLAGS = -O2 -fPIC -ffp-contract=off -std=c11
 
 CLASSIC = $(REF)/classic/step2.f90 $(REF)/classic/flux2.f90 $(REF)/classic/limiter.f90 \
           $(REF)/classic/philim.f90 $(REF)/classic/step2ds.f90 $(REF)/classic/step3ds.f90 \
-          $(REF)/classic/flux3.f90
+          $(REF)/classic/flux3.f90 $(REF)/classic/step3.f90
 SHARP = $(REF)/sharpclaw/ClawParams.f90 $(REF)/sharpclaw/workspace.f90 \
         $(REF)/sharpclaw/weno.f90 $(REF)/sharpclaw/reconstruct.f90 \
         $(REF)/sharpclaw/flux1.f90 $(REF)/sharpclaw/flux2.f90

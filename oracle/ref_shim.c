@@ -21,7 +21,10 @@ static int g_evec_dir = 1;
 static int g_evec_calls = 0;
 static int g_evec_xcalls = 0; /* number of x-slices flux2 makes: my + 2 */
 
+static int g_fwave = 0;        /* use_fwave passed to the step routines */
+
 void shim_set_rp(const orc_rp *p) { g_rp = *p; }
+void shim_set_fwave(int on) { g_fwave = on != 0; }
 void shim_set_evec_dir(int dir) { g_evec_mode = 0; g_evec_dir = dir; }
 void shim_set_evec_count(int xcalls) {
   g_evec_mode = 1; g_evec_calls = 0; g_evec_xcalls = xcalls;
@@ -96,6 +99,34 @@ void shim_rpn3_(const int *ixyz, const int *maxm, const int *meqn, const int *mw
   shim_rpn2_(ixyz, maxm, meqn, mwaves, maux, mbc, mx, ql, qr, auxl, auxr, wave, s, amdq, apdq);
 }
 
+/* rpt3(ixyz,icoor,imp,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,aux1,aux2,aux3,asdq,
+ *      bmasdq,bpasdq) (flux3.f90:272-289): problem i splits asdq(:,i) with the
+ * states qr(:,i-1) | ql(:,i). */
+void shim_rpt3_(const int *ixyz, const int *icoor, const int *imp, const int *maxm,
+                const int *meqn, const int *mwaves, const int *maux, const int *mbc,
+                const int *mx, const double *ql, const double *qr, const double *aux1,
+                const double *aux2, const double *aux3, const double *asdq, double *bmasdq,
+                double *bpasdq) {
+  (void)imp; (void)maxm; (void)mwaves; (void)maux; (void)aux1; (void)aux2; (void)aux3;
+  const int me = *meqn, ng = *mbc;
+  for (int i = 2 - ng; i <= *mx + ng; i++) {
+    long ci = i + ng - 1, cm = ci - 1;
+    orc_rpt3(&g_rp, *ixyz, *icoor, qr + (long)me * cm, ql + (long)me * ci,
+             asdq + (long)me * ci, bmasdq + (long)me * ci, bpasdq + (long)me * ci);
+  }
+}
+
+/* rptt3(ixyz,icoor,imp,impt,maxm,...): the same split of its input. */
+void shim_rptt3_(const int *ixyz, const int *icoor, const int *imp, const int *impt,
+                 const int *maxm, const int *meqn, const int *mwaves, const int *maux,
+                 const int *mbc, const int *mx, const double *ql, const double *qr,
+                 const double *aux1, const double *aux2, const double *aux3,
+                 const double *bsasdq, double *cmbsasdq, double *cpbsasdq) {
+  (void)impt;
+  shim_rpt3_(ixyz, icoor, imp, maxm, meqn, mwaves, maux, mbc, mx, ql, qr, aux1, aux2, aux3,
+             bsasdq, cmbsasdq, cpbsasdq);
+}
+
 /* never called on the split / tfluct_solver=False paths */
 void shim_unused_(void) { abort(); }
 
@@ -141,7 +172,7 @@ double ref_step2_fns(const orc_rp *p, rp_fn rpn2, rp_fn rpt2, int ids, int meqn,
   shim_set_rp(p);
   if (!rpn2) rpn2 = (rp_fn)shim_rpn2_;
   if (!rpt2) rpt2 = (rp_fn)shim_rpt2_;
-  int maxm = mx > my ? mx : my, maux = 0, use_fwave = 0;
+  int maxm = mx > my ? mx : my, maux = 0, use_fwave = g_fwave;
   long L = maxm + 2 * ng;
   int mwork = (int)(L * (5 * meqn + mwaves + meqn * mwaves));
   double *buf = (double *)calloc(L * (8 * meqn + 4) + mwork + 16, sizeof(double));
@@ -177,7 +208,7 @@ double ref_step2_aux(const orc_rp *p, rp_fn rpn2, int meqn, int mwaves, int maux
                      int mx, int my, const double *qold, double *qnew, const double *aux,
                      double dx, double dy, double dt, const int *method, const int *mthlim) {
   shim_set_rp(p);
-  int maxm = mx > my ? mx : my, use_fwave = 0;
+  int maxm = mx > my ? mx : my, use_fwave = g_fwave;
   long L = maxm + 2 * ng;
   int mwork = (int)(L * (5 * meqn + mwaves + meqn * mwaves));
   double *buf = (double *)calloc(L * (6 * meqn + 2 + 3 * maux) + mwork + 16, sizeof(double));
@@ -201,7 +232,7 @@ double ref_step2_aux(const orc_rp *p, rp_fn rpn2, int meqn, int mwaves, int maux
 double ref_step2ds_aux(rp_fn rpn2, int ids, int meqn, int mwaves, int maux, int ng, int mx,
                        int my, const double *qold, double *qnew, const double *aux, double dx,
                        double dy, double dt, const int *method, const int *mthlim) {
-  int maxm = mx > my ? mx : my, use_fwave = 0;
+  int maxm = mx > my ? mx : my, use_fwave = g_fwave;
   long L = maxm + 2 * ng;
   int mwork = (int)(L * (5 * meqn + mwaves + meqn * mwaves));
   double *buf = (double *)calloc(L * (6 * meqn + 2 + 3 * maux) + mwork + 16, sizeof(double));
@@ -218,11 +249,44 @@ double ref_step2ds_aux(rp_fn rpn2, int ids, int meqn, int mwaves, int maux, int 
   return cfl;
 }
 
+extern void step3_(const int *, const int *, const int *, const int *, const int *,
+                   const int *, const int *, const double *, double *, const double *,
+                   const double *, const double *, const double *, const double *,
+                   const int *, const int *, double *, double *, double *, double *,
+                   double *, double *, double *, double *, double *, double *, double *,
+                   double *, const int *, double *, const int *, const int *,
+                   rp_fn, rp_fn, rp_fn);
+
+/* The reference's unsplit step3 (classic/step3.f90) with the oracle's euler_3D
+ * rpn3 / rpt3 / rptt3; qnew starts as the caller's array. */
+double ref_step3(const orc_rp *p, int meqn, int mwaves, int ng, int mx, int my, int mz,
+                 const double *qold, double *qnew, double dx, double dy, double dz, double dt,
+                 const int *method, const int *mthlim) {
+  shim_set_rp(p);
+  int maxm = mx > my ? mx : my, maux = 0, use_fwave = g_fwave;
+  if (mz > maxm) maxm = mz;
+  long L = maxm + 2 * ng;
+  int mwork = (int)(L * (31 * meqn + mwaves + meqn * mwaves));
+  double *buf = (double *)calloc(L * (16 * meqn + 16) + mwork + 16, sizeof(double));
+  double *qadd = buf, *fadd = qadd + L * meqn, *gadd = fadd + L * meqn,
+         *hadd = gadd + 6 * L * meqn, *q1d = hadd + 6 * L * meqn, *dtdx1d = q1d + L * meqn,
+         *dtdy1d = dtdx1d + L, *dtdz1d = dtdy1d + L, *aux1 = dtdz1d + L, *aux2 = aux1 + 3 * L,
+         *aux3 = aux2 + 3 * L, *work = aux3 + 3 * L;
+  double aux[1] = {0.0};
+  double cfl = 0.0;
+  step3_(&maxm, &meqn, &mwaves, &ng, &mx, &my, &mz, qold, qnew, aux, &dx, &dy, &dz, &dt,
+         method, mthlim, &cfl, qadd, fadd, gadd, hadd, q1d, dtdx1d, dtdy1d, dtdz1d, aux1,
+         aux2, aux3, &maux, work, &mwork, &use_fwave, (rp_fn)shim_rpn3_, (rp_fn)shim_rpt3_,
+         (rp_fn)shim_rptt3_);
+  free(buf);
+  return cfl;
+}
+
 double ref_step3ds(const orc_rp *p, int idir, int meqn, int mwaves, int ng, int mx, int my,
                    int mz, const double *qold, double *qnew, double dx, double dy, double dz,
                    double dt, const int *method, const int *mthlim) {
   shim_set_rp(p);
-  int maxm = mx > my ? mx : my, maux = 0, use_fwave = 0;
+  int maxm = mx > my ? mx : my, maux = 0, use_fwave = g_fwave;
   if (mz > maxm) maxm = mz;
   long L = maxm + 2 * ng;
   int mwork = (int)(L * (31 * meqn + mwaves + meqn * mwaves));

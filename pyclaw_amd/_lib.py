@@ -27,7 +27,9 @@ class RP(ctypes.Structure):
                 ("rho", ctypes.c_double), ("bulk", ctypes.c_double),
                 ("cc", ctypes.c_double), ("zz", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("grav", ctypes.c_double),
-                ("u", ctypes.c_double), ("v", ctypes.c_double), ("meqn", ctypes.c_int)]
+                ("u", ctypes.c_double), ("v", ctypes.c_double), ("meqn", ctypes.c_int),
+                ("fwave", ctypes.c_int), ("dry_tolerance", ctypes.c_double),
+                ("sea_level", ctypes.c_double)]
 
 
 class StepCtl(ctypes.Structure):
@@ -100,6 +102,10 @@ _SIGS = {
     "clawhip_step2_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                             _D, _D, _D, _P, ctypes.POINTER(Geom), _IP, _IP, _P, _P],
+    "clawhip_step3_unsplit_work_elems": [_I, _I, _I, _I],
+    "clawhip_step3_unsplit": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
+                              ctypes.POINTER(Geom), _P, _IP, _IP, _IP, _I, _IP, _IP, _D, _D, _D,
+                              _D, _P, _P],
     "clawhip_step3_split": [ctypes.POINTER(RP), _I, _P, ctypes.POINTER(Geom), _P,
                             ctypes.POINTER(Geom), _P, _P, _P, _P, _IP, _IP, _IP, _I, _IP, _IP,
                             _D, _D, _D, _D, _P, _P],
@@ -142,7 +148,8 @@ _SIGS = {
     "clawhip_l1_diff": [_I, _P, _P, ctypes.c_longlong, _P, _P, _P],
 }
 
-_LONG_RESULT = {"clawhip_ensemble_table_bytes", "clawhip_sums_work_doubles"}
+_LONG_RESULT = {"clawhip_ensemble_table_bytes", "clawhip_sums_work_doubles",
+                "clawhip_step3_unsplit_work_elems"}
 
 _lib = None
 

@@ -79,8 +79,8 @@ class ClawSolver(Solver):
     def setup(self, solution):
         self._check_cfl_settings()
         self._set_mthlim()
-        if self.fwave:
-            raise NotImplementedError("f-wave Riemann solvers are not available on the device")
+        if self.fwave and self.num_dim != 2:
+            raise NotImplementedError("f-wave Riemann solvers are supported by the 2D steps")
         state = solution.states[0]
         if self.rp is None or not hasattr(self.rp, "kind"):
             raise Exception("the solver needs one of the pyclaw_amd.riemann modules")
@@ -97,14 +97,45 @@ class ClawSolver(Solver):
                                           "step with the variable-coefficient solvers")
             if state.index_capa >= state.num_aux:
                 raise ValueError("index_capa %d is not an aux field" % state.index_capa)
-        if (getattr(self.rp, "kind", 0) in (6, 9) and self.num_dim == 2
+        if (getattr(self.rp, "kind", 0) in (6, 9, 11) and self.num_dim == 2
                 and not self.dimensional_split and self.transverse_waves != 0):
             raise NotImplementedError("%s has no transverse solver here: use dimensional "
                                       "splitting or transverse_waves = 0"
                                       % self.rp.__name__.split(".")[-1])
         self._set_method(state)
-        self._rp_struct = K.make_rp(self.rp.kind, state.problem_data, meqn=state.num_eqn)
+        self._rp_struct = K.make_rp(self.rp.kind, state.problem_data, meqn=state.num_eqn,
+                                    fwave=self.fwave)
+        # the host qbc that custom BC functions fill: kept from step to step as
+        # the reference keeps solver.qbc (solver.py:341-343)
+        self.qbc = None
         self._is_set_up = True
+
+    def _custom_bc_input(self, state, lo, hi):
+        """User BC functions act on a host qbc (solver.py:393-398); the kernel then
+        reads the stored ghosts (BC_GIVEN)."""
+        import torch
+        from . import _lib as L
+        from .device import to_planar, torch_dtype
+        ng = self.num_ghost
+        q = state.q
+        shape = (q.shape[0],) + tuple(n + 2 * ng for n in q.shape[1:])
+        if self.qbc is None or self.qbc.shape != shape:
+            self.qbc = np.zeros(shape, order="F")
+        qbc = self.qbc
+        qbc[(slice(None),) + (slice(ng, -ng),) * self.num_dim] = q
+        if state.num_aux > 0:
+            # the user's functions get auxbc too (solver.py:371-398)
+            self._host_auxbc(state)
+        _fill_bcs_host(self, state, qbc, lo, hi)
+        t = torch.from_numpy(to_planar(qbc)).to("cuda", torch_dtype(self._dtype_code))
+        comp = state.device_q(self._dtype_code)[1]
+        if comp is not None:
+            cp = torch.zeros_like(t)
+            cp[(slice(None),) + (slice(ng, -ng),) * self.num_dim] = comp
+            comp = cp
+        if self.num_dim == 3:
+            return (t, comp), L.geom3(q.shape[1], q.shape[2], q.shape[3], ng)
+        return (t, comp), L.geom2(q.shape[1], q.shape[2], ng)
 
 
 class ClawSolver2D(ClawSolver):
@@ -156,10 +187,27 @@ class ClawSolver2D(ClawSolver):
         aux = alo = ahi = None
         if state.num_aux > 0 and getattr(self.rp, "num_aux", 0):
             aux, alo, ahi = self._aux_input(state)
-        if BC.custom in lo or BC.custom in hi:
+        custom = BC.custom in lo or BC.custom in hi
+        if custom:
             src, gin = self._custom_bc_input(state, lo, hi)
             lo, hi = [4, 4], [4, 4]
-        if self.dimensional_split:
+        if self.dimensional_split and custom:
+            # the sweeps write the whole qbc, ghost rows and columns included,
+            # which the next step's BC functions see where they set nothing
+            import torch
+            ng = self.num_ghost
+            whole = (torch.empty_like(src[0]),
+                     torch.empty_like(src[1]) if src[1] is not None else None)
+            K.step2_split(self._rp_struct, src[0], whole[0], cfl_dev, lo, hi,
+                          self.reflect_index, self._method, self._mthlim, dx, dy, self.dt,
+                          comp_in=src[1], comp_out=whole[1], gin=gin, num_ghost=ng, aux=aux,
+                          aux_bc_lower=alo, aux_bc_upper=ahi, whole=True)
+            for a, b in zip(out, whole):
+                if a is not None:
+                    a.copy_(b[:, ng:-ng, ng:-ng])
+            from .device import from_planar
+            self.qbc = np.asfortranarray(from_planar(whole[0].double().cpu().numpy()))
+        elif self.dimensional_split:
             self._work = K.step2_split(self._rp_struct, src[0], out[0], cfl_dev, lo, hi,
                                        self.reflect_index, self._method, self._mthlim, dx, dy,
                                        self.dt, comp_in=src[1], comp_out=out[1], gin=gin,
@@ -304,31 +352,10 @@ class ClawSolver2D(ClawSolver):
             self._graphs[key] = ent
         ent["graph"].replay()
 
-    def _custom_bc_input(self, state, lo, hi):
-        """User BC functions act on a host qbc (solver.py:393-398); the kernel then
-        reads the stored ghosts (BC_GIVEN)."""
-        import torch
-        from . import _lib as L
-        from .device import to_planar, torch_dtype
-        ng = self.num_ghost
-        q = state.q
-        qbc = np.zeros((q.shape[0],) + tuple(n + 2 * ng for n in q.shape[1:]), order="F")
-        qbc[(slice(None),) + (slice(ng, -ng),) * self.num_dim] = q
-        if state.num_aux > 0:
-            # the user's functions get auxbc too (solver.py:371-398)
-            self._host_auxbc(state)
-        _fill_bcs_host(self, state, qbc, lo, hi)
-        t = torch.from_numpy(to_planar(qbc)).to("cuda", torch_dtype(self._dtype_code))
-        comp = state.device_q(self._dtype_code)[1]
-        if comp is not None:
-            cp = torch.zeros_like(t)
-            cp[:, ng:-ng, ng:-ng] = comp
-            comp = cp
-        return (t, comp), L.geom2(q.shape[1], q.shape[2], ng)
-
 
 class ClawSolver3D(ClawSolver):
-    """classic/solver.py:554-706 (dimension-split path)."""
+    """classic/solver.py:554-706: the dimension-split step (step3ds) and the
+    unsplit step (step3, FP64, euler_3D)."""
     no_trans = 0
     trans_inc = 11
     trans_cor = 22
@@ -347,23 +374,42 @@ class ClawSolver3D(ClawSolver):
 
     def setup(self, solution):
         if not self.dimensional_split:
-            raise NotImplementedError("the unsplit 3D step (step3.f90) is not part of pyclaw_amd")
+            if self.precision != "fp64":
+                raise NotImplementedError("the unsplit 3D step runs in FP64")
+            if self.transverse_waves not in (0, 10, 11, 20, 21, 22):
+                raise NotImplementedError("transverse_waves %r is not one of 0, 10, 11, 20, 21, "
+                                          "22" % (self.transverse_waves,))
+            if self.transverse_waves >= 20 and self.order != 2:
+                raise NotImplementedError("the correction's transverse split needs order 2")
         super(ClawSolver3D, self).setup(solution)
 
     def step_hyperbolic(self, solution):
         state = solution.states[0]
         dx, dy, dz = state.grid.delta
         lo, hi = self._bc_codes()
-        if BC.custom in lo or BC.custom in hi:
-            raise NotImplementedError("custom boundary conditions in 3D are not supported")
+        custom = BC.custom in lo or BC.custom in hi
+        if custom and self.dimensional_split:
+            raise NotImplementedError("custom boundary conditions in 3D: the unsplit step only")
         qin = state.device_q(self._dtype_code)
         out = self._take_scratch(qin)
         cfl_dev = self._cfl_buffer()
         K.cfl_reset(cfl_dev)
-        self._buf3 = K.step3_split(self._rp_struct, qin[0], out[0], cfl_dev, lo, hi,
-                                   self.reflect_index, self._method, self._mthlim, dx, dy, dz,
-                                   self.dt, comp_in=qin[1], comp_out=out[1], work=self._buf3,
-                                   num_ghost=self.num_ghost)
+        if not self.dimensional_split:
+            src, gin = qin, None
+            if custom:
+                src, gin = self._custom_bc_input(state, lo, hi)
+                lo, hi = [4, 4, 4], [4, 4, 4]
+            self._buf3 = K.step3_unsplit(self._rp_struct, src[0], out[0], cfl_dev, lo, hi,
+                                         self.reflect_index, self._method, self._mthlim, dx, dy,
+                                         dz, self.dt, num_ghost=self.num_ghost, gin=gin,
+                                         work=self._buf3 if not isinstance(self._buf3, tuple)
+                                         else None)
+        else:
+            self._buf3 = K.step3_split(self._rp_struct, qin[0], out[0], cfl_dev, lo, hi,
+                                       self.reflect_index, self._method, self._mthlim, dx, dy,
+                                       dz, self.dt, comp_in=qin[1], comp_out=out[1],
+                                       work=self._buf3 if isinstance(self._buf3, tuple) else None,
+                                       num_ghost=self.num_ghost)
         self.cfl.update_global_max(float(cfl_dev.item()))
         state.set_device_q(*out)
         self._scratch = qin

@@ -66,10 +66,17 @@ static int make_aux_in(const void* aux, const clawhip_geom* gaux, const int* alo
 }
 
 static int rp_num_aux(const clawhip_rp* rp) {
+  if (rp->kind == CLAWHIP_RP_SHALLOW_FWAVE) return 1;
   return rp->kind == CLAWHIP_RP_VC_ADVECTION || rp->kind == CLAWHIP_RP_VC_ADVECTION_2D ||
                  rp->kind == CLAWHIP_RP_VC_ACOUSTICS
              ? 2
              : 0;
+}
+
+// solvers without a transverse solver (rpt2): split, or transverse_waves = 0
+static bool rp_no_rpt(const clawhip_rp* rp) {
+  return rp->kind == CLAWHIP_RP_VC_ADVECTION || rp->kind == CLAWHIP_RP_SHALLOW_HLLE ||
+         rp->kind == CLAWHIP_RP_SHALLOW_FWAVE;
 }
 
 int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
@@ -98,9 +105,7 @@ int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
   if (na == 0 ? method[6] != 0 : (method[6] < na || method[6] > 3)) return CLAWHIP_E_METHOD;
   if (method[5] < 0 || method[5] > method[6]) return CLAWHIP_E_METHOD;
   // solvers without a transverse solver
-  if ((rp->kind == CLAWHIP_RP_VC_ADVECTION || rp->kind == CLAWHIP_RP_SHALLOW_HLLE) &&
-      method[2] != 0)
-    return CLAWHIP_E_METHOD;
+  if (rp_no_rpt(rp) && method[2] != 0) return CLAWHIP_E_METHOD;
   for (int d = 0; d < 2; d++) {
     int lo = bc_lower[d], hi = bc_upper[d];
     if (lo < 1 || lo > 4 || hi < 1 || hi > 4) return CLAWHIP_E_ARG;
@@ -120,11 +125,15 @@ static int check_common(const clawhip_rp* rp, int dtype, const void* q_in, const
                         const void* comp_in, const void* comp_out, double* cfl_dev,
                         const int* bc_lower, const int* bc_upper, const int* reflect,
                         const int* method, const int* mthlim, int ndim, int num_ghost,
-                        const clawhip_geom* gin, const clawhip_geom* gout, int num_aux = 0) {
+                        const clawhip_geom* gin, const clawhip_geom* gout, int num_aux = 0,
+                        bool ghost_out = false) {
   if (!rp || !q_in || !q_out || !cfl_dev || !bc_lower || !bc_upper || !reflect || !method ||
       !mthlim)
     return CLAWHIP_E_ARG;
-  if (!geom_ok(gin, ndim) || !geom_ok(gout, ndim) || gout->ghosts != 0) return CLAWHIP_E_ARG;
+  if (!geom_ok(gin, ndim) || !geom_ok(gout, ndim)) return CLAWHIP_E_ARG;
+  if (gout->ghosts != 0 && !(ghost_out && gout->ghosts == num_ghost && gin->ghosts == num_ghost))
+    return CLAWHIP_E_ARG;
+  if (rp->fwave && ndim != 2) return CLAWHIP_E_METHOD;
   if (gin->mx != gout->mx || gin->my != gout->my) return CLAWHIP_E_ARG;
   if (ndim == 3 && gin->mz != gout->mz) return CLAWHIP_E_ARG;
   if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
@@ -156,9 +165,9 @@ int clawhip_step2_split(const clawhip_rp* rp, int dtype, const void* q_in,
                         void* stream) {
   if (!rp) return CLAWHIP_E_ARG;
   int rc = check_common(rp, dtype, q_in, q_out, comp_in, comp_out, cfl_dev, bc_lower, bc_upper,
-                        reflect, method, mthlim, 2, num_ghost, gin, gout, rp_num_aux(rp));
+                        reflect, method, mthlim, 2, num_ghost, gin, gout, rp_num_aux(rp), true);
   if (rc) return rc;
-  if (!work || (dtype == CLAWHIP_F32K && !comp_work)) return CLAWHIP_E_ARG;
+  if (gout->ghosts == 0 && (!work || (dtype == CLAWHIP_F32K && !comp_work))) return CLAWHIP_E_ARG;
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
   if (method[2] != -1) return CLAWHIP_E_METHOD;
   AuxIn ax;
@@ -188,6 +197,34 @@ int clawhip_step3_split(const clawhip_rp* rp, int dtype, const void* q_in,
                        dx, dy, dz, dt, (unsigned long long*)cfl_dev, (hipStream_t)stream);
 }
 
+long long clawhip_step3_unsplit_work_elems(int meqn, int mx, int my, int mz) {
+  if (meqn < 1 || mx < 1 || my < 1 || mz < 1) return 0;
+  return step3_work_elems(meqn, mx, my, mz);
+}
+
+int clawhip_step3_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
+                          const clawhip_geom* gin, void* q_out, const clawhip_geom* gout,
+                          void* work, const int* bc_lower, const int* bc_upper,
+                          const int* reflect, int num_ghost, const int* method,
+                          const int* mthlim, double dx, double dy, double dz, double dt,
+                          double* cfl_dev, void* stream) {
+  int rc = check_common(rp, dtype, q_in, q_out, nullptr, nullptr, cfl_dev, bc_lower, bc_upper,
+                        reflect, method, mthlim, 3, num_ghost, gin, gout);
+  if (rc) return rc;
+  if (!work) return CLAWHIP_E_ARG;
+  if (dtype != CLAWHIP_F64) return CLAWHIP_E_ARG;
+  if (rp->kind != CLAWHIP_RP_EULER3) return CLAWHIP_E_KIND;
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
+  // transverse method(3) = 10 m3 + m4: 0 (none), 10, 11, 20, 21, 22; the
+  // correction's transverse split (m3 = 2) needs the second-order correction
+  const int t = method[2], m3 = t / 10, m4 = t - 10 * (t / 10);
+  if (!(t == 0 || ((m3 == 1 || m3 == 2) && m4 >= 0 && m4 <= m3))) return CLAWHIP_E_METHOD;
+  if (m3 == 2 && method[1] != 2) return CLAWHIP_E_METHOD;
+  Geom g = to_geom(gin);
+  return launch_step3(*rp, q_in, g, q_out, work, bc_lower, bc_upper, reflect, num_ghost, method,
+                      mthlim, dx, dy, dz, dt, (unsigned long long*)cfl_dev, (hipStream_t)stream);
+}
+
 int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
                          const clawhip_geom* g, const int* bc_lower, const int* bc_upper,
                          const int* reflect, int y_mode, int out_mode, double dt, double s,
@@ -200,6 +237,7 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
                          const int* aux_bc_upper, int num_aux, double* cfl_dev, void* stream) {
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !P || !out || !cfl_dev)
     return CLAWHIP_E_ARG;
+  if (rp->fwave) return CLAWHIP_E_METHOD;   // SharpClaw's f-wave form is not built
   // P with stored ghosts (gin, BC code 4 = given): a stage input whose ghosts
   // the caller filled, read as it is (y_mode 0, no indicator flags)
   if (gin && (!geom_ok(gin, 2) || gin->mx != g->mx || gin->my != g->my || gin->ghosts < 0))
@@ -256,6 +294,7 @@ int clawhip_sharp2_ssp33_run(const clawhip_rp* rp, int dtype, int char_decomp,
   if (!rp || !g || !bc_lower || !bc_upper || !reflect || !q0 || !q1 || !dq_dt || !r2 ||
       !ctl_dev || nsteps < 0)
     return CLAWHIP_E_ARG;
+  if (rp->fwave) return CLAWHIP_E_METHOD;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
@@ -313,7 +352,7 @@ int clawhip_step2_unsplit_run(const clawhip_rp* rp, int dtype, const clawhip_geo
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
   if (method[2] < 0 || method[2] > 2) return CLAWHIP_E_METHOD;
   if (method[5] != 0 || method[6] != 0) return CLAWHIP_E_METHOD;
-  if (rp->kind == CLAWHIP_RP_SHALLOW_HLLE && method[2] != 0) return CLAWHIP_E_METHOD;
+  if (rp_no_rpt(rp) && method[2] != 0) return CLAWHIP_E_METHOD;
   for (int d = 0; d < 2; d++)
     if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
       return CLAWHIP_E_ARG;
@@ -387,6 +426,7 @@ static int check_sweep(const clawhip_rp* rp, int dtype, int ndim, int dir, const
   if (method[2] != -1 || method[5] != 0) return CLAWHIP_E_METHOD;
   if (num_aux == 0 ? method[6] != 0 : (method[6] < num_aux || method[6] > 3))
     return CLAWHIP_E_METHOD;
+  if (rp->fwave && ndim != 2) return CLAWHIP_E_METHOD;
   return 0;
 }
 
@@ -431,6 +471,7 @@ int clawhip_sharp2_flux(const clawhip_rp* rp, int dtype, int char_decomp, const 
                         double dx, double dy, const void* aux, const clawhip_geom* gaux,
                         double* cfl_dev, void* stream) {
   if (!rp || !q || !gin || !dq || !gout || !cfl_dev) return CLAWHIP_E_ARG;
+  if (rp->fwave) return CLAWHIP_E_METHOD;
   if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_ARG;
   if (!geom_ok(gin, 2) || !geom_ok(gout, 2) || gin->ghosts < 3 || gout->ghosts != 0)

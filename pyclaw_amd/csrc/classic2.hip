@@ -239,6 +239,13 @@ int launch_unsplit(const clawhip_rp& rp, int dtype, const void* q_in, const Geom
                                                       comp_out, lo, hi, refl, ng, method, mthlim,
                                                       dx, dy, dt, cfl_bits, stream, aux);
   }
+  if (rp.kind == CLAWHIP_RP_SHALLOW_FWAVE) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    return dispatch_unsplit_aux<SolverOf<ShallowFwave>>(dtype, rp, q_in, gin, q_out, gout,
+                                                       comp_in, comp_out, lo, hi, refl, ng,
+                                                       method, mthlim, dx, dy, dt, cfl_bits,
+                                                       stream, aux);
+  }
   if (aux.p) return CLAWHIP_E_ARG;
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:

@@ -39,12 +39,12 @@ __device__ __forceinline__ void second_order3(const T (&W)[MW][ME], const T (&s)
 
 template <int ME, int MW, bool HALF_IN_SUM, typename T>
 __device__ __forceinline__ void corr_flux(const T (&W)[MW][ME], const T (&s)[MW], T dtdx,
-                                          T (&F)[ME]) {
+                                          T (&F)[ME], int fwave) {
   if (HALF_IN_SUM) {
-    second_order3<ME, MW>(W, s, dtdx, F);
+    second_order3<ME, MW>(W, s, dtdx, F);   // 3D: the entry points refuse f-waves
   } else {
     T cq[ME];
-    second_order<ME, MW>(W, s, dtdx, cq, F);
+    second_order<ME, MW>(W, s, dtdx, cq, F, fwave);
   }
 }
 
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) k_xsweep(const SweepArgs<T> a) {
       wave_dots<ME, MW>(W, Wn, dotr);
       from_left(dotr, dotl);
       limit_waves<ME, MW>(W, s, dotl, dotr, a.mthlim);
-      corr_flux<ME, MW, HALF_IN_SUM>(W, s, a.dtdx, F);
+      corr_flux<ME, MW, HALF_IN_SUM>(W, s, a.dtdx, F, a.rp.fwave);
     }
     T amn[ME], Fr[ME];
     from_right(am, amn);
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) k_march(const SweepArgs<T> a) {
     if (a.order != 1) {
       wave_dots<ME, MW>(Wt, Wn, dotn);
       limit_waves<ME, MW>(Wt, st, dotl, dotn, a.mthlim);
-      corr_flux<ME, MW, HALF_IN_SUM>(Wt, st, dtdx, F);
+      corr_flux<ME, MW, HALF_IN_SUM>(Wt, st, dtdx, F, a.rp.fwave);
     }
     if (t > j0 && wcol) {
       // cell t-1: apdq of interface t-1 (carried), amdq of interface t
@@ -398,16 +398,27 @@ int run_split2(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_
   gw.mx = gout.mx; gw.my = gout.my; gw.mz = 1; gw.g = ng;
   gw.rs = gout.mx + 2 * ng; gw.fs = (long long)gw.rs * (gout.my + 2 * ng);
   gw.ss = 0;   // 2D: no k index (cell_off adds (k-1+g)*ss with k = 1)
+  // The whole qbc as output: the x-sweep writes it (ghost rows swept, ghost
+  // columns copied), then the y-sweep runs in place on it and also writes the
+  // ghost columns' interior rows, as step2ds.f90:180-255 does.
+  const bool whole = gout.g == ng;
+  if (whole) {
+    gw = gout;
+    gw.ss = 0;
+    work = q_out;
+    cwork = cout;
+  }
   auto ax = base_args<T>(rp, gin, gw, lo, hi, refl, 2, ng, method, mthlim, RS::mwaves, dt / dx,
                          q_in, work, cin, cwork, cfl);
   set_aux(ax, aux.p, &aux.g, aux.lo, aux.hi);
   int rc = launch_x<RS, T, KAHAN, 2, false>(ax, stream);
   if (rc) return rc;
-  auto ay = base_args<T>(rp, gw, gout, lo, hi, refl, 2, ng, method, mthlim, RS::mwaves, dt / dy,
-                         work, q_out, cwork, cout, cfl);
+  auto ay = base_args<T>(rp, gw, whole ? gw : gout, lo, hi, refl, 2, ng, method, mthlim,
+                         RS::mwaves, dt / dy, work, q_out, cwork, cout, cfl);
   set_aux(ay, aux.p, &aux.g, aux.lo, aux.hi);
   ay.i_lo = 1 - ng; ay.i_hi = gout.mx + ng; ay.p_lo = ay.p_hi = 1;
-  ay.w_ilo = 1; ay.w_ihi = gout.mx; ay.w_plo = 1; ay.w_phi = 1;
+  ay.w_ilo = whole ? 1 - ng : 1; ay.w_ihi = whole ? gout.mx + ng : gout.mx;
+  ay.w_plo = 1; ay.w_phi = 1;
   return launch_march<RS, T, KAHAN, 2, false>(ay, stream);
 }
 
@@ -456,9 +467,10 @@ int run_sweep(const clawhip_rp& rp, int ndim, int dir, const void* q_in, const G
   set_aux(a, aux.p, &aux.g, given, given);
   if (ndim == 2) {
     if (dir == 1) return launch_x<RS, double, false, 2, false>(a, stream);
-    // step2ds ids = 2 (step2ds.f90:180): columns 1-ng..mx+ng
+    // step2ds ids = 2 (step2ds.f90:180-255): columns 1-ng..mx+ng are swept and
+    // written (rows 1..my), ghost columns included
     a.i_lo = 1 - ng; a.i_hi = gout.mx + ng; a.p_lo = a.p_hi = 1;
-    a.w_ilo = 1; a.w_ihi = gout.mx; a.w_plo = 1; a.w_phi = 1;
+    a.w_ilo = 1 - ng; a.w_ihi = gout.mx + ng; a.w_plo = 1; a.w_phi = 1;
     return launch_march<RS, double, false, 2, false>(a, stream);
   }
   if constexpr (RS::maux > 0) {
@@ -520,6 +532,12 @@ int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
     return dispatch_split2<VCAcoustics>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                         work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
                                         dt, aux, cfl, s);
+  }
+  if (rp.kind == CLAWHIP_RP_SHALLOW_FWAVE) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    return dispatch_split2<ShallowFwave>(dtype, rp, q_in, gin, q_out, gout, comp_in, comp_out,
+                                         work, cwork, lo, hi, refl, ng, method, mthlim, dx, dy,
+                                         dt, aux, cfl, s);
   }
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:
@@ -586,6 +604,11 @@ int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& g
     if (!aux.p) return CLAWHIP_E_ARG;
     return run_sweep<VCAcoustics<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method, mthlim,
                                           dtdx, aux, cfl, s);
+  }
+  if (rp.kind == CLAWHIP_RP_SHALLOW_FWAVE) {
+    if (!aux.p) return CLAWHIP_E_ARG;
+    return run_sweep<ShallowFwave<double>>(rp, 2, ids, q_in, gin, q_out, gout, ng, method,
+                                           mthlim, dtdx, aux, cfl, s);
   }
   switch (rp.kind) {
     case CLAWHIP_RP_ACOUSTICS:

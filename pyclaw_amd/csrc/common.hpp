@@ -197,18 +197,22 @@ __device__ __forceinline__ void wave_dots(const T (&A)[MW][ME], const T (&B)[MW]
   }
 }
 
-// flux2.f90:134-146: cqxx from limited waves, fadd = 0.5*cqxx.  dtdxave is
+// flux2.f90:134-160: cqxx from limited waves, fadd = 0.5*cqxx.  dtdxave is
 // 0.5*(dtdx1d(i-1) + dtdx1d(i)), the mean over the interface's two cells.
+// With f-waves the factor |s| becomes dsign(1, s) (the f-wave already carries
+// the speed); copysign keeps dsign's sign of a negative zero.
 template <int ME, int MW, typename T>
 __device__ __forceinline__ void second_order_ave(const T (&W)[MW][ME], const T (&s)[MW],
-                                                 T dtdxave, T (&cq)[ME], T (&F)[ME]) {
+                                                 T dtdxave, T (&cq)[ME], T (&F)[ME],
+                                                 int fwave = 0) {
 #pragma unroll
   for (int m = 0; m < ME; m++) {
     cq[m] = T(0);
 #pragma unroll
     for (int w = 0; w < MW; w++) {
       T sa = fabs(s[w]);
-      cq[m] = cq[m] + sa * (T(1) - sa * dtdxave) * W[w][m];
+      T f = fwave ? copysign(T(1), s[w]) : sa;
+      cq[m] = cq[m] + f * (T(1) - sa * dtdxave) * W[w][m];
     }
     F[m] = T(0) + T(0.5) * cq[m];
   }
@@ -217,8 +221,8 @@ __device__ __forceinline__ void second_order_ave(const T (&W)[MW][ME], const T (
 // The same with one dtdx for every cell (no capacity function).
 template <int ME, int MW, typename T>
 __device__ __forceinline__ void second_order(const T (&W)[MW][ME], const T (&s)[MW], T dtdx,
-                                             T (&cq)[ME], T (&F)[ME]) {
-  second_order_ave<ME, MW>(W, s, T(0.5) * (dtdx + dtdx), cq, F);
+                                             T (&cq)[ME], T (&F)[ME], int fwave = 0) {
+  second_order_ave<ME, MW>(W, s, T(0.5) * (dtdx + dtdx), cq, F, fwave);
 }
 
 }  // namespace claw

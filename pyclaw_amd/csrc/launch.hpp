@@ -46,6 +46,7 @@ inline RPParams<T> to_params(const clawhip_rp& r) {
   p.rho = T(r.rho); p.bulk = T(r.bulk); p.cc = T(r.cc); p.zz = T(r.zz);
   p.gamma = T(r.gamma); p.gamma1 = T(r.gamma - 1.0);
   p.grav = T(r.grav); p.u = T(r.u); p.v = T(r.v); p.efix = r.efix;
+  p.dry_tolerance = T(r.dry_tolerance); p.fwave = r.fwave;
   return p;
 }
 
@@ -79,6 +80,11 @@ int launch_split2(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
                   int ng, const int* method, const int* mthlim, double dx, double dy,
                   double dt, const AuxIn& aux, unsigned long long* cfl, hipStream_t s);
 
+long long step3_work_elems(int meqn, int mx, int my, int mz);
+int launch_step3(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q_out,
+                 void* work, const int* lo, const int* hi, const int* refl, int ng,
+                 const int* method, const int* mthlim, double dx, double dy, double dz,
+                 double dt, unsigned long long* cfl, hipStream_t stream);
 int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom& gin,
                   void* q_out, const Geom& gout, const void* comp_in, void* comp_out,
                   void* work, void* cwork, const int* lo, const int* hi, const int* refl,

@@ -22,6 +22,7 @@
 //   vc_advection(_vector)_2D  1 wave, edge velocities in aux
 //   vc_acoustics_2D           2 waves (-c left, +c right), rho and c in aux
 //   shallow_hlle_2D           HLLE, 2 waves at the Einfeldt speeds
+//   shallow_bathymetry_fwave_2D  f-waves, 3 waves at the Einfeldt speeds, bathymetry in aux
 // Every expression keeps the reference's evaluation order; the library is
 // compiled with -ffp-contract=off so FP64 results match the CPU oracle bitwise.
 #pragma once
@@ -35,7 +36,9 @@ struct RPParams {
   T gamma, gamma1;       // Euler
   T grav;                // shallow water
   T u, v;                // advection
+  T dry_tolerance;       // shallow water with bathymetry
   int efix;
+  int fwave;             // the solver returns f-waves (step2's use_fwave)
 };
 
 template <typename T>
@@ -404,12 +407,14 @@ struct Advection {
 template <typename T>
 struct Euler3 {
   static constexpr int meqn = 5, mwaves = 3, maux = 0;
-  struct Roe {};
+  // the interface's Roe averages, kept for the transverse splits (u is the
+  // normal velocity, v and w the next two in the cycle x -> y -> z)
+  struct Roe { T u, v, w, enth, a, g1a2, euv, u2v2w2; };
   // IXY = 1, 2, 3 selects x, y, z as the normal direction.
   template <int IXY>
   __device__ static __forceinline__ void rpn(const RPParams<T>& p, const T* qL, const T* qR,
                                              T (&wave)[mwaves][meqn], T (&s)[mwaves],
-                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+                                             T (&amdq)[meqn], T (&apdq)[meqn], Roe& roe) {
     constexpr int mu = IXY == 1 ? 1 : (IXY == 2 ? 2 : 3);
     constexpr int mv = IXY == 1 ? 2 : (IXY == 2 ? 3 : 1);
     constexpr int mw = IXY == 1 ? 3 : (IXY == 2 ? 1 : 2);
@@ -428,6 +433,8 @@ struct Euler3 {
     T a = sqrt(a2);
     T g1a2 = gamma1 / a2;
     T euv = enth - u2v2w2;
+    roe.u = u; roe.v = v; roe.w = w; roe.enth = enth; roe.a = a; roe.g1a2 = g1a2;
+    roe.euv = euv; roe.u2v2w2 = u2v2w2;
     T d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv], d3 = qR[mw] - qL[mw],
       d4 = qR[4] - qL[4];
     T a4 = g1a2 * (euv * d0 + u * d1 + v * d2 + w * d3 - d4);
@@ -511,6 +518,43 @@ struct Euler3 {
                                              T (&bm)[meqn], T (&bp)[meqn]) {
 #pragma unroll
     for (int m = 0; m < meqn; m++) { bm[m] = T(0); bp[m] = T(0); }
+  }
+  // euler_3D rpt3 / rptt3 (clawpack.riemann rpt3_euler.f90, rptt3_euler.f90,
+  // published form): asdq split in the transverse direction ICOOR (2: the
+  // direction after IXY in the cycle, momentum mv; 3: the next, mw) with the
+  // interface's Roe averages; three waves at vt - a, vt, vt + a with the shear
+  // parts lumped into the middle one; bm / bp take the negative / positive speeds.
+  template <int IXY, int ICOOR>
+  __device__ static __forceinline__ void rpt3(const Roe& r, const T (&asdq)[meqn],
+                                              T (&bm)[meqn], T (&bp)[meqn]) {
+    constexpr int mu = IXY == 1 ? 1 : (IXY == 2 ? 2 : 3);
+    constexpr int mv = IXY == 1 ? 2 : (IXY == 2 ? 3 : 1);
+    constexpr int mw = IXY == 1 ? 3 : (IXY == 2 ? 1 : 2);
+    constexpr int mt = ICOOR == 2 ? mv : mw, m1 = mu, m2 = ICOOR == 2 ? mw : mv;
+    const T u = r.u, v = r.v, w = r.w, a = r.a, enth = r.enth;
+    const T vt = ICOOR == 2 ? v : w, c1 = u, c2 = ICOOR == 2 ? w : v;
+    T a4 = r.g1a2 * (r.euv * asdq[0] + u * asdq[mu] + v * asdq[mv] + w * asdq[mw] - asdq[4]);
+    T b1 = asdq[m1] - c1 * asdq[0];
+    T b2 = asdq[m2] - c2 * asdq[0];
+    T a5 = (asdq[mt] + (a - vt) * asdq[0] - a * a4) / (T(2) * a);
+    T a1 = asdq[0] - a4 - a5;
+    T wv[3][meqn], sb[3];
+    wv[0][0] = a1; wv[0][m1] = a1 * c1; wv[0][mt] = a1 * (vt - a); wv[0][m2] = a1 * c2;
+    wv[0][4] = a1 * (enth - vt * a); sb[0] = vt - a;
+    wv[1][0] = a4; wv[1][m1] = a4 * c1 + b1; wv[1][mt] = a4 * vt; wv[1][m2] = a4 * c2 + b2;
+    wv[1][4] = a4 * T(0.5) * r.u2v2w2 + b1 * c1 + b2 * c2; sb[1] = vt;
+    wv[2][0] = a5; wv[2][m1] = a5 * c1; wv[2][mt] = a5 * (vt + a); wv[2][m2] = a5 * c2;
+    wv[2][4] = a5 * (enth + vt * a); sb[2] = vt + a;
+#pragma unroll
+    for (int m = 0; m < meqn; m++) {
+      bm[m] = T(0);
+      bp[m] = T(0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        bm[m] = bm[m] + dmin(sb[k], T(0)) * wv[k][m];
+        bp[m] = bp[m] + dmax(sb[k], T(0)) * wv[k][m];
+      }
+    }
   }
 };
 
@@ -693,6 +737,81 @@ struct VCAcoustics {
     bp[0] = cp * b2 * zp;
     bp[mu] = T(0);
     bp[mv] = cp * b2;
+  }
+};
+
+// ---------------------------------------------------------------- shallow water over bathymetry, f-waves
+// shallow_bathymetry_fwave_2D (clawpack.riemann rpn2_shallow_bathymetry_fwave.f90,
+// published form; pinned by the reference's sill fixture through the oracle):
+// the Roe f-wave solver with the source -g h b_x folded into the flux
+// difference, so a flat surface over any bathymetry stays at rest.  aux[0] = b.
+//   u, v = hu/h, hv/h where h > dry_tolerance, else 0; both sides dry: no waves;
+//   u^ = (sqrt(hL) uL + sqrt(hR) uR)/(sqrt(hL) + sqrt(hR)), hbar = (hL + hR)/2,
+//   c^ = sqrt(g hbar); s1 = min(uL - sqrt(g hL), u^ - c^),
+//   s3 = max(uR + sqrt(g hR), u^ + c^), s2 = (s1 + s3)/2;
+//   d1 = (hu)R - (hu)L, d2 = (hu^2 + g h^2/2)R - (...)L + g hbar (bR - bL),
+//   d3 = (huv)R - (huv)L; b1 = (s3 d1 - d2)/(s3 - s1), b3 = (d2 - s1 d1)/(s3 - s1),
+//   b2 = d3 - b1 vL - b3 vR; f-waves b1 (1, s1, vL), b2 (0, 0, 1), b3 (1, s3, vR);
+//   each f-wave goes to amdq (s < 0) or apdq (s > 0), half to each at s = 0.
+// No transverse solver: dimension-split steps, or unsplit with transverse_waves = 0.
+template <typename T>
+struct ShallowFwave {
+  static constexpr int meqn = 3, mwaves = 3, maux = 1;
+  struct Roe {};
+  template <int IXY>
+  __device__ static __forceinline__ void rpn_aux(const RPParams<T>& p, const T* qL, const T* qR,
+                                                 const T* auxL, const T* auxR,
+                                                 T (&fw)[mwaves][meqn], T (&s)[mwaves],
+                                                 T (&amdq)[meqn], T (&apdq)[meqn], Roe&) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T g = p.grav, dry = p.dry_tolerance;
+#pragma unroll
+    for (int w = 0; w < mwaves; w++) {
+      s[w] = T(0);
+#pragma unroll
+      for (int m = 0; m < meqn; m++) fw[w][m] = T(0);
+    }
+#pragma unroll
+    for (int m = 0; m < meqn; m++) { amdq[m] = T(0); apdq[m] = T(0); }
+    const T hl = qL[0], hr = qR[0];
+    if (hl <= dry && hr <= dry) return;
+    T ul = T(0), vl = T(0), ur = T(0), vr = T(0);
+    if (hl > dry) { ul = qL[mu] / hl; vl = qL[mv] / hl; }
+    if (hr > dry) { ur = qR[mu] / hr; vr = qR[mv] / hr; }
+    const T sl = sqrt(hl), sr = sqrt(hr);
+    const T hbar = T(0.5) * (hl + hr);
+    const T uhat = (sl * ul + sr * ur) / (sl + sr);
+    const T chat = sqrt(g * hbar);
+    s[0] = dmin(ul - sqrt(g * hl), uhat - chat);
+    s[2] = dmax(ur + sqrt(g * hr), uhat + chat);
+    s[1] = T(0.5) * (s[0] + s[2]);
+    const T d1 = hr * ur - hl * ul;
+    const T d2 = (hr * ur * ur + T(0.5) * g * hr * hr) - (hl * ul * ul + T(0.5) * g * hl * hl) +
+                 g * hbar * (auxR[0] - auxL[0]);
+    const T d3 = hr * ur * vr - hl * ul * vl;
+    const T b1 = (s[2] * d1 - d2) / (s[2] - s[0]);
+    const T b3 = (d2 - s[0] * d1) / (s[2] - s[0]);
+    const T b2 = d3 - b1 * vl - b3 * vr;
+    fw[0][0] = b1; fw[0][mu] = b1 * s[0]; fw[0][mv] = b1 * vl;
+    fw[1][mv] = b2;
+    fw[2][0] = b3; fw[2][mu] = b3 * s[2]; fw[2][mv] = b3 * vr;
+#pragma unroll
+    for (int w = 0; w < mwaves; w++) {
+#pragma unroll
+      for (int m = 0; m < meqn; m++) {
+        if (s[w] < T(0)) amdq[m] = amdq[m] + fw[w][m];
+        else if (s[w] > T(0)) apdq[m] = apdq[m] + fw[w][m];
+        else { amdq[m] = amdq[m] + T(0.5) * fw[w][m]; apdq[m] = apdq[m] + T(0.5) * fw[w][m]; }
+      }
+    }
+  }
+  // Not reached: the entry points refuse transverse_waves > 0 for this solver.
+  template <int IXY>
+  __device__ static __forceinline__ void rpt_aux(const RPParams<T>&, const Roe&, const T*,
+                                                 const T*, const T*, const T (&)[meqn],
+                                                 T (&bm)[meqn], T (&bp)[meqn]) {
+#pragma unroll
+    for (int m = 0; m < meqn; m++) { bm[m] = T(0); bp[m] = T(0); }
   }
 };
 

@@ -127,8 +127,8 @@ __device__ __forceinline__ void xpass(const UnsplitArgs<T>& a, const T (&Qc)[RS:
     wave_dots<ME, MW>(W, Wn, dotr);
     from_left(dotr, dotl);
     limit_waves<ME, MW>(W, s, dotl, dotr, a.mthlim);
-    if constexpr (CAPA) second_order_ave<ME, MW>(W, s, T(0.5) * (dtdx_l + dtdx_i), cq, F);
-    else second_order<ME, MW>(W, s, a.dtdx, cq, F);
+    if constexpr (CAPA) second_order_ave<ME, MW>(W, s, T(0.5) * (dtdx_l + dtdx_i), cq, F, a.rp.fwave);
+    else second_order<ME, MW>(W, s, a.dtdx, cq, F, a.rp.fwave);
   }
   T amn[ME];
   from_right(am, amn);
@@ -287,8 +287,8 @@ __device__ __forceinline__ void unsplit_body(const UnsplitArgs<T>& a, int wid) {
     if (a.order != 1) {
       wave_dots<ME, MW>(Wt, Wn, dotn);
       limit_waves<ME, MW>(Wt, st, dotl, dotn, a.mthlim);
-      if constexpr (CAPA) second_order_ave<ME, MW>(Wt, st, T(0.5) * (dyA + dyP), cq, Fy);
-      else second_order<ME, MW>(Wt, st, a.dtdy, cq, Fy);
+      if constexpr (CAPA) second_order_ave<ME, MW>(Wt, st, T(0.5) * (dyA + dyP), cq, Fy, a.rp.fwave);
+      else second_order<ME, MW>(Wt, st, a.dtdy, cq, Fy, a.rp.fwave);
     }
     T bmA[ME], bpA[ME], bmP[ME], bpP[ME];
 #pragma unroll

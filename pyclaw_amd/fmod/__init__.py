@@ -28,19 +28,19 @@ class CParam(object):
     (state.py:206-231) copies problem_data onto it."""
 
 
-def rp_struct(rpn, cparam, meqn=0):
+def rp_struct(rpn, cparam, meqn=0, use_fwave=False):
     """Device Riemann-solver parameters from the solver tag and cparam (meqn: the
-    number of fields, for the solvers that take any)."""
+    number of fields, for the solvers that take any; use_fwave: step2's flag)."""
     if isinstance(rpn, bool) or not isinstance(rpn, (int, np.integer)):
         raise TypeError("pyclaw_amd.fmod runs the device Riemann solvers: use a "
                         "pyclaw_amd.riemann module as the solver's riemann_solver")
     pd = {k: v for k, v in vars(cparam).items() if not k.startswith("_")}
-    return K.make_rp(int(rpn), pd, meqn=meqn)
+    return K.make_rp(int(rpn), pd, meqn=meqn, fwave=use_fwave)
 
 
-def check_method(method, use_fwave, allow_aux=False, allow_capa=False):
-    if use_fwave:
-        raise NotImplementedError("f-wave Riemann solvers are not supported")
+def check_method(method, use_fwave, allow_aux=False, allow_capa=False, allow_fwave=False):
+    if use_fwave and not allow_fwave:
+        raise NotImplementedError("f-wave Riemann solvers are supported by the 2D steps")
     if int(method[5]) != 0 and not allow_capa:
         raise NotImplementedError("capacity functions are supported by the unsplit 2D step only")
     if int(method[6]) != 0 and not allow_aux:

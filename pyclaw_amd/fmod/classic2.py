@@ -16,8 +16,8 @@ def step2(maxm, num_ghost, mx, my, qold, qnew, aux, dx, dy, dt, method, mthlim, 
           aux3, work, use_fwave, rpn2, rpt2):
     """Unsplit step (classic/step2.f90:2-250): qnew's interior <- step of qold
     (ghost cells filled by the caller, in aux too).  Returns (qnew, cfl)."""
-    check_method(method, use_fwave, allow_aux=True, allow_capa=True)
-    rp = rp_struct(rpn2, cparam, meqn=qold.shape[0])
+    check_method(method, use_fwave, allow_aux=True, allow_capa=True, allow_fwave=True)
+    rp = rp_struct(rpn2, cparam, meqn=qold.shape[0], use_fwave=use_fwave)
     qin = to_device2(qold)
     out = qin.new_empty((qold.shape[0], my, mx))
     cfl = new_cfl()
@@ -33,8 +33,8 @@ def step2ds(maxm, num_ghost, mx, my, qold, qnew, aux, dx, dy, dt, method, mthlim
             aux3, work, ids, use_fwave, rpn2, rpt2):
     """One dimension-split sweep (classic/step2ds.f90:2-264): ids 1 = x, 2 = y.
     Returns (qnew, cfl)."""
-    check_method(method, use_fwave, allow_aux=True)
-    rp = rp_struct(rpn2, cparam, meqn=qold.shape[0])
+    check_method(method, use_fwave, allow_aux=True, allow_fwave=True)
+    rp = rp_struct(rpn2, cparam, meqn=qold.shape[0], use_fwave=use_fwave)
     g = L.geom2(mx, my, num_ghost)
     qin = to_device2(qold)
     qout = torch.empty_like(qin) if ids == 1 else qin

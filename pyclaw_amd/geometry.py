@@ -157,6 +157,13 @@ class Patch(object):
     def __init__(self, dimensions):
         self.level = 1
         self.patch_index = 1
+        if isinstance(dimensions, Dimension):
+            dimensions = [dimensions]
+        # one patch covers the domain: every side is a physical boundary
+        # (geometry.py:692-695)
+        for dim in dimensions:
+            dim.on_lower_boundary = True
+            dim.on_upper_boundary = True
         self.grid = Grid(dimensions)
 
     def __getattr__(self, key):

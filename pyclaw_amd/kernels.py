@@ -26,9 +26,10 @@ def _dtype_code(t):
     raise L.ClawhipError(f"unsupported dtype {t.dtype}")
 
 
-def make_rp(kind, problem_data, efix=1, meqn=0):
+def make_rp(kind, problem_data, efix=1, meqn=0, fwave=False):
     """Parameters of the Riemann solver from problem_data (state.py:206-231);
-    meqn: the number of fields, for solvers that take any (vc advection)."""
+    meqn: the number of fields, for solvers that take any (vc advection);
+    fwave: the solver returns f-waves (solver.fwave, step2's use_fwave)."""
     pd = problem_data
     rho = float(pd.get("rho", 1.0))
     bulk = float(pd.get("bulk", 1.0))
@@ -36,14 +37,15 @@ def make_rp(kind, problem_data, efix=1, meqn=0):
     zz = float(pd.get("zz", rho * cc))
     return L.RP(kind, int(pd.get("efix", efix)), rho, bulk, cc, zz,
                 float(pd.get("gamma", 1.4)), float(pd.get("grav", 1.0)),
-                float(pd.get("u", 0.0)), float(pd.get("v", 0.0)), int(meqn))
+                float(pd.get("u", 0.0)), float(pd.get("v", 0.0)), int(meqn), int(bool(fwave)),
+                float(pd.get("dry_tolerance", 1e-3)), float(pd.get("sea_level", 0.0)))
 
 
 def cfl_reset(cfl):
     L.check(L.lib().clawhip_cfl_reset(_ptr(cfl), _stream()), "clawhip_cfl_reset")
 
 
-_RP_MEQN = {1: 3, 2: 4, 3: 3, 4: 1, 5: 5, 7: 1, 8: 3, 9: 3, 10: 5}
+_RP_MEQN = {1: 3, 2: 4, 3: 3, 4: 1, 5: 5, 7: 1, 8: 3, 9: 3, 10: 5, 11: 3}
 
 
 def _check_fields(rp, meqn, aux, method):
@@ -105,19 +107,25 @@ def _work_like(q_out, ng, work):
 
 def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim, dx, dy, dt,
                 comp_in=None, comp_out=None, num_ghost=2, gin=None, work=None, aux=None,
-                aux_bc_lower=None, aux_bc_upper=None):
+                aux_bc_lower=None, aux_bc_upper=None, whole=False):
     """Both dimension-split sweeps (step2ds ids=1, 2); returns the work buffers.
-    aux: planar (maux, my, mx) tensor of q's dtype for solvers with aux fields."""
+    aux: planar (maux, my, mx) tensor of q's dtype for solvers with aux fields.
+    whole: q_in and q_out are whole qbc arrays (num_ghost stored ghosts); q_out
+    receives every cell the reference's sweeps write (the sweeps run in it)."""
     meqn, my, mx = q_out.shape
     gout = L.geom2(mx, my)
+    if whole:
+        my, mx = my - 2 * num_ghost, mx - 2 * num_ghost
+        gout = L.geom2(mx, my, num_ghost)
     if gin is None:
         gin = gout
     code = _dtype_code(q_out)
     _check_fields(rp, meqn, aux, method)
     w, cw = work if work is not None else (None, None)
-    w = _work_like(q_out, num_ghost, w)
-    if code == L.F32K:
-        cw = _work_like(q_out, num_ghost, cw)
+    if not whole:
+        w = _work_like(q_out, num_ghost, w)
+        if code == L.F32K:
+            cw = _work_like(q_out, num_ghost, cw)
     rc = L.lib().clawhip_step2_split(
         ctypes.byref(rp), code, _ptr(q_in), ctypes.byref(gin), _ptr(q_out), ctypes.byref(gout),
         _ptr(comp_in), _ptr(comp_out), _ptr(w), _ptr(cw), L.ints(bc_lower), L.ints(bc_upper),
@@ -128,6 +136,26 @@ def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthli
         L.ints(aux_bc_upper) if aux is not None else None, _ptr(cfl), _stream())
     L.check(rc, "clawhip_step2_split")
     return (w, cw)
+
+
+def step3_unsplit(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim, dx, dy,
+                  dz, dt, num_ghost=2, gin=None, work=None):
+    """The unsplit 3D step (step3), FP64: q_in (compact, or with stored ghosts
+    given by gin) -> q_out (planar (meqn, mz, my, mx)); returns the work buffer."""
+    meqn, mz, my, mx = q_out.shape
+    gout = L.geom3(mx, my, mz)
+    if gin is None:
+        gin = gout
+    n = int(L.lib().clawhip_step3_unsplit_work_elems(meqn, mx, my, mz))
+    if work is None or work.numel() != n:
+        work = torch.empty(n, dtype=torch.float64, device=q_out.device)
+    rc = L.lib().clawhip_step3_unsplit(
+        ctypes.byref(rp), _dtype_code(q_out), _ptr(q_in), ctypes.byref(gin), _ptr(q_out),
+        ctypes.byref(gout), _ptr(work), L.ints(bc_lower), L.ints(bc_upper), L.ints(reflect),
+        int(num_ghost), L.ints(method), L.ints(list(mthlim) + [0] * (4 - len(mthlim))),
+        float(dx), float(dy), float(dz), float(dt), _ptr(cfl), _stream())
+    L.check(rc, "clawhip_step3_unsplit")
+    return work
 
 
 def step3_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim, dx, dy, dz,

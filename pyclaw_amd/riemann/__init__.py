@@ -11,3 +11,4 @@ from . import static  # noqa: F401
 from . import acoustics_2D, acoustics_2D_ptwise, advection_2D, euler_3D, euler_4wave_2D, shallow_roe_with_efix_2D  # noqa: F401
 from . import euler_5wave_2D, euler_5wave_2D_constants  # noqa: F401
 from . import shallow_hlle_2D, vc_acoustics_2D, vc_advection_2D, vc_advection_vector_2D  # noqa: F401
+from . import shallow_bathymetry_fwave_2D  # noqa: F401

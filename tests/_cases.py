@@ -259,6 +259,72 @@ def shock_bubble_dq_src(gamma=1.4):
     return dq_src
 
 
+def shock_bubble_3d(num_cells=(64, 16, 16), gamma=1.4):
+    """examples/euler_3d/shock_bubble.py: a planar shock (p = 5 behind it,
+    40-43) entering at x = 0 (incoming_shock 63-72) and hitting a low-density
+    sphere (rho 0.1, radius 0.2, centre (0.5, 0, 0)) on [0,2]x[0,0.5]x[0,0.5],
+    walls at y = 0 and z = 0 (setup 75-150).  Cells within max(delta) of the
+    sphere's surface take the volume fraction inside it from scipy's dblquad
+    over x and y of the sphere's z-extent clipped to the cell (bubble 46-61,
+    epsabs 1e-3, epsrel 1e-2).  Returns (q, delta, incoming_state)."""
+    from scipy import integrate
+    g1 = gamma - 1.
+    x0, y0, z0, rb = 0.5, 0., 0., 0.2
+    rhoout, pout, rhoin, pin, xshock = 1.0, 1.0, 0.1, 1.0, 0.2
+    p_shock = 5.0
+    rho_shock = (g1 + p_shock * (gamma + 1.)) / ((gamma + 1.) + g1 * p_shock)
+    v_shock = (p_shock - 1.) / np.sqrt(0.5 * ((gamma + 1.) * p_shock + g1))
+    e_shock = 0.5 * rho_shock * v_shock ** 2 + p_shock / g1
+    (X, Y, Z), d = grid([0., 0., 0.], [2., 0.5, 0.5], list(num_cells))
+    r0 = np.sqrt((X - x0) ** 2 + (Y - y0) ** 2 + (Z - z0) ** 2)
+    q = np.zeros((5,) + tuple(num_cells), order="F")
+    q[0] = rho_shock * (X < xshock) + rhoout * (X >= xshock)
+    q[1] = rho_shock * v_shock * (X < xshock)
+    q[4] = e_shock * (X < xshock) + pout / g1 * (X >= xshock)
+
+    def extent(yy, xx, zdown, zup):
+        z2 = rb ** 2 - (xx - x0) ** 2 - (yy - y0) ** 2
+        top = 0 if z2 < 0 else z0 + np.sqrt(z2)
+        bot = 0 if z2 < 0 else z0 - np.sqrt(z2)
+        top = min(top, zup)
+        return top - min(top, max(bot, zdown))
+
+    dx, dy, dz = d
+    dmax = max(d)
+    for i, j, k in zip(*np.nonzero(r0 - dmax <= rb)):
+        frac, _ = integrate.dblquad(extent, X[i, j, k] - dx / 2., X[i, j, k] + dx / 2.,
+                                    lambda x, y=Y[i, j, k]: y - dy / 2.,
+                                    lambda x, y=Y[i, j, k]: y + dy / 2.,
+                                    args=(Z[i, j, k] - dz / 2., Z[i, j, k] + dz / 2.),
+                                    epsabs=1.e-3, epsrel=1.e-2)
+        frac = frac / (dx * dy * dz)
+        q[0, i, j, k] = rhoout * (1. - frac) + rhoin * frac
+        q[4, i, j, k] = (pout * (1. - frac) + pin * frac) / g1
+    return q, d, np.array([rho_shock, rho_shock * v_shock, 0., 0., e_shock])
+
+
+def sill():
+    """examples/shallow_2d/sill.py: bathymetry 22-24 (a Gaussian hump
+    b = 0.8 exp(-10((x-1)^2 + (y-0.5)^2))), the wave maker 26-36 (h mirrored into
+    the lower-x ghost cells; in ghost cell num_ghost-1 the momentum 2 vwall - hu
+    of the first interior cell, vwall = 0.1 sin(pi t / 1.5) until t = 5), setup
+    41-82 (80 x 20 cells on [0,4]x[0,1], h = 1 - b, g = 1, dry tolerance 1e-3).
+    Returns (q, aux, delta, wave_maker(t, qbc, ng)); wave_maker leaves the other
+    ghost values of qbc as they are, as the example's function does."""
+    (X, Y), d = grid([0., 0.], [4., 1.], [80, 20])
+    aux = (0.8 * np.exp(-10 * ((X - 1.) ** 2 + (Y - 0.5) ** 2)))[None]
+    q = np.zeros((3, 80, 20), order="F")
+    q[0] = 1. - aux[0]
+
+    def wave_maker(t, qbc, ng):
+        qbc[0, :ng, :] = qbc[0, ng, :]
+        vwall = 0.1 * np.sin(t * np.pi / 1.5) if t <= 5. else 0.
+        for ibc in range(ng - 1):
+            qbc[1, ng - ibc - 1, :] = 2 * vwall - qbc[1, ng + ibc, :]
+
+    return q, np.asfortranarray(aux), d, wave_maker
+
+
 def radial_dam_break():
     """examples/shallow_2d/radial_dam_break.py: qinit 22-30, setup 33-80."""
     (X, Y), d = grid([-2.5, -2.5], [2.5, 2.5], [150, 150])

@@ -23,6 +23,8 @@ Sources (reference test that consumes each fixture):
   examples/advection_2d_annulus/test_2d_advection_annulus.py:10-32
   examples/acoustics_2d_variable/test_acoustics_2d_variable.py:6-23 (pressure_classic.txt)
   examples/acoustics_2d_variable/test_acoustics_2d_variable_io.py:1-60 (io_test_verification)
+  examples/shallow_2d/test_shallow2d.py:38-39 (sill)
+  examples/psystem_2d/test_2d_psystem.py:9-25 (verify_gauge*.txt)
 """
 import os
 import pickletools
@@ -41,7 +43,7 @@ WANT = {
                      "sharpclaw"],
     "euler_2d": ["quadrants_roe", "shock_bubble_interaction"],
     "shallow_2d": ["radialdambreak_classic_roe", "radialdambreak_sharpclaw_roe",
-                   "radialdambreak_classic_hlle", "radialdambreak_sharpclaw_hlle"],
+                   "radialdambreak_classic_hlle", "radialdambreak_sharpclaw_hlle", "sill"],
     "euler_3d": ["shocktube", "shock_bubble"],
     "advection_2d_annulus": ["classic"],
 }
@@ -123,6 +125,14 @@ def main():
     dst = os.path.join(HERE, "advection_reaction_2d_expected.npz")
     np.savez_compressed(dst, density=rho)
     print(dst, rho.shape)
+    # the psystem_2d example's gauge records (t, stress, 10*stress per line)
+    gauges = {}
+    for name in ("0.25_0.25", "17.85_1.25", "3.25_18.75", "11.75_11.75"):
+        gauges["gauge" + name] = np.loadtxt(os.path.join(REF, "psystem_2d",
+                                                         f"verify_gauge{name}.txt"))
+    dst = os.path.join(HERE, "psystem_2d_gauges.npz")
+    np.savez_compressed(dst, **gauges)
+    print(dst, {k: v.shape for k, v in gauges.items()})
 
 
 if __name__ == "__main__":

@@ -2,6 +2,7 @@
 declared entry point; host-side validation of the Python wrappers."""
 import ctypes
 
+import numpy as np
 import pytest
 
 from pyclaw_amd import _lib as L
@@ -82,7 +83,18 @@ def test_unsupported_options_raise_at_setup():
     s.a, s.b = [[0.0]], [1.0]
     with pytest.raises(KeyError):         # RK has no default CFL (sharpclaw/solver.py:246-247)
         s.setup(_small_solution(pyclaw, 3))
-    c = pyclaw.ClawSolver2D(riemann.euler_4wave_2D)
+    c = pyclaw.ClawSolver3D(riemann.euler_3D)   # f-waves: the 2D steps only
     c.fwave = True
+    sol3 = pyclaw.Solution(5, pyclaw.Domain([0., 0., 0.], [1., 1., 1.], [4, 4, 4]))
     with pytest.raises(NotImplementedError):
-        c.setup(_small_solution(pyclaw, 4))
+        c.setup(sol3)
+    s = pyclaw.SharpClawSolver2D(riemann.shallow_roe_with_efix_2D)
+    s.fwave = True
+    with pytest.raises(NotImplementedError):
+        s.setup(_small_solution(pyclaw, 3))
+    c = pyclaw.ClawSolver2D(riemann.shallow_bathymetry_fwave_2D)   # no transverse solver
+    c.dimensional_split = False
+    sol = _small_solution(pyclaw, 3)
+    sol.state.aux = np.zeros((1,) + sol.state.q.shape[1:], order="F")
+    with pytest.raises(NotImplementedError):
+        c.setup(sol)

@@ -95,6 +95,26 @@ def test_fmod_step3ds():
     assert np.array_equal(O.interior(got, ng), O.interior(ref, ng))
 
 
+def test_fmod_step3():
+    """fmod.step3 (the unsplit 3D step, trans_cor) on a filled qbc: the interior
+    equals the oracle's step3 bit for bit, as does the CFL."""
+    kind = O.EULER3
+    fm = _fmod("classic3", kind)
+    rng = np.random.default_rng(71)
+    mx, my, mz, ng = 17, 12, 10, 2
+    q = seeded_state(rng, kind, (mx, my, mz))
+    qbc = O.with_ghosts(q, ng)
+    O.bc3(qbc, ng, [3, 1, 2], [1, 3, 2])
+    d = (1.0 / mx, 1.0 / my, 1.0 / mz)
+    meth = np.array([1, 2, 22, 0, 0, 0, 0])
+    mthlim = np.array([1, 1, 1])
+    ref = qbc.copy(order="F")
+    cref = O.step3(_orp(kind), qbc, ref, ng, *d, 0.003, list(meth), list(mthlim))
+    got, cfl = fm.step3(max(mx, my, mz), ng, mx, my, mz, qbc, qbc.copy(order="F"), None, *d,
+                        0.003, meth, mthlim, None, None, None, None, False, kind, kind, kind)
+    assert np.array_equal(O.interior(got, ng), O.interior(ref, ng)) and cfl == cref
+
+
 @pytest.mark.parametrize("kind,cd", [(O.SHALLOW, 2), (O.EULER, 2), (O.ACOUSTICS, 0)])
 def test_fmod_sharpclaw_flux2(kind, cd):
     fm = _fmod("sharpclaw2", kind)

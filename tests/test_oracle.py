@@ -659,6 +659,224 @@ def test_general_lmm_reduces_to_named_methods():
     assert 1.8 < rate < 3.2
 
 
+def sill_run(plant_zero_ghosts=False):
+    """examples/shallow_2d/sill.py on the oracle: split, minmod, f-waves
+    (shallow_bathymetry_fwave_2D), the wave maker on the lower x side, walls in
+    y, extrapolation on the upper x side and for the bathymetry.  The solver's
+    qbc keeps its ghost values from step to step (solver.py:341-343,
+    351-437): the wave maker sets only some ghost cells, the others hold what
+    the previous step's sweeps wrote there (step2ds.f90:180-255 sweeps the
+    ghost columns too).  plant_zero_ghosts clears them before each BC fill."""
+    q, aux, d, wave_maker = C.sill()
+
+    def ubc(run, dim, qbc):
+        if plant_zero_ghosts:
+            qbc[:, :run.ng, :] = 0.0
+        wave_maker(run.t, qbc, run.ng)
+
+    run = OracleRun(O.make_rp(O.SHALLOW_FWAVE, grav=1.0, dry_tolerance=1e-3), q, d,
+                    [O.CUSTOM, O.WALL], [O.EXTRAP, O.WALL], aux=aux, aux_bc_lower=[1, 1],
+                    aux_bc_upper=[1, 1], user_bc_lower=ubc, fwave=True, limiters=1)
+    return run, run.run(10.0, 40), d
+
+
+def test_fixture_sill_fwave(golden_dir):
+    """examples/shallow_2d/test_shallow2d.py:38-39 (L1 of the depth at t = 10
+    below 1e-6): the f-wave step2ds path with shallow_bathymetry_fwave_2D
+    restated, the wave maker and the persistent ghost cells.  It matches to
+    round-off; the Roe speeds instead of Einfeldt's (planted 11) or ghost cells
+    cleared before each fill fail the bar."""
+    exp = np.load(f"{golden_dir}/shallow_2d_expected.npz")["sill"]
+    run, fr, d = sill_run()
+    err = _l1(fr[-1][0], exp, d)
+    print("sill L1", err, "steps", run.numsteps)
+    assert err < 1e-12
+    _, fr0, _ = sill_run(plant_zero_ghosts=True)
+    assert _l1(fr0[-1][0], exp, d) > 1e-6
+    O.lib().orc_set_plant(11)
+    try:
+        _, fr11, _ = sill_run()
+    finally:
+        O.lib().orc_set_plant(0)
+    assert _l1(fr11[-1][0], exp, d) > 1e-6
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_live_reference_fwave_bit_exact():
+    """use_fwave = .true. through the reference's own step2 and step2ds
+    (classic/flux2.f90:134-160: dsign(1, s) in place of |s| in the second-order
+    correction, which trans_cor also folds into the transverse split): the
+    acoustics, Euler and shallow-water solvers' waves taken as f-waves (the
+    flux2 arithmetic does not care what the solver returns), and
+    shallow_bathymetry_fwave_2D with its bathymetry in both sweeps."""
+    rng = np.random.default_rng(17)
+    ng = 2
+    for kind in (O.ACOUSTICS, O.EULER, O.SHALLOW):
+        rp = O.make_rp(kind)
+        mw = O.MWAVES[kind]
+        for trans, ids in ((1, 0), (2, 0), (0, 1), (0, 2)):
+            qbc = O.with_ghosts(seeded_state(rng, kind, (15, 9)), ng)
+            O.bc2(qbc, ng, [O.WALL, O.EXTRAP], [O.EXTRAP, O.WALL])
+            meth = (O.method_array(transverse=trans) if ids == 0
+                    else O.method_array(dimensional_split=True))
+            a, b = qbc.copy(order="F"), qbc.copy(order="F")
+            c1 = O.step2(rp, qbc, a, ng, 0.07, 0.05, 0.011, meth, [4] * mw, ids=ids,
+                         use_fwave=True)
+            c2 = O.ref_step2(rp, qbc, b, ng, 0.07, 0.05, 0.011, meth, [4] * mw, ids=ids,
+                             use_fwave=True)
+            sl = (slice(None),) + (slice(ng, -ng),) * 2
+            assert np.array_equal(a[sl], b[sl]) and c1 == c2, (kind, trans, ids)
+            # the f-wave correction differs from the wave one on these states
+            c = qbc.copy(order="F")
+            O.step2(rp, qbc, c, ng, 0.07, 0.05, 0.011, meth, [4] * mw, ids=ids)
+            assert not np.array_equal(a[sl], c[sl])
+    rp = O.make_rp(O.SHALLOW_FWAVE, grav=1.0)
+    mx, my = 14, 11
+    for ids in (1, 2):
+        h = rng.uniform(0.5, 1.5, (mx, my))
+        q = np.asfortranarray(np.array([h, h * rng.uniform(-0.6, 0.6, (mx, my)),
+                                         h * rng.uniform(-0.6, 0.6, (mx, my))]))
+        qbc = O.with_ghosts(q, ng)
+        abc = O.with_ghosts(np.asfortranarray(rng.uniform(0, 0.4, (1, mx, my))), ng)
+        O.bc2(qbc, ng, [O.WALL, O.EXTRAP], [O.EXTRAP, O.WALL])
+        O.bc2(abc, ng, [O.EXTRAP, O.EXTRAP], [O.EXTRAP, O.EXTRAP], reflect=(-1, -1))
+        meth = O.method_array(dimensional_split=True, num_aux=1)
+        a, b = qbc.copy(order="F"), qbc.copy(order="F")
+        c1 = O.step2(rp, qbc, a, ng, 0.1, 0.08, 0.02, meth, [1] * 3, ids=ids, use_fwave=True,
+                     aux=abc)
+        c2 = O.ref_step2ds_aux(None, qbc, b, abc, ng, 0.1, 0.08, 0.02, meth, [1] * 3, ids,
+                               mwaves=3, rp=rp, use_fwave=True)
+        # the sweeps write every column (rows) including ghosts: compare them all
+        assert np.array_equal(a, b) and c1 == c2, ids
+
+
+def test_fwave_lake_at_rest():
+    """shallow_bathymetry_fwave_2D balances the bathymetry source: a flat
+    surface over rough bathymetry stays at rest to round-off through both
+    sweeps (the property the f-wave form with g hbar (bR - bL) exists for)."""
+    rng = np.random.default_rng(3)
+    ng, mx, my = 2, 20, 16
+    b = rng.uniform(0.0, 0.6, (1, mx, my))
+    q = np.zeros((3, mx, my), order="F")
+    q[0] = 1.0 - b[0]
+    qbc = O.with_ghosts(q, ng)
+    abc = O.with_ghosts(np.asfortranarray(b), ng)
+    O.bc2(qbc, ng, [O.WALL, O.WALL], [O.WALL, O.WALL])
+    O.bc2(abc, ng, [O.EXTRAP] * 2, [O.EXTRAP] * 2, reflect=(-1, -1))
+    rp = O.make_rp(O.SHALLOW_FWAVE, grav=1.0)
+    meth = O.method_array(dimensional_split=True, num_aux=1)
+    old = qbc.copy(order="F")
+    O.step2(rp, old, qbc, ng, 0.05, 0.06, 0.01, meth, [4] * 3, ids=1, use_fwave=True, aux=abc)
+    O.step2(rp, qbc, qbc, ng, 0.05, 0.06, 0.01, meth, [4] * 3, ids=2, use_fwave=True, aux=abc)
+    sl = (slice(None),) + (slice(ng, -ng),) * 2
+    assert np.abs(qbc[sl] - old[sl]).max() < 1e-14
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_live_reference_step3_bit_exact():
+    """The unsplit 3D step (classic/step3.f90 with flux3.f90's transverse and
+    double-transverse terms): the oracle equals the reference's own Fortran
+    step3, compiled here and given the oracle's euler_3D rpn3 / rpt3 / rptt3,
+    on every qbc cell (ghost layers included) for transverse 10, 11, 20, 21,
+    22, first and second order, minmod and MC."""
+    rng = np.random.default_rng(5)
+    ng = 2
+    rp = O.make_rp(O.EULER3)
+    for trans, order, lim in [(10, 1, 1), (10, 2, 4), (11, 1, 4), (11, 2, 1), (20, 2, 4),
+                              (21, 2, 1), (22, 2, 4), (22, 2, 1)]:
+        q = seeded_state(rng, O.EULER3, (7, 6, 5))
+        qbc = O.with_ghosts(q, ng)
+        O.bc3(qbc, ng, [1, 3, 2], [3, 1, 2])
+        meth = [1, order, trans, 0, 0, 0, 0]
+        a, b = qbc.copy(order="F"), qbc.copy(order="F")
+        c1 = O.step3(rp, qbc, a, ng, 0.1, 0.12, 0.09, 0.01, meth, [lim] * 3)
+        c2 = O.ref_step3(rp, qbc, b, ng, 0.1, 0.12, 0.09, 0.01, meth, [lim] * 3)
+        assert np.array_equal(a, b) and c1 == c2, (trans, order, lim)
+
+
+def _euler3_flux(q, d, gamma=1.4):
+    """The Euler flux in direction d (0, 1, 2) of a conserved state."""
+    rho, mom, E = q[0], q[1:4], q[4]
+    p = (gamma - 1.) * (E - 0.5 * (mom @ mom) / rho)
+    vel = mom[d] / rho
+    f = np.empty(5)
+    f[0] = mom[d]
+    f[1:4] = mom * vel
+    f[1 + d] += p
+    f[4] = (E + p) * vel
+    return f
+
+
+def test_rpt3_euler_transverse_roe_property():
+    """euler_3D rpt3: for asdq = qR - qL the two parts sum to the jump of the
+    flux in the transverse direction (the Roe property of the interface's
+    averages in that direction), and with a transverse flow faster than sound
+    everything goes one way.  The normal velocity as the transverse speed
+    (planted 12) fails it."""
+    rng = np.random.default_rng(9)
+    rp = O.make_rp(O.EULER3)
+    cyc = {1: (0, 1, 2), 2: (1, 2, 0), 3: (2, 0, 1)}
+
+    def state(vel):
+        rho, p = rng.uniform(0.6, 1.4), rng.uniform(0.5, 1.5)
+        return np.array([rho, *(rho * vel), p / 0.4 + 0.5 * rho * (vel @ vel)])
+
+    for plant in (0, 12):
+        O.lib().orc_set_plant(plant)
+        try:
+            ok = True
+            for ixyz in (1, 2, 3):
+                for icoor in (2, 3):
+                    d = cyc[ixyz][icoor - 1]        # the transverse direction's axis
+                    for _ in range(5):
+                        qL, qR = state(rng.uniform(-1, 1, 3)), state(rng.uniform(-1, 1, 3))
+                        bm, bp = O.rpt3(rp, ixyz, icoor, qL, qR, qR - qL)
+                        jump = _euler3_flux(qR, d) - _euler3_flux(qL, d)
+                        ok &= bool(np.allclose(bm + bp, jump, rtol=1e-12, atol=1e-12))
+                    vel = rng.uniform(-0.2, 0.2, 3)
+                    vel[d] = 4.0                    # supersonic along the transverse axis
+                    qL, qR = state(vel), state(vel * 1.1)
+                    bm, bp = O.rpt3(rp, ixyz, icoor, qL, qR, qR - qL)
+                    ok &= bool(np.all(bm == 0.0))
+        finally:
+            O.lib().orc_set_plant(0)
+        assert ok == (plant == 0), plant
+
+
+def shock_bubble_3d_run(transverse=22, plant=0):
+    """examples/euler_3d/shock_bubble.py on the oracle: unsplit (transverse
+    22 = trans_cor), minmod, the incoming-shock ghost cells at x = 0, walls at
+    y = 0 and z = 0, extrapolation elsewhere; 64 x 16 x 16 to t = 0.1, 10 outputs."""
+    q, d, inc = C.shock_bubble_3d()
+
+    def ubc(run, dim, qbc):
+        if dim == 0:
+            for i in range(run.ng):
+                qbc[:, i, ...] = inc[:, None, None]
+
+    run = OracleRun(O.make_rp(O.EULER3, efix=1, gamma=1.4), q, d, [O.CUSTOM, O.WALL, O.WALL],
+                    [O.EXTRAP] * 3, dimensional_split=False, transverse=transverse, limiters=1,
+                    user_bc_lower=ubc)
+    O.lib().orc_set_plant(plant)
+    try:
+        return run, run.run(0.1, 10), d
+    finally:
+        O.lib().orc_set_plant(0)
+
+
+def test_fixture_shock_bubble_3d(golden_dir):
+    """examples/euler_3d/test_euler3d.py:31-34 (density L1 at t = 0.1 below 1e-6):
+    the unsplit 3D step with euler_3D's restated transverse solvers matches to
+    round-off; planted 12 fails the bar."""
+    exp = np.load(f"{golden_dir}/euler_3d_expected.npz")["shock_bubble"]
+    run, fr, d = shock_bubble_3d_run()
+    err = d[0] * d[1] * d[2] * np.abs(fr[-1][0].reshape(-1) - exp).sum()
+    print("shock bubble 3D L1", err, "steps", run.numsteps)
+    assert err < 1e-12
+    _, fr12, _ = shock_bubble_3d_run(plant=12)
+    assert d[0] * d[1] * d[2] * np.abs(fr12[-1][0].reshape(-1) - exp).sum() > 1e-6
+
+
 def test_fixture_radial_dam_break_classic(golden_dir):
     q, d = C.radial_dam_break()
     run = OracleRun(O.make_rp(O.SHALLOW, efix=1, grav=1.0), q, d, [1, 1], [3, 3], limiters=4)
