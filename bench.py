@@ -335,7 +335,9 @@ def ensemble_leg(steps=32, members=512, seed=4):
             "value": cells * steps / wall, "unit": "cell-updates/s", "members": members,
             "cells": cells, "steps": steps, "ms_per_step": 1e3 * wall / steps,
             "rows_per_wavefront": ens.rows,
+            "wavefronts_per_workgroup": ens.group_size,
             "autotune_ms": {str(k): 1e3 * v for k, v in timing.items()},
+            "autotune_workgroup_ms": {str(k): 1e3 * v for k, v in ens.group_timing.items()},
             "accepted_fraction": acc / float(members * (steps + 16)),
             "hbm_frac": BYTES_PER_CELL_F32K * cells / (wall / steps) / 1e9 / HBM_PEAK_GBS}
 

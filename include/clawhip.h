@@ -353,8 +353,9 @@ long long clawhip_ensemble_table_bytes(int n);
 /* Validates the members and writes the launch table (member order, wavefront
  * offsets, method) into table_dev (device memory of clawhip_ensemble_table_bytes
  * bytes; a synchronous host-to-device copy).  rows: rows per wavefront (0 =
- * chosen from the occupancy and the total work).  waves[2] receives the
- * wavefronts of the step launch and the member count, to pass to
+ * chosen from the occupancy and the total work).  waves[3] receives the
+ * wavefronts of the step launch, the member count and the wavefronts per
+ * workgroup (4; the caller may set 1 or 2 instead), to pass to
  * clawhip_ensemble_run.  Boundary codes 1..3, num_ghost 2, method/mthlim as
  * clawhip_step2_unsplit. */
 int clawhip_ensemble_prepare(const clawhip_rp *rp, int dtype, const clawhip_ens_member *members,

@@ -11,6 +11,12 @@ Flags:
                       FP32 division and square root by the hardware reciprocal
                       and square-root instructions (the FP32 paths are checked
                       against tolerances; FP64 division and sqrt stay IEEE)
+  -fgpu-flush-denormals-to-zero
+                      FP32 denormals flushed (FP64 denormals are kept): the FP32
+                      reciprocal and square root then need no range scaling
+                      (about 8 % fewer VALU instructions in the FP32 stage march);
+                      the FP32 states are O(1), denormals appear only in exact
+                      cancellation's neighbourhood
 """
 import glob
 import os
@@ -26,6 +32,7 @@ ARCH = os.environ.get("CLAWHIP_ARCH", "gfx950")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-mllvm", "-amdgpu-mfma-vgpr-form", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+         "-fgpu-flush-denormals-to-zero",
          f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function"]
 
 

@@ -135,10 +135,13 @@ int ens_choose_rows(const std::vector<EnsMember>& v) {
 template <class RS, typename T, bool KAHAN>
 void ens_launch(const EnsHeader* h, const EnsMember* m, const long long* waves, int nsteps,
                 hipStream_t s) {
-  const unsigned blocks = (unsigned)((waves[0] + 3) / 4);
+  // wavefronts per workgroup: 1, 2 or 4 (the autotuner's second parameter)
+  const long long wpb = (waves[2] == 1 || waves[2] == 2) ? waves[2] : 4;
+  const unsigned blocks = (unsigned)((waves[0] + wpb - 1) / wpb);
   const unsigned ctl_blocks = (unsigned)((waves[1] + 255) / 256);
   for (int n = 0; n < nsteps; n++) {
-    hipLaunchKernelGGL((k_ens_step<RS, T, KAHAN>), dim3(blocks), dim3(256), 0, s, h, m);
+    hipLaunchKernelGGL((k_ens_step<RS, T, KAHAN>), dim3(blocks), dim3((unsigned)(64 * wpb)), 0,
+                       s, h, m);
     hipLaunchKernelGGL(k_ens_ctl, dim3(ctl_blocks), dim3(256), 0, s, h, m);
   }
 }
@@ -194,6 +197,7 @@ int ensemble_prepare(const clawhip_rp& rp, const clawhip_ens_member* members, in
   hipError_t e = hipMemcpy(table_dev, buf.data(), buf.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) return (int)e;
   waves[0] = base; waves[1] = n;   // wavefronts of a step launch; members
+  waves[2] = 4;                    // wavefronts per workgroup (the caller may change it)
   return 0;
 }
 

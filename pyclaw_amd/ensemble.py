@@ -91,7 +91,7 @@ class Ensemble(object):
         nbytes = int(L.lib().clawhip_ensemble_table_bytes(n))
         if self._table is None or self._table.numel() < nbytes:
             self._table = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-        waves = (ctypes.c_longlong * 2)()
+        waves = (ctypes.c_longlong * 3)()
         s = self.solver
         rc = L.lib().clawhip_ensemble_prepare(
             ctypes.byref(self._rp), self._code, self._members, n, L.ints(self._lo),
@@ -100,6 +100,7 @@ class Ensemble(object):
             ctypes.c_void_p(self._table.data_ptr()), waves)
         L.check(rc, "clawhip_ensemble_prepare")
         self._waves = waves
+        self.group_size = int(waves[2])
         self.rows = rows
         self._graphs = {}
 
@@ -133,15 +134,18 @@ class Ensemble(object):
                 self._launch(n)
             done += n
 
-    def autotune(self, candidates=(4, 6, 8, 10, 12, 16, 24, 32, 48, 64), steps=4, repeat=2):
-        """Times `steps` ensemble steps per candidate slices-per-wavefront shape
-        (on copies of the members' controllers, so the ensemble's state does not
-        advance) and keeps the fastest.  Returns {rows: seconds per step}."""
+    def autotune(self, candidates=(4, 6, 8, 10, 12, 16, 24, 32, 48, 64), steps=4, repeat=2,
+                 group_sizes=(1, 2, 4)):
+        """Times `steps` ensemble steps per candidate launch shape -- rows per
+        wavefront, then wavefronts per workgroup at the best rows (the strip
+        width is the 64 lanes of a wavefront) -- on copies of the members'
+        controllers, so the ensemble's state does not advance, and keeps the
+        fastest.  Returns {rows: seconds per step} (4 wavefronts per workgroup)
+        and records {wavefronts per workgroup: seconds} in self.group_timing."""
         saved_ctl = self._ctl.clone()
         saved = [[t.clone() if t is not None else None for t in b[:4]] for b in self._bufs]
-        timing = {}
-        for rows in candidates:
-            self._prepare(rows)
+
+        def measure():
             best = None
             for _ in range(repeat):
                 self._launch(1)
@@ -151,14 +155,26 @@ class Ensemble(object):
                 torch.cuda.synchronize()
                 el = (time.perf_counter() - t0) / steps
                 best = el if best is None else min(best, el)
-            timing[rows] = best
             self._ctl.copy_(saved_ctl)
             for b, sb in zip(self._bufs, saved):
                 for t, st in zip(b[:4], sb):
                     if t is not None:
                         t.copy_(st)
+            return best
+
+        timing = {}
+        for rows in candidates:
+            self._prepare(rows)
+            timing[rows] = measure()
         best_rows = min(timing, key=timing.get)
         self._prepare(best_rows)
+        self.group_timing = {}
+        for wpb in group_sizes:
+            self._waves[2] = int(wpb)
+            self.group_timing[int(wpb)] = measure()
+        self.group_size = min(self.group_timing, key=self.group_timing.get)
+        self._waves[2] = self.group_size
+        self._graphs = {}
         return timing
 
     # ---------------------------------------------------------------- results

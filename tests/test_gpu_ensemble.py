@@ -42,6 +42,7 @@ def test_ensemble_members_equal_single_runs(precision):
     ens = Ensemble(_solver(precision), sols)
     timing = ens.autotune(candidates=(4, 8, 16), steps=2, repeat=1)
     assert set(timing) == {4, 8, 16}
+    assert set(ens.group_timing) == {1, 2, 4} and ens.group_size in (1, 2, 4)
     assert all(c.attempts == 0 for c in ens.controllers())   # tuning did not advance
     ens.run_steps(21)
     ctls = ens.controllers()
