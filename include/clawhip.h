@@ -301,7 +301,10 @@ int clawhip_sharp2_stage(const clawhip_rp *rp, int dtype, int char_decomp,
  *   clawhip_mlp_flag_list), recon (as clawhip_sharp2_stage), and the judge:
  *   a step that leaves a field of positive_mask not > 0 (or not finite) is
  *   recomputed with full WENO5 into the same buffers (dq_dt_full: work of the
- *   state's size).  Boundary codes 1..3.
+ *   state's size).  The recomputation is launched once, after the nsteps
+ *   attempts: a failed judgement closes the gates of the attempts after it, so
+ *   one call completes between 1 and nsteps attempts -- the caller reads
+ *   ctl->attempts and calls again for the rest.  Boundary codes 1..3.
  *   stage_events (may be NULL; not while the stream is captured): 6*nsteps
  *   hipEvent_t handles, recorded before and after the march launch of stages
  *   1, 2, 3 of each attempt (timing). */
@@ -438,7 +441,8 @@ int clawhip_mlp_logits(const float *feat, long long n, const float *params, floa
 /* The interfaces whose flag, or a neighbour's along the normal, is set -- the
  * ones clawhip_sharp2_stage reconstructs with the full WENO5 -- as interface ids
  * in the flags' layout, in no particular order; *count = their number.  list
- * holds up to (mx+1)*my + mx*(my+1) entries. */
+ * holds up to (mx+1)*my + mx*(my+1) entries; flags is 4-byte aligned (read as
+ * dwords). */
 int clawhip_mlp_flag_list(const unsigned char *flags, int mx, int my, int *list, int *count,
                           void *stream);
 

@@ -577,6 +577,7 @@ __global__ void __launch_bounds__(256) k_mlp_flags(const T* q, Geom g, BCSpec bc
     if (!*sel.gate) return;
     q = (const T*)sel.q[*sel.cur];
   }
+  if (sel.zero_count && blockIdx.x == 0 && threadIdx.x == 0) *sel.zero_count = 0;
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int stride = gridDim.x * 4;
@@ -646,6 +647,7 @@ __global__ void __launch_bounds__(256) k_mlp_flags_row(const T* q, Geom g, BCSpe
     if (!*sel.gate) return;
     q = (const T*)sel.q[*sel.cur];
   }
+  if (sel.zero_count && blockIdx.x == 0 && threadIdx.x == 0) *sel.zero_count = 0;
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int stride = gridDim.x * 4;

@@ -93,10 +93,13 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
 
 // A kernel of a device-controlled run: it runs only while *gate != 0 and reads
 // the state buffer q[*cur] (gate NULL: an eager launch on the given pointer).
+// zero_count (flags kernels; may be NULL): a counter the kernel sets to 0 when
+// it runs, for the flag list launched after it.
 struct BufSel {
   const int* gate = nullptr;
   const int* cur = nullptr;
   const void* q[2] = {nullptr, nullptr};
+  int* zero_count = nullptr;
 };
 
 // The learned indicator's selection for a SharpClaw stage: per-interface flags
@@ -138,7 +141,7 @@ int launch_sharp_ssp33_run(const clawhip_rp& rp, int dtype, int char_decomp, con
                           const SharpRunBufs& b, SharpCtl* ctl, int nsteps,
                           hipStream_t stream);
 int launch_flag_list(const unsigned char* flags, int mx, int my, int* list, int* count,
-                     hipStream_t s, const int* gate = nullptr);
+                     hipStream_t s, const int* gate = nullptr, bool zeroed = false);
 int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
                        const Geom& gin, const int* lo, const int* hi, const int* refl, int ymode, int omode,
                        double dt, double s, double dx, double dy, const void* P, const void* Q,

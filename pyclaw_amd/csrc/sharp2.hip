@@ -67,6 +67,7 @@ struct SharpArgs {
   T dt;                 // dt passed to the operator (dq = dt*L)
   T s;                  // scalar of the Y / e1 axpy
   T dtdx, dtdy;
+  T rdt;                // T(1/dt), dt in FP64 (the FP32 stages' dq_dt = D * rdt)
   const T* P;           // Y source
   const T* Q;           // axpy partner of P
   const T* A;           // stage anchor (q at the start of the step)
@@ -320,15 +321,18 @@ struct StageView {
   const T* comp_in;
   T* comp_out;
   T dt, s, dtdx, dtdy;
+  T rdt;   // FP32: T(1/dt) (dq_dt = D * rdt; FP64 divides, as the reference)
 };
 
 template <typename T>
 __device__ __forceinline__ StageView<T> stage_view(const SharpArgs<T>& a) {
-  StageView<T> v{a.P, a.A, a.out, a.comp_in, a.comp_out, a.dt, a.s, a.dtdx, a.dtdy};
+  StageView<T> v{a.P, a.A, a.out, a.comp_in, a.comp_out, a.dt, a.s, a.dtdx, a.dtdy, T(0)};
+  v.rdt = a.rdt;
   if (a.ctl) {
     const int cur = a.ctl->cur;
     const double dt = a.ctl->dt;
     v.dt = T(dt);
+    if (sizeof(T) == 4) v.rdt = T(1.0 / dt);
     v.s = T(dt);
     v.dtdx = T(dt / a.dx);
     v.dtdy = T(dt / a.dy);
@@ -379,7 +383,8 @@ __device__ __forceinline__ T stage_store(const SharpArgs<T>& a, const StageView<
   const int om = a.omode;
   T o;
   if (om == OUT_DQDT) {
-    o = D / v.dt;
+    if constexpr (sizeof(T) == 4) o = D * v.rdt;
+    else o = D / v.dt;
   } else if (om == OUT_SSP33_2) {
     o = T(0.75) * si.pa[m] + T(0.25) * (yv + D);
   } else if (om == OUT_SSP33_3) {
@@ -457,46 +462,132 @@ __device__ __forceinline__ void load_recon(const T* p, T (&qrL)[ME], T (&qlR)[ME
 
 // ---------------------------------------------------------------- flagged interfaces
 // The ids of the interfaces whose dilated flag is set (xflag / yflag), appended
-// to `list` with one atomic per 1024-thread block (a single counter taking one
-// atomic per wavefront serialised the pass); the order is free, each entry is
-// computed on its own.  *count must be 0 on entry (k_zero_flag_count).
+// to `list`; the order is free, each entry is computed on its own.  A thread
+// takes 4 consecutive ids (one aligned dword of flags, its neighbours as
+// byte-aligned dwords: the x-neighbours at +-1, the y-neighbours at +-mx), a
+// block of 1024 threads FLAG_LIST_GROUPS such groups of 4096 ids; the hits are
+// scanned over the block group by group, so a block's ids land in the list in
+// increasing order (the full-WENO5 pass over the list then reads neighbouring
+// stencils together), and the block takes its range with one atomic.  Same-
+// address atomics serialise, so a block covers 16384 ids: about 500 atomics at
+// 2048^2.  *count must be 0 on entry (k_zero_flag_count, or the flags kernel of
+// a device-controlled run).
 __global__ void k_zero_flag_count(int* count, const int* gate) {
   if (gate && !*gate) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0;
 }
 
 constexpr int FLAG_LIST_THREADS = 1024;
+constexpr int FLAG_LIST_GROUPS = 4;
 
-__global__ void __launch_bounds__(FLAG_LIST_THREADS) k_flag_list(const unsigned char* f, int mx,
-                                                                 int my, int* list, int* count,
-                                                                 const int* gate) {
-  if (gate && !*gate) return;
-  __shared__ int wave_n[FLAG_LIST_THREADS / 64];
-  __shared__ int block_base;
-  const long long nx = (long long)(mx + 1) * my, n = nx + (long long)mx * (my + 1);
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  bool t = false;
-  if (i < nx) {
-    t = xflag(f, mx, 1 + (int)(i % (mx + 1)), 1 + (int)(i / (mx + 1)));
-  } else if (i < n) {
-    const long long j = i - nx;
-    t = yflag(f, mx, my, 1 + (int)(j % mx), 1 + (int)(j / mx));
+// dword i of the flags (bytes past the end read as 0; only the last partial
+// dword is assembled bytewise)
+__device__ __forceinline__ unsigned flag_dword(const unsigned char* f, int i, int nfull, int n) {
+  if (i < nfull) return reinterpret_cast<const unsigned*>(f)[i];
+  unsigned v = 0u;
+  for (int k = 0; k < 4; k++)
+    if (4 * i + k < n) v |= (unsigned)f[4 * i + k] << (8 * k);
+  return v;
+}
+
+// the 4 flag bytes from byte offset o >= 0
+__device__ __forceinline__ unsigned flag_bytes4(const unsigned char* f, int o, int nfull, int n) {
+  const int i = o >> 2;
+  return __builtin_amdgcn_alignbyte(flag_dword(f, i + 1, nfull, n), flag_dword(f, i, nfull, n),
+                                    (unsigned)(o & 3));
+}
+
+// bit k: interface 4q+k is listed (its flag or a neighbour's along its normal)
+__device__ __forceinline__ unsigned dilated_mask4(const unsigned char* f, int mx, int my, int q) {
+  const int nx = (mx + 1) * my, n = nx + mx * (my + 1), nfull = n >> 2;
+  const int p0 = 4 * q;
+  if (p0 >= n) return 0u;
+  const unsigned c = flag_dword(f, q, nfull, n);
+  unsigned m = 0u;
+  if (p0 < nx) {   // x-interfaces: neighbours in the row, clamped at its ends
+    const unsigned left = q > 0 ? flag_bytes4(f, p0 - 1, nfull, n) : (c << 8);
+    const unsigned right = flag_bytes4(f, p0 + 1, nfull, n);
+    int col = p0 % (mx + 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const unsigned b = (c >> (8 * k)) & 0xffu;
+      const unsigned a = col > 0 ? (left >> (8 * k)) & 0xffu : b;
+      const unsigned d = col < mx ? (right >> (8 * k)) & 0xffu : b;
+      if (p0 + k < nx) m |= (unsigned)((a | b | d) != 0u) << k;
+      col = col == mx ? 0 : col + 1;
+    }
   }
-  const unsigned long long mask = __ballot(t);
-  if (lane == 0) wave_n[wave] = __popcll(mask);
+  if (p0 + 3 >= nx) {   // y-interfaces: neighbours one row of mx away, clamped
+    const unsigned up = flag_bytes4(f, max(p0 - mx, 0), nfull, n);
+    const unsigned dn = flag_bytes4(f, p0 + mx, nfull, n);
+    int t = p0 >= nx ? (p0 - nx) / mx : -1;
+    int col = p0 >= nx ? (p0 - nx) - t * mx : 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int p = p0 + k;
+      if (p >= nx && p < n) {
+        if (p == nx) { t = 0; col = 0; }
+        const unsigned b = (c >> (8 * k)) & 0xffu;
+        const unsigned a = t > 0 ? (up >> (8 * k)) & 0xffu : b;
+        const unsigned d = t < my ? (dn >> (8 * k)) & 0xffu : b;
+        m |= (unsigned)((a | b | d) != 0u) << k;
+        if (++col == mx) { col = 0; t++; }
+      }
+    }
+  }
+  return m;
+}
+
+__global__ void __launch_bounds__(FLAG_LIST_THREADS)
+    k_flag_list(const unsigned char* f, int mx, int my, int* list, int* count, const int* gate) {
+  constexpr int U = FLAG_LIST_GROUPS, NW = FLAG_LIST_THREADS / 64;
+  if (gate && !*gate) return;
+  __shared__ int wave_base[U][NW];
+  __shared__ int block_base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned m[U];
+  int incl[U], cnt[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    m[u] = dilated_mask4(f, mx, my, (blockIdx.x * U + u) * FLAG_LIST_THREADS + threadIdx.x);
+    cnt[u] = __popc(m[u]);
+    incl[u] = cnt[u];
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int v = __shfl_up(incl[u], d, 64);
+      if (lane >= d) incl[u] += v;
+    }
+  }
+  if (lane == 63) {
+#pragma unroll
+    for (int u = 0; u < U; u++) wave_base[u][wave] = incl[u];
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     int tot = 0;
-    for (int w = 0; w < FLAG_LIST_THREADS / 64; w++) {
-      const int k = wave_n[w];
-      wave_n[w] = tot;   // exclusive prefix
-      tot += k;
-    }
+    for (int u = 0; u < U; u++)
+      for (int w = 0; w < NW; w++) {
+        const int v = wave_base[u][w];
+        wave_base[u][w] = tot;
+        tot += v;
+      }
     block_base = tot > 0 ? atomicAdd(count, tot) : 0;
   }
   __syncthreads();
-  if (t) list[block_base + wave_n[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = (int)i;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    int pos = block_base + wave_base[u][wave] + incl[u] - cnt[u];
+    const int p0 = 4 * ((blockIdx.x * U + u) * FLAG_LIST_THREADS + threadIdx.x);
+    unsigned mm = m[u];
+    while (mm) {
+      const int k = __ffs(mm) - 1;
+      mm &= mm - 1u;
+      list[pos++] = p0 + k;
+    }
+  }
 }
 
 // The full WENO5 (characteristic or component, as configured) of every listed
@@ -538,10 +629,13 @@ __global__ void __launch_bounds__(256) k_weno_flagged(const SharpArgs<T> a) {
 }
 
 int launch_flag_list(const unsigned char* flags, int mx, int my, int* list, int* count,
-                     hipStream_t s, const int* gate) {
-  hipLaunchKernelGGL(k_zero_flag_count, dim3(1), dim3(64), 0, s, count, gate);
+                     hipStream_t s, const int* gate, bool zeroed) {
   const long long n = (long long)(mx + 1) * my + (long long)mx * (my + 1);
-  hipLaunchKernelGGL(k_flag_list, dim3((unsigned)((n + FLAG_LIST_THREADS - 1) / FLAG_LIST_THREADS)),
+  if (n > 0x7fffffffLL - 4LL * FLAG_LIST_THREADS * FLAG_LIST_GROUPS) return CLAWHIP_E_ARG;
+  if ((reinterpret_cast<unsigned long long>(flags) & 3ull) != 0ull) return CLAWHIP_E_ARG;
+  if (!zeroed) hipLaunchKernelGGL(k_zero_flag_count, dim3(1), dim3(64), 0, s, count, gate);
+  const long long per_block = 4LL * FLAG_LIST_THREADS * FLAG_LIST_GROUPS;
+  hipLaunchKernelGGL(k_flag_list, dim3((unsigned)((n + per_block - 1) / per_block)),
                      dim3(FLAG_LIST_THREADS), 0, s, flags, mx, my, list, count, gate);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : (int)e;
@@ -898,6 +992,7 @@ SharpArgs<T> make_args(const clawhip_rp& rp, const Geom& g, const Geom& gin, con
   a.char_decomp = CHAR ? 2 : 0;
   a.ymode = ymode; a.omode = omode;
   a.dt = T(dt); a.s = T(s);
+  a.rdt = T(1.0 / dt);
   a.dtdx = T(dt / dx); a.dtdy = T(dt / dy);
   a.P = (const T*)P; a.Q = (const T*)Q; a.A = (const T*)A; a.B = (const T*)B;
   a.aux_in = (const T*)aux_in; a.comp_in = (const T*)comp_in;
@@ -945,7 +1040,11 @@ int run_stage(const clawhip_rp& rp, const Geom& g, const Geom& gin, const int* l
 //   stage 1: dq_dt = D(q)/dt                         gate run1, PH_MID
 //   stage 2: r2 (SSP33 or FP32 stage 2)              gate run23
 //   stage 3: q[1-cur] (SSP33 or Kahan stage 3)       gate run23, PH_END / PH_END_JUDGE
-//   [indicator: the same step with full WENO5]       gate run_fb, PH_END_FB
+// and, with the indicator, once after the nsteps attempts:
+//   the judged step again with full WENO5            gate run_fb, PH_END_FB
+// A judged step that fails positivity closes the attempt gates, so the attempts
+// after it in this launch do nothing and the recomputation ends it: a launch
+// completes at least one attempt and at most nsteps (ctl->attempts counts them).
 // No host synchronisation: the launches may be captured in a HIP graph.
 template <class RS, typename T, bool CHAR>
 int run_ssp33(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi, const int* refl,
@@ -1007,24 +1106,26 @@ int run_ssp33(const clawhip_rp& rp, const Geom& g, const int* lo, const int* hi,
   sel.cur = &ctl->cur;
   sel.q[0] = b.q[0];
   sel.q[1] = b.q[1];
+  sel.zero_count = b.flag_count;   // the flags kernel clears the list length
   for (int n = 0; n < nsteps; n++) {
     int rc = 0;
     if (judge) {
       rc = launch_mlp_flags(F32 ? CLAWHIP_F32K : CLAWHIP_F64, b.q[0], g, lo, hi, refl, RS::meqn,
                             b.flat, b.params, b.threshold, b.flags, stream, sel);
       if (!rc) rc = launch_flag_list(b.flags, g.mx, g.my, b.flag_list, b.flag_count, stream,
-                                     &ctl->run_flags);
+                                     &ctl->run_flags, true);
     }
     void* const* ev = b.stage_events ? b.stage_events + 6LL * n : nullptr;
     if (!rc) rc = launch_args<RS, T, CHAR>(s1, stream, ev);
     if (!rc) rc = launch_args<RS, T, CHAR>(s2, stream, ev ? ev + 2 : nullptr);
     if (!rc) rc = launch_args<RS, T, CHAR>(s3, stream, ev ? ev + 4 : nullptr);
-    if (judge) {
-      if (!rc) rc = launch_args<RS, T, CHAR>(f1, stream);
-      if (!rc) rc = launch_args<RS, T, CHAR>(f2, stream);
-      if (!rc) rc = launch_args<RS, T, CHAR>(f3, stream);
-    }
     if (rc) return rc;
+  }
+  if (judge) {
+    int rc = launch_args<RS, T, CHAR>(f1, stream);
+    if (!rc) rc = launch_args<RS, T, CHAR>(f2, stream);
+    if (!rc) rc = launch_args<RS, T, CHAR>(f3, stream);
+    return rc;
   }
   return 0;
 }

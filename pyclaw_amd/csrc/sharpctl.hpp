@@ -16,7 +16,10 @@
 //     when this attempt was rejected, since that attempt skips stage 1);
 //   after stage 3 on the indicator path (PH_END_JUDGE): the judge's step count
 //     and positivity check -- a non-positive or non-finite depth / density /
-//     energy opens the gate of the full-WENO5 recomputation instead of ending.
+//     energy opens the gate of the full-WENO5 recomputation instead of ending,
+//     and closes every other gate: the attempts still queued in the launch do
+//     nothing, the recomputation (launched once, after them) ends the attempt
+//     (PH_END_FB) and reopens the gates for the next launch.
 // The layout equals clawhip_sharpctl in include/clawhip.h.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -134,6 +137,9 @@ __device__ __forceinline__ void sharp_ctl_phase(SharpCtl* c, int phase) {
         // the judge's positivity fallback: the same step with full WENO5
         c->fallbacks += 1;
         c->run_fb = 1;
+        c->run_flags = 0;
+        c->run1 = 0;
+        c->run23 = 0;
         break;
       }
       sharp_end(c, false);

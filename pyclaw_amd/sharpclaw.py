@@ -100,6 +100,7 @@ class SharpClawSolver(Solver):
         self._ctl = None
         self._stage_events = None   # time_stages: events of the eager launches
         self._stage_ev_pos = 0
+        self._stage_ran = []        # time_stages: whether each timed stage's gate was open
         super(SharpClawSolver, self).__init__(riemann_solver, claw_package)
 
     # ---- sharpclaw/solver.py:202-265
@@ -630,26 +631,27 @@ class SharpClawSolver(Solver):
         return attempts, accepted
 
     def time_stages(self, solution, nsteps):
-        """nsteps attempts of the device loop launched without graphs, with HIP
-        events around the march launch of every main stage (on the stream it
-        runs on).  Returns the march durations in ms, three per attempt run on
-        the device (indicator audit attempts run on the host path and are not
-        timed)."""
+        """nsteps attempts of the device loop launched without graphs, one at a
+        time, with HIP events around the march launch of every main stage (on the
+        stream it runs on).  Returns the march durations in ms of the stages that
+        ran (a rejected attempt's successor skips stage 1; indicator audit
+        attempts run on the host path and are not timed)."""
         n = int(nsteps)
         events = [torch.cuda.Event(enable_timing=True) for _ in range(6 * n)]
         st = torch.cuda.current_stream()
         for e in events:
             e.record(st)   # creates the event handles
         use, self.use_graph = self.use_graph, False
-        self._stage_events, self._stage_ev_pos = events, 0
+        self._stage_events, self._stage_ev_pos, self._stage_ran = events, 0, []
         try:
             self.run_steps(solution, n)
-            used = self._stage_ev_pos
+            used, ran = self._stage_ev_pos, self._stage_ran
         finally:
             self.use_graph = use
-            self._stage_events, self._stage_ev_pos = None, 0
+            self._stage_events, self._stage_ev_pos, self._stage_ran = None, 0, []
         torch.cuda.synchronize()
-        return [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(used // 2)]
+        return [events[2 * i].elapsed_time(events[2 * i + 1]) for i in range(used // 2)
+                if ran[i]]
 
     def _host_attempt(self, solution, tstart, tend):
         """One attempt of Solver.evolve_to_time's loop (solver.py:607-651) on the
@@ -791,31 +793,37 @@ class SharpClawSolver(Solver):
             c.judge_steps = ind.judge_stats["steps"]
             c.fallbacks = ind.judge_stats["positivity_fallbacks"]
         self._ctl.copy_(torch.frombuffer(bytearray(bytes(c)), dtype=torch.uint8))
-        launched = 0
-        host = L.SharpCtl()
-        t_now, dt_now = solution.t, self.dt
-        while launched < n:
+        host = L.SharpCtl.from_buffer_copy(bytes(c))
+        timing = self._stage_events is not None
+        while not host.done and host.attempts < n:
             # as many attempts as the remaining time needs (estimated from the
-            # current dt) before reading the controller back
-            todo = _attempts_to_launch(n - launched, tend, t_now, dt_now, self.graph_chunk)
+            # current dt) before reading the controller back; a launch whose
+            # judge falls back to full WENO5 ends early, so the controller's
+            # attempt count, not the launched one, says what is left.  Timed
+            # runs launch one attempt at a time and keep the gates it ran with.
+            before = host.attempts
+            todo = _attempts_to_launch(n - host.attempts, tend, host.t, host.dt,
+                                       self.graph_chunk)
+            if timing:
+                todo = 1
+                self._stage_ran.extend([bool(host.run1), bool(host.run23), bool(host.run23)])
             while todo > 0:
                 m = min(self.graph_chunk, todo)
                 if self.use_graph:
                     self._graph(bufs, args, kw, m).replay()
-                elif self._stage_events is not None:
+                elif timing:
                     p0 = self._stage_ev_pos
                     K.sharp2_ssp33_run(*args, m, stage_events=self._stage_events[p0:p0 + 6 * m],
                                        **kw)
                     self._stage_ev_pos = p0 + 6 * m
                 else:
                     K.sharp2_ssp33_run(*args, m, **kw)
-                launched += m
                 todo -= m
             ctypes.memmove(ctypes.addressof(host), bytes(self._ctl.cpu().numpy().tobytes()),
                            ctypes.sizeof(host))
-            if host.done:
-                break
-            t_now, dt_now = host.t, host.dt
+            if not host.done and host.attempts == before:
+                raise RuntimeError("device SSP33 loop made no progress (controller attempts "
+                                   f"stayed at {before})")
         solution.t = host.t
         self.dt = host.dt
         self.dt_old = host.dt_old if host.have_dt_old else None

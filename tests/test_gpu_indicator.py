@@ -198,6 +198,46 @@ def _advection_dq(q, flags, dt, dx, dy, u, v):
     return dq
 
 
+def _dilated_ids(fl, mx, my):
+    """Ids of the interfaces a flagged stage reconstructs with full WENO5: an
+    x-interface whose flag or a row neighbour's is set, a y-interface whose
+    flag or a column neighbour's is set (neighbours clamped at the ends)."""
+    nx = (mx + 1) * my
+    fx = fl[:nx].reshape(my, mx + 1).astype(bool)
+    fy = fl[nx:].reshape(my + 1, mx).astype(bool)
+    dx = fx.copy()
+    dx[:, 1:] |= fx[:, :-1]
+    dx[:, :-1] |= fx[:, 1:]
+    dy = fy.copy()
+    dy[1:] |= fy[:-1]
+    dy[:-1] |= fy[1:]
+    return np.concatenate([np.flatnonzero(dx.ravel()), nx + np.flatnonzero(dy.ravel())])
+
+
+@pytest.mark.parametrize("mx,my,frac", [(3, 3, 0.5), (4, 5, 0.3), (255, 3, 0.05),
+                                        (256, 7, 0.5), (1000, 9, 0.01), (9000, 3, 0.02),
+                                        (300, 301, 1.0), (64, 64, 0.0)])
+def test_flag_list_matches_dilated_flags(mx, my, frac):
+    """clawhip_mlp_flag_list: the listed ids are exactly the dilated flags, each
+    once (the order is free), on the smallest grid the entry point takes (3 x 3),
+    rows narrower than a wavefront, rows split over several blocks, all and no
+    flags; a second call over the same buffers starts
+    its count again from zero."""
+    from pyclaw_amd import kernels as K
+    print("flag list", mx, my, frac, flush=True)
+    rng = np.random.default_rng(mx * 7 + my)
+    n = K.mlp_num_interfaces(mx, my)
+    out = None
+    for rep in range(2):
+        fl = (rng.uniform(size=n) < frac).astype(np.uint8)
+        out = K.mlp_flag_list(torch.tensor(fl, device="cuda"), mx, my, out)
+        cnt = int(out[1].item())
+        got = np.sort(out[0][:cnt].cpu().numpy())
+        want = _dilated_ids(fl, mx, my)
+        assert cnt == want.size, (rep, cnt, want.size)
+        assert np.array_equal(got, want), rep
+
+
 @pytest.mark.parametrize("pattern", ["none", "ones", "zeros", "random"])
 def test_stage_flag_selection(pattern):
     from pyclaw_amd import kernels as K
