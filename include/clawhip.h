@@ -33,7 +33,17 @@ extern "C" {
 #define CLAWHIP_E_KIND (-2)    /* unknown Riemann solver / unsupported combination */
 #define CLAWHIP_E_METHOD (-3)  /* unsupported method flags */
 
-enum { CLAWHIP_F64 = 0, CLAWHIP_F32K = 1 };
+/* dtype: the arithmetic of a step.
+ *   CLAWHIP_F64       FP64, every expression rounded as the reference's Fortran
+ *                     (no contraction): bit for bit the reference's step;
+ *   CLAWHIP_F32K      FP32 state and arithmetic, Kahan-compensated update, FP64
+ *                     CFL; a*b + c within one expression is one FMA;
+ *   CLAWHIP_F64_FAST  FP64 state and arithmetic with a*b + c within one
+ *                     expression as one FMA: not bitwise, within about 1e-13
+ *                     relative of CLAWHIP_F64 per step.  Arrays as CLAWHIP_F64.
+ * The step controllers' FP64 arithmetic (dt, accept/reject, t) is the same in
+ * all three. */
+enum { CLAWHIP_F64 = 0, CLAWHIP_F32K = 1, CLAWHIP_F64_FAST = 2 };
 
 /* Riemann solver identifiers: the clawpack.riemann modules the reference's
  * examples pass to ClawSolver2D(...) (e.g. examples/euler_2d/quadrants.py:80). */

@@ -17,6 +17,11 @@ Flags:
                       (about 8 % fewer VALU instructions in the FP32 stage march);
                       the FP32 states are O(1), denormals appear only in exact
                       cancellation's neighbourhood
+
+The kernel sources in VARIANT_SRCS are compiled twice: as they are (namespace
+claw::exact, no contraction: CLAWHIP_F64) and with -DCLAW_FAST=1 (claw::fast,
+a*b + c in one expression contracted to an FMA: CLAWHIP_F32K and
+CLAWHIP_F64_FAST); csrc/variant.hpp describes the two.
 """
 import glob
 import os
@@ -36,16 +41,24 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function"]
 
 
+VARIANT_SRCS = ("classic2.hip", "classic_split.hip", "ensemble.hip", "sharp2.hip")
+
+
 def build(verbose=False, jobs=8):
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    units = [(s, os.path.basename(s) + ".o", []) for s in srcs]
+    units += [(s, os.path.basename(s) + ".fast.o", ["-DCLAW_FAST=1"]) for s in srcs
+              if os.path.basename(s) in VARIANT_SRCS]
+    # the largest sources first, so the parallel compile ends together
+    units.sort(key=lambda u: -os.path.getsize(u[0]))
     procs = []
     objs = []
-    for s in srcs:
-        o = os.path.join(objdir, os.path.basename(s) + ".o")
+    for s, name, extra in units:
+        o = os.path.join(objdir, name)
         objs.append(o)
-        cmd = [HIPCC] + FLAGS + ["-c", s, "-o", o]
+        cmd = [HIPCC] + FLAGS + extra + ["-c", s, "-o", o]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

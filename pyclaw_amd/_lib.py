@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libclawhip.so")
 
-F64, F32K = 0, 1
+F64, F32K, F64_FAST = 0, 1, 2
 RP_ACOUSTICS, RP_EULER, RP_SHALLOW, RP_ADVECTION, RP_EULER3 = 1, 2, 3, 4, 5
 BC_EXTRAP, BC_PERIODIC, BC_WALL, BC_GIVEN = 1, 2, 3, 4
 

@@ -201,7 +201,7 @@ class ClawSolver2D(ClawSolver):
             K.step2_split(self._rp_struct, src[0], whole[0], cfl_dev, lo, hi,
                           self.reflect_index, self._method, self._mthlim, dx, dy, self.dt,
                           comp_in=src[1], comp_out=whole[1], gin=gin, num_ghost=ng, aux=aux,
-                          aux_bc_lower=alo, aux_bc_upper=ahi, whole=True)
+                          aux_bc_lower=alo, aux_bc_upper=ahi, whole=True, fast=self._fast)
             for a, b in zip(out, whole):
                 if a is not None:
                     a.copy_(b[:, ng:-ng, ng:-ng])
@@ -212,12 +212,12 @@ class ClawSolver2D(ClawSolver):
                                        self.reflect_index, self._method, self._mthlim, dx, dy,
                                        self.dt, comp_in=src[1], comp_out=out[1], gin=gin,
                                        num_ghost=self.num_ghost, work=self._work, aux=aux,
-                                       aux_bc_lower=alo, aux_bc_upper=ahi)
+                                       aux_bc_lower=alo, aux_bc_upper=ahi, fast=self._fast)
         else:
             K.step2_unsplit(self._rp_struct, src[0], out[0], cfl_dev, lo, hi, self.reflect_index,
                             self._method, self._mthlim, dx, dy, self.dt, comp_in=src[1],
                             comp_out=out[1], gin=gin, num_ghost=self.num_ghost, aux=aux,
-                            aux_bc_lower=alo, aux_bc_upper=ahi)
+                            aux_bc_lower=alo, aux_bc_upper=ahi, fast=self._fast)
         cfl = float(cfl_dev.item())
         self.cfl.update_global_max(cfl)
         state.set_device_q(*out)
@@ -309,7 +309,7 @@ class ClawSolver2D(ClawSolver):
                 if self.use_graph and n == self.graph_chunk:
                     self._replay_graph(bufs, args, n)
                 else:
-                    K.step2_unsplit_run(*args, n)
+                    K.step2_unsplit_run(*args, n, fast=self._fast)
                 launched += n
                 todo -= n
             raw = self._ctl.cpu().numpy().tobytes()
@@ -335,7 +335,7 @@ class ClawSolver2D(ClawSolver):
 
     def _replay_graph(self, bufs, args, n):
         import torch
-        key = (bufs[0][0].data_ptr(), bufs[1][0].data_ptr())
+        key = (bufs[0][0].data_ptr(), bufs[1][0].data_ptr(), self._fast)
         ent = self._graphs.get(key)
         if ent is None:
             from .device import no_gc
@@ -346,7 +346,7 @@ class ClawSolver2D(ClawSolver):
             # would destroy it mid-capture, which HIP refuses (an abort)
             with no_gc(), torch.cuda.stream(s):
                 with torch.cuda.graph(g, stream=s):
-                    K.step2_unsplit_run(*args, n)
+                    K.step2_unsplit_run(*args, n, fast=self._fast)
             torch.cuda.current_stream().wait_stream(s)
             ent = {"graph": g, "bufs": bufs, "n": n}
             self._graphs[key] = ent
@@ -409,7 +409,7 @@ class ClawSolver3D(ClawSolver):
                                        self.reflect_index, self._method, self._mthlim, dx, dy,
                                        dz, self.dt, comp_in=qin[1], comp_out=out[1],
                                        work=self._buf3 if isinstance(self._buf3, tuple) else None,
-                                       num_ghost=self.num_ghost)
+                                       num_ghost=self.num_ghost, fast=self._fast)
         self.cfl.update_global_max(float(cfl_dev.item()))
         state.set_device_q(*out)
         self._scratch = qin

@@ -20,6 +20,11 @@ bool geom_ok(const clawhip_geom* g, int ndim) {
   return true;
 }
 
+bool valid_dtype(int d) { return d == CLAWHIP_F64 || d == CLAWHIP_F32K || d == CLAWHIP_F64_FAST; }
+
+// the element type of the arrays: CLAWHIP_F64_FAST stores FP64 like CLAWHIP_F64
+int storage_dtype(int d) { return d == CLAWHIP_F64_FAST ? CLAWHIP_F64 : d; }
+
 Geom to_geom(const clawhip_geom* g) {
   Geom o;
   o.mx = g->mx; o.my = g->my; o.mz = g->mz; o.g = g->ghosts;
@@ -95,7 +100,7 @@ int clawhip_step2_unsplit(const clawhip_rp* rp, int dtype, const void* q_in,
   if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
   if (q_in == q_out) return CLAWHIP_E_ARG;
   if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   if (!(dx > 0.0) || !(dy > 0.0) || !(dt >= 0.0)) return CLAWHIP_E_ARG;
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
   if (method[2] < 0 || method[2] > 2) return CLAWHIP_E_METHOD;   // unsplit only
@@ -139,7 +144,7 @@ static int check_common(const clawhip_rp* rp, int dtype, const void* q_in, const
   if (num_ghost != 2 || gout->mx < num_ghost || gout->my < num_ghost) return CLAWHIP_E_ARG;
   if (ndim == 3 && gout->mz < num_ghost) return CLAWHIP_E_ARG;
   if (q_in == q_out) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   if (dtype == CLAWHIP_F32K && (!comp_in || !comp_out)) return CLAWHIP_E_ARG;
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
   // no capacity; a solver with aux fields may be given more than it reads
@@ -246,7 +251,7 @@ int clawhip_sharp2_stage(const clawhip_rp* rp, int dtype, int char_decomp,
   if (ghosted && (y_mode != 0 || flags)) return CLAWHIP_E_ARG;
   if (flags && (!flag_list || !flag_count || !recon)) return CLAWHIP_E_ARG;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
   if (y_mode < 0 || y_mode > 2 || out_mode < 0 || out_mode > 8) return CLAWHIP_E_ARG;
   if (y_mode != 0 && !Q) return CLAWHIP_E_ARG;
@@ -296,7 +301,7 @@ int clawhip_sharp2_ssp33_run(const clawhip_rp* rp, int dtype, int char_decomp,
     return CLAWHIP_E_ARG;
   if (rp->fwave) return CLAWHIP_E_METHOD;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_METHOD;
   if (q0 == q1) return CLAWHIP_E_ARG;
   if (dtype == CLAWHIP_F32K && (!comp0 || !comp1 || comp0 == comp1 || !aux23))
@@ -325,13 +330,13 @@ int clawhip_lmm_update(int dtype, int mode, long long n, const void* q, const vo
   if (!q || !u || !c || !out || n < 0 || mode < 0 || mode > 5) return CLAWHIP_E_ARG;
   if ((mode == 1 || mode == 2) && !R) return CLAWHIP_E_ARG;
   if ((mode == 2 || mode == 4 || mode == 5) && !v) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   return launch_lmm_update(dtype, mode, n, q, u, R, v, c, out, (hipStream_t)stream);
 }
 
 int clawhip_ssp104_mid(int dtype, void* q, void* s1, long long n, void* stream) {
   if (!q || !s1 || n < 0 || q == s1) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   return launch_ssp104_mid(dtype, q, s1, n, (hipStream_t)stream);
 }
 
@@ -346,7 +351,7 @@ int clawhip_step2_unsplit_run(const clawhip_rp* rp, int dtype, const clawhip_geo
   if (!geom_ok(g, 2) || g->ghosts != 0 || num_ghost != 2 || g->mx < 2 || g->my < 2)
     return CLAWHIP_E_ARG;
   if (q0 == q1) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   if (dtype == CLAWHIP_F32K && (!comp0 || !comp1 || comp0 == comp1)) return CLAWHIP_E_ARG;
   if (!(dx > 0.0) || !(dy > 0.0)) return CLAWHIP_E_ARG;
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
@@ -374,7 +379,7 @@ int clawhip_ensemble_prepare(const clawhip_rp* rp, int dtype, const clawhip_ens_
   if (!rp || !members || n < 1 || !bc_lower || !bc_upper || !reflect || !method || !mthlim ||
       !table_dev || !waves || rows < 0)
     return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   if (num_ghost != 2) return CLAWHIP_E_ARG;
   if (method[1] != 1 && method[1] != 2) return CLAWHIP_E_METHOD;
   if (method[2] < 0 || method[2] > 2) return CLAWHIP_E_METHOD;
@@ -399,7 +404,7 @@ int clawhip_ensemble_run(const clawhip_rp* rp, int dtype, const void* table_dev,
                          const long long* waves, int nsteps, void* stream) {
   if (!rp || !table_dev || !waves || nsteps < 0 || waves[0] < 0 || waves[1] < 0)
     return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   return ensemble_run(*rp, dtype, table_dev, waves, nsteps, (hipStream_t)stream);
 }
 
@@ -472,7 +477,7 @@ int clawhip_sharp2_flux(const clawhip_rp* rp, int dtype, int char_decomp, const 
                         double* cfl_dev, void* stream) {
   if (!rp || !q || !gin || !dq || !gout || !cfl_dev) return CLAWHIP_E_ARG;
   if (rp->fwave) return CLAWHIP_E_METHOD;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
   if (char_decomp != 0 && char_decomp != 2) return CLAWHIP_E_ARG;
   if (!geom_ok(gin, 2) || !geom_ok(gout, 2) || gin->ghosts < 3 || gout->ghosts != 0)
     return CLAWHIP_E_ARG;
@@ -498,7 +503,8 @@ int clawhip_mlp_features(int dtype, const void* q, const clawhip_geom* g, const 
                          float wtol, float* feat, unsigned char* label, void* stream) {
   if (!q || !g || !bc_lower || !bc_upper || !reflect || !feat || !label) return CLAWHIP_E_ARG;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3 || meqn < 1) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
+  dtype = storage_dtype(dtype);
   for (int d = 0; d < 2; d++)
     if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
       return CLAWHIP_E_ARG;
@@ -536,7 +542,8 @@ int clawhip_mlp_flags(int dtype, const void* q, const clawhip_geom* g, const int
                       void* stream) {
   if (!q || !g || !bc_lower || !bc_upper || !reflect || !params || !flags) return CLAWHIP_E_ARG;
   if (!geom_ok(g, 2) || g->ghosts != 0 || g->mx < 3 || g->my < 3 || meqn < 1) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
+  dtype = storage_dtype(dtype);
   for (int d = 0; d < 2; d++)
     if (bc_lower[d] < 1 || bc_lower[d] > 3 || bc_upper[d] < 1 || bc_upper[d] > 3)
       return CLAWHIP_E_ARG;
@@ -552,7 +559,8 @@ int clawhip_state_sums(int dtype, const void* q, const clawhip_geom* g, int meqn
                        unsigned positive_mask, double* out, double* work, void* stream) {
   if (!q || !g || !out || !work || meqn < 1 || meqn > 7) return CLAWHIP_E_ARG;
   if (!geom_ok(g, g->mz > 1 ? 3 : 2) || g->mz < 1) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
+  dtype = storage_dtype(dtype);
   return launch_state_sums(dtype, q, to_geom(g), meqn, positive_mask, out, work,
                            (hipStream_t)stream);
 }
@@ -562,7 +570,8 @@ int clawhip_state_sums_noalloc(int dtype, const void* q, const clawhip_geom* g, 
   if (!q || !g || !out) return CLAWHIP_E_ARG;
   if (meqn < 1 || meqn > CLAWHIP_SUMS_NOALLOC_MAX_MEQN) return CLAWHIP_E_ARG;
   if (!geom_ok(g, g->mz > 1 ? 3 : 2) || g->mz < 1) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
+  dtype = storage_dtype(dtype);
   return launch_state_sums_noalloc(dtype, q, to_geom(g), meqn, positive_mask, out,
                                    (hipStream_t)stream);
 }
@@ -570,7 +579,8 @@ int clawhip_state_sums_noalloc(int dtype, const void* q, const clawhip_geom* g, 
 int clawhip_l1_diff(int dtype, const void* a, const void* b, long long n, double* out,
                     double* work, void* stream) {
   if (!a || !b || !out || !work || n < 0) return CLAWHIP_E_ARG;
-  if (dtype != CLAWHIP_F64 && dtype != CLAWHIP_F32K) return CLAWHIP_E_ARG;
+  if (!valid_dtype(dtype)) return CLAWHIP_E_ARG;
+  dtype = storage_dtype(dtype);
   return launch_l1_diff(dtype, a, b, n, out, work, (hipStream_t)stream);
 }
 

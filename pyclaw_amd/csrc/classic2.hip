@@ -9,12 +9,14 @@
 // the two launches 78 + 132 us, because the slices' FP64 instruction count is
 // the bound and the split pays the halo recomputation and the work round trip
 // twice.
+#include "variant.hpp"  // first: selects the arithmetic variant
 #include <stdlib.h>
 
 #include "launch.hpp"
 #include "unsplit.hpp"
 
 namespace claw {
+namespace CLAW_VNS {
 
 template <class RS, typename T, bool KAHAN, bool AUX = false, bool CAPA = false>
 __global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
@@ -47,10 +49,12 @@ __global__ void __launch_bounds__(64) k_step_ctl(StepCtl* ctl) {
   if (threadIdx.x == 0) step_ctl_apply(ctl);
 }
 
+}  // namespace CLAW_VNS
 }  // namespace claw
 
 // ---------------------------------------------------------------- launchers
 namespace claw {
+namespace CLAW_VNS {
 
 namespace {
 // Rows per wavefront (march length H).  Each wavefront recomputes the x-slices of
@@ -151,7 +155,7 @@ int dispatch_unsplit_ctl(int dtype, const clawhip_rp& rp, const Geom& g, const i
                          const int* hi, const int* refl, int ng, const int* method,
                          const int* mthlim, StepCtl* ctl, const CtlBuffers& bufs, int nsteps,
                          hipStream_t stream) {
-  if (dtype == CLAWHIP_F64)
+  if (dtype != CLAWHIP_F32K)
     return run_unsplit_ctl<RSF<double>, double, false>(rp, g, lo, hi, refl, ng, method, mthlim,
                                                        ctl, bufs, nsteps, stream);
   return run_unsplit_ctl<RSF<float>, float, true>(rp, g, lo, hi, refl, ng, method, mthlim, ctl,
@@ -163,7 +167,7 @@ int dispatch_unsplit(int dtype, const clawhip_rp& rp, const void* q_in, const Ge
                      void* q_out, const Geom& gout, const void* comp_in, void* comp_out, const int* lo, const int* hi,
                      const int* refl, int ng, const int* method, const int* mthlim, double dx,
                      double dy, double dt, unsigned long long* cfl_bits, hipStream_t stream) {
-  if (dtype == CLAWHIP_F64)
+  if (dtype != CLAWHIP_F32K)
     return run_unsplit<RSF<double>, double, false>(rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi,
                                                    refl, ng, method, mthlim, dx, dy, dt,
                                                    cfl_bits, stream);
@@ -179,7 +183,7 @@ int dispatch_unsplit_aux(int dtype, const clawhip_rp& rp, const void* q_in, cons
                          unsigned long long* cfl_bits, hipStream_t stream, const AuxIn& aux) {
   (void)rp;
   const bool capa = method[5] > 0;
-  if (dtype == CLAWHIP_F64) {
+  if (dtype != CLAWHIP_F32K) {
     using R = typename RS::template with<double>;
     return capa ? run_unsplit<R, double, false, true, true>(rp, q_in, gin, q_out, gout, comp_in,
                                                             comp_out, lo, hi, refl, ng, method,
@@ -298,4 +302,5 @@ int launch_unsplit_ctl(const clawhip_rp& rp, int dtype, const Geom& g, const int
   return CLAWHIP_E_KIND;
 }
 
+}  // namespace CLAW_VNS
 }  // namespace claw

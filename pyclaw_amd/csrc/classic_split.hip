@@ -13,9 +13,11 @@
 //   step3ds.f90:215-216, 298-299).
 // The 2D split uses flux2's 0.5*cqxx; the 3D split flux3's cqxx with the 0.5
 // inside the sum (flux3.f90:244) -- the HALF_IN_SUM template flag.
+#include "variant.hpp"  // first: selects the arithmetic variant
 #include "launch.hpp"
 
 namespace claw {
+namespace CLAW_VNS {
 
 // flux3.f90:231-259: the 0.5 is applied inside the wave sum, fadd += cqxx.
 template <int ME, int MW, typename T>
@@ -494,7 +496,7 @@ int dispatch_split2(int dtype, const clawhip_rp& rp, const void* q_in, const Geo
                     void* cwork, const int* lo, const int* hi, const int* refl, int ng,
                     const int* method, const int* mthlim, double dx, double dy, double dt,
                     const AuxIn& aux, unsigned long long* cfl, hipStream_t s) {
-  if (dtype == CLAWHIP_F64)
+  if (dtype != CLAWHIP_F32K)
     return run_split2<RSF<double>, double, false>(rp, q_in, gin, q_out, gout, cin, cout, work,
                                                   cwork, lo, hi, refl, ng, method, mthlim, dx,
                                                   dy, dt, aux, cfl, s);
@@ -574,7 +576,7 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
                   int ng, const int* method, const int* mthlim, double dx, double dy,
                   double dz, double dt, unsigned long long* cfl, hipStream_t s) {
   if (rp.kind != CLAWHIP_RP_EULER3) return CLAWHIP_E_KIND;
-  if (dtype == CLAWHIP_F64)
+  if (dtype != CLAWHIP_F32K)
     return run_split3<Euler3<double>, double, false>(rp, q_in, gin, q_out, gout, comp_in,
                                                      comp_out, work, cwork, lo, hi, refl, ng,
                                                      method, mthlim, dx, dy, dz, dt, cfl, s);
@@ -642,4 +644,5 @@ int launch_sweep3(const clawhip_rp& rp, int idir, const void* q_in, const Geom& 
                                    none, cfl, s);
 }
 
+}  // namespace CLAW_VNS
 }  // namespace claw

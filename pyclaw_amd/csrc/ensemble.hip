@@ -9,6 +9,7 @@
 // a member applies that member's accept/reject and next dt (stepctl.hpp).  A
 // step of the whole ensemble is one launch with no host synchronisation, so
 // runs of steps are captured in a HIP graph.
+#include "variant.hpp"  // first: selects the arithmetic variant
 #include <stdlib.h>
 #include <string.h>
 
@@ -19,6 +20,7 @@
 #include "unsplit.hpp"
 
 namespace claw {
+namespace CLAW_VNS {
 
 struct EnsMember {
   int mx, my;
@@ -149,7 +151,7 @@ void ens_launch(const EnsHeader* h, const EnsMember* m, const long long* waves, 
 template <template <typename> class RSF>
 void ens_dispatch(int dtype, const EnsHeader* h, const EnsMember* m, const long long* waves,
                   int nsteps, hipStream_t s) {
-  if (dtype == CLAWHIP_F64) ens_launch<RSF<double>, double, false>(h, m, waves, nsteps, s);
+  if (dtype != CLAWHIP_F32K) ens_launch<RSF<double>, double, false>(h, m, waves, nsteps, s);
   else ens_launch<RSF<float>, float, true>(h, m, waves, nsteps, s);
 }
 
@@ -216,4 +218,5 @@ int ensemble_run(const clawhip_rp& rp, int dtype, const void* table_dev, const l
   return e == hipSuccess ? 0 : (int)e;
 }
 
+}  // namespace CLAW_VNS
 }  // namespace claw

@@ -192,3 +192,13 @@ int launch_l1_diff(int dtype, const void* a, const void* b, long long n, double*
                    double* work, hipStream_t s);
 
 }  // namespace claw
+
+// The launchers of the two arithmetic variants (variant.hpp).
+namespace claw {
+namespace exact {
+#include "variant_decls.inc"
+}  // namespace exact
+namespace fast {
+#include "variant_decls.inc"
+}  // namespace fast
+}  // namespace claw

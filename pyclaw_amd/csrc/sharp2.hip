@@ -17,12 +17,14 @@
 // cell's dq is complete.  With the learned indicator's flags, each interface
 // takes WENO5's linear weights unless its (dilated) flag is set; a flagged
 // interface takes the full WENO5, in the lanes that hold one.
+#include "variant.hpp"  // first: selects the arithmetic variant
 #include <stdlib.h>
 
 #include "launch.hpp"
 #include "sharpctl.hpp"
 
 namespace claw {
+namespace CLAW_VNS {
 
 // The WENO epsilon.  FP64: the reference's REAL(4) literal 1.e-36 widened to
 // double (reconstruct.f90:11, weno.f90:52-57).  FP32 keeps flat stencils away
@@ -1138,7 +1140,7 @@ int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const Geom& g
                    const void* aux_in, const void* comp_in, void* out, void* aux_out,
                    void* comp_out, unsigned long long* cfl, const FlagSet& flags,
                    hipStream_t stream, const AuxIn& ax, const void* src) {
-  if (dtype == CLAWHIP_F64)
+  if (dtype != CLAWHIP_F32K)
     return run_stage<RSF<double>, double, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx,
                                                 dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
                                                 comp_out, cfl, flags, stream, ax, src);
@@ -1194,7 +1196,7 @@ int launch_sharp_stage(const clawhip_rp& rp, int dtype, int char_decomp, const G
 int launch_sharp_ssp33_run(const clawhip_rp& rp, int dtype, int char_decomp, const Geom& g,
                           const int* lo, const int* hi, const int* refl, double dx, double dy,
                           const SharpRunBufs& b, SharpCtl* ctl, int nsteps, hipStream_t stream) {
-  const bool f64 = dtype == CLAWHIP_F64;
+  const bool f64 = dtype != CLAWHIP_F32K;
 #define RUN(RSF, CH)                                                                          \
   (f64 ? run_ssp33<RSF<double>, double, CH>(rp, g, lo, hi, refl, dx, dy, b, ctl, nsteps, stream) \
        : run_ssp33<RSF<float>, float, CH>(rp, g, lo, hi, refl, dx, dy, b, ctl, nsteps, stream))
@@ -1222,7 +1224,7 @@ int launch_lmm_update(int dtype, int mode, long long n, const void* q, const voi
                       const void* R, const void* v, const double* c, void* out,
                       hipStream_t stream) {
   const unsigned blocks = (unsigned)((n + 255) / 256);
-  if (dtype == CLAWHIP_F64)
+  if (dtype != CLAWHIP_F32K)
     hipLaunchKernelGGL(k_lmm_update<double>, dim3(blocks), dim3(256), 0, stream, mode, n,
                        (const double*)q, (const double*)u, (const double*)R, (const double*)v,
                        c[0], c[1], c[2], c[3], c[4], (double*)out);
@@ -1237,7 +1239,7 @@ int launch_lmm_update(int dtype, int mode, long long n, const void* q, const voi
 
 int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t stream) {
   unsigned blocks = (unsigned)((n + 255) / 256);
-  if (dtype == CLAWHIP_F64)
+  if (dtype != CLAWHIP_F32K)
     hipLaunchKernelGGL(k_ssp104_mid<double>, dim3(blocks), dim3(256), 0, stream, (double*)q,
                        (double*)s1, n);
   else
@@ -1247,4 +1249,5 @@ int launch_ssp104_mid(int dtype, void* q, void* s1, long long n, hipStream_t str
   return e == hipSuccess ? 0 : (int)e;
 }
 
+}  // namespace CLAW_VNS
 }  // namespace claw

@@ -56,6 +56,7 @@ __device__ __forceinline__ double py_max(double a, double b) { return b > a ? b 
 // evolve_to_time(solution, tend), then the rescale of the cached CFL by
 // dt/dt_old and dt_old = dt (sharpclaw/solver.py:299-303).
 __device__ __forceinline__ void sharp_get_dt(SharpCtl* c) {
+#pragma clang fp contract(off)   // the host loop's FP64 rounding, in either variant
   double cfl = c->cfl;
   if (c->have_dt_old) {
     if (cfl > 0.0) {
@@ -81,6 +82,7 @@ __device__ __forceinline__ void sharp_get_dt(SharpCtl* c) {
 // The end of an attempt: evolve_to_time's accept/reject (solver.py:616-651,
 // accept_reject_step sharpclaw/solver.py:622-650) and the next attempt's gates.
 __device__ __forceinline__ void sharp_end(SharpCtl* c, bool fallback_ran) {
+#pragma clang fp contract(off)   // the host loop's FP64 rounding, in either variant
   double cfl = py_max(c->cfl, bits_to_double(c->cfl23_bits));
   if (fallback_ran) cfl = py_max(cfl, bits_to_double(c->cflfb_bits));
   c->cfl = cfl;
@@ -124,6 +126,7 @@ __device__ __forceinline__ void sharp_end(SharpCtl* c, bool fallback_ran) {
 // launched after the stage on the same stream: the kernel boundary orders the
 // stage's atomics before these reads, so the stage kernels need no fences).
 __device__ __forceinline__ void sharp_ctl_phase(SharpCtl* c, int phase) {
+#pragma clang fp contract(off)   // the host loop's FP64 rounding, in either variant
   switch (phase) {
     case PH_MID: {
       // set_global_max(0.) then update_global_max(stage 1's CFL)

@@ -37,6 +37,7 @@ enum { CTL_RUNNING = 0, CTL_REACHED_TEND = 1, CTL_CFL_FIXED_DT = 2, CTL_MAX_STEP
 // before this read, so the step kernel needs no fences).  Nothing happens once
 // ctl->done is set (the step kernel did not run either).
 __device__ __forceinline__ void step_ctl_apply(StepCtl* ctl) {
+#pragma clang fp contract(off)   // the host loop's FP64 rounding, in either variant
   if (ctl->done) return;
   const unsigned long long bits = ctl->cfl_bits;
   const double cfl = __longlong_as_double((long long)bits);

@@ -45,6 +45,8 @@ class Ensemble(object):
             s.setup(first)
         s._set_method(first.state)
         self._code = s._dtype_code
+        # the kernels' arithmetic: fp64fast runs the FMA-contracted FP64 variant
+        self._run_code = L.F64_FAST if s._fast else self._code
         lo, hi = s._bc_codes()
         if 0 in lo or 0 in hi:
             raise ValueError("ensembles take built-in boundary conditions only")
@@ -106,7 +108,7 @@ class Ensemble(object):
 
     # ---------------------------------------------------------------- stepping
     def _launch(self, nsteps):
-        rc = L.lib().clawhip_ensemble_run(ctypes.byref(self._rp), self._code,
+        rc = L.lib().clawhip_ensemble_run(ctypes.byref(self._rp), self._run_code,
                                           ctypes.c_void_p(self._table.data_ptr()), self._waves,
                                           int(nsteps), K._stream())
         L.check(rc, "clawhip_ensemble_run")

@@ -18,9 +18,11 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-def _dtype_code(t):
+def _dtype_code(t, fast=False):
+    """The entry point's dtype for an array of t's type: FP64 (fast: the
+    FMA-contracted FP64 variant, CLAWHIP_F64_FAST) or FP32 + Kahan."""
     if t.dtype == torch.float64:
-        return L.F64
+        return L.F64_FAST if fast else L.F64
     if t.dtype == torch.float32:
         return L.F32K
     raise L.ClawhipError(f"unsupported dtype {t.dtype}")
@@ -71,7 +73,7 @@ def _aux_geom(aux, mx, my):
 
 def step2_unsplit(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim,
                   dx, dy, dt, comp_in=None, comp_out=None, num_ghost=2, gin=None, aux=None,
-                  aux_bc_lower=None, aux_bc_upper=None):
+                  aux_bc_lower=None, aux_bc_upper=None, fast=False):
     """One unsplit step q_in -> q_out (planar (meqn, my, mx) tensors).
     aux: planar aux fields of q's dtype, compact or with num_ghost stored ghosts
     (then with aux codes 4 = given), for solvers that read aux; method[5] > 0
@@ -80,7 +82,7 @@ def step2_unsplit(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mth
     gout = L.geom2(mx, my)
     if gin is None:
         gin = gout
-    code = _dtype_code(q_out)
+    code = _dtype_code(q_out, fast)
     assert q_in.dtype == q_out.dtype and q_in.is_contiguous() and q_out.is_contiguous()
     if aux is not None:
         assert aux.dtype == q_out.dtype and aux.is_contiguous()
@@ -107,7 +109,7 @@ def _work_like(q_out, ng, work):
 
 def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim, dx, dy, dt,
                 comp_in=None, comp_out=None, num_ghost=2, gin=None, work=None, aux=None,
-                aux_bc_lower=None, aux_bc_upper=None, whole=False):
+                aux_bc_lower=None, aux_bc_upper=None, whole=False, fast=False):
     """Both dimension-split sweeps (step2ds ids=1, 2); returns the work buffers.
     aux: planar (maux, my, mx) tensor of q's dtype for solvers with aux fields.
     whole: q_in and q_out are whole qbc arrays (num_ghost stored ghosts); q_out
@@ -119,7 +121,7 @@ def step2_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthli
         gout = L.geom2(mx, my, num_ghost)
     if gin is None:
         gin = gout
-    code = _dtype_code(q_out)
+    code = _dtype_code(q_out, fast)
     _check_fields(rp, meqn, aux, method)
     w, cw = work if work is not None else (None, None)
     if not whole:
@@ -159,11 +161,11 @@ def step3_unsplit(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mth
 
 
 def step3_split(rp, q_in, q_out, cfl, bc_lower, bc_upper, reflect, method, mthlim, dx, dy, dz,
-                dt, comp_in=None, comp_out=None, num_ghost=2, work=None):
+                dt, comp_in=None, comp_out=None, num_ghost=2, work=None, fast=False):
     """The three dimension-split sweeps of ClawSolver3D (step3ds idir=1,2,3)."""
     meqn, mz, my, mx = q_out.shape
     g = L.geom3(mx, my, mz)
-    code = _dtype_code(q_out)
+    code = _dtype_code(q_out, fast)
     w, cw = work if work is not None else (None, None)
     w = _work_like(q_out, num_ghost, w)
     if code == L.F32K:
@@ -187,7 +189,8 @@ OUT_DQDT, OUT_SSP33_2, OUT_SSP33_3, OUT_ADD6, OUT_SSP104_FIN, OUT_F32_S2, OUT_F3
 def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx, dy, P,
                  Q=None, s=0.0, y_mode=Y_ARRAY, out_mode=OUT_D, A=None, B=None, aux_in=None,
                  comp_in=None, aux_out=None, comp_out=None, flags=None, flag_list=None,
-                 recon=None, qaux=None, qaux_bc_lower=None, qaux_bc_upper=None, src=None):
+                 recon=None, qaux=None, qaux_bc_lower=None, qaux_bc_upper=None, src=None,
+                 fast=False):
     """D = dt*L(Y) (SharpClaw flux2) fused with one RK stage; planar (meqn, my, mx).
     flags: uint8 troubled-interface flags (mlp_flags) selecting full WENO5 per
     interface (the linear weights elsewhere), with flag_list = (ids, count) from
@@ -212,7 +215,7 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
     gp = (P.shape[2] - mx) // 2
     assert P.shape[1] == my + 2 * gp and P.shape[2] == mx + 2 * gp and P.shape[0] == meqn
     gin = L.geom2(mx, my, gp) if gp else None
-    code = _dtype_code(out)
+    code = _dtype_code(out, fast)
     gaux = _aux_geom(qaux, mx, my)
     if qaux is not None:
         assert qaux.dtype == out.dtype and qaux.is_contiguous()
@@ -232,7 +235,7 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
 
 def sharp2_ssp33_run(rp, char_decomp, bufs, comps, ctl_dev, bc_lower, bc_upper, reflect, dx, dy,
                      dq_dt, r2, aux23, nsteps, indicator=None, dq_dt_full=None, recon=None,
-                     positive=(), stage_events=None):
+                     positive=(), stage_events=None, fast=False):
     """nsteps device-controlled SSP33 attempts (clawhip_sharp2_ssp33_run) on the
     planar ping-pong buffers bufs = (q0, q1) (comps: their Kahan compensations
     or (None, None)); indicator: the learned indicator's path (its flags, list,
@@ -241,7 +244,7 @@ def sharp2_ssp33_run(rp, char_decomp, bufs, comps, ctl_dev, bc_lower, bc_upper, 
     march of stages 1-3 of every attempt (eager runs only)."""
     meqn, my, mx = bufs[0].shape
     g = L.geom2(mx, my)
-    code = _dtype_code(bufs[0])
+    code = _dtype_code(bufs[0], fast)
     fl = lst = cnt = params = None
     flat = thr = 0.0
     mask = 0
@@ -274,28 +277,28 @@ def _event_handles(events, n):
     return arr
 
 
-def lmm_update(mode, q, u, coeffs, out, R=None, v=None):
+def lmm_update(mode, q, u, coeffs, out, R=None, v=None, fast=False):
     """Euler / RK / SSPLMMk2 / SSPLMMk3 / LMM updates (include/clawhip.h,
     clawhip_lmm_update; q is read by modes 0, 1, 2 and 5)."""
     c = (ctypes.c_double * 5)(*[float(x) for x in (list(coeffs) + [0.0] * 5)[:5]])
     for t in (u, R, v, out):
         assert t is None or (t.shape == q.shape and t.dtype == q.dtype and t.is_contiguous())
-    rc = L.lib().clawhip_lmm_update(_dtype_code(q), int(mode), int(q.numel()), _ptr(q), _ptr(u),
+    rc = L.lib().clawhip_lmm_update(_dtype_code(q, fast), int(mode), int(q.numel()), _ptr(q), _ptr(u),
                                     _ptr(R), _ptr(v), c, _ptr(out), _stream())
     L.check(rc, "clawhip_lmm_update")
 
 
-def ssp104_mid(q, s1):
-    rc = L.lib().clawhip_ssp104_mid(_dtype_code(q), _ptr(q), _ptr(s1), int(q.numel()), _stream())
+def ssp104_mid(q, s1, fast=False):
+    rc = L.lib().clawhip_ssp104_mid(_dtype_code(q, fast), _ptr(q), _ptr(s1), int(q.numel()), _stream())
     L.check(rc, "clawhip_ssp104_mid")
 
 
 def step2_unsplit_run(rp, bufs, comps, ctl_dev, bc_lower, bc_upper, reflect, method, mthlim,
-                      dx, dy, nsteps, num_ghost=2):
+                      dx, dy, nsteps, num_ghost=2, fast=False):
     """nsteps device-controlled attempts (clawhip_step2_unsplit_run)."""
     meqn, my, mx = bufs[0].shape
     g = L.geom2(mx, my)
-    code = _dtype_code(bufs[0])
+    code = _dtype_code(bufs[0], fast)
     rc = L.lib().clawhip_step2_unsplit_run(
         ctypes.byref(rp), code, ctypes.byref(g), L.ints(bc_lower), L.ints(bc_upper),
         L.ints(reflect), int(num_ghost), L.ints(method),

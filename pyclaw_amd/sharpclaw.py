@@ -259,7 +259,8 @@ class SharpClawSolver(Solver):
                        self.dt, dx, dy, P, flags=self._flags,
                        flag_list=self.indicator.flag_list if self._flags is not None else None,
                        recon=self.indicator.recon_buffer(out) if self._flags is not None
-                       else None, qaux=qaux, qaux_bc_lower=alo, qaux_bc_upper=ahi, **kw)
+                       else None, qaux=qaux, qaux_bc_lower=alo, qaux_bc_upper=ahi,
+                       fast=self._fast, **kw)
         if now:
             self.cfl.update_global_max(self._read_back(cfl)[0])
 
@@ -415,9 +416,9 @@ class SharpClawSolver(Solver):
                 self._rk_stages(q, comp, self.dq_dt, out_q, dt, cs, method="SSP104")
                 return
             al, be = self.alpha, self.beta
-            K.lmm_update(4, regs[-1], regs[-1], (0.0, al[-1], be[-1] * dt), out_q[0], v=prev[-1])
+            self._lmm(4, regs[-1], regs[-1], (0.0, al[-1], be[-1] * dt), out_q[0], v=prev[-1])
             for i in range(self.lmm_steps - 1):
-                K.lmm_update(5, out_q[0], regs[i], (0.0, al[i], be[i] * dt), out_q[0], v=prev[i])
+                self._lmm(5, out_q[0], regs[i], (0.0, al[i], be[i] * dt), out_q[0], v=prev[i])
             return
         num_steps = self.lmm_steps
         if step_index < num_steps:
@@ -429,7 +430,7 @@ class SharpClawSolver(Solver):
             r = (omega_k_minus_1 - 1.) / omega_k_minus_1
             delta = 1. / omega_k_minus_1 ** 2
             beta = (omega_k_minus_1 + 1.) / omega_k_minus_1
-            K.lmm_update(1, q, self.dq_dt, (beta, r, dt, delta), out_q[0],
+            self._lmm(1, q, self.dq_dt, (beta, r, dt, delta), out_q[0],
                          R=self._registers[-num_steps])
         else:
             omega_k = omega_k_minus_1 + 1.
@@ -437,7 +438,7 @@ class SharpClawSolver(Solver):
             delta0 = (4 * omega_k - omega_k_minus_1 ** 2) / omega_k_minus_1 ** 3
             beta0 = omega_k / omega_k_minus_1 ** 2
             beta_k_minus_1 = omega_k ** 2 / omega_k_minus_1 ** 2
-            K.lmm_update(2, q, self.prev_dq_dt_values[-1],
+            self._lmm(2, q, self.prev_dq_dt_values[-1],
                          (beta_k_minus_1, r, dt, r * beta0 + delta0, beta0 * dt), out_q[0],
                          R=self._registers[-num_steps], v=self.prev_dq_dt_values[-num_steps])
 
@@ -449,7 +450,7 @@ class SharpClawSolver(Solver):
         ti = self.time_integrator if method is None else method
         t0 = self._t0
         if ti == "Euler":
-            K.lmm_update(0, q, dq_dt, (0.0, dt), out_q[0])
+            self._lmm(0, q, dq_dt, (0.0, dt), out_q[0])
         elif ti == "SSP22":
             self._stage(q, out_q[0], q, cfl=cs, stage_t=t0 + dt, Q=dq_dt, s=dt,
                         y_mode=K.Y_AXPY, out_mode=K.OUT_SSP22, A=q)
@@ -458,22 +459,22 @@ class SharpClawSolver(Solver):
             # by term, then registers[i] = dq(stage i); q += b[j]*registers[j]
             a, b = self.a, self.b
             regs = [self._buf("rk%d" % j, q) for j in range(len(b))]
-            K.lmm_update(3, dq_dt, dq_dt, (0.0, dt), regs[0])
+            self._lmm(3, dq_dt, dq_dt, (0.0, dt), regs[0])
             y = self._buf("rk_y", q)
             for i in range(1, len(b)):
                 y.copy_(q)
                 for j in range(i):
-                    K.lmm_update(0, y, regs[j], (0.0, float(a[i][j])), y)
+                    self._lmm(0, y, regs[j], (0.0, float(a[i][j])), y)
                 self._stage(q, regs[i], y, cfl=cs, stage_t=t0 + dt * self.c[i],
                             out_mode=K.OUT_D)
             out_q[0].copy_(q)
             for j in range(len(b)):
-                K.lmm_update(0, out_q[0], regs[j], (0.0, float(b[j])), out_q[0])
+                self._lmm(0, out_q[0], regs[j], (0.0, float(b[j])), out_q[0])
         elif ti == "SSP33":
             r2 = self._buf("r2", q)
             # register times t + dt, then t + 0.5*dt (sharpclaw/solver.py:313-319)
             t2, t3 = t0 + dt, t0 + 0.5 * dt
-            if self.precision == "fp64":
+            if self.precision != "fp32k":
                 self._stage(q, r2, q, cfl=cs, stage_t=t2, Q=dq_dt, s=dt, y_mode=K.Y_AXPY,
                             out_mode=K.OUT_SSP33_2, A=q)
                 self._stage(q, out_q[0], r2, cfl=cs, stage_t=t3, out_mode=K.OUT_SSP33_3, A=q)
@@ -497,7 +498,7 @@ class SharpClawSolver(Solver):
                 ts = ts + dt / 6.
                 cur, nxt = nxt, cur
             qm.copy_(q)
-            K.ssp104_mid(qm, cur)
+            K.ssp104_mid(qm, cur, fast=self._fast)
             ts = t0 + dt / 3.
             for _ in range(4):
                 self._stage(q, nxt, cur, cfl=cs, stage_t=ts, out_mode=K.OUT_ADD6)
@@ -712,6 +713,7 @@ class SharpClawSolver(Solver):
         comps = (bufs[0][1], bufs[1][1])
         args = (self._rp_struct, self.char_decomp, (bufs[0][0], bufs[1][0]), comps, self._ctl,
                 lo, hi, self.reflect_index, dx, dy, dqdt, r2, aux23)
+        kw["fast"] = self._fast
         return bufs, cur, args, kw, dqdt
 
     def capture_graphs(self, solution, nsteps):
@@ -877,6 +879,10 @@ class SharpClawSolver(Solver):
             ent = {"graph": g, "bufs": bufs, "args": args, "kw": kw}
             self._graphs[key] = ent
         return ent["graph"]
+
+    def _lmm(self, *args, **kw):
+        """K.lmm_update in this solver's arithmetic variant."""
+        K.lmm_update(*args, fast=self._fast, **kw)
 
     def dq(self, state):
         """dt*L(q) on the interior as a host array (sharpclaw/solver.py:532-542)."""

@@ -79,8 +79,10 @@ class Solver(object):
         self.num_eqn = None
         self.num_waves = None
         self.fwave = None
-        # Arithmetic of the device state: "fp64" (bitwise the reference's step) or
-        # "fp32k" (FP32 state with Kahan-compensated updates, FP64 CFL).
+        # Arithmetic of the device state: "fp64" (bitwise the reference's step),
+        # "fp64fast" (FP64 with a*b + c contracted to FMAs: within about 1e-13
+        # relative of "fp64" per step) or "fp32k" (FP32 state with
+        # Kahan-compensated updates, FP64 CFL).
         self.precision = "fp64"
         self.compute_gauge_values = default_compute_gauge_values
         self._cfl_dev = None
@@ -121,11 +123,18 @@ class Solver(object):
     # ---- device helpers
     @property
     def _dtype_code(self):
-        if self.precision == "fp64":
+        """The storage type of the device state: 0 FP64 ("fp64", "fp64fast"), 1
+        FP32 + Kahan ("fp32k")."""
+        if self.precision in ("fp64", "fp64fast"):
             return 0
         if self.precision == "fp32k":
             return 1
-        raise ValueError("solver.precision must be 'fp64' or 'fp32k'")
+        raise ValueError("solver.precision must be 'fp64', 'fp64fast' or 'fp32k'")
+
+    @property
+    def _fast(self):
+        """FP64 steps in the FMA-contracted variant (precision "fp64fast")."""
+        return self.precision == "fp64fast"
 
     def _cfl_buffer(self):
         import torch

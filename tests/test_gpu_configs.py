@@ -51,7 +51,7 @@ def _c4_solver(precision):
     return s
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64fast", "fp32k"])
 def test_c4_ensemble_members_match_oracle(precision):
     import pyclaw_amd as pyclaw
     from pyclaw_amd.ensemble import Ensemble
@@ -85,6 +85,10 @@ def test_c4_ensemble_members_match_oracle(precision):
             if precision == "fp64":
                 assert c.t == run.t and c.cfl == run.cfl, (k, attempt)
                 assert np.array_equal(q, run.q), (k, attempt, np.abs(q - run.q).max())
+            elif precision == "fp64fast":   # FMA-contracted FP64: 1e-12 per attempt
+                assert abs(c.t - run.t) <= 1e-12 * run.t, (k, attempt)
+                err = np.abs(q - run.q).max() / np.abs(run.q).max()
+                assert err <= 1e-12 * attempt, (k, attempt, err)
             else:
                 assert abs(c.t - run.t) <= 1e-6 * run.t, (k, attempt)
                 err = np.abs(q - run.q).max() / np.abs(run.q).max()

@@ -35,7 +35,7 @@ def _solution(mx, my, seed):
     return sol
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64fast", "fp32k"])
 def test_ensemble_members_equal_single_runs(precision):
     from pyclaw_amd.ensemble import Ensemble
     sols = [_solution(mx, my, 100 + k) for k, (mx, my) in enumerate(SIZES)]

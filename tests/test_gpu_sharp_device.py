@@ -57,7 +57,7 @@ def _same(a, b):
     assert sa.cfl.get_cached_max() == sb.cfl.get_cached_max()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32k"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64fast", "fp32k"])
 def test_device_loop_equals_host_loop(precision):
     """dt_initial 0.05 makes the first attempt exceed cfl_max: the rejected
     attempt, the retry's get_dt without stage 1 and the graph-replayed chunks
