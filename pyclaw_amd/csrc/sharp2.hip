@@ -717,8 +717,8 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   };
   // x-slice of row r (aux Ar of this lane's cell): interface c (left of this
   // lane's cell) -> qlR, qrL, am, ap
-  auto x_interface = [&](const T (&row)[ME], const T (&Ar)[MA], int r, T (&qrL)[ME],
-                         T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
+  auto x_interface = [&](const T (&row)[ME], const T (&Ar)[MA], int r, bool flagged,
+                         T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
     // st[k] = the row value of lane l + k - 3 (chained one-lane moves)
     T st[6][ME];
 #pragma unroll
@@ -729,12 +729,12 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     from_right<ME>(st[3], st[4]);
     from_right<ME>(st[4], st[5]);
     // flagged path: the linear weights, or, where the interface's dilated flag
-    // is set, the full WENO5 edge values k_weno_flagged stored (interfaces of
-    // ghost rows enter only the CFL and take the linear weights)
+    // is set (`flagged`, from the flag bytes loaded a row ahead), the full WENO5
+    // edge values k_weno_flagged stored (interfaces of ghost rows enter only the
+    // CFL and take the linear weights)
     if (FLAGS) {
       recon_linear<ME>(st, qrL, qlR);
-      if (c >= 1 && c <= mx + 1 && r >= 1 && r <= my && xflag(a.flags, mx, c, r))
-        load_recon<ME>(a.recon + xface_id(mx, c, r) * (2 * ME), qrL, qlR);
+      if (flagged) load_recon<ME>(a.recon + xface_id(mx, c, r) * (2 * ME), qrL, qlR);
     } else {
       recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
     }
@@ -751,11 +751,10 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   // y-interface t (rows t-1 | t, aux Alo | Ahi) of this lane's column from the
   // window win[0..5] = rows t-3..t+2
   auto y_interface = [&](const T (&win)[6][ME], const T (&Alo)[MA], const T (&Ahi)[MA], int t,
-                         T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
+                         bool flagged, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
     if (FLAGS) {
       recon_linear<ME>(win, qrL, qlR);
-      if (c >= 1 && c <= mx && t <= my + 1 && yflag(a.flags, mx, my, c, t))
-        load_recon<ME>(a.recon + yface_id(mx, my, c, t) * (2 * ME), qrL, qlR);
+      if (flagged) load_recon<ME>(a.recon + yface_id(mx, my, c, t) * (2 * ME), qrL, qlR);
     } else {
       recon_interface<RS, 2, T, CHAR>(rp, win, qrL, qlR);
     }
@@ -768,9 +767,9 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     }
   };
   // x contribution of this lane's cell in row r (a whole x-slice pass)
-  auto x_cell = [&](const T (&row)[ME], const T (&Ar)[MA], int r, T (&dqx)[ME]) {
+  auto x_cell = [&](const T (&row)[ME], const T (&Ar)[MA], int r, bool flagged, T (&dqx)[ME]) {
     T qrL[ME], qlR[ME], am[ME], ap[ME];
-    x_interface(row, Ar, r, qrL, qlR, am, ap);
+    x_interface(row, Ar, r, flagged, qrL, qlR, am, ap);
     T qr[ME], amn[ME];
     from_right<ME>(qrL, qr);     // right edge of this cell: interface c+1
     from_right<ME>(am, amn);     // amdq at interface c+1
@@ -799,13 +798,39 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   load_a(r0, Ahi);
   if (r0 == 1) {   // x-slice of row 0 (CFL only)
     T d[ME];
-    x_cell(win[2], Alo, 0, d);
+    x_cell(win[2], Alo, 0, false, d);
   }
+  // FLAGS: this lane's flag bytes, loaded one row iteration before their use so
+  // that their latency hides behind a row's arithmetic.  xbyte(r): the flag of
+  // x-interface c of row r (the column clamped to 1..mx+1, so the neighbours that
+  // xflag clamps come from the lanes beside this one); ybyte(k): the flag of
+  // y-interface c in y-row k (0-based, clamped to 0..my as yflag clamps).
+  auto xbyte = [&](int r) -> int {
+    if (!FLAGS || r < 1 || r > my) return 0;
+    return a.flags[(long long)(r - 1) * (mx + 1) + min(max(c - 1, 0), mx)];
+  };
+  auto ybyte = [&](int k) -> int {
+    if (!FLAGS || c < 1 || c > mx) return 0;
+    return a.flags[(long long)(mx + 1) * my + (long long)min(max(k, 0), my) * mx + (c - 1)];
+  };
+  // x-interface c of a row is listed when its byte or a row neighbour's is set
+  // (the cross-lane moves run on every lane, before any lane-dependent test)
+  auto xlisted = [&](int b) {
+    const int nb = b | from_left(b) | from_right(b);
+    return FLAGS && c >= 1 && c <= mx + 1 && nb != 0;
+  };
+  // y-interface t: y-rows t-2, t-1, t (yflag's dilation)
+  int ym2 = ybyte(r0 - 2), ym1 = ybyte(r0 - 1), y0 = ybyte(r0);
   T qlY[ME], apY[ME];   // interface t-1 carried: lower edge of row t-1, apdq
   {
     T qrL[ME], am[ME];
-    y_interface(win, Alo, Ahi, r0, qrL, qlY, am, apY);
+    y_interface(win, Alo, Ahi, r0, (ym2 | ym1 | y0) != 0, qrL, qlY, am, apY);
   }
+  // the bytes of the first loop iteration (y-interface r0+1, x-row r0)
+  ym2 = ym1;
+  ym1 = y0;
+  y0 = ybyte(r0 + 1);
+  int xb = xbyte(r0);
   T pf_p[ME], pf_q[ME];   // the loads of the next iteration's window row
 #pragma unroll
   for (int m = 0; m < ME; m++) pf_q[m] = T(0);
@@ -825,9 +850,12 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
 #pragma unroll
     for (int k = 0; k < MA; k++) Alo[k] = Ahi[k];
     load_a(t, Ahi);
+    // the flag bytes of the next iteration (y-row t+1, x-row t)
+    const int y_next = t <= rend ? ybyte(t + 1) : 0;
+    const int x_next = t <= rend ? xbyte(t) : 0;
     // y-interface t closes row t-1
     T qrL[ME], qlR[ME], am[ME], ap[ME];
-    y_interface(win, Alo, Ahi, t, qrL, qlR, am, ap);
+    y_interface(win, Alo, Ahi, t, (ym2 | ym1 | y0) != 0, qrL, qlR, am, ap);
     T dqy[ME];
     if constexpr (TFLUCT) {
       T fl[ME], fr[ME];
@@ -845,7 +873,7 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     }
     // x-slice of row t-1
     T dqx[ME];
-    x_cell(win[2], Alo, t - 1, dqx);
+    x_cell(win[2], Alo, t - 1, xlisted(xb), dqx);
     if (out_lane) {
       bool bad = false;
 #pragma unroll
@@ -857,10 +885,14 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     }
 #pragma unroll
     for (int m = 0; m < ME; m++) { qlY[m] = qlR[m]; apY[m] = ap[m]; }
+    ym2 = ym1;
+    ym1 = y0;
+    y0 = y_next;
+    xb = x_next;
   }
   if (rend == my) {   // x-slice of row my+1 (CFL only) = window row index 3 now
     T d[ME];
-    x_cell(win[3], Ahi, my + 1, d);
+    x_cell(win[3], Ahi, my + 1, false, d);
   }
   double cx = (double)v.dtdx * (double)smax_x;
   double cy = (double)v.dtdy * (double)smax_y;
