@@ -25,10 +25,22 @@
 //   shallow_bathymetry_fwave_2D  f-waves, 3 waves at the Einfeldt speeds, bathymetry in aux
 // Every expression keeps the reference's evaluation order; the library is
 // compiled with -ffp-contract=off so FP64 results match the CPU oracle bitwise.
+// In the contracted (fast) build of the kernel sources (variant.hpp) the Euler
+// solver also shares its reciprocals: one division per density, per Roe
+// denominator and per sound speed squared, multiplications elsewhere
+// (kSharedReciprocals).
 #pragma once
 #include <hip/hip_runtime.h>
 
+#ifndef CLAW_FAST
+#define CLAW_FAST 0
+#endif
+
 namespace claw {
+
+// The fast variant's solvers divide once per shared denominator (internal
+// linkage: each translation unit sees its own variant's value).
+constexpr bool kSharedReciprocals = CLAW_FAST != 0;
 
 template <typename T>
 struct RPParams {
@@ -103,13 +115,46 @@ struct EulerT {
   template <int IXY>
   __device__ static __forceinline__ void flux(const RPParams<T>& p, const T* q, T (&f)[meqn]) {
     constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
-    const T un = q[mu] / q[0];
-    const T pr = p.gamma1 * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    T un, pr;
+    if constexpr (kSharedReciprocals) {
+      const T ir = T(1) / q[0];
+      un = q[mu] * ir;
+      pr = p.gamma1 * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * ir);
+    } else {
+      un = q[mu] / q[0];
+      pr = p.gamma1 * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    }
     f[0] = q[mu];
     f[mu] = q[mu] * un + pr;
     f[mv] = q[mv] * un;
     f[3] = (q[3] + pr) * un;
     if constexpr (TR) f[4] = q[4] * un;
+  }
+
+  // The fast variant's Roe average: 1/rho per side (also 1/sqrt(rho) =
+  // sqrt(rho)/rho), 1/(sqrt(rhoL) + sqrt(rhoR)) and 1/a^2 once each.
+  template <int IXY>
+  __device__ static __forceinline__ Roe roe_avg_shared(T gamma1, const T* qL, const T* qR,
+                                                       T& irl, T& irr, T& ia2) {
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    Roe r;
+    irl = T(1) / qL[0];
+    irr = T(1) / qR[0];
+    const T rhsqrtl = sqrt(qL[0]), rhsqrtr = sqrt(qR[0]);
+    const T isl = rhsqrtl * irl, isr = rhsqrtr * irr;
+    const T pl = gamma1 * (qL[3] - T(0.5) * (qL[1] * qL[1] + qL[2] * qL[2]) * irl);
+    const T pr = gamma1 * (qR[3] - T(0.5) * (qR[1] * qR[1] + qR[2] * qR[2]) * irr);
+    const T irq = T(1) / (rhsqrtl + rhsqrtr);
+    r.u = (qL[mu] * isl + qR[mu] * isr) * irq;
+    r.v = (qL[mv] * isl + qR[mv] * isr) * irq;
+    r.enth = ((qL[3] + pl) * isl + (qR[3] + pr) * isr) * irq;
+    r.u2v2 = r.u * r.u + r.v * r.v;
+    const T a2 = gamma1 * (r.enth - T(.5) * r.u2v2);
+    r.a = sqrt(a2);
+    ia2 = T(1) / a2;
+    r.g1a2 = gamma1 * ia2;
+    r.euv = r.enth - r.u2v2;
+    return r;
   }
 
   template <int IXY>
@@ -138,11 +183,17 @@ struct EulerT {
                                              T (&amdq)[meqn], T (&apdq)[meqn], Roe& r) {
     constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
     const T gamma = p.gamma, gamma1 = p.gamma1;
-    r = roe_avg<IXY>(gamma1, qL, qR);
+    T irl = T(0), irr = T(0), ia2 = T(0);   // shared reciprocals (fast variant)
+    if constexpr (kSharedReciprocals) r = roe_avg_shared<IXY>(gamma1, qL, qR, irl, irr, ia2);
+    else r = roe_avg<IXY>(gamma1, qL, qR);
     T d0 = qR[0] - qL[0], d1 = qR[mu] - qL[mu], d2 = qR[mv] - qL[mv], d3 = qR[3] - qL[3];
     T a3 = r.g1a2 * (r.euv * d0 + r.u * d1 + r.v * d2 - d3);
     T a2 = d2 - r.v * d0;
-    T a4 = (d1 + (r.a - r.u) * d0 - r.a * a3) / (T(2) * r.a);
+    T a4;
+    if constexpr (kSharedReciprocals)   // 1/(2a) = a/(2a^2)
+      a4 = (d1 + (r.a - r.u) * d0 - r.a * a3) * (T(0.5) * r.a * ia2);
+    else
+      a4 = (d1 + (r.a - r.u) * d0 - r.a * a3) / (T(2) * r.a);
     T a1 = d0 - a3 - a4;
     wave[0][0] = a1; wave[0][mu] = a1 * (r.u - r.a); wave[0][mv] = a1 * r.v;
     wave[0][3] = a1 * (r.enth - r.u * r.a); s[0] = r.u - r.a;
@@ -171,9 +222,16 @@ struct EulerT {
     }
     do {
       T rhoim1 = qL[0];
-      T pim1 = gamma1 * (qL[3] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]) / rhoim1);
-      T cim1 = sqrt(gamma * pim1 / rhoim1);
-      T s0 = qL[mu] / rhoim1 - cim1;
+      T pim1, cim1, s0;
+      if constexpr (kSharedReciprocals) {
+        pim1 = gamma1 * (qL[3] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]) * irl);
+        cim1 = sqrt(gamma * pim1 * irl);
+        s0 = qL[mu] * irl - cim1;
+      } else {
+        pim1 = gamma1 * (qL[3] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv]) / rhoim1);
+        cim1 = sqrt(gamma * pim1 / rhoim1);
+        s0 = qL[mu] / rhoim1 - cim1;
+      }
       if (s0 >= T(0) && s[0] > T(0)) {
 #pragma unroll
         for (int m = 0; m < meqn; m++) amdq[m] = T(0);
@@ -183,9 +241,17 @@ struct EulerT {
       T rhou1 = qL[mu] + wave[0][mu];
       T rhov1 = qL[mv] + wave[0][mv];
       T en1 = qL[3] + wave[0][3];
-      T p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
-      T c1 = sqrt(gamma * p1 / rho1);
-      T s1 = rhou1 / rho1 - c1;
+      T p1, c1, s1;
+      if constexpr (kSharedReciprocals) {
+        const T ir1 = T(1) / rho1;
+        p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) * ir1);
+        c1 = sqrt(gamma * p1 * ir1);
+        s1 = rhou1 * ir1 - c1;
+      } else {
+        p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+        c1 = sqrt(gamma * p1 / rho1);
+        s1 = rhou1 / rho1 - c1;
+      }
       T sfract;
       if (s0 < T(0) && s1 > T(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
       else if (s[0] < T(0)) sfract = s[0];
@@ -200,16 +266,31 @@ struct EulerT {
         if constexpr (TR) amdq[m] = amdq[m] + s[4] * wave[4][m];
       }
       T rhoi = qR[0];
-      T pi = gamma1 * (qR[3] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
-      T ci = sqrt(gamma * pi / rhoi);
-      T s3 = qR[mu] / rhoi + ci;
+      T pi, ci, s3;
+      if constexpr (kSharedReciprocals) {
+        pi = gamma1 * (qR[3] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) * irr);
+        ci = sqrt(gamma * pi * irr);
+        s3 = qR[mu] * irr + ci;
+      } else {
+        pi = gamma1 * (qR[3] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv]) / rhoi);
+        ci = sqrt(gamma * pi / rhoi);
+        s3 = qR[mu] / rhoi + ci;
+      }
       T rho2 = qR[0] - wave[3][0];
       T rhou2 = qR[mu] - wave[3][mu];
       T rhov2 = qR[mv] - wave[3][mv];
       T en2 = qR[3] - wave[3][3];
-      T p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
-      T c2 = sqrt(gamma * p2 / rho2);
-      T s2 = rhou2 / rho2 + c2;
+      T p2, c2, s2;
+      if constexpr (kSharedReciprocals) {
+        const T ir2 = T(1) / rho2;
+        p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) * ir2);
+        c2 = sqrt(gamma * p2 * ir2);
+        s2 = rhou2 * ir2 + c2;
+      } else {
+        p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+        c2 = sqrt(gamma * p2 / rho2);
+        s2 = rhou2 / rho2 + c2;
+      }
       if (s2 < T(0) && s3 > T(0)) sfract = s2 * (s3 - s[3]) / (s3 - s2);
       else if (s[3] < T(0)) sfract = s[3];
       else break;
