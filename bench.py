@@ -576,6 +576,83 @@ def sharp_cpu_baseline(sample_steps=2, n=SHARP_N, seed=3):
                       f"{wall:.1f} s"}
 
 
+def _sharp_cpu_strip(rows, sample_steps, seed, n=SHARP_N):
+    """One worker of the all-cores CPU baseline: the reference SharpClaw SSP33 step
+    (as sharp_cpu_baseline) on an n x rows periodic strip of the shallow-water
+    family; returns the compute wall time."""
+    from oracle import oracle as O
+    from pyclaw_amd import indicator as I
+    flux = O.ref_sharp_flux2 if O.ref_available() else O.sharp_flux2
+    rp = O.make_rp(O.SHALLOW, efix=1, grav=1.0)
+    q = I.swe_problem(seed, n, rows)
+    dx, dy = 1.0 / n, 1.0 / rows
+    dt = 0.45 / n / 1.6
+    ng = 3
+
+    def dq(y):
+        ybc = O.with_ghosts(y, ng)
+        O.bc2(ybc, ng, [2, 2], [2, 2])
+        d, c = flux(rp, ybc, ng, dx, dy, dt, 2)
+        return O.interior(d, ng), c
+
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        d1, c1 = dq(q)
+        dt_new = dt * 0.45 / c1
+        d1 = d1 * (dt_new / dt)
+        dt = dt_new
+        r = q + d1
+        d2, _ = dq(r)
+        r = 0.75 * q + 0.25 * (r + d2)
+        d3, _ = dq(r)
+        q = 1. / 3. * q + 2. / 3. * (r + d3)
+    return time.perf_counter() - t0
+
+
+def sharp_cpu_baseline_parallel(workers=16, sample_steps=4, n=SHARP_N, seed=3):
+    """The same reference SharpClaw step on every core of the GPU box's CPU share:
+    `workers` processes (child processes of this one, started before they are
+    timed), each on an n x (n / workers) periodic strip, so that together they
+    advance n^2 cells; they start together (a line on stdin) and the rate is the
+    total over the slowest worker's wall time."""
+    import subprocess
+    rows = n // workers
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                               str(rows), str(sample_steps), str(seed + k), str(n)],
+                              stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+             for k in range(workers)]
+    for p in procs:
+        assert p.stdout.readline().strip() == "ready"
+    for p in procs:
+        p.stdin.write("go\n")
+        p.stdin.flush()
+    walls = []
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError("CPU baseline worker failed")
+        walls.append(float(out.strip().splitlines()[-1]))
+    wall = max(walls)
+    from oracle import oracle as O
+    return {"value": n * rows * workers * sample_steps / wall, "unit": "cell-updates/s",
+            "cores": workers, "kind": "reference" if O.ref_available() else "port",
+            "sample": f"{workers} processes, each {sample_steps} SSP33 steps of an {n}x{rows} "
+                      f"periodic strip of the shallow-water family ({n}x{rows * workers} cells "
+                      f"together; the reference's sharpclaw/flux2.f90 characteristic WENO5 "
+                      f"compiled with amdflang -O2), slowest worker {wall:.1f} s"}
+
+
+def _cpu_worker(argv):
+    rows, steps, seed = int(argv[0]), int(argv[1]), int(argv[2])
+    # imports and the reference build's loading happen before the start line
+    from oracle import oracle as O
+    from pyclaw_amd import indicator  # noqa: F401 (the problem generator)
+    O.ref_available()
+    print("ready", flush=True)
+    sys.stdin.readline()
+    print(f"{_sharp_cpu_strip(rows, steps, seed, n=int(argv[3])):.6f}", flush=True)
+
+
 def classic_leg(steps=400, warmup=16):
     """The classic unsplit 2D Euler step on 1024^2 (FP64 and FP32 + Kahan),
     device-controlled, graph captured before the timed region."""
@@ -614,6 +691,9 @@ def classic_leg(steps=400, warmup=16):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
+        _cpu_worker(sys.argv[2:])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -695,6 +775,10 @@ def main():
         print(f"[bench] {name} done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         rec["cpu_baseline"] = sharp_cpu_baseline()
+        try:
+            rec["cpu_baseline_all_cores"] = sharp_cpu_baseline_parallel()
+        except Exception as e:   # never hides the headline line
+            rec["cpu_baseline_all_cores"] = {"error": repr(e)}
     if args.calibrate_copy:
         import torch
         a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
