@@ -442,16 +442,12 @@ __device__ __forceinline__ long long yface_id(int mx, int my, int c, int t) {
   return (long long)(mx + 1) * my + (long long)(t - 1) * mx + (c - 1);
 }
 
-// Flags of the interfaces (dilated by one interface along the normal).
-__device__ __forceinline__ bool xflag(const unsigned char* f, int mx, int c, int r) {
-  const unsigned char* fr = f + (long long)(r - 1) * (mx + 1);
-  return (fr[c - 1] | fr[max(c - 2, 0)] | fr[min(c, mx)]) != 0;
-}
-__device__ __forceinline__ bool yflag(const unsigned char* f, int mx, int my, int c, int t) {
-  const unsigned char* fc = f + (long long)(mx + 1) * my + (c - 1);
-  return (fc[(long long)(t - 1) * mx] | fc[(long long)max(t - 2, 0) * mx] |
-          fc[(long long)min(t, my) * mx]) != 0;
-}
+// An interface is listed (reconstructed with the full WENO5) when its flag or
+// the flag of a neighbour along its normal is set: x-interface c of row r when
+// one of x-interfaces c-1, c, c+1 of that row is (clamped to 1..mx+1), y-interface
+// t of column c when one of y-interfaces t-1, t, t+1 of that column is (clamped
+// to 1..my+1).  k_flag_list builds the list, the flagged march tests the same
+// rule on flag bytes it loads a row ahead.
 
 template <int ME, typename T>
 __device__ __forceinline__ void load_recon(const T* p, T (&qrL)[ME], T (&qlR)[ME]) {
@@ -463,7 +459,7 @@ __device__ __forceinline__ void load_recon(const T* p, T (&qrL)[ME], T (&qlR)[ME
 }
 
 // ---------------------------------------------------------------- flagged interfaces
-// The ids of the interfaces whose dilated flag is set (xflag / yflag), appended
+// The ids of the listed interfaces (the dilation rule above), appended
 // to `list`; the order is free, each entry is computed on its own.  A thread
 // takes 4 consecutive ids (one aligned dword of flags, its neighbours as
 // byte-aligned dwords: the x-neighbours at +-1, the y-neighbours at +-mx), a
@@ -802,9 +798,9 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   }
   // FLAGS: this lane's flag bytes, loaded one row iteration before their use so
   // that their latency hides behind a row's arithmetic.  xbyte(r): the flag of
-  // x-interface c of row r (the column clamped to 1..mx+1, so the neighbours that
-  // xflag clamps come from the lanes beside this one); ybyte(k): the flag of
-  // y-interface c in y-row k (0-based, clamped to 0..my as yflag clamps).
+  // x-interface c of row r (the column clamped to 1..mx+1, so the clamped row
+  // neighbours come from the lanes beside this one); ybyte(k): the flag of
+  // y-interface c in y-row k (0-based, clamped to 0..my).
   auto xbyte = [&](int r) -> int {
     if (!FLAGS || r < 1 || r > my) return 0;
     return a.flags[(long long)(r - 1) * (mx + 1) + min(max(c - 1, 0), mx)];
@@ -819,7 +815,7 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
     const int nb = b | from_left(b) | from_right(b);
     return FLAGS && c >= 1 && c <= mx + 1 && nb != 0;
   };
-  // y-interface t: y-rows t-2, t-1, t (yflag's dilation)
+  // y-interface t: 0-based y-rows t-2, t-1, t (y-interfaces t-1, t, t+1)
   int ym2 = ybyte(r0 - 2), ym1 = ybyte(r0 - 1), y0 = ybyte(r0);
   T qlY[ME], apY[ME];   // interface t-1 carried: lower edge of row t-1, apdq
   {
