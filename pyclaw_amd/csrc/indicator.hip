@@ -685,19 +685,20 @@ __global__ void __launch_bounds__(256) k_mlp_flags_row(const T* q, Geom g, BCSpe
     }
     bool rough[ME];
     u32x2 lo[ME], hi[ME];
-    bool any = false;
 #pragma unroll
     for (int m = 0; m < ME; m++) {
       float f[6];
       rough[m] = norm_stencil(v[m], flat, f) > flat;
       lo[m] = u32x2{pack_bf16(f[0], f[1]), pack_bf16(f[2], f[3])};
       hi[m] = u32x2{pack_bf16(f[4], f[5]), 0x3f80u};   // bias input 1.0, then 0
-      any = any || rough[m];
     }
     bool hit = false;
-    if (__ballot(any) != 0ull) {
+    // a component's network runs only when one of the pass's 64 samples of it
+    // is rough (the others cannot set a flag): smooth fields, as the momenta
+    // away from moving fronts, skip their evaluation
 #pragma unroll
-      for (int m = 0; m < ME; m++) {
+    for (int m = 0; m < ME; m++) {
+      if (__ballot(rough[m]) != 0ull) {
         const float z = mlp_forward_regs(reg, lo[m], hi[m]);
         hit = hit || (rough[m] && z > zthr);
       }
