@@ -234,21 +234,27 @@ def test_api_custom_bc_matches_builtin():
     assert np.array_equal(a.frames[-1].state.q, b.frames[-1].state.q)
 
 
-@pytest.mark.parametrize("fixed_dt", [False, True])
-def test_device_loop_equals_host_loop(fixed_dt):
+@pytest.mark.parametrize("fixed_dt,shape,tfinal", [(False, (80, 60), 0.15),
+                                                   (True, (80, 60), 0.15),
+                                                   (False, (600, 480), 0.03)])
+def test_device_loop_equals_host_loop(fixed_dt, shape, tfinal):
     """The device-controlled loop (clawhip_step2_unsplit_run, graph replays of 16
     attempts plus a remainder) takes the same steps as the host loop: same
-    state bit for bit, same t, step count and status."""
+    state bit for bit, same t, step count and status, on a grid of a few
+    workgroups and on one of a few hundred."""
     from pyclaw_amd import riemann
+    print("classic device loop", fixed_dt, shape, flush=True)
     rng = np.random.default_rng(21)
-    q = seeded_state(rng, O.EULER, (80, 60))
+    q = seeded_state(rng, O.EULER, shape)
+    if shape[0] > 100:   # milder cell-to-cell noise: the wide grid's run stays finite
+        q = 0.8 * q.mean(axis=(1, 2))[:, None, None] + 0.2 * q
     opts = dict(problem_data={"gamma": 1.4}, dimensional_split=False, transverse_waves=2,
                 limiters=4, dt_initial=0.02)
     if fixed_dt:
         opts.update(dt_variable=False, dt_initial=0.0005)
     runs = []
     for extra in (dict(device_loop=False), dict(use_graph=False), dict()):
-        claw = _run_api(riemann.euler_4wave_2D, q, ([0, 0], [1, 1]), [1, 3], [1, 3], 0.15,
+        claw = _run_api(riemann.euler_4wave_2D, q, ([0, 0], [1, 1]), [1, 3], [1, 3], tfinal,
                         nout=3, **opts, **extra)
         runs.append(claw)
     ref = runs[0]
@@ -257,9 +263,11 @@ def test_device_loop_equals_host_loop(fixed_dt):
         assert claw.solution.t == ref.solution.t
         for key in ("numsteps", "cflmax", "dtmin", "dtmax"):
             assert claw.solver.status[key] == ref.solver.status[key], key
-        for fa, fb in zip(claw.frames, ref.frames):
+        for k, (fa, fb) in enumerate(zip(claw.frames, ref.frames)):
             assert fa.t == fb.t
-            assert np.array_equal(fa.state.q, fb.state.q)
+            assert np.isfinite(fb.state.q).all(), k
+            d = np.abs(fa.state.q - fb.state.q)
+            assert np.array_equal(fa.state.q, fb.state.q), (k, d.max(), np.argwhere(d > 0)[:4])
 
 
 @pytest.mark.parametrize("module", ["acoustics_2D", "acoustics_2D_ptwise"])
