@@ -158,8 +158,13 @@ int dispatch_unsplit_ctl(int dtype, const clawhip_rp& rp, const Geom& g, const i
   if (dtype != CLAWHIP_F32K)
     return run_unsplit_ctl<RSF<double>, double, false>(rp, g, lo, hi, refl, ng, method, mthlim,
                                                        ctl, bufs, nsteps, stream);
+#if CLAW_FAST
   return run_unsplit_ctl<RSF<float>, float, true>(rp, g, lo, hi, refl, ng, method, mthlim, ctl,
                                                   bufs, nsteps, stream);
+#else
+  // FP32 + Kahan is dispatched to the fast variant only
+  return CLAWHIP_E_ARG;
+#endif
 }
 
 template <template <typename> class RSF>
@@ -171,8 +176,13 @@ int dispatch_unsplit(int dtype, const clawhip_rp& rp, const void* q_in, const Ge
     return run_unsplit<RSF<double>, double, false>(rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi,
                                                    refl, ng, method, mthlim, dx, dy, dt,
                                                    cfl_bits, stream);
+#if CLAW_FAST
   return run_unsplit<RSF<float>, float, true>(rp, q_in, gin, q_out, gout, comp_in, comp_out, lo, hi, refl,
                                               ng, method, mthlim, dx, dy, dt, cfl_bits, stream);
+#else
+  // FP32 + Kahan is dispatched to the fast variant only
+  return CLAWHIP_E_ARG;
+#endif
 }
 // Solvers with aux fields: the aux window, with or without a capacity field.
 template <class RS>
@@ -194,6 +204,7 @@ int dispatch_unsplit_aux(int dtype, const clawhip_rp& rp, const void* q_in, cons
                                                              mthlim, dx, dy, dt, cfl_bits, stream,
                                                              &aux);
   }
+#if CLAW_FAST
   using R = typename RS::template with<float>;
   return capa ? run_unsplit<R, float, true, true, true>(rp, q_in, gin, q_out, gout, comp_in,
                                                         comp_out, lo, hi, refl, ng, method, mthlim,
@@ -202,6 +213,10 @@ int dispatch_unsplit_aux(int dtype, const clawhip_rp& rp, const void* q_in, cons
                                                          comp_out, lo, hi, refl, ng, method,
                                                          mthlim, dx, dy, dt, cfl_bits, stream,
                                                          &aux);
+#else
+  // FP32 + Kahan is dispatched to the fast variant only
+  return CLAWHIP_E_ARG;
+#endif
 }
 
 template <int ME>

@@ -500,9 +500,14 @@ int dispatch_split2(int dtype, const clawhip_rp& rp, const void* q_in, const Geo
     return run_split2<RSF<double>, double, false>(rp, q_in, gin, q_out, gout, cin, cout, work,
                                                   cwork, lo, hi, refl, ng, method, mthlim, dx,
                                                   dy, dt, aux, cfl, s);
+#if CLAW_FAST
   return run_split2<RSF<float>, float, true>(rp, q_in, gin, q_out, gout, cin, cout, work, cwork,
                                              lo, hi, refl, ng, method, mthlim, dx, dy, dt, aux,
                                              cfl, s);
+#else
+  // FP32 + Kahan is dispatched to the fast variant only
+  return CLAWHIP_E_ARG;
+#endif
 }
 }  // namespace
 
@@ -580,9 +585,14 @@ int launch_split3(const clawhip_rp& rp, int dtype, const void* q_in, const Geom&
     return run_split3<Euler3<double>, double, false>(rp, q_in, gin, q_out, gout, comp_in,
                                                      comp_out, work, cwork, lo, hi, refl, ng,
                                                      method, mthlim, dx, dy, dz, dt, cfl, s);
+#if CLAW_FAST
   return run_split3<Euler3<float>, float, true>(rp, q_in, gin, q_out, gout, comp_in, comp_out,
                                                 work, cwork, lo, hi, refl, ng, method, mthlim,
                                                 dx, dy, dz, dt, cfl, s);
+#else
+  // FP32 + Kahan is dispatched to the fast variant only
+  return CLAWHIP_E_ARG;
+#endif
 }
 
 int launch_sweep2(const clawhip_rp& rp, int ids, const void* q_in, const Geom& gin, void* q_out,

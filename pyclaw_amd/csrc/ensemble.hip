@@ -152,7 +152,9 @@ template <template <typename> class RSF>
 void ens_dispatch(int dtype, const EnsHeader* h, const EnsMember* m, const long long* waves,
                   int nsteps, hipStream_t s) {
   if (dtype != CLAWHIP_F32K) ens_launch<RSF<double>, double, false>(h, m, waves, nsteps, s);
+#if CLAW_FAST
   else ens_launch<RSF<float>, float, true>(h, m, waves, nsteps, s);
+#endif
 }
 
 }  // namespace

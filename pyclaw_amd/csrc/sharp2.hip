@@ -1172,9 +1172,14 @@ int dispatch_stage(int dtype, const clawhip_rp& rp, const Geom& g, const Geom& g
     return run_stage<RSF<double>, double, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx,
                                                 dy, P, Q, A, B, aux_in, comp_in, out, aux_out,
                                                 comp_out, cfl, flags, stream, ax, src);
+#if CLAW_FAST
   return run_stage<RSF<float>, float, CHAR>(rp, g, gin, lo, hi, refl, ymode, omode, dt, s, dx, dy, P,
                                             Q, A, B, aux_in, comp_in, out, aux_out, comp_out,
                                             cfl, flags, stream, ax, src);
+#else
+  // FP32 + Kahan is dispatched to the fast variant only
+  return CLAWHIP_E_ARG;
+#endif
 }
 }  // namespace
 
@@ -1227,7 +1232,12 @@ int launch_sharp_ssp33_run(const clawhip_rp& rp, int dtype, int char_decomp, con
   const bool f64 = dtype != CLAWHIP_F32K;
 #define RUN(RSF, CH)                                                                          \
   (f64 ? run_ssp33<RSF<double>, double, CH>(rp, g, lo, hi, refl, dx, dy, b, ctl, nsteps, stream) \
-       : run_ssp33<RSF<float>, float, CH>(rp, g, lo, hi, refl, dx, dy, b, ctl, nsteps, stream))
+       : RUN_F32(RSF, CH))
+#if CLAW_FAST
+#define RUN_F32(RSF, CH) run_ssp33<RSF<float>, float, CH>(rp, g, lo, hi, refl, dx, dy, b, ctl, nsteps, stream)
+#else
+#define RUN_F32(RSF, CH) CLAWHIP_E_ARG   // FP32 + Kahan: the fast variant only
+#endif
   if (char_decomp == 2) {
     switch (rp.kind) {
       case CLAWHIP_RP_SHALLOW: return RUN(Shallow, true);
@@ -1245,6 +1255,7 @@ int launch_sharp_ssp33_run(const clawhip_rp& rp, int dtype, int char_decomp, con
     case CLAWHIP_RP_ADVECTION: return RUN(Advection, false);
   }
 #undef RUN
+#undef RUN_F32
   return CLAWHIP_E_KIND;
 }
 
