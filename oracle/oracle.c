@@ -20,7 +20,11 @@
 
 /* The REAL(4) literal 1.e-36 / 1.0e-36 the reference adds in double precision
  * (reconstruct.f90:11, weno.f90:52-57): the float nearest 1e-36, widened. */
-#define EPWENO ((double)1.0e-36f)
+#define EPWENO g_epweno
+static double g_epweno = (double)1.0e-36f;
+
+/* Test hook: the WENO epsilon the restatement adds (default the reference's). */
+void orc_set_weno_eps(double eps) { g_epweno = eps; }
 
 static inline double dmax(double a, double b) { return a > b ? a : b; }
 static inline double dmin(double a, double b) { return a < b ? a : b; }

@@ -122,6 +122,8 @@ double orc_step3ds(int meqn, int mwaves, int ng, int mx, int my, int mz,
  * interior, 0 elsewhere.  char_decomp 0: component WENO5 (weno.f90:5-83);
  * 2: characteristic WENO5 (reconstruct.f90:201-299) with the eigenvectors of
  * orc_evec.  Returns the CFL number. */
+/* The WENO epsilon of weno5 / weno5_char (default the reference's 1.e-36 REAL(4)). */
+void orc_set_weno_eps(double eps);
 double orc_sharp_flux2(int meqn, int mwaves, int ng, int mx, int my,
                        const double *q, double *dq, double dx, double dy, double dt,
                        int char_decomp, const orc_rp *p, int maux, const double *aux);

@@ -205,6 +205,15 @@ def sharp_flux2(rp, q, ng, dx, dy, dt, char_decomp, aux=None):
     return dq, cfl
 
 
+def set_weno_eps(eps):
+    """The WENO epsilon of the restatement's weno5 / weno5_char (test hook; the
+    default is the reference's REAL(4) 1.e-36, (double)1.0e-36f)."""
+    lib().orc_set_weno_eps(ctypes.c_double(eps))
+
+
+REF_WENO_EPS = float(np.float32(1.0e-36))
+
+
 def sharp_flux1(rp, q1d, ng, ixy, dtdx, char_decomp):
     meqn, n2 = q1d.shape
     dq1d = np.zeros_like(q1d, order="F")

@@ -28,15 +28,16 @@ namespace CLAW_VNS {
 
 // The WENO epsilon.  FP64: the reference's REAL(4) literal 1.e-36 widened to
 // double (reconstruct.f90:11, weno.f90:52-57).  FP32 keeps flat stencils away
-// from 0/0: the component form divides by (eps + beta)^2, so eps = 1e-18 keeps
-// that square a normal float; the characteristic form divides by sums of
-// products of two such squares, (eps + beta)^4, so it takes eps = 1e-9.  The
-// weights of stencils whose smoothness indicators exceed eps are unchanged.
+// from 0/0: both forms divide by (eps + beta)^2 (the FP32 characteristic weights
+// are formed as d_k / (eps + beta_k)^2, not through the reference's products of
+// two such squares), so eps = 1e-18 keeps that square a normal float.  On the
+// shallow-water humps and dam breaks at 256^2 it moves dq by 1e-11 of its
+// largest value, against FP32's rounding of 6e-8.
 template <typename T>
 __device__ __forceinline__ T weno_eps() { return sizeof(T) == 8 ? T((double)1.0e-36f) : T(1.0e-18f); }
 template <typename T>
 __device__ __forceinline__ T weno_eps_char() {
-  return sizeof(T) == 8 ? T((double)1.0e-36f) : T(1.0e-9f);
+  return sizeof(T) == 8 ? T((double)1.0e-36f) : T(1.0e-18f);
 }
 
 // ---------------------------------------------------------------- stage modes
@@ -177,12 +178,21 @@ __device__ __forceinline__ T weno_char_side(T hm2, T hm1, T h0, T hp1, T hp2, in
   tt1 = (eps + tt1) * (eps + tt1);
   tt2 = (eps + tt2) * (eps + tt2);
   tt3 = (eps + tt3) * (eps + tt3);
-  T s1 = tt2 * tt3;
-  T s2 = T(6) * tt1 * tt3;
-  T s3 = T(3) * tt1 * tt2;
-  T t0 = T(1) / (s1 + s2 + s3);
-  s1 = s1 * t0;
-  s3 = s3 * t0;
+  T s1, s3;
+  if constexpr (sizeof(T) == 4) {
+    // the same weights as d_k / (eps + beta_k)^2: no fourth powers to underflow
+    const T r1 = T(1) / tt1, r2 = T(6) / tt2, r3 = T(3) / tt3;
+    const T t0 = T(1) / (r1 + r2 + r3);
+    s1 = r1 * t0;
+    s3 = r3 * t0;
+  } else {   // the reference's products of two squares, bit for bit
+    s1 = tt2 * tt3;
+    T s2 = T(6) * tt1 * tt3;
+    s3 = T(3) * tt1 * tt2;
+    T t0 = T(1) / (s1 + s2 + s3);
+    s1 = s1 * t0;
+    s3 = s3 * t0;
+  }
   return (s1 * (t2 - t1) + (T(0.5) * s3 - T(0.25)) * (t3 - t2)) / T(3);
 }
 

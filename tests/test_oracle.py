@@ -1412,3 +1412,35 @@ def test_ssplmm_convergence_order(ti, k, order):
         ep = _lmm_convergence(ti, k, plant=True)
         rp_ = [np.log2(ep[i] / ep[i + 1]) for i in range(2)]
         assert not all(abs(r - order) < 0.15 for r in rp_), rp_
+
+
+def test_fp32_weno_epsilon_effect():
+    """The FP32 stages add eps = 1e-18 in both WENO5 forms where the reference
+    adds its REAL(4) 1.e-36 (the FP32 weights keep (eps + beta)^2 a normal
+    float).  On the shallow-water humps and dam breaks of the headline family at
+    256^2, the FP64 operator with eps = 1e-18 moves dq by less than 1e-9 of its
+    largest value, characteristic and component-wise -- below FP32's rounding
+    of dq (2^-24 = 6e-8).  The bar discriminates: eps = 1e-9, what a fourth-power
+    form of the characteristic weights would need in FP32, moves it by ~4e-6."""
+    from pyclaw_amd import indicator as I
+    n = 256
+    q = I.swe_problem(3, n, n)
+    rp = O.make_rp(O.SHALLOW, efix=1, grav=1.0)
+    ng, dx = 3, 1.0 / n
+    dt = 0.45 / n / 1.6
+    qbc = O.with_ghosts(q, ng)
+    O.bc2(qbc, ng, [2, 2], [2, 2])
+
+    def rel(cd, eps):
+        O.set_weno_eps(O.REF_WENO_EPS)
+        ref = O.interior(O.sharp_flux2(rp, qbc, ng, dx, dx, dt, cd)[0], ng)
+        try:
+            O.set_weno_eps(float(np.float32(eps)))
+            got = O.interior(O.sharp_flux2(rp, qbc, ng, dx, dx, dt, cd)[0], ng)
+        finally:
+            O.set_weno_eps(O.REF_WENO_EPS)
+        return np.abs(got - ref).max() / np.abs(ref).max()
+
+    for cd in (0, 2):
+        assert rel(cd, 1e-18) < 1e-9, cd
+    assert rel(2, 1e-9) > 1e-7
