@@ -607,13 +607,21 @@ __global__ void __launch_bounds__(FLAG_LIST_THREADS)
 template <class RS, typename T, bool CHAR>
 __global__ void __launch_bounds__(256) k_weno_flagged(const SharpArgs<T> a) {
   constexpr int ME = RS::meqn;
-  if (a.ctl && !*a.gate) return;
+  // the gate, the controller's state, the list length and this thread's first
+  // list entry are independent loads: issued together, they cost one memory
+  // round trip instead of four (the list holds (mx+1)*my + mx*(my+1) entries,
+  // so the first entry is readable before the length says it is valid)
+  const int open = a.ctl ? *a.gate : 1;
   const StageView<T> v = stage_view(a);
   const int n = *a.flag_count;
   const int mx = a.g.mx, my = a.g.my;
   const long long nx = (long long)(mx + 1) * my;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const long long id = a.flag_list[i];
+  const long long nint = nx + (long long)mx * (my + 1);
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int first = i0 < nint ? a.flag_list[i0] : 0;
+  if (!open) return;
+  for (int i = i0; i < n; i += gridDim.x * blockDim.x) {
+    const long long id = i == i0 ? first : a.flag_list[i];
     T st[6][ME], qrL[ME], qlR[ME];
     if (id < nx) {
       const int c = 1 + (int)(id % (mx + 1)), r = 1 + (int)(id / (mx + 1));
@@ -684,8 +692,11 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   constexpr int MA = AUX ? RS::maux : 1;
   constexpr bool TFLUCT = sizeof(T) == 4 && !AUX;
   const SharpArgs<T>& a = ma.s;
-  if (a.ctl && !*a.gate) return;   // uniform over the launch
+  // the gate and the controller's state in one round trip (both readable
+  // whether or not the gate is open)
+  const int open = a.ctl ? *a.gate : 1;
   const StageView<T> v = stage_view(a);
+  if (!open) return;   // uniform over the launch
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wid >= ma.nstrips * ma.nrb) return;   // uniform per wavefront
