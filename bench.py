@@ -345,7 +345,7 @@ def ensemble_leg(steps=32, members=512, seed=4):
 def euler3d_leg(steps=10, n=384, seed=5):
     """3D Euler, Roe, dimension-split, MC, 384^3, extrapolation BCs: seeded
     spherical blast (radius U[0.1,0.2], inner p U[5,10], outer p 0.1, rho 1 with
-    1e-3 seeded noise) on [-1,1]^3; FP64 and FP32 + Kahan."""
+    1e-3 seeded noise) on [-1,1]^3; FP64, fp64fast and FP32 + Kahan."""
     import torch
     import pyclaw_amd as pyclaw
     from pyclaw_amd import riemann
@@ -354,7 +354,7 @@ def euler3d_leg(steps=10, n=384, seed=5):
     x = -1 + (np.arange(n) + 0.5) * (2.0 / n)
     out = {}
     q = None
-    for prec in ("fp64", "fp32k"):
+    for prec in ("fp64", "fp64fast", "fp32k"):
         solver = pyclaw.ClawSolver3D(riemann.euler_3D)
         solver.limiters = pyclaw.limiters.tvd.MC
         solver.all_bcs = pyclaw.BC.extrap
@@ -383,9 +383,11 @@ def euler3d_leg(steps=10, n=384, seed=5):
         out[prec] = {"value": n ** 3 * steps / wall, "ms_per_step": 1e3 * wall / steps,
                      "hbm_frac": 240.0 * n ** 3 / (wall / steps) / 1e9 / HBM_PEAK_GBS}
         out[prec + "_q"] = sol.state.q
-    qa, qb = out.pop("fp64_q"), out.pop("fp32k_q")
-    out["fp32k"]["rel_l1_vs_fp64"] = float(np.abs(qb - qa).sum() / np.abs(qa).sum())
-    out["fp32k"]["rel_max_vs_fp64"] = float(np.abs(qb - qa).max() / np.abs(qa).max())
+    qa = out.pop("fp64_q")
+    for prec in ("fp64fast", "fp32k"):
+        qb = out.pop(prec + "_q")
+        out[prec]["rel_l1_vs_fp64"] = float(np.abs(qb - qa).sum() / np.abs(qa).sum())
+        out[prec]["rel_max_vs_fp64"] = float(np.abs(qb - qa).max() / np.abs(qa).max())
     out.update({"workload": "euler3d_blast_roe_mc_split_384cubed", "unit": "cell-updates/s",
                 "steps": steps, "bytes_per_cell_step": 240})
     return out

@@ -26,7 +26,7 @@
 // Every expression keeps the reference's evaluation order; the library is
 // compiled with -ffp-contract=off so FP64 results match the CPU oracle bitwise.
 // In the contracted (fast) build of the kernel sources (variant.hpp) the Euler
-// solver also shares its reciprocals: one division per density, per Roe
+// solvers (2D and 3D) also share their reciprocals: one division per density, per Roe
 // denominator and per sound speed squared, multiplications elsewhere
 // (kSharedReciprocals).
 #pragma once
@@ -502,17 +502,40 @@ struct Euler3 {
     const T gamma = p.gamma, gamma1 = p.gamma1;
     T rhsqrtl = sqrt(qL[0]);
     T rhsqrtr = sqrt(qR[0]);
-    T pl = gamma1 * (qL[4] - T(0.5) * (qL[1] * qL[1] + qL[2] * qL[2] + qL[3] * qL[3]) / qL[0]);
-    T pr = gamma1 * (qR[4] - T(0.5) * (qR[1] * qR[1] + qR[2] * qR[2] + qR[3] * qR[3]) / qR[0]);
-    T rhsq2 = rhsqrtl + rhsqrtr;
-    T u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
-    T v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
-    T w = (qL[mw] / rhsqrtl + qR[mw] / rhsqrtr) / rhsq2;
-    T enth = (((qL[4] + pl) / rhsqrtl + (qR[4] + pr) / rhsqrtr)) / rhsq2;
+    // the fast variant's shared reciprocals (kSharedReciprocals): 1/rho per side
+    // (and 1/sqrt(rho) = sqrt(rho)/rho), the Roe denominator, 1/a^2
+    T irl = T(0), irr = T(0), ia2 = T(0);
+    T pl, pr, u, v, w, enth;
+    if constexpr (kSharedReciprocals) {
+      irl = T(1) / qL[0];
+      irr = T(1) / qR[0];
+      const T isl = rhsqrtl * irl, isr = rhsqrtr * irr;
+      pl = gamma1 * (qL[4] - T(0.5) * (qL[1] * qL[1] + qL[2] * qL[2] + qL[3] * qL[3]) * irl);
+      pr = gamma1 * (qR[4] - T(0.5) * (qR[1] * qR[1] + qR[2] * qR[2] + qR[3] * qR[3]) * irr);
+      const T irq = T(1) / (rhsqrtl + rhsqrtr);
+      u = (qL[mu] * isl + qR[mu] * isr) * irq;
+      v = (qL[mv] * isl + qR[mv] * isr) * irq;
+      w = (qL[mw] * isl + qR[mw] * isr) * irq;
+      enth = ((qL[4] + pl) * isl + (qR[4] + pr) * isr) * irq;
+    } else {
+      pl = gamma1 * (qL[4] - T(0.5) * (qL[1] * qL[1] + qL[2] * qL[2] + qL[3] * qL[3]) / qL[0]);
+      pr = gamma1 * (qR[4] - T(0.5) * (qR[1] * qR[1] + qR[2] * qR[2] + qR[3] * qR[3]) / qR[0]);
+      T rhsq2 = rhsqrtl + rhsqrtr;
+      u = (qL[mu] / rhsqrtl + qR[mu] / rhsqrtr) / rhsq2;
+      v = (qL[mv] / rhsqrtl + qR[mv] / rhsqrtr) / rhsq2;
+      w = (qL[mw] / rhsqrtl + qR[mw] / rhsqrtr) / rhsq2;
+      enth = (((qL[4] + pl) / rhsqrtl + (qR[4] + pr) / rhsqrtr)) / rhsq2;
+    }
     T u2v2w2 = u * u + v * v + w * w;
     T a2 = gamma1 * (enth - T(.5) * u2v2w2);
     T a = sqrt(a2);
-    T g1a2 = gamma1 / a2;
+    T g1a2;
+    if constexpr (kSharedReciprocals) {
+      ia2 = T(1) / a2;
+      g1a2 = gamma1 * ia2;
+    } else {
+      g1a2 = gamma1 / a2;
+    }
     T euv = enth - u2v2w2;
     roe.u = u; roe.v = v; roe.w = w; roe.enth = enth; roe.a = a; roe.g1a2 = g1a2;
     roe.euv = euv; roe.u2v2w2 = u2v2w2;
@@ -521,7 +544,11 @@ struct Euler3 {
     T a4 = g1a2 * (euv * d0 + u * d1 + v * d2 + w * d3 - d4);
     T b2 = d2 - v * d0;
     T b3 = d3 - w * d0;
-    T a5 = (d1 + (a - u) * d0 - a * a4) / (T(2) * a);
+    T a5;
+    if constexpr (kSharedReciprocals)   // 1/(2a) = a/(2a^2)
+      a5 = (d1 + (a - u) * d0 - a * a4) * (T(0.5) * a * ia2);
+    else
+      a5 = (d1 + (a - u) * d0 - a * a4) / (T(2) * a);
     T a1 = d0 - a4 - a5;
     wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v; wave[0][mw] = a1 * w;
     wave[0][4] = a1 * (enth - u * a); s[0] = u - a;
@@ -543,9 +570,16 @@ struct Euler3 {
     }
     do {
       T rhoim1 = qL[0];
-      T pim1 = gamma1 * (qL[4] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv] + qL[mw] * qL[mw]) / rhoim1);
-      T cim1 = sqrt(gamma * pim1 / rhoim1);
-      T s0 = qL[mu] / rhoim1 - cim1;
+      T pim1, cim1, s0;
+      if constexpr (kSharedReciprocals) {
+        pim1 = gamma1 * (qL[4] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv] + qL[mw] * qL[mw]) * irl);
+        cim1 = sqrt(gamma * pim1 * irl);
+        s0 = qL[mu] * irl - cim1;
+      } else {
+        pim1 = gamma1 * (qL[4] - T(0.5) * (qL[mu] * qL[mu] + qL[mv] * qL[mv] + qL[mw] * qL[mw]) / rhoim1);
+        cim1 = sqrt(gamma * pim1 / rhoim1);
+        s0 = qL[mu] / rhoim1 - cim1;
+      }
       if (s0 >= T(0) && s[0] > T(0)) {
 #pragma unroll
         for (int m = 0; m < meqn; m++) amdq[m] = T(0);
@@ -556,9 +590,17 @@ struct Euler3 {
       T rhov1 = qL[mv] + wave[0][mv];
       T rhow1 = qL[mw] + wave[0][mw];
       T en1 = qL[4] + wave[0][4];
-      T p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) / rho1);
-      T c1 = sqrt(gamma * p1 / rho1);
-      T s1 = rhou1 / rho1 - c1;
+      T p1, c1, s1;
+      if constexpr (kSharedReciprocals) {
+        const T ir1 = T(1) / rho1;
+        p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) * ir1);
+        c1 = sqrt(gamma * p1 * ir1);
+        s1 = rhou1 * ir1 - c1;
+      } else {
+        p1 = gamma1 * (en1 - T(0.5) * (rhou1 * rhou1 + rhov1 * rhov1 + rhow1 * rhow1) / rho1);
+        c1 = sqrt(gamma * p1 / rho1);
+        s1 = rhou1 / rho1 - c1;
+      }
       T sfract;
       if (s0 < T(0) && s1 > T(0)) sfract = s0 * (s1 - s[0]) / (s1 - s0);
       else if (s[0] < T(0)) sfract = s[0];
@@ -569,17 +611,32 @@ struct Euler3 {
 #pragma unroll
       for (int m = 0; m < meqn; m++) amdq[m] = amdq[m] + s[1] * wave[1][m];
       T rhoi = qR[0];
-      T pi = gamma1 * (qR[4] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv] + qR[mw] * qR[mw]) / rhoi);
-      T ci = sqrt(gamma * pi / rhoi);
-      T s3 = qR[mu] / rhoi + ci;
+      T pi, ci, s3;
+      if constexpr (kSharedReciprocals) {
+        pi = gamma1 * (qR[4] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv] + qR[mw] * qR[mw]) * irr);
+        ci = sqrt(gamma * pi * irr);
+        s3 = qR[mu] * irr + ci;
+      } else {
+        pi = gamma1 * (qR[4] - T(0.5) * (qR[mu] * qR[mu] + qR[mv] * qR[mv] + qR[mw] * qR[mw]) / rhoi);
+        ci = sqrt(gamma * pi / rhoi);
+        s3 = qR[mu] / rhoi + ci;
+      }
       T rho2 = qR[0] - wave[2][0];
       T rhou2 = qR[mu] - wave[2][mu];
       T rhov2 = qR[mv] - wave[2][mv];
       T rhow2 = qR[mw] - wave[2][mw];
       T en2 = qR[4] - wave[2][4];
-      T p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) / rho2);
-      T c2 = sqrt(gamma * p2 / rho2);
-      T s2 = rhou2 / rho2 + c2;
+      T p2, c2, s2;
+      if constexpr (kSharedReciprocals) {
+        const T ir2 = T(1) / rho2;
+        p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) * ir2);
+        c2 = sqrt(gamma * p2 * ir2);
+        s2 = rhou2 * ir2 + c2;
+      } else {
+        p2 = gamma1 * (en2 - T(0.5) * (rhou2 * rhou2 + rhov2 * rhov2 + rhow2 * rhow2) / rho2);
+        c2 = sqrt(gamma * p2 / rho2);
+        s2 = rhou2 / rho2 + c2;
+      }
       if (s2 < T(0) && s3 > T(0)) sfract = s2 * (s3 - s[2]) / (s3 - s2);
       else if (s[2] < T(0)) sfract = s[2];
       else break;
