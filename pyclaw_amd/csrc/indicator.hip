@@ -764,7 +764,9 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
   long long passes = (long long)((g.mx + 1 + ni - 1) / ni) * g.my +
                      (long long)((g.mx + ni - 1) / ni) * (g.my + 1);
   if (passes + 4LL * 65536 >= (1LL << 31)) return CLAWHIP_E_ARG;   // 32-bit pass index
-  // grid-stride: a few resident rounds of wavefronts, each loading the weights once
+  // grid-stride: one resident round of wavefronts (measured at 2048^2: one
+  // round 85 us, two 103 us, four 93 us, half a round 119 us), each loading the
+  // weights once
   long long cap = 2048;
   if (const char* env = getenv("CLAWHIP_FLAGS_BLOCKS")) {
     const long long b = atoll(env);
@@ -776,6 +778,15 @@ int launch_mlp_flags(int dtype, const void* q, const Geom& g, const int* lo, con
     const long long rpass = (long long)((g.mx + 1 + 63) / 64) * g.my +
                             (long long)((g.mx + 63) / 64) * (g.my + 1);
     long long rblocks = (rpass + 3) / 4;
+    if (!getenv("CLAWHIP_FLAGS_BLOCKS")) {
+      const void* kern = dtype == CLAWHIP_F64
+          ? (meqn == 1 ? (const void*)k_mlp_flags_row<double, 1> : meqn == 2 ? (const void*)k_mlp_flags_row<double, 2>
+             : meqn == 3 ? (const void*)k_mlp_flags_row<double, 3> : (const void*)k_mlp_flags_row<double, 4>)
+          : (meqn == 1 ? (const void*)k_mlp_flags_row<float, 1> : meqn == 2 ? (const void*)k_mlp_flags_row<float, 2>
+             : meqn == 3 ? (const void*)k_mlp_flags_row<float, 3> : (const void*)k_mlp_flags_row<float, 4>);
+      const long long resident = resident_waves(kern, 0);
+      if (resident >= 4) cap = resident / 4;
+    }
     if (rblocks > cap) rblocks = cap;
     const dim3 grid((unsigned)rblocks);
     if (dtype == CLAWHIP_F64) {
