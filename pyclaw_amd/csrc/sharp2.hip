@@ -692,11 +692,8 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   constexpr int MA = AUX ? RS::maux : 1;
   constexpr bool TFLUCT = sizeof(T) == 4 && !AUX;
   const SharpArgs<T>& a = ma.s;
-  // the gate and the controller's state in one round trip (both readable
-  // whether or not the gate is open)
-  const int open = a.ctl ? *a.gate : 1;
+  if (a.ctl && !*a.gate) return;   // uniform over the launch
   const StageView<T> v = stage_view(a);
-  if (!open) return;   // uniform over the launch
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (wid >= ma.nstrips * ma.nrb) return;   // uniform per wavefront
