@@ -19,7 +19,9 @@ namespace claw {
 namespace CLAW_VNS {
 
 template <class RS, typename T, bool KAHAN, bool AUX = false, bool CAPA = false>
-__global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(unsplit_waves<RS, T, AUX, CAPA>())))
+    k_unsplit(const UnsplitArgs<T> a) {
   unsplit_body<RS, T, KAHAN, AUX, CAPA>(a, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
@@ -28,8 +30,9 @@ __global__ void __launch_bounds__(256) k_unsplit(const UnsplitArgs<T> a) {
 // names the current one.  The last workgroup to finish applies the CFL check,
 // accepts or rejects the step and computes the next dt (k_step_ctl after it).
 template <class RS, typename T, bool KAHAN>
-__global__ void __launch_bounds__(256) k_unsplit_ctl(UnsplitArgs<T> a, StepCtl* ctl,
-                                                     CtlBuffers bufs) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(unsplit_waves<RS, T>())))
+    k_unsplit_ctl(UnsplitArgs<T> a, StepCtl* ctl, CtlBuffers bufs) {
   if (ctl->done) return;   // uniform across the grid
   const int cur = ctl->cur;
   const double dt = ctl->dt;
@@ -64,10 +67,11 @@ namespace {
 // H minimises that product (ties: the longer march).  CLAWHIP_UNSPLIT_ROWS
 // overrides the choice (tuning sweeps).
 template <class Kernel>
-int choose_rows(Kernel kernel, int nstrips, int my) {
+int choose_rows(Kernel kernel, int nstrips, int my, int waves_hint) {
   // beyond two wavefronts per SIMD the measured gain of a third is smaller than
-  // the halo rows the shorter marches add, so count at most two
-  const long long resident = resident_waves((const void*)kernel, 2);
+  // the halo rows the shorter marches add, so count at most two, unless the
+  // kernel is compiled for more (unsplit_waves)
+  const long long resident = resident_waves((const void*)kernel, waves_hint > 2 ? waves_hint : 2);
   if (resident == 0) return 16;
   if (const char* env = getenv("CLAWHIP_UNSPLIT_ROWS")) {
     int h = atoi(env);
@@ -111,7 +115,8 @@ int run_unsplit(const clawhip_rp& rp, const void* q_in, const Geom& gin, void* q
   for (int w = 0; w < kMaxWaves; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
   a.dtdx = T(dt / dx); a.dtdy = T(dt / dy);
   a.nstrips = (gout.mx + 59) / 60;
-  a.H = choose_rows(k_unsplit<RS, T, KAHAN, AUX, CAPA>, a.nstrips, gout.my);
+  a.H = choose_rows(k_unsplit<RS, T, KAHAN, AUX, CAPA>, a.nstrips, gout.my,
+                    unsplit_waves<RS, T, AUX, CAPA>());
   a.nrb = (gout.my + a.H - 1) / a.H;
   a.qin = (const T*)q_in; a.qout = (T*)q_out;
   a.comp_in = (const T*)comp_in; a.comp_out = (T*)comp_out;
@@ -135,7 +140,7 @@ int run_unsplit_ctl(const clawhip_rp& rp, const Geom& g, const int* lo, const in
   for (int w = 0; w < kMaxWaves; w++) a.mthlim[w] = w < RS::mwaves ? mthlim[w] : 0;
   a.dtdx = T(0); a.dtdy = T(0);
   a.nstrips = (g.mx + 59) / 60;
-  a.H = choose_rows(k_unsplit_ctl<RS, T, KAHAN>, a.nstrips, g.my);
+  a.H = choose_rows(k_unsplit_ctl<RS, T, KAHAN>, a.nstrips, g.my, unsplit_waves<RS, T>());
   a.nrb = (g.my + a.H - 1) / a.H;
   a.qin = nullptr; a.qout = nullptr; a.comp_in = nullptr; a.comp_out = nullptr;
   a.cfl_bits = nullptr;

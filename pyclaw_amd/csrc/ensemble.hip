@@ -58,7 +58,7 @@ __device__ __forceinline__ const RPParams<float>& ens_rp<float>(const EnsHeader&
 }
 
 template <class RS, typename T, bool KAHAN>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(unsplit_waves<RS, T>())))
 k_ens_step(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem) {
   const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (gw >= hdr->waves) return;
@@ -105,10 +105,13 @@ __global__ void __launch_bounds__(256) k_ens_ctl(const EnsHeader* __restrict__ h
 namespace {
 
 // The table does not know the state's type, so the default rows follow the
-// occupancy of the FP32 + Kahan Euler step (the ensemble workload's); the
-// autotuner (Ensemble.autotune) replaces them by measured rows.
+// occupancy of the FP32 + Kahan Euler step (the ensemble workload's, counted up
+// to the wavefronts per SIMD it is compiled for); the autotuner
+// (Ensemble.autotune) replaces them by measured rows.
 long long ens_resident_waves() {
-  const long long r = resident_waves((const void*)k_ens_step<Euler<float>, float, true>, 2);
+  constexpr int hint = unsplit_waves<Euler<float>, float>();
+  const long long r =
+      resident_waves((const void*)k_ens_step<Euler<float>, float, true>, hint > 2 ? hint : 2);
   return r > 0 ? r : 1024;
 }
 

@@ -193,6 +193,35 @@ __device__ __forceinline__ void cfl_y(const UnsplitArgs<T>& a, T& acc, const T (
   }
 }
 
+// Wavefronts per SIMD a step kernel is compiled for (the register budget is
+// 512 / that number), where a solver gains from more than the compiler's own
+// choice.  FP64 acoustics (180 VGPRs, 2 per SIMD) and FP32 + Kahan Euler (172
+// VGPRs, 2 per SIMD) fit 168 VGPRs with a few spilled dwords:
+//   - 4096^2 FP64 acoustics step: 476 -> 431 us;
+//   - 1024^2 FP32 + Kahan Euler step: 76.7 -> 71.0 us.
+// Forcing FP64 Euler (314 VGPRs) to 3 per SIMD doubles its time (heavy
+// spills); FP64 acoustics at 4 per SIMD (37 spilled dwords) takes 907 us.
+// 1 = no constraint.  The aux and capacity forms keep the compiler's choice.
+// The step kernels (k_unsplit, k_unsplit_ctl, k_ens_step) take it as their
+// amdgpu_waves_per_eu attribute, and their march-length choice counts that
+// many resident wavefronts per SIMD.
+template <class RS, typename T>
+struct UnsplitWaves {
+  static constexpr int value = 1;
+};
+template <>
+struct UnsplitWaves<Acoustics<double>, double> {
+  static constexpr int value = 3;
+};
+template <>
+struct UnsplitWaves<Euler<float>, float> {
+  static constexpr int value = 3;
+};
+template <class RS, typename T, bool AUX = false, bool CAPA = false>
+constexpr int unsplit_waves() {
+  return (AUX || CAPA) ? 1 : UnsplitWaves<RS, T>::value;
+}
+
 // `wid`: this wavefront's index among the a.nstrips * a.nrb of the launch.
 template <class RS, typename T, bool KAHAN, bool AUX = false, bool CAPA = false>
 __device__ __forceinline__ void unsplit_body(const UnsplitArgs<T>& a, int wid) {
