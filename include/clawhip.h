@@ -276,7 +276,9 @@ int clawhip_step3_split(const clawhip_rp *rp, int dtype, const void *q_in,
  * 0 uses WENO5's linear 5th-order weights; the others use the WENO5 selected by
  * char_decomp.  NULL flags = WENO5 everywhere (the reference's operator).
  * With flags: flag_list / flag_count from clawhip_mlp_flag_list of the same
- * flags, and recon, a device buffer of 2*num_eqn*((mx+1)*my + mx*(my+1))
+ * flags (flag_list sized as there, (mx+1)*my + mx*(my+1) entries: the full-WENO5
+ * pass reads its first entries before it reads the count), and recon, a
+ * device buffer of 2*num_eqn*((mx+1)*my + mx*(my+1))
  * elements of the state's type that takes the full-WENO5 edge values of the
  * listed interfaces (work space; its contents on entry do not matter).
  * aux (geometry gaux, num_aux fields, codes aux_bc_lower/upper 1..4 as for

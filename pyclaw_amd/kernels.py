@@ -204,6 +204,8 @@ def sharp2_stage(rp, char_decomp, out, cfl, bc_lower, bc_upper, reflect, dt, dx,
     if flags is not None:
         if flag_list is None:
             flag_list = mlp_flag_list(flags, mx, my)
+        if flag_list[0].numel() < mlp_num_interfaces(mx, my):
+            raise ValueError("flag_list ids must hold (mx+1)*my + mx*(my+1) entries")
         if recon is None:
             recon = torch.empty(recon_elems(mx, my, meqn), dtype=out.dtype, device=out.device)
         assert recon.dtype == out.dtype and recon.numel() >= recon_elems(mx, my, meqn)
