@@ -573,7 +573,14 @@ static void rpn_vc_advection(const orc_rp *p, int ixy, const double *qL, const d
  *  11  shallow_bathymetry_fwave_2D: the Roe speeds instead of Einfeldt's
  *      (caught by the sill fixture);
  *  12  euler_3D rpt3: the normal velocity as the transverse speed (caught by
- *      the transverse Roe property and by the 3D shock-bubble fixture). */
+ *      the transverse Roe property and by the 3D shock-bubble fixture);
+ *  13  shallow_bathymetry_fwave_2D: a dry side's velocity hu/h kept instead of
+ *      0 (caught by the dry-momentum test);
+ *  14  shallow_bathymetry_fwave_2D: an interface with both sides dry solved
+ *      instead of left without waves (caught by the both-dry test);
+ *  15  shallow_bathymetry_fwave_2D: the right side's transverse velocity in the
+ *      left-going f-wave (caught by the flux-difference sum and by the x -> -x
+ *      symmetry). */
 static int g_plant = 0;
 void orc_set_plant(int k) { g_plant = k; }
 
@@ -623,8 +630,11 @@ static void rpn_vc_acoustics(int ixy, const double *qL, const double *qR, const 
  *   fluctuations: each f-wave goes to amdq (s < 0) or apdq (s > 0), half to
  *   each when s = 0.
  * Pinned by the shallow_2d sill fixture (examples/shallow_2d/test_shallow2d.py:
- * 38-39; L1 difference 8e-16 against the bar 1e-6), which never dries: the
- * dry-state branch is unpinned. */
+ * 38-39; L1 difference 8e-16 against the bar 1e-6), which never dries; the
+ * dry-state branch by properties the equations fix (test_oracle.py): both
+ * sides dry give no waves, a dry side's momentum does not enter, amdq + apdq
+ * is the flux difference with the bathymetry source (dry velocities 0), and
+ * the solve is symmetric under x -> -x (plants 13, 14 and 15 fail them). */
 static void rpn_shallow_fwave(const orc_rp *p, int ixy, const double *qL, const double *qR,
                               const double *auxL, const double *auxR, double *fw, double *s,
                               double *amdq, double *apdq) {
@@ -633,10 +643,11 @@ static void rpn_shallow_fwave(const orc_rp *p, int ixy, const double *qL, const 
   for (int k = 0; k < 9; k++) fw[k] = 0.0;
   for (int k = 0; k < 3; k++) s[k] = amdq[k] = apdq[k] = 0.0;
   const double hl = qL[0], hr = qR[0];
-  if (hl <= dry && hr <= dry) return;
+  if (hl <= dry && hr <= dry && g_plant != 14) return;   /* planted 14: solved anyway */
   double ul = 0.0, vl = 0.0, ur = 0.0, vr = 0.0;
-  if (hl > dry) { ul = qL[mu] / hl; vl = qL[mv] / hl; }
-  if (hr > dry) { ur = qR[mu] / hr; vr = qR[mv] / hr; }
+  /* planted 13: a dry side's velocity hu/h kept */
+  if (hl > dry || (g_plant == 13 && hl > 0.0)) { ul = qL[mu] / hl; vl = qL[mv] / hl; }
+  if (hr > dry || (g_plant == 13 && hr > 0.0)) { ur = qR[mu] / hr; vr = qR[mv] / hr; }
   const double sl = sqrt(hl), sr = sqrt(hr);
   const double hbar = 0.5 * (hl + hr);
   const double uhat = (sl * ul + sr * ur) / (sl + sr);
@@ -660,7 +671,7 @@ static void rpn_shallow_fwave(const orc_rp *p, int ixy, const double *qL, const 
   const double b2 = d3 - b1 * vl - b3 * vr;
   fw[0] = b1;
   fw[mu] = b1 * s[0];
-  fw[mv] = b1 * vl;
+  fw[mv] = b1 * (g_plant == 15 ? vr : vl);   /* planted 15: the right side's v */
   fw[3 + mv] = b2;
   fw[6] = b3;
   fw[6 + mu] = b3 * s[2];

@@ -255,6 +255,19 @@ def rpn(rp, ixy, qL, qR):
     return wave.reshape(mw, meqn), s, amdq, apdq
 
 
+def rpn_aux(rp, ixy, qL, qR, auxL, auxR):
+    """The normal solve of the solvers that read aux (orc_rpn_aux)."""
+    meqn, mw = MEQN[rp.kind], MWAVES[rp.kind]
+    wave = np.zeros(meqn * mw)
+    s = np.zeros(mw)
+    amdq = np.zeros(meqn)
+    apdq = np.zeros(meqn)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (qL, qR, auxL, auxR)]
+    lib().orc_rpn_aux(ctypes.byref(rp), ixy, *[_dp(a) for a in args], _dp(wave), _dp(s),
+                      _dp(amdq), _dp(apdq))
+    return wave.reshape(mw, meqn), s, amdq, apdq
+
+
 # ---------------------------------------------------------------- reference build
 def ref_fwave(on):
     """use_fwave for the reference step routines called through the shim."""

@@ -69,6 +69,63 @@ def test_split_step_fwave_bathymetry_bit_exact(bcl, bcu):
     assert np.abs(q_from_device(out32) - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
+def _lake_with_island(rng, mx, my):
+    """A lake at rest (surface 1) over random bathymetry, a dry island (zero
+    depth on ground at 1.2) and a shore ring of cells below dry_tolerance 1e-3
+    that hold momentum the solver ignores; moving water away from the island."""
+    X, Y = np.meshgrid(np.arange(mx), np.arange(my), indexing="ij")
+    b = rng.uniform(0.0, 0.4, (mx, my))
+    r = np.hypot(X - 16, Y - 11)
+    island, shore = r < 5.0, (r >= 5.0) & (r < 6.5)
+    b[island] = 1.2
+    b[shore] = rng.uniform(0.9991, 0.99995, shore.sum())
+    h = np.maximum(1.0 - b, 0.0)
+    h[island] = 0.0
+    u, v = np.zeros((mx, my)), np.zeros((mx, my))
+    far = X > 32
+    u[far], v[far] = rng.uniform(-0.4, 0.4, far.sum()), rng.uniform(-0.4, 0.4, far.sum())
+    hu, hv = h * u, h * v
+    hu[shore] = rng.uniform(-2e-4, 2e-4, shore.sum())
+    hv[shore] = rng.uniform(-2e-4, 2e-4, shore.sum())
+    return np.asfortranarray(np.array([h, hu, hv])), np.asfortranarray(b[None])
+
+
+def test_split_step_fwave_dry_cells_bit_exact():
+    """shallow_bathymetry_fwave_2D where cells are dry (_lake_with_island):
+    interfaces between two dry cells, between a dry cell holding momentum and a
+    wet one, and between wet cells.  Both sweeps in FP64 equal the oracle's,
+    whose dry branch is pinned by properties in tests/test_oracle.py (no waves
+    between two dry cells, a dry cell's momentum not entering, the
+    flux-difference sum, x -> -x symmetry); the mistakes planted there (13, 14,
+    15) each change this step's result."""
+    from pyclaw_amd import kernels as K
+    from pyclaw_amd.device import q_from_device
+    print("f-wave split, dry cells", flush=True)
+    mx, my, ng = 53, 41, 2
+    q, aux = _lake_with_island(np.random.default_rng(47), mx, my)
+    dx, dy, dt = 1.0 / mx, 1.0 / my, 0.002
+    meth = O.method_array(dimensional_split=True, num_aux=1)
+    bcl, bcu = [1, 3], [3, 1]
+    qbc, abc = O.with_ghosts(q, ng), O.with_ghosts(aux, ng)
+    O.bc2(qbc, ng, bcl, bcu)
+    O.bc2(abc, ng, [1, 1], [1, 1], reflect=(-1, -1))
+    rp_o = O.make_rp(O.SHALLOW_FWAVE, grav=1.0, dry_tolerance=1e-3)
+    qold = qbc.copy(order="F")
+    c1 = O.step2(rp_o, qold, qbc, ng, dx, dy, dt, meth, [4] * 3, ids=1, use_fwave=True, aux=abc)
+    c2 = O.step2(rp_o, qbc, qbc, ng, dx, dy, dt, meth, [4] * 3, ids=2, use_fwave=True, aux=abc)
+    ref = O.interior(qbc, ng)
+    assert np.isfinite(ref).all() and (ref[0] <= 1e-3).sum() > 100
+    rp = K.make_rp(11, {"grav": 1.0, "dry_tolerance": 1e-3}, fwave=True)
+    out = torch.empty_like(_dev(q))
+    cfl = torch.zeros(1, dtype=torch.float64, device="cuda")
+    K.step2_split(rp, _dev(q), out, cfl, bcl, bcu, [1, 2], meth, [4] * 3, dx, dy, dt,
+                  aux=_dev(aux), aux_bc_lower=[1, 1], aux_bc_upper=[1, 1])
+    torch.cuda.synchronize()
+    got = q_from_device(out)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    assert float(cfl.item()) == max(c1, c2)
+
+
 def test_split_step_whole_qbc_output_bit_exact():
     """A stored-ghost input and the whole-qbc output: every cell, ghosts
     included, equals the oracle's qbc after its two in-place sweeps (the x-sweep

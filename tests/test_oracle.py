@@ -701,6 +701,104 @@ def test_fixture_sill_fwave(golden_dir):
     assert _l1(fr11[-1][0], exp, d) > 1e-6
 
 
+# shallow_bathymetry_fwave_2D at interfaces where one or both sides are dry
+# (depth at or below dry_tolerance 1e-3): (qL, qR, bL, bR) in the x-normal
+# component order (h, hu, hv)
+FW_DRY = [([0.0, 0.0, 0.0], [1.2, 0.3, -0.4], 0.5, -0.2),
+          ([5e-4, 0.2, 0.1], [0.8, -0.5, 0.3], 0.1, 0.0),
+          ([2.0, 1.0, 0.5], [0.0, 0.0, 0.0], -1.0, 0.3),
+          ([1.5, -0.7, 0.2], [7e-4, -0.1, 0.05], 0.0, 0.2),
+          ([0.9, 0.4, -0.6], [1e-3, 0.02, 0.3], 0.3, 0.25)]
+FW_G, FW_DRY_TOL = 9.81, 1e-3
+
+
+def _fw_rpn(ixy, qL, qR, bL, bR, plant=0):
+    """The solve with the x-normal state (h, hu, hv) placed in the ixy component
+    order; returns s, amdq, apdq back in the x-normal order."""
+    perm = [0, 1, 2] if ixy == 1 else [0, 2, 1]
+    rp = O.make_rp(O.SHALLOW_FWAVE, grav=FW_G, dry_tolerance=FW_DRY_TOL)
+    O.lib().orc_set_plant(plant)
+    try:
+        _, s, amdq, apdq = O.rpn_aux(rp, ixy, np.array(qL)[perm], np.array(qR)[perm], [bL], [bR])
+    finally:
+        O.lib().orc_set_plant(0)
+    return s, amdq[perm], apdq[perm]
+
+
+def _fw_flux_difference(qL, qR, bL, bR):
+    """f(qR) - f(qL) + (0, g hbar (bR - bL), 0) for the normal flux
+    f = (hu, hu^2 + g h^2/2, huv), with the velocity of a dry side taken as 0."""
+    def flux(q):
+        h = q[0]
+        u, v = (q[1] / h, q[2] / h) if h > FW_DRY_TOL else (0.0, 0.0)
+        return np.array([h * u, h * u * u + 0.5 * FW_G * h * h, h * u * v])
+    hbar = 0.5 * (qL[0] + qR[0])
+    return flux(qR) - flux(qL) + np.array([0.0, FW_G * hbar * (bR - bL), 0.0])
+
+
+@pytest.mark.parametrize("plant", [0, 14])
+def test_fwave_both_sides_dry_no_waves(plant):
+    """Both sides at or below dry_tolerance: no waves, no speeds, no
+    fluctuations, whatever momentum the dry cells hold; solving such an
+    interface anyway (planted 14) gives fluctuations."""
+    out = [_fw_rpn(ixy, qL, qR, 0.2, 0.7, plant)
+           for ixy in (1, 2)
+           for qL, qR in (([0.0, 0.0, 0.0], [0.0, 0.0, 0.0]),
+                          ([5e-4, 0.3, -0.2], [8e-4, -0.1, 0.4]))]
+    zero = all(np.all(a == 0.0) for o in out for a in o)
+    assert zero == (plant == 0)
+
+
+@pytest.mark.parametrize("plant", [0, 13])
+def test_fwave_dry_momentum_does_not_enter(plant):
+    """A dry side's velocity is 0, so its momentum does not enter the solve:
+    replacing it by zero leaves the result unchanged; keeping hu/h (planted 13)
+    changes it."""
+    same = True
+    for ixy in (1, 2):
+        for qL, qR, bL, bR in FW_DRY:
+            a = _fw_rpn(ixy, qL, qR, bL, bR, plant)
+            qL0 = [qL[0], 0.0, 0.0] if qL[0] <= FW_DRY_TOL else qL
+            qR0 = [qR[0], 0.0, 0.0] if qR[0] <= FW_DRY_TOL else qR
+            b = _fw_rpn(ixy, qL0, qR0, bL, bR, plant)
+            same = same and all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert same == (plant == 0)
+
+
+@pytest.mark.parametrize("plant", [0, 13, 15])
+def test_fwave_sum_is_flux_difference(plant):
+    """The f-waves split the flux difference with the bathymetry source:
+    amdq + apdq = f(qR) - f(qL) + (0, g hbar (bR - bL), 0), dry velocities 0
+    (relative 1e-13).  A dry side's velocity kept (13) or the right side's
+    transverse velocity in the left-going f-wave (15) breaks it."""
+    ok = True
+    for ixy in (1, 2):
+        for qL, qR, bL, bR in FW_DRY:
+            _, amdq, apdq = _fw_rpn(ixy, qL, qR, bL, bR, plant)
+            d = _fw_flux_difference(qL, qR, bL, bR)
+            assert np.all(np.isfinite(amdq + apdq))
+            ok = ok and np.allclose(amdq + apdq, d, rtol=1e-13, atol=1e-13 * np.abs(d).max())
+    assert ok == (plant == 0)
+
+
+@pytest.mark.parametrize("plant", [0, 15])
+def test_fwave_mirror_symmetry(plant):
+    """x -> -x maps the interface (L, R) to (mirror R, mirror L), mirror(h, hu,
+    hv) = (h, -hu, hv): the speeds become -reversed, amdq' = M apdq and apdq' = M
+    amdq with M = diag(1, -1, 1) (relative 1e-14).  Planted 15 breaks it."""
+    M = np.array([1.0, -1.0, 1.0])
+    ok = True
+    for ixy in (1, 2):
+        for qL, qR, bL, bR in FW_DRY:
+            s, amdq, apdq = _fw_rpn(ixy, qL, qR, bL, bR, plant)
+            s2, amdq2, apdq2 = _fw_rpn(ixy, M * np.array(qR), M * np.array(qL), bR, bL, plant)
+            tol = 1e-14 * max(np.abs(amdq).max(), np.abs(apdq).max(), 1.0)
+            ok = (ok and np.allclose(s2, -s[::-1], rtol=0, atol=1e-14 * np.abs(s).max())
+                  and np.allclose(amdq2, M * apdq, rtol=0, atol=tol)
+                  and np.allclose(apdq2, M * amdq, rtol=0, atol=tol))
+    assert ok == (plant == 0)
+
+
 @pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
 def test_live_reference_fwave_bit_exact():
     """use_fwave = .true. through the reference's own step2 and step2ds
