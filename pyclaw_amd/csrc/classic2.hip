@@ -77,9 +77,13 @@ int choose_rows(Kernel kernel, int nstrips, int my, int waves_hint) {
     int h = atoi(env);
     if (h >= 1 && h <= 4096) return h;
   }
+  // the shortest march considered: 3 rows.  A grid that fills less than one
+  // resident round takes it (256^2 FP64 acoustics step: H = 1, 2, 3, 4, 6 ran
+  // 26.8, 21.3, 21.0, 22.0, 25.0 us); shorter marches add more wavefronts than
+  // they save in each one's dependent row chain
   int best = 16;
   long long best_cost = -1;
-  for (int h = 4; h <= 64; h++) {
+  for (int h = 3; h <= 64; h++) {
     const long long waves = (long long)nstrips * ((my + h - 1) / h);
     const long long cost = ((waves + resident - 1) / resident) * (long long)(h + 2);
     if (best_cost < 0 || cost < best_cost || (cost == best_cost && h > best)) {
