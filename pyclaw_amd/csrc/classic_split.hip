@@ -14,6 +14,8 @@
 // The 2D split uses flux2's 0.5*cqxx; the 3D split flux3's cqxx with the 0.5
 // inside the sum (flux3.f90:244) -- the HALF_IN_SUM template flag.
 #include "variant.hpp"  // first: selects the arithmetic variant
+#include <stdlib.h>
+
 #include "launch.hpp"
 
 namespace claw {
@@ -325,6 +327,17 @@ __global__ void __launch_bounds__(256) k_march(const SweepArgs<T> a) {
 namespace {
 constexpr int kMarchRows = 32;
 
+// Rows per wavefront of an out-of-place y/z sweep: kMarchRows, or
+// CLAWHIP_SPLIT_ROWS (tuning sweeps).  In-place sweeps march whole pencils
+// (launch_march).
+int march_rows_split() {
+  if (const char* env = getenv("CLAWHIP_SPLIT_ROWS")) {
+    const int h = atoi(env);
+    if (h >= 1 && h <= 65536) return h;
+  }
+  return kMarchRows;
+}
+
 template <typename T>
 SweepArgs<T> base_args(const clawhip_rp& rp, const Geom& gin, const Geom& gout, const int* lo,
                        const int* hi, const int* refl, int ndim, int ng, const int* method,
@@ -342,7 +355,7 @@ SweepArgs<T> base_args(const clawhip_rp& rp, const Geom& gin, const Geom& gout, 
   a.cfl_bits = cfl;
   a.nstrips = 0; a.axis = 0; a.i_lo = a.i_hi = a.p_lo = a.p_hi = 0;
   a.w_ilo = a.w_ihi = a.w_plo = a.w_phi = 0;
-  a.H = kMarchRows; a.nrb = a.npencil = a.nistrips = 0;
+  a.H = march_rows_split(); a.nrb = a.npencil = a.nistrips = 0;
   a.aux = nullptr;
   a.gaux = gout;
   a.bcaux = a.bc;
