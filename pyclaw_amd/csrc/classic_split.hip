@@ -62,6 +62,7 @@ struct SweepArgs {
   T dtdx;               // dt / delta of the sweep direction
   // k_xsweep
   int nstrips;
+  int xrows;            // rows per wavefront
   // k_march
   int axis;             // 2: sweep along j, 3: sweep along k
   int i_lo, i_hi;       // columns swept
@@ -81,6 +82,11 @@ struct SweepArgs {
 };
 
 // ---------------------------------------------------------------- x-sweeps
+// A wavefront sweeps a run of a.xrows rows of one 60-column strip (the next
+// row's cells are loaded before the current row's arithmetic), and a workgroup
+// adds its Courant number with one atomic: one row per wavefront with an atomic
+// each made the 384^3 x-sweep 1.05 M same-address atomics, which serialise
+// (11.9 ms of an 18 ms step).  Every wavefront reaches the workgroup barrier.
 template <class RS, typename T, bool KAHAN, int DIM, bool HALF_IN_SUM>
 __global__ void __launch_bounds__(256) k_xsweep(const SweepArgs<T> a) {
   constexpr int ME = RS::meqn, MW = RS::mwaves;
@@ -90,93 +96,111 @@ __global__ void __launch_bounds__(256) k_xsweep(const SweepArgs<T> a) {
   const int mx = a.gin.mx, my = a.gin.my, mz = DIM == 3 ? a.gin.mz : 1;
   const int nrows_j = my + 2 * ng;
   const int nrows = DIM == 3 ? nrows_j * (mz + 2 * ng) : nrows_j;
-  if (wid >= a.nstrips * nrows) return;
-  const int strip = wid % a.nstrips, row = wid / a.nstrips;
-  const int j = 1 - ng + row % nrows_j;
-  const int k = DIM == 3 ? 1 - ng + row / nrows_j : 1;
+  const int ngroups = (nrows + a.xrows - 1) / a.xrows;
+  const bool active = wid < a.nstrips * ngroups;   // uniform per wavefront
+  const int strip = active ? wid % a.nstrips : 0, grp = active ? wid / a.nstrips : 0;
+  const int row0 = grp * a.xrows;
+  const int row_end = active ? min(row0 + a.xrows, nrows) : row0;
   const int c = 1 + strip * 60 - 2 + lane;
-  const bool swept = DIM == 2 ? true : (j >= 0 && j <= my + 1 && k >= 0 && k <= mz + 1);
-  const bool interior_row = j >= 1 && j <= my && (DIM == 2 || (k >= 1 && k <= mz));
-
-  T Q[ME];
-  if (DIM == 2) load_cell2<ME>(a.qin, a.gin, a.bc, ng, c, j, Q);
-  else load_cell3<ME>(a.qin, a.gin, a.bc, ng, c, j, k, Q);
   constexpr int MA = RS::maux > 0 ? RS::maux : 1;
-  T A[MA], AL[MA];
-  if constexpr (RS::maux > 0) {
-    static_assert(DIM == 2, "aux fields: 2D sweeps");
-    load_cell2<MA>(a.aux, a.gaux, a.bcaux, ng, c, j, A);
-    from_left(A, AL);
-  }
-  T upd[ME];
-#pragma unroll
-  for (int m = 0; m < ME; m++) upd[m] = Q[m];
+  auto row_jk = [&](int row, int& j, int& k) {
+    j = 1 - ng + row % nrows_j;
+    k = DIM == 3 ? 1 - ng + row / nrows_j : 1;
+  };
+  auto load_row = [&](int row, T (&Q)[ME]) {
+    int j, k;
+    row_jk(row, j, k);
+    if (DIM == 2) load_cell2<ME>(a.qin, a.gin, a.bc, ng, c, j, Q);
+    else load_cell3<ME>(a.qin, a.gin, a.bc, ng, c, j, k, Q);
+  };
   double cfl = 0.0;
-  if (swept) {   // uniform across the wavefront
-    T QL[ME];
-    from_left(Q, QL);
-    T W[MW][ME], s[MW], am[ME], ap[ME];
-    typename RS::Roe roe;
-    rpn_call<RS, 1>(a.rp, QL, Q, AL, A, W, s, am, ap, roe);
-    T smax = T(0);
-    if (lane >= 1 && c >= 1 && c <= mx + 1) {
+  T Qn[ME];
+  if (row0 < row_end) load_row(row0, Qn);
+  for (int row = row0; row < row_end; row++) {
+    T Q[ME];
 #pragma unroll
-      for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(s[w]));
+    for (int m = 0; m < ME; m++) Q[m] = Qn[m];
+    if (row + 1 < row_end) load_row(row + 1, Qn);
+    int j, k;
+    row_jk(row, j, k);
+    const bool swept = DIM == 2 ? true : (j >= 0 && j <= my + 1 && k >= 0 && k <= mz + 1);
+    const bool interior_row = j >= 1 && j <= my && (DIM == 2 || (k >= 1 && k <= mz));
+    T A[MA], AL[MA];
+    if constexpr (RS::maux > 0) {
+      static_assert(DIM == 2, "aux fields: 2D sweeps");
+      load_cell2<MA>(a.aux, a.gaux, a.bcaux, ng, c, j, A);
+      from_left(A, AL);
     }
-    cfl = (double)a.dtdx * (double)smax;
-    T F[ME];
+    T upd[ME];
 #pragma unroll
-    for (int m = 0; m < ME; m++) F[m] = T(0);
-    if (a.order != 1) {
-      T Wn[MW][ME];
+    for (int m = 0; m < ME; m++) upd[m] = Q[m];
+    if (swept) {   // uniform across the wavefront
+      T QL[ME];
+      from_left(Q, QL);
+      T W[MW][ME], s[MW], am[ME], ap[ME];
+      typename RS::Roe roe;
+      rpn_call<RS, 1>(a.rp, QL, Q, AL, A, W, s, am, ap, roe);
+      T smax = T(0);
+      if (lane >= 1 && c >= 1 && c <= mx + 1) {
 #pragma unroll
-      for (int w = 0; w < MW; w++) from_right(W[w], Wn[w]);
-      T dotr[MW], dotl[MW];
-      wave_dots<ME, MW>(W, Wn, dotr);
-      from_left(dotr, dotl);
-      limit_waves<ME, MW>(W, s, dotl, dotr, a.mthlim);
-      corr_flux<ME, MW, HALF_IN_SUM>(W, s, a.dtdx, F, a.rp.fwave);
+        for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(s[w]));
+      }
+      const double row_cfl = (double)a.dtdx * (double)smax;
+      cfl = row_cfl > cfl ? row_cfl : cfl;
+      T F[ME];
+#pragma unroll
+      for (int m = 0; m < ME; m++) F[m] = T(0);
+      if (a.order != 1) {
+        T Wn[MW][ME];
+#pragma unroll
+        for (int w = 0; w < MW; w++) from_right(W[w], Wn[w]);
+        T dotr[MW], dotl[MW];
+        wave_dots<ME, MW>(W, Wn, dotr);
+        from_left(dotr, dotl);
+        limit_waves<ME, MW>(W, s, dotl, dotr, a.mthlim);
+        corr_flux<ME, MW, HALF_IN_SUM>(W, s, a.dtdx, F, a.rp.fwave);
+      }
+      T amn[ME], Fr[ME];
+      from_right(am, amn);
+      from_right(F, Fr);
+#pragma unroll
+      for (int m = 0; m < ME; m++) {
+        T qadd = (T(0) - a.dtdx * ap[m]) - a.dtdx * amn[m];
+        upd[m] = KAHAN ? (qadd - a.dtdx * (Fr[m] - F[m]))
+                       : ((Q[m] + qadd) - a.dtdx * (Fr[m] - F[m]));
+      }
     }
-    T amn[ME], Fr[ME];
-    from_right(am, amn);
-    from_right(F, Fr);
+    const bool own = lane >= 2 && lane < 62 && c >= 1 && c <= mx;
+    const bool ghost_col = (c < 1 || c > mx) && c >= 1 - ng && c <= mx + ng;
+    if (own || ghost_col) {
+      const bool update = own && swept;
+      T* po = DIM == 2 ? cell_ptr2(a.qout, a.gout, c, j) : cell_ptr3(a.qout, a.gout, c, j, k);
+      if (KAHAN) {
+        T* pco = DIM == 2 ? cell_ptr2(a.comp_out, a.gout, c, j) : cell_ptr3(a.comp_out, a.gout, c, j, k);
+        if (update && interior_row) {
+          // compensation of the compact input state
+          const T* pci = DIM == 2 ? cell_ptr2(a.comp_in, a.gin, c, j) : cell_ptr3(a.comp_in, a.gin, c, j, k);
 #pragma unroll
-    for (int m = 0; m < ME; m++) {
-      T qadd = (T(0) - a.dtdx * ap[m]) - a.dtdx * amn[m];
-      upd[m] = KAHAN ? (qadd - a.dtdx * (Fr[m] - F[m]))
-                     : ((Q[m] + qadd) - a.dtdx * (Fr[m] - F[m]));
-    }
-  }
-  const bool own = lane >= 2 && lane < 62 && c >= 1 && c <= mx;
-  const bool ghost_col = (c < 1 || c > mx) && c >= 1 - ng && c <= mx + ng;
-  if (own || ghost_col) {
-    const bool update = own && swept;
-    T* po = DIM == 2 ? cell_ptr2(a.qout, a.gout, c, j) : cell_ptr3(a.qout, a.gout, c, j, k);
-    if (KAHAN) {
-      T* pco = DIM == 2 ? cell_ptr2(a.comp_out, a.gout, c, j) : cell_ptr3(a.comp_out, a.gout, c, j, k);
-      if (update && interior_row) {
-        // compensation of the compact input state
-        const T* pci = DIM == 2 ? cell_ptr2(a.comp_in, a.gin, c, j) : cell_ptr3(a.comp_in, a.gin, c, j, k);
+          for (int m = 0; m < ME; m++) {
+            T y = upd[m] - pci[(long long)m * a.gin.fs];
+            T tq = Q[m] + y;
+            pco[(long long)m * a.gout.fs] = (tq - Q[m]) - y;
+            po[(long long)m * a.gout.fs] = tq;
+          }
+        } else {
 #pragma unroll
-        for (int m = 0; m < ME; m++) {
-          T y = upd[m] - pci[(long long)m * a.gin.fs];
-          T tq = Q[m] + y;
-          pco[(long long)m * a.gout.fs] = (tq - Q[m]) - y;
-          po[(long long)m * a.gout.fs] = tq;
+          for (int m = 0; m < ME; m++) {
+            po[(long long)m * a.gout.fs] = update ? Q[m] + upd[m] : Q[m];
+            pco[(long long)m * a.gout.fs] = T(0);
+          }
         }
       } else {
 #pragma unroll
-        for (int m = 0; m < ME; m++) {
-          po[(long long)m * a.gout.fs] = update ? Q[m] + upd[m] : Q[m];
-          pco[(long long)m * a.gout.fs] = T(0);
-        }
+        for (int m = 0; m < ME; m++) po[(long long)m * a.gout.fs] = update ? upd[m] : Q[m];
       }
-    } else {
-#pragma unroll
-      for (int m = 0; m < ME; m++) po[(long long)m * a.gout.fs] = update ? upd[m] : Q[m];
     }
   }
-  wave_atomic_max(cfl, a.cfl_bits);
+  block_atomic_max(cfl, a.cfl_bits);
 }
 
 // ---------------------------------------------------------------- y / z march
@@ -353,7 +377,7 @@ SweepArgs<T> base_args(const clawhip_rp& rp, const Geom& gin, const Geom& gout, 
   a.qin = (const T*)qin; a.qout = (T*)qout;
   a.comp_in = (const T*)cin; a.comp_out = (T*)cout;
   a.cfl_bits = cfl;
-  a.nstrips = 0; a.axis = 0; a.i_lo = a.i_hi = a.p_lo = a.p_hi = 0;
+  a.nstrips = 0; a.xrows = 1; a.axis = 0; a.i_lo = a.i_hi = a.p_lo = a.p_hi = 0;
   a.w_ilo = a.w_ihi = a.w_plo = a.w_phi = 0;
   a.H = march_rows_split(); a.nrb = a.npencil = a.nistrips = 0;
   a.aux = nullptr;
@@ -378,7 +402,17 @@ int launch_x(SweepArgs<T> a, hipStream_t stream) {
   const int mx = a.gin.mx, my = a.gin.my, mz = DIM == 3 ? a.gin.mz : 1;
   a.nstrips = (mx + 59) / 60;
   long long rows = (long long)(my + 2 * a.ng) * (DIM == 3 ? (mz + 2 * a.ng) : 1);
-  long long waves = (long long)a.nstrips * rows;
+  // rows per wavefront: one resident round of wavefronts over the (strip, row)
+  // tasks, or CLAWHIP_XSWEEP_ROWS (tuning sweeps)
+  const long long tasks = (long long)a.nstrips * rows;
+  const long long resident = resident_waves((const void*)k_xsweep<RS, T, KAHAN, DIM, HALF>, 0);
+  long long xr = resident > 0 ? (tasks + resident - 1) / resident : 1;
+  if (const char* env = getenv("CLAWHIP_XSWEEP_ROWS")) {
+    const long long h = atoll(env);
+    if (h >= 1 && h <= 65536) xr = h;
+  }
+  a.xrows = (int)(xr < 1 ? 1 : xr > rows ? rows : xr);
+  const long long waves = (long long)a.nstrips * ((rows + a.xrows - 1) / a.xrows);
   hipLaunchKernelGGL((k_xsweep<RS, T, KAHAN, DIM, HALF>), dim3((unsigned)((waves + 3) / 4)),
                      dim3(256), 0, stream, a);
   hipError_t e = hipGetLastError();

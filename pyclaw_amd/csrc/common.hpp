@@ -151,6 +151,25 @@ __device__ __forceinline__ void wave_atomic_max(double v, unsigned long long* ds
   if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(v));
 }
 
+// The same for a workgroup of up to 256 threads: one atomic per workgroup.
+// Every thread of the workgroup calls it (it holds a barrier).
+__device__ __forceinline__ void block_atomic_max(double v, unsigned long long* dst) {
+  __shared__ double part[256 / 64];
+  if (!(v >= 0.0)) v = 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = part[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = part[w] > m ? part[w] : m;
+    if (m > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
 // philim.f90:2-36
 template <typename T>
 __device__ __forceinline__ T philim(T a, T b, int meth) {
