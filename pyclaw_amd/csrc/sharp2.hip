@@ -696,221 +696,225 @@ __global__ void __launch_bounds__(256) k_sharp_march(const MarchArgs<T> ma) {
   const StageView<T> v = stage_view(a);
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (wid >= ma.nstrips * ma.nrb) return;   // uniform per wavefront
-  const int strip = wid % ma.nstrips, rb = wid / ma.nstrips;
-  const int mx = a.g.mx, my = a.g.my;
-  const int c0 = 1 + strip * 58;
-  const int c = c0 - 3 + lane;                 // this lane's column
-  const int r0 = 1 + rb * ma.H;
-  const int rend = min(r0 + ma.H - 1, my);     // last output row
-  const bool out_lane = lane >= 3 && lane <= 60 && c >= 1 && c <= mx;
-  const bool xint_lane = lane >= 3 && lane <= 61 && c >= 1 && c <= mx + 1;
-  const bool ycfl_lane = c >= 0 && c <= mx + 1;
-  const RPParams<T>& rp = a.rp;
+  // wavefronts past the last strip skip the march but still reach the
+  // workgroup's Courant-number reduction (a barrier)
+  const bool live = wid < ma.nstrips * ma.nrb;   // uniform per wavefront
   T smax_x = T(0), smax_y = T(0);
   int nbad = 0;   // stored cells that lost positivity (count_bad)
+  if (live) {
+    const int strip = wid % ma.nstrips, rb = wid / ma.nstrips;
+    const int mx = a.g.mx, my = a.g.my;
+    const int c0 = 1 + strip * 58;
+    const int c = c0 - 3 + lane;                 // this lane's column
+    const int r0 = 1 + rb * ma.H;
+    const int rend = min(r0 + ma.H - 1, my);     // last output row
+    const bool out_lane = lane >= 3 && lane <= 60 && c >= 1 && c <= mx;
+    const bool xint_lane = lane >= 3 && lane <= 61 && c >= 1 && c <= mx + 1;
+    const bool ycfl_lane = c >= 0 && c <= mx + 1;
+    const RPParams<T>& rp = a.rp;
 
-  auto load_y = [&](int r, T (&y)[ME]) { stage_input<ME>(a, v, c, r, y); };
-  // a window row in two parts: its loads (issued one iteration ahead) and the
-  // stage-input combination stage_input applies (the same expression)
-  auto load_raw = [&](int r, T (&p)[ME], T (&q)[ME]) {
-    load_cell2<ME>(v.P, a.gin, a.bc, a.ng, c, r, p);
-    if (a.ymode != Y_ARRAY) load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, q);
-  };
-  auto combine = [&](const T (&p)[ME], const T (&q)[ME], T (&y)[ME]) {
-#pragma unroll
-    for (int m = 0; m < ME; m++) {
-      y[m] = p[m];
-      if (a.ymode != Y_ARRAY)
-        y[m] = a.ymode == Y_AXPY ? y[m] + v.s * q[m] : y[m] + (v.s * q[m]) / T(6);
-    }
-  };
-  auto load_a = [&](int r, T (&A)[MA]) {
-    if constexpr (AUX) load_cell2<MA>(a.qaux, a.gaux, a.bcaux, a.ng, c, r, A);
-    else A[0] = T(0);
-  };
-  // x-slice of row r (aux Ar of this lane's cell): interface c (left of this
-  // lane's cell) -> qlR, qrL, am, ap
-  auto x_interface = [&](const T (&row)[ME], const T (&Ar)[MA], int r, bool flagged,
-                         T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
-    // st[k] = the row value of lane l + k - 3 (chained one-lane moves)
-    T st[6][ME];
-#pragma unroll
-    for (int m = 0; m < ME; m++) st[3][m] = row[m];
-    from_left<ME>(st[3], st[2]);
-    from_left<ME>(st[2], st[1]);
-    from_left<ME>(st[1], st[0]);
-    from_right<ME>(st[3], st[4]);
-    from_right<ME>(st[4], st[5]);
-    // flagged path: the linear weights, or, where the interface's dilated flag
-    // is set (`flagged`, from the flag bytes loaded a row ahead), the full WENO5
-    // edge values k_weno_flagged stored (interfaces of ghost rows enter only the
-    // CFL and take the linear weights)
-    if (FLAGS) {
-      recon_linear<ME>(st, qrL, qlR);
-      if (flagged) load_recon<ME>(a.recon + xface_id(mx, c, r) * (2 * ME), qrL, qlR);
-    } else {
-      recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
-    }
-    T W[MW][ME], sp[MW];
-    typename RS::Roe roe;
-    T AL[MA];
-    if constexpr (AUX) from_left<MA>(Ar, AL);
-    rpn_call<RS, 1>(rp, qrL, qlR, AL, Ar, W, sp, am, ap, roe);
-    if (xint_lane) {
-#pragma unroll
-      for (int w = 0; w < MW; w++) smax_x = cfl_term(smax_x, (T)fabs(sp[w]));
-    }
-  };
-  // y-interface t (rows t-1 | t, aux Alo | Ahi) of this lane's column from the
-  // window win[0..5] = rows t-3..t+2
-  auto y_interface = [&](const T (&win)[6][ME], const T (&Alo)[MA], const T (&Ahi)[MA], int t,
-                         bool flagged, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
-    if (FLAGS) {
-      recon_linear<ME>(win, qrL, qlR);
-      if (flagged) load_recon<ME>(a.recon + yface_id(mx, my, c, t) * (2 * ME), qrL, qlR);
-    } else {
-      recon_interface<RS, 2, T, CHAR>(rp, win, qrL, qlR);
-    }
-    T W[MW][ME], sp[MW];
-    typename RS::Roe roe;
-    rpn_call<RS, 2>(rp, qrL, qlR, Alo, Ahi, W, sp, am, ap, roe);
-    if (ycfl_lane && t <= my + 1) {
-#pragma unroll
-      for (int w = 0; w < MW; w++) smax_y = cfl_term(smax_y, (T)fabs(sp[w]));
-    }
-  };
-  // x contribution of this lane's cell in row r (a whole x-slice pass)
-  auto x_cell = [&](const T (&row)[ME], const T (&Ar)[MA], int r, bool flagged, T (&dqx)[ME]) {
-    T qrL[ME], qlR[ME], am[ME], ap[ME];
-    x_interface(row, Ar, r, flagged, qrL, qlR, am, ap);
-    T qr[ME], amn[ME];
-    from_right<ME>(qrL, qr);     // right edge of this cell: interface c+1
-    from_right<ME>(am, amn);     // amdq at interface c+1
-    if constexpr (TFLUCT) {
-      T fl[ME], fr[ME];
-      RS::template flux<1>(rp, qlR, fl);
-      RS::template flux<1>(rp, qr, fr);
-#pragma unroll
-      for (int m = 0; m < ME; m++) dqx[m] = T(0) - v.dtdx * ((amn[m] + ap[m]) + (fr[m] - fl[m]));
-    } else {
-      T W[MW][ME], sp[MW], am2[ME], ap2[ME];
-      typename RS::Roe roe;
-      rpn_call<RS, 1>(rp, qlR, qr, Ar, Ar, W, sp, am2, ap2, roe);
-#pragma unroll
-      for (int m = 0; m < ME; m++)
-        dqx[m] = T(0) - v.dtdx * (((amn[m] + ap[m]) + am2[m]) + ap2[m]);
-    }
-  };
-
-  // ---- window rows r0-3 .. r0+2; y-interface r0; aux of rows t-1 (Alo), t (Ahi)
-  T win[6][ME];
-#pragma unroll
-  for (int k = 0; k < 6; k++) load_y(r0 - 3 + k, win[k]);
-  T Alo[MA], Ahi[MA];
-  load_a(r0 - 1, Alo);
-  load_a(r0, Ahi);
-  if (r0 == 1) {   // x-slice of row 0 (CFL only)
-    T d[ME];
-    x_cell(win[2], Alo, 0, false, d);
-  }
-  // FLAGS: this lane's flag bytes, loaded one row iteration before their use so
-  // that their latency hides behind a row's arithmetic.  xbyte(r): the flag of
-  // x-interface c of row r (the column clamped to 1..mx+1, so the clamped row
-  // neighbours come from the lanes beside this one); ybyte(k): the flag of
-  // y-interface c in y-row k (0-based, clamped to 0..my).
-  auto xbyte = [&](int r) -> int {
-    if (!FLAGS || r < 1 || r > my) return 0;
-    return a.flags[(long long)(r - 1) * (mx + 1) + min(max(c - 1, 0), mx)];
-  };
-  auto ybyte = [&](int k) -> int {
-    if (!FLAGS || c < 1 || c > mx) return 0;
-    return a.flags[(long long)(mx + 1) * my + (long long)min(max(k, 0), my) * mx + (c - 1)];
-  };
-  // x-interface c of a row is listed when its byte or a row neighbour's is set
-  // (the cross-lane moves run on every lane, before any lane-dependent test)
-  auto xlisted = [&](int b) {
-    const int nb = b | from_left(b) | from_right(b);
-    return FLAGS && c >= 1 && c <= mx + 1 && nb != 0;
-  };
-  // y-interface t: 0-based y-rows t-2, t-1, t (y-interfaces t-1, t, t+1)
-  int ym2 = ybyte(r0 - 2), ym1 = ybyte(r0 - 1), y0 = ybyte(r0);
-  T qlY[ME], apY[ME];   // interface t-1 carried: lower edge of row t-1, apdq
-  {
-    T qrL[ME], am[ME];
-    y_interface(win, Alo, Ahi, r0, (ym2 | ym1 | y0) != 0, qrL, qlY, am, apY);
-  }
-  // the bytes of the first loop iteration (y-interface r0+1, x-row r0)
-  ym2 = ym1;
-  ym1 = y0;
-  y0 = ybyte(r0 + 1);
-  int xb = xbyte(r0);
-  T pf_p[ME], pf_q[ME];   // the loads of the next iteration's window row
-#pragma unroll
-  for (int m = 0; m < ME; m++) pf_q[m] = T(0);
-  load_raw(r0 + 3, pf_p, pf_q);
-  for (int t = r0 + 1; t <= rend + 1; t++) {
-    // row t-1 = win[3]; shift the window to rows t-3..t+2
-#pragma unroll
-    for (int k = 0; k < 5; k++)
-#pragma unroll
-      for (int m = 0; m < ME; m++) win[k][m] = win[k + 1][m];
-    combine(pf_p, pf_q, win[5]);
-    if (t <= rend) load_raw(t + 3, pf_p, pf_q);
-    // the store-time inputs of row t-1
-    StoreIn<ME, T> si;
-    const long long off = (long long)(c - 1) + (long long)(t - 2) * a.g.rs;
-    if (out_lane) load_store_in<ME>(a, v, off, si);
-#pragma unroll
-    for (int k = 0; k < MA; k++) Alo[k] = Ahi[k];
-    load_a(t, Ahi);
-    // the flag bytes of the next iteration (y-row t+1, x-row t)
-    const int y_next = t <= rend ? ybyte(t + 1) : 0;
-    const int x_next = t <= rend ? xbyte(t) : 0;
-    // y-interface t closes row t-1
-    T qrL[ME], qlR[ME], am[ME], ap[ME];
-    y_interface(win, Alo, Ahi, t, (ym2 | ym1 | y0) != 0, qrL, qlR, am, ap);
-    T dqy[ME];
-    if constexpr (TFLUCT) {
-      T fl[ME], fr[ME];
-      RS::template flux<2>(rp, qlY, fl);
-      RS::template flux<2>(rp, qrL, fr);
-#pragma unroll
-      for (int m = 0; m < ME; m++) dqy[m] = T(0) - v.dtdy * ((am[m] + apY[m]) + (fr[m] - fl[m]));
-    } else {
-      T W[MW][ME], sp[MW], am2[ME], ap2[ME];
-      typename RS::Roe roe;
-      rpn_call<RS, 2>(rp, qlY, qrL, Alo, Alo, W, sp, am2, ap2, roe);
-#pragma unroll
-      for (int m = 0; m < ME; m++)
-        dqy[m] = T(0) - v.dtdy * (((am[m] + apY[m]) + am2[m]) + ap2[m]);
-    }
-    // x-slice of row t-1
-    T dqx[ME];
-    x_cell(win[2], Alo, t - 1, xlisted(xb), dqx);
-    if (out_lane) {
-      bool bad = false;
-#pragma unroll
+    auto load_y = [&](int r, T (&y)[ME]) { stage_input<ME>(a, v, c, r, y); };
+    // a window row in two parts: its loads (issued one iteration ahead) and the
+    // stage-input combination stage_input applies (the same expression)
+    auto load_raw = [&](int r, T (&p)[ME], T (&q)[ME]) {
+      load_cell2<ME>(v.P, a.gin, a.bc, a.ng, c, r, p);
+      if (a.ymode != Y_ARRAY) load_cell2<ME>(a.Q, a.gin, a.bc, a.ng, c, r, q);
+    };
+    auto combine = [&](const T (&p)[ME], const T (&q)[ME], T (&y)[ME]) {
+  #pragma unroll
       for (int m = 0; m < ME; m++) {
-        const T o = stage_store<ME>(a, v, si, off, m, (T(0) + dqx[m]) + dqy[m], win[2][m]);
-        bad = bad || (((a.pos_mask >> m) & 1u) && (!(o > T(0)) || !isfinite(o)));
+        y[m] = p[m];
+        if (a.ymode != Y_ARRAY)
+          y[m] = a.ymode == Y_AXPY ? y[m] + v.s * q[m] : y[m] + (v.s * q[m]) / T(6);
       }
-      nbad += bad ? 1 : 0;
+    };
+    auto load_a = [&](int r, T (&A)[MA]) {
+      if constexpr (AUX) load_cell2<MA>(a.qaux, a.gaux, a.bcaux, a.ng, c, r, A);
+      else A[0] = T(0);
+    };
+    // x-slice of row r (aux Ar of this lane's cell): interface c (left of this
+    // lane's cell) -> qlR, qrL, am, ap
+    auto x_interface = [&](const T (&row)[ME], const T (&Ar)[MA], int r, bool flagged,
+                           T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
+      // st[k] = the row value of lane l + k - 3 (chained one-lane moves)
+      T st[6][ME];
+  #pragma unroll
+      for (int m = 0; m < ME; m++) st[3][m] = row[m];
+      from_left<ME>(st[3], st[2]);
+      from_left<ME>(st[2], st[1]);
+      from_left<ME>(st[1], st[0]);
+      from_right<ME>(st[3], st[4]);
+      from_right<ME>(st[4], st[5]);
+      // flagged path: the linear weights, or, where the interface's dilated flag
+      // is set (`flagged`, from the flag bytes loaded a row ahead), the full WENO5
+      // edge values k_weno_flagged stored (interfaces of ghost rows enter only the
+      // CFL and take the linear weights)
+      if (FLAGS) {
+        recon_linear<ME>(st, qrL, qlR);
+        if (flagged) load_recon<ME>(a.recon + xface_id(mx, c, r) * (2 * ME), qrL, qlR);
+      } else {
+        recon_interface<RS, 1, T, CHAR>(rp, st, qrL, qlR);
+      }
+      T W[MW][ME], sp[MW];
+      typename RS::Roe roe;
+      T AL[MA];
+      if constexpr (AUX) from_left<MA>(Ar, AL);
+      rpn_call<RS, 1>(rp, qrL, qlR, AL, Ar, W, sp, am, ap, roe);
+      if (xint_lane) {
+  #pragma unroll
+        for (int w = 0; w < MW; w++) smax_x = cfl_term(smax_x, (T)fabs(sp[w]));
+      }
+    };
+    // y-interface t (rows t-1 | t, aux Alo | Ahi) of this lane's column from the
+    // window win[0..5] = rows t-3..t+2
+    auto y_interface = [&](const T (&win)[6][ME], const T (&Alo)[MA], const T (&Ahi)[MA], int t,
+                           bool flagged, T (&qrL)[ME], T (&qlR)[ME], T (&am)[ME], T (&ap)[ME]) {
+      if (FLAGS) {
+        recon_linear<ME>(win, qrL, qlR);
+        if (flagged) load_recon<ME>(a.recon + yface_id(mx, my, c, t) * (2 * ME), qrL, qlR);
+      } else {
+        recon_interface<RS, 2, T, CHAR>(rp, win, qrL, qlR);
+      }
+      T W[MW][ME], sp[MW];
+      typename RS::Roe roe;
+      rpn_call<RS, 2>(rp, qrL, qlR, Alo, Ahi, W, sp, am, ap, roe);
+      if (ycfl_lane && t <= my + 1) {
+  #pragma unroll
+        for (int w = 0; w < MW; w++) smax_y = cfl_term(smax_y, (T)fabs(sp[w]));
+      }
+    };
+    // x contribution of this lane's cell in row r (a whole x-slice pass)
+    auto x_cell = [&](const T (&row)[ME], const T (&Ar)[MA], int r, bool flagged, T (&dqx)[ME]) {
+      T qrL[ME], qlR[ME], am[ME], ap[ME];
+      x_interface(row, Ar, r, flagged, qrL, qlR, am, ap);
+      T qr[ME], amn[ME];
+      from_right<ME>(qrL, qr);     // right edge of this cell: interface c+1
+      from_right<ME>(am, amn);     // amdq at interface c+1
+      if constexpr (TFLUCT) {
+        T fl[ME], fr[ME];
+        RS::template flux<1>(rp, qlR, fl);
+        RS::template flux<1>(rp, qr, fr);
+  #pragma unroll
+        for (int m = 0; m < ME; m++) dqx[m] = T(0) - v.dtdx * ((amn[m] + ap[m]) + (fr[m] - fl[m]));
+      } else {
+        T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+        typename RS::Roe roe;
+        rpn_call<RS, 1>(rp, qlR, qr, Ar, Ar, W, sp, am2, ap2, roe);
+  #pragma unroll
+        for (int m = 0; m < ME; m++)
+          dqx[m] = T(0) - v.dtdx * (((amn[m] + ap[m]) + am2[m]) + ap2[m]);
+      }
+    };
+
+    // ---- window rows r0-3 .. r0+2; y-interface r0; aux of rows t-1 (Alo), t (Ahi)
+    T win[6][ME];
+  #pragma unroll
+    for (int k = 0; k < 6; k++) load_y(r0 - 3 + k, win[k]);
+    T Alo[MA], Ahi[MA];
+    load_a(r0 - 1, Alo);
+    load_a(r0, Ahi);
+    if (r0 == 1) {   // x-slice of row 0 (CFL only)
+      T d[ME];
+      x_cell(win[2], Alo, 0, false, d);
     }
-#pragma unroll
-    for (int m = 0; m < ME; m++) { qlY[m] = qlR[m]; apY[m] = ap[m]; }
+    // FLAGS: this lane's flag bytes, loaded one row iteration before their use so
+    // that their latency hides behind a row's arithmetic.  xbyte(r): the flag of
+    // x-interface c of row r (the column clamped to 1..mx+1, so the clamped row
+    // neighbours come from the lanes beside this one); ybyte(k): the flag of
+    // y-interface c in y-row k (0-based, clamped to 0..my).
+    auto xbyte = [&](int r) -> int {
+      if (!FLAGS || r < 1 || r > my) return 0;
+      return a.flags[(long long)(r - 1) * (mx + 1) + min(max(c - 1, 0), mx)];
+    };
+    auto ybyte = [&](int k) -> int {
+      if (!FLAGS || c < 1 || c > mx) return 0;
+      return a.flags[(long long)(mx + 1) * my + (long long)min(max(k, 0), my) * mx + (c - 1)];
+    };
+    // x-interface c of a row is listed when its byte or a row neighbour's is set
+    // (the cross-lane moves run on every lane, before any lane-dependent test)
+    auto xlisted = [&](int b) {
+      const int nb = b | from_left(b) | from_right(b);
+      return FLAGS && c >= 1 && c <= mx + 1 && nb != 0;
+    };
+    // y-interface t: 0-based y-rows t-2, t-1, t (y-interfaces t-1, t, t+1)
+    int ym2 = ybyte(r0 - 2), ym1 = ybyte(r0 - 1), y0 = ybyte(r0);
+    T qlY[ME], apY[ME];   // interface t-1 carried: lower edge of row t-1, apdq
+    {
+      T qrL[ME], am[ME];
+      y_interface(win, Alo, Ahi, r0, (ym2 | ym1 | y0) != 0, qrL, qlY, am, apY);
+    }
+    // the bytes of the first loop iteration (y-interface r0+1, x-row r0)
     ym2 = ym1;
     ym1 = y0;
-    y0 = y_next;
-    xb = x_next;
-  }
-  if (rend == my) {   // x-slice of row my+1 (CFL only) = window row index 3 now
-    T d[ME];
-    x_cell(win[3], Ahi, my + 1, false, d);
+    y0 = ybyte(r0 + 1);
+    int xb = xbyte(r0);
+    T pf_p[ME], pf_q[ME];   // the loads of the next iteration's window row
+  #pragma unroll
+    for (int m = 0; m < ME; m++) pf_q[m] = T(0);
+    load_raw(r0 + 3, pf_p, pf_q);
+    for (int t = r0 + 1; t <= rend + 1; t++) {
+      // row t-1 = win[3]; shift the window to rows t-3..t+2
+  #pragma unroll
+      for (int k = 0; k < 5; k++)
+  #pragma unroll
+        for (int m = 0; m < ME; m++) win[k][m] = win[k + 1][m];
+      combine(pf_p, pf_q, win[5]);
+      if (t <= rend) load_raw(t + 3, pf_p, pf_q);
+      // the store-time inputs of row t-1
+      StoreIn<ME, T> si;
+      const long long off = (long long)(c - 1) + (long long)(t - 2) * a.g.rs;
+      if (out_lane) load_store_in<ME>(a, v, off, si);
+  #pragma unroll
+      for (int k = 0; k < MA; k++) Alo[k] = Ahi[k];
+      load_a(t, Ahi);
+      // the flag bytes of the next iteration (y-row t+1, x-row t)
+      const int y_next = t <= rend ? ybyte(t + 1) : 0;
+      const int x_next = t <= rend ? xbyte(t) : 0;
+      // y-interface t closes row t-1
+      T qrL[ME], qlR[ME], am[ME], ap[ME];
+      y_interface(win, Alo, Ahi, t, (ym2 | ym1 | y0) != 0, qrL, qlR, am, ap);
+      T dqy[ME];
+      if constexpr (TFLUCT) {
+        T fl[ME], fr[ME];
+        RS::template flux<2>(rp, qlY, fl);
+        RS::template flux<2>(rp, qrL, fr);
+  #pragma unroll
+        for (int m = 0; m < ME; m++) dqy[m] = T(0) - v.dtdy * ((am[m] + apY[m]) + (fr[m] - fl[m]));
+      } else {
+        T W[MW][ME], sp[MW], am2[ME], ap2[ME];
+        typename RS::Roe roe;
+        rpn_call<RS, 2>(rp, qlY, qrL, Alo, Alo, W, sp, am2, ap2, roe);
+  #pragma unroll
+        for (int m = 0; m < ME; m++)
+          dqy[m] = T(0) - v.dtdy * (((am[m] + apY[m]) + am2[m]) + ap2[m]);
+      }
+      // x-slice of row t-1
+      T dqx[ME];
+      x_cell(win[2], Alo, t - 1, xlisted(xb), dqx);
+      if (out_lane) {
+        bool bad = false;
+  #pragma unroll
+        for (int m = 0; m < ME; m++) {
+          const T o = stage_store<ME>(a, v, si, off, m, (T(0) + dqx[m]) + dqy[m], win[2][m]);
+          bad = bad || (((a.pos_mask >> m) & 1u) && (!(o > T(0)) || !isfinite(o)));
+        }
+        nbad += bad ? 1 : 0;
+      }
+  #pragma unroll
+      for (int m = 0; m < ME; m++) { qlY[m] = qlR[m]; apY[m] = ap[m]; }
+      ym2 = ym1;
+      ym1 = y0;
+      y0 = y_next;
+      xb = x_next;
+    }
+    if (rend == my) {   // x-slice of row my+1 (CFL only) = window row index 3 now
+      T d[ME];
+      x_cell(win[3], Ahi, my + 1, false, d);
+    }
   }
   double cx = (double)v.dtdx * (double)smax_x;
   double cy = (double)v.dtdy * (double)smax_y;
-  wave_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
+  block_atomic_max(cx > cy ? cx : cy, a.cfl_bits);
   if (a.ctl && a.count_bad) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) nbad += __shfl_xor(nbad, off, 64);
