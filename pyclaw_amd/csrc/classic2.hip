@@ -22,7 +22,11 @@ template <class RS, typename T, bool KAHAN, bool AUX = false, bool CAPA = false>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(unsplit_waves<RS, T, AUX, CAPA>())))
     k_unsplit(const UnsplitArgs<T> a) {
-  unsplit_body<RS, T, KAHAN, AUX, CAPA>(a, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  // one Courant-number atomic per workgroup: one per wavefront serialises
+  // (same address) over the thousands of wavefronts of a large grid
+  block_atomic_max(
+      unsplit_body<RS, T, KAHAN, AUX, CAPA>(a, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)),
+      a.cfl_bits);
 }
 
 // Device-resident stepping: one launch = one attempted step of
@@ -44,7 +48,7 @@ __global__ void __launch_bounds__(256)
   a.dtdy = T(dt / bufs.dy);
   a.cfl_bits = &ctl->cfl_bits;
   const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  unsplit_body<RS, T, KAHAN>(a, wid);
+  block_atomic_max(unsplit_body<RS, T, KAHAN>(a, wid), a.cfl_bits);
 }
 
 // The step's control after it (step_ctl_apply).

@@ -91,7 +91,9 @@ k_ens_step(const EnsHeader* __restrict__ hdr, const EnsMember* __restrict__ mem)
   a.dtdy = T(dt / m.dy);
   a.cfl_bits = &ctl->cfl_bits;
   const int local = (int)(gw - m.base);
-  unsplit_body<RS, T, KAHAN>(a, local);
+  // the members' Courant numbers go to their own controllers (many
+  // addresses), and a workgroup may span members that return above
+  wave_atomic_max(unsplit_body<RS, T, KAHAN>(a, local), a.cfl_bits);
 }
 
 // Every member's step control after the step launch (step_ctl_apply), one

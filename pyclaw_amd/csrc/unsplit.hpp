@@ -223,13 +223,16 @@ constexpr int unsplit_waves() {
 }
 
 // `wid`: this wavefront's index among the a.nstrips * a.nrb of the launch.
+// Returns this lane's Courant number (0 past the last strip); the kernel adds it
+// to *a.cfl_bits (one atomic per workgroup where every wavefront of the
+// workgroup reaches that point, else one per wavefront).
 template <class RS, typename T, bool KAHAN, bool AUX = false, bool CAPA = false>
-__device__ __forceinline__ void unsplit_body(const UnsplitArgs<T>& a, int wid) {
+__device__ __forceinline__ double unsplit_body(const UnsplitArgs<T>& a, int wid) {
   constexpr int ME = RS::meqn, MW = RS::mwaves, NA = UNSPLIT_MAX_AUX;
   static_assert(AUX || (RS::maux == 0 && !CAPA), "aux solvers and capacity need the aux window");
   using Roe = typename RS::Roe;
   const int lane = threadIdx.x & 63;
-  if (wid >= a.nstrips * a.nrb) return;   // uniform per wavefront
+  if (wid >= a.nstrips * a.nrb) return 0.0;   // uniform per wavefront
   const int strip = wid % a.nstrips, rb = wid / a.nstrips;
   const int mx = a.gout.mx, my = a.gout.my;
   const int c = 1 + strip * 60 - 2 + lane;
@@ -425,7 +428,7 @@ __device__ __forceinline__ void unsplit_body(const UnsplitArgs<T>& a, int wid) {
     cy = (double)a.dtdy * (double)smy;
   }
   if (cy > cfl) cfl = cy;
-  wave_atomic_max(cfl, a.cfl_bits);
+  return cfl;
 }
 
 }  // namespace claw
