@@ -75,185 +75,191 @@ __global__ void __launch_bounds__(256) k_flux3(const Step3Args<double> a) {
   int n, na, nb;
   dir_extents<DIR>(a.g, n, na, nb);
   const long long slices = (long long)(na + 2) * (nb + 2);
-  if (wid >= a.nstrips * slices) return;
-  const int strip = wid % a.nstrips;
-  const long long sl = wid / a.nstrips;
-  const int sa = (int)(sl % (na + 2)), sb = (int)(sl / (na + 2));   // 0..na+1, 0..nb+1
-  const int c = 1 + strip * 60 - 2 + lane;
-  const T dtn = DIR == 1 ? a.dtdx : (DIR == 2 ? a.dtdy : a.dtdz);
-  // flux3's dtdy / dtdz: the first / second transverse direction's dt/d
-  const T dt1 = DIR == 1 ? a.dtdy : (DIR == 2 ? a.dtdz : a.dtdx);
-  const T dt2 = DIR == 1 ? a.dtdz : (DIR == 2 ? a.dtdx : a.dtdy);
-  int i, j, k;
-  if (DIR == 1) { i = c; j = sa; k = sb; }
-  else if (DIR == 2) { j = c; k = sa; i = sb; }
-  else { k = c; i = sa; j = sb; }
-  T Q[ME], QL[ME];
-  load_cell3<ME>(a.qin, a.g, a.bc, a.ng, i, j, k, Q);
-  from_left(Q, QL);
-  T W[MW][ME], s[MW], am[ME], ap[ME];
-  typename RS::Roe roe;
-  RS::template rpn<DIR>(a.rp, QL, Q, W, s, am, ap, roe);
-  // Courant number over interfaces 1..n+1 (flux3.f90:214-221)
-  T smax = T(0);
-  if (lane >= 1 && c >= 1 && c <= n + 1) {
-#pragma unroll
-    for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(s[w]));
-  }
-  wave_atomic_max((double)dtn * (double)smax, a.cfl_bits);
-  // qadd(c) = (0 - dtn apdq(c)) - dtn amdq(c+1)  (flux3.f90:205-208)
-  T amn[ME], qadd[ME];
-  from_right(am, amn);
-#pragma unroll
-  for (int m = 0; m < ME; m++) qadd[m] = (T(0) - dtn * ap[m]) - dtn * amn[m];
-  // second-order correction (flux3.f90:224-259); cqxx carries the 0.5
-  T cq[ME];
-#pragma unroll
-  for (int m = 0; m < ME; m++) cq[m] = T(0);
-  if (a.order != 1) {
-    T Wn[MW][ME];
-#pragma unroll
-    for (int w = 0; w < MW; w++) from_right(W[w], Wn[w]);
-    T dotr[MW], dotl[MW];
-    wave_dots<ME, MW>(W, Wn, dotr);
-    from_left(dotr, dotl);
-    limit_waves<ME, MW>(W, s, dotl, dotr, a.mthlim);
-    const T dtdxave = T(0.5) * (dtn + dtn);
-#pragma unroll
-    for (int w = 0; w < MW; w++) {
-#pragma unroll
-      for (int m = 0; m < ME; m++) {
-        const T sa2 = fabs(s[w]);
-        const T f = a.rp.fwave ? copysign(T(1), s[w]) : sa2;
-        cq[m] = cq[m] + T(0.5) * f * (T(1) - sa2 * dtdxave) * W[w][m];
+  // wavefronts past the last slice skip the work but reach the workgroup's
+  // Courant-number reduction (a barrier): one atomic per workgroup
+  const bool live = wid < a.nstrips * slices;   // uniform per wavefront
+  double cfl_wave = 0.0;
+  if (live) {
+    const int strip = wid % a.nstrips;
+    const long long sl = wid / a.nstrips;
+    const int sa = (int)(sl % (na + 2)), sb = (int)(sl / (na + 2));   // 0..na+1, 0..nb+1
+    const int c = 1 + strip * 60 - 2 + lane;
+    const T dtn = DIR == 1 ? a.dtdx : (DIR == 2 ? a.dtdy : a.dtdz);
+    // flux3's dtdy / dtdz: the first / second transverse direction's dt/d
+    const T dt1 = DIR == 1 ? a.dtdy : (DIR == 2 ? a.dtdz : a.dtdx);
+    const T dt2 = DIR == 1 ? a.dtdz : (DIR == 2 ? a.dtdx : a.dtdy);
+    int i, j, k;
+    if (DIR == 1) { i = c; j = sa; k = sb; }
+    else if (DIR == 2) { j = c; k = sa; i = sb; }
+    else { k = c; i = sa; j = sb; }
+    T Q[ME], QL[ME];
+    load_cell3<ME>(a.qin, a.g, a.bc, a.ng, i, j, k, Q);
+    from_left(Q, QL);
+    T W[MW][ME], s[MW], am[ME], ap[ME];
+    typename RS::Roe roe;
+    RS::template rpn<DIR>(a.rp, QL, Q, W, s, am, ap, roe);
+    // Courant number over interfaces 1..n+1 (flux3.f90:214-221)
+    T smax = T(0);
+    if (lane >= 1 && c >= 1 && c <= n + 1) {
+  #pragma unroll
+      for (int w = 0; w < MW; w++) smax = cfl_term(smax, fabs(s[w]));
+    }
+    cfl_wave = (double)dtn * (double)smax;
+    // qadd(c) = (0 - dtn apdq(c)) - dtn amdq(c+1)  (flux3.f90:205-208)
+    T amn[ME], qadd[ME];
+    from_right(am, amn);
+  #pragma unroll
+    for (int m = 0; m < ME; m++) qadd[m] = (T(0) - dtn * ap[m]) - dtn * amn[m];
+    // second-order correction (flux3.f90:224-259); cqxx carries the 0.5
+    T cq[ME];
+  #pragma unroll
+    for (int m = 0; m < ME; m++) cq[m] = T(0);
+    if (a.order != 1) {
+      T Wn[MW][ME];
+  #pragma unroll
+      for (int w = 0; w < MW; w++) from_right(W[w], Wn[w]);
+      T dotr[MW], dotl[MW];
+      wave_dots<ME, MW>(W, Wn, dotr);
+      from_left(dotr, dotl);
+      limit_waves<ME, MW>(W, s, dotl, dotr, a.mthlim);
+      const T dtdxave = T(0.5) * (dtn + dtn);
+  #pragma unroll
+      for (int w = 0; w < MW; w++) {
+  #pragma unroll
+        for (int m = 0; m < ME; m++) {
+          const T sa2 = fabs(s[w]);
+          const T f = a.rp.fwave ? copysign(T(1), s[w]) : sa2;
+          cq[m] = cq[m] + T(0.5) * f * (T(1) - sa2 * dtdxave) * W[w][m];
+        }
       }
     }
-  }
-  T fadd[ME];
-#pragma unroll
-  for (int m = 0; m < ME; m++) fadd[m] = T(0) + cq[m];
+    T fadd[ME];
+  #pragma unroll
+    for (int m = 0; m < ME; m++) fadd[m] = T(0) + cq[m];
 
-  // stores of this lane's cell c (cells 1..n; fadd at interfaces 1..n+1)
-  const bool own = lane >= 2 && lane < 62 && c >= 1;
-  const bool own_cell = own && c <= n;
-  T* wk = a.work[DIR - 1];
-  const long long st = a.wstride[DIR - 1];
-  if (own && c <= n + 1) {
-#pragma unroll
-    for (int m = 0; m < ME; m++) wk[widx(WL::FA + m, st, sa, sb, c, n, na)] = fadd[m];
-  }
-  if (own_cell) {
-#pragma unroll
-    for (int m = 0; m < ME; m++) wk[widx(WL::QA + m, st, sa, sb, c, n, na)] = qadd[m];
-  }
-  const int m3 = a.m3, m4 = a.m4;
-  T bmam[ME], bpam[ME], bmap[ME], bpap[ME], cmam[ME], cpam[ME], cmap[ME], cpap[ME];
-  T bmcq[ME], bpcq[ME], cmcq[ME], cpcq[ME];
-#pragma unroll
-  for (int m = 0; m < ME; m++) {
-    bmam[m] = bpam[m] = bmap[m] = bpap[m] = cmam[m] = cpam[m] = cmap[m] = cpap[m] = T(0);
-    bmcq[m] = bpcq[m] = cmcq[m] = cpcq[m] = T(0);
-  }
-  if (m3 > 0) {
-    RS::template rpt3<DIR, 2>(roe, am, bmam, bpam);
-    RS::template rpt3<DIR, 2>(roe, ap, bmap, bpap);
-    RS::template rpt3<DIR, 3>(roe, am, cmam, cpam);
-    RS::template rpt3<DIR, 3>(roe, ap, cmap, cpap);
-    if (m3 == 2) {
-      RS::template rpt3<DIR, 2>(roe, cq, bmcq, bpcq);
-      RS::template rpt3<DIR, 3>(roe, cq, cmcq, cpcq);
+    // stores of this lane's cell c (cells 1..n; fadd at interfaces 1..n+1)
+    const bool own = lane >= 2 && lane < 62 && c >= 1;
+    const bool own_cell = own && c <= n;
+    T* wk = a.work[DIR - 1];
+    const long long st = a.wstride[DIR - 1];
+    if (own && c <= n + 1) {
+  #pragma unroll
+      for (int m = 0; m < ME; m++) wk[widx(WL::FA + m, st, sa, sb, c, n, na)] = fadd[m];
     }
-  }
-  // One family of transverse modifications (gadd: family 0, split direction
-  // 2, double-transverse factor dt2; hadd: family 1, direction 3, factor dt1)
-  // for cell c: interface c's own terms as a running sum in statement order,
-  // then interface c+1's terms for its left cell, shifted from the right lane
-  // (flux3.f90:374-456, 488-580).  x*: the 2D splits (bm/bp of am and ap for
-  // gadd, cm/cp for hadd); y*: the correction's split in that direction;
-  // z*: the other direction's splits, which the double-transverse solve
-  // splits again.
-  auto family = [&](auto fam, const T (&xmam)[ME], const T (&xpam)[ME], const T (&xmap)[ME],
-                    const T (&xpap)[ME], const T (&ymcq)[ME], const T (&ypcq)[ME],
-                    const T (&zmam)[ME], const T (&zpam)[ME], const T (&zmap)[ME],
-                    const T (&zpap)[ME], const T (&zmcq)[ME], const T (&zpcq)[ME], T dtf) {
-    constexpr int FAM = decltype(fam)::value;
-    constexpr int ICO = FAM == 0 ? 2 : 3;
-    const T sixth = T(1) / T(6);
-    T cmcpap[ME], cpcpap[ME], cmcpam[ME], cpcpam[ME], cmcmap[ME], cpcmap[ME], cmcmam[ME],
-        cpcmam[ME];
-    if (m3 > 0 && m4 > 0) {
-      T x2[ME];
-#pragma unroll
-      for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zpap[m] - T(3) * zpcq[m] : zpap[m];
-      RS::template rpt3<DIR, ICO>(roe, x2, cmcpap, cpcpap);
-#pragma unroll
-      for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zpam[m] + T(3) * zpcq[m] : zpam[m];
-      RS::template rpt3<DIR, ICO>(roe, x2, cmcpam, cpcpam);
-#pragma unroll
-      for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zmap[m] - T(3) * zmcq[m] : zmap[m];
-      RS::template rpt3<DIR, ICO>(roe, x2, cmcmap, cpcmap);
-#pragma unroll
-      for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zmam[m] + T(3) * zmcq[m] : zmam[m];
-      RS::template rpt3<DIR, ICO>(roe, x2, cmcmam, cpcmam);
+    if (own_cell) {
+  #pragma unroll
+      for (int m = 0; m < ME; m++) wk[widx(WL::QA + m, st, sa, sb, c, n, na)] = qadd[m];
     }
-#pragma unroll
+    const int m3 = a.m3, m4 = a.m4;
+    T bmam[ME], bpam[ME], bmap[ME], bpap[ME], cmam[ME], cpam[ME], cmap[ME], cpap[ME];
+    T bmcq[ME], bpcq[ME], cmcq[ME], cpcq[ME];
+  #pragma unroll
     for (int m = 0; m < ME; m++) {
-      // [side][off + 1]: the running sums of cell c
-      T v[2][3];
-      v[0][0] = v[0][1] = v[0][2] = v[1][0] = v[1][1] = v[1][2] = T(0);
-      // interface c+1's terms for this cell, shifted from the right lane
-      T l0[2], l4[2], l3[2], lp[2], lm[2];
-      if (m3 > 0) {
-        v[0][1] = v[0][1] - T(0.5) * dtn * xmap[m];
-        v[1][1] = v[1][1] - T(0.5) * dtn * xpap[m];
-        l0[0] = T(0.5) * dtn * xmam[m];
-        l0[1] = T(0.5) * dtn * xpam[m];
-        if (m4 > 0) {
-          v[1][1] = v[1][1] + sixth * dtn * dtf * (cpcpap[m] - cpcmap[m]);
-          v[0][1] = v[0][1] + sixth * dtn * dtf * (cmcpap[m] - cmcmap[m]);
-          v[1][2] = v[1][2] - sixth * dtn * dtf * cpcpap[m];
-          v[0][2] = v[0][2] - sixth * dtn * dtf * cmcpap[m];
-          v[1][0] = v[1][0] + sixth * dtn * dtf * cpcmap[m];
-          v[0][0] = v[0][0] + sixth * dtn * dtf * cmcmap[m];
-          l4[1] = sixth * dtn * dtf * (cpcpam[m] - cpcmam[m]);
-          l4[0] = sixth * dtn * dtf * (cmcpam[m] - cmcmam[m]);
-          lp[1] = sixth * dtn * dtf * cpcpam[m];
-          lp[0] = sixth * dtn * dtf * cmcpam[m];
-          lm[1] = sixth * dtn * dtf * cpcmam[m];
-          lm[0] = sixth * dtn * dtf * cmcmam[m];
-        }
-        if (m3 >= 2) {
-          v[1][1] = v[1][1] + dtn * ypcq[m];
-          v[0][1] = v[0][1] + dtn * ymcq[m];
-          l3[1] = dtn * ypcq[m];
-          l3[0] = dtn * ymcq[m];
-        }
-#pragma unroll
-        for (int sd = 0; sd < 2; sd++) {
-          v[sd][1] = v[sd][1] - from_right(l0[sd]);
-          if (m4 > 0) {
-            v[sd][1] = v[sd][1] + from_right(l4[sd]);
-            v[sd][2] = v[sd][2] - from_right(lp[sd]);
-            v[sd][0] = v[sd][0] + from_right(lm[sd]);
-          }
-          if (m3 >= 2) v[sd][1] = v[sd][1] - from_right(l3[sd]);
-        }
-      }
-      if (own_cell) {
-        constexpr int base = FAM == 0 ? WL::G : WL::H;
-#pragma unroll
-        for (int sd = 0; sd < 2; sd++)
-#pragma unroll
-          for (int o = 0; o < 3; o++)
-            wk[widx(WL::gh(base, m, sd + 1, o - 1), st, sa, sb, c, n, na)] = v[sd][o];
+      bmam[m] = bpam[m] = bmap[m] = bpap[m] = cmam[m] = cpam[m] = cmap[m] = cpap[m] = T(0);
+      bmcq[m] = bpcq[m] = cmcq[m] = cpcq[m] = T(0);
+    }
+    if (m3 > 0) {
+      RS::template rpt3<DIR, 2>(roe, am, bmam, bpam);
+      RS::template rpt3<DIR, 2>(roe, ap, bmap, bpap);
+      RS::template rpt3<DIR, 3>(roe, am, cmam, cpam);
+      RS::template rpt3<DIR, 3>(roe, ap, cmap, cpap);
+      if (m3 == 2) {
+        RS::template rpt3<DIR, 2>(roe, cq, bmcq, bpcq);
+        RS::template rpt3<DIR, 3>(roe, cq, cmcq, cpcq);
       }
     }
-  };
-  family(std::integral_constant<int, 0>(), bmam, bpam, bmap, bpap, bmcq, bpcq, cmam, cpam,
-         cmap, cpap, cmcq, cpcq, dt2);
-  // hadd's double-transverse input: the y-like splits with the correction
-  family(std::integral_constant<int, 1>(), cmam, cpam, cmap, cpap, cmcq, cpcq, bmam, bpam,
-         bmap, bpap, bmcq, bpcq, dt1);
+    // One family of transverse modifications (gadd: family 0, split direction
+    // 2, double-transverse factor dt2; hadd: family 1, direction 3, factor dt1)
+    // for cell c: interface c's own terms as a running sum in statement order,
+    // then interface c+1's terms for its left cell, shifted from the right lane
+    // (flux3.f90:374-456, 488-580).  x*: the 2D splits (bm/bp of am and ap for
+    // gadd, cm/cp for hadd); y*: the correction's split in that direction;
+    // z*: the other direction's splits, which the double-transverse solve
+    // splits again.
+    auto family = [&](auto fam, const T (&xmam)[ME], const T (&xpam)[ME], const T (&xmap)[ME],
+                      const T (&xpap)[ME], const T (&ymcq)[ME], const T (&ypcq)[ME],
+                      const T (&zmam)[ME], const T (&zpam)[ME], const T (&zmap)[ME],
+                      const T (&zpap)[ME], const T (&zmcq)[ME], const T (&zpcq)[ME], T dtf) {
+      constexpr int FAM = decltype(fam)::value;
+      constexpr int ICO = FAM == 0 ? 2 : 3;
+      const T sixth = T(1) / T(6);
+      T cmcpap[ME], cpcpap[ME], cmcpam[ME], cpcpam[ME], cmcmap[ME], cpcmap[ME], cmcmam[ME],
+          cpcmam[ME];
+      if (m3 > 0 && m4 > 0) {
+        T x2[ME];
+  #pragma unroll
+        for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zpap[m] - T(3) * zpcq[m] : zpap[m];
+        RS::template rpt3<DIR, ICO>(roe, x2, cmcpap, cpcpap);
+  #pragma unroll
+        for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zpam[m] + T(3) * zpcq[m] : zpam[m];
+        RS::template rpt3<DIR, ICO>(roe, x2, cmcpam, cpcpam);
+  #pragma unroll
+        for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zmap[m] - T(3) * zmcq[m] : zmap[m];
+        RS::template rpt3<DIR, ICO>(roe, x2, cmcmap, cpcmap);
+  #pragma unroll
+        for (int m = 0; m < ME; m++) x2[m] = m4 == 2 ? zmam[m] + T(3) * zmcq[m] : zmam[m];
+        RS::template rpt3<DIR, ICO>(roe, x2, cmcmam, cpcmam);
+      }
+  #pragma unroll
+      for (int m = 0; m < ME; m++) {
+        // [side][off + 1]: the running sums of cell c
+        T v[2][3];
+        v[0][0] = v[0][1] = v[0][2] = v[1][0] = v[1][1] = v[1][2] = T(0);
+        // interface c+1's terms for this cell, shifted from the right lane
+        T l0[2], l4[2], l3[2], lp[2], lm[2];
+        if (m3 > 0) {
+          v[0][1] = v[0][1] - T(0.5) * dtn * xmap[m];
+          v[1][1] = v[1][1] - T(0.5) * dtn * xpap[m];
+          l0[0] = T(0.5) * dtn * xmam[m];
+          l0[1] = T(0.5) * dtn * xpam[m];
+          if (m4 > 0) {
+            v[1][1] = v[1][1] + sixth * dtn * dtf * (cpcpap[m] - cpcmap[m]);
+            v[0][1] = v[0][1] + sixth * dtn * dtf * (cmcpap[m] - cmcmap[m]);
+            v[1][2] = v[1][2] - sixth * dtn * dtf * cpcpap[m];
+            v[0][2] = v[0][2] - sixth * dtn * dtf * cmcpap[m];
+            v[1][0] = v[1][0] + sixth * dtn * dtf * cpcmap[m];
+            v[0][0] = v[0][0] + sixth * dtn * dtf * cmcmap[m];
+            l4[1] = sixth * dtn * dtf * (cpcpam[m] - cpcmam[m]);
+            l4[0] = sixth * dtn * dtf * (cmcpam[m] - cmcmam[m]);
+            lp[1] = sixth * dtn * dtf * cpcpam[m];
+            lp[0] = sixth * dtn * dtf * cmcpam[m];
+            lm[1] = sixth * dtn * dtf * cpcmam[m];
+            lm[0] = sixth * dtn * dtf * cmcmam[m];
+          }
+          if (m3 >= 2) {
+            v[1][1] = v[1][1] + dtn * ypcq[m];
+            v[0][1] = v[0][1] + dtn * ymcq[m];
+            l3[1] = dtn * ypcq[m];
+            l3[0] = dtn * ymcq[m];
+          }
+  #pragma unroll
+          for (int sd = 0; sd < 2; sd++) {
+            v[sd][1] = v[sd][1] - from_right(l0[sd]);
+            if (m4 > 0) {
+              v[sd][1] = v[sd][1] + from_right(l4[sd]);
+              v[sd][2] = v[sd][2] - from_right(lp[sd]);
+              v[sd][0] = v[sd][0] + from_right(lm[sd]);
+            }
+            if (m3 >= 2) v[sd][1] = v[sd][1] - from_right(l3[sd]);
+          }
+        }
+        if (own_cell) {
+          constexpr int base = FAM == 0 ? WL::G : WL::H;
+  #pragma unroll
+          for (int sd = 0; sd < 2; sd++)
+  #pragma unroll
+            for (int o = 0; o < 3; o++)
+              wk[widx(WL::gh(base, m, sd + 1, o - 1), st, sa, sb, c, n, na)] = v[sd][o];
+        }
+      }
+    };
+    family(std::integral_constant<int, 0>(), bmam, bpam, bmap, bpap, bmcq, bpcq, cmam, cpam,
+           cmap, cpap, cmcq, cpcq, dt2);
+    // hadd's double-transverse input: the y-like splits with the correction
+    family(std::integral_constant<int, 1>(), cmam, cpam, cmap, cpap, cmcq, cpcq, bmam, bpam,
+           bmap, bpap, bmcq, bpcq, dt1);
+  }
+  block_atomic_max(cfl_wave, a.cfl_bits);
 }
 
 // One slice's update of a cell at transverse offsets (da, db) from the slice
