@@ -19,9 +19,8 @@ PD = {"gamma": 1.4}
 # euler_5wave_2D, component WENO5) did not finish on the GPU once, in the stage
 # of test_sharp_stage_bit_exact's first case (efix 1, 61x37), built from a form
 # of the kernel that loaded its gate after the stage view.  That form is taken
-# out (the kernel compiles to the same code as before it, which passed these
-# tests), but the cause of the hang was not shown, so the tests that launch
-# this kernel stay off the GPU.
+# out, but the cause of the hang was not shown, so the tests that launch this
+# kernel stay off the GPU.
 SHARP5_SKIP = ("the FP64 euler_5wave_2D SharpClaw stage kernel did not finish once on the "
                "GPU; the change before it is taken out, the cause is not shown")
 
